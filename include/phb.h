@@ -1,0 +1,161 @@
+/*
+ * phb.h -- C ABI of ph_brush_amd, the MI355X (gfx950) implementation of
+ * ph_brush's Monte Carlo energy path for protonation and ion-swap moves.
+ *
+ * Plain C: opaque handles, pointers and sizes, int status codes.  No
+ * exception crosses this boundary; on failure a function returns a non-zero
+ * PHB_ERR_* code and phb_last_error() holds the message (thread-local).
+ * Every call that returns a value synchronises its stream first, like the
+ * reference's "return by value after a device sync" convention.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).  The C++ layer in include/ph_brush/
+ * restores the reference signatures on top of these functions.
+ *
+ * Data conventions (shared with the reference):
+ *   - positions wrapped into [0, L); orthorhombic box; Lz already extended
+ *     by the caller when the slab correction is used (MC/src/main.cu:110-117);
+ *   - device types 0-based (MC/src/main.cu:33); MC type ids 1-based
+ *     (MC/include/types.hpp:51-54);
+ *   - CSR rows sorted ascending; grids x-fastest ((z*ny)+y)*nx+x;
+ *     half-spectrum k-grids ((kz*ny)+ky)*(nx/2+1)+kx, complex interleaved.
+ */
+#ifndef PHB_H
+#define PHB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHB_ABI_VERSION 1
+
+enum {
+    PHB_OK = 0,
+    PHB_ERR_ARG = 1,          /* invalid argument / size / range              */
+    PHB_ERR_HIP = 2,          /* HIP runtime error                            */
+    PHB_ERR_ALLOC = 3,        /* device or host allocation failed             */
+    PHB_ERR_STATE = 4,        /* infeasible plan, no valid move, bad state    */
+    PHB_ERR_UNSUPPORTED = 5,  /* option not supported by this build          */
+    PHB_ERR_INTERNAL = 6
+};
+
+enum { PHB_FP64 = 0, PHB_FP32 = 1 };
+enum { PHB_COUL_CUT = 0, PHB_COUL_PME = 1 };
+
+const char *phb_last_error(void);
+int phb_abi_version(void);
+/* Number of visible GPUs (0 when none); never fails on a CPU-only host. */
+int phb_device_count(int *n);
+
+typedef struct { double Lx, Ly, Lz; } phb_box;
+
+/* ------------------------------------------------------------------ atoms */
+/* Device-resident SoA atoms.  Replaces DeviceAtoms (MC/include/device_data.hpp:5-12)
+ * and upload_atoms_to_device (MC/src/main.cu:18-41). */
+typedef struct phb_atoms_s *phb_atoms;
+int phb_atoms_create(int natoms, const double *x, const double *y, const double *z,
+                     const double *q, const int *type0, phb_atoms *out);
+int phb_atoms_destroy(phb_atoms a);
+int phb_atoms_natoms(phb_atoms a, int *natoms);
+/* Batched identity update.  Replaces set_atom_identity/write_device_value
+ * (MC/src/mc.cu:19-41): one transfer for n atoms instead of 2n memcpys. */
+int phb_atoms_set_identity(phb_atoms a, int n, const int *idx, const int *type0, const double *q);
+int phb_atoms_get_identity(phb_atoms a, double *q, int *type0);
+
+/* -------------------------------------------------------------------- CSR */
+/* Device CSR (rows+1 heads, nnz column ids).  Replaces DeviceCSR
+ * (MC/include/device_data.hpp:14-19) and copy_csr_to_device (MC/include/neighbor.hpp:6). */
+typedef struct phb_csr_s *phb_csr;
+int phb_csr_create(int rows, const int *head, const int *list, phb_csr *out);
+int phb_csr_destroy(phb_csr c);
+int phb_csr_shape(phb_csr c, int *rows, long long *nnz);
+int phb_csr_download(phb_csr c, int *head, int *list);
+
+/* Half neighbour list: pairs i<j with min-image r^2 <= rc^2, 1-2 bonded pairs
+ * (bonds12, may be NULL) excluded, rows ascending.  Replaces
+ * build_half_neighbors_gpu (MC/include/neighbor.hpp:10-13, MC/src/neighbor.cu:74-125);
+ * built from binned cells with wavefront-ballot compaction, bit-exact with
+ * the reference's O(N^2) scan. */
+int phb_nlist_build_half(phb_atoms a, const phb_box *box, phb_csr bonds12, double rc, phb_csr *out);
+
+/* ----------------------------------------------------------------- params */
+/* Replaces DeviceParams (MC/include/device_data.hpp:21-36) and
+ * upload_params_to_device (MC/src/main.cu:43-66).  Fields marked EXT extend the
+ * reference; their zero value selects the reference behaviour. */
+typedef struct {
+    int ntypes;
+    const double *lj_eps;      /* ntypes*ntypes row-major (ti*T+tj), 0-based types */
+    const double *lj_sig;
+    int lj_shift;              /* shift each LJ pair to zero at the LJ cutoff */
+    double rc;                 /* cutoff_nm: neighbour list and real-space cutoff */
+    double lj_rc;              /* EXT: LJ cutoff (<= rc); 0 = rc */
+    double epsilon_r;
+    int coulombtype;           /* PHB_COUL_CUT or PHB_COUL_PME */
+    double ewald_alpha;        /* > 0: used as is; else derived from ewald_rtol */
+    double ewald_rtol;
+    double pme_spacing;        /* grid from spacing when nx/ny/nz are 0 */
+    int pme_order;             /* 4 = reference cubic; EXT: 3..8 */
+    int nx, ny, nz;            /* EXT: explicit grid */
+    int pme_3dc;               /* Yeh-Berkowitz slab correction */
+} phb_params_desc;
+typedef struct phb_params_s *phb_params;
+int phb_params_create(const phb_params_desc *d, phb_params *out);
+int phb_params_destroy(phb_params p);
+
+/* Short-range energies over a half list.  Replace compute_lj_sr_energy_gpu and
+ * compute_coul_sr_energy_gpu (MC/include/energy.hpp:7-18). */
+int phb_energy_lj(phb_atoms a, phb_params p, phb_csr neigh, const phb_box *box, double *E);
+int phb_energy_coul_cut(phb_atoms a, phb_params p, phb_csr neigh, const phb_box *box, double *E);
+
+/* -------------------------------------------------------------------- PME */
+/* PMEEnergyComponents (MC/include/pme.hpp:7-12); qcorr_term = qcorr + slab as
+ * in the reference (MC/src/pme.cu:453), slab_term reports the slab part alone. */
+typedef struct {
+    double real_space, recip_space, self_term, qcorr_term, slab_term;
+} phb_pme_components;
+typedef struct phb_pme_s *phb_pme;
+/* Replaces pmeCreate / pmeDestroy (MC/include/pme.hpp:42-43). */
+int phb_pme_create(phb_params p, const phb_box *box, double rc, int precision, phb_pme *out);
+int phb_pme_destroy(phb_pme pme);
+/* The PMEPlan fields main reads (MC/src/main.cu:134-137). */
+int phb_pme_info(phb_pme pme, int *nx, int *ny, int *nz, int *order, double *alpha);
+/* Replaces pmeEnergy (MC/include/pme.hpp:45-49). */
+int phb_pme_energy(phb_pme pme, phb_atoms a, phb_csr neigh, const phb_box *box,
+                   phb_pme_components *comps, double *E);
+/* Replaces the static total_energy (MC/src/mc.cu:43-60): E_LJ + E_Coulomb.
+ * comps (may be NULL): [lj, real_or_cut, recip, self, qcorr, slab]. */
+int phb_total_energy(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box *box,
+                     double *comps, double *E);
+
+/* Stage-level access for parity tests and tooling (no reference counterpart;
+ * the reference keeps these buffers inside PMEPlan). Host buffers in/out. */
+int phb_pme_spread(phb_pme pme, phb_atoms a, double *grid);                 /* nx*ny*nz */
+int phb_pme_fft_r2c(phb_pme pme, const double *grid, double *kgrid);        /* 2*(nx/2+1)*ny*nz */
+int phb_pme_influence(phb_pme pme, double *eta);                            /* (nx/2+1)*ny*nz */
+/* phi = G*grid (unnormalised inverse of eta * FFT(grid)) and E = 0.5 grid.phi */
+int phb_pme_potential(phb_pme pme, const double *grid, double *phi, double *E);
+
+/* ----------------------------------------------------------- generator */
+/* Synthetic brush (stands in for the missing LAMMPS inputs). */
+typedef struct {
+    int chains_x, chains_y;    /* graft lattice */
+    double graft_spacing;      /* first graft point at spacing/2 */
+    double graft_z;            /* z of each chain's first bead */
+    int chain_len;
+    int n_perm;                /* beads 0..n_perm-1 of each chain are permanently +1 */
+    double p_prot;             /* initial protonation probability of titratable beads */
+    int n_salt;                /* salt pairs (cation + Cl-) */
+    double Lx, Ly, Lz;         /* data-file box */
+    double bond;               /* random-walk step */
+    double min_dist;           /* overlap rejection distance */
+    unsigned long long seed;
+    int type_perm, type_cation, type_nh_n, type_nh_p, type_w, type_cl;   /* 1-based */
+} phb_brush_spec;
+/* Call with x == NULL to query natoms/nbonds; types written 1-based. */
+int phb_gen_brush(const phb_brush_spec *spec, int cap, double *x, double *y, double *z, double *q,
+                  int *type1, int *mol, int *bond_i, int *bond_j, int bond_cap, int *natoms, int *nbonds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHB_H */

@@ -1,0 +1,111 @@
+/*
+ * oracle.h -- CPU FP64 restatement of ph_brush's Monte Carlo energy path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in ph_brush_amd/: only tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Every function restates the reference algorithm at the file:line given in
+ * its comment (paths relative to the reference repository root).  Additions
+ * that the reference lacks but the workloads need are marked EXTENSION and
+ * reduce to the reference behaviour at their default values.
+ *
+ * Conventions (same as the reference):
+ *   - positions are wrapped into [0, L) per axis; box is orthorhombic;
+ *   - types are 0-based on the device side (MC/src/main.cu:33);
+ *   - MC type ids in or_mc_params are 1-based, like Params (MC/include/types.hpp:51-54);
+ *   - grids are x-fastest: ((z*ny)+y)*nx + x (MC/src/pme.cu:159);
+ *   - half-spectrum k-grid is ((kz*ny)+ky)*(nx/2+1) + kx, complex interleaved
+ *     (MC/src/pme.cu:225-226).
+ */
+#ifndef PHB_ORACLE_H
+#define PHB_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int natoms;
+    const double *x, *y, *z;   /* wrapped positions */
+    const double *q;           /* charges */
+    const int *type;           /* 0-based type ids */
+    double Lx, Ly, Lz;         /* box (Lz already extended when pme_3dc is on) */
+    const int *bhead;          /* 1-2 bond CSR rows (natoms+1), sorted unique; may be NULL */
+    const int *blist;
+} or_system;
+
+typedef struct {
+    int ntypes;
+    const double *lj_eps;      /* ntypes*ntypes, row-major ti*T+tj */
+    const double *lj_sig;
+    int lj_shift;              /* subtract LJ(lj_rc) per pair */
+    double rc;                 /* neighbour list and real-space cutoff (cutoff_nm) */
+    double lj_rc;              /* EXTENSION: LJ cutoff; <= 0 means rc (reference) */
+    double epsilon_r;
+    int coulombtype;           /* 0 = coul_cut, 1 = pme */
+    double ewald_alpha;        /* > 0 */
+    int pme_order;             /* stencil points per axis (4 = reference cubic) */
+    int nx, ny, nz;            /* explicit grid (EXTENSION; reference derives from spacing) */
+    int pme_3dc;               /* Yeh-Berkowitz slab correction on/off */
+} or_params;
+
+typedef struct {
+    int nh_n_type, nh_p_type, w_type, cl_type;   /* 1-based, MC/include/types.hpp:51-54 */
+    double beta;
+    double mu_eff;             /* fixed chemical-potential offset (energy units) */
+    double w_z_extra;          /* z_cut = max site z + w_z_extra (MC/src/mc.cu:93-96) */
+} or_mc_params;
+
+/* Per-trial record of a GC chain; any pointer may be NULL. */
+typedef struct {
+    signed char *fwd;          /* 1 forward (protonate), 0 backward */
+    int *i, *j;                /* titratable site, swapped ion */
+    signed char *acc;          /* accepted */
+    double *dE;                /* E_trial - E_current */
+    double *expo;              /* clamped Metropolis exponent */
+    double *u;                 /* uniform drawn for the test, NaN when none was drawn */
+} or_mc_record;
+
+/* ---- helpers (MC/src/pme.cu:46-113, MC/src/io.cpp:72-77) ---- */
+int or_good_fft_dim(int n);
+double or_alpha_from_rtol(double rc, double rtol);
+void or_bspline(int order, double t, double *w);
+void or_moduli(int n, int order, double *mod);
+
+/* ---- neighbour list (MC/src/neighbor.cu:21-63, 74-125) ---- */
+long long or_nlist_half(const or_system *s, double rc, int *head, int *list, long long cap);
+
+/* ---- pair energies over the half list ---- */
+double or_energy_lj(const or_system *s, const or_params *p, const int *head, const int *list);
+double or_energy_coul_cut(const or_system *s, const or_params *p, const int *head, const int *list);
+double or_energy_real_ewald(const or_system *s, const or_params *p, const int *head, const int *list);
+
+/* ---- SPME ---- */
+void or_spread(const or_system *s, const or_params *p, double *grid);
+void or_fft_r2c_3d(int nx, int ny, int nz, const double *in, double *out);
+void or_fft_c2r_3d(int nx, int ny, int nz, const double *in, double *out);
+void or_influence(const or_params *p, double Lx, double Ly, double Lz, double *eta);
+double or_recip_energy(const or_params *p, double Lx, double Ly, double Lz, const double *kgrid);
+double or_pme_energy(const or_system *s, const or_params *p, const int *head, const int *list,
+                     double *comps /* [5]: real, recip, self, qcorr, slab */);
+double or_total_energy(const or_system *s, const or_params *p, const int *head, const int *list,
+                       double *comps /* [6]: lj, real, recip, self, qcorr, slab (may be NULL) */);
+
+/* ---- Monte Carlo, fixed-mu GC protonation + ion swap (MC/src/mc.cu:371-503) ---- */
+int or_gc_run(const or_system *s, const or_params *p, const or_mc_params *m,
+              const int *head, const int *list, long long ntrials, unsigned seed,
+              or_mc_record *rec, double *q_out, int *type_out, double *E_out,
+              long long *n_done);
+
+/* ---- mt19937 + libstdc++ distributions, for known-answer tests ---- */
+void or_mt_draws(unsigned seed, long long n, unsigned *out);
+void or_uniform_int_draws(unsigned seed, long long n, const int *range_hi, int *out);
+void or_canonical_draws(unsigned seed, long long n, double *out);
+
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,243 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The CPU FP64 restatement of the reference's energy path and move rule.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; the product (ph_brush_amd/) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_pu = C.POINTER(C.c_uint)
+_pb = C.POINTER(C.c_byte)
+
+
+class OrSystem(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("x", _pd), ("y", _pd), ("z", _pd), ("q", _pd), ("type", _pi),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double), ("bhead", _pi), ("blist", _pi)]
+
+
+class OrParams(C.Structure):
+    _fields_ = [("ntypes", C.c_int), ("lj_eps", _pd), ("lj_sig", _pd), ("lj_shift", C.c_int), ("rc", C.c_double),
+                ("lj_rc", C.c_double), ("epsilon_r", C.c_double), ("coulombtype", C.c_int),
+                ("ewald_alpha", C.c_double), ("pme_order", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("nz", C.c_int), ("pme_3dc", C.c_int)]
+
+
+class OrMcParams(C.Structure):
+    _fields_ = [("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int), ("cl_type", C.c_int),
+                ("beta", C.c_double), ("mu_eff", C.c_double), ("w_z_extra", C.c_double)]
+
+
+class OrRecord(C.Structure):
+    _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("acc", _pb), ("dE", _pd), ("expo", _pd), ("u", _pd)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile liboracle.so with the committed Makefile (g++)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = C.CDLL(str(LIB_PATH))
+        lib.or_good_fft_dim.restype = C.c_int
+        lib.or_alpha_from_rtol.restype = C.c_double
+        lib.or_alpha_from_rtol.argtypes = [C.c_double, C.c_double]
+        lib.or_bspline.argtypes = [C.c_int, C.c_double, _pd]
+        lib.or_moduli.argtypes = [C.c_int, C.c_int, _pd]
+        lib.or_nlist_half.restype = C.c_longlong
+        lib.or_nlist_half.argtypes = [C.POINTER(OrSystem), C.c_double, _pi, _pi, C.c_longlong]
+        for f in ("or_energy_lj", "or_energy_coul_cut", "or_energy_real_ewald"):
+            getattr(lib, f).restype = C.c_double
+            getattr(lib, f).argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), _pi, _pi]
+        lib.or_spread.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), _pd]
+        lib.or_fft_r2c_3d.argtypes = [C.c_int, C.c_int, C.c_int, _pd, _pd]
+        lib.or_fft_c2r_3d.argtypes = [C.c_int, C.c_int, C.c_int, _pd, _pd]
+        lib.or_influence.argtypes = [C.POINTER(OrParams), C.c_double, C.c_double, C.c_double, _pd]
+        lib.or_recip_energy.restype = C.c_double
+        lib.or_recip_energy.argtypes = [C.POINTER(OrParams), C.c_double, C.c_double, C.c_double, _pd]
+        lib.or_pme_energy.restype = C.c_double
+        lib.or_pme_energy.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), _pi, _pi, _pd]
+        lib.or_total_energy.restype = C.c_double
+        lib.or_total_energy.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), _pi, _pi, _pd]
+        lib.or_gc_run.restype = C.c_int
+        lib.or_gc_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
+                                  C.c_longlong, C.c_uint, C.POINTER(OrRecord), _pd, _pi, _pd,
+                                  C.POINTER(C.c_longlong)]
+        lib.or_mt_draws.argtypes = [C.c_uint, C.c_longlong, _pu]
+        lib.or_uniform_int_draws.argtypes = [C.c_uint, C.c_longlong, _pi, _pi]
+        lib.or_canonical_draws.argtypes = [C.c_uint, C.c_longlong, _pd]
+        lib.or_set_threads.argtypes = [C.c_int]
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_pd)
+
+
+class Oracle:
+    """Holds ctypes views of one System + Params (keeps arrays alive)."""
+
+    def __init__(self, s, p, bonds_csr=None):
+        self.lib = load()
+        self.s, self.p = s, p
+        # private copies: set_charges must never write through to the caller's arrays
+        self._arr = [np.array(a, dtype=np.float64, copy=True) for a in (s.x, s.y, s.z, s.q)]
+        self._t = np.array(s.type0, dtype=np.int32, copy=True)
+        if bonds_csr is not None:
+            self._bh = np.ascontiguousarray(bonds_csr[0], np.int32)
+            self._bl = np.ascontiguousarray(bonds_csr[1] if len(bonds_csr[1]) else np.zeros(1), np.int32)
+            bh, bl = self._bh.ctypes.data_as(_pi), self._bl.ctypes.data_as(_pi)
+        else:
+            bh, bl = None, None
+        self.sys = OrSystem(s.natoms, *(a.ctypes.data_as(_pd) for a in self._arr), self._t.ctypes.data_as(_pi),
+                            s.Lx, s.Ly, s.Lz, bh, bl)
+        self._eps = np.ascontiguousarray(p.lj_eps, np.float64)
+        self._sig = np.ascontiguousarray(p.lj_sig, np.float64)
+        nx, ny, nz = p.grid
+        self.par = OrParams(p.ntypes, self._eps.ctypes.data_as(_pd), self._sig.ctypes.data_as(_pd), int(p.lj_shift),
+                            p.rc, p.lj_rc, p.epsilon_r, p.coulombtype, p.ewald_alpha, p.pme_order, nx, ny, nz,
+                            int(p.pme_3dc))
+        self.head = None
+        self.list = None
+
+    def set_charges(self, q, type0):
+        self._arr[3][:] = q
+        self._t[:] = type0
+
+    def nlist(self, rc=None):
+        rc = self.p.rc if rc is None else rc
+        head = np.zeros(self.s.natoms + 1, np.int32)
+        nnz = self.lib.or_nlist_half(C.byref(self.sys), rc, head.ctypes.data_as(_pi), None, 0)
+        lst = np.zeros(max(nnz, 1), np.int32)
+        self.lib.or_nlist_half(C.byref(self.sys), rc, head.ctypes.data_as(_pi), lst.ctypes.data_as(_pi), nnz)
+        self.head, self.list = head, lst
+        return head, lst[:nnz]
+
+    def _hl(self):
+        if self.head is None:
+            self.nlist()
+        return self.head.ctypes.data_as(_pi), self.list.ctypes.data_as(_pi)
+
+    def energy_lj(self):
+        return self.lib.or_energy_lj(C.byref(self.sys), C.byref(self.par), *self._hl())
+
+    def energy_coul_cut(self):
+        return self.lib.or_energy_coul_cut(C.byref(self.sys), C.byref(self.par), *self._hl())
+
+    def energy_real(self):
+        return self.lib.or_energy_real_ewald(C.byref(self.sys), C.byref(self.par), *self._hl())
+
+    def pme_energy(self):
+        c = np.zeros(5)
+        e = self.lib.or_pme_energy(C.byref(self.sys), C.byref(self.par), *self._hl(), c.ctypes.data_as(_pd))
+        return e, c
+
+    def total_energy(self):
+        c = np.zeros(6)
+        e = self.lib.or_total_energy(C.byref(self.sys), C.byref(self.par), *self._hl(), c.ctypes.data_as(_pd))
+        return e, c
+
+    def spread(self):
+        nx, ny, nz = self.p.grid
+        g = np.zeros(nx * ny * nz)
+        self.lib.or_spread(C.byref(self.sys), C.byref(self.par), g.ctypes.data_as(_pd))
+        return g
+
+    def influence(self):
+        nx, ny, nz = self.p.grid
+        e = np.zeros((nx // 2 + 1) * ny * nz)
+        self.lib.or_influence(C.byref(self.par), self.s.Lx, self.s.Ly, self.s.Lz, e.ctypes.data_as(_pd))
+        return e
+
+    def gc_run(self, mc, ntrials, seed, record=True):
+        """Fixed-mu GC chain with full recompute per trial (the reference algorithm)."""
+        N = self.s.natoms
+        rec = {k: np.zeros(ntrials, dt) for k, dt in
+               (("fwd", np.int8), ("i", np.int32), ("j", np.int32), ("acc", np.int8), ("dE", np.float64),
+                ("expo", np.float64), ("u", np.float64))}
+        r = OrRecord(*(rec[k].ctypes.data_as(t) for k, t in
+                       (("fwd", _pb), ("i", _pi), ("j", _pi), ("acc", _pb), ("dE", _pd), ("expo", _pd), ("u", _pd))))
+        m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
+        q, t = np.zeros(N), np.zeros(N, np.int32)
+        E = C.c_double()
+        nd = C.c_longlong()
+        st = self.lib.or_gc_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), ntrials, seed,
+                                C.byref(r) if record else None, q.ctypes.data_as(_pd), t.ctypes.data_as(_pi),
+                                C.byref(E), C.byref(nd))
+        n = nd.value
+        return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}
+
+
+def good_fft_dim(n):
+    return load().or_good_fft_dim(n)
+
+
+def alpha_from_rtol(rc, rtol):
+    return load().or_alpha_from_rtol(rc, rtol)
+
+
+def bspline(order, t):
+    w = np.zeros(order)
+    load().or_bspline(order, t, w.ctypes.data_as(_pd))
+    return w
+
+
+def moduli(n, order):
+    m = np.zeros(n)
+    load().or_moduli(n, order, m.ctypes.data_as(_pd))
+    return m
+
+
+def fft_r2c_3d(grid, nx, ny, nz):
+    out = np.zeros(2 * (nx // 2 + 1) * ny * nz)
+    load().or_fft_r2c_3d(nx, ny, nz, _d(grid), out.ctypes.data_as(_pd))
+    return out.view(np.complex128)
+
+
+def fft_c2r_3d(kgrid, nx, ny, nz):
+    k = np.ascontiguousarray(kgrid, np.complex128).view(np.float64)
+    out = np.zeros(nx * ny * nz)
+    load().or_fft_c2r_3d(nx, ny, nz, k.ctypes.data_as(_pd), out.ctypes.data_as(_pd))
+    return out
+
+
+def mt_draws(seed, n):
+    out = np.zeros(n, np.uint32)
+    load().or_mt_draws(seed, n, out.ctypes.data_as(_pu))
+    return out
+
+
+def uniform_int_draws(seed, his):
+    his = np.ascontiguousarray(his, np.int32)
+    out = np.zeros(len(his), np.int32)
+    load().or_uniform_int_draws(seed, len(his), his.ctypes.data_as(_pi), out.ctypes.data_as(_pi))
+    return out
+
+
+def canonical_draws(seed, n):
+    out = np.zeros(n)
+    load().or_canonical_draws(seed, n, out.ctypes.data_as(_pd))
+    return out
+
+
+def set_threads(n):
+    load().or_set_threads(n)

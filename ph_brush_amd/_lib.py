@@ -1,0 +1,156 @@
+"""ctypes bindings of the C ABI declared in include/phb.h.
+
+The library is loaded from ph_brush_amd/libphb.so (built in-tree by
+ph_brush_amd.build).  There is no CPU fallback: if the library is missing, or
+a compute call is made without a GPU, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libphb.so"
+
+PHB_OK = 0
+PHB_FP64, PHB_FP32 = 0, 1
+PHB_COUL_CUT, PHB_COUL_PME = 0, 1
+
+_ERRNAMES = {1: "ARG", 2: "HIP", 3: "ALLOC", 4: "STATE", 5: "UNSUPPORTED", 6: "INTERNAL"}
+
+
+class PhbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[PHB_ERR_{_ERRNAMES.get(code, code)}] {msg}")
+        self.code = code
+
+
+class Box(C.Structure):
+    _fields_ = [("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double)]
+
+
+class ParamsDesc(C.Structure):
+    _fields_ = [
+        ("ntypes", C.c_int), ("lj_eps", C.POINTER(C.c_double)), ("lj_sig", C.POINTER(C.c_double)),
+        ("lj_shift", C.c_int), ("rc", C.c_double), ("lj_rc", C.c_double), ("epsilon_r", C.c_double),
+        ("coulombtype", C.c_int), ("ewald_alpha", C.c_double), ("ewald_rtol", C.c_double),
+        ("pme_spacing", C.c_double), ("pme_order", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+        ("nz", C.c_int), ("pme_3dc", C.c_int),
+    ]
+
+
+class PmeComponents(C.Structure):
+    _fields_ = [("real_space", C.c_double), ("recip_space", C.c_double), ("self_term", C.c_double),
+                ("qcorr_term", C.c_double), ("slab_term", C.c_double)]
+
+
+class BrushSpec(C.Structure):
+    _fields_ = [
+        ("chains_x", C.c_int), ("chains_y", C.c_int), ("graft_spacing", C.c_double), ("graft_z", C.c_double),
+        ("chain_len", C.c_int), ("n_perm", C.c_int), ("p_prot", C.c_double), ("n_salt", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double), ("bond", C.c_double),
+        ("min_dist", C.c_double), ("seed", C.c_ulonglong),
+        ("type_perm", C.c_int), ("type_cation", C.c_int), ("type_nh_n", C.c_int), ("type_nh_p", C.c_int),
+        ("type_w", C.c_int), ("type_cl", C.c_int),
+    ]
+
+
+class McDesc(C.Structure):
+    _fields_ = [
+        ("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int), ("cl_type", C.c_int),
+        ("beta", C.c_double), ("mu_eff", C.c_double), ("w_z_extra", C.c_double),
+        ("precision", C.c_int), ("pending_max", C.c_int), ("spec_window", C.c_int),
+        ("record", C.c_int),
+    ]
+
+
+class McStats(C.Structure):
+    _fields_ = [
+        ("attempted", C.c_longlong), ("accepted", C.c_longlong), ("accepted_forward", C.c_longlong),
+        ("accepted_backward", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int),
+        ("E", C.c_double), ("refreshes", C.c_longlong), ("status", C.c_int),
+    ]
+
+
+_h = C.c_void_p
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_pll = C.POINTER(C.c_longlong)
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "phb_last_error": [],
+    "phb_abi_version": [],
+    "phb_device_count": [_pi],
+    "phb_atoms_create": [C.c_int, _pd, _pd, _pd, _pd, _pi, C.POINTER(_h)],
+    "phb_atoms_destroy": [_h],
+    "phb_atoms_natoms": [_h, _pi],
+    "phb_atoms_set_identity": [_h, C.c_int, _pi, _pi, _pd],
+    "phb_atoms_get_identity": [_h, _pd, _pi],
+    "phb_csr_create": [C.c_int, _pi, _pi, C.POINTER(_h)],
+    "phb_csr_destroy": [_h],
+    "phb_csr_shape": [_h, _pi, _pll],
+    "phb_csr_download": [_h, _pi, _pi],
+    "phb_nlist_build_half": [_h, C.POINTER(Box), _h, C.c_double, C.POINTER(_h)],
+    "phb_params_create": [C.POINTER(ParamsDesc), C.POINTER(_h)],
+    "phb_params_destroy": [_h],
+    "phb_energy_lj": [_h, _h, _h, C.POINTER(Box), _pd],
+    "phb_energy_coul_cut": [_h, _h, _h, C.POINTER(Box), _pd],
+    "phb_pme_create": [_h, C.POINTER(Box), C.c_double, C.c_int, C.POINTER(_h)],
+    "phb_pme_destroy": [_h],
+    "phb_pme_info": [_h, _pi, _pi, _pi, _pi, _pd],
+    "phb_pme_energy": [_h, _h, _h, C.POINTER(Box), C.POINTER(PmeComponents), _pd],
+    "phb_total_energy": [_h, _h, _h, _h, C.POINTER(Box), _pd, _pd],
+    "phb_pme_spread": [_h, _h, _pd],
+    "phb_pme_fft_r2c": [_h, _pd, _pd],
+    "phb_pme_influence": [_h, _pd],
+    "phb_pme_potential": [_h, _pd, _pd, _pd],
+    "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
+}
+_RESTYPE = {"phb_last_error": C.c_char_p}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libphb.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m ph_brush_amd.build` "
+                          "(there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        if not hasattr(lib, name):
+            continue
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _RESTYPE.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point; raise PhbError on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != PHB_OK:
+        raise PhbError(rc, lib.phb_last_error().decode(errors="replace"))
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pd)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pi)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    call("phb_device_count", C.byref(n))
+    return n.value

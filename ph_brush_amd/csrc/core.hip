@@ -1,0 +1,275 @@
+// core.hip -- error state, library stream, FFT plans, reductions and the
+// atoms / CSR / params handles of the C ABI.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "device.hpp"
+#include "handles.hpp"
+
+namespace phb {
+
+namespace {
+thread_local std::string g_last_error;
+hipStream_t g_stream = nullptr;
+std::once_flag g_stream_once;
+}  // namespace
+
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+void require_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        throw Error(PHB_ERR_UNSUPPORTED, "no GPU visible: ph_brush_amd has no CPU fallback");
+}
+
+hipStream_t stream() {
+    require_device();
+    std::call_once(g_stream_once, [] {
+        if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) g_stream = nullptr;
+    });
+    if (!g_stream) throw Error(PHB_ERR_HIP, "could not create the library stream");
+    return g_stream;
+}
+
+void sync() { PHB_HIP(hipStreamSynchronize(stream())); }
+
+bool fft_len_ok(int n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5}) while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+FftPlan make_fft_plan(int n) {
+    if (!fft_len_ok(n)) throw Error(PHB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not 2/3/5-smooth");
+    FftPlan p;
+    p.n = n;
+    int m = n;
+    while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
+    return p;
+}
+
+std::vector<cplx<double>> twiddles(int n) {
+    std::vector<cplx<double>> w(n);
+    for (int k = 0; k < n; ++k) {
+        double a = 2.0 * M_PI * double(k) / double(n);
+        w[k] = {std::cos(a), -std::sin(a)};
+    }
+    return w;
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partials, int n, int stride, int nsum,
+                                  double* __restrict__ out) {
+    __shared__ double sred[16];
+    for (int c = 0; c < nsum; ++c) {
+        double v = 0.0;
+        // fixed assignment of partials to threads, fixed tree: deterministic
+        for (int k = threadIdx.x; k < n; k += blockDim.x) v += partials[(size_t)k * stride + c];
+        v = block_sum(v, sred);
+        if (threadIdx.x == 0) out[c] = v;
+        __syncthreads();
+    }
+}
+
+void reduce_partials(const double* d_partials, int n, int stride, int nsum, double* d_out) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, stream(), d_partials, n, stride, nsum, d_out);
+    PHB_HIP(hipGetLastError());
+}
+
+// One block of 1024 threads: each thread owns a contiguous segment.
+__global__ void k_exclusive_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+    __shared__ long long part[1024];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int seg = (n + T - 1) / T;
+    const int b = min(n, t * seg), e = min(n, b + seg);
+    long long s = 0;
+    for (int k = b; k < e; ++k) s += in[k];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        long long run = 0;
+        for (int k = 0; k < T; ++k) { long long v = part[k]; part[k] = run; run += v; }
+        out[n] = (int)run;
+    }
+    __syncthreads();
+    long long run = part[t];
+    for (int k = b; k < e; ++k) { out[k] = (int)run; run += in[k]; }
+}
+
+void exclusive_scan(const int* d_in, int* d_out, int n) {
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, stream(), d_in, d_out, n);
+    PHB_HIP(hipGetLastError());
+}
+
+}  // namespace phb
+
+using namespace phb;
+
+// ----------------------------------------------------------------------- ABI
+extern "C" const char* phb_last_error(void) { return g_last_error.c_str(); }
+extern "C" int phb_abi_version(void) { return PHB_ABI_VERSION; }
+
+extern "C" int phb_device_count(int* n) {
+    return guard([&] {
+        require(n != nullptr, "phb_device_count: null");
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+        *n = c;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_atoms_create(int natoms, const double* x, const double* y, const double* z,
+                                const double* q, const int* type0, phb_atoms* out) {
+    return guard([&] {
+        require(out && natoms >= 0, "phb_atoms_create: bad arguments");
+        require(natoms == 0 || (x && y && z && q && type0), "phb_atoms_create: null array");
+        require_device();
+        auto a = std::make_unique<phb_atoms_s>();
+        a->n = natoms;
+        a->x.alloc(natoms); a->y.alloc(natoms); a->z.alloc(natoms);
+        a->q.alloc(natoms); a->type.alloc(natoms);
+        a->x.upload(x, natoms); a->y.upload(y, natoms); a->z.upload(z, natoms);
+        a->q.upload(q, natoms); a->type.upload(type0, natoms);
+        sync();
+        *out = a.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_atoms_destroy(phb_atoms a) {
+    return guard([&] { delete a; return PHB_OK; });
+}
+
+extern "C" int phb_atoms_natoms(phb_atoms a, int* n) {
+    return guard([&] { require(a && n, "phb_atoms_natoms: null"); *n = a->n; return PHB_OK; });
+}
+
+namespace {
+__global__ void k_set_identity(int n, const int* __restrict__ idx, const int* __restrict__ t,
+                               const double* __restrict__ qv, int natoms, int* __restrict__ type,
+                               double* __restrict__ q) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        int i = idx[k];
+        if (i >= 0 && i < natoms) { type[i] = t[k]; q[i] = qv[k]; }
+    }
+}
+}  // namespace
+
+extern "C" int phb_atoms_set_identity(phb_atoms a, int n, const int* idx, const int* type0, const double* q) {
+    return guard([&] {
+        require(a && n >= 0 && (n == 0 || (idx && type0 && q)), "phb_atoms_set_identity: bad arguments");
+        for (int k = 0; k < n; ++k)
+            require(idx[k] >= 0 && idx[k] < a->n, "set_atom_identity: atom index out of range");
+        if (n == 0) return PHB_OK;
+        DevBuf<int> di(n), dt(n);
+        DevBuf<double> dq(n);
+        di.upload(idx, n); dt.upload(type0, n); dq.upload(q, n);
+        hipLaunchKernelGGL(k_set_identity, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, di.p, dt.p, dq.p,
+                           a->n, a->type.p, a->q.p);
+        PHB_HIP(hipGetLastError());
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_atoms_get_identity(phb_atoms a, double* q, int* type0) {
+    return guard([&] {
+        require(a != nullptr, "phb_atoms_get_identity: null");
+        if (q) a->q.download(q, a->n);
+        if (type0) a->type.download(type0, a->n);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_csr_create(int rows, const int* head, const int* list, phb_csr* out) {
+    return guard([&] {
+        require(out && rows >= 0 && (rows == 0 || head), "phb_csr_create: bad arguments");
+        long long nnz = rows > 0 ? head[rows] : 0;
+        require(nnz >= 0 && (nnz == 0 || list), "phb_csr_create: bad nnz/list");
+        for (int i = 0; i < rows; ++i) require(head[i] <= head[i + 1], "phb_csr_create: heads not monotone");
+        require_device();
+        auto c = std::make_unique<phb_csr_s>();
+        c->rows = rows; c->nnz = nnz;
+        c->head.alloc(rows + 1);
+        std::vector<int> h0;
+        if (rows == 0) { h0.assign(1, 0); head = h0.data(); }
+        c->head.upload(head, rows + 1);
+        c->list.alloc(nnz);
+        c->list.upload(list, nnz);
+        sync();
+        *out = c.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_csr_destroy(phb_csr c) {
+    return guard([&] { delete c; return PHB_OK; });
+}
+
+extern "C" int phb_csr_shape(phb_csr c, int* rows, long long* nnz) {
+    return guard([&] {
+        require(c != nullptr, "phb_csr_shape: null");
+        if (rows) *rows = c->rows;
+        if (nnz) *nnz = c->nnz;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_csr_download(phb_csr c, int* head, int* list) {
+    return guard([&] {
+        require(c != nullptr, "phb_csr_download: null");
+        if (head) c->head.download(head, c->rows + 1);
+        if (list) c->list.download(list, c->nnz);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_params_create(const phb_params_desc* d, phb_params* out) {
+    return guard([&] {
+        require(d && out, "phb_params_create: null");
+        require(d->ntypes > 0 && d->lj_eps && d->lj_sig, "phb_params_create: LJ tables required");
+        // MC/src/io.cpp:258
+        require(d->rc > 0.0, "cutoff_nm must be > 0 in params");
+        require(d->lj_rc >= 0.0 && d->lj_rc <= d->rc, "lj_rc must lie in [0, rc]");
+        require(d->epsilon_r > 0.0, "epsilon_r must be > 0");
+        require(d->coulombtype == PHB_COUL_CUT || d->coulombtype == PHB_COUL_PME, "unsupported coulombtype");
+        require_device();
+        auto p = std::make_unique<phb_params_s>();
+        p->d = *d;
+        const int T = d->ntypes;
+        p->eps.assign(d->lj_eps, d->lj_eps + (size_t)T * T);
+        p->sig.assign(d->lj_sig, d->lj_sig + (size_t)T * T);
+        p->lj_rc = d->lj_rc > 0 ? d->lj_rc : d->rc;
+        p->lj_cut_active = d->lj_rc > 0 && d->lj_rc < d->rc;
+        // MC/src/io.cpp:270-281, evaluated at the LJ cutoff
+        p->shift.assign((size_t)T * T, 0.0);
+        if (d->lj_shift)
+            for (int i = 0; i < T * T; ++i) {
+                double eps = p->eps[i], sig = p->sig[i];
+                if (eps > 0.0 && sig > 0.0) {
+                    double sr = sig / p->lj_rc, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+                    p->shift[i] = 4.0 * eps * (sr12 - sr6);
+                }
+            }
+        p->d.lj_eps = nullptr; p->d.lj_sig = nullptr;
+        p->d_eps.alloc(T * T); p->d_sig.alloc(T * T); p->d_shift.alloc(T * T);
+        p->d_eps.upload(p->eps.data(), T * T);
+        p->d_sig.upload(p->sig.data(), T * T);
+        p->d_shift.upload(p->shift.data(), T * T);
+        sync();
+        *out = p.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_params_destroy(phb_params p) {
+    return guard([&] { delete p; return PHB_OK; });
+}

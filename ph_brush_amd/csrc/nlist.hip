@@ -1,0 +1,193 @@
+// nlist.hip -- half neighbour list from binned cells (replaces the reference's
+// O(N^2) k_count_half / k_fill_half, MC/src/neighbor.cu:21-63).
+//
+// Predicate, identical to the reference so that the CSR is bit-exact:
+//   i < j, j not 1-2 bonded to i, min-image r^2 <= rc^2 (inclusive), with
+//   dx -= L*rint(dx/L) and r^2 = dx*dx + dy*dy + dz*dz rounded as written.
+// Rows come out ascending in j (the reference's order, MC/src/neighbor.cu:55-61).
+//
+// Layout: atoms are counting-sorted into cells of width >= rc*(1+1e-9) (one
+// cell along an axis shorter than three widths).  One wave per atom i walks
+// the 27 neighbouring cells 64 candidates at a time; a wavefront ballot
+// compacts the hits.  Count pass -> scan -> fill pass -> per-row rank sort.
+#include <algorithm>
+#include <memory>
+
+#include "device.hpp"
+#include "handles.hpp"
+
+namespace {
+
+using namespace phb;
+
+struct CellGeom {
+    int ncx, ncy, ncz;
+    double Lx, Ly, Lz;
+};
+
+__device__ __forceinline__ int cell_of(double v, double L, int nc) {
+    int c = (int)floor(v / L * nc);
+    return min(max(c, 0), nc - 1);
+}
+
+__global__ void k_cell_count(const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, int N, CellGeom g, int* __restrict__ cell,
+                             int* __restrict__ ccount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = (cell_of(z[i], g.Lz, g.ncz) * g.ncy + cell_of(y[i], g.Ly, g.ncy)) * g.ncx + cell_of(x[i], g.Lx, g.ncx);
+    cell[i] = c;
+    atomicAdd(&ccount[c], 1);
+}
+
+__global__ void k_cell_fill(const int* __restrict__ cell, int N, const int* __restrict__ cstart,
+                            int* __restrict__ cursor, int* __restrict__ catoms) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = cell[i];
+    int slot = atomicAdd(&cursor[c], 1);
+    catoms[cstart[c] + slot] = i;
+}
+
+__device__ __forceinline__ bool bonded12(int i, int j, const int* __restrict__ bh, const int* __restrict__ bl) {
+    if (!bh) return false;
+    int lo = bh[i], hi = bh[i + 1] - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        int v = bl[mid];
+        if (v == j) return true;
+        if (v < j) lo = mid + 1; else hi = mid - 1;
+    }
+    return false;
+}
+
+// MODE 0: count into counts[i]; MODE 1: write unsorted row into tmp at head[i].
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, const double* __restrict__ y,
+                                               const double* __restrict__ z, int N, CellGeom g, double rc2,
+                                               const int* __restrict__ cell, const int* __restrict__ cstart,
+                                               const int* __restrict__ catoms, const int* __restrict__ bh,
+                                               const int* __restrict__ bl, int* __restrict__ counts,
+                                               const int* __restrict__ head, int* __restrict__ tmp) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= N) return;  // wave-uniform
+    const double xi = x[i], yi = y[i], zi = z[i];
+    const int c = cell[i];
+    const int cx = c % g.ncx, cy = (c / g.ncx) % g.ncy, cz = c / (g.ncx * g.ncy);
+    const int ox = g.ncx >= 3 ? 1 : 0, oy = g.ncy >= 3 ? 1 : 0, oz = g.ncz >= 3 ? 1 : 0;
+    int found = 0;
+    const int base = MODE == 1 ? head[i] : 0;
+    for (int dz = -oz; dz <= oz; ++dz) {
+        const int kz = (cz + dz + g.ncz) % g.ncz;
+        for (int dy = -oy; dy <= oy; ++dy) {
+            const int ky = (cy + dy + g.ncy) % g.ncy;
+            for (int dx = -ox; dx <= ox; ++dx) {
+                const int kx = (cx + dx + g.ncx) % g.ncx;
+                const int kc = (kz * g.ncy + ky) * g.ncx + kx;
+                const int s = cstart[kc], e = cstart[kc + 1];
+                for (int b = s; b < e; b += 64) {
+                    const int idx = b + lane;
+                    bool pass = false;
+                    int j = -1;
+                    if (idx < e) {
+                        j = catoms[idx];
+                        if (j > i && !bonded12(i, j, bh, bl)) {
+                            double ddx = min_image_exact(xi - x[j], g.Lx);
+                            double ddy = min_image_exact(yi - y[j], g.Ly);
+                            double ddz = min_image_exact(zi - z[j], g.Lz);
+                            pass = r2_exact(ddx, ddy, ddz) <= rc2;
+                        }
+                    }
+                    const unsigned long long m = __ballot(pass);
+                    if (MODE == 1 && pass) {
+                        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+                        tmp[base + found + pre] = j;
+                    }
+                    found += __popcll(m);
+                }
+            }
+        }
+    }
+    if (MODE == 0 && lane == 0) counts[i] = found;
+}
+
+// One wave per row: rank sort (entries are distinct).
+__global__ __launch_bounds__(256) void k_sort_rows(int N, const int* __restrict__ head, const int* __restrict__ tmp,
+                                                   int* __restrict__ list) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= N) return;
+    const int s = head[i], e = head[i + 1];
+    for (int a = s + lane; a < e; a += 64) {
+        const int v = tmp[a];
+        int rank = 0;
+        for (int b = s; b < e; ++b) rank += tmp[b] < v ? 1 : 0;
+        list[s + rank] = v;
+    }
+}
+
+int cells_along(double L, double rc) {
+    int nc = (int)std::floor(L / (rc * (1.0 + 1e-9)));
+    return nc < 3 ? 1 : nc;
+}
+
+}  // namespace
+
+extern "C" int phb_nlist_build_half(phb_atoms a, const phb_box* box, phb_csr bonds12, double rc, phb_csr* out) {
+    return guard([&] {
+        require(a && box && out, "build_half_neighbors_gpu: null argument");
+        require(rc > 0.0, "build_half_neighbors_gpu: rc must be > 0");
+        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "build_half_neighbors_gpu: box must be positive");
+        const int N = a->n;
+        if (bonds12) require(bonds12->rows == N, "build_half_neighbors_gpu: bond CSR rows != natoms");
+        auto c = std::make_unique<phb_csr_s>();
+        c->rows = N;
+        c->head.alloc(N + 1);
+        if (N == 0) {
+            int zero = 0;
+            c->head.upload(&zero, 1);
+            sync();
+            *out = c.release();
+            return PHB_OK;
+        }
+        CellGeom g{cells_along(box->Lx, rc), cells_along(box->Ly, rc), cells_along(box->Lz, rc), box->Lx, box->Ly,
+                   box->Lz};
+        const long long ncell = (long long)g.ncx * g.ncy * g.ncz;
+        require(ncell < (1LL << 30), "build_half_neighbors_gpu: too many cells");
+        hipStream_t st = stream();
+        DevBuf<int> cell(N), ccount(ncell), cstart(ncell + 1), cursor(ncell), catoms(N), counts(N);
+        PHB_HIP(hipMemsetAsync(ccount.p, 0, ncell * sizeof(int), st));
+        PHB_HIP(hipMemsetAsync(cursor.p, 0, ncell * sizeof(int), st));
+        const int tb = 256;
+        hipLaunchKernelGGL(k_cell_count, dim3(ceil_div(N, tb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, N, g, cell.p,
+                           ccount.p);
+        exclusive_scan(ccount.p, cstart.p, (int)ncell);
+        hipLaunchKernelGGL(k_cell_fill, dim3(ceil_div(N, tb)), dim3(tb), 0, st, cell.p, N, cstart.p, cursor.p,
+                           catoms.p);
+        const int* bh = bonds12 && bonds12->nnz > 0 ? bonds12->head.p : nullptr;
+        const int* bl = bonds12 && bonds12->nnz > 0 ? bonds12->list.p : nullptr;
+        const double rc2 = rc * rc;
+        const int wpb = tb / 64;
+        hipLaunchKernelGGL(k_pairs<0>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, N, g, rc2,
+                           cell.p, cstart.p, catoms.p, bh, bl, counts.p, (const int*)nullptr, (int*)nullptr);
+        PHB_HIP(hipGetLastError());
+        exclusive_scan(counts.p, c->head.p, N);
+        int total = 0;
+        PHB_HIP(hipMemcpyAsync(&total, c->head.p + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        PHB_HIP(hipStreamSynchronize(st));
+        require(total >= 0, "build_half_neighbors_gpu: neighbour count overflow", PHB_ERR_INTERNAL);
+        c->nnz = total;
+        c->list.alloc(total);
+        if (total > 0) {
+            DevBuf<int> tmp(total);
+            hipLaunchKernelGGL(k_pairs<1>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, N, g,
+                               rc2, cell.p, cstart.p, catoms.p, bh, bl, (int*)nullptr, c->head.p, tmp.p);
+            hipLaunchKernelGGL(k_sort_rows, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, N, c->head.p, tmp.p, c->list.p);
+            PHB_HIP(hipGetLastError());
+            sync();
+        }
+        *out = c.release();
+        return PHB_OK;
+    });
+}

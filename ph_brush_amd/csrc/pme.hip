@@ -1,0 +1,750 @@
+// pme.hip -- short-range pair energies, SPME (spread, hand-written LDS FFT,
+// influence-function convolution, reciprocal energy) and the Ewald self,
+// net-charge and slab corrections.
+//
+// Replaces compute_lj_sr_energy_gpu / compute_coul_sr_energy_gpu
+// (MC/src/energy_gpu.cu:14-135) and pmeCreate / pmeEnergy (MC/src/pme.cu:291-457).
+//
+// FFT layout: the real grid is x-fastest [nz][ny][nx]; the half spectrum is
+// [nz][ny][nx/2+1].  k_fft_xy_r2c transforms one z-plane per workgroup
+// entirely in LDS (pairs of real rows packed as one complex row for the x
+// pass, untangled, then the y pass); k_fft_z loads a tile of kx-columns of
+// one ky, transforms along z in LDS, applies the influence function (energy
+// and/or convolution) and optionally transforms back; k_fft_xy_c2r is the
+// inverse plane pass.  Every energy is reduced per workgroup into FP64
+// partials and then summed in a fixed order (deterministic).
+#include <algorithm>
+#include <memory>
+
+#include "device.hpp"
+#include "handles.hpp"
+#include "pme.hpp"
+
+namespace phb {
+
+// ------------------------------------------------------------ pair energy
+enum { PAIR_LJ = 1, PAIR_COUL_CUT = 2, PAIR_EWALD = 4 };
+
+struct PairArgs {
+    const double *x, *y, *z, *q;
+    const int* type;
+    const int *head, *list;
+    int N, T;
+    double Lx, Ly, Lz;
+    const double *eps, *sig, *shift;
+    int use_shift, use_cut;
+    double lj_rc2, kf, alpha;
+};
+
+// One thread per row i (grid-stride); FP64 per-thread sums (T = double) or
+// FP32 row sums (T = float) folded into FP64, block partials in FP64.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_pair_energy(PairArgs A, double* __restrict__ partials) {
+    __shared__ double sred[16];
+    double e_lj = 0.0, e_c = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.N; i += gridDim.x * blockDim.x) {
+        const double xi = A.x[i], yi = A.y[i], zi = A.z[i], qi = A.q[i];
+        const int ti = A.type[i];
+        T row_lj = 0, row_c = 0;
+        for (int k = A.head[i], ke = A.head[i + 1]; k < ke; ++k) {
+            const int j = A.list[k];
+            const double dx = min_image_exact(xi - A.x[j], A.Lx);
+            const double dy = min_image_exact(yi - A.y[j], A.Ly);
+            const double dz = min_image_exact(zi - A.z[j], A.Lz);
+            const double r2d = r2_exact(dx, dy, dz);
+            const T r2 = (T)r2d;
+            const T r = sqrt(r2);
+            if (MODE & PAIR_LJ) {
+                if (!A.use_cut || r2d <= A.lj_rc2) {
+                    const int tj = A.type[j];
+                    const T eps = (T)A.eps[ti * A.T + tj], sig = (T)A.sig[ti * A.T + tj];
+                    if (eps > T(0) && sig > T(0)) {
+                        const T sr = sig / r, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+                        T e = T(4.0) * eps * (sr12 - sr6);
+                        if (A.use_shift) e -= (T)A.shift[ti * A.T + tj];
+                        row_lj += e;
+                    }
+                }
+            }
+            if (MODE & (PAIR_COUL_CUT | PAIR_EWALD)) {
+                const T qq = (T)(qi * A.q[j]);
+                if (qq != T(0)) {
+                    if (MODE & PAIR_COUL_CUT) row_c += (T)A.kf * qq * (T(1.0) / r);
+                    else row_c += (T)A.kf * qq * (erfc((T)A.alpha * r) / r);
+                }
+            }
+        }
+        e_lj += (double)row_lj;
+        e_c += (double)row_c;
+    }
+    double s0 = block_sum(e_lj, sred);
+    if (threadIdx.x == 0) partials[2 * blockIdx.x] = s0;
+    double s1 = block_sum(e_c, sred);
+    if (threadIdx.x == 0) partials[2 * blockIdx.x + 1] = s1;
+}
+
+// ------------------------------------------------------------ corrections
+// Q, sum q^2, Mz = sum q z, sum q z^2 (MC/src/pme.cu:427-443).
+__global__ __launch_bounds__(256) void k_charge_moments(const double* __restrict__ q, const double* __restrict__ z,
+                                                        int N, double* __restrict__ partials) {
+    __shared__ double sred[16];
+    double s[4] = {0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double qi = q[i], zi = z[i];
+        s[0] += qi; s[1] += qi * qi; s[2] += qi * zi; s[3] += qi * zi * zi;
+    }
+    for (int c = 0; c < 4; ++c) {
+        double v = block_sum(s[c], sred);
+        if (threadIdx.x == 0) partials[4 * blockIdx.x + c] = v;
+    }
+}
+
+// ------------------------------------------------------------ spread
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, size_t n, T v) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+// Order-p B-spline spreading (MC/src/pme.cu:140-172 at p = 4) with hardware
+// floating-point atomics; atoms with q = 0 contribute nothing and are skipped.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ z, const double* __restrict__ q, int N,
+                                                GridGeom g, T* __restrict__ grid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double qi = q[i];
+    if (qi == 0.0) return;
+    int i0[3];
+    T w[3][8];
+    stencil_of(x[i], y[i], z[i], g, i0, w);
+    const int P = g.order, off = stencil_offset(P);
+    for (int a = 0; a < P; ++a) {
+        const int ixa = wrap(i0[0] + a - off, g.nx);
+        for (int b = 0; b < P; ++b) {
+            const int iyb = wrap(i0[1] + b - off, g.ny);
+            const T wxy = (T)qi * w[0][a] * w[1][b];
+            for (int c = 0; c < P; ++c) {
+                const int izc = wrap(i0[2] + c - off, g.nz);
+                unsafeAtomicAdd(grid + ((size_t)izc * g.ny + iyb) * g.nx + ixa, wxy * w[2][c]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ FFT kernels
+template <typename T>
+__global__ void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ kg, int nx, int ny, FftPlan px,
+                             FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* stwy = stwx + nx;
+    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, hs = ny + 1;
+    cplx<T>* A = stwy + ny;
+    cplx<T>* H = A + nyp * nx;
+    const int tid = threadIdx.x, bd = blockDim.x, zp = blockIdx.x;
+    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
+    for (int k = tid; k < ny; k += bd) stwy[k] = twy[k];
+    const T* plane = grid + (size_t)zp * nx * ny;
+    for (int idx = tid; idx < nyp * nx; idx += bd) {
+        const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+        const T re = plane[y0 * nx + xx];
+        const T im = (y0 + 1 < ny) ? plane[(y0 + 1) * nx + xx] : T(0);
+        A[idx] = {re, im};
+    }
+    __syncthreads();
+    lds_fft(A, nyp, nx, px, stwx, -1);
+    for (int idx = tid; idx < nyp * nkx; idx += bd) {
+        const int yp = idx / nkx, k = idx - yp * nkx;
+        const cplx<T> Zk = A[yp * nx + k], Zn = A[yp * nx + (k == 0 ? 0 : nx - k)];
+        const cplx<T> a = {(Zk.x + Zn.x) * T(0.5), (Zk.y - Zn.y) * T(0.5)};
+        const cplx<T> d = {Zk.x - Zn.x, Zk.y + Zn.y};
+        H[k * hs + 2 * yp] = a;
+        if (2 * yp + 1 < ny) H[k * hs + 2 * yp + 1] = {d.y * T(0.5), -d.x * T(0.5)};
+    }
+    __syncthreads();
+    lds_fft(H, nkx, hs, py, stwy, -1);
+    cplx<T>* out = kg + (size_t)zp * ny * nkx;
+    for (int idx = tid; idx < ny * nkx; idx += bd) {
+        const int ky = idx / nkx, kx = idx - ky * nkx;
+        out[idx] = H[kx * hs + ky];
+    }
+}
+
+// mode bit 1: energy sum corner*eta*|Q|^2 into partials[blockIdx.x];
+// mode bit 2: multiply by eta, inverse z transform, store back.
+template <typename T>
+__global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPlan pz,
+                        const cplx<T>* __restrict__ twz, const T* __restrict__ eta, int tx, int mode,
+                        double* __restrict__ partials) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double sred[16];
+    cplx<T>* stw = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* C = stw + nz;
+    const int nkx = nx / 2 + 1, hs = nz + 1;
+    const int ntile = (nkx + tx - 1) / tx;
+    const int ky = blockIdx.x / ntile, kx0 = (blockIdx.x - ky * ntile) * tx;
+    const int cnt = min(tx, nkx - kx0);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int k = tid; k < nz; k += bd) stw[k] = twz[k];
+    for (int idx = tid; idx < nz * cnt; idx += bd) {
+        const int kz = idx / cnt, c = idx - kz * cnt;
+        C[c * hs + kz] = kg[((size_t)kz * ny + ky) * nkx + kx0 + c];
+    }
+    __syncthreads();
+    lds_fft(C, cnt, hs, pz, stw, -1);
+    double e = 0.0;
+    for (int idx = tid; idx < cnt * nz; idx += bd) {
+        const int c = idx / nz, kz = idx - c * nz, kx = kx0 + c;
+        const cplx<T> v = C[c * hs + kz];
+        const T et = eta[((size_t)kz * ny + ky) * nkx + kx];
+        if (mode & 1) {
+            const double corner = (kx == 0 || kx == nx / 2) ? 0.5 : 1.0;  // MC/src/pme.cu:234
+            e += corner * (double)et * ((double)v.x * v.x + (double)v.y * v.y);
+        }
+        if (mode & 2) C[c * hs + kz] = {v.x * et, v.y * et};
+    }
+    if (mode & 1) {
+        e = block_sum(e, sred);
+        if (tid == 0) partials[blockIdx.x] = e;
+    }
+    if (mode & 2) {
+        __syncthreads();
+        lds_fft(C, cnt, hs, pz, stw, +1);
+        for (int idx = tid; idx < nz * cnt; idx += bd) {
+            const int kz = idx / cnt, c = idx - kz * cnt;
+            kg[((size_t)kz * ny + ky) * nkx + kx0 + c] = C[c * hs + kz];
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_fft_xy_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny, FftPlan px,
+                             FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
+                             int accumulate) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* stwy = stwx + nx;
+    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, hs = ny + 1;
+    cplx<T>* A = stwy + ny;
+    cplx<T>* H = A + nyp * nx;
+    const int tid = threadIdx.x, bd = blockDim.x, zp = blockIdx.x;
+    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
+    for (int k = tid; k < ny; k += bd) stwy[k] = twy[k];
+    const cplx<T>* in = kg + (size_t)zp * ny * nkx;
+    for (int idx = tid; idx < ny * nkx; idx += bd) {
+        const int ky = idx / nkx, kx = idx - ky * nkx;
+        H[kx * hs + ky] = in[idx];
+    }
+    __syncthreads();
+    lds_fft(H, nkx, hs, py, stwy, +1);
+    for (int idx = tid; idx < nyp * nx; idx += bd) {
+        const int yp = idx / nx, k = idx - yp * nx, y0 = 2 * yp;
+        const bool hb = y0 + 1 < ny;
+        cplx<T> a, b;
+        if (k < nkx) {
+            a = H[k * hs + y0];
+            b = hb ? H[k * hs + y0 + 1] : cplx<T>{T(0), T(0)};
+            if (k == 0 || 2 * k == nx) { a.y = T(0); b.y = T(0); }
+        } else {
+            const int kk = nx - k;
+            a = H[kk * hs + y0]; a.y = -a.y;
+            b = hb ? H[kk * hs + y0 + 1] : cplx<T>{T(0), T(0)};
+            b.y = -b.y;
+        }
+        A[idx] = {a.x - b.y, a.y + b.x};
+    }
+    __syncthreads();
+    lds_fft(A, nyp, nx, px, stwx, +1);
+    T* plane = out + (size_t)zp * nx * ny;
+    for (int idx = tid; idx < nyp * nx; idx += bd) {
+        const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+        const cplx<T> v = A[idx];
+        if (accumulate) {
+            plane[y0 * nx + xx] += v.x;
+            if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] += v.y;
+        } else {
+            plane[y0 * nx + xx] = v.x;
+            if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] = v.y;
+        }
+    }
+}
+
+// ------------------------------------------------------------ host side
+namespace {
+
+int good_fft_dim(int n) {  // MC/src/pme.cu:46-51
+    const int NMAX = std::max(16, n * 2 + 32);
+    for (int m = std::max(8, n); m <= NMAX; ++m) if (fft_len_ok(m)) return m;
+    return n;
+}
+
+double alpha_from_rtol(double rc, double rtol) {  // MC/src/pme.cu:54-63
+    double beta = 5.0;
+    int i = 0;
+    while (std::erfc(beta * rc) > rtol) { beta *= 2.0; if (++i > 60) break; }
+    double low = 0.0, high = beta;
+    for (int it = 0; it < 60; ++it) {
+        beta = 0.5 * (low + high);
+        if (std::erfc(beta * rc) > rtol) low = beta; else high = beta;
+    }
+    return beta;
+}
+
+int fft_threads(long long elems) {
+    long long t = (elems + 14) / 15;
+    int b = 256;
+    while (b < t && b < 1024) b *= 2;
+    require(t <= 1024, "FFT plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
+    return b;
+}
+
+}  // namespace
+
+// MC/src/pme.cu:77-113 with the near-zero fix described in the header.
+std::vector<double> spme_moduli(int n, int pme_order) {
+    const int splineOrder = pme_order - 1;
+    require(splineOrder >= 1, "pme_order must be >= 2");
+    std::vector<double> data(splineOrder, 0.0);
+    data[0] = 1.0;
+    for (int k = 2; k <= splineOrder; ++k) {
+        const double invk = 1.0 / k;
+        for (int m = k - 1; m > 0; --m) data[m] = invk * ((k - m) * data[m - 1] + (m + 1) * data[m]);
+        data[0] = invk * data[0];
+    }
+    std::vector<double> mod(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double sc = 0.0, ss = 0.0;
+        for (int j = 0; j < splineOrder; ++j) {
+            const double arg = (2.0 * M_PI * i * (j + 1)) / double(n);
+            sc += data[j] * std::cos(arg);
+            ss += data[j] * std::sin(arg);
+        }
+        double v = sc * sc + ss * ss;
+        if (!(v > 0.0)) v = 1e-30;
+        mod[i] = v;
+    }
+    if (n % 2 == 0) { const int h = n / 2; if (!(mod[h] > 0.0)) mod[h] = mod[h - 1]; }
+    std::vector<double> orig = mod;
+    for (int i = 0; i < n; ++i)
+        if (orig[i] < 1e-7) mod[i] = 0.5 * (orig[(i + n - 1) % n] + orig[(i + 1) % n]);
+    return mod;
+}
+
+}  // namespace phb
+
+using namespace phb;
+
+phb_pme_s::~phb_pme_s() = default;
+
+namespace phb {
+
+void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
+    hipStream_t st = stream();
+    const size_t G = (size_t)P->nx * P->ny * P->nz;
+    const int nb = (int)std::min<size_t>((G + 255) / 256, 4096);
+    if (P->precision == PHB_FP64) {
+        hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, st, P->grid64.p, G, 0.0);
+        if (a->n > 0)
+            hipLaunchKernelGGL(k_spread<double>, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
+                               a->q.p, a->n, P->geom, P->grid64.p);
+    } else {
+        hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, st, P->grid32.p, G, 0.0f);
+        if (a->n > 0)
+            hipLaunchKernelGGL(k_spread<float>, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
+                               a->q.p, a->n, P->geom, P->grid32.p);
+    }
+    PHB_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void fft_forward_impl(phb_pme_s* P, const T* grid, cplx<T>* kg, const cplx<T>* twx, const cplx<T>* twy,
+                             const cplx<T>* twz, const T* eta, int zmode, double* partials) {
+    hipStream_t st = stream();
+    const int nx = P->nx, ny = P->ny, nz = P->nz;
+    hipLaunchKernelGGL(k_fft_xy_r2c<T>, dim3(nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), st, grid, kg,
+                       nx, ny, P->px, P->py, twx, twy);
+    hipLaunchKernelGGL(k_fft_z<T>, dim3(P->z_blocks), dim3(P->z_threads), P->z_lds(sizeof(cplx<T>)), st, kg, nx, ny,
+                       nz, P->pz, twz, eta, P->z_tx, zmode, partials);
+    PHB_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void fft_inverse_impl(phb_pme_s* P, const cplx<T>* kg, T* out, const cplx<T>* twx, const cplx<T>* twy,
+                             int accumulate) {
+    hipLaunchKernelGGL(k_fft_xy_c2r<T>, dim3(P->nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), stream(),
+                       kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate);
+    PHB_HIP(hipGetLastError());
+}
+
+// grid -> kgrid (-> eta, energy partials) [-> phi]
+void PmeKernels::launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate) {
+    if (P->precision == PHB_FP64) {
+        fft_forward_impl<double>(P, P->grid64.p, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
+                                 P->partials.p);
+        if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, P->phi64.p, P->twx64.p, P->twy64.p, accumulate);
+    } else {
+        fft_forward_impl<float>(P, P->grid32.p, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
+                                P->partials.p);
+        if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, P->phi32.p, P->twx32.p, P->twy32.p, accumulate);
+    }
+}
+
+void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
+                             int mode, double alpha, double kf, int precision, double* partials, int nblocks) {
+    PairArgs A;
+    A.x = a->x.p; A.y = a->y.p; A.z = a->z.p; A.q = a->q.p; A.type = a->type.p;
+    A.head = nl->head.p; A.list = nl->list.p;
+    A.N = a->n; A.T = p->d.ntypes;
+    A.Lx = box->Lx; A.Ly = box->Ly; A.Lz = box->Lz;
+    A.eps = p->d_eps.p; A.sig = p->d_sig.p; A.shift = p->d_shift.p;
+    A.use_shift = p->d.lj_shift ? 1 : 0;
+    A.use_cut = p->lj_cut_active ? 1 : 0;
+    A.lj_rc2 = p->lj_rc * p->lj_rc;
+    A.kf = kf;
+    A.alpha = alpha;
+    hipStream_t st = stream();
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), 0, st, A, partials); };
+    if (precision == PHB_FP64) {
+        switch (mode) {
+            case PAIR_LJ: go(k_pair_energy<double, PAIR_LJ>); break;
+            case PAIR_COUL_CUT: go(k_pair_energy<double, PAIR_COUL_CUT>); break;
+            case PAIR_EWALD: go(k_pair_energy<double, PAIR_EWALD>); break;
+            case PAIR_LJ | PAIR_COUL_CUT: go(k_pair_energy<double, PAIR_LJ | PAIR_COUL_CUT>); break;
+            default: go(k_pair_energy<double, PAIR_LJ | PAIR_EWALD>); break;
+        }
+    } else {
+        switch (mode) {
+            case PAIR_LJ: go(k_pair_energy<float, PAIR_LJ>); break;
+            case PAIR_COUL_CUT: go(k_pair_energy<float, PAIR_COUL_CUT>); break;
+            case PAIR_EWALD: go(k_pair_energy<float, PAIR_EWALD>); break;
+            case PAIR_LJ | PAIR_COUL_CUT: go(k_pair_energy<float, PAIR_LJ | PAIR_COUL_CUT>); break;
+            default: go(k_pair_energy<float, PAIR_LJ | PAIR_EWALD>); break;
+        }
+    }
+    PHB_HIP(hipGetLastError());
+}
+
+int pair_blocks(int N) { return std::max(1, std::min(ceil_div(N, 256), 2048)); }
+
+void PmeKernels::launch_moments(const phb_atoms_s* a, double* partials, int nblocks) {
+    hipLaunchKernelGGL(k_charge_moments, dim3(nblocks), dim3(256), 0, stream(), a->q.p, a->z.p, a->n, partials);
+    PHB_HIP(hipGetLastError());
+}
+
+// Self, net-charge and slab terms from the charge moments (MC/src/pme.cu:425-447).
+void corrections(const phb_pme_s* P, const phb_box* box, const double mom[4], double* e_self, double* e_qcorr,
+                 double* e_slab) {
+    const double elF = P->elF, alpha = P->alpha, V = P->vol;
+    const double Q = mom[0], Q2 = mom[1];
+    *e_self = -elF * alpha / std::sqrt(M_PI) * Q2;
+    *e_qcorr = -(M_PI * elF) * (Q * Q) / (2.0 * V * alpha * alpha);
+    double E_slab = 0.0;
+    if (P->use3dc) {
+        const double dc = (2.0 * M_PI / V) * elF;
+        E_slab = dc * mom[2] * mom[2];
+        if (std::abs(Q) > 1e-4) E_slab -= dc * Q * (mom[3] + Q * (box->Lz * box->Lz) / 12.0);
+    }
+    *e_slab = E_slab;
+}
+
+}  // namespace phb
+
+// ----------------------------------------------------------------------- ABI
+extern "C" int phb_pme_create(phb_params p, const phb_box* box, double rc, int precision, phb_pme* out) {
+    return guard([&] {
+        require(p && box && out, "pmeCreate: null plan");
+        require(precision == PHB_FP64 || precision == PHB_FP32, "pmeCreate: precision must be PHB_FP64 or PHB_FP32");
+        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "pmeCreate: box must be positive");
+        const phb_params_desc& d = p->d;
+        require(d.pme_order >= 3 && d.pme_order <= 8, "pmeCreate: pme_order must be in [3, 8]", PHB_ERR_UNSUPPORTED);
+        require_device();
+        auto P = std::make_unique<phb_pme_s>();
+        P->precision = precision;
+        P->order = d.pme_order;
+        P->use3dc = d.pme_3dc != 0;
+        // MC/src/pme.cu:299-306
+        P->alpha = d.ewald_alpha > 0 ? d.ewald_alpha : alpha_from_rtol(rc, d.ewald_rtol > 0 ? d.ewald_rtol : 1e-5);
+        P->ewF = (M_PI / P->alpha) * (M_PI / P->alpha);
+        P->elF = kKe / d.epsilon_r;
+        P->Lx = box->Lx; P->Ly = box->Ly; P->Lz = box->Lz;
+        P->vol = box->Lx * box->Ly * box->Lz;
+        // MC/src/pme.cu:312-318, EXT explicit dims
+        const double sp = d.pme_spacing > 0 ? d.pme_spacing : 0.12;
+        int nx = d.nx > 0 ? d.nx : good_fft_dim((int)std::ceil(box->Lx / sp));
+        int ny = d.ny > 0 ? d.ny : good_fft_dim((int)std::ceil(box->Ly / sp));
+        int nz = d.nz > 0 ? d.nz : good_fft_dim((int)std::ceil(box->Lz / sp));
+        nx = std::max(nx, P->order); ny = std::max(ny, P->order); nz = std::max(nz, P->order);
+        P->nx = nx; P->ny = ny; P->nz = nz;
+        P->px = make_fft_plan(nx); P->py = make_fft_plan(ny); P->pz = make_fft_plan(nz);
+        P->geom = GridGeom{nx, ny, nz, P->order, box->Lx, box->Ly, box->Lz};
+        const int nkx = nx / 2 + 1;
+        const size_t G = (size_t)nx * ny * nz, Gk = (size_t)nkx * ny * nz;
+        // launch shapes
+        P->plane_threads = fft_threads(std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny));
+        const size_t csz = precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+        require(P->plane_lds(csz) <= 160 * 1024, "pmeCreate: xy plane too large for LDS", PHB_ERR_UNSUPPORTED);
+        const int ntile0 = (nkx + 7) / 8;
+        P->z_tx = (nkx + ntile0 - 1) / ntile0;
+        P->z_blocks = ny * ((nkx + P->z_tx - 1) / P->z_tx);
+        P->z_threads = fft_threads((long long)P->z_tx * nz);
+        require(P->z_lds(csz) <= 160 * 1024, "pmeCreate: z pencil too large for LDS", PHB_ERR_UNSUPPORTED);
+        // influence function (MC/src/pme.cu:196-223), half spectrum, no corner weight
+        std::vector<double> mX = spme_moduli(nx, P->order), mY = spme_moduli(ny, P->order), mZ = spme_moduli(nz, P->order);
+        std::vector<double> eta(Gk, 0.0);
+        const double Hxx = 1.0 / box->Lx, Hyy = 1.0 / box->Ly, Hzz = 1.0 / box->Lz;
+        for (int kz = 0; kz < nz; ++kz)
+            for (int ky = 0; ky < ny; ++ky)
+                for (int kx = 0; kx < nkx; ++kx) {
+                    size_t o = ((size_t)kz * ny + ky) * nkx + kx;
+                    if (kx == 0 && ky == 0 && kz == 0) continue;
+                    int mx = kx, my = (ky <= ny / 2) ? ky : ky - ny, mz = (kz <= nz / 2) ? kz : kz - nz;
+                    double kxv = mx * Hxx, kyv = my * Hyy, kzv = mz * Hzz;
+                    double m2k = kxv * kxv + kyv * kyv + kzv * kzv;
+                    if (m2k == 0.0) continue;
+                    double denom = m2k * M_PI * P->vol * mX[kx] * mY[ky] * mZ[kz];
+                    if (!(denom > 0.0)) continue;
+                    eta[o] = P->elF * std::exp(-P->ewF * m2k) / denom;
+                }
+        P->eta_host = eta;
+        auto tx = twiddles(nx), ty = twiddles(ny), tz = twiddles(nz);
+        if (precision == PHB_FP64) {
+            P->grid64.alloc(G); P->phi64.alloc(G); P->kg64.alloc(Gk); P->eta64.alloc(Gk);
+            P->eta64.upload(eta.data(), Gk);
+            P->twx64.alloc(nx); P->twy64.alloc(ny); P->twz64.alloc(nz);
+            P->twx64.upload(tx.data(), nx); P->twy64.upload(ty.data(), ny); P->twz64.upload(tz.data(), nz);
+        } else {
+            std::vector<float> e32(eta.begin(), eta.end());
+            auto cv = [](const std::vector<cplx<double>>& v) {
+                std::vector<cplx<float>> o(v.size());
+                for (size_t k = 0; k < v.size(); ++k) o[k] = {(float)v[k].x, (float)v[k].y};
+                return o;
+            };
+            auto fx = cv(tx), fy = cv(ty), fz = cv(tz);
+            P->grid32.alloc(G); P->phi32.alloc(G); P->kg32.alloc(Gk); P->eta32.alloc(Gk);
+            P->eta32.upload(e32.data(), Gk);
+            P->twx32.alloc(nx); P->twy32.alloc(ny); P->twz32.alloc(nz);
+            P->twx32.upload(fx.data(), nx); P->twy32.upload(fy.data(), ny); P->twz32.upload(fz.data(), nz);
+        }
+        P->partials.alloc(std::max<size_t>(4 * 2048, (size_t)P->z_blocks));
+        P->result.alloc(16);
+        sync();
+        *out = P.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_pme_destroy(phb_pme pme) {
+    return guard([&] { delete pme; return PHB_OK; });
+}
+
+extern "C" int phb_pme_info(phb_pme P, int* nx, int* ny, int* nz, int* order, double* alpha) {
+    return guard([&] {
+        require(P != nullptr, "pme_info: null plan");
+        if (nx) *nx = P->nx;
+        if (ny) *ny = P->ny;
+        if (nz) *nz = P->nz;
+        if (order) *order = P->order;
+        if (alpha) *alpha = P->alpha;
+        return PHB_OK;
+    });
+}
+
+namespace {
+
+// E_real (slot 0 of the returned pair) and the reciprocal + correction terms.
+void pme_energy_core(phb_pme_s* P, const phb_atoms_s* a, const phb_csr_s* nl, const phb_params_s* par,
+                     const phb_box* box, int pair_mode, double out[8]) {
+    // out: [pair_lj, pair_coul, recip, Q, Q2, Mz, Qz2, -]
+    hipStream_t st = stream();
+    const int nbp = pair_blocks(a->n);
+    double* part = P->partials.p;
+    double* res = P->result.p;
+    if (pair_mode) {
+        PmeKernels::launch_pair(a, par, nl, box, pair_mode, P->alpha, P->elF, P->precision, part, nbp);
+        reduce_partials(part, nbp, 2, 2, res + 0);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, res, (size_t)2, 0.0);
+    }
+    PmeKernels::launch_spread(P, a);
+    // pair partials are reduced already (stream order); reuse the buffer
+    PmeKernels::launch_fft(P, 1, false, false);
+    reduce_partials(part, P->z_blocks, 1, 1, res + 2);
+    PmeKernels::launch_moments(a, part, nbp);
+    reduce_partials(part, nbp, 4, 4, res + 3);
+    PHB_HIP(hipMemcpyAsync(out, res, 7 * sizeof(double), hipMemcpyDeviceToHost, st));
+    PHB_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" int phb_energy_lj(phb_atoms a, phb_params p, phb_csr nl, const phb_box* box, double* E) {
+    return guard([&] {
+        require(a && p && nl && box && E, "compute_lj_sr_energy_gpu: null argument");
+        require(nl->rows == a->n, "compute_lj_sr_energy_gpu: neighbour list rows != natoms");
+        if (a->n == 0 || nl->nnz == 0) { *E = 0.0; return PHB_OK; }  // MC/src/energy_gpu.cu:54
+        phb::DevBuf<double> part(2 * 2048), res(2);
+        const int nb = pair_blocks(a->n);
+        PmeKernels::launch_pair(a, p, nl, box, PAIR_LJ, 0.0, 0.0, PHB_FP64, part.p, nb);
+        reduce_partials(part.p, nb, 2, 2, res.p);
+        double h[2];
+        res.download(h, 2);
+        sync();
+        *E = h[0];
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_energy_coul_cut(phb_atoms a, phb_params p, phb_csr nl, const phb_box* box, double* E) {
+    return guard([&] {
+        require(a && p && nl && box && E, "compute_coul_sr_energy_gpu: null argument");
+        require(nl->rows == a->n, "compute_coul_sr_energy_gpu: neighbour list rows != natoms");
+        if (a->n == 0 || nl->nnz == 0) { *E = 0.0; return PHB_OK; }  // MC/src/energy_gpu.cu:116
+        phb::DevBuf<double> part(2 * 2048), res(2);
+        const int nb = pair_blocks(a->n);
+        PmeKernels::launch_pair(a, p, nl, box, PAIR_COUL_CUT, 0.0, kKe / p->d.epsilon_r, PHB_FP64, part.p, nb);
+        reduce_partials(part.p, nb, 2, 2, res.p);
+        double h[2];
+        res.download(h, 2);
+        sync();
+        *E = h[1];
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_pme_energy(phb_pme P, phb_atoms a, phb_csr nl, const phb_box* box, phb_pme_components* comps,
+                              double* E) {
+    return guard([&] {
+        require(P && a && nl && box && E, "pmeEnergy: null argument");
+        require(nl->rows == a->n, "pmeEnergy: neighbour list rows != natoms");
+        if (a->n == 0) {  // MC/src/pme.cu:350-359
+            if (comps) *comps = phb_pme_components{0, 0, 0, 0, 0};
+            *E = 0.0;
+            return PHB_OK;
+        }
+        // real-space only needs q, positions and the list: use a dummy LJ table
+        phb_params_s dummy;
+        dummy.d.ntypes = 1; dummy.d.lj_shift = 0;
+        double o[8];
+        pme_energy_core(P, a, nl, &dummy, box, PAIR_EWALD, o);
+        double es, eq, esl;
+        corrections(P, box, o + 3, &es, &eq, &esl);
+        if (comps) *comps = phb_pme_components{o[1], o[2], es, eq + esl, esl};
+        *E = o[1] + o[2] + es + eq + esl;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_total_energy(phb_atoms a, phb_params p, phb_csr nl, phb_pme P, const phb_box* box, double* comps,
+                                double* E) {
+    return guard([&] {
+        require(a && p && nl && box && E, "total_energy: null argument");
+        require(nl->rows == a->n, "total_energy: neighbour list rows != natoms");
+        double c[6] = {0, 0, 0, 0, 0, 0};
+        if (p->d.coulombtype == PHB_COUL_PME) {
+            require(P != nullptr, "total_energy: PME plan is null");  // MC/src/mc.cu:54
+            if (a->n > 0) {
+                double o[8];
+                pme_energy_core(P, a, nl, p, box, nl->nnz > 0 ? (PAIR_LJ | PAIR_EWALD) : 0, o);
+                double es, eq, esl;
+                corrections(P, box, o + 3, &es, &eq, &esl);
+                c[0] = o[0]; c[1] = o[1]; c[2] = o[2]; c[3] = es; c[4] = eq; c[5] = esl;
+            }
+        } else {
+            if (a->n > 0 && nl->nnz > 0) {
+                phb::DevBuf<double> part(2 * 2048), res(2);
+                const int nb = pair_blocks(a->n);
+                PmeKernels::launch_pair(a, p, nl, box, PAIR_LJ | PAIR_COUL_CUT, 0.0, kKe / p->d.epsilon_r, PHB_FP64, part.p, nb);
+                reduce_partials(part.p, nb, 2, 2, res.p);
+                double h[2];
+                res.download(h, 2);
+                sync();
+                c[0] = h[0]; c[1] = h[1];
+            }
+        }
+        if (comps) for (int k = 0; k < 6; ++k) comps[k] = c[k];
+        *E = c[0] + (c[1] + c[2] + c[3] + c[4] + c[5]);
+        return PHB_OK;
+    });
+}
+
+// ------------------------------------------------------------ stage access
+extern "C" int phb_pme_spread(phb_pme P, phb_atoms a, double* grid) {
+    return guard([&] {
+        require(P && a && grid, "pme_spread: null argument");
+        PmeKernels::launch_spread(P, a);
+        const size_t G = (size_t)P->nx * P->ny * P->nz;
+        if (P->precision == PHB_FP64) P->grid64.download(grid, G);
+        else {
+            std::vector<float> g(G);
+            P->grid32.download(g.data(), G);
+            sync();
+            std::copy(g.begin(), g.end(), grid);
+        }
+        sync();
+        return PHB_OK;
+    });
+}
+
+namespace {
+void upload_grid(phb_pme_s* P, const double* grid) {
+    const size_t G = (size_t)P->nx * P->ny * P->nz;
+    if (P->precision == PHB_FP64) P->grid64.upload(grid, G);
+    else {
+        std::vector<float> g(grid, grid + G);
+        P->grid32.upload(g.data(), G);
+        sync();
+    }
+}
+}  // namespace
+
+extern "C" int phb_pme_fft_r2c(phb_pme P, const double* grid, double* kgrid) {
+    return guard([&] {
+        require(P && grid && kgrid, "pme_fft_r2c: null argument");
+        upload_grid(P, grid);
+        const size_t Gk = (size_t)(P->nx / 2 + 1) * P->ny * P->nz;
+        // z pass with mode 0: forward transform only
+        PmeKernels::launch_fft(P, 0, false, false);
+        if (P->precision == PHB_FP64) P->kg64.download(reinterpret_cast<cplx<double>*>(kgrid), Gk);
+        else {
+            std::vector<cplx<float>> k(Gk);
+            P->kg32.download(k.data(), Gk);
+            sync();
+            for (size_t i = 0; i < Gk; ++i) { kgrid[2 * i] = k[i].x; kgrid[2 * i + 1] = k[i].y; }
+        }
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_pme_influence(phb_pme P, double* eta) {
+    return guard([&] {
+        require(P && eta, "pme_influence: null argument");
+        std::copy(P->eta_host.begin(), P->eta_host.end(), eta);
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_pme_potential(phb_pme P, const double* grid, double* phi, double* E) {
+    return guard([&] {
+        require(P && grid, "pme_potential: null argument");
+        upload_grid(P, grid);
+        PmeKernels::launch_fft(P, 3, true, false);
+        reduce_partials(P->partials.p, P->z_blocks, 1, 1, P->result.p);
+        const size_t G = (size_t)P->nx * P->ny * P->nz;
+        if (phi) {
+            if (P->precision == PHB_FP64) P->phi64.download(phi, G);
+            else {
+                std::vector<float> f(G);
+                P->phi32.download(f.data(), G);
+                sync();
+                std::copy(f.begin(), f.end(), phi);
+            }
+        }
+        double e = 0.0;
+        P->result.download(&e, 1);
+        sync();
+        if (E) *E = e;
+        return PHB_OK;
+    });
+}

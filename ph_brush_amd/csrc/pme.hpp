@@ -1,0 +1,75 @@
+// pme.hpp -- SPME plan (the handle behind phb_pme) and the launchers the MC
+// module reuses.
+#pragma once
+
+#include <vector>
+
+#include "device.hpp"
+#include "handles.hpp"
+
+namespace phb {
+
+struct GridGeom {
+    int nx, ny, nz, order;
+    double Lx, Ly, Lz;
+};
+
+__host__ __device__ __forceinline__ int wrap(int i, int n) { return ((i % n) + n) % n; }
+
+// Base grid index and B-spline weights of one atom along each axis
+// (MC/src/pme.cu:151-156 generalised to order p): g = (x/L)*n, i0 = floor(g),
+// t = g - i0, computed in FP64; weights in T.
+template <typename T>
+__host__ __device__ __forceinline__ void stencil_of(double x, double y, double z, const GridGeom& g, int i0[3],
+                                                    T w[3][8]) {
+    const double gx = (x / g.Lx) * g.nx, gy = (y / g.Ly) * g.ny, gz = (z / g.Lz) * g.nz;
+    i0[0] = (int)floor(gx); i0[1] = (int)floor(gy); i0[2] = (int)floor(gz);
+    bspline_weights<T>(g.order, (T)(gx - i0[0]), w[0]);
+    bspline_weights<T>(g.order, (T)(gy - i0[1]), w[1]);
+    bspline_weights<T>(g.order, (T)(gz - i0[2]), w[2]);
+}
+
+std::vector<double> spme_moduli(int n, int pme_order);
+int pair_blocks(int N);
+
+}  // namespace phb
+
+struct phb_pme_s {
+    int precision = PHB_FP64;
+    int order = 4;
+    bool use3dc = false;
+    double alpha = 0, ewF = 0, elF = 0, vol = 0, Lx = 0, Ly = 0, Lz = 0;
+    int nx = 0, ny = 0, nz = 0;
+    phb::FftPlan px, py, pz;
+    phb::GridGeom geom{};
+    int plane_threads = 256, z_tx = 8, z_blocks = 0, z_threads = 256;
+    std::vector<double> eta_host;
+    phb::DevBuf<double> grid64, phi64, eta64;
+    phb::DevBuf<phb::cplx<double>> kg64, twx64, twy64, twz64;
+    phb::DevBuf<float> grid32, phi32, eta32;
+    phb::DevBuf<phb::cplx<float>> kg32, twx32, twy32, twz32;
+    phb::DevBuf<double> partials, result;
+    size_t plane_lds(size_t csz) const {
+        const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1;
+        return ((size_t)nx + ny + (size_t)nyp * nx + (size_t)nkx * (ny + 1)) * csz;
+    }
+    size_t z_lds(size_t csz) const { return ((size_t)nz + (size_t)z_tx * (nz + 1)) * csz; }
+    ~phb_pme_s();
+};
+
+namespace phb {
+
+struct PmeKernels {
+    static void launch_spread(phb_pme_s* P, const phb_atoms_s* a);
+    // forward r2c + z pass (zmode bits: 1 energy, 2 convolve + inverse z),
+    // optionally the inverse plane pass into phi (store or accumulate)
+    static void launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate);
+    static void launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
+                            int mode, double alpha, double kf, int precision, double* partials, int nblocks);
+    static void launch_moments(const phb_atoms_s* a, double* partials, int nblocks);
+};
+
+void corrections(const phb_pme_s* P, const phb_box* box, const double mom[4], double* e_self, double* e_qcorr,
+                 double* e_slab);
+
+}  // namespace phb

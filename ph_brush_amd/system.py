@@ -1,0 +1,310 @@
+"""Host-side mirror of the reference's setup path and device objects.
+
+Names follow the reference's host API (MC/include/*.hpp): DeviceAtoms,
+DeviceCSR, DeviceParams, build_half_neighbors_gpu, compute_lj_sr_energy_gpu,
+compute_coul_sr_energy_gpu, PMEPlan (pmeCreate / pmeEnergy / pmeDestroy).
+Each wraps one handle of the C ABI and frees it when garbage-collected.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+KE = 138.935458  # kJ mol^-1 nm e^-2 (MC/src/pme.cu:305)
+
+# type ids of the synthetic brushes (1-based, like the reference's Params;
+# NH_N/NH_P/W/Cl are the reference defaults, MC/include/types.hpp:51-54)
+TYPE_PERM, TYPE_CATION, TYPE_NH_N, TYPE_NH_P, TYPE_W, TYPE_CL = 1, 2, 4, 5, 7, 8
+NTYPES = 8
+
+
+@dataclass
+class System:
+    """Host system: wrapped positions, charges, 0-based types, bonds, box."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    q: np.ndarray
+    type0: np.ndarray
+    bonds: np.ndarray          # (nb, 2) int32, 0-based atom indices
+    Lx: float
+    Ly: float
+    Lz: float                  # simulation box (extended under the slab correction)
+    Lz_data: float = 0.0       # data-file box height
+    mol: np.ndarray | None = None
+
+    @property
+    def natoms(self) -> int:
+        return int(self.x.shape[0])
+
+    def copy(self) -> "System":
+        return System(self.x.copy(), self.y.copy(), self.z.copy(), self.q.copy(), self.type0.copy(),
+                      self.bonds.copy(), self.Lx, self.Ly, self.Lz, self.Lz_data,
+                      None if self.mol is None else self.mol.copy())
+
+
+def bond_adjacency(natoms: int, bonds: np.ndarray):
+    """Symmetric 1-2 bond CSR, rows sorted and unique (MC/src/io.cpp:148-169)."""
+    rows = [[] for _ in range(natoms)]
+    for i, j in np.asarray(bonds, dtype=np.int64).reshape(-1, 2):
+        rows[i].append(int(j))
+        rows[j].append(int(i))
+    head = np.zeros(natoms + 1, dtype=np.int32)
+    lst = []
+    for i in range(natoms):
+        r = sorted(set(rows[i]))
+        head[i + 1] = head[i] + len(r)
+        lst.extend(r)
+    return head, np.asarray(lst, dtype=np.int32)
+
+
+def wrap_positions(v: np.ndarray, L: float) -> np.ndarray:
+    """MC/src/io.cpp:303-316: w = r - floor(r/L)*L, folded into [0, L)."""
+    w = v - np.floor(v / L) * L
+    w = np.where(w < 0, w + L, w)
+    return np.where(w >= L, w - L, w)
+
+
+def gen_brush(chains_x, chains_y, chain_len, Lx, Ly, Lz, *, spacing=2.0, n_perm=0, p_prot=0.5, n_salt=0,
+              bond=0.97, min_dist=0.8, seed=1, graft_z=0.0, zfac=None) -> System:
+    """Synthetic brush from the C generator (phb_gen_brush)."""
+    spec = L.BrushSpec(chains_x, chains_y, spacing, graft_z, chain_len, n_perm, p_prot, n_salt, Lx, Ly, Lz, bond,
+                       min_dist, seed, TYPE_PERM, TYPE_CATION, TYPE_NH_N, TYPE_NH_P, TYPE_W, TYPE_CL)
+    n = C.c_int(0)
+    nb = C.c_int(0)
+    null_d = C.POINTER(C.c_double)()
+    null_i = C.POINTER(C.c_int)()
+    L.call("phb_gen_brush", C.byref(spec), 0, null_d, null_d, null_d, null_d, null_i, null_i, null_i, null_i, 0,
+           C.byref(n), C.byref(nb))
+    N, NB = n.value, nb.value
+    x, y, z, q = (np.zeros(N) for _ in range(4))
+    t1, mol = np.zeros(N, np.int32), np.zeros(N, np.int32)
+    bi, bj = np.zeros(max(NB, 1), np.int32), np.zeros(max(NB, 1), np.int32)
+    L.call("phb_gen_brush", C.byref(spec), N, L.dptr(x), L.dptr(y), L.dptr(z), L.dptr(q), L.iptr(t1), L.iptr(mol),
+           L.iptr(bi), L.iptr(bj), max(NB, 1), C.byref(n), C.byref(nb))
+    x, y, z = wrap_positions(x, Lx), wrap_positions(y, Ly), wrap_positions(z, Lz)
+    bonds = np.stack([bi[:NB], bj[:NB]], axis=1).astype(np.int32)
+    Lz_sim = Lz * (max(2.0, zfac) if zfac else 1.0)   # MC/src/main.cu:110-117
+    return System(x, y, z, q, (t1 - 1).astype(np.int32), bonds, Lx, Ly, Lz_sim, Lz, mol)
+
+
+@dataclass
+class Params:
+    """Physics parameters (params.in keys of MC/src/io.cpp:184-256 plus extensions)."""
+    rc: float = 3.5
+    lj_rc: float = 2.0 ** (1.0 / 6.0)
+    lj_shift: bool = True
+    epsilon_r: float = KE / 3.0        # Bjerrum length 3 sigma at beta = 1
+    coulombtype: int = L.PHB_COUL_PME
+    ewald_alpha: float = 0.9
+    ewald_rtol: float = 1e-5
+    pme_spacing: float = 0.12
+    pme_order: int = 5
+    grid: tuple = (32, 32, 128)
+    pme_3dc: bool = True
+    ntypes: int = NTYPES
+    lj_eps: np.ndarray = field(default_factory=lambda: np.ones(NTYPES * NTYPES))
+    lj_sig: np.ndarray = field(default_factory=lambda: np.ones(NTYPES * NTYPES))
+
+    def desc(self) -> L.ParamsDesc:
+        self._eps = np.ascontiguousarray(self.lj_eps, dtype=np.float64)
+        self._sig = np.ascontiguousarray(self.lj_sig, dtype=np.float64)
+        nx, ny, nz = self.grid if self.grid else (0, 0, 0)
+        return L.ParamsDesc(self.ntypes, L.dptr(self._eps), L.dptr(self._sig), int(self.lj_shift), self.rc,
+                            self.lj_rc, self.epsilon_r, self.coulombtype, self.ewald_alpha, self.ewald_rtol,
+                            self.pme_spacing, self.pme_order, nx, ny, nz, int(self.pme_3dc))
+
+
+@dataclass
+class McParams:
+    """Move parameters (MC/include/types.hpp:51-68 plus fixed-mu extension)."""
+    nh_n_type: int = TYPE_NH_N
+    nh_p_type: int = TYPE_NH_P
+    w_type: int = TYPE_W
+    cl_type: int = TYPE_CL
+    beta: float = 1.0
+    mu_eff: float = 0.0
+    w_z_extra: float = 1.0e6
+
+    @staticmethod
+    def at_ph(ph: float, pka: float = 7.5, beta: float = 1.0, **kw) -> "McParams":
+        """Fixed mu_eff = kT ln10 (pKa - pH)."""
+        return McParams(beta=beta, mu_eff=math.log(10.0) * (pka - ph) / beta, **kw)
+
+
+# ------------------------------------------------------------------ workloads
+def config_small_fp64(seed: int = 1):
+    """64 chains x 20 beads, 16x16x40 box (x3 slab), every bead titratable at
+    pH = pKa = 7.5 with p = 0.5, 40 salt pairs; SPME 32x32x128, order 5."""
+    s = gen_brush(8, 8, 20, 16.0, 16.0, 40.0, n_perm=0, p_prot=0.5, n_salt=40, seed=seed, zfac=3.0)
+    return s, Params(grid=(32, 32, 128)), McParams.at_ph(7.5)
+
+
+def config_30k_fp64(seed: int = 1):
+    """400 chains x 50 beads on a 20x20 lattice, 40x40x150 box (x3 slab); beads
+    1-25 permanently +1, 26-50 titratable (pKa 7.5, pH 7.0); 2,000 salt pairs;
+    SPME 64x64x256, order 5."""
+    s = gen_brush(20, 20, 50, 40.0, 40.0, 150.0, n_perm=25, p_prot=0.5, n_salt=2000, seed=seed, zfac=3.0)
+    return s, Params(grid=(64, 64, 256)), McParams.at_ph(7.0)
+
+
+def config_tiny(seed: int = 3, order: int = 4, grid=(16, 16, 48), chains=(3, 3), chain_len=6, n_perm=0,
+                n_salt=6, box=(6.0, 6.0, 16.0)):
+    """A few hundred atoms for quick parity checks."""
+    s = gen_brush(chains[0], chains[1], chain_len, box[0], box[1], box[2], n_perm=n_perm, p_prot=0.5,
+                  n_salt=n_salt, seed=seed, zfac=3.0)
+    return s, Params(grid=grid, pme_order=order), McParams.at_ph(7.5)
+
+
+# ------------------------------------------------------------ device objects
+class _Handle:
+    _destroy = ""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h:
+                getattr(L.load(), self._destroy)(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class DeviceAtoms(_Handle):
+    """Replaces DeviceAtoms + upload_atoms_to_device (MC/src/main.cu:18-41)."""
+    _destroy = "phb_atoms_destroy"
+
+    def __init__(self, s: System):
+        super().__init__()
+        self.n = s.natoms
+        self._keep = [np.ascontiguousarray(a) for a in (s.x, s.y, s.z, s.q)]
+        t = np.ascontiguousarray(s.type0, dtype=np.int32)
+        L.call("phb_atoms_create", self.n, *(L.dptr(a) for a in self._keep), L.iptr(t), C.byref(self.h))
+
+    def set_identity(self, idx, type0, q):
+        idx = np.ascontiguousarray(idx, np.int32)
+        type0 = np.ascontiguousarray(type0, np.int32)
+        q = np.ascontiguousarray(q, np.float64)
+        L.call("phb_atoms_set_identity", self.h, len(idx), L.iptr(idx), L.iptr(type0), L.dptr(q))
+
+    def identity(self):
+        q, t = np.zeros(self.n), np.zeros(self.n, np.int32)
+        L.call("phb_atoms_get_identity", self.h, L.dptr(q), L.iptr(t))
+        return q, t
+
+
+class DeviceCSR(_Handle):
+    """Replaces DeviceCSR / copy_csr_to_device (MC/src/neighbor.cu:65-72)."""
+    _destroy = "phb_csr_destroy"
+
+    @classmethod
+    def from_host(cls, head, lst):
+        c = cls()
+        head = np.ascontiguousarray(head, np.int32)
+        lst = np.ascontiguousarray(lst if len(lst) else np.zeros(1, np.int32), np.int32)
+        L.call("phb_csr_create", len(head) - 1, L.iptr(head), L.iptr(lst), C.byref(c.h))
+        return c
+
+    def shape(self):
+        r, nnz = C.c_int(0), C.c_longlong(0)
+        L.call("phb_csr_shape", self.h, C.byref(r), C.byref(nnz))
+        return r.value, nnz.value
+
+    def download(self):
+        rows, nnz = self.shape()
+        head = np.zeros(rows + 1, np.int32)
+        lst = np.zeros(max(nnz, 1), np.int32)
+        L.call("phb_csr_download", self.h, L.iptr(head), L.iptr(lst))
+        return head, lst[:nnz]
+
+
+def box_of(s: System) -> L.Box:
+    return L.Box(s.Lx, s.Ly, s.Lz)
+
+
+def build_half_neighbors_gpu(atoms: DeviceAtoms, box: L.Box, bonds12: DeviceCSR | None, rc: float) -> DeviceCSR:
+    """MC/include/neighbor.hpp:10-13."""
+    c = DeviceCSR()
+    L.call("phb_nlist_build_half", atoms.h, C.byref(box), bonds12.h if bonds12 else None, rc, C.byref(c.h))
+    return c
+
+
+class DeviceParams(_Handle):
+    """Replaces DeviceParams + upload_params_to_device (MC/src/main.cu:43-66)."""
+    _destroy = "phb_params_destroy"
+
+    def __init__(self, p: Params):
+        super().__init__()
+        self.p = p
+        d = p.desc()
+        L.call("phb_params_create", C.byref(d), C.byref(self.h))
+
+
+def compute_lj_sr_energy_gpu(atoms, params, neigh, box) -> float:
+    e = C.c_double(0)
+    L.call("phb_energy_lj", atoms.h, params.h, neigh.h, C.byref(box), C.byref(e))
+    return e.value
+
+
+def compute_coul_sr_energy_gpu(atoms, params, neigh, box) -> float:
+    e = C.c_double(0)
+    L.call("phb_energy_coul_cut", atoms.h, params.h, neigh.h, C.byref(box), C.byref(e))
+    return e.value
+
+
+class PMEPlan(_Handle):
+    """pmeCreate / pmeDestroy / pmeEnergy (MC/include/pme.hpp:42-49)."""
+    _destroy = "phb_pme_destroy"
+
+    def __init__(self, params: DeviceParams, box: L.Box, rc: float, precision: int = L.PHB_FP64):
+        super().__init__()
+        L.call("phb_pme_create", params.h, C.byref(box), rc, precision, C.byref(self.h))
+        nx, ny, nz, order = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        alpha = C.c_double()
+        L.call("phb_pme_info", self.h, C.byref(nx), C.byref(ny), C.byref(nz), C.byref(order), C.byref(alpha))
+        self.nx, self.ny, self.nz, self.order, self.alpha = nx.value, ny.value, nz.value, order.value, alpha.value
+
+    def energy(self, atoms, neigh, box):
+        comps = L.PmeComponents()
+        e = C.c_double(0)
+        L.call("phb_pme_energy", self.h, atoms.h, neigh.h, C.byref(box), C.byref(comps), C.byref(e))
+        return e.value, comps
+
+    def spread(self, atoms):
+        g = np.zeros(self.nx * self.ny * self.nz)
+        L.call("phb_pme_spread", self.h, atoms.h, L.dptr(g))
+        return g
+
+    def fft_r2c(self, grid):
+        grid = np.ascontiguousarray(grid, np.float64)
+        k = np.zeros(2 * (self.nx // 2 + 1) * self.ny * self.nz)
+        L.call("phb_pme_fft_r2c", self.h, L.dptr(grid), L.dptr(k))
+        return k.view(np.complex128)
+
+    def influence(self):
+        e = np.zeros((self.nx // 2 + 1) * self.ny * self.nz)
+        L.call("phb_pme_influence", self.h, L.dptr(e))
+        return e
+
+    def potential(self, grid):
+        grid = np.ascontiguousarray(grid, np.float64)
+        phi = np.zeros(self.nx * self.ny * self.nz)
+        e = C.c_double()
+        L.call("phb_pme_potential", self.h, L.dptr(grid), L.dptr(phi), C.byref(e))
+        return phi, e.value
+
+
+def total_energy(atoms, params, neigh, pme, box):
+    """MC/src/mc.cu:43-60; returns (E, [lj, real|cut, recip, self, qcorr, slab])."""
+    comps = np.zeros(6)
+    e = C.c_double(0)
+    L.call("phb_total_energy", atoms.h, params.h, neigh.h, pme.h if pme else None, C.byref(box), L.dptr(comps),
+           C.byref(e))
+    return e.value, comps
