@@ -135,6 +135,59 @@ int phb_pme_influence(phb_pme pme, double *eta);                            /* (
 /* phi = G*grid (unnormalised inverse of eta * FFT(grid)) and E = 0.5 grid.phi */
 int phb_pme_potential(phb_pme pme, const double *grid, double *phi, double *E);
 
+/* --------------------------------------------------------------------- MC */
+/* Protonation + ion-swap Monte Carlo kept entirely on the device.  One trial
+ * is the reference's GC move (MC/src/mc.cu:379-503): direction drawn with
+ * probability a*c/(a*c+b*d), a site and an ion drawn from the pools with
+ * uniform_int_distribution, Metropolis-Hastings with logH = ln S_before -
+ * ln S_after, pools updated by erase(find)+push_back.  The RNG is a device
+ * MT19937 with libstdc++'s Lemire and generate_canonical<double,53>, so the
+ * accept/reject sequence equals the reference's under the same seed (except
+ * for moves within rounding of the Metropolis threshold).  Delta-E is exact
+ * (FP64): neighbour rows for LJ and real space, the reciprocal term from the
+ * quadratic form 0.5 Q^T G Q through a maintained potential grid, and O(1)
+ * self / net-charge / slab updates.  mu_eff is held fixed (titration at a
+ * given pH: mu_eff = kT ln10 (pKa - pH)). */
+typedef struct {
+    int nh_n_type, nh_p_type, w_type, cl_type;   /* 1-based type ids */
+    double beta;
+    double mu_eff;
+    double w_z_extra;          /* pool z filter: z < max site z + w_z_extra (MC/src/mc.cu:93-100) */
+    int precision;             /* PHB_FP64 */
+    int pending_max;           /* accepted atom changes buffered before a potential refresh (2..64; 0 = 32) */
+    int recompute_interval;    /* full energy recompute (drift reset) every N trials; 0 = never */
+    long long record_cap;      /* per-trial records kept (0 = none) */
+} phb_mc_desc;
+typedef struct {
+    long long attempted, accepted, accepted_forward, accepted_backward;
+    int nprot, nsites, status;         /* status 1: no valid pair to move (MC/src/mc.cu:398-402) */
+    double E;                          /* running total energy */
+    double E_recomputed;               /* last full recompute (NaN if none) */
+    double max_drift;                  /* max |running - recomputed| over recomputes */
+    long long refreshes, recomputes, launches;
+} phb_mc_stats;
+typedef struct phb_mc_s *phb_mc;
+/* Builds pools (build_plan, MC/src/mc.cu:67-122), full neighbour rows with
+ * real-space coefficients, spline stencils, the mesh kernel and the potential
+ * grid; seeds the device MT19937 like std::mt19937(seed).  The atoms handle is
+ * updated in place by every accepted move and must outlive the MC object. */
+int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box *box,
+                  const phb_mc_desc *d, unsigned seed, phb_mc *out);
+int phb_mc_destroy(phb_mc mc);
+/* Runs ntrials more trials (replaces the trial loop of phase_gc_mu_control at fixed mu). */
+int phb_mc_run(phb_mc mc, long long ntrials, phb_mc_stats *out);
+int phb_mc_stats_get(phb_mc mc, phb_mc_stats *out);
+/* First n per-trial records (any output may be NULL); u is NaN when no uniform was drawn. */
+int phb_mc_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, signed char *acc, double *dE,
+                  double *u);
+/* Delta-E of n independent candidate moves (site i, ion j, fwd 1 = protonate)
+ * against the current state, without accepting them. */
+int phb_mc_delta_e(phb_mc mc, int n, const int *i, const int *j, const int *fwd, double *dE);
+/* Full recompute now (drift reset); returns the recomputed total energy. */
+int phb_mc_recompute(phb_mc mc, double *E);
+/* MT19937 state: 624 words then the index (625 unsigned). */
+int phb_mc_rng_state(phb_mc mc, unsigned *state);
+
 /* ----------------------------------------------------------- generator */
 /* Synthetic brush (stands in for the missing LAMMPS inputs). */
 typedef struct {

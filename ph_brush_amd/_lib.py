@@ -60,17 +60,21 @@ class McDesc(C.Structure):
     _fields_ = [
         ("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int), ("cl_type", C.c_int),
         ("beta", C.c_double), ("mu_eff", C.c_double), ("w_z_extra", C.c_double),
-        ("precision", C.c_int), ("pending_max", C.c_int), ("spec_window", C.c_int),
-        ("record", C.c_int),
+        ("precision", C.c_int), ("pending_max", C.c_int), ("recompute_interval", C.c_int),
+        ("record_cap", C.c_longlong),
     ]
 
 
 class McStats(C.Structure):
     _fields_ = [
         ("attempted", C.c_longlong), ("accepted", C.c_longlong), ("accepted_forward", C.c_longlong),
-        ("accepted_backward", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int),
-        ("E", C.c_double), ("refreshes", C.c_longlong), ("status", C.c_int),
+        ("accepted_backward", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int), ("status", C.c_int),
+        ("E", C.c_double), ("E_recomputed", C.c_double), ("max_drift", C.c_double),
+        ("refreshes", C.c_longlong), ("recomputes", C.c_longlong), ("launches", C.c_longlong),
     ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _h = C.c_void_p
@@ -106,6 +110,14 @@ SIGNATURES = {
     "phb_pme_fft_r2c": [_h, _pd, _pd],
     "phb_pme_influence": [_h, _pd],
     "phb_pme_potential": [_h, _pd, _pd, _pd],
+    "phb_mc_create": [_h, _h, _h, _h, C.POINTER(Box), C.POINTER(McDesc), C.c_uint, C.POINTER(_h)],
+    "phb_mc_destroy": [_h],
+    "phb_mc_run": [_h, C.c_longlong, C.POINTER(McStats)],
+    "phb_mc_stats_get": [_h, C.POINTER(McStats)],
+    "phb_mc_record": [_h, C.c_longlong, C.POINTER(C.c_byte), _pi, _pi, C.POINTER(C.c_byte), _pd, _pd],
+    "phb_mc_delta_e": [_h, C.c_int, _pi, _pi, _pi, _pd],
+    "phb_mc_recompute": [_h, _pd],
+    "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

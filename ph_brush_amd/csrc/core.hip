@@ -1,6 +1,7 @@
 // core.hip -- error state, library stream, FFT plans, reductions and the
 // atoms / CSR / params handles of the C ABI.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -128,9 +129,17 @@ extern "C" int phb_atoms_create(int natoms, const double* x, const double* y, co
     return guard([&] {
         require(out && natoms >= 0, "phb_atoms_create: bad arguments");
         require(natoms == 0 || (x && y && z && q && type0), "phb_atoms_create: null array");
+        int tmax = 0;
+        for (int i = 0; i < natoms; ++i) {
+            require(std::isfinite(x[i]) && std::isfinite(y[i]) && std::isfinite(z[i]) && std::isfinite(q[i]),
+                    "phb_atoms_create: non-finite position or charge");
+            require(type0[i] >= 0, "phb_atoms_create: negative type id");
+            tmax = std::max(tmax, type0[i]);
+        }
         require_device();
         auto a = std::make_unique<phb_atoms_s>();
         a->n = natoms;
+        a->max_type = tmax;
         a->x.alloc(natoms); a->y.alloc(natoms); a->z.alloc(natoms);
         a->q.alloc(natoms); a->type.alloc(natoms);
         a->x.upload(x, natoms); a->y.upload(y, natoms); a->z.upload(z, natoms);
@@ -164,8 +173,12 @@ __global__ void k_set_identity(int n, const int* __restrict__ idx, const int* __
 extern "C" int phb_atoms_set_identity(phb_atoms a, int n, const int* idx, const int* type0, const double* q) {
     return guard([&] {
         require(a && n >= 0 && (n == 0 || (idx && type0 && q)), "phb_atoms_set_identity: bad arguments");
-        for (int k = 0; k < n; ++k)
+        for (int k = 0; k < n; ++k) {
             require(idx[k] >= 0 && idx[k] < a->n, "set_atom_identity: atom index out of range");
+            require(type0[k] >= 0, "set_atom_identity: type out of range");
+            require(std::isfinite(q[k]), "set_atom_identity: non-finite charge");
+        }
+        for (int k = 0; k < n; ++k) a->max_type = std::max(a->max_type, type0[k]);
         if (n == 0) return PHB_OK;
         DevBuf<int> di(n), dt(n);
         DevBuf<double> dq(n);

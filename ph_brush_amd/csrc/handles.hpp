@@ -7,6 +7,7 @@
 
 struct phb_atoms_s {
     int n = 0;
+    int max_type = 0;                 // largest type id uploaded (host-side bound check)
     phb::DevBuf<double> x, y, z, q;   // SoA, wrapped positions
     phb::DevBuf<int> type;            // 0-based
 };

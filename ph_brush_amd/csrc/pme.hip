@@ -135,7 +135,9 @@ __global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, co
 // ------------------------------------------------------------ FFT kernels
 template <typename T>
 __global__ void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ kg, int nx, int ny, FftPlan px,
-                             FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy) {
+                             FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
+                             const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* stwy = stwx + nx;
@@ -172,11 +174,13 @@ __global__ void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ k
 }
 
 // mode bit 1: energy sum corner*eta*|Q|^2 into partials[blockIdx.x];
-// mode bit 2: multiply by eta, inverse z transform, store back.
+// mode bit 2: multiply by eta, inverse z transform, store back;
+// mode bit 4: store the forward transform (no eta) back.
 template <typename T>
 __global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPlan pz,
                         const cplx<T>* __restrict__ twz, const T* __restrict__ eta, int tx, int mode,
-                        double* __restrict__ partials) {
+                        double* __restrict__ partials, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double sred[16];
     cplx<T>* stw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -211,6 +215,9 @@ __global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPla
     if (mode & 2) {
         __syncthreads();
         lds_fft(C, cnt, hs, pz, stw, +1);
+    }
+    if (mode & 6) {
+        __syncthreads();
         for (int idx = tid; idx < nz * cnt; idx += bd) {
             const int kz = idx / cnt, c = idx - kz * cnt;
             kg[((size_t)kz * ny + ky) * nkx + kx0 + c] = C[c * hs + kz];
@@ -221,7 +228,8 @@ __global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPla
 template <typename T>
 __global__ void k_fft_xy_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny, FftPlan px,
                              FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
-                             int accumulate) {
+                             int accumulate, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* stwy = stwx + nx;
@@ -359,39 +367,58 @@ void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
 
 template <typename T>
 static void fft_forward_impl(phb_pme_s* P, const T* grid, cplx<T>* kg, const cplx<T>* twx, const cplx<T>* twy,
-                             const cplx<T>* twz, const T* eta, int zmode, double* partials) {
+                             const cplx<T>* twz, const T* eta, int zmode, double* partials,
+                             const int* gate = nullptr) {
     hipStream_t st = stream();
     const int nx = P->nx, ny = P->ny, nz = P->nz;
     hipLaunchKernelGGL(k_fft_xy_r2c<T>, dim3(nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), st, grid, kg,
-                       nx, ny, P->px, P->py, twx, twy);
+                       nx, ny, P->px, P->py, twx, twy, gate);
     hipLaunchKernelGGL(k_fft_z<T>, dim3(P->z_blocks), dim3(P->z_threads), P->z_lds(sizeof(cplx<T>)), st, kg, nx, ny,
-                       nz, P->pz, twz, eta, P->z_tx, zmode, partials);
+                       nz, P->pz, twz, eta, P->z_tx, zmode, partials, gate);
     PHB_HIP(hipGetLastError());
 }
 
 template <typename T>
 static void fft_inverse_impl(phb_pme_s* P, const cplx<T>* kg, T* out, const cplx<T>* twx, const cplx<T>* twy,
-                             int accumulate) {
+                             int accumulate, const int* gate = nullptr) {
     hipLaunchKernelGGL(k_fft_xy_c2r<T>, dim3(P->nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), stream(),
-                       kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate);
+                       kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate, gate);
     PHB_HIP(hipGetLastError());
 }
 
 // grid -> kgrid (-> eta, energy partials) [-> phi]
 void PmeKernels::launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate) {
-    if (P->precision == PHB_FP64) {
-        fft_forward_impl<double>(P, P->grid64.p, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
-                                 P->partials.p);
-        if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, P->phi64.p, P->twx64.p, P->twy64.p, accumulate);
-    } else {
-        fft_forward_impl<float>(P, P->grid32.p, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
-                                P->partials.p);
-        if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, P->phi32.p, P->twx32.p, P->twy32.p, accumulate);
-    }
+    if (P->precision == PHB_FP64) launch_fft_buf(P, P->grid64.p, P->phi64.p, zmode, inverse_to_phi, accumulate);
+    else launch_fft_buf(P, P->grid32.p, P->phi32.p, zmode, inverse_to_phi, accumulate);
+}
+
+void PmeKernels::launch_fft_buf(phb_pme_s* P, const double* grid, double* phi, int zmode, bool inverse_to_phi,
+                                bool accumulate, const int* gate) {
+    require(P->precision == PHB_FP64, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
+    fft_forward_impl<double>(P, grid, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
+                             P->partials.p, gate);
+    if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate, gate);
+}
+
+void PmeKernels::launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, const int* gate) {
+    fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate, gate);
+}
+
+void PmeKernels::launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, const int* gate) {
+    fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate, gate);
+}
+
+void PmeKernels::launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int zmode, bool inverse_to_phi,
+                                bool accumulate, const int* gate) {
+    require(P->precision == PHB_FP32, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
+    fft_forward_impl<float>(P, grid, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
+                            P->partials.p, gate);
+    if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate, gate);
 }
 
 void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
                              int mode, double alpha, double kf, int precision, double* partials, int nblocks) {
+    require(a->max_type < p->d.ntypes, "atom type id outside the LJ parameter table");
     PairArgs A;
     A.x = a->x.p; A.y = a->y.p; A.z = a->z.p; A.q = a->q.p; A.type = a->type.p;
     A.head = nl->head.p; A.list = nl->list.p;
@@ -703,8 +730,8 @@ extern "C" int phb_pme_fft_r2c(phb_pme P, const double* grid, double* kgrid) {
         require(P && grid && kgrid, "pme_fft_r2c: null argument");
         upload_grid(P, grid);
         const size_t Gk = (size_t)(P->nx / 2 + 1) * P->ny * P->nz;
-        // z pass with mode 0: forward transform only
-        PmeKernels::launch_fft(P, 0, false, false);
+        // z pass with mode 4: forward transform stored, no influence function
+        PmeKernels::launch_fft(P, 4, false, false);
         if (P->precision == PHB_FP64) P->kg64.download(reinterpret_cast<cplx<double>*>(kgrid), Gk);
         else {
             std::vector<cplx<float>> k(Gk);

@@ -308,3 +308,74 @@ def total_energy(atoms, params, neigh, pme, box):
     L.call("phb_total_energy", atoms.h, params.h, neigh.h, pme.h if pme else None, C.byref(box), L.dptr(comps),
            C.byref(e))
     return e.value, comps
+
+
+class MonteCarlo(_Handle):
+    """Device-resident GC protonation / ion-swap chain (phb_mc_*).
+
+    Replaces the trial loop of phase_gc_mu_control (MC/src/mc.cu:371-503) at a
+    fixed mu_eff; the atoms handle is updated in place by accepted moves."""
+    _destroy = "phb_mc_destroy"
+
+    def __init__(self, atoms, params, neigh, pme, box, mc: McParams, seed: int, *, pending_max=32,
+                 recompute_interval=1000, record_cap=0, precision=L.PHB_FP64):
+        super().__init__()
+        self._keep = (atoms, params, neigh, pme)
+        self.record_cap = record_cap
+        d = L.McDesc(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra,
+                     precision, pending_max, recompute_interval, record_cap)
+        L.call("phb_mc_create", atoms.h, params.h, neigh.h, pme.h, C.byref(box), C.byref(d), seed & 0xFFFFFFFF,
+               C.byref(self.h))
+
+    def run(self, ntrials: int) -> dict:
+        st = L.McStats()
+        L.call("phb_mc_run", self.h, ntrials, C.byref(st))
+        return st.as_dict()
+
+    def stats(self) -> dict:
+        st = L.McStats()
+        L.call("phb_mc_stats_get", self.h, C.byref(st))
+        return st.as_dict()
+
+    def record(self, n: int) -> dict:
+        out = {"fwd": np.zeros(n, np.int8), "i": np.zeros(n, np.int32), "j": np.zeros(n, np.int32),
+               "acc": np.zeros(n, np.int8), "dE": np.zeros(n), "u": np.zeros(n)}
+        L.call("phb_mc_record", self.h, n, out["fwd"].ctypes.data_as(C.POINTER(C.c_byte)), L.iptr(out["i"]),
+               L.iptr(out["j"]), out["acc"].ctypes.data_as(C.POINTER(C.c_byte)), L.dptr(out["dE"]), L.dptr(out["u"]))
+        return out
+
+    def delta_e(self, i, j, fwd) -> np.ndarray:
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        f = np.ascontiguousarray(fwd, np.int32)
+        out = np.zeros(len(i))
+        L.call("phb_mc_delta_e", self.h, len(i), L.iptr(i), L.iptr(j), L.iptr(f), L.dptr(out))
+        return out
+
+    def recompute(self) -> float:
+        e = C.c_double()
+        L.call("phb_mc_recompute", self.h, C.byref(e))
+        return e.value
+
+    def rng_state(self) -> np.ndarray:
+        st = (C.c_uint * 625)()
+        L.call("phb_mc_rng_state", self.h, st)
+        return np.array(st[:], dtype=np.uint32)
+
+
+class Replica:
+    """One system set up on the device the way main() does (MC/src/main.cu:106-138)."""
+
+    def __init__(self, s: System, p: Params, mc: McParams | None = None, seed: int = 1, **mc_kw):
+        self.s, self.p = s, p
+        self.bonds_host = bond_adjacency(s.natoms, s.bonds)
+        self.atoms = DeviceAtoms(s)
+        self.bonds = DeviceCSR.from_host(*self.bonds_host)
+        self.box = box_of(s)
+        self.neigh = build_half_neighbors_gpu(self.atoms, self.box, self.bonds, p.rc)
+        self.params = DeviceParams(p)
+        self.pme = PMEPlan(self.params, self.box, p.rc) if p.coulombtype == L.PHB_COUL_PME else None
+        self.mc = MonteCarlo(self.atoms, self.params, self.neigh, self.pme, self.box, mc, seed, **mc_kw) if mc else None
+
+    def total_energy(self):
+        return total_energy(self.atoms, self.params, self.neigh, self.pme, self.box)
