@@ -165,6 +165,8 @@ typedef struct {
     double E_recomputed;               /* last full recompute (NaN if none) */
     double max_drift;                  /* max |running - recomputed| over recomputes */
     long long refreshes, recomputes, launches;
+    /* with profiling on: summed HIP-event times on the library stream */
+    double chain_ms, refresh_ms, recompute_ms;
 } phb_mc_stats;
 typedef struct phb_mc_s *phb_mc;
 /* Builds pools (build_plan, MC/src/mc.cu:67-122), full neighbour rows with
@@ -187,6 +189,8 @@ int phb_mc_delta_e(phb_mc mc, int n, const int *i, const int *j, const int *fwd,
 int phb_mc_recompute(phb_mc mc, double *E);
 /* MT19937 state: 624 words then the index (625 unsigned). */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
+/* Time every chain launch, refresh and recompute with HIP events (on = 1). */
+int phb_mc_set_profiling(phb_mc mc, int on);
 
 /* ----------------------------------------------------------- generator */
 /* Synthetic brush (stands in for the missing LAMMPS inputs). */

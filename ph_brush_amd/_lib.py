@@ -71,6 +71,7 @@ class McStats(C.Structure):
         ("accepted_backward", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int), ("status", C.c_int),
         ("E", C.c_double), ("E_recomputed", C.c_double), ("max_drift", C.c_double),
         ("refreshes", C.c_longlong), ("recomputes", C.c_longlong), ("launches", C.c_longlong),
+        ("chain_ms", C.c_double), ("refresh_ms", C.c_double), ("recompute_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -118,6 +119,7 @@ SIGNATURES = {
     "phb_mc_delta_e": [_h, C.c_int, _pi, _pi, _pi, _pd],
     "phb_mc_recompute": [_h, _pd],
     "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
+    "phb_mc_set_profiling": [_h, C.c_int],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

@@ -31,6 +31,8 @@
 #include <memory>
 #include <random>
 #include <sstream>
+#include <utility>
+#include <vector>
 
 #include "device.hpp"
 #include "handles.hpp"
@@ -40,7 +42,7 @@ namespace phb {
 
 constexpr int MC_THREADS = 1024;
 constexpr int MC_KMAX = 64;                     // pending atom changes
-constexpr int MC_PAIRS = 3 + 2 * MC_KMAX;       // (i,i), (j,j), (i,j), (i|j, pending)
+constexpr int MC_PAIRS = 1 + 2 * MC_KMAX;       // (i,j), (i|j, pending)
 constexpr int MC_PMAX = 8;                      // spline order
 constexpr int MC_CW = 2 * MC_PMAX - 1;          // correlation width
 
@@ -68,6 +70,7 @@ struct McDev {
     const int* sbase;     // 3 per atom: first stencil index per axis, wrapped
     const double* sw;     // 3*P per atom: weights x[P], y[P], z[P]
     const double *phi, *gk;
+    const double* selfw;  // per atom W_a^T G W_a (positions are fixed)
     int nx, ny, nz, P;
     int T;
     const double *eps, *sig, *shift;
@@ -208,11 +211,9 @@ __device__ double block_delta_e(const McDev& D, const MoveInfo& mv, const int* p
     const int tid = threadIdx.x, bd = blockDim.x;
     const int P = D.P, P3 = 3 * P, CW = 2 * P - 1;
     const double dqi = mv.qi_new - mv.qi_old, dqj = mv.qj_new - mv.qj_old;
-    // pair list
+    // pair list (the (i,i) and (j,j) self pairs use the precomputed W^T G W)
     if (tid == 0) {
         int n = 0;
-        PL.a[n] = mv.i; PL.b[n] = mv.i; PL.w[n++] = 0.5 * dqi * dqi;
-        PL.a[n] = mv.j; PL.b[n] = mv.j; PL.w[n++] = 0.5 * dqj * dqj;
         PL.a[n] = mv.i; PL.b[n] = mv.j; PL.w[n++] = dqi * dqj;
         for (int k = 0; k < pend_n; ++k) {
             PL.a[n] = mv.i; PL.b[n] = pend_atom[k]; PL.w[n++] = dqi * pend_dq[k];
@@ -241,7 +242,7 @@ __device__ double block_delta_e(const McDev& D, const MoveInfo& mv, const int* p
     const int nS = 2 * P * P;
     const int nX = npairs * CW * CW;
     const int total = Ri + Rj + nS + nX;
-    double acc = 0.0;
+    double acc = tid == 0 ? 0.5 * dqi * dqi * D.selfw[mv.i] + 0.5 * dqj * dqj * D.selfw[mv.j] : 0.0;
     for (int it = tid; it < total; it += bd) {
         if (it < Ri + Rj) {
             // neighbour rows: atom i first (j still old), then atom j (i already new)
@@ -617,6 +618,33 @@ __global__ void k_stencils(int N, const double* __restrict__ x, const double* __
         for (int k = 0; k < P; ++k) sw[(size_t)i * 3 * P + ax * P + k] = w[ax][k];
 }
 
+// W_a^T G W_a for every atom: correlation of the stencil with itself against g.
+__global__ void k_self_terms(int N, int P, int nx, int ny, int nz, const double* __restrict__ sw,
+                             const double* __restrict__ gk, double* __restrict__ selfw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int CW = 2 * P - 1;
+    double C[3][MC_CW];
+    for (int ax = 0; ax < 3; ++ax) {
+        const double* w = sw + (size_t)i * 3 * P + ax * P;
+        for (int dd = 0; dd < CW; ++dd) {
+            const int d = dd - (P - 1);
+            double c = 0.0;
+            for (int u = max(0, d); u < min(P, P + d); ++u) c += w[u] * w[u - d];
+            C[ax][dd] = c;
+        }
+    }
+    double s = 0.0;
+    for (int dz = 0; dz < CW; ++dz)
+        for (int dy = 0; dy < CW; ++dy) {
+            const double* row = gk + ((size_t)wrap(dz - (P - 1), nz) * ny + wrap(dy - (P - 1), ny)) * nx;
+            double sx = 0.0;
+            for (int dx = 0; dx < CW; ++dx) sx += C[0][dx] * row[wrap(dx - (P - 1), nx)];
+            s += C[2][dz] * C[1][dy] * sx;
+        }
+    selfw[i] = s;
+}
+
 __global__ void k_delta_grid(double* __restrict__ g) {
     if (threadIdx.x == 0 && blockIdx.x == 0) g[0] = 1.0;
 }
@@ -635,7 +663,7 @@ struct phb_mc_s {
     phb_mc_desc d{};
     int nsites = 0, nions = 0;
     DevBuf<int> fhead, fnbr, sbase;
-    DevBuf<double> kreal, r2, sw, phi, gk, dgrid;
+    DevBuf<double> kreal, r2, sw, phi, gk, dgrid, selfw;
     DevBuf<int> pools[4];
     DevBuf<McCtl> ctl;
     DevBuf<signed char> rec_fwd, rec_acc;
@@ -643,6 +671,42 @@ struct phb_mc_s {
     DevBuf<double> rec_dE, rec_u;
     McDev dev{};
     long long refreshes = 0, recomputes = 0, launches = 0, next_recompute = 0;
+    // profiling: event triples around (chain launch, refresh) pairs and recomputes
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)
+    double chain_ms = 0, refresh_ms = 0, recompute_ms = 0;
+    ~phb_mc_s() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipEvent_t event() {
+        if (ev_used == ev.size()) {
+            hipEvent_t e;
+            PHB_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        hipEvent_t e = ev[ev_used++];
+        PHB_HIP(hipEventRecord(e, stream()));
+        return e;
+    }
+    // after a stream sync: fold recorded intervals into the totals
+    void collect() {
+        for (auto& m : marks) {
+            float a = 0, b = 0;
+            if (m.second == 0) {
+                PHB_HIP(hipEventElapsedTime(&a, ev[m.first], ev[m.first + 1]));
+                PHB_HIP(hipEventElapsedTime(&b, ev[m.first + 1], ev[m.first + 2]));
+                chain_ms += a;
+                refresh_ms += b;
+            } else {
+                PHB_HIP(hipEventElapsedTime(&a, ev[m.first], ev[m.first + 1]));
+                recompute_ms += a;
+            }
+        }
+        marks.clear();
+        ev_used = 0;
+    }
 };
 
 namespace {
@@ -651,12 +715,15 @@ McCtl read_ctl(phb_mc_s* M) {
     McCtl c;
     PHB_HIP(hipMemcpyAsync(&c, M->ctl.p, sizeof(McCtl), hipMemcpyDeviceToHost, stream()));
     sync();
+    if (M->profiling) M->collect();
     return c;
 }
 
 // Full recompute: phi = G Q from scratch, total energy, moments, E reset.
 void full_recompute(phb_mc_s* M) {
     phb_pme_s* P = M->pme;
+    size_t e0 = M->ev_used;
+    if (M->profiling) M->event();
     PmeKernels::launch_spread(P, M->atoms);
     PmeKernels::launch_fft_buf(P, P->grid64.p, M->phi.p, 3, true, false);
     double* res = P->result.p;
@@ -669,6 +736,7 @@ void full_recompute(phb_mc_s* M) {
     reduce_partials(P->partials.p, nbp, 4, 4, res + 3);
     hipLaunchKernelGGL(k_finish_recompute, dim3(1), dim3(64), 0, stream(), M->dev, res, 0.0);
     PHB_HIP(hipGetLastError());
+    if (M->profiling) { M->event(); M->marks.push_back({e0, 1}); }
     ++M->recomputes;
 }
 
@@ -698,6 +766,9 @@ void fill_stats(phb_mc_s* M, const McCtl& c, phb_mc_stats* s) {
     s->refreshes = M->refreshes;
     s->recomputes = M->recomputes;
     s->launches = M->launches;
+    s->chain_ms = M->chain_ms;
+    s->refresh_ms = M->refresh_ms;
+    s->recompute_ms = M->recompute_ms;
 }
 
 }  // namespace
@@ -785,6 +856,9 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         hipLaunchKernelGGL(k_delta_grid, dim3(1), dim3(64), 0, st, M->dgrid.p);
         PmeKernels::launch_fft_buf(pme, M->dgrid.p, M->gk.p, 2, true, false);
         PHB_HIP(hipMemsetAsync(M->dgrid.p, 0, G * sizeof(double), st));
+        M->selfw.alloc(N);
+        hipLaunchKernelGGL(k_self_terms, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, P, pme->nx, pme->ny, pme->nz,
+                           M->sw.p, M->gk.p, M->selfw.p);
         // ---- records
         const long long rc = d->record_cap;
         M->rec_fwd.alloc(std::max<long long>(rc, 1)); M->rec_acc.alloc(std::max<long long>(rc, 1));
@@ -810,7 +884,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         McDev& D = M->dev;
         D.N = N; D.z = a->z.p; D.q = a->q.p; D.type = a->type.p;
         D.fhead = M->fhead.p; D.fnbr = M->fnbr.p; D.kreal = M->kreal.p; D.r2 = M->r2.p;
-        D.sbase = M->sbase.p; D.sw = M->sw.p; D.phi = M->phi.p; D.gk = M->gk.p;
+        D.sbase = M->sbase.p; D.sw = M->sw.p; D.phi = M->phi.p; D.gk = M->gk.p; D.selfw = M->selfw.p;
         D.nx = pme->nx; D.ny = pme->ny; D.nz = pme->nz; D.P = P;
         D.T = T; D.eps = p->d_eps.p; D.sig = p->d_sig.p; D.shift = p->d_shift.p;
         D.use_shift = p->d.lj_shift ? 1 : 0; D.use_cut = p->lj_cut_active ? 1 : 0; D.lj_rc2 = p->lj_rc * p->lj_rc;
@@ -851,9 +925,13 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
             int batch = 4;
             while (done < chunk_end && status == 0) {
                 for (int b = 0; b < batch; ++b) {
+                    const size_t e0 = M->ev_used;
+                    if (M->profiling) M->event();
                     hipLaunchKernelGGL(k_mc_chain, dim3(1), dim3(MC_THREADS), 0, stream(), M->dev);
                     ++M->launches;
+                    if (M->profiling) M->event();
                     refresh(M);
+                    if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
                 }
                 PHB_HIP(hipGetLastError());
                 c = read_ctl(M);
@@ -923,6 +1001,17 @@ extern "C" int phb_mc_recompute(phb_mc M, double* E) {
         full_recompute(M);
         McCtl c = read_ctl(M);
         if (E) *E = c.E;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_mc_set_profiling(phb_mc M, int on) {
+    return guard([&] {
+        require(M != nullptr, "phb_mc_set_profiling: null");
+        sync();
+        if (M->profiling) M->collect();
+        M->profiling = on != 0;
+        M->chain_ms = M->refresh_ms = M->recompute_ms = 0.0;
         return PHB_OK;
     });
 }

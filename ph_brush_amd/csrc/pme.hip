@@ -418,7 +418,7 @@ void PmeKernels::launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int
 
 void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
                              int mode, double alpha, double kf, int precision, double* partials, int nblocks) {
-    require(a->max_type < p->d.ntypes, "atom type id outside the LJ parameter table");
+    if (mode & PAIR_LJ) require(a->max_type < p->d.ntypes, "atom type id outside the LJ parameter table");
     PairArgs A;
     A.x = a->x.p; A.y = a->y.p; A.z = a->z.p; A.q = a->q.p; A.type = a->type.p;
     A.head = nl->head.p; A.list = nl->list.p;

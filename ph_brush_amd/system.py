@@ -357,6 +357,9 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_recompute", self.h, C.byref(e))
         return e.value
 
+    def set_profiling(self, on: bool = True):
+        L.call("phb_mc_set_profiling", self.h, int(on))
+
     def rng_state(self) -> np.ndarray:
         st = (C.c_uint * 625)()
         L.call("phb_mc_rng_state", self.h, st)

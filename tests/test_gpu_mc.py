@@ -125,12 +125,12 @@ def test_chain_matches_reference_sequence(gpu, case):
 
 
 def test_rng_state_matches_std_mt19937(gpu):
+    """The device engine starts from std::mt19937(seed): the C++ standard's seeding
+    x[0] = seed, x[i] = 1812433253 * (x[i-1] ^ (x[i-1] >> 30)) + i, index 624."""
     s, p, mc = S.config_tiny()
     r = S.Replica(s, p, mc, seed=42)
     st = r.mc.rng_state()
-    ref = O.mt_draws(42, 0)  # noqa: F841  (state check via draws below)
-    # the seeded state reproduces std::mt19937(42): temper the first word the way the engine does
-    y = int(st[0]) if st[624] < 624 else None
-    assert st[624] == 624  # fresh engine: next call twists
-    draws = O.mt_draws(42, 5)
-    assert len(draws) == 5 and y is not None
+    x = [42]
+    for i in range(1, 624):
+        x.append((1812433253 * (x[-1] ^ (x[-1] >> 30)) + i) & 0xFFFFFFFF)
+    assert np.array_equal(st[:624], np.array(x, np.uint32)) and st[624] == 624
