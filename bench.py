@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--trials-per-step", type=int, default=20000)
     ap.add_argument("--pending-max", type=int, default=32)
+    ap.add_argument("--spec-window", type=int, default=16)
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -89,7 +90,8 @@ def main():
     cfg = S.config_30k_fp64
     s, p, mc = cfg()
     t_setup = time.perf_counter()
-    rep = S.Replica(s, p, mc, seed=1, pending_max=args.pending_max, recompute_interval=1000)
+    rep = S.Replica(s, p, mc, seed=1, pending_max=args.pending_max, recompute_interval=1000,
+                    spec_window=args.spec_window)
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] setup {t_setup:.2f}s natoms={s.natoms}", file=sys.stderr, flush=True)
 
@@ -142,6 +144,7 @@ def main():
             "natoms": s.natoms,
             "trials_per_step": T,
             "pending_max": args.pending_max,
+            "spec_window": args.spec_window,
             "acceptance": (st1["accepted"] - st0["accepted"]) / max(trials, 1),
             "parallelism": "1 chain / 1 GPU",
         },

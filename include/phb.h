@@ -157,6 +157,7 @@ typedef struct {
     int pending_max;           /* accepted atom changes buffered before a potential refresh (2..64; 0 = 32) */
     int recompute_interval;    /* full energy recompute (drift reset) every N trials; 0 = never */
     long long record_cap;      /* per-trial records kept (0 = none) */
+    int spec_window;           /* trials evaluated concurrently per round: 1, 2, 4, 8, 16 (0 = 16) */
 } phb_mc_desc;
 typedef struct {
     long long attempted, accepted, accepted_forward, accepted_backward;

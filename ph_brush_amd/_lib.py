@@ -61,7 +61,7 @@ class McDesc(C.Structure):
         ("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int), ("cl_type", C.c_int),
         ("beta", C.c_double), ("mu_eff", C.c_double), ("w_z_extra", C.c_double),
         ("precision", C.c_int), ("pending_max", C.c_int), ("recompute_interval", C.c_int),
-        ("record_cap", C.c_longlong),
+        ("record_cap", C.c_longlong), ("spec_window", C.c_int),
     ]
 
 

@@ -42,7 +42,6 @@ namespace phb {
 
 constexpr int MC_THREADS = 1024;
 constexpr int MC_KMAX = 64;                     // pending atom changes
-constexpr int MC_PAIRS = 1 + 2 * MC_KMAX;       // (i,j), (i|j, pending)
 constexpr int MC_PMAX = 8;                      // spline order
 constexpr int MC_CW = 2 * MC_PMAX - 1;          // correlation width
 
@@ -51,6 +50,7 @@ struct McCtl {
     long long accepted, acc_fwd, acc_bwd;
     long long rec_cap;
     int pending_n, pending_max, status, nprot;
+    int pending_folded;           // set by the k-space refresh: phi already holds the pending changes
     int sizes[4];                 // nh_n, nh_p, w, cl
     double E, Q, Q2, Mz, Qz2;
     double E_recomputed, max_drift;
@@ -70,13 +70,17 @@ struct McDev {
     const int* sbase;     // 3 per atom: first stencil index per axis, wrapped
     const double* sw;     // 3*P per atom: weights x[P], y[P], z[P]
     const double *phi, *gk;
+    const double* gf;     // g folded to |offset| per axis: (nz/2+1) x (ny/2+1) x (nx/2+1)
+    int fnx, fny;         // nx/2+1, ny/2+1
     const double* selfw;  // per atom W_a^T G W_a (positions are fixed)
     int nx, ny, nz, P;
     int T;
     const double *eps, *sig, *shift;
     int use_shift, use_cut;
     double lj_rc2;
-    int* pool[4];
+    int* pbuf;            // pools, nsites + nions slots (layout at pool_slot)
+    int nsites, nions;
+    int W;                // speculative trials per round (1, 2, 4, 8 or 16)
     int t_nhn, t_nhp, t_w, t_cl;   // 0-based
     double beta, mu, elF, alpha, vol, Lz, self_coef;
     int use3dc;
@@ -87,12 +91,20 @@ struct McDev {
     double *rec_dE, *rec_u;
 };
 
-// ------------------------------------------------------------ MT19937
+// ------------------------------------------------------------ MT19937 stream
 constexpr unsigned MT_UPPER = 0x80000000u, MT_LOWER = 0x7fffffffu, MT_A = 0x9908b0dfu;
 
 __device__ __forceinline__ unsigned mt_mix(unsigned a, unsigned b, unsigned c) {
     unsigned y = (a & MT_UPPER) | (b & MT_LOWER);
     return c ^ (y >> 1) ^ ((y & 1u) ? MT_A : 0u);
+}
+
+__device__ __forceinline__ unsigned mt_temper(unsigned y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
 }
 
 // The standard sequential twist split into dependency phases
@@ -139,26 +151,70 @@ __device__ void mt_twist_wave(unsigned* mt, int lane) {
     __builtin_amdgcn_wave_barrier();
 }
 
-// All 64 lanes call with identical mti (uniform control flow).
-__device__ __forceinline__ unsigned mt_next(unsigned* mt, int& mti, int lane) {
-    if (mti >= 624) { mt_twist_wave(mt, lane); mti = 0; }
-    unsigned y = mt[mti++];
-    y ^= y >> 11;
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
-    y ^= y >> 18;
+// The engine's output stream addressed by position, so that speculative draws
+// can be rolled back: block b (positions [624b, 624b+624)) is kept tempered in
+// out[b & 1]; cur holds the raw state of the newest block, prev the one before.
+struct RngLds {
+    unsigned cur[624], prev[624];
+    unsigned out[2][624];
+};
+
+// All lanes of wave 0 call with identical (blk, pos).
+__device__ __forceinline__ unsigned rng_draw(RngLds& R, int& blk, int& pos, int lane) {
+    if (pos >= (blk + 1) * 624) {
+        for (int k = lane; k < 624; k += 64) R.prev[k] = R.cur[k];
+        __builtin_amdgcn_wave_barrier();
+        mt_twist_wave(R.cur, lane);
+        const int h = (blk + 1) & 1;
+        for (int k = lane; k < 624; k += 64) R.out[h][k] = mt_temper(R.cur[k]);
+        __builtin_amdgcn_wave_barrier();
+        ++blk;
+    }
+    const unsigned y = R.out[(pos / 624) & 1][pos % 624];
+    ++pos;
     return y;
+}
+
+// Make positions up to pos_end (exclusive) readable; needs pos_end - pos < 624.
+__device__ __forceinline__ void rng_ensure(RngLds& R, int& blk, int pos_end, int lane) {
+    if (pos_end > (blk + 1) * 624) {
+        for (int k = lane; k < 624; k += 64) R.prev[k] = R.cur[k];
+        __builtin_amdgcn_wave_barrier();
+        mt_twist_wave(R.cur, lane);
+        const int h = (blk + 1) & 1;
+        for (int k = lane; k < 624; k += 64) R.out[h][k] = mt_temper(R.cur[k]);
+        __builtin_amdgcn_wave_barrier();
+        ++blk;
+    }
+}
+
+__device__ __forceinline__ unsigned rng_at(const RngLds& R, int q) { return R.out[(q / 624) & 1][q % 624]; }
+
+__device__ __forceinline__ double canonical_at(const RngLds& R, int q) {
+#pragma clang fp contract(off)
+    const double g1 = (double)rng_at(R, q), g2 = (double)rng_at(R, q + 1);
+    double r = (g1 + g2 * 4294967296.0) / 18446744073709551616.0;
+    if (r >= 1.0) r = __longlong_as_double(0x3FEFFFFFFFFFFFFFLL);
+    return r;
+}
+
+// Lemire draw at a fixed position; false when the rejection branch would need more draws.
+__device__ __forceinline__ bool index_at(const RngLds& R, int q, unsigned n, unsigned& out) {
+    const unsigned long long prod = (unsigned long long)rng_at(R, q) * n;
+    const unsigned low = (unsigned)prod;
+    out = (unsigned)(prod >> 32);
+    return !(low < n && low < (0u - n) % n);
 }
 
 // libstdc++ uniform_int_distribution<int>(0, n-1) on a 32-bit engine: Lemire's
 // nearly divisionless method with a 64-bit product.
-__device__ __forceinline__ unsigned draw_index(unsigned* mt, int& mti, int lane, unsigned n) {
-    unsigned long long prod = (unsigned long long)mt_next(mt, mti, lane) * n;
+__device__ __forceinline__ unsigned draw_index(RngLds& R, int& blk, int& pos, int lane, unsigned n) {
+    unsigned long long prod = (unsigned long long)rng_draw(R, blk, pos, lane) * n;
     unsigned low = (unsigned)prod;
     if (low < n) {
         const unsigned thr = (0u - n) % n;
         while (low < thr) {
-            prod = (unsigned long long)mt_next(mt, mti, lane) * n;
+            prod = (unsigned long long)rng_draw(R, blk, pos, lane) * n;
             low = (unsigned)prod;
         }
     }
@@ -166,23 +222,30 @@ __device__ __forceinline__ unsigned draw_index(unsigned* mt, int& mti, int lane,
 }
 
 // generate_canonical<double,53>: (g1 + g2*2^32) / 2^64, clamped below 1.
-__device__ __forceinline__ double draw_canonical(unsigned* mt, int& mti, int lane) {
+__device__ __forceinline__ double draw_canonical(RngLds& R, int& blk, int& pos, int lane) {
 #pragma clang fp contract(off)
-    const double g1 = (double)mt_next(mt, mti, lane);
-    const double g2 = (double)mt_next(mt, mti, lane);
+    const double g1 = (double)rng_draw(R, blk, pos, lane);
+    const double g2 = (double)rng_draw(R, blk, pos, lane);
     double s = g1 + g2 * 4294967296.0;
     double r = s / 18446744073709551616.0;
     if (r >= 1.0) r = __longlong_as_double(0x3FEFFFFFFFFFFFFFLL);
     return r;
 }
 
-// ------------------------------------------------------------ Delta-E
-struct MoveInfo {
-    int i, j;
-    int ti_old, ti_new, tj_old, tj_new;
-    double qi_old, qi_new, qj_old, qj_new;
-};
+// ------------------------------------------------------------ pools
+// The four pools share one buffer of nsites + nions slots: nh_n grows up from
+// slot 0 and nh_p down from slot nsites-1; w grows up from nsites and cl down
+// from nsites+nions-1.  Logical order is the reference's std::vector order.
+__device__ __forceinline__ int pool_slot(const McDev& D, int pool, int k) {
+    switch (pool) {
+        case 0: return k;
+        case 1: return D.nsites - 1 - k;
+        case 2: return D.nsites + k;
+        default: return D.nsites + D.nions - 1 - k;
+    }
+}
 
+// ------------------------------------------------------------ Delta-E
 __device__ __forceinline__ double lj_pair(const McDev& D, int t, int tn, double r2) {
     if (D.use_cut && r2 > D.lj_rc2) return 0.0;
     const int k = t * D.T + tn;
@@ -195,96 +258,120 @@ __device__ __forceinline__ double lj_pair(const McDev& D, int t, int tn, double 
     return e;
 }
 
-struct PairLds {
-    int npairs;
-    int a[MC_PAIRS], b[MC_PAIRS];
-    double w[MC_PAIRS];
-    int dlt[MC_PAIRS][3];            // (base_a - base_b) mod n per axis
-    double C[MC_PAIRS][3][MC_CW];    // stencil correlations per axis
+// One speculative trial (LDS).
+template <int P>
+struct TrialT {
+    int i, j, fwd, pos_prop;
+    int ti_old, ti_new, tj_old, tj_new;
+    double qi_old, qi_new, qj_old, qj_new;
+    double zi, zj;
+    int sb[2][3];            // first stencil index per axis of i and j
+    int hrow[2], nrow[2];    // neighbour-row start and length of i and j
+    double w[2][3][P];       // spline weights of i and j
+    double self[2];          // W^T G W of i and j
+    double u;                // uniform pre-drawn for the Metropolis test
 };
 
-// Block-wide Delta-E of the move in `mv` (LDS) against the current state,
-// with the pending changes in pend_atom/pend_dq.  Returns the sum of pair,
-// real-space and reciprocal terms in thread 0 (O(1) corrections excluded).
-__device__ double block_delta_e(const McDev& D, const MoveInfo& mv, const int* pend_atom, const double* pend_dq,
-                                int pend_n, PairLds& PL, double* sred) {
-    const int tid = threadIdx.x, bd = blockDim.x;
-    const int P = D.P, P3 = 3 * P, CW = 2 * P - 1;
-    const double dqi = mv.qi_new - mv.qi_old, dqj = mv.qj_new - mv.qj_old;
-    // pair list (the (i,i) and (j,j) self pairs use the precomputed W^T G W)
-    if (tid == 0) {
-        int n = 0;
-        PL.a[n] = mv.i; PL.b[n] = mv.j; PL.w[n++] = dqi * dqj;
-        for (int k = 0; k < pend_n; ++k) {
-            PL.a[n] = mv.i; PL.b[n] = pend_atom[k]; PL.w[n++] = dqi * pend_dq[k];
-            PL.a[n] = mv.j; PL.b[n] = pend_atom[k]; PL.w[n++] = dqj * pend_dq[k];
-        }
-        PL.npairs = n;
+template <int P>
+struct PendT {               // atom changes since the last refresh of phi
+    int atom[MC_KMAX];
+    double dq[MC_KMAX];
+    int sb[MC_KMAX][3];
+    double w[MC_KMAX][3][P];
+};
+
+__device__ __forceinline__ int wrap1(int x, int n) {  // x in (-n, 2n)
+    x += x < 0 ? n : 0;
+    return x >= n ? x - n : x;
+}
+
+__device__ __forceinline__ int fold(int x, int n) {   // x in [0, n) -> periodic |x|
+    return 2 * x <= n ? x : n - x;
+}
+
+// Stencil correlation sum_{u - v = d} wa[u] wb[v] for d = dd - (P-1).
+template <int P>
+__device__ __forceinline__ double corr(const double* wa, const double* wb, int dd) {
+    const int d = dd - (P - 1);
+    double c = 0.0;
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        const int v = u - d;
+        if (v >= 0 && v < P) c += wa[u] * wb[v];
     }
-    __syncthreads();
-    const int npairs = PL.npairs;
-    // stencil correlations C_axis(d) = sum_{u-v=d} w_a[u] w_b[v], d in [-(P-1), P-1]
-    for (int it = tid; it < npairs * 3 * CW; it += bd) {
-        const int pr = it / (3 * CW), rem = it - pr * 3 * CW, ax = rem / CW, d = rem - ax * CW - (P - 1);
-        const double* wa = D.sw + (size_t)PL.a[pr] * P3 + ax * P;
-        const double* wb = D.sw + (size_t)PL.b[pr] * P3 + ax * P;
-        double c = 0.0;
-        for (int u = max(0, d); u < min(P, P + d); ++u) c += wa[u] * wb[u - d];
-        PL.C[pr][ax][d + P - 1] = c;
-        if (d == 0) {
-            const int n = ax == 0 ? D.nx : (ax == 1 ? D.ny : D.nz);
-            PL.dlt[pr][ax] = wrap(D.sbase[3 * PL.a[pr] + ax] - D.sbase[3 * PL.b[pr] + ax], n);
-        }
-    }
-    __syncthreads();
-    const int hi = D.fhead[mv.i], Ri = D.fhead[mv.i + 1] - hi;
-    const int hj = D.fhead[mv.j], Rj = D.fhead[mv.j + 1] - hj;
-    const int nS = 2 * P * P;
-    const int nX = npairs * CW * CW;
-    const int total = Ri + Rj + nS + nX;
-    double acc = tid == 0 ? 0.5 * dqi * dqi * D.selfw[mv.i] + 0.5 * dqj * dqj * D.selfw[mv.j] : 0.0;
-    for (int it = tid; it < total; it += bd) {
-        if (it < Ri + Rj) {
-            // neighbour rows: atom i first (j still old), then atom j (i already new)
+    return c;
+}
+
+// This thread's share (items lt, lt+nt, ...) of the pair, real-space and
+// reciprocal Delta-E of trial t against phi plus the pending changes.
+//   items [0, Ri+Rj):       neighbour-row entries (atom i first, then j with i new)
+//   items [.., + 2P^2):     one (z, y) row of dq_a W_a . phi
+//   items [.., + (1+2np)CW): one dz plane of w_ab sum_d C(d) g(delta + d) for the
+//                            pair (i,j) and the pairs (i|j, pending atom)
+template <int P>
+__device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<P>& pd, int pend_n, int lt, int nt) {
+    constexpr int CW = 2 * P - 1, PP = P * P;
+    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+    const int Ri = t.nrow[0], nA = Ri + t.nrow[1], nB = nA + 2 * PP;
+    const int total = nB + (1 + 2 * pend_n) * CW * CW;
+    double acc = 0.0;
+    for (int it = lt; it < total; it += nt) {
+        if (it < nA) {
             const bool first = it < Ri;
-            const int e = first ? hi + it : hj + (it - Ri);
+            const int e = first ? t.hrow[0] + it : t.hrow[1] + (it - Ri);
             const int n = D.fnbr[e];
             double qn;
             int tn;
-            if (!first && n == mv.i) { qn = mv.qi_new; tn = mv.ti_new; }
+            if (!first && n == t.i) { qn = t.qi_new; tn = t.ti_new; }
             else { qn = D.q[n]; tn = D.type[n]; }
-            const double dq = first ? dqi : dqj;
-            const int to = first ? mv.ti_old : mv.tj_old, tnw = first ? mv.ti_new : mv.tj_new;
-            double v = dq * qn * D.kreal[e];
-            if (to != tnw) {
+            double v = (first ? dqi : dqj) * qn * D.kreal[e];
+            const int to = first ? t.ti_old : t.tj_old, tw = first ? t.ti_new : t.tj_new;
+            if (to != tw) {
                 const double r2 = D.r2[e];
-                v += lj_pair(D, tnw, tn, r2) - lj_pair(D, to, tn, r2);
+                v += lj_pair(D, tw, tn, r2) - lj_pair(D, to, tn, r2);
             }
             acc += v;
-        } else if (it < Ri + Rj + nS) {
-            // dq_a * W_a . phi, one (z, y) stencil row per item
-            const int s = it - Ri - Rj, which = s / (P * P), r = s - which * P * P, wz = r / P, wy = r - wz * P;
-            const int a = which ? mv.j : mv.i;
-            const double dq = which ? dqj : dqi;
-            const int* sb = D.sbase + 3 * a;
-            const double* w = D.sw + (size_t)a * P3;
-            const int gz = wrap(sb[2] + wz, D.nz), gy = wrap(sb[1] + wy, D.ny);
+        } else if (it < nB) {
+            const int s = it - nA, which = s / PP, r = s - which * PP, wz = r / P, wy = r - wz * P;
+            const int* sb = t.sb[which];
+            const int gz = wrap1(sb[2] + wz, D.nz), gy = wrap1(sb[1] + wy, D.ny);
             const double* row = D.phi + ((size_t)gz * D.ny + gy) * D.nx;
             double sx = 0.0;
-            for (int u = 0; u < P; ++u) sx += w[u] * row[wrap(sb[0] + u, D.nx)];
-            acc += dq * w[2 * P + wz] * w[P + wy] * sx;
+#pragma unroll
+            for (int u = 0; u < P; ++u) sx += t.w[which][0][u] * row[wrap1(sb[0] + u, D.nx)];
+            acc += (which ? dqj : dqi) * t.w[which][2][wz] * t.w[which][1][wy] * sx;
         } else {
-            // w_ab * sum_d C(d) g(delta + d), one (dz, dy) row per item
-            const int s = it - Ri - Rj - nS, pr = s / (CW * CW), r = s - pr * CW * CW, dz = r / CW, dy = r - dz * CW;
-            const int gz = wrap(PL.dlt[pr][2] + dz - (P - 1), D.nz), gy = wrap(PL.dlt[pr][1] + dy - (P - 1), D.ny);
-            const double* row = D.gk + ((size_t)gz * D.ny + gy) * D.nx;
-            const int x0 = PL.dlt[pr][0] - (P - 1);
+            const int s = it - nB, pr = s / (CW * CW), r = s - pr * (CW * CW), dz = r / CW, dy = r - dz * CW;
+            int ta;
+            const int* sbb;
+            const double(*wb)[P];
+            double wgt;
+            if (pr == 0) { ta = 0; sbb = t.sb[1]; wb = t.w[1]; wgt = dqi * dqj; }
+            else {
+                const int m = (pr - 1) >> 1;
+                ta = (pr - 1) & 1;
+                sbb = pd.sb[m]; wb = pd.w[m];
+                wgt = (ta ? dqj : dqi) * pd.dq[m];
+            }
+            const int* sba = t.sb[ta];
+            const double(*wa)[P] = t.w[ta];
+            // g is even along each axis: read the folded table at |offset|
+            const int x0 = wrap1(sba[0] - sbb[0], D.nx) - (P - 1);
+            const int gy = fold(wrap1(wrap1(sba[1] - sbb[1], D.ny) - (P - 1) + dy, D.ny), D.ny);
+            const int gz = fold(wrap1(wrap1(sba[2] - sbb[2], D.nz) - (P - 1) + dz, D.nz), D.nz);
+            const double* row = D.gf + ((size_t)gz * D.fny + gy) * D.fnx;
             double sx = 0.0;
-            for (int dx = 0; dx < CW; ++dx) sx += PL.C[pr][0][dx] * row[wrap(x0 + dx, D.nx)];
-            acc += PL.w[pr] * PL.C[pr][2][dz] * PL.C[pr][1][dy] * sx;
+#pragma unroll
+            for (int d = 0; d < CW; ++d) sx += corr<P>(wa[0], wb[0], d) * row[fold(wrap1(x0 + d, D.nx), D.nx)];
+            acc += wgt * corr<P>(wa[2], wb[2], dz) * corr<P>(wa[1], wb[1], dy) * sx;
         }
     }
-    return block_sum(acc, sred);
+    return acc;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
 }
 
 __device__ __forceinline__ double slab_energy(const McDev& D, double Mz, double Q, double Qz2) {
@@ -296,16 +383,18 @@ __device__ __forceinline__ double slab_energy(const McDev& D, double Mz, double 
 }
 
 // O(1) self, net-charge and slab terms of a move (MC/src/pme.cu:425-447).
-__device__ double correction_delta(const McDev& D, const MoveInfo& mv, double Q, double Q2, double Mz, double Qz2) {
-    const double dqi = mv.qi_new - mv.qi_old, dqj = mv.qj_new - mv.qj_old;
-    const double zi = D.z[mv.i], zj = D.z[mv.j];
-    const double dQ2 = (mv.qi_new * mv.qi_new - mv.qi_old * mv.qi_old) + (mv.qj_new * mv.qj_new - mv.qj_old * mv.qj_old);
+template <int P>
+__device__ double correction_delta(const McDev& D, const TrialT<P>& t, double Q, double Q2, double Mz, double Qz2) {
+    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+    const double zi = t.zi, zj = t.zj;
+    const double dQ2 = (t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old);
     const double Qn = Q + dqi + dqj;
     const double Mzn = Mz + dqi * zi + dqj * zj;
     const double Qz2n = Qz2 + dqi * zi * zi + dqj * zj * zj;
     const double e_self = -D.self_coef * dQ2;
     const double e_q = -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q);
-    return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2));
+    return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2)) +
+           0.5 * dqi * dqi * t.self[0] + 0.5 * dqj * dqj * t.self[1];
 }
 
 // Metropolis exponent exactly as the reference forms it (MC/src/mc.cu:434-439).
@@ -317,215 +406,403 @@ __device__ __forceinline__ double mh_exponent(double beta, double dE, int s, dou
     return fmin(fmax(expo, -40.0), 40.0);
 }
 
+// The move of a trial from its (i, j, direction) (any one thread; no memory reads).
+template <int P>
+__device__ __forceinline__ void stage_trial(const McDev& D, TrialT<P>& t, int i, int j, int fwd) {
+    t.i = i; t.j = j; t.fwd = fwd;
+    t.ti_new = fwd ? D.t_nhp : D.t_nhn; t.qi_new = fwd ? 1.0 : 0.0;
+    t.tj_new = fwd ? D.t_cl : D.t_w; t.qj_new = fwd ? -1.0 : 0.0;
+}
+
+// Current identities, positions, stencils and rows of the trial atoms, loaded
+// in parallel by `nthr` threads from `lt`.
+template <int P>
+__device__ __forceinline__ void stage_trial_data(const McDev& D, TrialT<P>& t, int lt, int nthr) {
+    constexpr int NW = 2 * 3 * P;
+    for (int k = lt; k < NW + 18; k += nthr) {
+        if (k < NW) {
+            const int a = k / (3 * P), r = k - a * 3 * P;
+            const int atom = a ? t.j : t.i;
+            (&t.w[a][0][0])[r] = D.sw[(size_t)atom * 3 * P + r];
+        } else {
+            const int q = k - NW;
+            const int a = q / 9, r = q - a * 9;
+            const int atom = a ? t.j : t.i;
+            if (r < 3) t.sb[a][r] = D.sbase[3 * atom + r];
+            else if (r == 3) t.hrow[a] = D.fhead[atom];
+            else if (r == 4) t.nrow[a] = D.fhead[atom + 1] - D.fhead[atom];
+            else if (r == 5) t.self[a] = D.selfw[atom];
+            else if (r == 6) { if (a) t.tj_old = D.type[atom]; else t.ti_old = D.type[atom]; }
+            else if (r == 7) { if (a) t.qj_old = D.q[atom]; else t.qi_old = D.q[atom]; }
+            else { if (a) t.zj = D.z[atom]; else t.zi = D.z[atom]; }
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void stage_pending(const McDev& D, PendT<P>& pd, int k, int atom, double dq, int lt, int nthr) {
+    for (int r = lt; r < 3 * P + 3; r += nthr) {
+        if (r < 3 * P) (&pd.w[k][0][0])[r] = D.sw[(size_t)atom * 3 * P + r];
+        else pd.sb[k][r - 3 * P] = D.sbase[3 * atom + (r - 3 * P)];
+    }
+    if (lt == 0) { pd.atom[k] = atom; pd.dq[k] = dq; }
+}
+
 // ------------------------------------------------------------ chain kernel
-struct TrialLds {
-    MoveInfo mv;
-    int stop, accepted, fwd;
-    int er_pool[2], er_val[2], er_n[2];     // erase value from pool (size before erase)
-    int pb_pool[2], pb_val[2], pb_n[2];     // push value onto pool (size before push)
+struct DecisionLds {
+    int ntr, stop, acc_k;                    // trials in this round, stop flag, accepted trial (-1 none)
+    int er_pool[2], er_val[2], er_n[2];      // pool erase: pool id, value, size before
+    int pb_pool[2], pb_val[2], pb_n[2];      // pool push: pool id, value, size before
     int pos[2];
-    double dsum;
 };
 
-__device__ void pool_update(const McDev& D, TrialLds& t) {
+template <typename PoolT>
+__device__ void pool_update(const McDev& D, PoolT* pbuf, DecisionLds& t) {
     const int tid = threadIdx.x, bd = blockDim.x;
     if (tid < 2) t.pos[tid] = -1;
     __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-        const int* pool = D.pool[t.er_pool[s]];
+    for (int s = 0; s < 2; ++s)
         for (int k = tid; k < t.er_n[s]; k += bd)
-            if (pool[k] == t.er_val[s]) t.pos[s] = k;
-    }
+            if ((int)pbuf[pool_slot(D, t.er_pool[s], k)] == t.er_val[s]) t.pos[s] = k;
     __syncthreads();
-    // shift left by one after the erased slot; chunks of 32 elements per thread
-    constexpr int M = 32;
+    constexpr int M = 8;
     for (int s = 0; s < 2; ++s) {
-        int* pool = D.pool[t.er_pool[s]];
-        const int n = t.er_n[s], pos = t.pos[s];
-        if (pos < 0) continue;  // not found: cannot happen for a consistent state; never shift from -1
+        const int pool = t.er_pool[s], n = t.er_n[s], pos = t.pos[s];
+        if (pos < 0) continue;  // cannot happen for a consistent state; never shift from -1
         for (int base = pos + 1; base < n; base += M * bd) {
-            int v[M];
+            PoolT v[M];
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 const int k = base + tid + m * bd;
-                v[m] = k < n ? pool[k] : 0;
+                v[m] = k < n ? pbuf[pool_slot(D, pool, k)] : PoolT(0);
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 const int k = base + tid + m * bd;
-                if (k < n) pool[k - 1] = v[m];
+                if (k < n) pbuf[pool_slot(D, pool, k - 1)] = v[m];
             }
             __syncthreads();
         }
     }
-    if (tid < 2) D.pool[t.pb_pool[tid]][t.pb_n[tid]] = t.pb_val[tid];
+    if (tid < 2) pbuf[pool_slot(D, t.pb_pool[tid], t.pb_n[tid])] = (PoolT)t.pb_val[tid];
     __syncthreads();
 }
 
+// One persistent workgroup runs the chain.  Each round, wave 0 draws up to W
+// proposals assuming every one is rejected (a rejected trial always draws its
+// Metropolis uniform), all waves evaluate their Delta-E concurrently (16/W
+// waves per trial), then wave 0 decides them in order; at the first accepted
+// trial the later ones are discarded and the engine's stream position rolls
+// back to just after the accepted trial's own draws.  The decisions are
+// therefore exactly those of the sequential chain.
+struct ChainState {            // chain scalars, read and written by wave 0 only inside its sections
+    long long done, acc_n, acc_f, acc_b;
+    int sz[4];
+    int status, nprot, blk, pos, pend_n;
+    double E, Q, Q2, Mz, Qz2;
+};
+
+template <int P, typename PoolT, bool LDS_POOLS>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_chain(McDev D) {
-    __shared__ unsigned s_mt[624];
-    __shared__ TrialLds s_t;
-    __shared__ PairLds s_pl;
-    __shared__ double s_red[16];
-    __shared__ int s_pend_atom[MC_KMAX];
-    __shared__ double s_pend_dq[MC_KMAX];
+    __shared__ RngLds s_rng;
+    __shared__ TrialT<P> s_tr[16];
+    __shared__ PendT<P> s_pd;
+    __shared__ DecisionLds s_dc;
+    __shared__ ChainState s_st;
+    __shared__ double s_wsum[16];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
     McCtl* C = D.ctl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k = tid; k < 624; k += blockDim.x) s_mt[k] = C->mt[k];
-    for (int k = tid; k < MC_KMAX; k += blockDim.x) { s_pend_atom[k] = C->pend_atom[k]; s_pend_dq[k] = C->pend_dq[k]; }
-    // wave-0 state (identical in every lane)
-    int mti = C->mti;
-    int sz[4] = {C->sizes[0], C->sizes[1], C->sizes[2], C->sizes[3]};
-    long long done = C->trials_done;
+    const int W = D.W, wpt = 16 / W;   // waves per trial
     const long long target = C->trials_target, rec_cap = C->rec_cap;
-    long long acc_n = C->accepted, acc_f = C->acc_fwd, acc_b = C->acc_bwd;
-    int pend_n = C->pending_n, status = C->status, nprot = C->nprot;
     const int pend_max = C->pending_max;
-    double E = C->E, Q = C->Q, Q2 = C->Q2, Mz = C->Mz, Qz2 = C->Qz2;
+    PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
+    if (LDS_POOLS)
+        for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
+    for (int k = tid; k < 624; k += blockDim.x) {
+        s_rng.cur[k] = C->mt[k];
+        s_rng.out[0][k] = mt_temper(C->mt[k]);
+    }
+    const int pn0 = C->pending_folded ? 0 : C->pending_n;
+    for (int k = wave; k < pn0; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
+    if (tid == 0) {
+        s_st.done = C->trials_done;
+        s_st.acc_n = C->accepted; s_st.acc_f = C->acc_fwd; s_st.acc_b = C->acc_bwd;
+        for (int k = 0; k < 4; ++k) s_st.sz[k] = C->sizes[k];
+        s_st.status = C->status; s_st.nprot = C->nprot;
+        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = pn0;
+        s_st.E = C->E; s_st.Q = C->Q; s_st.Q2 = C->Q2; s_st.Mz = C->Mz; s_st.Qz2 = C->Qz2;
+    }
     __syncthreads();
-    long long Sb = 0;
     for (;;) {
+        // ---- 1. proposals (wave 0).  Without a Lemire rejection every trial uses a
+        // fixed number of engine outputs, so lane k draws trial k at its own
+        // position; a round that meets a rejection is redrawn serially.
         if (wave == 0) {
-            int stop = (done >= target || pend_n + 2 > pend_max || status != 0) ? 1 : 0;
-            int fwd = 0;
-            if (!stop) {
-                const long long a = sz[0], b = sz[1], c = sz[2], d = sz[3];
-                const long long wf = a * c, wb = b * d;
-                Sb = wf + wb;
-                if (wf > 0 && wb > 0) {
+            int blk = s_st.blk, pos = s_st.pos, status = s_st.status;
+            const long long done = s_st.done;
+            const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
+            int ntr = 0, stop = 0;
+            if (done >= target || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            else {
+                const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
+                if (wf == 0 && wb == 0) { status = 1; stop = 1; }
+                else {
+                    const long long room = target - done;
+                    ntr = (int)(room < W ? room : W);
                     const double sum = (double)(wf + wb);
-                    fwd = draw_canonical(s_mt, mti, lane) < (double)wf / sum ? 1 : 0;
-                } else if (wf > 0) fwd = 1;
-                else if (wb > 0) fwd = 0;
-                else { status = 1; stop = 1; }
-            }
-            if (!stop) {
-                int i, j;
-                if (fwd) {
-                    i = D.pool[0][draw_index(s_mt, mti, lane, (unsigned)sz[0])];
-                    j = D.pool[2][draw_index(s_mt, mti, lane, (unsigned)sz[2])];
-                } else {
-                    i = D.pool[1][draw_index(s_mt, mti, lane, (unsigned)sz[1])];
-                    j = D.pool[3][draw_index(s_mt, mti, lane, (unsigned)sz[3])];
+                    const bool both = wf > 0 && wb > 0;
+                    const int per = (both ? 2 : 0) + 4;
+                    rng_ensure(s_rng, blk, pos + ntr * per, lane);
+                    bool ok = true;
+                    int fwd = 0, ii = 0, jj = 0, pp = 0;
+                    double u = 0.0;
+                    if (lane < ntr) {
+                        int q = pos + lane * per;
+                        if (both) { fwd = canonical_at(s_rng, q) < (double)wf / sum ? 1 : 0; q += 2; }
+                        else fwd = wf > 0 ? 1 : 0;
+                        unsigned a1, a2;
+                        ok = index_at(s_rng, q, (unsigned)(fwd ? sz0 : sz1), a1);
+                        ok = index_at(s_rng, q + 1, (unsigned)(fwd ? sz2 : sz3), a2) && ok;
+                        ii = (int)pbuf[pool_slot(D, fwd ? 0 : 1, (int)a1)];
+                        jj = (int)pbuf[pool_slot(D, fwd ? 2 : 3, (int)a2)];
+                        pp = q + 2;
+                        u = canonical_at(s_rng, pp);
+                    }
+                    if (__all(ok)) {
+                        if (lane < ntr) {
+                            s_tr[lane].pos_prop = pp;
+                            s_tr[lane].u = u;
+                            stage_trial<P>(D, s_tr[lane], ii, jj, fwd);
+                        }
+                        pos += ntr * per;
+                    } else {
+                        for (int k = 0; k < ntr; ++k) {
+                            int f2;
+                            if (both) f2 = draw_canonical(s_rng, blk, pos, lane) < (double)wf / sum ? 1 : 0;
+                            else f2 = wf > 0 ? 1 : 0;
+                            int i2, j2;
+                            if (f2) {
+                                i2 = (int)pbuf[pool_slot(D, 0, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz0))];
+                                j2 = (int)pbuf[pool_slot(D, 2, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz2))];
+                            } else {
+                                i2 = (int)pbuf[pool_slot(D, 1, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz1))];
+                                j2 = (int)pbuf[pool_slot(D, 3, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz3))];
+                            }
+                            const int p2 = pos;
+                            const double u2 = draw_canonical(s_rng, blk, pos, lane);
+                            if (lane == 0) {
+                                s_tr[k].pos_prop = p2;
+                                s_tr[k].u = u2;
+                                stage_trial<P>(D, s_tr[k], i2, j2, f2);
+                            }
+                        }
+                    }
                 }
-                if (lane == 0) {
-                    MoveInfo& mv = s_t.mv;
-                    mv.i = i; mv.j = j;
-                    mv.ti_old = D.type[i]; mv.tj_old = D.type[j];
-                    mv.qi_old = D.q[i]; mv.qj_old = D.q[j];
-                    mv.ti_new = fwd ? D.t_nhp : D.t_nhn; mv.qi_new = fwd ? 1.0 : 0.0;
-                    mv.tj_new = fwd ? D.t_cl : D.t_w; mv.qj_new = fwd ? -1.0 : 0.0;
-                    s_t.fwd = fwd;
-                }
             }
-            if (lane == 0) s_t.stop = stop;
+            if (lane == 0) {
+                s_dc.ntr = ntr; s_dc.stop = stop;
+                s_st.blk = blk; s_st.pos = pos; s_st.status = status;
+            }
         }
         __syncthreads();
-        if (s_t.stop) break;
-        const double dsum = block_delta_e(D, s_t.mv, s_pend_atom, s_pend_dq, pend_n, s_pl, s_red);
-        if (tid == 0) s_t.dsum = dsum;
+        if (s_dc.stop) break;
+        const int ntr = s_dc.ntr, pend_n = s_st.pend_n;
+        // ---- 2. stage per-trial data, then evaluate Delta-E (wpt waves per trial)
+        const int k = wave / wpt, lt = (wave - k * wpt) * 64 + lane, nt = wpt * 64;
+        if (k < ntr) stage_trial_data<P>(D, s_tr[k], lt, nt);
         __syncthreads();
+        double part = 0.0;
+        if (k < ntr) part = trial_partial<P>(D, s_tr[k], s_pd, pend_n, lt, nt);
+        part = wave_sum(part);
+        if (lane == 0) s_wsum[wave] = part;
+        __syncthreads();
+        // ---- 3. decisions (wave 0): lane k evaluates trial k against the round's
+        // starting state; the first accepted trial ends the round.
         if (wave == 0) {
-            const MoveInfo mv = s_t.mv;
-            const int fwd = s_t.fwd;
-            const double dE = s_t.dsum + correction_delta(D, mv, Q, Q2, Mz, Qz2);
-            const long long a = sz[0], b = sz[1], c = sz[2], d = sz[3];
-            const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
-            const double expo = mh_exponent(D.beta, dE, fwd ? 1 : -1, D.mu, Sb, Sa);
-            bool acc = expo >= 0.0;
-            double u = __longlong_as_double(0x7FF8000000000000LL);
-            if (!acc) {
-                u = draw_canonical(s_mt, mti, lane);
-                acc = exp(expo) > u;
+            const long long done = s_st.done;
+            const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
+            const long long Sb = a * c + b * d;
+            const double Q = s_st.Q, Q2 = s_st.Q2, Mz = s_st.Mz, Qz2 = s_st.Qz2;
+            const int kk = lane;
+            bool acc = false;
+            double dE = 0.0, expo = 0.0, u = __longlong_as_double(0x7FF8000000000000LL);
+            if (kk < ntr) {
+                const TrialT<P>& t = s_tr[kk];
+                double dsum = 0.0;
+                for (int w = kk * wpt; w < (kk + 1) * wpt; ++w) dsum += s_wsum[w];
+                dE = dsum + correction_delta<P>(D, t, Q, Q2, Mz, Qz2);
+                const int fwd = t.fwd;
+                const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
+                expo = mh_exponent(D.beta, dE, fwd ? 1 : -1, D.mu, Sb, Sa);
+                acc = expo >= 0.0;
+                if (!acc) { u = t.u; acc = exp(expo) > u; }
             }
-            if (lane == 0 && done < rec_cap) {
-                D.rec_fwd[done] = (signed char)fwd; D.rec_i[done] = mv.i; D.rec_j[done] = mv.j;
-                D.rec_acc[done] = acc ? 1 : 0; D.rec_dE[done] = dE; D.rec_u[done] = u;
+            const unsigned long long m = __ballot(acc);
+            const int first = m ? __ffsll((long long)m) - 1 : ntr;
+            const int nproc = first < ntr ? first + 1 : ntr;
+            if (kk < nproc && done + kk < rec_cap) {
+                const TrialT<P>& t = s_tr[kk];
+                D.rec_fwd[done + kk] = (signed char)t.fwd; D.rec_i[done + kk] = t.i; D.rec_j[done + kk] = t.j;
+                D.rec_acc[done + kk] = acc ? 1 : 0; D.rec_dE[done + kk] = dE; D.rec_u[done + kk] = u;
             }
-            ++done;
-            if (acc) {
-                const double dqi = mv.qi_new - mv.qi_old, dqj = mv.qj_new - mv.qj_old;
-                const double zi = D.z[mv.i], zj = D.z[mv.j];
-                E += dE;
-                Q2 += (mv.qi_new * mv.qi_new - mv.qi_old * mv.qi_old) + (mv.qj_new * mv.qj_new - mv.qj_old * mv.qj_old);
-                Q += dqi + dqj;
-                Mz += dqi * zi + dqj * zj;
-                Qz2 += dqi * zi * zi + dqj * zj * zj;
-                ++acc_n;
-                if (fwd) { ++acc_f; ++nprot; } else { ++acc_b; --nprot; }
-                if (lane == 0) {
-                    D.q[mv.i] = mv.qi_new; D.type[mv.i] = mv.ti_new;
-                    D.q[mv.j] = mv.qj_new; D.type[mv.j] = mv.tj_new;
-                    s_pend_atom[pend_n] = mv.i; s_pend_dq[pend_n] = dqi;
-                    s_pend_atom[pend_n + 1] = mv.j; s_pend_dq[pend_n + 1] = dqj;
-                    // fwd: nh_n -> nh_p, w -> cl; bwd: nh_p -> nh_n, cl -> w
-                    const int ps = fwd ? 0 : 1, pd = fwd ? 1 : 0, is = fwd ? 2 : 3, id = fwd ? 3 : 2;
-                    s_t.er_pool[0] = ps; s_t.er_val[0] = mv.i; s_t.er_n[0] = sz[ps];
-                    s_t.er_pool[1] = is; s_t.er_val[1] = mv.j; s_t.er_n[1] = sz[is];
-                    s_t.pb_pool[0] = pd; s_t.pb_val[0] = mv.i; s_t.pb_n[0] = sz[pd];
-                    s_t.pb_pool[1] = id; s_t.pb_val[1] = mv.j; s_t.pb_n[1] = sz[id];
-                }
-                pend_n += 2;
-                if (fwd) { --sz[0]; ++sz[1]; --sz[2]; ++sz[3]; }
-                else { ++sz[0]; --sz[1]; ++sz[2]; --sz[3]; }
+            if (first < ntr && kk == first) {
+                // accepted: state update, roll the stream back behind this trial's own draws
+                const TrialT<P>& t = s_tr[kk];
+                const int fwd = t.fwd;
+                const int pos = t.pos_prop + (expo >= 0.0 ? 0 : 2);
+                if (pos < s_st.blk * 624 - 624) s_st.status = 2;   // rollback beyond the kept block
+                s_st.pos = pos;
+                const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+                s_st.E += dE;
+                s_st.Q2 = Q2 + ((t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old));
+                s_st.Q = Q + (dqi + dqj);
+                s_st.Mz = Mz + (dqi * t.zi + dqj * t.zj);
+                s_st.Qz2 = Qz2 + (dqi * t.zi * t.zi + dqj * t.zj * t.zj);
+                ++s_st.acc_n;
+                if (fwd) { ++s_st.acc_f; ++s_st.nprot; } else { ++s_st.acc_b; --s_st.nprot; }
+                D.q[t.i] = t.qi_new; D.type[t.i] = t.ti_new;
+                D.q[t.j] = t.qj_new; D.type[t.j] = t.tj_new;
+                // fwd: nh_n -> nh_p, w -> cl; bwd: nh_p -> nh_n, cl -> w
+                const int ps = fwd ? 0 : 1, pdd = fwd ? 1 : 0, is = fwd ? 2 : 3, id = fwd ? 3 : 2;
+                s_dc.er_pool[0] = ps; s_dc.er_val[0] = t.i; s_dc.er_n[0] = s_st.sz[ps];
+                s_dc.er_pool[1] = is; s_dc.er_val[1] = t.j; s_dc.er_n[1] = s_st.sz[is];
+                s_dc.pb_pool[0] = pdd; s_dc.pb_val[0] = t.i; s_dc.pb_n[0] = s_st.sz[pdd];
+                s_dc.pb_pool[1] = id; s_dc.pb_val[1] = t.j; s_dc.pb_n[1] = s_st.sz[id];
+                if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
+                else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
             }
-            if (lane == 0) s_t.accepted = acc ? 1 : 0;
+            if (lane == 0) { s_dc.acc_k = first < ntr ? first : -1; s_st.done = done + nproc; }
         }
         __syncthreads();
-        if (s_t.accepted) pool_update(D, s_t);
+        // ---- 4. accepted move: pending list and pools (all threads)
+        const int acc_k = s_dc.acc_k;
+        if (acc_k >= 0) {
+            const TrialT<P>& t = s_tr[acc_k];
+            if (wave == 1) stage_pending<P>(D, s_pd, pend_n, t.i, t.qi_new - t.qi_old, lane, 64);
+            if (wave == 2) stage_pending<P>(D, s_pd, pend_n + 1, t.j, t.qj_new - t.qj_old, lane, 64);
+            pool_update<PoolT>(D, pbuf, s_dc);
+            if (tid == 0) s_st.pend_n = pend_n + 2;
+            __syncthreads();
+        }
     }
     // write back
-    for (int k = tid; k < 624; k += blockDim.x) C->mt[k] = s_mt[k];
-    for (int k = tid; k < MC_KMAX; k += blockDim.x) { C->pend_atom[k] = s_pend_atom[k]; C->pend_dq[k] = s_pend_dq[k]; }
-    if (tid == 0) {
-        C->mti = mti;
-        for (int k = 0; k < 4; ++k) C->sizes[k] = sz[k];
-        C->trials_done = done;
-        C->accepted = acc_n; C->acc_fwd = acc_f; C->acc_bwd = acc_b;
-        C->pending_n = pend_n; C->status = status; C->nprot = nprot;
-        C->E = E; C->Q = Q; C->Q2 = Q2; C->Mz = Mz; C->Qz2 = Qz2;
+    if (LDS_POOLS)
+        for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) D.pbuf[k] = (int)pbuf[k];
+    const int pend_n = s_st.pend_n;
+    if (wave == 0) {
+        const int blk = s_st.blk, pos = s_st.pos;
+        const bool in_cur = pos >= blk * 624;
+        const unsigned* st = in_cur ? s_rng.cur : s_rng.prev;
+        for (int k = lane; k < 624; k += 64) C->mt[k] = st[k];
+        if (lane == 0) {
+            C->mti = in_cur ? pos - blk * 624 : pos - (blk - 1) * 624;
+            for (int k = 0; k < 4; ++k) C->sizes[k] = s_st.sz[k];
+            C->trials_done = s_st.done;
+            C->accepted = s_st.acc_n; C->acc_fwd = s_st.acc_f; C->acc_bwd = s_st.acc_b;
+            C->pending_n = pend_n; C->pending_folded = 0; C->status = s_st.status; C->nprot = s_st.nprot;
+            C->E = s_st.E; C->Q = s_st.Q; C->Q2 = s_st.Q2; C->Mz = s_st.Mz; C->Qz2 = s_st.Qz2;
+        }
     }
+    for (int k = tid; k < pend_n; k += blockDim.x) { C->pend_atom[k] = s_pd.atom[k]; C->pend_dq[k] = s_pd.dq[k]; }
 }
 
 // Delta-E of candidate moves, one block per move, no acceptance.
+template <int P>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_probe(McDev D, int n, const int* __restrict__ mi,
                                                          const int* __restrict__ mj, const int* __restrict__ mf,
                                                          double* __restrict__ out) {
-    __shared__ MoveInfo s_mv;
-    __shared__ PairLds s_pl;
-    __shared__ double s_red[16];
-    __shared__ int s_pend_atom[MC_KMAX];
-    __shared__ double s_pend_dq[MC_KMAX];
+    __shared__ TrialT<P> s_t;
+    __shared__ PendT<P> s_pd;
+    __shared__ double s_wsum[16];
     const McCtl* C = D.ctl;
-    const int m = blockIdx.x;
+    const int m = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (m >= n) return;
-    for (int k = threadIdx.x; k < MC_KMAX; k += blockDim.x) { s_pend_atom[k] = C->pend_atom[k]; s_pend_dq[k] = C->pend_dq[k]; }
-    if (threadIdx.x == 0) {
-        const int i = mi[m], j = mj[m], f = mf[m];
-        s_mv.i = i; s_mv.j = j;
-        s_mv.ti_old = D.type[i]; s_mv.tj_old = D.type[j];
-        s_mv.qi_old = D.q[i]; s_mv.qj_old = D.q[j];
-        s_mv.ti_new = f ? D.t_nhp : D.t_nhn; s_mv.qi_new = f ? 1.0 : 0.0;
-        s_mv.tj_new = f ? D.t_cl : D.t_w; s_mv.qj_new = f ? -1.0 : 0.0;
-    }
+    const int pend_n = C->pending_folded ? 0 : C->pending_n;
+    for (int k = wave; k < pend_n; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
+    if (tid == 0) stage_trial<P>(D, s_t, mi[m], mj[m], mf[m]);
     __syncthreads();
-    const double dsum = block_delta_e(D, s_mv, s_pend_atom, s_pend_dq, C->pending_n, s_pl, s_red);
-    if (threadIdx.x == 0) out[m] = dsum + correction_delta(D, s_mv, C->Q, C->Q2, C->Mz, C->Qz2);
+    stage_trial_data<P>(D, s_t, tid, blockDim.x);
+    __syncthreads();
+    double part = wave_sum(trial_partial<P>(D, s_t, s_pd, pend_n, tid, blockDim.x));
+    if (lane == 0) s_wsum[wave] = part;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += s_wsum[w];
+        out[m] = s + correction_delta<P>(D, s_t, C->Q, C->Q2, C->Mz, C->Qz2);
+    }
 }
 
 // ------------------------------------------------------------ refresh
-__global__ void k_scatter_pending(McDev D, double* __restrict__ grid, int clear) {
-    const McCtl* C = D.ctl;
-    const int np = C->pending_n, P = D.P, P3 = P * P * P;
-    for (int it = threadIdx.x; it < np * P3; it += blockDim.x) {
-        const int k = it / P3, r = it - k * P3, wz = r / (P * P), wy = (r / P) % P, wx = r % P;
-        const int a = C->pend_atom[k];
-        const int* sb = D.sbase + 3 * a;
-        const double* w = D.sw + (size_t)a * 3 * P;
-        const size_t g = ((size_t)wrap(sb[2] + wz, D.nz) * D.ny + wrap(sb[1] + wy, D.ny)) * D.nx + wrap(sb[0] + wx, D.nx);
-        if (clear) grid[g] = 0.0;
-        else unsafeAtomicAdd(grid + g, C->pend_dq[k] * w[wx] * w[P + wy] * w[2 * P + wz]);
+// Folds the pending charge changes into phi without a charge grid: their
+// structure factor is separable per atom,
+//   dQ^(k) = sum_p dq_p Fx_p(kx) Fy_p(ky) Fz_p(kz),  F(k) = sum_u w[u] e^{-2 pi i k (s+u)/n},
+// so each z-pencil tile builds eta * dQ^ in LDS and runs the inverse z pass;
+// k_fft_xy_c2r then accumulates the inverse plane pass into phi.
+template <int P>
+__global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* __restrict__ kg, const cplx<double>* __restrict__ twx,
+                                 const cplx<double>* __restrict__ twy, const cplx<double>* __restrict__ twz,
+                                 const double* __restrict__ eta, FftPlan pz, int tx) {
+    McCtl* C = D.ctl;
+    const int np = C->pending_folded ? 0 : C->pending_n;
+    if (np == 0) return;
+    __shared__ int s_sb[MC_KMAX][3];
+    __shared__ double s_w[MC_KMAX][3][P];
+    __shared__ double s_dq[MC_KMAX];
+    __shared__ cplx<double> s_fxy[MC_KMAX][16];     // dq_p * Fx_p(kx0 + c) * Fy_p(ky), c < tx <= 16
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nx = D.nx, ny = D.ny, nz = D.nz, nkx = nx / 2 + 1, hs = nz + 1;
+    cplx<double>* stw = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* Cc = stw + nz;
+    const int ntile = (nkx + tx - 1) / tx;
+    const int ky = blockIdx.x / ntile, kx0 = (blockIdx.x - ky * ntile) * tx, cnt = min(tx, nkx - kx0);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    if (blockIdx.x == 0 && tid == 0) C->pending_folded = 1;
+    for (int k = tid; k < nz; k += bd) stw[k] = twz[k];
+    for (int r = tid; r < np * (3 * P + 4); r += bd) {
+        const int p = r / (3 * P + 4), q = r - p * (3 * P + 4), atom = C->pend_atom[p];
+        if (q < 3 * P) (&s_w[p][0][0])[q] = D.sw[(size_t)atom * 3 * P + q];
+        else if (q < 3 * P + 3) s_sb[p][q - 3 * P] = D.sbase[3 * atom + (q - 3 * P)];
+        else s_dq[p] = C->pend_dq[p];
+    }
+    __syncthreads();
+    for (int r = tid; r < np * cnt; r += bd) {
+        const int p = r / cnt, c = r - p * cnt, kx = kx0 + c;
+        cplx<double> fx = {0.0, 0.0}, fy = {0.0, 0.0};
+        int ix = (kx * s_sb[p][0]) % nx, iy = (ky * s_sb[p][1]) % ny;   // k*(s+u) mod n, stepped by k
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            fx = cadd(fx, cscale(twx[ix], s_w[p][0][u]));
+            fy = cadd(fy, cscale(twy[iy], s_w[p][1][u]));
+            ix += kx; ix -= ix >= nx ? nx : 0;
+            iy += ky; iy -= iy >= ny ? ny : 0;
+        }
+        s_fxy[p][c] = cscale(cmul(fx, fy), s_dq[p]);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < cnt * nz; idx += bd) {
+        const int c = idx / nz, kz = idx - c * nz, kx = kx0 + c;
+        cplx<double> v = {0.0, 0.0};
+        for (int p = 0; p < np; ++p) {
+            cplx<double> fz = {0.0, 0.0};
+            int iz = (kz * s_sb[p][2]) % nz;
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                fz = cadd(fz, cscale(stw[iz], s_w[p][2][u]));
+                iz += kz; iz -= iz >= nz ? nz : 0;
+            }
+            v = cadd(v, cmul(s_fxy[p][c], fz));
+        }
+        const double et = eta[((size_t)kz * ny + ky) * nkx + kx];
+        Cc[c * hs + kz] = {v.x * et, v.y * et};
+    }
+    __syncthreads();
+    lds_fft(Cc, cnt, hs, pz, stw, +1);
+    for (int idx = tid; idx < nz * cnt; idx += bd) {
+        const int kz = idx / cnt, c = idx - kz * cnt;
+        kg[((size_t)kz * ny + ky) * nkx + kx0 + c] = Cc[c * hs + kz];
     }
 }
 
@@ -551,6 +828,7 @@ __global__ void k_finish_recompute(McDev D, const double* __restrict__ res, doub
     C->E = E; C->E_recomputed = E;
     C->Q = Q; C->Q2 = Q2; C->Mz = Mz; C->Qz2 = Qz2;
     C->pending_n = 0;
+    C->pending_folded = 0;
 }
 
 // ------------------------------------------------------------ setup kernels
@@ -645,6 +923,15 @@ __global__ void k_self_terms(int N, int P, int nx, int ny, int nz, const double*
     selfw[i] = s;
 }
 
+__global__ void k_fold_g(int nx, int ny, int nz, const double* __restrict__ gk, double* __restrict__ gf) {
+    const int fnx = nx / 2 + 1, fny = ny / 2 + 1, fnz = nz / 2 + 1;
+    const long long tot = (long long)fnx * fny * fnz;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < tot; k += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(k % fnx), y = (int)((k / fnx) % fny), z = (int)(k / ((long long)fnx * fny));
+        gf[k] = gk[((size_t)z * ny + y) * nx + x];
+    }
+}
+
 __global__ void k_delta_grid(double* __restrict__ g) {
     if (threadIdx.x == 0 && blockIdx.x == 0) g[0] = 1.0;
 }
@@ -663,8 +950,10 @@ struct phb_mc_s {
     phb_mc_desc d{};
     int nsites = 0, nions = 0;
     DevBuf<int> fhead, fnbr, sbase;
-    DevBuf<double> kreal, r2, sw, phi, gk, dgrid, selfw;
-    DevBuf<int> pools[4];
+    DevBuf<double> kreal, r2, sw, phi, gk, gf, dgrid, selfw;
+    DevBuf<int> pbuf;
+    bool lds_pools = false;
+    size_t pool_lds_bytes = 0;
     DevBuf<McCtl> ctl;
     DevBuf<signed char> rec_fwd, rec_acc;
     DevBuf<int> rec_i, rec_j;
@@ -740,16 +1029,64 @@ void full_recompute(phb_mc_s* M) {
     ++M->recomputes;
 }
 
-void refresh(phb_mc_s* M) {
-    phb_pme_s* P = M->pme;
-    const int* gate = &M->ctl.p->pending_n;
-    hipLaunchKernelGGL(k_scatter_pending, dim3(1), dim3(1024), 0, stream(), M->dev, M->dgrid.p, 0);
-    PmeKernels::launch_fft_buf(P, M->dgrid.p, M->phi.p, 2, false, false, gate);
-    hipLaunchKernelGGL(k_scatter_pending, dim3(1), dim3(1024), 0, stream(), M->dev, M->dgrid.p, 1);
-    PmeKernels::launch_c2r_buf(P, M->phi.p, true, gate);
-    hipLaunchKernelGGL(k_reset_pending, dim3(1), dim3(64), 0, stream(), M->ctl.p);
+template <int P>
+void refresh_p(phb_mc_s* M) {
+    phb_pme_s* Pm = M->pme;
+    require(Pm->z_tx <= 16, "refresh: z tile wider than 16", PHB_ERR_INTERNAL);
+    hipLaunchKernelGGL(k_refresh_kspace<P>, dim3(Pm->z_blocks), dim3(Pm->z_threads), Pm->z_lds(sizeof(cplx<double>)),
+                       stream(), M->dev, Pm->kg64.p, Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->eta64.p, Pm->pz,
+                       Pm->z_tx);
+    PmeKernels::launch_c2r_buf(Pm, M->phi.p, true, &M->ctl.p->pending_n);
     PHB_HIP(hipGetLastError());
     ++M->refreshes;
+}
+
+void refresh(phb_mc_s* M) {
+    switch (M->dev.P) {
+        case 3: refresh_p<3>(M); break;
+        case 4: refresh_p<4>(M); break;
+        case 5: refresh_p<5>(M); break;
+        case 6: refresh_p<6>(M); break;
+        case 7: refresh_p<7>(M); break;
+        default: refresh_p<8>(M); break;
+    }
+}
+
+template <int P>
+void launch_chain_p(phb_mc_s* M) {
+    if (M->lds_pools) {
+        auto kern = k_mc_chain<P, unsigned short, true>;
+        PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)M->pool_lds_bytes));
+        hipLaunchKernelGGL(kern, dim3(1), dim3(MC_THREADS), M->pool_lds_bytes, stream(), M->dev);
+    } else {
+        hipLaunchKernelGGL((k_mc_chain<P, int, false>), dim3(1), dim3(MC_THREADS), 0, stream(), M->dev);
+    }
+}
+
+void launch_chain(phb_mc_s* M) {
+    switch (M->dev.P) {
+        case 3: launch_chain_p<3>(M); break;
+        case 4: launch_chain_p<4>(M); break;
+        case 5: launch_chain_p<5>(M); break;
+        case 6: launch_chain_p<6>(M); break;
+        case 7: launch_chain_p<7>(M); break;
+        default: launch_chain_p<8>(M); break;
+    }
+}
+
+void launch_probe(phb_mc_s* M, int n, const int* i, const int* j, const int* f, double* out) {
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(n), dim3(MC_THREADS), 0, stream(), M->dev, n, i, j, f, out);
+    };
+    switch (M->dev.P) {
+        case 3: go(k_mc_probe<3>); break;
+        case 4: go(k_mc_probe<4>); break;
+        case 5: go(k_mc_probe<5>); break;
+        case 6: go(k_mc_probe<6>); break;
+        case 7: go(k_mc_probe<7>); break;
+        default: go(k_mc_probe<8>); break;
+    }
 }
 
 void fill_stats(phb_mc_s* M, const McCtl& c, phb_mc_stats* s) {
@@ -790,6 +1127,11 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         const int pm = d->pending_max > 0 ? d->pending_max : 32;
         require(pm >= 2 && pm <= MC_KMAX, "phb_mc_create: pending_max must be in [2, 64]");
         require(d->record_cap >= 0, "phb_mc_create: record_cap must be >= 0");
+        const int sw_ = d->spec_window > 0 ? d->spec_window : 16;
+        require(sw_ == 1 || sw_ == 2 || sw_ == 4 || sw_ == 8 || sw_ == 16,
+                "phb_mc_create: spec_window must be 1, 2, 4, 8 or 16");
+        require(std::min(pme->nx, std::min(pme->ny, pme->nz)) >= 2 * pme->order - 1,
+                "phb_mc_create: every grid dimension must be >= 2*order-1", PHB_ERR_UNSUPPORTED);
         auto M = std::make_unique<phb_mc_s>();
         M->atoms = a; M->params = p; M->neigh = neigh; M->pme = pme; M->box = *box; M->d = *d;
         M->d.pending_max = pm;
@@ -815,11 +1157,18 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         for (int j = 0; j < N; ++j) if (ht[j] + 1 == d->cl_type && hz[j] < z_cut) cl.push_back(j);
         M->nsites = (int)(nh_n.size() + nh_p.size());
         M->nions = (int)(w.size() + cl.size());
-        const std::vector<int>* pv[4] = {&nh_n, &nh_p, &w, &cl};
-        for (int k = 0; k < 4; ++k) {
-            const int cap = k < 2 ? M->nsites : M->nions;
-            M->pools[k].alloc(std::max(cap, 1));
-            M->pools[k].upload(pv[k]->data(), pv[k]->size());
+        {
+            // shared layout: nh_n up from 0, nh_p down from nsites-1, w up from nsites, cl down from the end
+            const int ns = M->nsites, ni = M->nions;
+            std::vector<int> h(std::max(ns + ni, 1), -1);
+            for (size_t k = 0; k < nh_n.size(); ++k) h[k] = nh_n[k];
+            for (size_t k = 0; k < nh_p.size(); ++k) h[ns - 1 - k] = nh_p[k];
+            for (size_t k = 0; k < w.size(); ++k) h[ns + k] = w[k];
+            for (size_t k = 0; k < cl.size(); ++k) h[ns + ni - 1 - k] = cl[k];
+            M->pbuf.alloc(h.size());
+            M->pbuf.upload(h.data(), h.size());
+            M->pool_lds_bytes = ((size_t)(ns + ni) * sizeof(unsigned short) + 15) / 16 * 16;
+            M->lds_pools = N <= 65535 && M->pool_lds_bytes <= 96 * 1024;
         }
         // ---- full rows + coefficients
         hipStream_t st = stream();
@@ -855,7 +1204,12 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         PHB_HIP(hipMemsetAsync(M->dgrid.p, 0, G * sizeof(double), st));
         hipLaunchKernelGGL(k_delta_grid, dim3(1), dim3(64), 0, st, M->dgrid.p);
         PmeKernels::launch_fft_buf(pme, M->dgrid.p, M->gk.p, 2, true, false);
-        PHB_HIP(hipMemsetAsync(M->dgrid.p, 0, G * sizeof(double), st));
+        sync();
+        M->dgrid.release();
+        const size_t GF = (size_t)(pme->nx / 2 + 1) * (pme->ny / 2 + 1) * (pme->nz / 2 + 1);
+        M->gf.alloc(GF);
+        hipLaunchKernelGGL(k_fold_g, dim3(std::min<size_t>((GF + 255) / 256, 4096)), dim3(256), 0, st, pme->nx, pme->ny,
+                           pme->nz, M->gk.p, M->gf.p);
         M->selfw.alloc(N);
         hipLaunchKernelGGL(k_self_terms, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, P, pme->nx, pme->ny, pme->nz,
                            M->sw.p, M->gk.p, M->selfw.p);
@@ -885,10 +1239,12 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         D.N = N; D.z = a->z.p; D.q = a->q.p; D.type = a->type.p;
         D.fhead = M->fhead.p; D.fnbr = M->fnbr.p; D.kreal = M->kreal.p; D.r2 = M->r2.p;
         D.sbase = M->sbase.p; D.sw = M->sw.p; D.phi = M->phi.p; D.gk = M->gk.p; D.selfw = M->selfw.p;
+        D.gf = M->gf.p; D.fnx = pme->nx / 2 + 1; D.fny = pme->ny / 2 + 1;
         D.nx = pme->nx; D.ny = pme->ny; D.nz = pme->nz; D.P = P;
         D.T = T; D.eps = p->d_eps.p; D.sig = p->d_sig.p; D.shift = p->d_shift.p;
         D.use_shift = p->d.lj_shift ? 1 : 0; D.use_cut = p->lj_cut_active ? 1 : 0; D.lj_rc2 = p->lj_rc * p->lj_rc;
-        for (int k = 0; k < 4; ++k) D.pool[k] = M->pools[k].p;
+        D.pbuf = M->pbuf.p; D.nsites = M->nsites; D.nions = M->nions;
+        D.W = d->spec_window > 0 ? d->spec_window : 16;
         D.t_nhn = d->nh_n_type - 1; D.t_nhp = d->nh_p_type - 1; D.t_w = d->w_type - 1; D.t_cl = d->cl_type - 1;
         D.beta = d->beta; D.mu = d->mu_eff; D.elF = pme->elF; D.alpha = pme->alpha; D.vol = pme->vol;
         D.Lz = box->Lz; D.use3dc = pme->use3dc ? 1 : 0;
@@ -927,7 +1283,7 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
                 for (int b = 0; b < batch; ++b) {
                     const size_t e0 = M->ev_used;
                     if (M->profiling) M->event();
-                    hipLaunchKernelGGL(k_mc_chain, dim3(1), dim3(MC_THREADS), 0, stream(), M->dev);
+                    launch_chain(M);
                     ++M->launches;
                     if (M->profiling) M->event();
                     refresh(M);
@@ -937,6 +1293,7 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
                 c = read_ctl(M);
                 require(c.trials_done > done || c.status != 0 || c.trials_done >= chunk_end,
                         "phb_mc_run: the chain made no progress", PHB_ERR_INTERNAL);
+                require(c.status != 2, "phb_mc_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
                 done = c.trials_done;
                 status = c.status;
                 batch = std::min(batch * 2, 64);
@@ -987,7 +1344,7 @@ extern "C" int phb_mc_delta_e(phb_mc M, int n, const int* i, const int* j, const
         DevBuf<int> di(n), dj(n), df(n);
         DevBuf<double> dout(n);
         di.upload(i, n); dj.upload(j, n); df.upload(fwd, n);
-        hipLaunchKernelGGL(k_mc_probe, dim3(n), dim3(MC_THREADS), 0, stream(), M->dev, n, di.p, dj.p, df.p, dout.p);
+        launch_probe(M, n, di.p, dj.p, df.p, dout.p);
         PHB_HIP(hipGetLastError());
         dout.download(dE, n);
         sync();

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, co
 
 // ------------------------------------------------------------ FFT kernels
 template <typename T>
-__global__ void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ kg, int nx, int ny, FftPlan px,
+__global__ __launch_bounds__(512) void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ kg, int nx, int ny, FftPlan px,
                              FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
                              const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
@@ -177,7 +177,7 @@ __global__ void k_fft_xy_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ k
 // mode bit 2: multiply by eta, inverse z transform, store back;
 // mode bit 4: store the forward transform (no eta) back.
 template <typename T>
-__global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPlan pz,
+__global__ __launch_bounds__(512) void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPlan pz,
                         const cplx<T>* __restrict__ twz, const T* __restrict__ eta, int tx, int mode,
                         double* __restrict__ partials, const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
@@ -226,7 +226,7 @@ __global__ void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPla
 }
 
 template <typename T>
-__global__ void k_fft_xy_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny, FftPlan px,
+__global__ __launch_bounds__(512) void k_fft_xy_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny, FftPlan px,
                              FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
                              int accumulate, const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
@@ -302,8 +302,8 @@ double alpha_from_rtol(double rc, double rtol) {  // MC/src/pme.cu:54-63
 int fft_threads(long long elems) {
     long long t = (elems + 14) / 15;
     int b = 256;
-    while (b < t && b < 1024) b *= 2;
-    require(t <= 1024, "FFT plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
+    while (b < t && b < 512) b *= 2;
+    require(t <= 512, "FFT plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
     return b;
 }
 
