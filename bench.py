@@ -41,12 +41,13 @@ def chain_bytes_per_trial(rep) -> float:
 
 
 def refresh_bytes(rep) -> float:
-    """Algorithmic HBM bytes of one potential refresh (FP64): forward plane pass
-    (read G reals, write Gk complex), z pass (read+write Gk complex, read Gk
-    influence values), inverse plane pass (read Gk complex, read+write G reals)."""
+    """Algorithmic HBM bytes of one potential refresh (FP64): the k-space pass
+    writes the Gk-point complex half spectrum and reads the Gk influence values;
+    the inverse plane pass reads the half spectrum and reads + writes the G-point
+    potential grid."""
     nx, ny, nz = rep.p.grid
     G, Gk = nx * ny * nz, (nx // 2 + 1) * ny * nz
-    return 8 * G + 16 * Gk + (16 * Gk * 2 + 8 * Gk) + (16 * Gk + 16 * G)
+    return (16 * Gk + 8 * Gk) + (16 * Gk + 16 * G)
 
 
 def cpu_baseline(cfg, trials: int, threads: int):
@@ -71,8 +72,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--trials-per-step", type=int, default=20000)
-    ap.add_argument("--pending-max", type=int, default=32)
-    ap.add_argument("--spec-window", type=int, default=16)
+    ap.add_argument("--pending-max", type=int, default=8)
+    ap.add_argument("--spec-window", type=int, default=8)
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

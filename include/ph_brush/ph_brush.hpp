@@ -1,0 +1,279 @@
+// ph_brush.hpp -- header-only C++ layer restoring the reference's host API
+// (MC/include/types.hpp, device_data.hpp, neighbor.hpp, energy.hpp, pme.hpp,
+// mc.hpp) on top of the C ABI in phb.h.  Same names, argument meaning and
+// error behaviour: functions return by value after a device sync and throw
+// std::runtime_error carrying the library's message (the reference's CUDA
+// checks throw the same type, e.g. MC/src/energy_gpu.cu:7-10).
+//
+// Differences a caller sees:
+//   - device objects own opaque handles instead of thrust vectors;
+//   - `real` is double (the FP64 plain path); the FP32 form is selected per
+//     PME plan at run time;
+//   - the Monte Carlo entry point advances a fixed-mu chain on the device
+//     (phase_gc_fixed_mu) instead of the host loop of phase_gc_mu_control.
+#pragma once
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../phb.h"
+
+namespace ph_brush {
+
+using real = double;
+
+inline void check(int rc) {
+    if (rc != PHB_OK) throw std::runtime_error(std::string("ph_brush: ") + phb_last_error());
+}
+
+// ---------------------------------------------------------------- host types
+struct Box { real Lx{0}, Ly{0}, Lz{0}; };                                     // MC/include/types.hpp:6-8
+struct Atom { int id{0}, mol{0}, type{0}; real q{0}, x{0}, y{0}, z{0}; };      // :10-16 (type 1-based)
+struct Bond { int id{0}, type{0}, i{0}, j{0}; };                               // :18-23 (0-based atoms)
+struct System { Box box; std::vector<Atom> atoms; std::vector<Bond> bonds; int natoms{0}, nbonds{0}, ntypes{0}; };
+struct CSR { std::vector<int> head, list; };                                  // :71-74
+
+// Params (MC/include/types.hpp:34-69) with the extension keys used by the
+// device path: lj_rc, explicit grid, fixed mu.
+struct Params {
+    real rc{0.0};
+    int ntypes{0};
+    std::vector<real> lj_eps, lj_sig;
+    bool lj_shift{true};
+    real lj_rc{0.0};
+    std::string coulombtype{"coul_cut"};
+    real epsilon_r{1.0};
+    real ewald_rtol{1e-5};
+    real ewald_alpha{0.0};
+    real pme_spacing{0.12};
+    int pme_order{4};
+    int nx{0}, ny{0}, nz{0};
+    bool pme_3dc_enabled{false};
+    real pme_3dc_zfac{3.0};
+    int NH_N_type{4}, NH_P_type{5}, W_type{7}, Cl_type{8};
+    int rng_seed{2025};
+    real beta{1.0 / (0.008314462 * 300.0)};
+    real w_z_extra_nm{1.0};
+    real mu_eff{0.0};
+};
+
+// build_bond_adjacency (MC/src/io.cpp:148-169)
+inline CSR build_bond_adjacency(const System& sys) {
+    std::vector<std::vector<int>> tmp(sys.natoms);
+    for (const auto& b : sys.bonds) { tmp[b.i].push_back(b.j); tmp[b.j].push_back(b.i); }
+    CSR out;
+    out.head.assign(sys.natoms + 1, 0);
+    for (int i = 0; i < sys.natoms; ++i) {
+        auto& v = tmp[i];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        out.head[i + 1] = out.head[i] + (int)v.size();
+    }
+    for (auto& v : tmp) out.list.insert(out.list.end(), v.begin(), v.end());
+    return out;
+}
+
+// ------------------------------------------------------------ device objects
+template <class H, int (*Destroy)(H)>
+class Handle {
+public:
+    Handle() = default;
+    explicit Handle(H h) : h_(h) {}
+    Handle(const Handle&) = delete;
+    Handle& operator=(const Handle&) = delete;
+    Handle(Handle&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+    Handle& operator=(Handle&& o) noexcept { if (this != &o) { reset(); h_ = o.h_; o.h_ = nullptr; } return *this; }
+    ~Handle() { reset(); }
+    H get() const { return h_; }
+    void reset() { if (h_) Destroy(h_); h_ = nullptr; }
+protected:
+    H h_ = nullptr;
+};
+
+// DeviceAtoms (MC/include/device_data.hpp:5-12) + upload_atoms_to_device (MC/src/main.cu:18-41)
+struct DeviceAtoms : Handle<phb_atoms, phb_atoms_destroy> {
+    int natoms{0};
+    DeviceAtoms() = default;
+    explicit DeviceAtoms(const System& sys) : natoms(sys.natoms) {
+        std::vector<double> x(natoms), y(natoms), z(natoms), q(natoms);
+        std::vector<int> t(natoms);
+        for (int i = 0; i < natoms; ++i) {
+            const Atom& a = sys.atoms[i];
+            x[i] = a.x; y[i] = a.y; z[i] = a.z; q[i] = a.q; t[i] = a.type - 1;
+        }
+        phb_atoms h = nullptr;
+        check(phb_atoms_create(natoms, x.data(), y.data(), z.data(), q.data(), t.data(), &h));
+        h_ = h;
+    }
+};
+
+// DeviceCSR (MC/include/device_data.hpp:14-19)
+struct DeviceCSR : Handle<phb_csr, phb_csr_destroy> {
+    int rows{0};
+    long long nnz{0};
+    DeviceCSR() = default;
+    explicit DeviceCSR(phb_csr h) : Handle(h) { check(phb_csr_shape(h, &rows, &nnz)); }
+    CSR download() const {
+        CSR c;
+        c.head.resize(rows + 1);
+        c.list.resize(nnz);
+        check(phb_csr_download(h_, c.head.data(), c.list.data()));
+        return c;
+    }
+};
+
+// copy_csr_to_device (MC/include/neighbor.hpp:6)
+inline DeviceCSR copy_csr_to_device(const CSR& host) {
+    phb_csr h = nullptr;
+    const int rows = host.head.empty() ? 0 : (int)host.head.size() - 1;
+    check(phb_csr_create(rows, host.head.data(), host.list.data(), &h));
+    return DeviceCSR(h);
+}
+
+// build_half_neighbors_gpu (MC/include/neighbor.hpp:10-13)
+inline DeviceCSR build_half_neighbors_gpu(const DeviceAtoms& atoms, const Box& box, const DeviceCSR& bonds12_dev,
+                                          real rc) {
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    phb_csr h = nullptr;
+    check(phb_nlist_build_half(atoms.get(), &b, bonds12_dev.get(), rc, &h));
+    return DeviceCSR(h);
+}
+
+// DeviceParams (MC/include/device_data.hpp:21-36) + upload_params_to_device (MC/src/main.cu:43-66)
+struct DeviceParams : Handle<phb_params, phb_params_destroy> {
+    int ntypes{0};
+    std::string coulombtype;
+    DeviceParams() = default;
+    explicit DeviceParams(const Params& p) : ntypes(p.ntypes), coulombtype(p.coulombtype) {
+        phb_params_desc d{};
+        d.ntypes = p.ntypes;
+        d.lj_eps = p.lj_eps.data();
+        d.lj_sig = p.lj_sig.data();
+        d.lj_shift = p.lj_shift ? 1 : 0;
+        d.rc = p.rc;
+        d.lj_rc = p.lj_rc;
+        d.epsilon_r = p.epsilon_r;
+        d.coulombtype = p.coulombtype == "pme" ? PHB_COUL_PME : PHB_COUL_CUT;
+        d.ewald_alpha = p.ewald_alpha;
+        d.ewald_rtol = p.ewald_rtol;
+        d.pme_spacing = p.pme_spacing;
+        d.pme_order = p.pme_order;
+        d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+        d.pme_3dc = p.pme_3dc_enabled ? 1 : 0;
+        if (p.coulombtype != "pme" && p.coulombtype != "coul_cut")
+            throw std::runtime_error("total_energy: unsupported coulombtype");  // MC/src/mc.cu:57
+        phb_params h = nullptr;
+        check(phb_params_create(&d, &h));
+        h_ = h;
+    }
+};
+
+// compute_lj_sr_energy_gpu / compute_coul_sr_energy_gpu (MC/include/energy.hpp:7-18)
+inline real compute_lj_sr_energy_gpu(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, const Box& box) {
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    double e = 0;
+    check(phb_energy_lj(a.get(), p.get(), neigh.get(), &b, &e));
+    return e;
+}
+inline real compute_coul_sr_energy_gpu(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh,
+                                       const Box& box) {
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    double e = 0;
+    check(phb_energy_coul_cut(a.get(), p.get(), neigh.get(), &b, &e));
+    return e;
+}
+
+// PMEEnergyComponents / PMEPlan / pmeCreate / pmeDestroy / pmeEnergy (MC/include/pme.hpp:7-49)
+struct PMEEnergyComponents { real real_space{0}, recip_space{0}, self_term{0}, qcorr_term{0}; };
+
+struct PMEPlan {
+    int nx{0}, ny{0}, nz{0};
+    int order{4};
+    real alpha{0.0};
+    bool use3dc{false};
+    phb_pme handle{nullptr};
+};
+
+inline void pmeCreate(PMEPlan* plan, const DeviceParams& params, const Box& box, real rc, int precision = PHB_FP64) {
+    if (!plan) throw std::runtime_error("pmeCreate: null plan");  // MC/src/pme.cu:292
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    check(phb_pme_create(params.get(), &b, rc, precision, &plan->handle));
+    check(phb_pme_info(plan->handle, &plan->nx, &plan->ny, &plan->nz, &plan->order, &plan->alpha));
+}
+
+inline void pmeDestroy(PMEPlan* plan) {
+    if (!plan) return;
+    if (plan->handle) phb_pme_destroy(plan->handle);
+    plan->handle = nullptr;
+}
+
+inline real pmeEnergy(PMEPlan& plan, const DeviceAtoms& atoms, const DeviceCSR& neigh, const Box& box,
+                      PMEEnergyComponents* components = nullptr) {
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    phb_pme_components c{};
+    double e = 0;
+    check(phb_pme_energy(plan.handle, atoms.get(), neigh.get(), &b, &c, &e));
+    if (components) *components = PMEEnergyComponents{c.real_space, c.recip_space, c.self_term, c.qcorr_term};
+    return e;
+}
+
+// total_energy (static in MC/src/mc.cu:43-60)
+inline real total_energy(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, const Box& box,
+                         PMEPlan* plan) {
+    phb_box b{box.Lx, box.Ly, box.Lz};
+    double e = 0;
+    if (p.coulombtype == "pme" && !plan) throw std::runtime_error("total_energy: PME plan is null");
+    check(phb_total_energy(a.get(), p.get(), neigh.get(), plan ? plan->handle : nullptr, &b, nullptr, &e));
+    return e;
+}
+
+// ------------------------------------------------------------ Monte Carlo
+// PhaseGCResult-like summary of a fixed-mu run (MC/include/mc.hpp:32-44).
+struct PhaseGCResult {
+    long long attempted{0}, accepted{0}, accepted_forward{0}, accepted_backward{0};
+    int Nh{0}, Nprot{0};
+    real E{0.0};
+    real mu_eff{0.0};
+};
+
+// Device chain at fixed mu: the move rule of phase_gc_mu_control
+// (MC/src/mc.cu:379-503) with std::mt19937(p.rng_seed) semantics.
+class MonteCarlo : public Handle<phb_mc, phb_mc_destroy> {
+public:
+    MonteCarlo(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, PMEPlan& plan, const Box& box,
+               const Params& hp, int pending_max = 8, int spec_window = 8, int recompute_interval = 1000) {
+        phb_box b{box.Lx, box.Ly, box.Lz};
+        phb_mc_desc d{};
+        d.nh_n_type = hp.NH_N_type; d.nh_p_type = hp.NH_P_type; d.w_type = hp.W_type; d.cl_type = hp.Cl_type;
+        d.beta = hp.beta; d.mu_eff = hp.mu_eff; d.w_z_extra = hp.w_z_extra_nm;
+        d.precision = PHB_FP64; d.pending_max = pending_max; d.recompute_interval = recompute_interval;
+        d.record_cap = 0; d.spec_window = spec_window;
+        mu_ = hp.mu_eff;
+        phb_mc h = nullptr;
+        check(phb_mc_create(a.get(), p.get(), neigh.get(), plan.handle, &b, &d, (unsigned)hp.rng_seed, &h));
+        h_ = h;
+    }
+    PhaseGCResult run(long long ntrials) {
+        phb_mc_stats s{};
+        check(phb_mc_run(h_, ntrials, &s));
+        if (s.status == 1) throw std::runtime_error("[sgmc] no valid pairs to flip; stop.");  // MC/src/mc.cu:399
+        PhaseGCResult r;
+        r.attempted = s.attempted; r.accepted = s.accepted;
+        r.accepted_forward = s.accepted_forward; r.accepted_backward = s.accepted_backward;
+        r.Nh = s.nsites; r.Nprot = s.nprot; r.E = s.E; r.mu_eff = mu_;
+        return r;
+    }
+private:
+    double mu_ = 0.0;
+};
+
+inline PhaseGCResult phase_gc_fixed_mu(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh,
+                                       PMEPlan& plan, const Box& box, const Params& hp, long long ntrials) {
+    MonteCarlo mc(a, p, neigh, plan, box, hp);
+    return mc.run(ntrials);
+}
+
+}  // namespace ph_brush

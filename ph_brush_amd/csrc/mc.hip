@@ -959,7 +959,7 @@ struct phb_mc_s {
     DevBuf<int> rec_i, rec_j;
     DevBuf<double> rec_dE, rec_u;
     McDev dev{};
-    long long refreshes = 0, recomputes = 0, launches = 0, next_recompute = 0;
+    long long refreshes = 0, recomputes = 0, launches = 0, next_recompute = 0, trials_seen = 0;
     // profiling: event triples around (chain launch, refresh) pairs and recomputes
     bool profiling = false;
     std::vector<hipEvent_t> ev;
@@ -1277,9 +1277,13 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
             long long chunk_end = target;
             if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
             hipLaunchKernelGGL(k_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
-            // each launch runs until the chunk ends or pending_max changes accumulate
-            int batch = 4;
+            // each launch runs until the chunk ends or pending_max changes accumulate;
+            // batch size from the trials one launch has been covering
             while (done < chunk_end && status == 0) {
+                const double tpl = M->launches > 0 && M->trials_seen > 0 ? (double)M->trials_seen / M->launches : 8.0;
+                int batch = (int)std::ceil((double)(chunk_end - done) / std::max(tpl, 1.0)) + 1;
+                batch = std::max(1, std::min(batch, 64));
+                const long long before = done;
                 for (int b = 0; b < batch; ++b) {
                     const size_t e0 = M->ev_used;
                     if (M->profiling) M->event();
@@ -1296,7 +1300,7 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
                 require(c.status != 2, "phb_mc_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
                 done = c.trials_done;
                 status = c.status;
-                batch = std::min(batch * 2, 64);
+                M->trials_seen += done - before;
             }
             if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
                 full_recompute(M);

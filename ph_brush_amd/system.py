@@ -317,8 +317,8 @@ class MonteCarlo(_Handle):
     fixed mu_eff; the atoms handle is updated in place by accepted moves."""
     _destroy = "phb_mc_destroy"
 
-    def __init__(self, atoms, params, neigh, pme, box, mc: McParams, seed: int, *, pending_max=32,
-                 recompute_interval=1000, record_cap=0, precision=L.PHB_FP64, spec_window=16):
+    def __init__(self, atoms, params, neigh, pme, box, mc: McParams, seed: int, *, pending_max=8,
+                 recompute_interval=1000, record_cap=0, precision=L.PHB_FP64, spec_window=8):
         super().__init__()
         self._keep = (atoms, params, neigh, pme)
         self.record_cap = record_cap
