@@ -649,4 +649,138 @@ void or_canonical_draws(unsigned seed, long long n, double* out) {
     for (long long k = 0; k < n; ++k) out[k] = d(rng);
 }
 
+// ---------------------------------------------------------------------------
+// Delayed-acceptance surrogate (EXTENSION; no reference counterpart).
+void or_desc_thresholds(double r_desc, double* th) {
+    for (int k = 0; k < 12; ++k) {
+        const double rk = (double)k * r_desc / 12.0;
+        th[k] = rk * rk;
+    }
+}
+
+void or_da_descriptors(const or_system* s, const or_desc_params* dp, int n, const int* i, const int* j, int* d,
+                       double* rij) {
+    double th[12];
+    or_desc_thresholds(dp->r_desc, th);
+    const double rc2 = dp->r_desc * dp->r_desc;
+    const int N = s->natoms;
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < n; ++m) {
+        int* out = d + (size_t)m * 48;
+        for (int c = 0; c < 48; ++c) out[c] = 0;
+        const int a = i[m], b = j[m];
+        for (int which = 0; which < 2; ++which) {
+            const int ctr = which ? b : a, sg = which ? -1 : 1;
+            for (int k = 0; k < N; ++k) {
+                if (k == a || k == b) continue;
+                const int sp = dp->species_of_type[s->type[k]];
+                if (sp < 0) continue;
+                const double r2 = pair_r2(s, ctr, k);
+                if (!(r2 <= rc2)) continue;
+                int sh = 0;
+                while (sh < 11 && r2 >= th[sh + 1]) ++sh;
+                out[sh * 4 + sp] += sg;
+            }
+        }
+        if (rij) rij[m] = std::sqrt(pair_r2(s, a, b));
+    }
+}
+
+int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,
+              long long ntrials, unsigned seed, const double* sur_trial, const double* sur_pair, or_da_record* rec,
+              double* q_out, int* type_out, double* E_out, long long* n_done) {
+    const int N = s0->natoms;
+    std::vector<double> q(s0->q, s0->q + N);
+    std::vector<int> type(s0->type, s0->type + N);
+    or_system s = *s0;
+    s.q = q.data(); s.type = type.data();
+    // pools exactly as or_gc_run (build_plan, MC/src/mc.cu:67-100)
+    std::vector<int> nh_n, nh_p, w, cl;
+    for (int i = 0; i < N; ++i) {
+        int t = type[i] + 1;
+        if (t == m->nh_n_type) nh_n.push_back(i);
+        else if (t == m->nh_p_type) nh_p.push_back(i);
+        else if (t == m->w_type) w.push_back(i);
+    }
+    double z_max_site = 0.0;
+    for (int i : nh_n) z_max_site = std::max(z_max_site, s.z[i]);
+    for (int i : nh_p) z_max_site = std::max(z_max_site, s.z[i]);
+    const double z_cut = z_max_site + m->w_z_extra;
+    { std::vector<int> f; for (int j : w) if (s.z[j] < z_cut) f.push_back(j); w.swap(f); }
+    for (int j = 0; j < N; ++j) if (type[j] + 1 == m->cl_type && s.z[j] < z_cut) cl.push_back(j);
+
+    double E = or_total_energy(&s, p, head, list, nullptr);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    auto choose_one = [&](const std::vector<int>& pool) {
+        std::uniform_int_distribution<int> dd(0, (int)pool.size() - 1);
+        return pool[dd(rng)];
+    };
+    int status = 0;
+    long long t = 0;
+    for (; t < ntrials; ++t) {
+        long long a = (long long)nh_n.size(), b = (long long)nh_p.size();
+        long long c = (long long)w.size(), d = (long long)cl.size();
+        long long w_fwd = a * c, w_bwd = b * d, S_before = a * c + b * d;
+        bool forward;
+        if (w_fwd > 0 && w_bwd > 0) forward = uni(rng) < (double)w_fwd / (double)(w_fwd + w_bwd);
+        else if (w_fwd > 0) forward = true;
+        else if (w_bwd > 0) forward = false;
+        else { status = 1; break; }
+        int i, j;
+        if (forward) { i = choose_one(nh_n); j = choose_one(w); }
+        else { i = choose_one(nh_p); j = choose_one(cl); }
+        const int sg = forward ? 1 : -1;
+        const double dEs = sur_trial ? sur_trial[t] : (double)sg * sur_pair[(size_t)i * N + j];
+        long long S_after = forward ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
+        double logH = 0.0;
+        if (S_before > 0 && S_after > 0) logH = std::log((double)S_before) - std::log((double)S_after);
+        double e1 = (double)(-m->beta * (dEs - sg * m->mu_eff)) + logH;
+        e1 = std::clamp(e1, -40.0, 40.0);
+        bool acc1 = e1 >= 0.0;
+        if (!acc1) acc1 = std::exp(e1) > uni(rng);
+        int stage = 0;
+        double dE = std::numeric_limits<double>::quiet_NaN();
+        if (acc1) {
+            const int oti = type[i], otj = type[j];
+            const double oqi = q[i], oqj = q[j];
+            if (forward) { type[i] = m->nh_p_type - 1; q[i] = 1.0; type[j] = m->cl_type - 1; q[j] = -1.0; }
+            else { type[i] = m->nh_n_type - 1; q[i] = 0.0; type[j] = m->w_type - 1; q[j] = 0.0; }
+            const double E_trial = or_total_energy(&s, p, head, list, nullptr);
+            dE = E_trial - E;
+            const double e2 = ((double)(-m->beta * (dE - sg * m->mu_eff)) + logH) - e1;
+            bool acc2 = e2 >= 0.0;
+            if (!acc2) acc2 = std::exp(e2) > uni(rng);
+            if (acc2) {
+                stage = 1;
+                E = E_trial;
+                if (forward) {
+                    nh_n.erase(std::find(nh_n.begin(), nh_n.end(), i)); nh_p.push_back(i);
+                    w.erase(std::find(w.begin(), w.end(), j)); cl.push_back(j);
+                } else {
+                    nh_p.erase(std::find(nh_p.begin(), nh_p.end(), i)); nh_n.push_back(i);
+                    cl.erase(std::find(cl.begin(), cl.end(), j)); w.push_back(j);
+                }
+            } else {
+                stage = 2;
+                type[i] = oti; q[i] = oqi; type[j] = otj; q[j] = oqj;
+            }
+        }
+        if (rec) {
+            if (rec->fwd) rec->fwd[t] = forward ? 1 : 0;
+            if (rec->i) rec->i[t] = i;
+            if (rec->j) rec->j[t] = j;
+            if (rec->stage) rec->stage[t] = (signed char)stage;
+            if (rec->dE_star) rec->dE_star[t] = dEs;
+            if (rec->dE) rec->dE[t] = dE;
+        }
+    }
+    if (q_out) std::copy(q.begin(), q.end(), q_out);
+    if (type_out) std::copy(type.begin(), type.end(), type_out);
+    if (E_out) *E_out = E;
+    if (n_done) *n_done = t;
+    return status;
+}
+
 }  // extern "C"
+

@@ -103,6 +103,43 @@ void or_mt_draws(unsigned seed, long long n, unsigned *out);
 void or_uniform_int_draws(unsigned seed, long long n, const int *range_hi, int *out);
 void or_canonical_draws(unsigned seed, long long n, double *out);
 
+/* ---- delayed-acceptance surrogate (EXTENSION: the reference has no learned part) ----
+ * Descriptor of a candidate move (site i, ion j): d = h_i - h_j with 48 counts
+ * per centre, index shell*4 + species.  h_a counts the atoms k (k != i, k != j)
+ * with min-image r_ak^2 <= r_desc^2 by species_of_type[type k] (-1 = not
+ * counted) and shell = largest s in 0..11 with r_ak^2 >= th[s], th[s] =
+ * (s*r_desc/12)^2.  Both centres exclude the pair itself, so a move and its
+ * reverse see the same descriptor. */
+typedef struct {
+    const signed char *species_of_type;  /* ntypes entries, -1 or 0..3 */
+    int ntypes;
+    double r_desc;
+} or_desc_params;
+
+void or_desc_thresholds(double r_desc, double *th /* 12 */);
+void or_da_descriptors(const or_system *s, const or_desc_params *dp, int n, const int *i, const int *j,
+                       int *d /* n*48 */, double *rij /* n, min-image |r_i - r_j| */);
+
+/* Per-trial record of a delayed-acceptance chain; any pointer may be NULL. */
+typedef struct {
+    signed char *fwd;
+    int *i, *j;
+    signed char *stage;        /* 0 rejected at stage 1, 1 accepted, 2 rejected at stage 2 */
+    double *dE_star;           /* surrogate Delta-E used at stage 1 */
+    double *dE;                /* exact Delta-E (NaN when stage 1 rejected) */
+} or_da_record;
+
+/* Delayed-acceptance GC chain (Christen & Fox 2005) with the proposal, pools
+ * and draw order of or_gc_run.  Stage 1: e1 = clamp(-beta(dE* - s mu) + logH,
+ * +-40), accept if e1 >= 0 or exp(e1) > u1 (u1 drawn only when e1 < 0).
+ * Stage 2 (exact Delta-E): e2 = (-beta(dE - s mu) + logH) - e1, accept if
+ * e2 >= 0 or exp(e2) > u2 (u2 drawn only when e2 < 0).  dE* comes from
+ * sur_trial[t] when given, else s * sur_pair[i*natoms + j] (s = +1 forward). */
+int or_da_run(const or_system *s, const or_params *p, const or_mc_params *m,
+              const int *head, const int *list, long long ntrials, unsigned seed,
+              const double *sur_trial, const double *sur_pair, or_da_record *rec,
+              double *q_out, int *type_out, double *E_out, long long *n_done);
+
 void or_set_threads(int n);
 
 #ifdef __cplusplus

@@ -42,6 +42,14 @@ class OrRecord(C.Structure):
     _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("acc", _pb), ("dE", _pd), ("expo", _pd), ("u", _pd)]
 
 
+class OrDescParams(C.Structure):
+    _fields_ = [("species_of_type", _pb), ("ntypes", C.c_int), ("r_desc", C.c_double)]
+
+
+class OrDaRecord(C.Structure):
+    _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE", _pd)]
+
+
 _lib = None
 
 
@@ -81,6 +89,12 @@ def load():
         lib.or_gc_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
                                   C.c_longlong, C.c_uint, C.POINTER(OrRecord), _pd, _pi, _pd,
                                   C.POINTER(C.c_longlong)]
+        lib.or_da_run.restype = C.c_int
+        lib.or_da_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
+                                  C.c_longlong, C.c_uint, _pd, _pd, C.POINTER(OrDaRecord), _pd, _pi, _pd,
+                                  C.POINTER(C.c_longlong)]
+        lib.or_desc_thresholds.argtypes = [C.c_double, _pd]
+        lib.or_da_descriptors.argtypes = [C.POINTER(OrSystem), C.POINTER(OrDescParams), C.c_int, _pi, _pi, _pi, _pd]
         lib.or_mt_draws.argtypes = [C.c_uint, C.c_longlong, _pu]
         lib.or_uniform_int_draws.argtypes = [C.c_uint, C.c_longlong, _pi, _pi]
         lib.or_canonical_draws.argtypes = [C.c_uint, C.c_longlong, _pd]
@@ -185,6 +199,46 @@ class Oracle:
                                 C.byref(E), C.byref(nd))
         n = nd.value
         return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}
+
+    def da_descriptors(self, species_of_type, r_desc, i, j):
+        """Brute-force shell-count descriptors d = h_i - h_j (int, n x 48) and r_ij."""
+        sp = np.ascontiguousarray(species_of_type, dtype=np.int8)
+        dp = OrDescParams(sp.ctypes.data_as(_pb), len(sp), float(r_desc))
+        i = np.ascontiguousarray(i, dtype=np.int32); j = np.ascontiguousarray(j, dtype=np.int32)
+        n = len(i)
+        d = np.zeros((n, 48), np.int32)
+        rij = np.zeros(n)
+        self.lib.or_da_descriptors(C.byref(self.sys), C.byref(dp), n, i.ctypes.data_as(_pi), j.ctypes.data_as(_pi),
+                                   d.ctypes.data_as(_pi), rij.ctypes.data_as(_pd))
+        return d, rij
+
+    def da_run(self, mc, ntrials, seed, sur_trial=None, sur_pair=None):
+        """Delayed-acceptance chain with the surrogate given per trial or as an
+        N x N antisymmetric-by-direction table."""
+        N = self.s.natoms
+        rec = {k: np.zeros(ntrials, dt) for k, dt in
+               (("fwd", np.int8), ("i", np.int32), ("j", np.int32), ("stage", np.int8), ("dE_star", np.float64),
+                ("dE", np.float64))}
+        r = OrDaRecord(*(rec[k].ctypes.data_as(t) for k, t in
+                         (("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE", _pd))))
+        m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
+        st_ = np.ascontiguousarray(sur_trial, dtype=np.float64) if sur_trial is not None else None
+        sp_ = np.ascontiguousarray(sur_pair, dtype=np.float64) if sur_pair is not None else None
+        q, t = np.zeros(N), np.zeros(N, np.int32)
+        E = C.c_double()
+        nd = C.c_longlong()
+        st = self.lib.or_da_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), ntrials, seed,
+                                st_.ctypes.data_as(_pd) if st_ is not None else None,
+                                sp_.ctypes.data_as(_pd) if sp_ is not None else None, C.byref(r),
+                                q.ctypes.data_as(_pd), t.ctypes.data_as(_pi), C.byref(E), C.byref(nd))
+        n = nd.value
+        return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}
+
+
+def desc_thresholds(r_desc):
+    th = np.zeros(12)
+    load().or_desc_thresholds(float(r_desc), th.ctypes.data_as(_pd))
+    return th
 
 
 def good_fft_dim(n):

@@ -1,0 +1,80 @@
+// mc_host.hpp -- host object behind phb_mc and the host steps shared by the
+// plain and delayed-acceptance chains.
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "mc_dev.hpp"
+
+using phb::DevBuf;
+using phb::McCtl;
+using phb::McDev;
+using phb::stream;
+
+struct phb_mc_s {
+    phb_atoms_s* atoms = nullptr;
+    phb_params_s* params = nullptr;
+    phb_csr_s* neigh = nullptr;
+    phb_pme_s* pme = nullptr;
+    phb_box box{};
+    phb_mc_desc d{};
+    int nsites = 0, nions = 0;
+    DevBuf<int> fhead, fnbr, sbase;
+    DevBuf<double> kreal, r2, sw, phi, gk, gf, dgrid, selfw;
+    DevBuf<int> pbuf;
+    bool lds_pools = false;
+    size_t pool_lds_bytes = 0;
+    DevBuf<McCtl> ctl;
+    DevBuf<signed char> rec_fwd, rec_acc;
+    DevBuf<int> rec_i, rec_j;
+    DevBuf<double> rec_dE, rec_u;
+    McDev dev{};
+    long long refreshes = 0, recomputes = 0, launches = 0, next_recompute = 0, trials_seen = 0;
+    // profiling: event triples around (chain launch, refresh) pairs and recomputes
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)
+    double chain_ms = 0, refresh_ms = 0, recompute_ms = 0;
+    ~phb_mc_s() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipEvent_t event() {
+        if (ev_used == ev.size()) {
+            hipEvent_t e;
+            PHB_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        hipEvent_t e = ev[ev_used++];
+        PHB_HIP(hipEventRecord(e, stream()));
+        return e;
+    }
+    // after a stream sync: fold recorded intervals into the totals
+    void collect() {
+        for (auto& m : marks) {
+            float a = 0, b = 0;
+            if (m.second == 0) {
+                PHB_HIP(hipEventElapsedTime(&a, ev[m.first], ev[m.first + 1]));
+                PHB_HIP(hipEventElapsedTime(&b, ev[m.first + 1], ev[m.first + 2]));
+                chain_ms += a;
+                refresh_ms += b;
+            } else {
+                PHB_HIP(hipEventElapsedTime(&a, ev[m.first], ev[m.first + 1]));
+                recompute_ms += a;
+            }
+        }
+        marks.clear();
+        ev_used = 0;
+    }
+};
+
+namespace phb {
+McCtl read_ctl(phb_mc_s* M);
+// Full recompute: phi = G Q from scratch, total energy, moments, E reset.
+void full_recompute(phb_mc_s* M);
+// Folds the pending accepted charge changes into phi (k-space refresh).
+void refresh(phb_mc_s* M);
+void fill_stats(phb_mc_s* M, const McCtl& c, phb_mc_stats* s);
+void launch_probe(phb_mc_s* M, int n, const int* i, const int* j, const int* f, double* out);
+}  // namespace phb

@@ -22,6 +22,21 @@ KE = 138.935458  # kJ mol^-1 nm e^-2 (MC/src/pme.cu:305)
 TYPE_PERM, TYPE_CATION, TYPE_NH_N, TYPE_NH_P, TYPE_W, TYPE_CL = 1, 2, 4, 5, 7, 8
 NTYPES = 8
 
+# descriptor species of the surrogate: 0 charged bead, 1 neutral titratable
+# bead, 2 cation, 3 anion; -1 not counted (neutral placeholders)
+SPECIES_CHARGED, SPECIES_NEUTRAL_SITE, SPECIES_CATION, SPECIES_ANION = 0, 1, 2, 3
+
+
+def default_species(ntypes: int = NTYPES) -> np.ndarray:
+    """species_of_type (0-based type index) for the synthetic brushes."""
+    sp = np.full(ntypes, -1, np.int8)
+    sp[TYPE_PERM - 1] = SPECIES_CHARGED
+    sp[TYPE_NH_P - 1] = SPECIES_CHARGED
+    sp[TYPE_NH_N - 1] = SPECIES_NEUTRAL_SITE
+    sp[TYPE_CATION - 1] = SPECIES_CATION
+    sp[TYPE_CL - 1] = SPECIES_ANION
+    return sp
+
 
 @dataclass
 class System:
