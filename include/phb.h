@@ -213,6 +213,82 @@ typedef struct {
 int phb_gen_brush(const phb_brush_spec *spec, int cap, double *x, double *y, double *z, double *q,
                   int *type1, int *mol, int *bond_i, int *bond_j, int bond_cap, int *natoms, int *nbonds);
 
+
+/* ------------------------------------------------- learned surrogate + delayed acceptance
+ * No counterpart in the reference (it has no learned part).  A move's
+ * descriptor is d = h_i - h_j: 48 counts per centre (12 shells of r_desc/12 x
+ * species 0 charged bead, 1 neutral titratable bead, 2 cation, 3 anion), each
+ * centre excluding the other.  The network f (48 -> 128 -> 128 -> 1, GELU,
+ * BF16 MFMA with FP32 accumulation) predicts t = s dE + elF / r_ij, so that
+ * dE* = s (f(d) - elF / r_ij) is antisymmetric under the reverse move.
+ * Flat FP32 parameter layout: W1[48][128], b1[128], W2[128][128], b2[128],
+ * w3[128], b3 (PHB_NN_TRAIN trainable values), then in_scale[48], out_mean,
+ * out_std. */
+#define PHB_NN_IN 48
+#define PHB_NN_HIDDEN 128
+#define PHB_NN_TRAIN 22913
+#define PHB_NN_PARAMS 22963
+typedef struct phb_surrogate_s *phb_surrogate;
+/* Network with He-uniform initial weights (std::mt19937_64(seed)) and a device
+ * dataset of `capacity` (descriptor, target) samples. */
+int phb_surrogate_create(long long capacity, unsigned seed, phb_surrogate *out);
+int phb_surrogate_destroy(phb_surrogate s);
+int phb_surrogate_set_params(phb_surrogate s, const float *theta);    /* PHB_NN_PARAMS; resets Adam */
+int phb_surrogate_get_params(phb_surrogate s, float *theta);
+/* y = out_mean + out_std * f(d) for n raw descriptors (n x 48). */
+int phb_surrogate_forward(phb_surrogate s, int n, const short *d, float *y);
+/* Gradient of mean((f_norm(d) - (t - out_mean)/out_std)^2) over n rows (n % 64 == 0), no update. */
+int phb_surrogate_grad(phb_surrogate s, int n, const short *d, const float *t, float *grad, double *loss);
+int phb_surrogate_data_add(phb_surrogate s, int n, const short *d, const float *t);
+int phb_surrogate_data_get(phb_surrogate s, long long first, int n, short *d, float *t);
+int phb_surrogate_data_size(phb_surrogate s, long long *n);
+int phb_surrogate_data_clear(phb_surrogate s);
+typedef struct {
+    int epochs;
+    int batch;                 /* multiple of 64; 0 = 4096 */
+    double lr;                 /* Adam step size */
+    unsigned seed;             /* shuffling (std::mt19937_64) */
+    long long n_train;         /* first n_train samples are trained on; 0 = all */
+    int normalize;             /* set in_scale / out_mean / out_std from the training data first */
+} phb_train_desc;
+typedef struct {
+    long long steps;
+    double first_loss, last_loss;   /* normalised mean-squared loss of the first and last step */
+    double ms;
+} phb_train_stats;
+/* Adam training on the GPU (own forward/backward kernels). */
+int phb_surrogate_train(phb_surrogate s, const phb_train_desc *d, phb_train_stats *st);
+int phb_surrogate_loss_history(phb_surrogate s, long long n, float *loss);
+/* Error of out_mean + out_std f(d) against the stored targets over a dataset range (target units). */
+int phb_surrogate_eval(phb_surrogate s, long long first, long long n, double *rmse, double *mae);
+
+typedef struct {
+    const signed char *species_of_type;  /* ntypes entries (0-based types): -1 not counted, else 0..3 */
+    int ntypes;
+    double r_desc;                       /* histogram radius */
+    int candidates;                      /* surrogate screens per round, 1..64 (0 = 64) */
+} phb_da_desc;
+/* Builds the r_desc neighbour rows and per-atom histograms of a chain. */
+int phb_da_attach(phb_mc mc, const phb_da_desc *d);
+/* Descriptors (n x 48) and pair distances of moves in the chain's current state. */
+int phb_da_descriptors(phb_mc mc, int n, const int *i, const int *j, short *d, double *rij);
+/* Appends (descriptor, s dE + elF / r_ij) of n moves, dE exact, to the dataset. */
+int phb_da_collect(phb_mc mc, phb_surrogate s, int n, const int *i, const int *j, const int *fwd);
+/* Surrogate dE* of n moves. */
+int phb_da_predict(phb_mc mc, phb_surrogate s, int n, const int *i, const int *j, const int *fwd, double *dE_star);
+typedef struct {
+    long long attempted, stage1_accepted, accepted, rounds, launches;
+    int nprot, nsites, status;
+    double E;
+    double chain_ms, refresh_ms;         /* with phb_mc_set_profiling */
+} phb_da_stats;
+/* Runs ntrials delayed-acceptance trials: stage 1 on dE*, stage 2 on the exact dE,
+ * same proposals, pools and engine stream as phb_mc_run. */
+int phb_da_run(phb_mc mc, phb_surrogate s, long long ntrials, phb_da_stats *out);
+/* Per-trial record: stage 0 rejected at stage 1, 1 accepted, 2 rejected at stage 2; dE NaN when not evaluated. */
+int phb_da_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, signed char *stage, double *dE_star,
+                  double *dE);
+
 #ifdef __cplusplus
 }
 #endif

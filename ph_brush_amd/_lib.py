@@ -78,10 +78,40 @@ class McStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TrainDesc(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double), ("seed", C.c_uint),
+                ("n_train", C.c_longlong), ("normalize", C.c_int)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("first_loss", C.c_double), ("last_loss", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DaDesc(C.Structure):
+    _fields_ = [("species_of_type", C.POINTER(C.c_byte)), ("ntypes", C.c_int), ("r_desc", C.c_double),
+                ("candidates", C.c_int)]
+
+
+class DaStats(C.Structure):
+    _fields_ = [("attempted", C.c_longlong), ("stage1_accepted", C.c_longlong), ("accepted", C.c_longlong),
+                ("rounds", C.c_longlong), ("launches", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int),
+                ("status", C.c_int), ("E", C.c_double), ("chain_ms", C.c_double), ("refresh_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+NN_IN, NN_HIDDEN, NN_TRAIN, NN_PARAMS = 48, 128, 22913, 22963
+
 _h = C.c_void_p
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
 _pll = C.POINTER(C.c_longlong)
+_pf = C.POINTER(C.c_float)
+_ps = C.POINTER(C.c_short)
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 SIGNATURES = {
@@ -120,6 +150,25 @@ SIGNATURES = {
     "phb_mc_recompute": [_h, _pd],
     "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
     "phb_mc_set_profiling": [_h, C.c_int],
+    "phb_surrogate_create": [C.c_longlong, C.c_uint, C.POINTER(_h)],
+    "phb_surrogate_destroy": [_h],
+    "phb_surrogate_set_params": [_h, _pf],
+    "phb_surrogate_get_params": [_h, _pf],
+    "phb_surrogate_forward": [_h, C.c_int, _ps, _pf],
+    "phb_surrogate_grad": [_h, C.c_int, _ps, _pf, _pf, _pd],
+    "phb_surrogate_data_add": [_h, C.c_int, _ps, _pf],
+    "phb_surrogate_data_get": [_h, C.c_longlong, C.c_int, _ps, _pf],
+    "phb_surrogate_data_size": [_h, _pll],
+    "phb_surrogate_data_clear": [_h],
+    "phb_surrogate_train": [_h, C.POINTER(TrainDesc), C.POINTER(TrainStats)],
+    "phb_surrogate_loss_history": [_h, C.c_longlong, _pf],
+    "phb_surrogate_eval": [_h, C.c_longlong, C.c_longlong, _pd, _pd],
+    "phb_da_attach": [_h, C.POINTER(DaDesc)],
+    "phb_da_descriptors": [_h, C.c_int, _pi, _pi, _ps, _pd],
+    "phb_da_collect": [_h, _h, C.c_int, _pi, _pi, _pi],
+    "phb_da_predict": [_h, _h, C.c_int, _pi, _pi, _pi, _pd],
+    "phb_da_run": [_h, _h, C.c_longlong, C.POINTER(DaStats)],
+    "phb_da_record": [_h, C.c_longlong, C.POINTER(C.c_byte), _pi, _pi, C.POINTER(C.c_byte), _pd, _pd],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}
@@ -157,6 +206,16 @@ def call(name: str, *args) -> None:
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_pd)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pf)
+
+
+def sptr(a: np.ndarray):
+    assert a.dtype == np.int16 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ps)
 
 
 def iptr(a: np.ndarray):

@@ -1,0 +1,1324 @@
+// da.hip -- learned surrogate for the move's Delta-E and the delayed-acceptance
+// chain.  No counterpart in the reference (it has no learned part); the chain
+// keeps the proposal, pools and draw order of the trial loop of
+// phase_gc_mu_control (MC/src/mc.cu:379-503) so that the exact Delta-E stays
+// the judge of every accepted move.
+//
+// Descriptor of a move (site i, ion j): d = h_i - h_j, 48 shell counts per
+// centre (12 shells of r_desc/12 x 4 species), each centre excluding the other
+// one, so a move and its reverse see the same d.  Per-atom histograms are kept
+// in HBM (uint16 x 48) and updated incrementally from the accepted atoms'
+// r_desc rows.  Surrogate: dE* = s (f(d) - elF / r_ij), s = +1 forward, with
+// f the MLP 48 -> 128 -> 128 -> 1 (GELU; BF16 weights and activations on MFMA,
+// FP32 accumulation).  It is antisymmetric by construction, so the second
+// stage reduces to min(1, exp(-beta (dE - dE*))) up to the +-40 clamp
+// (Christen & Fox 2005):  e2 = (-beta (dE - s mu) + logH) - clamp(e1).
+//
+// Chain round (one workgroup of 16 waves): wave 0 draws up to 64 proposals
+// assuming each is rejected at stage 1; all waves build their descriptors and
+// run the MLP on MFMA; lane k takes candidate k's stage-1 decision, a ballot
+// finds the first stage-1 acceptance; only that candidate gets the exact
+// Delta-E (all 16 waves), its stage-2 decision, and on acceptance the state,
+// pool, pending-list and histogram updates.  The engine's stream position then
+// rolls back to just after that candidate's own draws, as in the plain chain.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "mc_host.hpp"
+
+namespace phb {
+
+constexpr int NN_IN = 48, NN_H = 128;
+constexpr int OFF_W1 = 0, OFF_B1 = NN_IN * NN_H, OFF_W2 = OFF_B1 + NN_H, OFF_B2 = OFF_W2 + NN_H * NN_H;
+constexpr int OFF_W3 = OFF_B2 + NN_H, OFF_B3 = OFF_W3 + NN_H;
+constexpr int NN_TRAIN = OFF_B3 + 1;                      // 22,913
+constexpr int OFF_SCALE = NN_TRAIN, OFF_MEAN = OFF_SCALE + NN_IN, OFF_STD = OFF_MEAN + 1;
+constexpr int NN_PARAM = OFF_STD + 1;                     // 22,963
+constexpr int NTP = NN_TRAIN + 1;                         // partial-gradient record: gradient + loss
+constexpr int DA_M = 64;                                  // candidates per round
+constexpr int XP = NN_IN + 8;                             // LDS row pitch (bf16) of input tiles
+constexpr int HP = NN_H + 8;                              // LDS row pitch (bf16) of hidden tiles
+constexpr int DA_SHIFT = 24;                              // r_desc row entry: atom | shell << 24
+
+static_assert(PHB_NN_TRAIN == NN_TRAIN && PHB_NN_PARAMS == NN_PARAM, "parameter layout");
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {      // round to nearest even
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad_f(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+__device__ __forceinline__ s4 ld4(const unsigned short* p) { return *reinterpret_cast<const s4*>(p); }
+__device__ __forceinline__ f4 mfma16(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// a 16x16x16 tile: A[i][k] in lane i + 16 (k/4), element k%4; B[k][j] in lane j + 16 (k/4);
+// D[i][j] in lane j + 16 (i/4), element i%4.
+
+struct NetDev {
+    const unsigned short *w1t;   // [128 out][48 in]   bf16 (B operand of the forward pass)
+    const unsigned short *w2t;   // [128 out][128 in]  bf16
+    const unsigned short *w2;    // [128 in][128 out]  bf16 (B operand of dh1 = dz2 W2^T)
+    const unsigned short *w3;    // [128]              bf16
+    const float* theta;          // FP32 master copy: biases, in_scale, out_mean, out_std
+};
+
+// ---------------------------------------------------------------- 4-wave tile passes (64 rows)
+// Layer 1 for rows r0..r0+15 of X: z[ct] holds D-layout tiles (rows r0 + 4g + e, column 16 ct + rr).
+__device__ __forceinline__ void layer1(const unsigned short* X, int r0, const NetDev& n, f4 (&z)[8], int lane) {
+    const int rr = lane & 15, g = lane >> 4;
+    s4 a[3];
+#pragma unroll
+    for (int kb = 0; kb < 3; ++kb) a[kb] = ld4(X + (r0 + rr) * XP + kb * 16 + 4 * g);
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        const unsigned short* wr = n.w1t + (ct * 16 + rr) * NN_IN + 4 * g;
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+        const float b = n.theta[OFF_B1 + ct * 16 + rr];
+        z[ct] = acc + b;
+    }
+}
+
+__device__ __forceinline__ void layer2(const unsigned short* H1, int r0, const NetDev& n, f4 (&z)[8], int lane) {
+    const int rr = lane & 15, g = lane >> 4;
+    s4 a[8];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) a[kb] = ld4(H1 + (r0 + rr) * HP + kb * 16 + 4 * g);
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        const unsigned short* wr = n.w2t + (ct * 16 + rr) * NN_H + 4 * g;
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+        const float b = n.theta[OFF_B2 + ct * 16 + rr];
+        z[ct] = acc + b;
+    }
+}
+
+__device__ __forceinline__ float xor_sum16(float v) {   // sum over the 16 lanes sharing lane >> 4
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    return v;
+}
+__device__ __forceinline__ float xor_sum_groups(float v) {  // sum over the 4 lanes sharing lane & 15
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Loads 64 descriptor rows (dataset indices from idx, or rows first.. when idx is null) into X.
+__device__ __forceinline__ void load_rows(const short* __restrict__ dd, const int* __restrict__ idx, long long first,
+                                          int nrows, const float* __restrict__ theta, unsigned short* X, int tid,
+                                          int nthr) {
+    for (int e = tid; e < 64 * NN_IN; e += nthr) {
+        const int r = e / NN_IN, c = e - r * NN_IN;
+        float v = 0.f;
+        if (r < nrows) {
+            const long long s = idx ? (long long)idx[r] : first + r;
+            v = (float)dd[s * NN_IN + c] * theta[OFF_SCALE + c];
+        }
+        X[r * XP + c] = f2bf(v);
+    }
+}
+
+// Forward only: y = out_mean + out_std * net(d), 64 rows per workgroup of 256.
+__global__ __launch_bounds__(256) void k_mlp_forward(NetDev n, const short* __restrict__ dd, long long nrows,
+                                                     float* __restrict__ y) {
+    __shared__ __align__(16) unsigned short X[64 * XP];
+    __shared__ __align__(16) unsigned short H1[64 * HP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rr = lane & 15, g = lane >> 4;
+    const long long first = (long long)blockIdx.x * 64;
+    const int nr = (int)std::min<long long>(64, nrows - first);
+    load_rows(dd, nullptr, first, nr, n.theta, X, tid, 256);
+    __syncthreads();
+    const int r0 = wave * 16;
+    f4 z[8];
+    layer1(X, r0, n, z, lane);
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(z[ct][e]));
+    __syncthreads();
+    layer2(H1, r0, n, z, lane);
+    float yp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        const float w = bf2f(n.w3[ct * 16 + rr]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(z[ct][e]))) * w;
+    }
+    const float b3 = n.theta[OFF_B3], om = n.theta[OFF_MEAN], os = n.theta[OFF_STD];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float v = xor_sum16(yp[e]);
+        const int r = r0 + 4 * g + e;
+        if (rr == 0 && r < nr) y[first + r] = om + os * (v + b3);
+    }
+}
+
+// Forward + backward over 64 rows per workgroup (4 waves, wave w owns rows 16w..16w+15);
+// writes the workgroup's gradient of sum_rows (y - t_hat)^2 / B and its sum of squares.
+__global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restrict__ dd, const float* __restrict__ dt,
+                                                  const int* __restrict__ idx, long long first, int B,
+                                                  float* __restrict__ part) {
+    __shared__ __align__(16) unsigned short X[64 * XP];
+    __shared__ __align__(16) unsigned short H1[64 * HP];
+    __shared__ __align__(16) unsigned short DZ1[64 * HP];
+    __shared__ __align__(16) unsigned short DZ2[64 * HP];
+    __shared__ float s_t[64];
+    __shared__ float s_red[4][3 * NN_H];      // per wave: db1, db2, dw3
+    __shared__ float s_red2[4][2];            // per wave: db3, loss
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rr = lane & 15, g = lane >> 4;
+    const int b = blockIdx.x;
+    const int* bidx = idx ? idx + (long long)b * 64 : nullptr;
+    const long long bfirst = first + (long long)b * 64;
+    load_rows(dd, bidx, bfirst, 64, n.theta, X, tid, 256);
+    const float om = n.theta[OFF_MEAN], os = n.theta[OFF_STD];
+    if (tid < 64) {
+        const long long s = bidx ? (long long)bidx[tid] : bfirst + tid;
+        s_t[tid] = (dt[s] - om) / os;
+    }
+    __syncthreads();
+    const int r0 = wave * 16;
+    f4 z1[8], z2[8];
+    layer1(X, r0, n, z1, lane);
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(z1[ct][e]));
+    __syncthreads();
+    layer2(H1, r0, n, z2, lane);
+    float yp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        const float w = bf2f(n.w3[ct * 16 + rr]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(z2[ct][e]))) * w;
+    }
+    const float b3 = n.theta[OFF_B3], invB = 1.0f / (float)B;
+    float gy[4], lsum = 0.f, gsum = 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float y = xor_sum16(yp[e]) + b3;          // every lane of the group holds the sum
+        const float err = y - s_t[r0 + 4 * g + e];
+        gy[e] = 2.0f * err * invB;
+        lsum += err * err;
+        gsum += gy[e];
+    }
+    // backward through layer 3 and GELU 2; column reductions over this wave's 16 rows
+#pragma unroll
+    for (int ct = 0; ct < 8; ++ct) {
+        const int col = ct * 16 + rr;
+        const float w = bf2f(n.w3[col]);
+        float dw3 = 0.f, db2 = 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float h2 = bf2f(f2bf(gelu_f(z2[ct][e])));
+            dw3 += gy[e] * h2;
+            const float dz = gy[e] * w * gelu_grad_f(z2[ct][e]);
+            db2 += dz;
+            DZ2[(r0 + 4 * g + e) * HP + col] = f2bf(dz);
+        }
+        dw3 = xor_sum_groups(dw3);
+        db2 = xor_sum_groups(db2);
+        if (g == 0) { s_red[wave][NN_H + col] = db2; s_red[wave][2 * NN_H + col] = dw3; }
+    }
+    if (lane == 0) { s_red2[wave][0] = 0.f; s_red2[wave][1] = 0.f; }
+    __syncthreads();
+    // dh1 = dz2 W2^T (this wave's rows), then GELU 1
+    {
+        s4 a[8];
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) a[kb] = ld4(DZ2 + (r0 + rr) * HP + kb * 16 + 4 * g);
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            const unsigned short* wr = n.w2 + (ct * 16 + rr) * NN_H + 4 * g;
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+            const int col = ct * 16 + rr;
+            float db1 = 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float dz = acc[e] * gelu_grad_f(z1[ct][e]);
+                db1 += dz;
+                DZ1[(r0 + 4 * g + e) * HP + col] = f2bf(dz);
+            }
+            db1 = xor_sum_groups(db1);
+            if (g == 0) s_red[wave][col] = db1;
+        }
+    }
+    // gy and loss of this wave (each group of 16 lanes holds 4 rows; lanes rr == 0 contribute)
+    {
+        float l = rr == 0 ? lsum : 0.f, gs = rr == 0 ? gsum : 0.f;
+        l = xor_sum_groups(l);
+        gs = xor_sum_groups(gs);
+        if (lane == 0) { s_red2[wave][0] = gs; s_red2[wave][1] = l; }
+    }
+    __syncthreads();
+    float* out = part + (long long)b * NTP;
+    // dW2[in][out] = H1^T DZ2 over the 64 rows: 64 tiles, 16 per wave
+    for (int t = wave * 16; t < wave * 16 + 16; ++t) {
+        const int ti = t >> 3, tj = t & 7;
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            s4 a, bb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = kb * 16 + 4 * g + e;
+                a[e] = (short)H1[row * HP + ti * 16 + rr];
+                bb[e] = (short)DZ2[row * HP + tj * 16 + rr];
+            }
+            acc = mfma16(a, bb, acc);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) out[OFF_W2 + (ti * 16 + 4 * g + e) * NN_H + tj * 16 + rr] = acc[e];
+    }
+    // dW1[in][out] = X^T DZ1: 3 x 8 tiles, 6 per wave
+    for (int t = wave * 6; t < wave * 6 + 6; ++t) {
+        const int ti = t >> 3, tj = t & 7;
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            s4 a, bb;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = kb * 16 + 4 * g + e;
+                a[e] = (short)X[row * XP + ti * 16 + rr];
+                bb[e] = (short)DZ1[row * HP + tj * 16 + rr];
+            }
+            acc = mfma16(a, bb, acc);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) out[OFF_W1 + (ti * 16 + 4 * g + e) * NN_H + tj * 16 + rr] = acc[e];
+    }
+    for (int c = tid; c < NN_H; c += 256) {
+        out[OFF_B1 + c] = ((s_red[0][c] + s_red[1][c]) + s_red[2][c]) + s_red[3][c];
+        out[OFF_B2 + c] = ((s_red[0][NN_H + c] + s_red[1][NN_H + c]) + s_red[2][NN_H + c]) + s_red[3][NN_H + c];
+        out[OFF_W3 + c] =
+            ((s_red[0][2 * NN_H + c] + s_red[1][2 * NN_H + c]) + s_red[2][2 * NN_H + c]) + s_red[3][2 * NN_H + c];
+    }
+    if (tid == 0) {
+        out[OFF_B3] = ((s_red2[0][0] + s_red2[1][0]) + s_red2[2][0]) + s_red2[3][0];
+        out[NN_TRAIN] = ((s_red2[0][1] + s_red2[1][1]) + s_red2[2][1]) + s_red2[3][1];
+    }
+}
+
+// Sum of the partial records in workgroup order (deterministic).
+__device__ __forceinline__ float sum_parts(const float* __restrict__ part, int nparts, int k) {
+    float s = 0.f;
+    for (int p = 0; p < nparts; ++p) s += part[(long long)p * NTP + k];
+    return s;
+}
+
+__device__ __forceinline__ void cast_param(int k, float v, unsigned short* w1t, unsigned short* w2t,
+                                           unsigned short* w2, unsigned short* w3) {
+    if (k < OFF_B1) {
+        const int in = k / NN_H, o = k - in * NN_H;
+        w1t[o * NN_IN + in] = f2bf(v);
+    } else if (k >= OFF_W2 && k < OFF_B2) {
+        const int r = k - OFF_W2, in = r / NN_H, o = r - in * NN_H;
+        w2t[o * NN_H + in] = f2bf(v);
+        w2[in * NN_H + o] = f2bf(v);
+    } else if (k >= OFF_W3 && k < OFF_B3) {
+        w3[k - OFF_W3] = f2bf(v);
+    }
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nparts, float* __restrict__ g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < NTP) g[k] = sum_parts(part, nparts, k);
+}
+
+__global__ void k_mlp_cast(const float* __restrict__ theta, unsigned short* w1t, unsigned short* w2t,
+                           unsigned short* w2, unsigned short* w3) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < NN_TRAIN) cast_param(k, theta[k], w1t, w2t, w2, w3);
+}
+
+// Reduce the partial gradients, Adam (Kingma & Ba 2015, Algorithm 1), refresh the BF16 copies.
+__global__ void k_mlp_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ part, int nparts, float lr, float b1, float b2, float eps,
+                           float bc1, float bc2, unsigned short* w1t, unsigned short* w2t, unsigned short* w2,
+                           unsigned short* w3, float* __restrict__ loss_out, float invB) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == NN_TRAIN && loss_out) *loss_out = sum_parts(part, nparts, NN_TRAIN) * invB;
+    if (k >= NN_TRAIN) return;
+    const float gk = sum_parts(part, nparts, k);
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    const float th = theta[k] - lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+    theta[k] = th;
+    cast_param(k, th, w1t, w2t, w2, w3);
+}
+
+// ---------------------------------------------------------------- descriptors
+struct DaDev {
+    const int* dhead;             // r_desc rows (both directions)
+    const unsigned* dlist;        // atom | shell << 24
+    unsigned short* H;            // per-atom histograms [N][48]
+    const signed char* species;   // per 0-based type
+    const double *x, *y;
+    double Lx, Ly, Lz, rd2;
+    double th[12];                // shell thresholds (s r_desc / 12)^2
+    int M;                        // candidates per round
+};
+
+__device__ __forceinline__ int shell_of(const DaDev& A, double r2) {
+    int s = 0;
+#pragma unroll
+    for (int k = 1; k < 12; ++k) s += r2 >= A.th[k] ? 1 : 0;
+    return s;
+}
+
+// r_ij and the two exclusion slots of a move: exA (j's entry in h_i) and exB (i's entry in h_j).
+__device__ __forceinline__ void pair_terms(const DaDev& A, const McDev& D, int i, int j, double& r, int& exA, int& exB) {
+    const double dx = min_image_exact(A.x[i] - A.x[j], A.Lx), dy = min_image_exact(A.y[i] - A.y[j], A.Ly),
+                 dz = min_image_exact(D.z[i] - D.z[j], A.Lz);
+    const double r2 = r2_exact(dx, dy, dz);
+    r = sqrt(r2);
+    exA = -1; exB = -1;
+    if (r2 <= A.rd2) {
+        const int sh = shell_of(A, r2);
+        const int sj = A.species[D.type[j]], si = A.species[D.type[i]];
+        if (sj >= 0) exA = sh * 4 + sj;
+        if (si >= 0) exB = sh * 4 + si;
+    }
+}
+
+__device__ __forceinline__ int desc_value(const DaDev& A, int i, int j, int c, int exA, int exB) {
+    return (int)A.H[(size_t)i * NN_IN + c] - (int)A.H[(size_t)j * NN_IN + c] - (c == exA ? 1 : 0) + (c == exB ? 1 : 0);
+}
+
+__global__ void k_da_row_entries(int N, const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, DaDev A, const int* __restrict__ nbr,
+                                 unsigned* __restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    for (int e = A.dhead[a]; e < A.dhead[a + 1]; ++e) {
+        const int b = nbr[e];
+        const double dx = min_image_exact(x[a] - x[b], A.Lx), dy = min_image_exact(y[a] - y[b], A.Ly),
+                     dz = min_image_exact(z[a] - z[b], A.Lz);
+        out[e] = (unsigned)b | ((unsigned)shell_of(A, r2_exact(dx, dy, dz)) << DA_SHIFT);
+    }
+}
+
+// One wave per atom: counts by shell and species of its row.
+__global__ __launch_bounds__(256) void k_da_hist(int N, DaDev A, const int* __restrict__ type) {
+    __shared__ unsigned cnt[4][NN_IN];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int a = blockIdx.x * 4 + w;
+    if (lane < NN_IN) cnt[w][lane] = 0;
+    __syncthreads();
+    if (a < N) {
+        for (int e = A.dhead[a] + lane; e < A.dhead[a + 1]; e += 64) {
+            const unsigned v = A.dlist[e];
+            const int sp = A.species[type[v & 0xFFFFFFu]];
+            if (sp >= 0) atomicAdd(&cnt[w][(v >> DA_SHIFT) * 4 + sp], 1u);
+        }
+    }
+    __syncthreads();
+    if (a < N && lane < NN_IN) A.H[(size_t)a * NN_IN + lane] = (unsigned short)cnt[w][lane];
+}
+
+// Descriptors and pair distances of given moves (one thread per feature).
+__global__ void k_da_desc(DaDev A, McDev D, int n, const int* __restrict__ mi, const int* __restrict__ mj,
+                          short* __restrict__ d, double* __restrict__ rij) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)n * NN_IN) return;
+    const int m = (int)(e / NN_IN), c = (int)(e - (long long)m * NN_IN);
+    double r;
+    int exA, exB;
+    pair_terms(A, D, mi[m], mj[m], r, exA, exB);
+    d[e] = (short)desc_value(A, mi[m], mj[m], c, exA, exB);
+    if (c == 0 && rij) rij[m] = r;
+}
+
+// Training target t = s dE + elF / r_ij and the surrogate dE* = s (y - elF / r_ij).
+__global__ void k_da_target(int n, const int* __restrict__ mf, const double* __restrict__ dE,
+                            const double* __restrict__ rij, double elF, float* __restrict__ t) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    t[m] = (float)((mf[m] ? dE[m] : -dE[m]) + elF / rij[m]);
+}
+
+__global__ void k_da_surrogate(int n, const int* __restrict__ mf, const float* __restrict__ y,
+                               const double* __restrict__ rij, double elF, double* __restrict__ out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const double f = (double)y[m] - elF / rij[m];
+    out[m] = mf[m] ? f : -f;
+}
+
+// Every stride-th dataset row (normalisation statistics).
+__global__ void k_gather_rows(const short* __restrict__ dd, const float* __restrict__ dt, long long stride, long long ns,
+                              short* __restrict__ od, float* __restrict__ ot) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= ns * NN_IN) return;
+    const long long r = e / NN_IN, c = e - r * NN_IN;
+    od[e] = dd[r * stride * NN_IN + c];
+    if (c == 0) ot[r] = dt[r * stride];
+}
+
+__global__ void k_da_set_target(McCtl* C, long long target) {
+    if (threadIdx.x == 0) C->trials_target = target;
+}
+
+// ---------------------------------------------------------------- delayed-acceptance chain
+struct CandLds {
+    int i[DA_M], j[DA_M], fwd[DA_M], pos[DA_M], exA[DA_M], exB[DA_M];
+    double u1[DA_M], rij[DA_M], dEs[DA_M], e1[DA_M];
+};
+
+template <int P, typename PoolT, bool LDS_POOLS>
+__global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDev net) {
+    __shared__ RngLds s_rng;
+    __shared__ TrialT<P> s_tr[1];
+    __shared__ PendT<P> s_pd;
+    __shared__ DecisionLds s_dc;
+    __shared__ ChainState s_st;
+    __shared__ CandLds s_c;
+    __shared__ double s_wsum[16];
+    __shared__ __align__(16) unsigned short s_X[DA_M * XP];
+    __shared__ __align__(16) unsigned short s_H1[DA_M * HP];
+    __shared__ float s_y[DA_M][4];
+    __shared__ int s_first;
+    __shared__ long long s_s1, s_rounds;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    McCtl* C = D.ctl;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long long target = C->trials_target, rec_cap = C->rec_cap;
+    const int pend_max = C->pending_max, M = A.M;
+    PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
+    if (LDS_POOLS)
+        for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
+    for (int k = tid; k < 624; k += blockDim.x) {
+        s_rng.cur[k] = C->mt[k];
+        s_rng.out[0][k] = mt_temper(C->mt[k]);
+    }
+    const int pn0 = C->pending_folded ? 0 : C->pending_n;
+    for (int k = wave; k < pn0; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
+    if (tid == 0) {
+        s_st.done = C->trials_done;
+        s_st.acc_n = C->accepted; s_st.acc_f = C->acc_fwd; s_st.acc_b = C->acc_bwd;
+        for (int k = 0; k < 4; ++k) s_st.sz[k] = C->sizes[k];
+        s_st.status = C->status; s_st.nprot = C->nprot;
+        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = pn0;
+        s_st.E = C->E; s_st.Q = C->Q; s_st.Q2 = C->Q2; s_st.Mz = C->Mz; s_st.Qz2 = C->Qz2;
+        s_s1 = C->da_stage1; s_rounds = C->da_rounds;
+    }
+    const float om = net.theta[OFF_MEAN], osd = net.theta[OFF_STD], b3 = net.theta[OFF_B3];
+    __syncthreads();
+    for (;;) {
+        // ---- 1. proposals (wave 0), each assumed rejected at stage 1
+        if (wave == 0) {
+            int blk = s_st.blk, pos = s_st.pos, status = s_st.status;
+            const long long done = s_st.done;
+            const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
+            int ntr = 0, stop = 0;
+            if (done >= target || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            else {
+                const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
+                if (wf == 0 && wb == 0) { status = 1; stop = 1; }
+                else {
+                    const long long room = target - done;
+                    ntr = (int)(room < M ? room : M);
+                    const double sum = (double)(wf + wb);
+                    const bool both = wf > 0 && wb > 0;
+                    const int per = (both ? 2 : 0) + 4;
+                    rng_ensure(s_rng, blk, pos + ntr * per, lane);
+                    bool ok = true;
+                    int fwd = 0, ii = 0, jj = 0, pp = 0;
+                    double u = 0.0;
+                    if (lane < ntr) {
+                        int q = pos + lane * per;
+                        if (both) { fwd = canonical_at(s_rng, q) < (double)wf / sum ? 1 : 0; q += 2; }
+                        else fwd = wf > 0 ? 1 : 0;
+                        unsigned a1, a2;
+                        ok = index_at(s_rng, q, (unsigned)(fwd ? sz0 : sz1), a1);
+                        ok = index_at(s_rng, q + 1, (unsigned)(fwd ? sz2 : sz3), a2) && ok;
+                        ii = (int)pbuf[pool_slot(D, fwd ? 0 : 1, (int)a1)];
+                        jj = (int)pbuf[pool_slot(D, fwd ? 2 : 3, (int)a2)];
+                        pp = q + 2;
+                        u = canonical_at(s_rng, pp);
+                    }
+                    if (__all(ok)) {
+                        if (lane < ntr) {
+                            s_c.i[lane] = ii; s_c.j[lane] = jj; s_c.fwd[lane] = fwd;
+                            s_c.pos[lane] = pp; s_c.u1[lane] = u;
+                        }
+                        pos += ntr * per;
+                    } else {
+                        for (int k = 0; k < ntr; ++k) {
+                            int f2;
+                            if (both) f2 = draw_canonical(s_rng, blk, pos, lane) < (double)wf / sum ? 1 : 0;
+                            else f2 = wf > 0 ? 1 : 0;
+                            int i2, j2;
+                            if (f2) {
+                                i2 = (int)pbuf[pool_slot(D, 0, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz0))];
+                                j2 = (int)pbuf[pool_slot(D, 2, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz2))];
+                            } else {
+                                i2 = (int)pbuf[pool_slot(D, 1, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz1))];
+                                j2 = (int)pbuf[pool_slot(D, 3, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz3))];
+                            }
+                            const int p2 = pos;
+                            const double u2 = draw_canonical(s_rng, blk, pos, lane);
+                            if (lane == 0) {
+                                s_c.i[k] = i2; s_c.j[k] = j2; s_c.fwd[k] = f2; s_c.pos[k] = p2; s_c.u1[k] = u2;
+                            }
+                        }
+                    }
+                }
+            }
+            if (lane == 0) {
+                s_dc.ntr = ntr; s_dc.stop = stop;
+                s_st.blk = blk; s_st.pos = pos; s_st.status = status;
+            }
+        }
+        __syncthreads();
+        if (s_dc.stop) break;
+        const int ntr = s_dc.ntr, pend_n = s_st.pend_n;
+        // ---- 2. descriptors -> X (bf16, scaled); rows >= ntr are zero
+        if (tid < ntr) {
+            double r;
+            int exA, exB;
+            pair_terms(A, D, s_c.i[tid], s_c.j[tid], r, exA, exB);
+            s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
+        }
+        __syncthreads();
+        for (int e = tid; e < DA_M * NN_IN; e += MC_THREADS) {
+            const int k = e / NN_IN, c = e - k * NN_IN;
+            float v = 0.f;
+            if (k < ntr)
+                v = (float)desc_value(A, s_c.i[k], s_c.j[k], c, s_c.exA[k], s_c.exB[k]) * net.theta[OFF_SCALE + c];
+            s_X[k * XP + c] = f2bf(v);
+        }
+        __syncthreads();
+        // ---- 3. MLP: wave w -> rows 16 (w & 3).., column tiles 2 (w >> 2) + {0, 1}
+        {
+            const int rb = wave & 3, cg = wave >> 2, rr = lane & 15, g = lane >> 4, r0 = rb * 16;
+            s4 a[3];
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) a[kb] = ld4(s_X + (r0 + rr) * XP + kb * 16 + 4 * g);
+#pragma unroll
+            for (int c2 = 0; c2 < 2; ++c2) {
+                const int ct = 2 * cg + c2;
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                const unsigned short* wr = net.w1t + (ct * 16 + rr) * NN_IN + 4 * g;
+#pragma unroll
+                for (int kb = 0; kb < 3; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+                const float b = net.theta[OFF_B1 + ct * 16 + rr];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) s_H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(acc[e] + b));
+            }
+            __syncthreads();
+            float yp[4] = {0.f, 0.f, 0.f, 0.f};
+            {
+                f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+                const unsigned short* wr0 = net.w2t + (2 * cg * 16 + rr) * NN_H + 4 * g;
+                const unsigned short* wr1 = wr0 + 16 * NN_H;
+#pragma unroll 2
+                for (int kb = 0; kb < 8; ++kb) {
+                    const s4 a2 = ld4(s_H1 + (r0 + rr) * HP + kb * 16 + 4 * g);
+                    acc0 = mfma16(a2, ld4(wr0 + kb * 16), acc0);
+                    acc1 = mfma16(a2, ld4(wr1 + kb * 16), acc1);
+                }
+#pragma unroll
+                for (int c2 = 0; c2 < 2; ++c2) {
+                    const int col = (2 * cg + c2) * 16 + rr;
+                    const float b = net.theta[OFF_B2 + col], w = bf2f(net.w3[col]);
+                    const f4 acc = c2 ? acc1 : acc0;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(acc[e] + b))) * w;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = xor_sum16(yp[e]);
+                if (rr == 0) s_y[r0 + 4 * g + e][cg] = v;
+            }
+        }
+        __syncthreads();
+        // ---- 4. stage-1 decisions (wave 0): the first stage-1 acceptance ends the round
+        if (wave == 0) {
+            const long long done = s_st.done;
+            const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
+            const long long Sb = a * c + b * d;
+            const int k = lane;
+            bool acc1 = false;
+            double e1 = 0.0, dEs = 0.0;
+            if (k < ntr) {
+                const float y = ((s_y[k][0] + s_y[k][1]) + s_y[k][2]) + s_y[k][3] + b3;
+                const double f = (double)(om + osd * y) - D.elF / s_c.rij[k];
+                const int fwd = s_c.fwd[k];
+                dEs = fwd ? f : -f;
+                const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
+                e1 = mh_exponent(D.beta, dEs, fwd ? 1 : -1, D.mu, Sb, Sa);
+                acc1 = e1 >= 0.0 || exp(e1) > s_c.u1[k];
+                s_c.dEs[k] = dEs;
+                s_c.e1[k] = e1;
+            }
+            const unsigned long long m = __ballot(acc1);
+            const int first = m ? __ffsll((long long)m) - 1 : ntr;
+            if (k < first && k < ntr && done + k < rec_cap) {
+                D.rec_fwd[done + k] = (signed char)s_c.fwd[k]; D.rec_i[done + k] = s_c.i[k]; D.rec_j[done + k] = s_c.j[k];
+                D.rec_acc[done + k] = 0; D.rec_dE[done + k] = __longlong_as_double(0x7FF8000000000000LL);
+                D.rec_u[done + k] = dEs;
+            }
+            if (lane == 0) {
+                s_first = first;
+                s_dc.acc_k = -1;
+                ++s_rounds;
+                if (first >= ntr) s_st.done = done + ntr;
+                else {
+                    ++s_s1;
+                    stage_trial<P>(D, s_tr[0], s_c.i[first], s_c.j[first], s_c.fwd[first]);
+                }
+            }
+        }
+        __syncthreads();
+        const int first = s_first;
+        if (first < ntr) {
+            // ---- 5. exact Delta-E of the stage-1 acceptance (all waves)
+            stage_trial_data<P>(D, s_tr[0], tid, MC_THREADS);
+            __syncthreads();
+            const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, pend_n, tid, MC_THREADS));
+            if (lane == 0) s_wsum[wave] = part;
+            __syncthreads();
+            if (wave == 0) {
+                const TrialT<P>& t = s_tr[0];
+                const long long done = s_st.done;
+                const double Q = s_st.Q, Q2 = s_st.Q2, Mz = s_st.Mz, Qz2 = s_st.Qz2;
+                const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
+                const long long Sb = a * c + b * d;
+                double dsum = 0.0;
+                for (int w = 0; w < 16; ++w) dsum += s_wsum[w];
+                const double dE = dsum + correction_delta<P>(D, t, Q, Q2, Mz, Qz2);
+                const int fwd = t.fwd, sg = fwd ? 1 : -1;
+                const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
+                const double e1 = s_c.e1[first];
+                double e2;
+                {
+#pragma clang fp contract(off)
+                    double logH = 0.0;
+                    if (Sb > 0 && Sa > 0) logH = log((double)Sb) - log((double)Sa);
+                    e2 = ((double)((-D.beta) * (dE - (double)sg * D.mu)) + logH) - e1;
+                }
+                // roll the stream back to just after this candidate's own draws
+                int blk = s_st.blk, pos = s_c.pos[first] + (e1 >= 0.0 ? 0 : 2);
+                const bool back_ok = pos >= blk * 624 - 624;
+                bool acc2 = e2 >= 0.0;
+                if (!acc2 && back_ok) acc2 = exp(e2) > draw_canonical(s_rng, blk, pos, lane);
+                if (lane == 0) {
+                    if (!back_ok) s_st.status = 2;
+                    s_st.blk = blk; s_st.pos = pos;
+                    const long long tk = done + first;
+                    if (tk < rec_cap) {
+                        D.rec_fwd[tk] = (signed char)fwd; D.rec_i[tk] = t.i; D.rec_j[tk] = t.j;
+                        D.rec_acc[tk] = acc2 ? 1 : 2; D.rec_dE[tk] = dE; D.rec_u[tk] = s_c.dEs[first];
+                    }
+                    s_st.done = done + first + 1;
+                    if (acc2) {
+                        const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+                        s_st.E += dE;
+                        s_st.Q2 = Q2 + ((t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old));
+                        s_st.Q = Q + (dqi + dqj);
+                        s_st.Mz = Mz + (dqi * t.zi + dqj * t.zj);
+                        s_st.Qz2 = Qz2 + (dqi * t.zi * t.zi + dqj * t.zj * t.zj);
+                        ++s_st.acc_n;
+                        if (fwd) { ++s_st.acc_f; ++s_st.nprot; } else { ++s_st.acc_b; --s_st.nprot; }
+                        D.q[t.i] = t.qi_new; D.type[t.i] = t.ti_new;
+                        D.q[t.j] = t.qj_new; D.type[t.j] = t.tj_new;
+                        const int ps = fwd ? 0 : 1, pdd = fwd ? 1 : 0, is = fwd ? 2 : 3, id = fwd ? 3 : 2;
+                        s_dc.er_pool[0] = ps; s_dc.er_val[0] = t.i; s_dc.er_n[0] = s_st.sz[ps];
+                        s_dc.er_pool[1] = is; s_dc.er_val[1] = t.j; s_dc.er_n[1] = s_st.sz[is];
+                        s_dc.pb_pool[0] = pdd; s_dc.pb_val[0] = t.i; s_dc.pb_n[0] = s_st.sz[pdd];
+                        s_dc.pb_pool[1] = id; s_dc.pb_val[1] = t.j; s_dc.pb_n[1] = s_st.sz[id];
+                        if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
+                        else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
+                        s_dc.acc_k = 0;
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- 6. accepted: pending list, pools, histograms of the atoms around i and j
+            if (s_dc.acc_k == 0) {
+                const TrialT<P>& t = s_tr[0];
+                if (wave == 1) stage_pending<P>(D, s_pd, pend_n, t.i, t.qi_new - t.qi_old, lane, 64);
+                if (wave == 2) stage_pending<P>(D, s_pd, pend_n + 1, t.j, t.qj_new - t.qj_old, lane, 64);
+                pool_update<PoolT>(D, pbuf, s_dc);
+                for (int side = 0; side < 2; ++side) {
+                    const int atom = side ? t.j : t.i;
+                    const int so = A.species[side ? t.tj_old : t.ti_old], sn = A.species[side ? t.tj_new : t.ti_new];
+                    if (so != sn)
+                        for (int e = A.dhead[atom] + tid; e < A.dhead[atom + 1]; e += MC_THREADS) {
+                            const unsigned v = A.dlist[e];
+                            unsigned short* h = A.H + (size_t)(v & 0xFFFFFFu) * NN_IN + (v >> DA_SHIFT) * 4;
+                            if (so >= 0) h[so] = (unsigned short)(h[so] - 1);
+                            if (sn >= 0) h[sn] = (unsigned short)(h[sn] + 1);
+                        }
+                    __syncthreads();
+                }
+                if (tid == 0) s_st.pend_n = pend_n + 2;
+                __syncthreads();
+            }
+        }
+    }
+    // write back
+    if (LDS_POOLS)
+        for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) D.pbuf[k] = (int)pbuf[k];
+    const int pend_n = s_st.pend_n;
+    if (wave == 0) {
+        const int blk = s_st.blk, pos = s_st.pos;
+        const bool in_cur = pos >= blk * 624;
+        const unsigned* st = in_cur ? s_rng.cur : s_rng.prev;
+        for (int k = lane; k < 624; k += 64) C->mt[k] = st[k];
+        if (lane == 0) {
+            C->mti = in_cur ? pos - blk * 624 : pos - (blk - 1) * 624;
+            for (int k = 0; k < 4; ++k) C->sizes[k] = s_st.sz[k];
+            C->trials_done = s_st.done;
+            C->accepted = s_st.acc_n; C->acc_fwd = s_st.acc_f; C->acc_bwd = s_st.acc_b;
+            C->pending_n = pend_n; C->pending_folded = 0; C->status = s_st.status; C->nprot = s_st.nprot;
+            C->E = s_st.E; C->Q = s_st.Q; C->Q2 = s_st.Q2; C->Mz = s_st.Mz; C->Qz2 = s_st.Qz2;
+            C->da_stage1 = s_s1; C->da_rounds = s_rounds;
+        }
+    }
+    for (int k = tid; k < pend_n; k += blockDim.x) { C->pend_atom[k] = s_pd.atom[k]; C->pend_dq[k] = s_pd.dq[k]; }
+}
+
+}  // namespace phb
+
+using namespace phb;
+
+// ----------------------------------------------------------------------- host
+struct phb_surrogate_s {
+    DevBuf<float> theta, m, v, part, grad, loss_hist;
+    DevBuf<unsigned short> w1t, w2t, w2, w3;
+    DevBuf<short> dd;          // dataset descriptors [cap][48]
+    DevBuf<float> dt;          // dataset targets [cap]
+    DevBuf<int> idx;
+    long long cap = 0, n = 0, steps = 0;
+    int part_cap = 0;
+    NetDev net() const { return NetDev{w1t.p, w2t.p, w2.p, w3.p, theta.p}; }
+    void cast() {
+        hipLaunchKernelGGL(k_mlp_cast, dim3(ceil_div(NN_TRAIN, 256)), dim3(256), 0, stream(), theta.p, w1t.p, w2t.p,
+                           w2.p, w3.p);
+        PHB_HIP(hipGetLastError());
+    }
+    void ensure_parts(int nparts) {
+        if (nparts > part_cap) {
+            part.alloc((size_t)nparts * NTP);
+            part_cap = nparts;
+        }
+    }
+};
+
+namespace {
+
+struct DaState {
+    phb_da_desc d{};
+    std::vector<signed char> species;
+    DevBuf<signed char> sp;
+    DevBuf<int> dhead;
+    DevBuf<unsigned> dlist;
+    DevBuf<unsigned short> H;
+    DevBuf<double> x, y;       // copies of the positions (fixed)
+    DaDev dev{};
+    long long nentries = 0;
+};
+
+void da_free(void* p) { delete static_cast<DaState*>(p); }
+
+DaState* da_of(phb_mc_s* M) {
+    require(M->da != nullptr, "delayed acceptance: call phb_da_attach first", PHB_ERR_STATE);
+    return static_cast<DaState*>(M->da);
+}
+
+void rebuild_hist(phb_mc_s* M, DaState* S) {
+    const int N = M->atoms->n;
+    hipLaunchKernelGGL(k_da_hist, dim3(ceil_div(N, 4)), dim3(256), 0, stream(), N, S->dev, M->atoms->type.p);
+    PHB_HIP(hipGetLastError());
+}
+
+template <int P>
+void launch_da_p(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
+    if (M->lds_pools) {
+        auto kern = k_da_chain<P, unsigned short, true>;
+        PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)M->pool_lds_bytes));
+        hipLaunchKernelGGL(kern, dim3(1), dim3(MC_THREADS), M->pool_lds_bytes, stream(), M->dev, S->dev, sur->net());
+    } else {
+        hipLaunchKernelGGL((k_da_chain<P, int, false>), dim3(1), dim3(MC_THREADS), 0, stream(), M->dev, S->dev,
+                           sur->net());
+    }
+}
+
+void launch_da(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
+    switch (M->dev.P) {
+        case 3: launch_da_p<3>(M, S, sur); break;
+        case 4: launch_da_p<4>(M, S, sur); break;
+        case 5: launch_da_p<5>(M, S, sur); break;
+        case 6: launch_da_p<6>(M, S, sur); break;
+        case 7: launch_da_p<7>(M, S, sur); break;
+        default: launch_da_p<8>(M, S, sur); break;
+    }
+}
+
+void check_moves(phb_mc_s* M, int n, const int* i, const int* j, const int* f) {
+    for (int k = 0; k < n; ++k)
+        require(i[k] >= 0 && i[k] < M->atoms->n && j[k] >= 0 && j[k] < M->atoms->n && i[k] != j[k] &&
+                    (f == nullptr || f[k] == 0 || f[k] == 1),
+                "delayed acceptance: move index out of range");
+}
+
+// descriptors + r_ij of n moves (device buffers)
+void descriptors(phb_mc_s* M, DaState* S, int n, const int* di, const int* dj, short* dd, double* drij) {
+    rebuild_hist(M, S);
+    const long long tot = (long long)n * NN_IN;
+    hipLaunchKernelGGL(k_da_desc, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream(), S->dev, M->dev, n, di,
+                       dj, dd, drij);
+    PHB_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" int phb_surrogate_create(long long capacity, unsigned seed, phb_surrogate* out) {
+    return guard([&] {
+        require(out != nullptr && capacity >= 0, "phb_surrogate_create: bad arguments");
+        auto S = std::make_unique<phb_surrogate_s>();
+        S->theta.alloc(NN_PARAM); S->m.alloc(NN_TRAIN); S->v.alloc(NN_TRAIN);
+        S->grad.alloc(NTP);
+        S->w1t.alloc(NN_IN * NN_H); S->w2t.alloc(NN_H * NN_H); S->w2.alloc(NN_H * NN_H); S->w3.alloc(NN_H);
+        S->cap = capacity;
+        S->dd.alloc(std::max<long long>(capacity, 1) * NN_IN);
+        S->dt.alloc(std::max<long long>(capacity, 1));
+        // He-uniform initial weights, zero biases, unit scales (std::mt19937_64(seed))
+        std::vector<float> th(NN_PARAM, 0.0f);
+        std::mt19937_64 g(seed);
+        std::uniform_real_distribution<double> u(-1.0, 1.0);
+        for (int k = 0; k < NN_IN * NN_H; ++k) th[OFF_W1 + k] = (float)(u(g) * std::sqrt(6.0 / NN_IN));
+        for (int k = 0; k < NN_H * NN_H; ++k) th[OFF_W2 + k] = (float)(u(g) * std::sqrt(6.0 / NN_H));
+        for (int k = 0; k < NN_H; ++k) th[OFF_W3 + k] = (float)(u(g) * std::sqrt(6.0 / NN_H) * 0.1);
+        for (int k = 0; k < NN_IN; ++k) th[OFF_SCALE + k] = 1.0f;
+        th[OFF_MEAN] = 0.0f; th[OFF_STD] = 1.0f;
+        S->theta.upload(th.data(), NN_PARAM);
+        PHB_HIP(hipMemsetAsync(S->m.p, 0, NN_TRAIN * sizeof(float), stream()));
+        PHB_HIP(hipMemsetAsync(S->v.p, 0, NN_TRAIN * sizeof(float), stream()));
+        S->cast();
+        sync();
+        *out = S.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_destroy(phb_surrogate s) {
+    return guard([&] { delete s; return PHB_OK; });
+}
+
+extern "C" int phb_surrogate_set_params(phb_surrogate s, const float* theta) {
+    return guard([&] {
+        require(s && theta, "phb_surrogate_set_params: null");
+        require(theta[OFF_STD] > 0.0f, "phb_surrogate_set_params: out_std must be > 0");
+        s->theta.upload(theta, NN_PARAM);
+        PHB_HIP(hipMemsetAsync(s->m.p, 0, NN_TRAIN * sizeof(float), stream()));
+        PHB_HIP(hipMemsetAsync(s->v.p, 0, NN_TRAIN * sizeof(float), stream()));
+        s->steps = 0;
+        s->cast();
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_get_params(phb_surrogate s, float* theta) {
+    return guard([&] {
+        require(s && theta, "phb_surrogate_get_params: null");
+        s->theta.download(theta, NN_PARAM);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_forward(phb_surrogate s, int n, const short* d, float* y) {
+    return guard([&] {
+        require(s && n >= 0 && (n == 0 || (d && y)), "phb_surrogate_forward: bad arguments");
+        if (n == 0) return PHB_OK;
+        DevBuf<short> dd((size_t)n * NN_IN);
+        DevBuf<float> dy(n);
+        dd.upload(d, (size_t)n * NN_IN);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(ceil_div(n, 64)), dim3(256), 0, stream(), s->net(), dd.p, (long long)n,
+                           dy.p);
+        PHB_HIP(hipGetLastError());
+        dy.download(y, n);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_grad(phb_surrogate s, int n, const short* d, const float* t, float* grad, double* loss) {
+    return guard([&] {
+        require(s && n > 0 && n % 64 == 0 && d && t && grad, "phb_surrogate_grad: n must be a positive multiple of 64");
+        DevBuf<short> dd((size_t)n * NN_IN);
+        DevBuf<float> dt(n);
+        dd.upload(d, (size_t)n * NN_IN);
+        dt.upload(t, n);
+        const int nb = n / 64;
+        s->ensure_parts(nb);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, stream(), s->net(), dd.p, dt.p, (const int*)nullptr, 0LL,
+                           n, s->part.p);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3(ceil_div(NTP, 256)), dim3(256), 0, stream(), s->part.p, nb, s->grad.p);
+        PHB_HIP(hipGetLastError());
+        std::vector<float> g(NTP);
+        s->grad.download(g.data(), NTP);
+        sync();
+        std::copy(g.begin(), g.begin() + NN_TRAIN, grad);
+        if (loss) *loss = (double)g[NN_TRAIN] / n;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_data_add(phb_surrogate s, int n, const short* d, const float* t) {
+    return guard([&] {
+        require(s && n >= 0 && (n == 0 || (d && t)), "phb_surrogate_data_add: bad arguments");
+        require(s->n + n <= s->cap, "phb_surrogate_data_add: dataset capacity exceeded");
+        if (n == 0) return PHB_OK;
+        PHB_HIP(hipMemcpyAsync(s->dd.p + s->n * NN_IN, d, (size_t)n * NN_IN * sizeof(short), hipMemcpyHostToDevice,
+                               stream()));
+        PHB_HIP(hipMemcpyAsync(s->dt.p + s->n, t, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream()));
+        sync();
+        s->n += n;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_data_get(phb_surrogate s, long long first, int n, short* d, float* t) {
+    return guard([&] {
+        require(s && first >= 0 && n >= 0 && first + n <= s->n, "phb_surrogate_data_get: range outside the dataset");
+        if (n == 0) return PHB_OK;
+        if (d) PHB_HIP(hipMemcpyAsync(d, s->dd.p + first * NN_IN, (size_t)n * NN_IN * sizeof(short),
+                                      hipMemcpyDeviceToHost, stream()));
+        if (t) PHB_HIP(hipMemcpyAsync(t, s->dt.p + first, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream()));
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_data_size(phb_surrogate s, long long* n) {
+    return guard([&] {
+        require(s && n, "phb_surrogate_data_size: null");
+        *n = s->n;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_data_clear(phb_surrogate s) {
+    return guard([&] {
+        require(s != nullptr, "phb_surrogate_data_clear: null");
+        s->n = 0;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_train(phb_surrogate s, const phb_train_desc* td, phb_train_stats* st) {
+    return guard([&] {
+        require(s && td, "phb_surrogate_train: null");
+        const long long ntr = td->n_train > 0 ? std::min(td->n_train, s->n) : s->n;
+        const int B = td->batch > 0 ? td->batch : 4096;
+        require(B % 64 == 0, "phb_surrogate_train: batch must be a multiple of 64");
+        require(ntr >= B, "phb_surrogate_train: fewer training samples than one batch");
+        require(td->epochs >= 0 && td->lr > 0.0, "phb_surrogate_train: bad epochs or learning rate");
+        const auto t0 = std::chrono::steady_clock::now();
+        if (td->normalize) {
+            // input scales and target normalisation from (up to) 65,536 evenly spaced training samples
+            const long long ns = std::min<long long>(ntr, 65536);
+            std::vector<short> hd((size_t)ns * NN_IN);
+            std::vector<float> ht(ns);
+            const long long stride = ntr / ns;
+            {
+                DevBuf<short> gd((size_t)ns * NN_IN);
+                DevBuf<float> gt(ns);
+                hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((ns * NN_IN + 255) / 256)), dim3(256), 0, stream(),
+                                   s->dd.p, s->dt.p, stride, ns, gd.p, gt.p);
+                PHB_HIP(hipGetLastError());
+                gd.download(hd.data(), (size_t)ns * NN_IN);
+                gt.download(ht.data(), ns);
+                sync();
+            }
+            std::vector<float> th(NN_PARAM);
+            s->theta.download(th.data(), NN_PARAM);
+            sync();
+            for (int c = 0; c < NN_IN; ++c) {
+                double s1 = 0, s2 = 0;
+                for (long long k = 0; k < ns; ++k) { const double v = hd[k * NN_IN + c]; s1 += v; s2 += v * v; }
+                const double var = s2 / ns - (s1 / ns) * (s1 / ns);
+                th[OFF_SCALE + c] = (float)(1.0 / std::sqrt(std::max(var, 1.0)));
+            }
+            double s1 = 0, s2 = 0;
+            for (long long k = 0; k < ns; ++k) { s1 += ht[k]; s2 += (double)ht[k] * ht[k]; }
+            const double mean = s1 / ns, var = s2 / ns - mean * mean;
+            th[OFF_MEAN] = (float)mean;
+            th[OFF_STD] = (float)std::sqrt(std::max(var, 1e-12));
+            s->theta.upload(th.data(), NN_PARAM);
+        }
+        const long long spe = ntr / B;
+        const long long total = spe * td->epochs;
+        s->idx.alloc(std::max<long long>(spe * B, 1));
+        s->loss_hist.alloc(std::max<long long>(total, 1));
+        const int nb = B / 64;
+        s->ensure_parts(nb);
+        std::vector<int> perm(ntr);
+        std::mt19937_64 g(td->seed);
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        long long k = 0;
+        for (int ep = 0; ep < td->epochs; ++ep) {
+            std::iota(perm.begin(), perm.end(), 0);
+            std::shuffle(perm.begin(), perm.end(), g);
+            s->idx.upload(perm.data(), spe * B);
+            for (long long b = 0; b < spe; ++b, ++k) {
+                ++s->steps;
+                const double bc1 = 1.0 - std::pow((double)b1, (double)s->steps);
+                const double bc2 = 1.0 - std::pow((double)b2, (double)s->steps);
+                hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, stream(), s->net(), s->dd.p, s->dt.p,
+                                   s->idx.p + b * B, 0LL, B, s->part.p);
+                hipLaunchKernelGGL(k_mlp_adam, dim3(ceil_div(NTP, 256)), dim3(256), 0, stream(), s->theta.p, s->m.p,
+                                   s->v.p, s->part.p, nb, (float)td->lr, b1, b2, eps, (float)bc1, (float)bc2, s->w1t.p,
+                                   s->w2t.p, s->w2.p, s->w3.p, s->loss_hist.p + k, 1.0f / B);
+            }
+            PHB_HIP(hipGetLastError());
+            sync();   // the index buffer is reused by the next epoch
+        }
+        sync();
+        if (st) {
+            st->steps = total;
+            st->first_loss = st->last_loss = std::numeric_limits<double>::quiet_NaN();
+            if (total > 0) {
+                float a = 0, b = 0;
+                s->loss_hist.download(&a, 1);
+                PHB_HIP(hipMemcpyAsync(&b, s->loss_hist.p + total - 1, sizeof(float), hipMemcpyDeviceToHost, stream()));
+                sync();
+                st->first_loss = a; st->last_loss = b;
+            }
+            st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_loss_history(phb_surrogate s, long long n, float* loss) {
+    return guard([&] {
+        require(s && loss && n >= 0 && (size_t)n <= s->loss_hist.n, "phb_surrogate_loss_history: bad arguments");
+        if (n) s->loss_hist.download(loss, n);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_surrogate_eval(phb_surrogate s, long long first, long long n, double* rmse, double* mae) {
+    return guard([&] {
+        require(s && first >= 0 && n > 0 && first + n <= s->n, "phb_surrogate_eval: range outside the dataset");
+        DevBuf<float> dy(n);
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, stream(), s->net(),
+                           s->dd.p + first * NN_IN, n, dy.p);
+        PHB_HIP(hipGetLastError());
+        std::vector<float> y(n), t(n);
+        dy.download(y.data(), n);
+        PHB_HIP(hipMemcpyAsync(t.data(), s->dt.p + first, n * sizeof(float), hipMemcpyDeviceToHost, stream()));
+        sync();
+        double se = 0, ae = 0;
+        for (long long k = 0; k < n; ++k) { const double e = (double)y[k] - t[k]; se += e * e; ae += std::fabs(e); }
+        if (rmse) *rmse = std::sqrt(se / n);
+        if (mae) *mae = ae / n;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
+    return guard([&] {
+        require(M && d && d->species_of_type, "phb_da_attach: null argument");
+        require(d->ntypes == M->params->d.ntypes, "phb_da_attach: ntypes must match the parameter set");
+        require(d->r_desc > 0.0, "phb_da_attach: r_desc must be > 0");
+        const int cand = d->candidates > 0 ? d->candidates : DA_M;
+        require(cand >= 1 && cand <= DA_M, "phb_da_attach: candidates must be in [1, 64]");
+        const int N = M->atoms->n;
+        require(N < (1 << DA_SHIFT), "phb_da_attach: more than 2^24 atoms");
+        for (int t = 0; t < d->ntypes; ++t)
+            require(d->species_of_type[t] >= -1 && d->species_of_type[t] <= 3,
+                    "phb_da_attach: species must be -1 or 0..3");
+        auto S = std::make_unique<DaState>();
+        S->d = *d;
+        S->d.candidates = cand;
+        S->species.assign(d->species_of_type, d->species_of_type + d->ntypes);
+        S->d.species_of_type = nullptr;
+        S->sp.alloc(d->ntypes);
+        S->sp.upload(S->species.data(), d->ntypes);
+        // r_desc rows: half list without bond exclusion, expanded, entries packed with their shell
+        phb_csr half = nullptr;
+        if (phb_nlist_build_half(M->atoms, &M->box, nullptr, d->r_desc, &half) != PHB_OK)
+            throw Error(PHB_ERR_INTERNAL, std::string("phb_da_attach: ") + phb_last_error());
+        std::unique_ptr<phb_csr_s> half_owner(half);
+        DevBuf<int> nbr;
+        S->nentries = build_full_rows(N, half, S->dhead, nbr);
+        half_owner.reset();
+        S->dlist.alloc(std::max<long long>(S->nentries, 1));
+        S->H.alloc((size_t)std::max(N, 1) * NN_IN);
+        DaDev& A = S->dev;
+        A.dhead = S->dhead.p; A.dlist = S->dlist.p; A.H = S->H.p; A.species = S->sp.p;
+        A.x = M->atoms->x.p; A.y = M->atoms->y.p;
+        A.Lx = M->box.Lx; A.Ly = M->box.Ly; A.Lz = M->box.Lz; A.rd2 = d->r_desc * d->r_desc;
+        for (int k = 0; k < 12; ++k) {
+            const double rk = (double)k * d->r_desc / 12.0;
+            A.th[k] = rk * rk;
+        }
+        A.M = cand;
+        if (S->nentries > 0)
+            hipLaunchKernelGGL(k_da_row_entries, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
+                               M->atoms->y.p, M->atoms->z.p, A, nbr.p, S->dlist.p);
+        PHB_HIP(hipGetLastError());
+        rebuild_hist(M, S.get());
+        sync();
+        if (M->da_free) M->da_free(M->da);
+        M->da = S.release();
+        M->da_free = da_free;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_descriptors(phb_mc M, int n, const int* i, const int* j, short* d, double* rij) {
+    return guard([&] {
+        require(M && n >= 0 && (n == 0 || (i && j && d)), "phb_da_descriptors: bad arguments");
+        DaState* S = da_of(M);
+        if (n == 0) return PHB_OK;
+        check_moves(M, n, i, j, nullptr);
+        DevBuf<int> di(n), dj(n);
+        DevBuf<short> dd((size_t)n * NN_IN);
+        DevBuf<double> dr(n);
+        di.upload(i, n); dj.upload(j, n);
+        descriptors(M, S, n, di.p, dj.p, dd.p, dr.p);
+        dd.download(d, (size_t)n * NN_IN);
+        if (rij) dr.download(rij, n);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_collect(phb_mc M, phb_surrogate s, int n, const int* i, const int* j, const int* fwd) {
+    return guard([&] {
+        require(M && s && n >= 0 && (n == 0 || (i && j && fwd)), "phb_da_collect: bad arguments");
+        DaState* S = da_of(M);
+        require(s->n + n <= s->cap, "phb_da_collect: dataset capacity exceeded");
+        if (n == 0) return PHB_OK;
+        check_moves(M, n, i, j, fwd);
+        DevBuf<int> di(n), dj(n), df(n);
+        DevBuf<double> dr(n), dE(n);
+        di.upload(i, n); dj.upload(j, n); df.upload(fwd, n);
+        descriptors(M, S, n, di.p, dj.p, s->dd.p + s->n * NN_IN, dr.p);
+        launch_probe(M, n, di.p, dj.p, df.p, dE.p);
+        hipLaunchKernelGGL(k_da_target, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dE.p, dr.p,
+                           M->pme->elF, s->dt.p + s->n);
+        PHB_HIP(hipGetLastError());
+        sync();
+        s->n += n;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_predict(phb_mc M, phb_surrogate s, int n, const int* i, const int* j, const int* fwd,
+                              double* dE_star) {
+    return guard([&] {
+        require(M && s && n >= 0 && (n == 0 || (i && j && fwd && dE_star)), "phb_da_predict: bad arguments");
+        DaState* S = da_of(M);
+        if (n == 0) return PHB_OK;
+        check_moves(M, n, i, j, fwd);
+        DevBuf<int> di(n), dj(n), df(n);
+        DevBuf<short> dd((size_t)n * NN_IN);
+        DevBuf<double> dr(n), out(n);
+        DevBuf<float> dy(n);
+        di.upload(i, n); dj.upload(j, n); df.upload(fwd, n);
+        descriptors(M, S, n, di.p, dj.p, dd.p, dr.p);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(ceil_div(n, 64)), dim3(256), 0, stream(), s->net(), dd.p, (long long)n,
+                           dy.p);
+        hipLaunchKernelGGL(k_da_surrogate, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dy.p, dr.p,
+                           M->pme->elF, out.p);
+        PHB_HIP(hipGetLastError());
+        out.download(dE_star, n);
+        sync();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_stats* out) {
+    return guard([&] {
+        require(M && s && ntrials >= 0, "phb_da_run: bad arguments");
+        DaState* S = da_of(M);
+        rebuild_hist(M, S);
+        McCtl c = read_ctl(M);
+        const long long target = c.trials_done + ntrials;
+        long long done = c.trials_done;
+        int status = c.status;
+        while (done < target && status == 0) {
+            long long chunk_end = target;
+            if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
+            hipLaunchKernelGGL(k_da_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
+            while (done < chunk_end && status == 0) {
+                const size_t e0 = M->ev_used;
+                if (M->profiling) M->event();
+                launch_da(M, S, s);
+                ++M->launches;
+                if (M->profiling) M->event();
+                refresh(M);
+                if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
+                PHB_HIP(hipGetLastError());
+                c = read_ctl(M);
+                require(c.trials_done > done || c.status != 0 || c.trials_done >= chunk_end,
+                        "phb_da_run: the chain made no progress", PHB_ERR_INTERNAL);
+                require(c.status != 2, "phb_da_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
+                done = c.trials_done;
+                status = c.status;
+            }
+            if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
+                full_recompute(M);
+                M->next_recompute += M->d.recompute_interval;
+            }
+        }
+        sync();
+        if (out) {
+            c = read_ctl(M);
+            out->attempted = c.trials_done;
+            out->stage1_accepted = c.da_stage1;
+            out->accepted = c.accepted;
+            out->rounds = c.da_rounds;
+            out->launches = M->launches;
+            out->nprot = c.nprot;
+            out->nsites = M->nsites;
+            out->status = c.status;
+            out->E = c.E;
+            out->chain_ms = M->chain_ms;
+            out->refresh_ms = M->refresh_ms;
+        }
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_record(phb_mc M, long long n, signed char* fwd, int* i, int* j, signed char* stage,
+                             double* dE_star, double* dE) {
+    // the delayed-acceptance chain stores the stage in the acceptance record and dE* in the uniform record
+    return phb_mc_record(M, n, fwd, i, j, stage, dE, dE_star);
+}

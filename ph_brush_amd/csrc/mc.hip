@@ -477,6 +477,27 @@ using namespace phb;
 
 namespace phb {
 
+long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr) {
+    hipStream_t st = stream();
+    DevBuf<int> deg(N), cursor(N);
+    PHB_HIP(hipMemsetAsync(deg.p, 0, N * sizeof(int), st));
+    PHB_HIP(hipMemsetAsync(cursor.p, 0, N * sizeof(int), st));
+    head.alloc(N + 1);
+    hipLaunchKernelGGL(k_full_degree, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, half->head.p, half->list.p, deg.p);
+    exclusive_scan(deg.p, head.p, N);
+    const long long nfull = 2 * half->nnz;
+    nbr.alloc(std::max<long long>(nfull, 1));
+    if (nfull > 0) {
+        DevBuf<int> tmp(nfull);
+        hipLaunchKernelGGL(k_full_fill, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, half->head.p, half->list.p, head.p,
+                           cursor.p, tmp.p);
+        hipLaunchKernelGGL(k_full_sort, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, head.p, tmp.p, nbr.p);
+        PHB_HIP(hipGetLastError());
+        sync();
+    }
+    return nfull;
+}
+
 McCtl read_ctl(phb_mc_s* M) {
     McCtl c;
     PHB_HIP(hipMemcpyAsync(&c, M->ctl.p, sizeof(McCtl), hipMemcpyDeviceToHost, stream()));
@@ -648,21 +669,10 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         }
         // ---- full rows + coefficients
         hipStream_t st = stream();
-        DevBuf<int> deg(N), cursor(N);
-        PHB_HIP(hipMemsetAsync(deg.p, 0, N * sizeof(int), st));
-        PHB_HIP(hipMemsetAsync(cursor.p, 0, N * sizeof(int), st));
-        M->fhead.alloc(N + 1);
-        hipLaunchKernelGGL(k_full_degree, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, neigh->head.p, neigh->list.p, deg.p);
-        exclusive_scan(deg.p, M->fhead.p, N);
-        const long long nfull = 2 * neigh->nnz;
-        M->fnbr.alloc(std::max<long long>(nfull, 1));
+        const long long nfull = build_full_rows(N, neigh, M->fhead, M->fnbr);
         M->kreal.alloc(std::max<long long>(nfull, 1));
         M->r2.alloc(std::max<long long>(nfull, 1));
         if (nfull > 0) {
-            DevBuf<int> tmp(nfull);
-            hipLaunchKernelGGL(k_full_fill, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, neigh->head.p, neigh->list.p,
-                               M->fhead.p, cursor.p, tmp.p);
-            hipLaunchKernelGGL(k_full_sort, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, M->fhead.p, tmp.p, M->fnbr.p);
             hipLaunchKernelGGL(k_full_coeffs, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, a->x.p, a->y.p, a->z.p,
                                box->Lx, box->Ly, box->Lz, pme->elF, pme->alpha, M->fhead.p, M->fnbr.p, M->kreal.p,
                                M->r2.p);

@@ -30,6 +30,7 @@ struct McCtl {
     int mti;
     int pend_atom[MC_KMAX];
     double pend_dq[MC_KMAX];
+    long long da_stage1, da_rounds;   // delayed-acceptance chain: stage-1 acceptances, rounds
 };
 
 struct McDev {

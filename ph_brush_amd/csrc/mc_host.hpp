@@ -37,7 +37,11 @@ struct phb_mc_s {
     size_t ev_used = 0;
     std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)
     double chain_ms = 0, refresh_ms = 0, recompute_ms = 0;
+    // delayed-acceptance state (descriptor lists and histograms), owned here
+    void* da = nullptr;
+    void (*da_free)(void*) = nullptr;
     ~phb_mc_s() {
+        if (da_free) da_free(da);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
     hipEvent_t event() {
@@ -70,6 +74,8 @@ struct phb_mc_s {
 };
 
 namespace phb {
+// Full (both directions) neighbour rows, ascending, from a half list; returns the entry count.
+long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr);
 McCtl read_ctl(phb_mc_s* M);
 // Full recompute: phi = G Q from scratch, total energy, moments, E reset.
 void full_recompute(phb_mc_s* M);

@@ -381,6 +381,116 @@ class MonteCarlo(_Handle):
         return np.array(st[:], dtype=np.uint32)
 
 
+    # ---- delayed acceptance (no reference counterpart)
+    def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64):
+        sp = np.ascontiguousarray(default_species(NTYPES) if species is None else species, np.int8)
+        self._species = sp
+        d = L.DaDesc(sp.ctypes.data_as(C.POINTER(C.c_byte)), len(sp), float(r_desc), int(candidates))
+        L.call("phb_da_attach", self.h, C.byref(d))
+
+    def da_descriptors(self, i, j):
+        i = np.ascontiguousarray(i, np.int32); j = np.ascontiguousarray(j, np.int32)
+        d = np.zeros((len(i), L.NN_IN), np.int16)
+        r = np.zeros(len(i))
+        L.call("phb_da_descriptors", self.h, len(i), L.iptr(i), L.iptr(j), L.sptr(d), L.dptr(r))
+        return d, r
+
+    def da_collect(self, sur: "Surrogate", i, j, fwd):
+        i = np.ascontiguousarray(i, np.int32); j = np.ascontiguousarray(j, np.int32)
+        f = np.ascontiguousarray(fwd, np.int32)
+        L.call("phb_da_collect", self.h, sur.h, len(i), L.iptr(i), L.iptr(j), L.iptr(f))
+
+    def da_predict(self, sur: "Surrogate", i, j, fwd) -> np.ndarray:
+        i = np.ascontiguousarray(i, np.int32); j = np.ascontiguousarray(j, np.int32)
+        f = np.ascontiguousarray(fwd, np.int32)
+        out = np.zeros(len(i))
+        L.call("phb_da_predict", self.h, sur.h, len(i), L.iptr(i), L.iptr(j), L.iptr(f), L.dptr(out))
+        return out
+
+    def da_run(self, sur: "Surrogate", ntrials: int) -> dict:
+        st = L.DaStats()
+        L.call("phb_da_run", self.h, sur.h, ntrials, C.byref(st))
+        return st.as_dict()
+
+    def da_record(self, n: int) -> dict:
+        out = {"fwd": np.zeros(n, np.int8), "i": np.zeros(n, np.int32), "j": np.zeros(n, np.int32),
+               "stage": np.zeros(n, np.int8), "dE_star": np.zeros(n), "dE": np.zeros(n)}
+        L.call("phb_da_record", self.h, n, out["fwd"].ctypes.data_as(C.POINTER(C.c_byte)), L.iptr(out["i"]),
+               L.iptr(out["j"]), out["stage"].ctypes.data_as(C.POINTER(C.c_byte)), L.dptr(out["dE_star"]),
+               L.dptr(out["dE"]))
+        return out
+
+
+class Surrogate(_Handle):
+    """MLP surrogate of a move's Delta-E with a device dataset (phb_surrogate_*)."""
+    _destroy = "phb_surrogate_destroy"
+
+    def __init__(self, capacity: int = 0, seed: int = 1):
+        super().__init__()
+        L.call("phb_surrogate_create", int(capacity), seed & 0xFFFFFFFF, C.byref(self.h))
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == L.NN_PARAMS
+        L.call("phb_surrogate_set_params", self.h, L.fptr(th))
+
+    def get_params(self) -> np.ndarray:
+        th = np.zeros(L.NN_PARAMS, np.float32)
+        L.call("phb_surrogate_get_params", self.h, L.fptr(th))
+        return th
+
+    def forward(self, d) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.int16)
+        y = np.zeros(len(d), np.float32)
+        L.call("phb_surrogate_forward", self.h, len(d), L.sptr(d), L.fptr(y))
+        return y
+
+    def grad(self, d, t):
+        d = np.ascontiguousarray(d, np.int16)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(L.NN_TRAIN, np.float32)
+        loss = C.c_double()
+        L.call("phb_surrogate_grad", self.h, len(d), L.sptr(d), L.fptr(t), L.fptr(g), C.byref(loss))
+        return loss.value, g
+
+    def add(self, d, t):
+        d = np.ascontiguousarray(d, np.int16)
+        t = np.ascontiguousarray(t, np.float32)
+        L.call("phb_surrogate_data_add", self.h, len(d), L.sptr(d), L.fptr(t))
+
+    def data(self, first: int = 0, n: int | None = None):
+        n = self.size() - first if n is None else n
+        d = np.zeros((n, L.NN_IN), np.int16)
+        t = np.zeros(n, np.float32)
+        L.call("phb_surrogate_data_get", self.h, first, n, L.sptr(d), L.fptr(t))
+        return d, t
+
+    def size(self) -> int:
+        n = C.c_longlong()
+        L.call("phb_surrogate_data_size", self.h, C.byref(n))
+        return n.value
+
+    def clear(self):
+        L.call("phb_surrogate_data_clear", self.h)
+
+    def train(self, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1, n_train: int = 0,
+              normalize: bool = True) -> dict:
+        d = L.TrainDesc(epochs, batch, lr, seed & 0xFFFFFFFF, n_train, int(normalize))
+        st = L.TrainStats()
+        L.call("phb_surrogate_train", self.h, C.byref(d), C.byref(st))
+        return st.as_dict()
+
+    def loss_history(self, n: int) -> np.ndarray:
+        out = np.zeros(n, np.float32)
+        L.call("phb_surrogate_loss_history", self.h, n, L.fptr(out))
+        return out
+
+    def eval(self, first: int, n: int):
+        r, a = C.c_double(), C.c_double()
+        L.call("phb_surrogate_eval", self.h, first, n, C.byref(r), C.byref(a))
+        return r.value, a.value
+
+
 class Replica:
     """One system set up on the device the way main() does (MC/src/main.cu:106-138)."""
 

@@ -1,0 +1,209 @@
+"""GPU parity of the learned surrogate and the delayed-acceptance chain.
+
+Bars:
+  - descriptors (integer shell counts, incrementally maintained histograms):
+    identical to the oracle's brute force, before and after a chain run;
+  - MLP forward on MFMA: within 2e-3 * (1 + |y|) of the CPU restatement with the
+    same BF16 rounding points (differences: FP32 summation order, erff vs erf);
+  - gradient of one batch: relative L2 error below 1e-2 against the restatement
+    (BF16 operands of the backward products rounded at the same points);
+  - one Adam step: parameter change within 5 % of lr of the restatement's for every
+    parameter whose gradient is not tiny;
+  - DA chain: the stage sequence, picked atoms and final identities equal the
+    oracle's DA chain fed the same surrogate values, except at a decision within
+    1e-9 of its threshold; running energy equal to a full recompute;
+  - degree of ionisation alpha(pH) of the DA chain within 0.005 of the plain chain.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp as MLP
+from oracle import pyoracle as O
+from ph_brush_amd import surrogate as SG
+from ph_brush_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _theta(seed):
+    th = MLP.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 1)
+    th[:MLP.NTRAIN] += rng.normal(0, 0.05, MLP.NTRAIN)
+    o = MLP.NTRAIN
+    th[o:o + 48] = rng.uniform(0.05, 0.3, 48)
+    th[o + 48], th[o + 49] = 0.7, 2.5
+    return th.astype(np.float32)
+
+
+def test_mlp_forward_matches_restatement(gpu):
+    th = _theta(11)
+    sur = S.Surrogate(0)
+    sur.set_params(th)
+    d = np.random.default_rng(1).integers(-30, 30, (1000, 48)).astype(np.int16)
+    y = sur.forward(d).astype(np.float64)
+    ref, _ = MLP.forward(th.astype(np.float64), d.astype(np.float64), bf=True)
+    assert np.all(np.abs(y - ref) <= 2e-3 * (1.0 + np.abs(ref))), np.abs(y - ref).max()
+    ref64, _ = MLP.forward(th.astype(np.float64), d.astype(np.float64), bf=False)
+    print("forward: max |gpu - bf16 restatement| =", np.abs(y - ref).max(),
+          " max |gpu - fp64| =", np.abs(y - ref64).max())
+
+
+def test_mlp_gradient_matches_restatement(gpu):
+    th = _theta(12)
+    sur = S.Surrogate(0)
+    sur.set_params(th)
+    rng = np.random.default_rng(2)
+    d = rng.integers(-15, 15, (256, 48)).astype(np.int16)
+    t = rng.normal(0.5, 3.0, 256).astype(np.float32)
+    loss, g = sur.grad(d, t)
+    lo, go = MLP.loss_and_grad(th.astype(np.float64), d.astype(np.float64), t.astype(np.float64), bf=True)
+    assert abs(loss - lo) <= 1e-4 * max(1.0, lo)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    # every block of the gradient, separately
+    o = 0
+    for name, size in (("W1", 6144), ("b1", 128), ("W2", 16384), ("b2", 128), ("w3", 128), ("b3", 1)):
+        gb, ob = g[o:o + size], go[o:o + size]
+        rb = np.linalg.norm(gb - ob) / max(np.linalg.norm(ob), 1e-30)
+        assert rb < 1e-2, (name, rb)
+        o += size
+    assert rel < 1e-2, rel
+
+
+def test_adam_step_matches_restatement(gpu):
+    th = _theta(13)
+    rng = np.random.default_rng(3)
+    d = rng.integers(-15, 15, (512, 48)).astype(np.int16)
+    t = rng.normal(0.0, 2.0, 512).astype(np.float32)
+    sur = S.Surrogate(512)
+    sur.set_params(th)
+    sur.add(d, t)
+    lr = 1e-3
+    st = sur.train(1, batch=512, lr=lr, seed=1, normalize=False)
+    assert st["steps"] == 1
+    th1 = sur.get_params().astype(np.float64)
+    _, go = MLP.loss_and_grad(th.astype(np.float64), d.astype(np.float64), t.astype(np.float64), bf=True)
+    ref, _, _ = MLP.adam_step(th.astype(np.float64), go, np.zeros(MLP.NTRAIN), np.zeros(MLP.NTRAIN), 1, lr)
+    big = np.abs(go) > 1e-3 * np.abs(go).max()
+    dg = (th1 - th.astype(np.float64))[:MLP.NTRAIN][big]
+    dr = (ref - th.astype(np.float64))[:MLP.NTRAIN][big]
+    assert np.all(np.abs(dg - dr) <= 0.05 * lr), np.abs(dg - dr).max()
+    assert np.array_equal(th1[MLP.NTRAIN:], th[MLP.NTRAIN:].astype(np.float64))
+
+
+def _tiny():
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
+    return s, p
+
+
+def test_descriptors_match_brute_force_before_and_after_chain(gpu):
+    s, p = _tiny()
+    mc = S.McParams.at_ph(7.5)
+    r = S.Replica(s, p, mc, seed=3)
+    r.mc.da_attach(r_desc=6.0)
+    sp = S.default_species()
+    rng = np.random.default_rng(4)
+    o = O.Oracle(s, p)
+    for phase in range(2):
+        _, t = r.atoms.identity()
+        i, j, f = SG.sample_moves(rng, t, s.z, mc, 300)
+        dg, rg = r.mc.da_descriptors(i, j)
+        q, _ = r.atoms.identity()
+        o.set_charges(q, t)
+        do, ro = o.da_descriptors(sp, 6.0, i, j)
+        assert np.array_equal(dg.astype(np.int32), do), phase
+        assert np.allclose(rg, ro, rtol=1e-14, atol=0)
+        if phase == 0:
+            sur = S.Surrogate(0)
+            st = r.mc.da_run(sur, 3000)      # untrained network: many stage-2 decisions
+            assert st["accepted"] > 10
+
+
+def _trained_surrogate(s, p, n_per_ph=8192, ph=(6.5, 7.5, 8.5), epochs=30):
+    sur = S.Surrogate(n_per_ph * len(ph))
+    spec = SG.CollectSpec(ph_values=ph, samples_per_ph=n_per_ph, per_state=1024, equilibrate=2000, between=500)
+    SG.collect(s, p, sur, spec)
+    st = SG.train_holdout(sur, epochs=epochs, batch=1024, lr=2e-3)
+    return sur, st
+
+
+def _compare_da(rg, ro, n):
+    for k in range(n):
+        same = (rg["fwd"][k] == ro["fwd"][k] and rg["i"][k] == ro["i"][k] and rg["j"][k] == ro["j"][k]
+                and rg["stage"][k] == ro["stage"][k])
+        if not same:
+            assert rg["fwd"][k] == ro["fwd"][k] and rg["i"][k] == ro["i"][k] and rg["j"][k] == ro["j"][k], \
+                f"proposal diverged at trial {k}"
+            return k
+    return n
+
+
+def test_da_chain_replays_on_oracle(gpu):
+    s, p = _tiny()
+    sur, st = _trained_surrogate(s, p, n_per_ph=4096, epochs=20)
+    mc = S.McParams.at_ph(7.9)
+    n = 3000
+    r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=250, record_cap=n)
+    r.mc.da_attach(r_desc=6.0)
+    res = r.mc.da_run(sur, n)
+    assert res["attempted"] == n and res["status"] == 0
+    rg = r.mc.da_record(n)
+    assert (rg["stage"] == 0).any() and (rg["stage"] == 1).any() and (rg["stage"] == 2).any()
+    assert res["stage1_accepted"] == int((rg["stage"] > 0).sum())
+    o = O.Oracle(s, p, r.bonds_host)
+    o.nlist()
+    ro = o.da_run(mc, n, 5, sur_trial=rg["dE_star"])
+    k = _compare_da(rg, ro, n)
+    if k < n:
+        # a split is only allowed at a stage-2 decision within 1e-9 of its threshold
+        assert ro["stage"][k] > 0 and abs(rg["dE"][k] - ro["dE"][k]) < 1e-6
+    ev = np.flatnonzero(rg["stage"][:k] > 0)
+    assert np.all(np.abs(rg["dE"][ev] - ro["dE"][ev]) <= 1e-10 * np.maximum(1.0, np.abs(ro["dE"][ev]))
+                  + 1e-14 * abs(ro["E"]))
+    if k == n:
+        q, ty = r.atoms.identity()
+        assert np.array_equal(ty, ro["type0"]) and np.array_equal(q, ro["q"])
+    e_run = r.mc.stats()["E"]
+    assert abs(e_run - r.mc.recompute()) <= 1e-9 * max(1.0, abs(e_run))
+
+
+def _alpha_series(run, n_chunks, chunk, nsites):
+    a = np.zeros(n_chunks)
+    for k in range(n_chunks):
+        a[k] = run(chunk)["nprot"] / nsites
+    return a
+
+
+def test_da_alpha_matches_plain_and_surrogate_gains(gpu):
+    """2,640-atom brush (64 chains x 20 titratable beads): alpha(pH) from the DA
+    chain equals the plain chain's; the trained network lets most trials skip
+    the exact Delta-E and beats an untrained one on held-out error."""
+    s, p, _ = S.config_small_fp64()
+    sur, st = _trained_surrogate(s, p, n_per_ph=16384, ph=(7.0, 7.5, 8.0), epochs=30)
+    print("training:", st)
+    assert st["rmse_holdout"] < 0.5 * st["target_std"]
+    untrained = S.Surrogate(0)
+    mc = S.McParams.at_ph(7.5)
+    res = {}
+    for name in ("plain", "da", "da_untrained"):
+        r = S.Replica(s, p, mc, seed=9, pending_max=8, recompute_interval=1000)
+        if name != "plain":
+            r.mc.da_attach(r_desc=6.0)
+        run = (lambda n, r=r: r.mc.run(n)) if name == "plain" else \
+              (lambda n, r=r, sv=(sur if name == "da" else untrained): r.mc.da_run(sv, n))
+        run(100000)
+        chunks = 300 if name != "da_untrained" else 30
+        a = _alpha_series(run, chunks, 2000, r.mc.stats()["nsites"])
+        stt = r.mc.stats()
+        res[name] = {"alpha": a.mean(), "sem": a.std() / math.sqrt(len(a) / 10), "stats": stt}
+        if name != "plain":
+            dst = r.mc.da_run(sur if name == "da" else untrained, 0)
+            res[name]["exact_per_trial"] = dst["stage1_accepted"] / dst["attempted"]
+    print({k: (v["alpha"], v["sem"], v.get("exact_per_trial")) for k, v in res.items()})
+    assert abs(res["da"]["alpha"] - res["plain"]["alpha"]) < 0.005
+    # the trained surrogate screens out most trials; without training the exact path runs far more often
+    assert res["da"]["exact_per_trial"] < 0.5
+    assert res["da"]["exact_per_trial"] < res["da_untrained"]["exact_per_trial"] or \
+        res["da"]["stats"]["accepted"] / res["da"]["stats"]["attempted"] > \
+        res["da_untrained"]["stats"]["accepted"] / res["da_untrained"]["stats"]["attempted"]
