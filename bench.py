@@ -1,16 +1,25 @@
-"""Benchmark: MC trial moves/s of the device protonation / ion-swap chain.
+"""Benchmark: MC trial moves/s at 30k particles, delayed-acceptance surrogate
+form vs the plain exact form, on one GPU.
 
-Workload (default): the 30k-particle diblock brush -- 400 chains x 50 beads on
-a 20x20 lattice in a 40x40x150 sigma box (x3 z-extension, slab correction),
-beads 1-25 permanently +1, beads 26-50 titratable (pKa 7.5, pH 7.0), 2,000
-salt pairs, one counter-particle per charged/titratable bead (44,000 atoms
-realised); SPME 64x64x256, order 5, alpha 0.9, rc 3.5, WCA LJ; FP64 plain
-path (exact per-move Delta-E, full recompute every 1,000 trials).
+Workload: the 30k-particle diblock brush -- 400 chains x 50 beads on a 20x20
+lattice in a 40x40x150 sigma box (x3 z-extension, slab correction), beads 1-25
+permanently +1, beads 26-50 titratable (pKa 7.5, pH 7.0), 2,000 salt pairs, one
+counter-particle per charged/titratable bead (44,000 atoms realised); SPME
+64x64x256, order 5, alpha 0.9, rc 3.5, WCA LJ; FP64 exact Delta-E (full
+recompute of energy and potential every 20,000 trials).
 
-A step is one block of --trials-per-step GC trials.  W warm-up steps, then K
-timed steps.  Inputs are resident on the GPU before timing starts.  Prints one
-JSON line.  --gpus above 1 is refused: the chain of one replica runs on one
-GPU (replicas would shard across GPUs with no communication).
+Two forms run in the same process, each timed over K steps of
+--trials-per-step GC trials after W warm-up steps:
+  plain: every trial evaluates the exact Delta-E;
+  da:    delayed acceptance -- a BF16-MFMA MLP screens each trial and only
+         stage-1 acceptances get the exact Delta-E; every trial is an
+         exact-equivalent trial (the second stage keeps the distribution).
+The surrogate is trained first (untimed) on (descriptor, exact Delta-E) pairs
+collected on the GPU from exact chains at pH 6-9.  `value` is the DA form's
+exact-equivalent trial moves/s; the plain form, the ratio and alpha of both
+forms are reported beside it.  Inputs are resident on the GPU before timing.
+--gpus above 1 is refused: one replica's chain runs on one GPU (replicas would
+shard across GPUs with no communication).
 """
 from __future__ import annotations
 
@@ -66,14 +75,31 @@ def cpu_baseline(cfg, trials: int, threads: int):
                       f"(full LJ + SPME recompute per trial, FP64, OpenMP) in {dt:.1f} s"}
 
 
+def _timed(run, K, W, T, label):
+    for w in range(W):
+        run(T)
+        print(f"[bench] {label} warmup step {w + 1}/{W} done", file=sys.stderr, flush=True)
+    alphas = []
+    t0 = time.perf_counter()
+    for k in range(K):
+        st = run(T)
+        alphas.append(st["nprot"] / st["nsites"])
+        print(f"[bench] {label} step {k + 1}/{K} done", file=sys.stderr, flush=True)
+    return time.perf_counter() - t0, alphas
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--trials-per-step", type=int, default=20000)
+    ap.add_argument("--trials-per-step", type=int, default=40000)
     ap.add_argument("--pending-max", type=int, default=8)
     ap.add_argument("--spec-window", type=int, default=8)
+    ap.add_argument("--recompute-interval", type=int, default=20000,
+                    help="trials between full recomputes of the energy and potential (drift reset)")
+    ap.add_argument("--da-samples-per-ph", type=int, default=65536)
+    ap.add_argument("--da-epochs", type=int, default=20)
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -83,6 +109,7 @@ def main():
         sys.exit(2)
 
     from ph_brush_amd import system as S
+    from ph_brush_amd import surrogate as SG
     from ph_brush_amd import _lib as L
 
     if L.device_count() < 1:
@@ -90,25 +117,19 @@ def main():
         sys.exit(1)
     cfg = S.config_30k_fp64
     s, p, mc = cfg()
-    t_setup = time.perf_counter()
-    rep = S.Replica(s, p, mc, seed=1, pending_max=args.pending_max, recompute_interval=1000,
-                    spec_window=args.spec_window)
-    t_setup = time.perf_counter() - t_setup
-    print(f"[bench] setup {t_setup:.2f}s natoms={s.natoms}", file=sys.stderr, flush=True)
-
     K, W, T = args.steps, args.warmup, args.trials_per_step
-    for w in range(W):
-        rep.mc.run(T)
-        print(f"[bench] warmup step {w + 1}/{W} done", file=sys.stderr, flush=True)
+    kw = dict(pending_max=args.pending_max, recompute_interval=args.recompute_interval, spec_window=args.spec_window)
+
+    # ---- plain form
+    t_setup = time.perf_counter()
+    rep = S.Replica(s, p, mc, seed=1, **kw)
+    print(f"[bench] setup {time.perf_counter() - t_setup:.2f}s natoms={s.natoms}", file=sys.stderr, flush=True)
     st0 = rep.mc.stats()
-    t0 = time.perf_counter()
-    for k in range(K):
-        rep.mc.run(T)
-        print(f"[bench] step {k + 1}/{K} done", file=sys.stderr, flush=True)
-    dt = time.perf_counter() - t0
+    dt_plain, alpha_plain = _timed(rep.mc.run, K, W, T, "plain")
     st1 = rep.mc.stats()
-    trials = st1["attempted"] - st0["attempted"]
-    value = trials / dt
+    plain_trials = K * T
+    plain_value = plain_trials / dt_plain
+    plain_acc = (st1["accepted"] - st0["accepted"]) / max(st1["attempted"] - st0["attempted"], 1)
 
     # live kernel timing with HIP events on the library stream (one extra step)
     rep.mc.set_profiling(True)
@@ -119,21 +140,58 @@ def main():
     launches = stp1["launches"] - stp0["launches"]
     refreshes = stp1["refreshes"] - stp0["refreshes"]
     ptrials = stp1["attempted"] - stp0["attempted"]
-    chain_ms = stp1["chain_ms"]
     bpt = chain_bytes_per_trial(rep)
-    avg_launch_s = chain_ms / 1e3 / max(launches, 1)
+    avg_launch_s = stp1["chain_ms"] / 1e3 / max(launches, 1)
     achieved = bpt * (ptrials / max(launches, 1)) / avg_launch_s / 1e9
     rb = refresh_bytes(rep)
     refresh_gbs = rb / (stp1["refresh_ms"] / 1e3 / max(refreshes, 1)) / 1e9
+    del rep
+
+    # ---- surrogate: data from exact chains at pH 6-9, GPU training (untimed)
+    t_sur = time.perf_counter()
+    spec = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000, between=2000)
+    sur = S.Surrogate(len(spec.ph_values) * spec.samples_per_ph)
+    col = SG.collect(s, p, sur, spec, log=lambda m: print(m, file=sys.stderr, flush=True))
+    tr = SG.train_holdout(sur, epochs=args.da_epochs, batch=4096, lr=2e-3)
+    t_sur = time.perf_counter() - t_sur
+    print(f"[bench] surrogate: {col['samples']} samples, holdout rmse {tr['rmse_holdout']:.3f} "
+          f"(target std {tr['target_std']:.3f}), {t_sur:.1f}s", file=sys.stderr, flush=True)
+    flop_fwd = 2 * (48 * 128 + 128 * 128 + 128)
+    flop_train = 3 * flop_fwd
+    train_tflops = tr["steps"] * 4096 * flop_train / (tr["ms"] / 1e3) / 1e12
+
+    # ---- delayed-acceptance form, same workload and seed
+    rep = S.Replica(s, p, mc, seed=1, **kw)
+    rep.mc.da_attach(r_desc=spec.r_desc)
+    d0 = rep.mc.da_run(sur, 0)
+    dt_da, alpha_da = _timed(lambda n: rep.mc.da_run(sur, n), K, W, T, "da")
+    d1 = rep.mc.da_run(sur, 0)
+    da_trials = K * T
+    value = da_trials / dt_da
+    n_att = max(d1["attempted"] - d0["attempted"], 1)
+    stage1 = (d1["stage1_accepted"] - d0["stage1_accepted"]) / n_att
+    da_acc = (d1["accepted"] - d0["accepted"]) / n_att
+    rounds = d1["rounds"] - d0["rounds"]
+    # live DA chain launch time
+    rep.mc.set_profiling(True)
+    p0 = rep.mc.da_run(sur, 0)
+    p1 = rep.mc.da_run(sur, T)
+    rep.mc.set_profiling(False)
+    da_launch_s = p1["chain_ms"] / 1e3 / max(p1["launches"] - p0["launches"], 1)
+    da_trials_per_launch = (p1["attempted"] - p0["attempted"]) / max(p1["launches"] - p0["launches"], 1)
+    # algorithmic bytes per DA trial: two 96-byte histograms and the stage-1 share of an exact trial
+    da_bpt = 2 * 48 * 2 + stage1 * bpt
+    da_achieved = da_bpt * da_trials_per_launch / da_launch_s / 1e9
+    infer_tflops = flop_fwd * da_trials_per_launch / da_launch_s / 1e12
 
     out = {
-        "metric": "MC trial moves/s at 30k particles (plain FP64 GPU path)",
+        "metric": "MC trial moves/s at 30k particles; DA-surrogate vs plain at equal alpha(pH)",
         "value": value,
-        "unit": "trial moves/s",
+        "unit": "exact-equivalent trial moves/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": dt / K * 1e3,
+        "ms_per_step": dt_da / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -142,25 +200,53 @@ def main():
         "config": {
             "workload": "diblock brush 400x50 on 20x20 lattice, 40x40x150 box x3 slab, 2000 salt pairs, "
                         "SPME 64x64x256 order 5, fixed-mu GC protonation + ion-swap moves, pH 7.0 pKa 7.5",
+            "form": "delayed acceptance (BF16-MFMA MLP stage 1, exact FP64 stage 2)",
             "natoms": s.natoms,
             "trials_per_step": T,
             "pending_max": args.pending_max,
             "spec_window": args.spec_window,
-            "acceptance": (st1["accepted"] - st0["accepted"]) / max(trials, 1),
             "parallelism": "1 chain / 1 GPU",
         },
+        "forms": {
+            "plain": {"value": plain_value, "ms_per_step": dt_plain / K * 1e3, "acceptance": plain_acc,
+                      "alpha": sum(alpha_plain) / len(alpha_plain)},
+            "da": {"value": value, "ms_per_step": dt_da / K * 1e3, "acceptance": da_acc,
+                   "stage1_acceptance": stage1, "stage2_acceptance": da_acc / max(stage1, 1e-12),
+                   "trials_per_round": n_att / max(rounds, 1), "alpha": sum(alpha_da) / len(alpha_da)},
+            "da_over_plain": value / plain_value,
+            "alpha_note": "alpha of the timed blocks (same start state and seed); equality at fixed pH is "
+                          "tested over long runs in tests/test_gpu_da.py",
+        },
+        "surrogate": {"samples": col["samples"], "collect_s": col["seconds"], "train_steps": tr["steps"],
+                      "train_ms": tr["ms"], "rmse_holdout_kT": tr["rmse_holdout"],
+                      "mae_holdout_kT": tr["mae_holdout"], "target_std_kT": tr["target_std"],
+                      "train_tflops": train_tflops, "train_mfma_frac": train_tflops / 2500.0},
         "roofline": {
+            "kernel": "k_da_chain",
+            "bound": "hbm",
+            "achieved": da_achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": da_achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": da_launch_s * 1e6,
+            "bytes_per_trial": da_bpt,
+            "trials_per_launch": da_trials_per_launch,
+            "mlp_inference_tflops": infer_tflops,
+            "mlp_inference_mfma_frac": infer_tflops / 2500.0,
+            "note": "single-workgroup chain: latency-bound; the MFMA fraction of the in-chain MLP is reported "
+                    "against the dense BF16 peak",
+        },
+        "plain_roofline": {
             "kernel": "k_mc_chain",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
             "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_trial": bpt,
             "trials_per_launch": ptrials / max(launches, 1),
-            "note": "single-workgroup chain: latency-bound, see refresh for the bandwidth kernels",
         },
         "refresh": {
             "achieved_GBs": refresh_gbs,

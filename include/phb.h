@@ -218,9 +218,11 @@ int phb_gen_brush(const phb_brush_spec *spec, int cap, double *x, double *y, dou
  * No counterpart in the reference (it has no learned part).  A move's
  * descriptor is d = h_i - h_j: 48 counts per centre (12 shells of r_desc/12 x
  * species 0 charged bead, 1 neutral titratable bead, 2 cation, 3 anion), each
- * centre excluding the other.  The network f (48 -> 128 -> 128 -> 1, GELU,
- * BF16 MFMA with FP32 accumulation) predicts t = s dE + elF / r_ij, so that
- * dE* = s (f(d) - elF / r_ij) is antisymmetric under the reverse move.
+ * centre excluding the other.  The surrogate is dE* = B + s f(d): B the exact
+ * reciprocal-space potential term sum_a dq_a W_a . phi, the mesh self terms, a
+ * pair term and the O(1) corrections; f the network (48 -> 128 -> 128 -> 1,
+ * GELU, BF16 MFMA with FP32 accumulation) trained on t = s (dE - B), s = +1
+ * forward.
  * Flat FP32 parameter layout: W1[48][128], b1[128], W2[128][128], b2[128],
  * w3[128], b3 (PHB_NN_TRAIN trainable values), then in_scale[48], out_mean,
  * out_std. */
@@ -272,7 +274,7 @@ typedef struct {
 int phb_da_attach(phb_mc mc, const phb_da_desc *d);
 /* Descriptors (n x 48) and pair distances of moves in the chain's current state. */
 int phb_da_descriptors(phb_mc mc, int n, const int *i, const int *j, short *d, double *rij);
-/* Appends (descriptor, s dE + elF / r_ij) of n moves, dE exact, to the dataset. */
+/* Appends (descriptor, s (dE - B)) of n moves, dE exact, to the dataset. */
 int phb_da_collect(phb_mc mc, phb_surrogate s, int n, const int *i, const int *j, const int *fwd);
 /* Surrogate dE* of n moves. */
 int phb_da_predict(phb_mc mc, phb_surrogate s, int n, const int *i, const int *j, const int *fwd, double *dE_star);
@@ -281,13 +283,16 @@ typedef struct {
     int nprot, nsites, status;
     double E;
     double chain_ms, refresh_ms;         /* with phb_mc_set_profiling */
+    double phase_us[8];                  /* with profiling: chain time by phase (launch prologue, proposals,
+                                            descriptors, MLP, stage 1, exact dE, stage 2 + updates, epilogue) */
 } phb_da_stats;
 /* Runs ntrials delayed-acceptance trials: stage 1 on dE*, stage 2 on the exact dE,
  * same proposals, pools and engine stream as phb_mc_run. */
 int phb_da_run(phb_mc mc, phb_surrogate s, long long ntrials, phb_da_stats *out);
-/* Per-trial record: stage 0 rejected at stage 1, 1 accepted, 2 rejected at stage 2; dE NaN when not evaluated. */
+/* Per-trial record: stage 0 rejected at stage 1, 1 accepted, 2 rejected at stage 2; dE* of the move and of its
+ * reverse from the proposed state (NaN when stage 1 rejected); dE NaN when not evaluated. */
 int phb_da_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, signed char *stage, double *dE_star,
-                  double *dE);
+                  double *dE_star_rev, double *dE);
 
 #ifdef __cplusplus
 }

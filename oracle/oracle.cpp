@@ -687,8 +687,9 @@ void or_da_descriptors(const or_system* s, const or_desc_params* dp, int n, cons
 }
 
 int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,
-              long long ntrials, unsigned seed, const double* sur_trial, const double* sur_pair, or_da_record* rec,
-              double* q_out, int* type_out, double* E_out, long long* n_done) {
+              long long ntrials, unsigned seed, const double* sur_trial, const double* sur_rev_trial,
+              const double* sur_pair, const double* sur_pair_sym, or_da_record* rec, double* q_out, int* type_out,
+              double* E_out, long long* n_done) {
     const int N = s0->natoms;
     std::vector<double> q(s0->q, s0->q + N);
     std::vector<int> type(s0->type, s0->type + N);
@@ -731,7 +732,10 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
         if (forward) { i = choose_one(nh_n); j = choose_one(w); }
         else { i = choose_one(nh_p); j = choose_one(cl); }
         const int sg = forward ? 1 : -1;
-        const double dEs = sur_trial ? sur_trial[t] : (double)sg * sur_pair[(size_t)i * N + j];
+        const double sym = sur_pair_sym ? sur_pair_sym[(size_t)i * N + j] : 0.0;
+        const double dEs = sur_trial ? sur_trial[t] : (double)sg * sur_pair[(size_t)i * N + j] + sym;
+        const double dEr = sur_trial ? (sur_rev_trial ? sur_rev_trial[t] : -sur_trial[t])
+                                     : -(double)sg * sur_pair[(size_t)i * N + j] + sym;
         long long S_after = forward ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
         double logH = 0.0;
         if (S_before > 0 && S_after > 0) logH = std::log((double)S_before) - std::log((double)S_after);
@@ -748,7 +752,10 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
             else { type[i] = m->nh_n_type - 1; q[i] = 0.0; type[j] = m->w_type - 1; q[j] = 0.0; }
             const double E_trial = or_total_energy(&s, p, head, list, nullptr);
             dE = E_trial - E;
-            const double e2 = ((double)(-m->beta * (dE - sg * m->mu_eff)) + logH) - e1;
+            double r1 = (double)(-m->beta * (dEr + sg * m->mu_eff)) - logH;
+            r1 = std::clamp(r1, -40.0, 40.0);
+            const double e2 = (((double)(-m->beta * (dE - sg * m->mu_eff)) + logH) + std::min(0.0, r1)) -
+                              std::min(0.0, e1);
             bool acc2 = e2 >= 0.0;
             if (!acc2) acc2 = std::exp(e2) > uni(rng);
             if (acc2) {
@@ -772,6 +779,7 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
             if (rec->j) rec->j[t] = j;
             if (rec->stage) rec->stage[t] = (signed char)stage;
             if (rec->dE_star) rec->dE_star[t] = dEs;
+            if (rec->dE_star_rev) rec->dE_star_rev[t] = stage ? dEr : std::numeric_limits<double>::quiet_NaN();
             if (rec->dE) rec->dE[t] = dE;
         }
     }

@@ -126,18 +126,26 @@ typedef struct {
     int *i, *j;
     signed char *stage;        /* 0 rejected at stage 1, 1 accepted, 2 rejected at stage 2 */
     double *dE_star;           /* surrogate Delta-E used at stage 1 */
+    double *dE_star_rev;       /* surrogate Delta-E of the reverse move (NaN when stage 1 rejected) */
     double *dE;                /* exact Delta-E (NaN when stage 1 rejected) */
 } or_da_record;
 
 /* Delayed-acceptance GC chain (Christen & Fox 2005) with the proposal, pools
- * and draw order of or_gc_run.  Stage 1: e1 = clamp(-beta(dE* - s mu) + logH,
- * +-40), accept if e1 >= 0 or exp(e1) > u1 (u1 drawn only when e1 < 0).
- * Stage 2 (exact Delta-E): e2 = (-beta(dE - s mu) + logH) - e1, accept if
- * e2 >= 0 or exp(e2) > u2 (u2 drawn only when e2 < 0).  dE* comes from
- * sur_trial[t] when given, else s * sur_pair[i*natoms + j] (s = +1 forward). */
+ * and draw order of or_gc_run.  The surrogate gives dE*(x->y) for the proposed
+ * move and dE*(y->x) for its reverse from the proposed state.
+ *   Stage 1: e1 = clamp(-beta(dE*(x->y) - s mu) + logH, +-40); accept if
+ *            e1 >= 0 or exp(e1) > u1 (u1 drawn only when e1 < 0).
+ *   Stage 2: r1 = clamp(-beta(dE*(y->x) + s mu) - logH, +-40),
+ *            e2 = (-beta(dE - s mu) + logH) + min(0, r1) - min(0, e1);
+ *            accept if e2 >= 0 or exp(e2) > u2 (u2 drawn only when e2 < 0).
+ * Surrogate sources: per trial, sur_trial[t] and sur_rev_trial[t] (NULL
+ * reverse = -sur_trial[t]); else from N x N tables, dE*(x->y) = s A[i,j] +
+ * B[i,j] and dE*(y->x) = -s A[i,j] + B[i,j] (sur_pair = A, sur_pair_sym = B or
+ * NULL for zero), s = +1 forward. */
 int or_da_run(const or_system *s, const or_params *p, const or_mc_params *m,
               const int *head, const int *list, long long ntrials, unsigned seed,
-              const double *sur_trial, const double *sur_pair, or_da_record *rec,
+              const double *sur_trial, const double *sur_rev_trial, const double *sur_pair,
+              const double *sur_pair_sym, or_da_record *rec,
               double *q_out, int *type_out, double *E_out, long long *n_done);
 
 void or_set_threads(int n);

@@ -47,7 +47,8 @@ class OrDescParams(C.Structure):
 
 
 class OrDaRecord(C.Structure):
-    _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE", _pd)]
+    _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE_star_rev", _pd),
+                ("dE", _pd)]
 
 
 _lib = None
@@ -91,8 +92,8 @@ def load():
                                   C.POINTER(C.c_longlong)]
         lib.or_da_run.restype = C.c_int
         lib.or_da_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
-                                  C.c_longlong, C.c_uint, _pd, _pd, C.POINTER(OrDaRecord), _pd, _pi, _pd,
-                                  C.POINTER(C.c_longlong)]
+                                  C.c_longlong, C.c_uint, _pd, _pd, _pd, _pd, C.POINTER(OrDaRecord), _pd, _pi,
+                                  _pd, C.POINTER(C.c_longlong)]
         lib.or_desc_thresholds.argtypes = [C.c_double, _pd]
         lib.or_da_descriptors.argtypes = [C.POINTER(OrSystem), C.POINTER(OrDescParams), C.c_int, _pi, _pi, _pi, _pd]
         lib.or_mt_draws.argtypes = [C.c_uint, C.c_longlong, _pu]
@@ -212,24 +213,24 @@ class Oracle:
                                    d.ctypes.data_as(_pi), rij.ctypes.data_as(_pd))
         return d, rij
 
-    def da_run(self, mc, ntrials, seed, sur_trial=None, sur_pair=None):
-        """Delayed-acceptance chain with the surrogate given per trial or as an
-        N x N antisymmetric-by-direction table."""
+    def da_run(self, mc, ntrials, seed, sur_trial=None, sur_rev_trial=None, sur_pair=None, sur_pair_sym=None):
+        """Delayed-acceptance chain with the surrogate given per trial (forward and
+        reverse values) or as N x N tables (antisymmetric part A, symmetric part B)."""
         N = self.s.natoms
         rec = {k: np.zeros(ntrials, dt) for k, dt in
                (("fwd", np.int8), ("i", np.int32), ("j", np.int32), ("stage", np.int8), ("dE_star", np.float64),
-                ("dE", np.float64))}
+                ("dE_star_rev", np.float64), ("dE", np.float64))}
         r = OrDaRecord(*(rec[k].ctypes.data_as(t) for k, t in
-                         (("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE", _pd))))
+                         (("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd),
+                          ("dE_star_rev", _pd), ("dE", _pd))))
         m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
-        st_ = np.ascontiguousarray(sur_trial, dtype=np.float64) if sur_trial is not None else None
-        sp_ = np.ascontiguousarray(sur_pair, dtype=np.float64) if sur_pair is not None else None
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (sur_trial, sur_rev_trial, sur_pair, sur_pair_sym)]
         q, t = np.zeros(N), np.zeros(N, np.int32)
         E = C.c_double()
         nd = C.c_longlong()
         st = self.lib.or_da_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), ntrials, seed,
-                                st_.ctypes.data_as(_pd) if st_ is not None else None,
-                                sp_.ctypes.data_as(_pd) if sp_ is not None else None, C.byref(r),
+                                *(a.ctypes.data_as(_pd) if a is not None else None for a in arrs), C.byref(r),
                                 q.ctypes.data_as(_pd), t.ctypes.data_as(_pi), C.byref(E), C.byref(nd))
         n = nd.value
         return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}

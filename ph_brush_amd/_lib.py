@@ -98,10 +98,13 @@ class DaDesc(C.Structure):
 class DaStats(C.Structure):
     _fields_ = [("attempted", C.c_longlong), ("stage1_accepted", C.c_longlong), ("accepted", C.c_longlong),
                 ("rounds", C.c_longlong), ("launches", C.c_longlong), ("nprot", C.c_int), ("nsites", C.c_int),
-                ("status", C.c_int), ("E", C.c_double), ("chain_ms", C.c_double), ("refresh_ms", C.c_double)]
+                ("status", C.c_int), ("E", C.c_double), ("chain_ms", C.c_double), ("refresh_ms", C.c_double),
+                ("phase_us", C.c_double * 8)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["phase_us"] = list(self.phase_us)
+        return d
 
 
 NN_IN, NN_HIDDEN, NN_TRAIN, NN_PARAMS = 48, 128, 22913, 22963
@@ -168,7 +171,7 @@ SIGNATURES = {
     "phb_da_collect": [_h, _h, C.c_int, _pi, _pi, _pi],
     "phb_da_predict": [_h, _h, C.c_int, _pi, _pi, _pi, _pd],
     "phb_da_run": [_h, _h, C.c_longlong, C.POINTER(DaStats)],
-    "phb_da_record": [_h, C.c_longlong, C.POINTER(C.c_byte), _pi, _pi, C.POINTER(C.c_byte), _pd, _pd],
+    "phb_da_record": [_h, C.c_longlong, C.POINTER(C.c_byte), _pi, _pi, C.POINTER(C.c_byte), _pd, _pd, _pd],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

@@ -8,19 +8,28 @@
 // centre (12 shells of r_desc/12 x 4 species), each centre excluding the other
 // one, so a move and its reverse see the same d.  Per-atom histograms are kept
 // in HBM (uint16 x 48) and updated incrementally from the accepted atoms'
-// r_desc rows.  Surrogate: dE* = s (f(d) - elF / r_ij), s = +1 forward, with
-// f the MLP 48 -> 128 -> 128 -> 1 (GELU; BF16 weights and activations on MFMA,
-// FP32 accumulation).  It is antisymmetric by construction, so the second
-// stage reduces to min(1, exp(-beta (dE - dE*))) up to the +-40 clamp
-// (Christen & Fox 2005):  e2 = (-beta (dE - s mu) + logH) - clamp(e1).
+// r_desc rows.  f is the MLP 48 -> 128 -> 128 -> 1 (GELU; BF16 weights and
+// activations on MFMA, FP32 accumulation); the surrogate is dE* = B + s f(d)
+// with B the physics terms below (s = +1 forward).  Second stage (Christen &
+// Fox 2005), with r1 the clamped stage-1 exponent of the reverse move:
+//   e2 = (-beta (dE - s mu) + logH) + min(0, r1) - min(0, e1).
+//
+// The surrogate adds to f the physics it can get exactly and cheaply: the
+// reciprocal-space potential at the two atoms from the maintained potential
+// grid, the mesh self terms, a pair term and the O(1) corrections (k_da_base).
+// That part does not flip sign under the reverse move, so stage 2 uses the
+// general rule with the reverse move's stage-1 probability, computed from the
+// same quantities the exact Delta-E yields.
 //
 // Chain round (one workgroup of 16 waves): wave 0 draws up to 64 proposals
 // assuming each is rejected at stage 1; all waves build their descriptors and
-// run the MLP on MFMA; lane k takes candidate k's stage-1 decision, a ballot
-// finds the first stage-1 acceptance; only that candidate gets the exact
-// Delta-E (all 16 waves), its stage-2 decision, and on acceptance the state,
-// pool, pending-list and histogram updates.  The engine's stream position then
-// rolls back to just after that candidate's own draws, as in the plain chain.
+// physics terms and run the MLP on MFMA; lane k takes candidate k's stage-1
+// decision, a ballot finds the first stage-1 acceptance; only that candidate
+// gets the exact Delta-E (all 16 waves), its stage-2 decision, and on
+// acceptance the state, pool and histogram updates.  The engine's stream
+// position then rolls back to just after that candidate's own draws, as in the
+// plain chain.  A launch ends at its accepted move; the refresh that follows
+// folds it into the potential grid.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -42,7 +51,10 @@ constexpr int NN_TRAIN = OFF_B3 + 1;                      // 22,913
 constexpr int OFF_SCALE = NN_TRAIN, OFF_MEAN = OFF_SCALE + NN_IN, OFF_STD = OFF_MEAN + 1;
 constexpr int NN_PARAM = OFF_STD + 1;                     // 22,963
 constexpr int NTP = NN_TRAIN + 1;                         // partial-gradient record: gradient + loss
-constexpr int DA_M = 64;                                  // candidates per round
+constexpr int DA_M = 64;                                  // candidates per round (max)
+constexpr int DA_THREADS = 512;                           // chain workgroup: 8 waves, 256 VGPRs per lane
+constexpr int DA_WAVES = DA_THREADS / 64;
+constexpr int DA_FEAT = (DA_M * 48 + DA_THREADS - 1) / DA_THREADS;   // descriptor features per thread
 constexpr int XP = NN_IN + 8;                             // LDS row pitch (bf16) of input tiles
 constexpr int HP = NN_H + 8;                              // LDS row pitch (bf16) of hidden tiles
 constexpr int DA_SHIFT = 24;                              // r_desc row entry: atom | shell << 24
@@ -380,7 +392,22 @@ struct DaDev {
     const double *x, *y;
     double Lx, Ly, Lz, rd2;
     double th[12];                // shell thresholds (s r_desc / 12)^2
+    double rc2;                   // real-space cutoff^2 of the pair term
     int M;                        // candidates per round
+    int prof;                     // accumulate per-phase times into McCtl::da_prof
+    double* rec_fy;               // per-trial record of the reverse surrogate dE*(y->x)
+};
+
+// Phase clock of the chain (100 MHz real-time counter), thread 0 only.
+struct PhaseClock {
+    long long t = 0;
+    __device__ __forceinline__ void start(bool on) { if (on) t = (long long)__builtin_amdgcn_s_memrealtime(); }
+    __device__ __forceinline__ void lap(bool on, long long* acc, int k) {
+        if (!on) return;
+        const long long n = (long long)__builtin_amdgcn_s_memrealtime();
+        acc[k] += n - t;
+        t = n;
+    }
 };
 
 __device__ __forceinline__ int shell_of(const DaDev& A, double r2) {
@@ -453,20 +480,89 @@ __global__ void k_da_desc(DaDev A, McDev D, int n, const int* __restrict__ mi, c
     if (c == 0 && rij) rij[m] = r;
 }
 
-// Training target t = s dE + elF / r_ij and the surrogate dE* = s (y - elF / r_ij).
-__global__ void k_da_target(int n, const int* __restrict__ mf, const double* __restrict__ dE,
-                            const double* __restrict__ rij, double elF, float* __restrict__ t) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n) return;
-    t[m] = (float)((mf[m] ? dE[m] : -dE[m]) + elF / rij[m]);
+// ---------------------------------------------------------------- physics part of the surrogate
+// dE*(x -> y) = lin + selfq + pr + corr + s f(d) for the move of site i and ion j:
+//   lin   = sum_a dq_a W_a . phi        reciprocal-space potential at the two atoms (phi exact:
+//                                       no accepted change is pending while the surrogate runs)
+//   selfq = 0.5 sum_a dq_a^2 W_a^T G W_a
+//   pr    = (qi' qj' - qi qj) elF erfc(alpha r)/r [r <= rc] + dqi dqj elF erf(alpha r)/r
+//           (real-space pair, and the mesh pair term in its continuum form)
+//   corr  = the exact self, net-charge and slab change
+// and f the MLP of the shell-count descriptor, which learns the rest (the
+// real-space energy of the environment).
+struct MoveTerms {
+    double dqi, dqj, selfq, perfc, perf, corr;
+};
+
+__device__ __forceinline__ MoveTerms move_terms(const McDev& D, const DaDev& A, int i, int j, int fwd, double r,
+                                                double Q, double Mz, double Qz2) {
+    MoveTerms m;
+    const double qi = D.q[i], qj = D.q[j];
+    const double qin = fwd ? 1.0 : 0.0, qjn = fwd ? -1.0 : 0.0;
+    m.dqi = qin - qi;
+    m.dqj = qjn - qj;
+    m.selfq = 0.5 * (m.dqi * m.dqi * D.selfw[i] + m.dqj * m.dqj * D.selfw[j]);
+    // the pair term is part of the surrogate only: single precision is enough
+    const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
+    m.perfc = r * r <= A.rc2 ? (double)((float)(qin * qjn - qi * qj) * kr * ec) : 0.0;
+    m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
+    m.corr = corr_move(D, qi, qin, qj, qjn, D.z[i], D.z[j], Q, Mz, Qz2);
+    return m;
 }
 
-__global__ void k_da_surrogate(int n, const int* __restrict__ mf, const float* __restrict__ y,
-                               const double* __restrict__ rij, double elF, double* __restrict__ out) {
+// One (z, y) row r < P^2 of W_a . phi.
+template <int P>
+__device__ __forceinline__ double stencil_row(const McDev& D, int a, int r) {
+    const int wz = r / P, wy = r - wz * P;
+    const int* sb = D.sbase + 3 * a;
+    const double* w = D.sw + (size_t)a * 3 * P;
+    const int gz = wrap1(sb[2] + wz, D.nz), gy = wrap1(sb[1] + wy, D.ny);
+    const double* row = D.phi + ((size_t)gz * D.ny + gy) * D.nx;
+    double sx = 0.0;
+#pragma unroll
+    for (int u = 0; u < P; ++u) sx += w[u] * row[wrap1(sb[0] + u, D.nx)];
+    return w[2 * P + wz] * w[P + wy] * sx;
+}
+
+template <int P>
+__device__ double stencil_dot(const McDev& D, int a) {
+    double s = 0.0;
+    for (int r = 0; r < P * P; ++r) s += stencil_row<P>(D, a, r);
+    return s;
+}
+
+// Physics part B = lin + selfq + pr + corr of n moves (one thread per move).
+template <int P>
+__global__ void k_da_base(McDev D, DaDev A, int n, const int* __restrict__ mi, const int* __restrict__ mj,
+                          const int* __restrict__ mf, double* __restrict__ base) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
-    const double f = (double)y[m] - elF / rij[m];
-    out[m] = mf[m] ? f : -f;
+    const McCtl* C = D.ctl;
+    const int i = mi[m], j = mj[m];
+    double r;
+    int exA, exB;
+    pair_terms(A, D, i, j, r, exA, exB);
+    const MoveTerms t = move_terms(D, A, i, j, mf[m], r, C->Q, C->Mz, C->Qz2);
+    const double lin = t.dqi * stencil_dot<P>(D, i) + t.dqj * stencil_dot<P>(D, j);
+    base[m] = lin + t.selfq + t.perfc + t.perf + t.corr;
+}
+
+// Training target t = s (dE - B).
+__global__ void k_da_target(int n, const int* __restrict__ mf, const double* __restrict__ dE,
+                            const double* __restrict__ base, float* __restrict__ t) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const double v = dE[m] - base[m];
+    t[m] = (float)(mf[m] ? v : -v);
+}
+
+// Surrogate dE* = B + s f(d).
+__global__ void k_da_surrogate(int n, const int* __restrict__ mf, const float* __restrict__ y,
+                               const double* __restrict__ base, double* __restrict__ out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const double f = (double)y[m];
+    out[m] = base[m] + (mf[m] ? f : -f);
 }
 
 // Every stride-th dataset row (normalisation statistics).
@@ -486,28 +582,57 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
 // ---------------------------------------------------------------- delayed-acceptance chain
 struct CandLds {
     int i[DA_M], j[DA_M], fwd[DA_M], pos[DA_M], exA[DA_M], exB[DA_M];
-    double u1[DA_M], rij[DA_M], dEs[DA_M], e1[DA_M];
+    double u1[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
+    double lin[DA_M][2];          // W_a . phi of the site and the ion
+    MoveTerms mt[DA_M];
+    float sf[DA_M];               // s f(d)
 };
 
+// The chain of one replica.  Each launch ends after one accepted move, so that
+// the refresh between launches keeps phi exact for the surrogate's potential term.
 template <int P, typename PoolT, bool LDS_POOLS>
-__global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDev net) {
+__global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict__ Dp, const DaDev* __restrict__ Ap,
+                                                         const NetDev* __restrict__ Np) {
+    // the views live in device memory: read field by field instead of holding every field in registers
+    const McDev& D = *Dp;
+    const DaDev& A = *Ap;
+    const NetDev& net = *Np;
+    // network weights staged in LDS for the whole launch.  W1 rows are padded to
+    // XP elements; the 8-byte chunks of W2's 256-byte rows are XOR-swizzled by
+    // row (chunk c of row j at c ^ (2j mod 32)), so that the 16 rows one MFMA
+    // B operand reads fall into distinct banks.
+    __shared__ __align__(16) unsigned short s_w1t[NN_H * XP];
+    __shared__ __align__(16) unsigned short s_w2t[NN_H * NN_H];
+    __shared__ float s_b1[NN_H], s_b2[NN_H], s_w3[NN_H];
     __shared__ RngLds s_rng;
     __shared__ TrialT<P> s_tr[1];
-    __shared__ PendT<P> s_pd;
     __shared__ DecisionLds s_dc;
     __shared__ ChainState s_st;
     __shared__ CandLds s_c;
-    __shared__ double s_wsum[16];
+    __shared__ double s_wsum[DA_WAVES], s_wwin[DA_WAVES];
     __shared__ __align__(16) unsigned short s_X[DA_M * XP];
     __shared__ __align__(16) unsigned short s_H1[DA_M * HP];
-    __shared__ float s_y[DA_M][4];
+    __shared__ float s_y[DA_M][8];
     __shared__ int s_first;
     __shared__ long long s_s1, s_rounds;
+    __shared__ long long s_prof[8];
     extern __shared__ __align__(16) unsigned char s_dyn[];
+    PhaseClock clk;
+    const bool prof = A.prof != 0 && threadIdx.x == 0;
+    clk.start(prof);
     McCtl* C = D.ctl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long long target = C->trials_target, rec_cap = C->rec_cap;
-    const int pend_max = C->pending_max, M = A.M;
+    // a launch queued beyond the target (batches are sized ahead) leaves at once
+    if (C->trials_done >= target || C->status != 0) return;
+    const int pn0 = C->pending_folded ? 0 : C->pending_n;
+    if (pn0 > 0) {   // phi lags the state: the surrogate's potential term would not be a function of the state
+        if (tid == 0) C->status = 3;
+        return;
+    }
+    const int M = A.M;
+    // pending is empty for the whole launch; trial_partial never reads this view then
+    const PendT<P>& s_pd = *reinterpret_cast<const PendT<P>*>(s_wsum);
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     if (LDS_POOLS)
         for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
@@ -515,19 +640,36 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
         s_rng.cur[k] = C->mt[k];
         s_rng.out[0][k] = mt_temper(C->mt[k]);
     }
-    const int pn0 = C->pending_folded ? 0 : C->pending_n;
-    for (int k = wave; k < pn0; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
+    {
+        const uint2* w1 = reinterpret_cast<const uint2*>(net.w1t);
+        const uint2* w2 = reinterpret_cast<const uint2*>(net.w2t);
+        for (int k = tid; k < NN_H * NN_IN / 4; k += blockDim.x) {       // 12 chunks per row
+            const int j = k / (NN_IN / 4), c = k - j * (NN_IN / 4);
+            reinterpret_cast<uint2*>(s_w1t)[j * (XP / 4) + c] = w1[k];
+        }
+        for (int k = tid; k < NN_H * NN_H / 4; k += blockDim.x) {        // 32 chunks per row
+            const int j = k >> 5, c = k & 31;
+            reinterpret_cast<uint2*>(s_w2t)[j * 32 + (c ^ ((2 * j) & 31))] = w2[k];
+        }
+        for (int k = tid; k < NN_H; k += blockDim.x) {
+            s_b1[k] = net.theta[OFF_B1 + k];
+            s_b2[k] = net.theta[OFF_B2 + k];
+            s_w3[k] = bf2f(net.w3[k]);
+        }
+    }
     if (tid == 0) {
         s_st.done = C->trials_done;
         s_st.acc_n = C->accepted; s_st.acc_f = C->acc_fwd; s_st.acc_b = C->acc_bwd;
         for (int k = 0; k < 4; ++k) s_st.sz[k] = C->sizes[k];
         s_st.status = C->status; s_st.nprot = C->nprot;
-        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = pn0;
+        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = 0;
         s_st.E = C->E; s_st.Q = C->Q; s_st.Q2 = C->Q2; s_st.Mz = C->Mz; s_st.Qz2 = C->Qz2;
         s_s1 = C->da_stage1; s_rounds = C->da_rounds;
+        for (int k = 0; k < 8; ++k) s_prof[k] = C->da_prof[k];
     }
     const float om = net.theta[OFF_MEAN], osd = net.theta[OFF_STD], b3 = net.theta[OFF_B3];
     __syncthreads();
+    clk.lap(prof, s_prof, 0);
     for (;;) {
         // ---- 1. proposals (wave 0), each assumed rejected at stage 1
         if (wave == 0) {
@@ -535,7 +677,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
             const long long done = s_st.done;
             const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
             int ntr = 0, stop = 0;
-            if (done >= target || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            if (done >= target || s_st.pend_n > 0 || status != 0) stop = 1;
             else {
                 const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
                 if (wf == 0 && wb == 0) { status = 1; stop = 1; }
@@ -595,69 +737,102 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
             }
         }
         __syncthreads();
+        clk.lap(prof, s_prof, 1);
         if (s_dc.stop) break;
-        const int ntr = s_dc.ntr, pend_n = s_st.pend_n;
-        // ---- 2. descriptors -> X (bf16, scaled); rows >= ntr are zero
+        const int ntr = s_dc.ntr;
+        // ---- 2. per-candidate physics terms, W . phi of both atoms (one half-wave per atom),
+        //         and the raw shell-count differences
         if (tid < ntr) {
             double r;
             int exA, exB;
-            pair_terms(A, D, s_c.i[tid], s_c.j[tid], r, exA, exB);
+            const int i = s_c.i[tid], j = s_c.j[tid];
+            pair_terms(A, D, i, j, r, exA, exB);
             s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
+            s_c.mt[tid] = move_terms(D, A, i, j, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
         }
-        __syncthreads();
-        for (int e = tid; e < DA_M * NN_IN; e += MC_THREADS) {
-            const int k = e / NN_IN, c = e - k * NN_IN;
-            float v = 0.f;
-            if (k < ntr)
-                v = (float)desc_value(A, s_c.i[k], s_c.j[k], c, s_c.exA[k], s_c.exB[k]) * net.theta[OFF_SCALE + c];
-            s_X[k * XP + c] = f2bf(v);
-        }
-        __syncthreads();
-        // ---- 3. MLP: wave w -> rows 16 (w & 3).., column tiles 2 (w >> 2) + {0, 1}
         {
-            const int rb = wave & 3, cg = wave >> 2, rr = lane & 15, g = lane >> 4, r0 = rb * 16;
-            s4 a[3];
+            // one half-wave per (candidate, atom); a half-wave's rows of all its pairs are
+            // loaded before any reduction, so the potential-grid reads overlap
+            constexpr int NHW = DA_THREADS / 32, PER = (2 * DA_M + NHW - 1) / NHW;
+            const int hw = tid >> 5, l32 = tid & 31;
+            double v[PER];
 #pragma unroll
-            for (int kb = 0; kb < 3; ++kb) a[kb] = ld4(s_X + (r0 + rr) * XP + kb * 16 + 4 * g);
+            for (int q = 0; q < PER; ++q) {
+                const int pidx = hw + q * NHW;
+                v[q] = 0.0;
+                if (pidx < 2 * ntr && l32 < P * P) {
+                    const int k = pidx >> 1, a = (pidx & 1) ? s_c.j[k] : s_c.i[k];
+                    v[q] = stencil_row<P>(D, a, l32);
+                }
+            }
 #pragma unroll
-            for (int c2 = 0; c2 < 2; ++c2) {
-                const int ct = 2 * cg + c2;
+            for (int q = 0; q < PER; ++q) {
+                const int pidx = hw + q * NHW;
+                double x = v[q];
+                x += __shfl_xor(x, 16, 64);
+                x += __shfl_xor(x, 8, 64);
+                x += __shfl_xor(x, 4, 64);
+                x += __shfl_xor(x, 2, 64);
+                x += __shfl_xor(x, 1, 64);
+                if (l32 == 0 && pidx < 2 * ntr) s_c.lin[pidx >> 1][pidx & 1] = x;
+            }
+        }
+        int raw[DA_FEAT];
+#pragma unroll
+        for (int q = 0; q < DA_FEAT; ++q) {
+            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
+            raw[q] = 0;
+            if (e < DA_M * NN_IN && k < ntr)
+                raw[q] = (int)A.H[(size_t)s_c.i[k] * NN_IN + c] - (int)A.H[(size_t)s_c.j[k] * NN_IN + c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DA_FEAT; ++q) {
+            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
+            if (e < DA_M * NN_IN) {
+                float v = 0.f;
+                if (k < ntr) {
+                    const int d = raw[q] - (c == s_c.exA[k] ? 1 : 0) + (c == s_c.exB[k] ? 1 : 0);
+                    v = (float)d * net.theta[OFF_SCALE + c];
+                }
+                s_X[k * XP + c] = f2bf(v);
+            }
+        }
+        __syncthreads();
+        clk.lap(prof, s_prof, 2);
+        // ---- 3. MLP on MFMA: nrb row blocks x 8 column tiles, tile t on wave t mod DA_WAVES
+        const int nrb = (ntr + 15) >> 4;
+        {
+            const int rr = lane & 15, g = lane >> 4;
+            for (int t = wave; t < nrb * 8; t += DA_WAVES) {
+                const int rb = t % nrb, ct = t / nrb, r0 = rb * 16;
                 f4 acc = {0.f, 0.f, 0.f, 0.f};
-                const unsigned short* wr = net.w1t + (ct * 16 + rr) * NN_IN + 4 * g;
+                const unsigned short* wr = s_w1t + (ct * 16 + rr) * XP + 4 * g;
 #pragma unroll
-                for (int kb = 0; kb < 3; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
-                const float b = net.theta[OFF_B1 + ct * 16 + rr];
+                for (int kb = 0; kb < 3; ++kb) acc = mfma16(ld4(s_X + (r0 + rr) * XP + kb * 16 + 4 * g), ld4(wr + kb * 16), acc);
+                const float b = s_b1[ct * 16 + rr];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) s_H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(acc[e] + b));
             }
             __syncthreads();
-            float yp[4] = {0.f, 0.f, 0.f, 0.f};
-            {
-                f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-                const unsigned short* wr0 = net.w2t + (2 * cg * 16 + rr) * NN_H + 4 * g;
-                const unsigned short* wr1 = wr0 + 16 * NN_H;
-#pragma unroll 2
-                for (int kb = 0; kb < 8; ++kb) {
-                    const s4 a2 = ld4(s_H1 + (r0 + rr) * HP + kb * 16 + 4 * g);
-                    acc0 = mfma16(a2, ld4(wr0 + kb * 16), acc0);
-                    acc1 = mfma16(a2, ld4(wr1 + kb * 16), acc1);
+            for (int t = wave; t < nrb * 8; t += DA_WAVES) {
+                const int rb = t % nrb, ct = t / nrb, r0 = rb * 16;
+                const int row = ct * 16 + rr, swz = (2 * row) & 31;
+                const unsigned short* wr = s_w2t + row * NN_H;
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+                for (int kb = 0; kb < 8; ++kb)
+                    acc = mfma16(ld4(s_H1 + (r0 + rr) * HP + kb * 16 + 4 * g), ld4(wr + (((4 * kb + g) ^ swz) << 2)), acc);
+                const float b = s_b2[row], w = s_w3[row];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = xor_sum16(bf2f(f2bf(gelu_f(acc[e] + b))) * w);
+                    if (rr == 0) s_y[r0 + 4 * g + e][ct] = v;
                 }
-#pragma unroll
-                for (int c2 = 0; c2 < 2; ++c2) {
-                    const int col = (2 * cg + c2) * 16 + rr;
-                    const float b = net.theta[OFF_B2 + col], w = bf2f(net.w3[col]);
-                    const f4 acc = c2 ? acc1 : acc0;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(acc[e] + b))) * w;
-                }
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float v = xor_sum16(yp[e]);
-                if (rr == 0) s_y[r0 + 4 * g + e][cg] = v;
             }
         }
         __syncthreads();
+        clk.lap(prof, s_prof, 3);
         // ---- 4. stage-1 decisions (wave 0): the first stage-1 acceptance ends the round
         if (wave == 0) {
             const long long done = s_st.done;
@@ -665,24 +840,29 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
             const long long Sb = a * c + b * d;
             const int k = lane;
             bool acc1 = false;
-            double e1 = 0.0, dEs = 0.0;
+            double Fx = 0.0;
             if (k < ntr) {
-                const float y = ((s_y[k][0] + s_y[k][1]) + s_y[k][2]) + s_y[k][3] + b3;
-                const double f = (double)(om + osd * y) - D.elF / s_c.rij[k];
+                float y = b3;
+#pragma unroll
+                for (int ct = 0; ct < 8; ++ct) y += s_y[k][ct];
+                const float f = om + osd * y;
                 const int fwd = s_c.fwd[k];
-                dEs = fwd ? f : -f;
+                const float sf = fwd ? f : -f;
+                const MoveTerms& m = s_c.mt[k];
+                Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr + (double)sf;
                 const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
-                e1 = mh_exponent(D.beta, dEs, fwd ? 1 : -1, D.mu, Sb, Sa);
+                const double e1 = mh_exponent(D.beta, Fx, fwd ? 1 : -1, D.mu, Sb, Sa);
                 acc1 = e1 >= 0.0 || exp(e1) > s_c.u1[k];
-                s_c.dEs[k] = dEs;
+                s_c.sf[k] = sf;
                 s_c.e1[k] = e1;
+                s_c.Fx[k] = Fx;
             }
-            const unsigned long long m = __ballot(acc1);
-            const int first = m ? __ffsll((long long)m) - 1 : ntr;
+            const unsigned long long msk = __ballot(acc1);
+            const int first = msk ? __ffsll((long long)msk) - 1 : ntr;
             if (k < first && k < ntr && done + k < rec_cap) {
+                const double nan = __longlong_as_double(0x7FF8000000000000LL);
                 D.rec_fwd[done + k] = (signed char)s_c.fwd[k]; D.rec_i[done + k] = s_c.i[k]; D.rec_j[done + k] = s_c.j[k];
-                D.rec_acc[done + k] = 0; D.rec_dE[done + k] = __longlong_as_double(0x7FF8000000000000LL);
-                D.rec_u[done + k] = dEs;
+                D.rec_acc[done + k] = 0; D.rec_dE[done + k] = nan; D.rec_u[done + k] = Fx; A.rec_fy[done + k] = nan;
             }
             if (lane == 0) {
                 s_first = first;
@@ -696,32 +876,43 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
             }
         }
         __syncthreads();
+        clk.lap(prof, s_prof, 4);
         const int first = s_first;
         if (first < ntr) {
-            // ---- 5. exact Delta-E of the stage-1 acceptance (all waves)
-            stage_trial_data<P>(D, s_tr[0], tid, MC_THREADS);
+            // ---- 5. exact Delta-E of the stage-1 acceptance (all waves); its mesh-kernel pair part apart
+            stage_trial_data<P>(D, s_tr[0], tid, DA_THREADS);
             __syncthreads();
-            const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, pend_n, tid, MC_THREADS));
-            if (lane == 0) s_wsum[wave] = part;
+            double win = 0.0;
+            const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win));
+            win = wave_sum(win);
+            if (lane == 0) { s_wsum[wave] = part; s_wwin[wave] = win; }
             __syncthreads();
+            clk.lap(prof, s_prof, 5);
             if (wave == 0) {
                 const TrialT<P>& t = s_tr[0];
                 const long long done = s_st.done;
                 const double Q = s_st.Q, Q2 = s_st.Q2, Mz = s_st.Mz, Qz2 = s_st.Qz2;
                 const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
                 const long long Sb = a * c + b * d;
-                double dsum = 0.0;
-                for (int w = 0; w < 16; ++w) dsum += s_wsum[w];
+                double dsum = 0.0, wsum = 0.0;
+                for (int w = 0; w < DA_WAVES; ++w) { dsum += s_wsum[w]; wsum += s_wwin[w]; }
                 const double dE = dsum + correction_delta<P>(D, t, Q, Q2, Mz, Qz2);
                 const int fwd = t.fwd, sg = fwd ? 1 : -1;
                 const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
                 const double e1 = s_c.e1[first];
+                // the surrogate of the reverse move from the proposed state: the potential
+                // there is phi + G dQ, so W_a . phi_y = W_a . phi + sum_b dq_b W_a^T G W_b
+                const MoveTerms& m = s_c.mt[first];
+                const double lin = m.dqi * s_c.lin[first][0] + m.dqj * s_c.lin[first][1];
+                const double Fy = -lin - m.selfq - 2.0 * wsum - m.perfc + m.perf - m.corr - (double)s_c.sf[first];
                 double e2;
                 {
 #pragma clang fp contract(off)
                     double logH = 0.0;
                     if (Sb > 0 && Sa > 0) logH = log((double)Sb) - log((double)Sa);
-                    e2 = ((double)((-D.beta) * (dE - (double)sg * D.mu)) + logH) - e1;
+                    double r1 = (double)((-D.beta) * (Fy + (double)sg * D.mu)) - logH;
+                    r1 = fmin(fmax(r1, -40.0), 40.0);
+                    e2 = (((double)((-D.beta) * (dE - (double)sg * D.mu)) + logH) + fmin(0.0, r1)) - fmin(0.0, e1);
                 }
                 // roll the stream back to just after this candidate's own draws
                 int blk = s_st.blk, pos = s_c.pos[first] + (e1 >= 0.0 ? 0 : 2);
@@ -734,7 +925,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
                     const long long tk = done + first;
                     if (tk < rec_cap) {
                         D.rec_fwd[tk] = (signed char)fwd; D.rec_i[tk] = t.i; D.rec_j[tk] = t.j;
-                        D.rec_acc[tk] = acc2 ? 1 : 2; D.rec_dE[tk] = dE; D.rec_u[tk] = s_c.dEs[first];
+                        D.rec_acc[tk] = acc2 ? 1 : 2; D.rec_dE[tk] = dE; D.rec_u[tk] = s_c.Fx[first]; A.rec_fy[tk] = Fy;
                     }
                     s_st.done = done + first + 1;
                     if (acc2) {
@@ -756,21 +947,20 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
                         if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
                         else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
                         s_dc.acc_k = 0;
+                        s_st.pend_n = 2;      // ends the launch: the refresh folds the move into phi
                     }
                 }
             }
             __syncthreads();
-            // ---- 6. accepted: pending list, pools, histograms of the atoms around i and j
+            // ---- 6. accepted: pools and the histograms of the atoms around i and j
             if (s_dc.acc_k == 0) {
                 const TrialT<P>& t = s_tr[0];
-                if (wave == 1) stage_pending<P>(D, s_pd, pend_n, t.i, t.qi_new - t.qi_old, lane, 64);
-                if (wave == 2) stage_pending<P>(D, s_pd, pend_n + 1, t.j, t.qj_new - t.qj_old, lane, 64);
                 pool_update<PoolT>(D, pbuf, s_dc);
                 for (int side = 0; side < 2; ++side) {
                     const int atom = side ? t.j : t.i;
                     const int so = A.species[side ? t.tj_old : t.ti_old], sn = A.species[side ? t.tj_new : t.ti_new];
                     if (so != sn)
-                        for (int e = A.dhead[atom] + tid; e < A.dhead[atom + 1]; e += MC_THREADS) {
+                        for (int e = A.dhead[atom] + tid; e < A.dhead[atom + 1]; e += DA_THREADS) {
                             const unsigned v = A.dlist[e];
                             unsigned short* h = A.H + (size_t)(v & 0xFFFFFFu) * NN_IN + (v >> DA_SHIFT) * 4;
                             if (so >= 0) h[so] = (unsigned short)(h[so] - 1);
@@ -778,15 +968,13 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
                         }
                     __syncthreads();
                 }
-                if (tid == 0) s_st.pend_n = pend_n + 2;
-                __syncthreads();
             }
+            clk.lap(prof, s_prof, 6);
         }
     }
     // write back
     if (LDS_POOLS)
         for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) D.pbuf[k] = (int)pbuf[k];
-    const int pend_n = s_st.pend_n;
     if (wave == 0) {
         const int blk = s_st.blk, pos = s_st.pos;
         const bool in_cur = pos >= blk * 624;
@@ -797,12 +985,22 @@ __global__ __launch_bounds__(MC_THREADS) void k_da_chain(McDev D, DaDev A, NetDe
             for (int k = 0; k < 4; ++k) C->sizes[k] = s_st.sz[k];
             C->trials_done = s_st.done;
             C->accepted = s_st.acc_n; C->acc_fwd = s_st.acc_f; C->acc_bwd = s_st.acc_b;
-            C->pending_n = pend_n; C->pending_folded = 0; C->status = s_st.status; C->nprot = s_st.nprot;
+            C->status = s_st.status; C->nprot = s_st.nprot;
             C->E = s_st.E; C->Q = s_st.Q; C->Q2 = s_st.Q2; C->Mz = s_st.Mz; C->Qz2 = s_st.Qz2;
             C->da_stage1 = s_s1; C->da_rounds = s_rounds;
+            if (s_st.pend_n > 0) {
+                const TrialT<P>& t = s_tr[0];
+                C->pend_atom[0] = t.i; C->pend_dq[0] = t.qi_new - t.qi_old;
+                C->pend_atom[1] = t.j; C->pend_dq[1] = t.qj_new - t.qj_old;
+                C->pending_n = 2;
+                C->pending_folded = 0;
+            }
         }
     }
-    for (int k = tid; k < pend_n; k += blockDim.x) { C->pend_atom[k] = s_pd.atom[k]; C->pend_dq[k] = s_pd.dq[k]; }
+    if (prof) {
+        clk.lap(prof, s_prof, 7);
+        for (int k = 0; k < 8; ++k) C->da_prof[k] = s_prof[k];
+    }
 }
 
 }  // namespace phb
@@ -835,6 +1033,9 @@ struct phb_surrogate_s {
 namespace {
 
 struct DaState {
+    DevBuf<McDev> dmc;         // device copies of the chain's views (kernel reads them by pointer)
+    DevBuf<DaDev> dda;
+    DevBuf<NetDev> dnet;
     phb_da_desc d{};
     std::vector<signed char> species;
     DevBuf<signed char> sp;
@@ -842,8 +1043,10 @@ struct DaState {
     DevBuf<unsigned> dlist;
     DevBuf<unsigned short> H;
     DevBuf<double> x, y;       // copies of the positions (fixed)
+    DevBuf<double> rec_fy;
     DaDev dev{};
     long long nentries = 0;
+    long long launches = 0, trials_seen = 0;   // batch sizing of phb_da_run
 };
 
 void da_free(void* p) { delete static_cast<DaState*>(p); }
@@ -859,27 +1062,33 @@ void rebuild_hist(phb_mc_s* M, DaState* S) {
     PHB_HIP(hipGetLastError());
 }
 
-template <int P>
-void launch_da_p(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
-    if (M->lds_pools) {
-        auto kern = k_da_chain<P, unsigned short, true>;
-        PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)M->pool_lds_bytes));
-        hipLaunchKernelGGL(kern, dim3(1), dim3(MC_THREADS), M->pool_lds_bytes, stream(), M->dev, S->dev, sur->net());
-    } else {
-        hipLaunchKernelGGL((k_da_chain<P, int, false>), dim3(1), dim3(MC_THREADS), 0, stream(), M->dev, S->dev,
-                           sur->net());
-    }
+// Uploads the views the chain kernel reads by pointer (before a run; the stream orders it).
+void upload_views(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
+    S->dev.prof = M->profiling ? 1 : 0;
+    const NetDev net = sur->net();
+    S->dmc.upload(&M->dev, 1);
+    S->dda.upload(&S->dev, 1);
+    S->dnet.upload(&net, 1);
+    sync();   // the host copies are stack/member values
 }
 
-void launch_da(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
+template <int P>
+void launch_da_p(phb_mc_s* M, DaState* S) {
+    (void)M;
+    // Pools stay in HBM: a launch covers a single accepted move, so copying them
+    // into LDS and back each launch would cost more than the proposals' reads save.
+    hipLaunchKernelGGL((k_da_chain<P, int, false>), dim3(1), dim3(DA_THREADS), 0, stream(), S->dmc.p, S->dda.p,
+                       S->dnet.p);
+}
+
+void launch_da(phb_mc_s* M, DaState* S) {
     switch (M->dev.P) {
-        case 3: launch_da_p<3>(M, S, sur); break;
-        case 4: launch_da_p<4>(M, S, sur); break;
-        case 5: launch_da_p<5>(M, S, sur); break;
-        case 6: launch_da_p<6>(M, S, sur); break;
-        case 7: launch_da_p<7>(M, S, sur); break;
-        default: launch_da_p<8>(M, S, sur); break;
+        case 3: launch_da_p<3>(M, S); break;
+        case 4: launch_da_p<4>(M, S); break;
+        case 5: launch_da_p<5>(M, S); break;
+        case 6: launch_da_p<6>(M, S); break;
+        case 7: launch_da_p<7>(M, S); break;
+        default: launch_da_p<8>(M, S); break;
     }
 }
 
@@ -888,6 +1097,23 @@ void check_moves(phb_mc_s* M, int n, const int* i, const int* j, const int* f) {
         require(i[k] >= 0 && i[k] < M->atoms->n && j[k] >= 0 && j[k] < M->atoms->n && i[k] != j[k] &&
                     (f == nullptr || f[k] == 0 || f[k] == 1),
                 "delayed acceptance: move index out of range");
+}
+
+// physics part B of n moves (device buffers); phi must hold every accepted change
+void base_terms(phb_mc_s* M, DaState* S, int n, const int* di, const int* dj, const int* df, double* out) {
+    refresh(M);   // folds pending accepted changes into phi (no-op when none are pending)
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(ceil_div(n, 128)), dim3(128), 0, stream(), M->dev, S->dev, n, di, dj, df, out);
+    };
+    switch (M->dev.P) {
+        case 3: go(k_da_base<3>); break;
+        case 4: go(k_da_base<4>); break;
+        case 5: go(k_da_base<5>); break;
+        case 6: go(k_da_base<6>); break;
+        case 7: go(k_da_base<7>); break;
+        default: go(k_da_base<8>); break;
+    }
+    PHB_HIP(hipGetLastError());
 }
 
 // descriptors + r_ij of n moves (device buffers)
@@ -1163,6 +1389,7 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
             require(d->species_of_type[t] >= -1 && d->species_of_type[t] <= 3,
                     "phb_da_attach: species must be -1 or 0..3");
         auto S = std::make_unique<DaState>();
+        S->dmc.alloc(1); S->dda.alloc(1); S->dnet.alloc(1);
         S->d = *d;
         S->d.candidates = cand;
         S->species.assign(d->species_of_type, d->species_of_type + d->ntypes);
@@ -1188,6 +1415,9 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
             A.th[k] = rk * rk;
         }
         A.M = cand;
+        A.rc2 = M->params->d.rc * M->params->d.rc;
+        S->rec_fy.alloc(std::max<long long>(M->d.record_cap, 1));
+        A.rec_fy = S->rec_fy.p;
         if (S->nentries > 0)
             hipLaunchKernelGGL(k_da_row_entries, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
                                M->atoms->y.p, M->atoms->z.p, A, nbr.p, S->dlist.p);
@@ -1227,12 +1457,13 @@ extern "C" int phb_da_collect(phb_mc M, phb_surrogate s, int n, const int* i, co
         if (n == 0) return PHB_OK;
         check_moves(M, n, i, j, fwd);
         DevBuf<int> di(n), dj(n), df(n);
-        DevBuf<double> dr(n), dE(n);
+        DevBuf<double> dr(n), dE(n), base(n);
         di.upload(i, n); dj.upload(j, n); df.upload(fwd, n);
         descriptors(M, S, n, di.p, dj.p, s->dd.p + s->n * NN_IN, dr.p);
+        base_terms(M, S, n, di.p, dj.p, df.p, base.p);
         launch_probe(M, n, di.p, dj.p, df.p, dE.p);
-        hipLaunchKernelGGL(k_da_target, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dE.p, dr.p,
-                           M->pme->elF, s->dt.p + s->n);
+        hipLaunchKernelGGL(k_da_target, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dE.p, base.p,
+                           s->dt.p + s->n);
         PHB_HIP(hipGetLastError());
         sync();
         s->n += n;
@@ -1249,14 +1480,15 @@ extern "C" int phb_da_predict(phb_mc M, phb_surrogate s, int n, const int* i, co
         check_moves(M, n, i, j, fwd);
         DevBuf<int> di(n), dj(n), df(n);
         DevBuf<short> dd((size_t)n * NN_IN);
-        DevBuf<double> dr(n), out(n);
+        DevBuf<double> dr(n), out(n), base(n);
         DevBuf<float> dy(n);
         di.upload(i, n); dj.upload(j, n); df.upload(fwd, n);
         descriptors(M, S, n, di.p, dj.p, dd.p, dr.p);
+        base_terms(M, S, n, di.p, dj.p, df.p, base.p);
         hipLaunchKernelGGL(k_mlp_forward, dim3(ceil_div(n, 64)), dim3(256), 0, stream(), s->net(), dd.p, (long long)n,
                            dy.p);
-        hipLaunchKernelGGL(k_da_surrogate, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dy.p, dr.p,
-                           M->pme->elF, out.p);
+        hipLaunchKernelGGL(k_da_surrogate, dim3(ceil_div(n, 256)), dim3(256), 0, stream(), n, df.p, dy.p, base.p,
+                           out.p);
         PHB_HIP(hipGetLastError());
         out.download(dE_star, n);
         sync();
@@ -1269,6 +1501,8 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
         require(M && s && ntrials >= 0, "phb_da_run: bad arguments");
         DaState* S = da_of(M);
         rebuild_hist(M, S);
+        upload_views(M, S, s);
+        refresh(M);   // the surrogate needs phi to hold every accepted change
         McCtl c = read_ctl(M);
         const long long target = c.trials_done + ntrials;
         long long done = c.trials_done;
@@ -1278,18 +1512,25 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
             if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
             hipLaunchKernelGGL(k_da_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
             while (done < chunk_end && status == 0) {
-                const size_t e0 = M->ev_used;
-                if (M->profiling) M->event();
-                launch_da(M, S, s);
-                ++M->launches;
-                if (M->profiling) M->event();
-                refresh(M);
-                if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
+                const int batch = launch_batch(c, chunk_end - done, 2);   // a launch ends at its accepted move
+                const long long before = done;
+                for (int b = 0; b < batch; ++b) {
+                    const size_t e0 = M->ev_used;
+                    if (M->profiling) M->event();
+                    launch_da(M, S);
+                    ++M->launches;
+                    ++S->launches;
+                    if (M->profiling) M->event();
+                    refresh(M);
+                    if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
+                }
                 PHB_HIP(hipGetLastError());
                 c = read_ctl(M);
+                S->trials_seen += c.trials_done - before;
                 require(c.trials_done > done || c.status != 0 || c.trials_done >= chunk_end,
                         "phb_da_run: the chain made no progress", PHB_ERR_INTERNAL);
                 require(c.status != 2, "phb_da_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
+                require(c.status != 3, "phb_da_run: accepted changes pending at a chain launch", PHB_ERR_INTERNAL);
                 done = c.trials_done;
                 status = c.status;
             }
@@ -1312,13 +1553,21 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
             out->E = c.E;
             out->chain_ms = M->chain_ms;
             out->refresh_ms = M->refresh_ms;
+            for (int k = 0; k < 8; ++k) out->phase_us[k] = (double)c.da_prof[k] * 0.01;   // 100 MHz ticks
         }
         return PHB_OK;
     });
 }
 
 extern "C" int phb_da_record(phb_mc M, long long n, signed char* fwd, int* i, int* j, signed char* stage,
-                             double* dE_star, double* dE) {
+                             double* dE_star, double* dE_star_rev, double* dE) {
     // the delayed-acceptance chain stores the stage in the acceptance record and dE* in the uniform record
-    return phb_mc_record(M, n, fwd, i, j, stage, dE, dE_star);
+    const int rc = phb_mc_record(M, n, fwd, i, j, stage, dE, dE_star);
+    if (rc != PHB_OK || !dE_star_rev || n == 0) return rc;
+    return guard([&] {
+        DaState* S = da_of(M);
+        S->rec_fy.download(dE_star_rev, n);
+        sync();
+        return PHB_OK;
+    });
 }

@@ -5,8 +5,13 @@
 // lattice grow as self-avoiding random walks, every titratable bead gets one
 // counter-particle (Cl- when protonated, neutral placeholder W otherwise, which
 // is what the reference's ion-swap move needs: MC/src/mc.cu:384-388), every
-// permanently charged bead gets a Cl- counterion (both placed uniformly in the
-// brush layer), and salt pairs are placed uniformly in the box.  Generator RNG: std::mt19937_64 seeded with spec->seed, draws via
+// permanently charged bead gets a Cl- counterion, and salt pairs are placed
+// uniformly in the box.  Counter-particles sit in a shell of 1.0-2.5 sigma
+// (widened to 4 and 6 sigma when crowded) around their own bead (local electroneutrality of a dense brush: positions
+// are frozen, so a layer of beads with its counterions spread over the whole
+// brush height would keep a capacitor-like field of hundreds of kT per sigma);
+// a bead without room nearby gets its counter-particle anywhere in the brush
+// layer.  Generator RNG: std::mt19937_64 seeded with spec->seed, draws via
 // std::uniform_real_distribution<double>.
 #include <algorithm>
 #include <cmath>
@@ -113,13 +118,13 @@ extern "C" int phb_gen_brush(const phb_brush_spec* sp, int cap, double* x, doubl
                 throw phb::Error(PHB_ERR_STATE, "phb_gen_brush: graft points closer than min_dist");
             grid.cells[((size_t)grid.cz(Z[id]) * grid.ncy + grid.cy(Y[id])) * grid.ncx + grid.cx(X[id])].push_back(id);
         }
-        // Step directions: azimuth uniform, cos(theta) uniform on [-0.5, 1]
-        // (a random walk mildly stretched away from the grafting surface, as
-        // excluded volume stretches a brush); a trapped bead retries from the
-        // upper hemisphere only.
+        // Step directions: azimuth uniform, cos(theta) uniform on [0, 1] (a walk
+        // stretched away from the grafting surface, as excluded volume and
+        // charge stretch a dense brush); a trapped bead may also step sideways
+        // or back (cos(theta) on [-0.5, 1]).
         auto try_place = [&](int id) {
             for (int tries = 0; tries < 400; ++tries) {
-                double ct = tries < 200 ? 1.5 * U(rng) - 0.5 : U(rng), ph = 2.0 * M_PI * U(rng);
+                double ct = tries < 200 ? U(rng) : 1.5 * U(rng) - 0.5, ph = 2.0 * M_PI * U(rng);
                 double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
                 double nx = X[id - 1] + sp->bond * st * std::cos(ph);
                 double ny = Y[id - 1] + sp->bond * st * std::sin(ph);
@@ -155,15 +160,17 @@ extern "C" int phb_gen_brush(const phb_brush_spec* sp, int cap, double* x, doubl
             }
         int nb = 0;
         std::vector<char> site_prot;  // protonation state of each titratable bead, in bead order
+        std::vector<int> perm_ids, site_ids;
         for (int c = 0; c < nch; ++c)
             for (int b = 0; b < len; ++b) {
                 const int id = c * len + b;
                 M[id] = c + 1;
                 if (b > 0) { bond_i[nb] = id - 1; bond_j[nb] = id; ++nb; }
-                if (b < sp->n_perm) { Q[id] = 1.0; T[id] = sp->type_perm; }
+                if (b < sp->n_perm) { Q[id] = 1.0; T[id] = sp->type_perm; perm_ids.push_back(id); }
                 else {
                     bool prot = U(rng) < sp->p_prot;
                     site_prot.push_back(prot ? 1 : 0);
+                    site_ids.push_back(id);
                     Q[id] = prot ? 1.0 : 0.0;
                     T[id] = prot ? sp->type_nh_p : sp->type_nh_n;
                 }
@@ -174,7 +181,7 @@ extern "C" int phb_gen_brush(const phb_brush_spec* sp, int cap, double* x, doubl
             X[id] = px; Y[id] = py; Z[id] = pz; Q[id] = pq; T[id] = pt; M[id] = pm;
             cell_of(id).push_back(id);
         };
-        // Counter-particles fill the brush layer z < (highest bead + 2);
+        // Fallback region for counter-particles: the brush layer z < (highest bead + 2);
         // salt fills the whole data box.
         double z_top = 0.0;
         for (long long i = 0; i < n_beads; ++i) z_top = std::max(z_top, Z[i]);
@@ -186,9 +193,42 @@ extern "C" int phb_gen_brush(const phb_brush_spec* sp, int cap, double* x, doubl
             }
             throw phb::Error(PHB_ERR_STATE, "phb_gen_brush: no free site for an ion (box too dense)");
         };
+        // Shell of radius [1.0, 2.5] sigma around bead `id`, isotropic, above the
+        // grafting plane.  Among up to 8 free candidates the one that keeps the
+        // running bead -> counter-particle dipole sum_k (z_k - z_bead_k) nearest
+        // zero is taken: rejection in the dense interior would otherwise push
+        // counter-particles outward and leave a net dipole layer.
+        double dip = 0.0;
+        auto place_near = [&](int id, double qv, int tv, int mv) {
+            double best[3] = {0, 0, 0}, best_cost = 0.0;
+            int found = 0;
+            for (int tries = 0; tries < 3000 && found < 8; ++tries) {
+                const double rmax = tries < 1000 ? 2.5 : (tries < 2000 ? 4.0 : 6.0);   // widen when crowded
+                const double r = 1.0 + (rmax - 1.0) * U(rng), ct = 2.0 * U(rng) - 1.0, ph = 2.0 * M_PI * U(rng);
+                const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+                double px = X[id] + r * st * std::cos(ph), py = Y[id] + r * st * std::sin(ph);
+                const double pz = Z[id] + r * ct;
+                if (pz < 0.0 || pz >= sp->Lz) continue;
+                px -= sp->Lx * std::floor(px / sp->Lx);
+                py -= sp->Ly * std::floor(py / sp->Ly);
+                if (px >= sp->Lx) px = 0.0;
+                if (py >= sp->Ly) py = 0.0;
+                if (!grid.clear_of(px, py, pz, sp->min_dist)) continue;
+                const double cost = std::fabs(dip + (pz - Z[id]));
+                if (found == 0 || cost < best_cost) { best[0] = px; best[1] = py; best[2] = pz; best_cost = cost; }
+                ++found;
+            }
+            if (found) {
+                dip += best[2] - Z[id];
+                push(best[0], best[1], best[2], qv, tv, mv);
+            } else {
+                place_free(qv, tv, mv, z_layer);
+            }
+        };
         const int ion_mol = nch + 1;
-        for (long long k = 0; k < n_perm; ++k) place_free(-1.0, sp->type_cl, ion_mol, z_layer);
-        for (char prot : site_prot) place_free(prot ? -1.0 : 0.0, prot ? sp->type_cl : sp->type_w, ion_mol, z_layer);
+        for (int id : perm_ids) place_near(id, -1.0, sp->type_cl, ion_mol);
+        for (size_t k = 0; k < site_ids.size(); ++k)
+            place_near(site_ids[k], site_prot[k] ? -1.0 : 0.0, site_prot[k] ? sp->type_cl : sp->type_w, ion_mol);
         for (int k = 0; k < sp->n_salt; ++k) {
             place_free(+1.0, sp->type_cation, ion_mol, sp->Lz);
             place_free(-1.0, sp->type_cl, ion_mol, sp->Lz);

@@ -51,6 +51,8 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_chain(McDev D) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = D.W, wpt = 16 / W;   // waves per trial
     const long long target = C->trials_target, rec_cap = C->rec_cap;
+    // a launch queued beyond the target (batches are sized ahead) leaves at once
+    if (C->trials_done >= target || C->status != 0) return;
     const int pend_max = C->pending_max;
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     if (LDS_POOLS)
@@ -283,44 +285,46 @@ __global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* _
     McCtl* C = D.ctl;
     const int np = C->pending_folded ? 0 : C->pending_n;
     if (np == 0) return;
-    __shared__ int s_sb[MC_KMAX][3];
-    __shared__ double s_w[MC_KMAX][3][P];
-    __shared__ double s_dq[MC_KMAX];
-    __shared__ cplx<double> s_fxy[MC_KMAX][16];     // dq_p * Fx_p(kx0 + c) * Fy_p(ky), c < tx <= 16
+    constexpr int NB = REFRESH_BATCH;   // pending atoms per batch
+    __shared__ int s_sb[NB][3];
+    __shared__ double s_w[NB][3][P];
+    __shared__ double s_dq[NB];
+    __shared__ cplx<double> s_fxy[NB][16];   // dq_p * Fx_p(kx0 + c) * Fy_p(ky), c < tx <= 16
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nx = D.nx, ny = D.ny, nz = D.nz, nkx = nx / 2 + 1, hs = nz + 1;
     cplx<double>* stw = reinterpret_cast<cplx<double>*>(smem_raw);
     cplx<double>* Cc = stw + nz;
+    cplx<double>* s_fz = Cc + (size_t)tx * hs;        // [NB][nz]: Fz_p(kz), the same for every tile
     const int ntile = (nkx + tx - 1) / tx;
     const int ky = blockIdx.x / ntile, kx0 = (blockIdx.x - ky * ntile) * tx, cnt = min(tx, nkx - kx0);
     const int tid = threadIdx.x, bd = blockDim.x;
     if (blockIdx.x == 0 && tid == 0) C->pending_folded = 1;
     for (int k = tid; k < nz; k += bd) stw[k] = twz[k];
-    for (int r = tid; r < np * (3 * P + 4); r += bd) {
-        const int p = r / (3 * P + 4), q = r - p * (3 * P + 4), atom = C->pend_atom[p];
-        if (q < 3 * P) (&s_w[p][0][0])[q] = D.sw[(size_t)atom * 3 * P + q];
-        else if (q < 3 * P + 3) s_sb[p][q - 3 * P] = D.sbase[3 * atom + (q - 3 * P)];
-        else s_dq[p] = C->pend_dq[p];
-    }
-    __syncthreads();
-    for (int r = tid; r < np * cnt; r += bd) {
-        const int p = r / cnt, c = r - p * cnt, kx = kx0 + c;
-        cplx<double> fx = {0.0, 0.0}, fy = {0.0, 0.0};
-        int ix = (kx * s_sb[p][0]) % nx, iy = (ky * s_sb[p][1]) % ny;   // k*(s+u) mod n, stepped by k
-#pragma unroll
-        for (int u = 0; u < P; ++u) {
-            fx = cadd(fx, cscale(twx[ix], s_w[p][0][u]));
-            fy = cadd(fy, cscale(twy[iy], s_w[p][1][u]));
-            ix += kx; ix -= ix >= nx ? nx : 0;
-            iy += ky; iy -= iy >= ny ? ny : 0;
+    for (int pb = 0; pb < np; pb += NB) {
+        const int nb = min(NB, np - pb);
+        for (int r = tid; r < nb * (3 * P + 4); r += bd) {
+            const int p = r / (3 * P + 4), q = r - p * (3 * P + 4), atom = C->pend_atom[pb + p];
+            if (q < 3 * P) (&s_w[p][0][0])[q] = D.sw[(size_t)atom * 3 * P + q];
+            else if (q < 3 * P + 3) s_sb[p][q - 3 * P] = D.sbase[3 * atom + (q - 3 * P)];
+            else s_dq[p] = C->pend_dq[pb + p];
         }
-        s_fxy[p][c] = cscale(cmul(fx, fy), s_dq[p]);
-    }
-    __syncthreads();
-    for (int idx = tid; idx < cnt * nz; idx += bd) {
-        const int c = idx / nz, kz = idx - c * nz, kx = kx0 + c;
-        cplx<double> v = {0.0, 0.0};
-        for (int p = 0; p < np; ++p) {
+        __syncthreads();
+        // separable structure factors: dq Fx Fy per column, Fz per kz
+        for (int r = tid; r < nb * cnt; r += bd) {
+            const int p = r / cnt, c = r - p * cnt, kx = kx0 + c;
+            cplx<double> fx = {0.0, 0.0}, fy = {0.0, 0.0};
+            int ix = (kx * s_sb[p][0]) % nx, iy = (ky * s_sb[p][1]) % ny;   // k*(s+u) mod n, stepped by k
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                fx = cadd(fx, cscale(twx[ix], s_w[p][0][u]));
+                fy = cadd(fy, cscale(twy[iy], s_w[p][1][u]));
+                ix += kx; ix -= ix >= nx ? nx : 0;
+                iy += ky; iy -= iy >= ny ? ny : 0;
+            }
+            s_fxy[p][c] = cscale(cmul(fx, fy), s_dq[p]);
+        }
+        for (int r = tid; r < nb * nz; r += bd) {
+            const int p = r / nz, kz = r - p * nz;
             cplx<double> fz = {0.0, 0.0};
             int iz = (kz * s_sb[p][2]) % nz;
 #pragma unroll
@@ -328,10 +332,39 @@ __global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* _
                 fz = cadd(fz, cscale(stw[iz], s_w[p][2][u]));
                 iz += kz; iz -= iz >= nz ? nz : 0;
             }
-            v = cadd(v, cmul(s_fxy[p][c], fz));
+            s_fz[(size_t)p * nz + kz] = fz;
         }
-        const double et = eta[((size_t)kz * ny + ky) * nkx + kx];
-        Cc[c * hs + kz] = {v.x * et, v.y * et};
+        __syncthreads();
+        // kx fastest (the influence table is read coalesced below)
+        for (int idx = tid; idx < cnt * nz; idx += bd) {
+            const int kz = idx / cnt, c = idx - kz * cnt;
+            cplx<double> v = pb == 0 ? cplx<double>{0.0, 0.0} : Cc[c * hs + kz];
+            for (int p = 0; p < nb; ++p) v = cadd(v, cmul(s_fxy[p][c], s_fz[(size_t)p * nz + kz]));
+            Cc[c * hs + kz] = v;
+        }
+        __syncthreads();
+    }
+    // multiply by the influence function; every load of a chunk issued first
+    for (int base = tid; base < cnt * nz; base += 8 * bd) {
+        double et[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            et[u] = 0.0;
+            if (idx < cnt * nz) {
+                const int kz = idx / cnt, c = idx - kz * cnt;
+                et[u] = eta[((size_t)kz * ny + ky) * nkx + kx0 + c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            if (idx < cnt * nz) {
+                const int kz = idx / cnt, c = idx - kz * cnt;
+                const cplx<double> v = Cc[c * hs + kz];
+                Cc[c * hs + kz] = {v.x * et[u], v.y * et[u]};
+            }
+        }
     }
     __syncthreads();
     lds_fft(Cc, cnt, hs, pz, stw, +1);
@@ -477,6 +510,15 @@ using namespace phb;
 
 namespace phb {
 
+int launch_batch(const McCtl& c, long long remaining, int pending_max) {
+    // a launch ends after pending_max / 2 accepted moves or at the target: expected
+    // launches = accepted moves expected in the remaining trials / moves per launch, + 1
+    const double acc = c.trials_done > 0 ? (double)c.accepted / (double)c.trials_done : 0.05;
+    const double per = std::max(1, pending_max / 2);
+    const double n = std::ceil(std::max(acc, 1e-3) * (double)remaining * 1.25 / per) + 1.0;
+    return (int)std::max(1.0, std::min(n, 64.0));
+}
+
 long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr) {
     hipStream_t st = stream();
     DevBuf<int> deg(N), cursor(N);
@@ -530,9 +572,10 @@ template <int P>
 void refresh_p(phb_mc_s* M) {
     phb_pme_s* Pm = M->pme;
     require(Pm->z_tx <= 16, "refresh: z tile wider than 16", PHB_ERR_INTERNAL);
-    hipLaunchKernelGGL(k_refresh_kspace<P>, dim3(Pm->z_blocks), dim3(Pm->z_threads), Pm->z_lds(sizeof(cplx<double>)),
-                       stream(), M->dev, Pm->kg64.p, Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->eta64.p, Pm->pz,
-                       Pm->z_tx);
+    const size_t lds = Pm->z_lds(sizeof(cplx<double>)) + (size_t)REFRESH_BATCH * Pm->nz * sizeof(cplx<double>);
+    require(lds <= 160 * 1024, "refresh: z pencil and structure factors exceed LDS", PHB_ERR_UNSUPPORTED);
+    hipLaunchKernelGGL(k_refresh_kspace<P>, dim3(Pm->z_blocks), dim3(Pm->z_threads), lds, stream(), M->dev, Pm->kg64.p,
+                       Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->eta64.p, Pm->pz, Pm->z_tx);
     PmeKernels::launch_c2r_buf(Pm, M->phi.p, true, &M->ctl.p->pending_n);
     PHB_HIP(hipGetLastError());
     ++M->refreshes;
@@ -766,9 +809,7 @@ extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
             // each launch runs until the chunk ends or pending_max changes accumulate;
             // batch size from the trials one launch has been covering
             while (done < chunk_end && status == 0) {
-                const double tpl = M->launches > 0 && M->trials_seen > 0 ? (double)M->trials_seen / M->launches : 8.0;
-                int batch = (int)std::ceil((double)(chunk_end - done) / std::max(tpl, 1.0)) + 1;
-                batch = std::max(1, std::min(batch, 64));
+                const int batch = launch_batch(c, chunk_end - done, M->d.pending_max);
                 const long long before = done;
                 for (int b = 0; b < batch; ++b) {
                     const size_t e0 = M->ev_used;
@@ -859,6 +900,8 @@ extern "C" int phb_mc_set_profiling(phb_mc M, int on) {
         if (M->profiling) M->collect();
         M->profiling = on != 0;
         M->chain_ms = M->refresh_ms = M->recompute_ms = 0.0;
+        PHB_HIP(hipMemsetAsync(M->ctl.p->da_prof, 0, sizeof(M->ctl.p->da_prof), stream()));
+        sync();
         return PHB_OK;
     });
 }

@@ -16,6 +16,7 @@ constexpr int MC_THREADS = 1024;
 constexpr int MC_KMAX = 64;                     // pending atom changes
 constexpr int MC_PMAX = 8;                      // spline order
 constexpr int MC_CW = 2 * MC_PMAX - 1;          // correlation width
+constexpr int REFRESH_BATCH = 8;                // pending atoms per structure-factor batch of the refresh
 
 struct McCtl {
     long long trials_target, trials_done;
@@ -31,6 +32,7 @@ struct McCtl {
     int pend_atom[MC_KMAX];
     double pend_dq[MC_KMAX];
     long long da_stage1, da_rounds;   // delayed-acceptance chain: stage-1 acceptances, rounds
+    long long da_prof[8];             // delayed-acceptance phase times (100 MHz ticks) when profiling
 };
 
 struct McDev {
@@ -276,13 +278,15 @@ __device__ __forceinline__ double corr(const double* wa, const double* wb, int d
 }
 
 // This thread's share (items lt, lt+nt, ...) of the pair, real-space and
-// reciprocal Delta-E of trial t against phi plus the pending changes.
+// reciprocal Delta-E of trial t against phi plus the pending changes; `win`
+// (optional) also receives the share of the mesh-kernel pair items.
 //   items [0, Ri+Rj):       neighbour-row entries (atom i first, then j with i new)
 //   items [.., + 2P^2):     one (z, y) row of dq_a W_a . phi
 //   items [.., + (1+2np)CW): one dz plane of w_ab sum_d C(d) g(delta + d) for the
 //                            pair (i,j) and the pairs (i|j, pending atom)
 template <int P>
-__device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<P>& pd, int pend_n, int lt, int nt) {
+__device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<P>& pd, int pend_n, int lt, int nt,
+                                double* win = nullptr) {
     constexpr int CW = 2 * P - 1, PP = P * P;
     const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
     const int Ri = t.nrow[0], nA = Ri + t.nrow[1], nB = nA + 2 * PP;
@@ -336,7 +340,9 @@ __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<
             double sx = 0.0;
 #pragma unroll
             for (int d = 0; d < CW; ++d) sx += corr<P>(wa[0], wb[0], d) * row[fold(wrap1(x0 + d, D.nx), D.nx)];
-            acc += wgt * corr<P>(wa[2], wb[2], dz) * corr<P>(wa[1], wb[1], dy) * sx;
+            const double v = wgt * corr<P>(wa[2], wb[2], dz) * corr<P>(wa[1], wb[1], dy) * sx;
+            acc += v;
+            if (win) *win += v;   // the mesh-kernel (pair) part on its own
         }
     }
     return acc;
@@ -355,18 +361,26 @@ __device__ __forceinline__ double slab_energy(const McDev& D, double Mz, double 
     return e;
 }
 
-// O(1) self, net-charge and slab terms of a move (MC/src/pme.cu:425-447).
-template <int P>
-__device__ double correction_delta(const McDev& D, const TrialT<P>& t, double Q, double Q2, double Mz, double Qz2) {
-    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
-    const double zi = t.zi, zj = t.zj;
-    const double dQ2 = (t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old);
+// O(1) self, net-charge and slab terms of a move of charges qi -> qin, qj -> qjn
+// (MC/src/pme.cu:425-447).
+__device__ __forceinline__ double corr_move(const McDev& D, double qi, double qin, double qj, double qjn, double zi,
+                                            double zj, double Q, double Mz, double Qz2) {
+    const double dqi = qin - qi, dqj = qjn - qj;
+    const double dQ2 = (qin * qin - qi * qi) + (qjn * qjn - qj * qj);
     const double Qn = Q + dqi + dqj;
     const double Mzn = Mz + dqi * zi + dqj * zj;
     const double Qz2n = Qz2 + dqi * zi * zi + dqj * zj * zj;
     const double e_self = -D.self_coef * dQ2;
     const double e_q = -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q);
-    return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2)) +
+    return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2));
+}
+
+// ... plus the mesh self terms 0.5 dq^2 W^T G W of the two atoms.
+template <int P>
+__device__ double correction_delta(const McDev& D, const TrialT<P>& t, double Q, double Q2, double Mz, double Qz2) {
+    (void)Q2;
+    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+    return corr_move(D, t.qi_old, t.qi_new, t.qj_old, t.qj_new, t.zi, t.zj, Q, Mz, Qz2) +
            0.5 * dqi * dqi * t.self[0] + 0.5 * dqj * dqj * t.self[1];
 }
 

@@ -74,6 +74,8 @@ struct phb_mc_s {
 };
 
 namespace phb {
+// Launches to queue before reading the control block again.
+int launch_batch(const McCtl& c, long long remaining, int pending_max);
 // Full (both directions) neighbour rows, ascending, from a half list; returns the entry count.
 long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr);
 McCtl read_ctl(phb_mc_s* M);

@@ -240,9 +240,22 @@ __global__ __launch_bounds__(512) void k_fft_xy_c2r(const cplx<T>* __restrict__ 
     for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
     for (int k = tid; k < ny; k += bd) stwy[k] = twy[k];
     const cplx<T>* in = kg + (size_t)zp * ny * nkx;
-    for (int idx = tid; idx < ny * nkx; idx += bd) {
-        const int ky = idx / nkx, kx = idx - ky * nkx;
-        H[kx * hs + ky] = in[idx];
+    // all of a thread's loads are issued before the first LDS store (latency overlap)
+    for (int base = tid; base < ny * nkx; base += 8 * bd) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            if (idx < ny * nkx) v[u] = in[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            if (idx < ny * nkx) {
+                const int ky = idx / nkx, kx = idx - ky * nkx;
+                H[kx * hs + ky] = v[u];
+            }
+        }
     }
     __syncthreads();
     lds_fft(H, nkx, hs, py, stwy, +1);
@@ -265,15 +278,32 @@ __global__ __launch_bounds__(512) void k_fft_xy_c2r(const cplx<T>* __restrict__ 
     __syncthreads();
     lds_fft(A, nyp, nx, px, stwx, +1);
     T* plane = out + (size_t)zp * nx * ny;
-    for (int idx = tid; idx < nyp * nx; idx += bd) {
-        const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
-        const cplx<T> v = A[idx];
-        if (accumulate) {
-            plane[y0 * nx + xx] += v.x;
-            if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] += v.y;
+    for (int base = tid; base < nyp * nx; base += 4 * bd) {
+        T o0[4], o1[4];
+        if (accumulate) {   // every load of the chunk first
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = base + u * bd;
+                o0[u] = o1[u] = T(0);
+                if (idx < nyp * nx) {
+                    const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+                    o0[u] = plane[y0 * nx + xx];
+                    if (y0 + 1 < ny) o1[u] = plane[(y0 + 1) * nx + xx];
+                }
+            }
         } else {
-            plane[y0 * nx + xx] = v.x;
-            if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] = v.y;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) o0[u] = o1[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = base + u * bd;
+            if (idx < nyp * nx) {
+                const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+                const cplx<T> v = A[idx];
+                plane[y0 * nx + xx] = o0[u] + v.x;
+                if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] = o1[u] + v.y;
+            }
         }
     }
 }

@@ -414,10 +414,10 @@ class MonteCarlo(_Handle):
 
     def da_record(self, n: int) -> dict:
         out = {"fwd": np.zeros(n, np.int8), "i": np.zeros(n, np.int32), "j": np.zeros(n, np.int32),
-               "stage": np.zeros(n, np.int8), "dE_star": np.zeros(n), "dE": np.zeros(n)}
+               "stage": np.zeros(n, np.int8), "dE_star": np.zeros(n), "dE_star_rev": np.zeros(n), "dE": np.zeros(n)}
         L.call("phb_da_record", self.h, n, out["fwd"].ctypes.data_as(C.POINTER(C.c_byte)), L.iptr(out["i"]),
                L.iptr(out["j"]), out["stage"].ctypes.data_as(C.POINTER(C.c_byte)), L.dptr(out["dE_star"]),
-               L.dptr(out["dE"]))
+               L.dptr(out["dE_star_rev"]), L.dptr(out["dE"]))
         return out
 
 

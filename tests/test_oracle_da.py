@@ -81,9 +81,11 @@ def _tiny4():
     return s, p, t0
 
 
-def test_da_chain_samples_exact_distribution():
+@pytest.mark.parametrize("sym", [False, True])
+def test_da_chain_samples_exact_distribution(sym):
     """Stage 2 restores the exact distribution whatever the surrogate: here a
-    random table that is badly wrong for most moves."""
+    random table that is badly wrong for most moves, antisymmetric under the
+    reverse move or not (the general second-stage rule)."""
     s, p, t0 = _tiny4()
     mc = S.McParams(beta=1.0, mu_eff=0.3)
     states = [(a, b, c, d) for a in (0, 1) for b in (0, 1) for c in (0, 1) for d in (0, 1) if a + b == c + d]
@@ -99,9 +101,11 @@ def test_da_chain_samples_exact_distribution():
     o.set_charges(np.zeros(4), t0)
     w = np.array([math.exp(-(E - mc.mu_eff * (st[0] + st[1]))) for E, st in zip(Es, states)])
     pi = w / w.sum()
-    F = np.random.default_rng(5).normal(0.0, 3.0, (4, 4))
+    rng = np.random.default_rng(5)
+    F = rng.normal(0.0, 3.0, (4, 4))
+    B = rng.normal(0.0, 2.0, (4, 4)) if sym else None    # a part that does not flip with the move
     n = 80000
-    r = o.da_run(mc, n, 2024, sur_pair=F)
+    r = o.da_run(mc, n, 2024, sur_pair=F, sur_pair_sym=B)
     assert r["status"] == 0 and r["n"] == n
     assert (r["stage"] == 2).any() and (r["stage"] == 0).any()
     cur = [0, 0, 0, 0]
