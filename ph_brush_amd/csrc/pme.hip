@@ -329,9 +329,9 @@ double alpha_from_rtol(double rc, double rtol) {  // MC/src/pme.cu:54-63
     return beta;
 }
 
-int fft_threads(long long elems) {
+int fft_threads(long long elems, int min_threads = 256) {
     long long t = (elems + 14) / 15;
-    int b = 256;
+    int b = min_threads;
     while (b < t && b < 512) b *= 2;
     require(t <= 512, "FFT plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
     return b;
@@ -538,7 +538,9 @@ extern "C" int phb_pme_create(phb_params p, const phb_box* box, double rc, int p
         const int nkx = nx / 2 + 1;
         const size_t G = (size_t)nx * ny * nz, Gk = (size_t)nkx * ny * nz;
         // launch shapes
-        P->plane_threads = fft_threads(std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny));
+        // planes of >= 1024 elements get 8 waves (their LDS stages are latency-bound)
+        const long long plane_elems = std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny);
+        P->plane_threads = fft_threads(plane_elems, plane_elems >= 1024 ? 512 : 256);
         const size_t csz = precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
         require(P->plane_lds(csz) <= 160 * 1024, "pmeCreate: xy plane too large for LDS", PHB_ERR_UNSUPPORTED);
         const int ntile0 = (nkx + 7) / 8;
