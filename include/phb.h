@@ -294,6 +294,35 @@ int phb_da_run(phb_mc mc, phb_surrogate s, long long ntrials, phb_da_stats *out)
 int phb_da_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, signed char *stage, double *dE_star,
                   double *dE_star_rev, double *dE);
 
+
+/* ------------------------------------------------------------ batched replicas
+ * No counterpart in the reference (one system per process there).  Replicas of
+ * a titration grid (own system, PME plan and chain each; same spline order,
+ * FP64) advance together: one workgroup per replica in each chain launch, the
+ * potential refreshes of all replicas in two launches, the (chain, refresh)
+ * launch pairs captured in a hipGraph.  Each replica's accept/reject sequence
+ * is the one phb_mc_run would give it. */
+typedef struct phb_batch_s *phb_batch;
+typedef struct {
+    int graph_pairs;           /* (chain, refresh) launch pairs per replay; 0 = 4 */
+    int use_graph;             /* capture the pairs in a hipGraph (1) or launch them directly (0) */
+    int spec_window;           /* out: current speculation window */
+    int pending_max;           /* out: current pending-change budget */
+} phb_batch_desc;
+typedef struct {
+    long long attempted, accepted;   /* totals over the replicas (since creation) */
+    long long launches, replays;     /* chain launches and replays of this call */
+    double ms, trials_per_s;         /* wall time of this call; autotune: rate of the chosen shape */
+    int spec_window, pending_max, graph_pairs;
+} phb_batch_stats;
+int phb_batch_create(const phb_mc *mcs, int n, const phb_batch_desc *d, phb_batch *out);
+int phb_batch_destroy(phb_batch b);
+/* `sweeps` sweeps: every replica attempts as many trials as it has titratable sites per sweep. */
+int phb_batch_run(phb_batch b, int sweeps, phb_batch_stats *st);
+/* Times speculation window x pending budget x graph length on the batch (each candidate runs
+ * `sweeps_per_candidate` real sweeps) and keeps the fastest. */
+int phb_batch_autotune(phb_batch b, int sweeps_per_candidate, phb_batch_stats *best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,19 @@ class DaStats(C.Structure):
         return d
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [("graph_pairs", C.c_int), ("use_graph", C.c_int), ("spec_window", C.c_int), ("pending_max", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("launches", C.c_longlong),
+                ("replays", C.c_longlong), ("ms", C.c_double), ("trials_per_s", C.c_double),
+                ("spec_window", C.c_int), ("pending_max", C.c_int), ("graph_pairs", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 NN_IN, NN_HIDDEN, NN_TRAIN, NN_PARAMS = 48, 128, 22913, 22963
 
 _h = C.c_void_p
@@ -172,6 +185,10 @@ SIGNATURES = {
     "phb_da_predict": [_h, _h, C.c_int, _pi, _pi, _pi, _pd],
     "phb_da_run": [_h, _h, C.c_longlong, C.POINTER(DaStats)],
     "phb_da_record": [_h, C.c_longlong, C.POINTER(C.c_byte), _pi, _pi, C.POINTER(C.c_byte), _pd, _pd, _pd],
+    "phb_batch_create": [C.POINTER(_h), C.c_int, C.POINTER(BatchDesc), C.POINTER(_h)],
+    "phb_batch_destroy": [_h],
+    "phb_batch_run": [_h, C.c_int, C.POINTER(BatchStats)],
+    "phb_batch_autotune": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

@@ -23,7 +23,8 @@ struct McCtl {
     long long accepted, acc_fwd, acc_bwd;
     long long rec_cap;
     int pending_n, pending_max, status, nprot;
-    int pending_folded;           // set by the k-space refresh: phi already holds the pending changes
+    int pending_folded;           // set by the refresh: phi already holds the pending changes
+    int refresh_count;            // workgroups of the refresh's plane pass that finished (reset by the last)
     int sizes[4];                 // nh_n, nh_p, w, cl
     double E, Q, Q2, Mz, Qz2;
     double E_recomputed, max_drift;

@@ -228,84 +228,8 @@ __global__ __launch_bounds__(512) void k_fft_z(cplx<T>* __restrict__ kg, int nx,
 template <typename T>
 __global__ __launch_bounds__(512) void k_fft_xy_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny, FftPlan px,
                              FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
-                             int accumulate, const int* __restrict__ gate) {
-    if (gate && *gate == 0) return;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
-    cplx<T>* stwy = stwx + nx;
-    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, hs = ny + 1;
-    cplx<T>* A = stwy + ny;
-    cplx<T>* H = A + nyp * nx;
-    const int tid = threadIdx.x, bd = blockDim.x, zp = blockIdx.x;
-    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
-    for (int k = tid; k < ny; k += bd) stwy[k] = twy[k];
-    const cplx<T>* in = kg + (size_t)zp * ny * nkx;
-    // all of a thread's loads are issued before the first LDS store (latency overlap)
-    for (int base = tid; base < ny * nkx; base += 8 * bd) {
-        cplx<T> v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int idx = base + u * bd;
-            if (idx < ny * nkx) v[u] = in[idx];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int idx = base + u * bd;
-            if (idx < ny * nkx) {
-                const int ky = idx / nkx, kx = idx - ky * nkx;
-                H[kx * hs + ky] = v[u];
-            }
-        }
-    }
-    __syncthreads();
-    lds_fft(H, nkx, hs, py, stwy, +1);
-    for (int idx = tid; idx < nyp * nx; idx += bd) {
-        const int yp = idx / nx, k = idx - yp * nx, y0 = 2 * yp;
-        const bool hb = y0 + 1 < ny;
-        cplx<T> a, b;
-        if (k < nkx) {
-            a = H[k * hs + y0];
-            b = hb ? H[k * hs + y0 + 1] : cplx<T>{T(0), T(0)};
-            if (k == 0 || 2 * k == nx) { a.y = T(0); b.y = T(0); }
-        } else {
-            const int kk = nx - k;
-            a = H[kk * hs + y0]; a.y = -a.y;
-            b = hb ? H[kk * hs + y0 + 1] : cplx<T>{T(0), T(0)};
-            b.y = -b.y;
-        }
-        A[idx] = {a.x - b.y, a.y + b.x};
-    }
-    __syncthreads();
-    lds_fft(A, nyp, nx, px, stwx, +1);
-    T* plane = out + (size_t)zp * nx * ny;
-    for (int base = tid; base < nyp * nx; base += 4 * bd) {
-        T o0[4], o1[4];
-        if (accumulate) {   // every load of the chunk first
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = base + u * bd;
-                o0[u] = o1[u] = T(0);
-                if (idx < nyp * nx) {
-                    const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
-                    o0[u] = plane[y0 * nx + xx];
-                    if (y0 + 1 < ny) o1[u] = plane[(y0 + 1) * nx + xx];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) o0[u] = o1[u] = T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int idx = base + u * bd;
-            if (idx < nyp * nx) {
-                const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
-                const cplx<T> v = A[idx];
-                plane[y0 * nx + xx] = o0[u] + v.x;
-                if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] = o1[u] + v.y;
-            }
-        }
-    }
+                             int accumulate, RefreshGate gate) {
+    fft_xy_c2r_plane<T>(kg, out, nx, ny, px, py, twx, twy, accumulate, gate, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------ host side
@@ -410,7 +334,7 @@ static void fft_forward_impl(phb_pme_s* P, const T* grid, cplx<T>* kg, const cpl
 
 template <typename T>
 static void fft_inverse_impl(phb_pme_s* P, const cplx<T>* kg, T* out, const cplx<T>* twx, const cplx<T>* twy,
-                             int accumulate, const int* gate = nullptr) {
+                             int accumulate, RefreshGate gate = {}) {
     hipLaunchKernelGGL(k_fft_xy_c2r<T>, dim3(P->nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), stream(),
                        kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate, gate);
     PHB_HIP(hipGetLastError());
@@ -427,14 +351,14 @@ void PmeKernels::launch_fft_buf(phb_pme_s* P, const double* grid, double* phi, i
     require(P->precision == PHB_FP64, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
     fft_forward_impl<double>(P, grid, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
                              P->partials.p, gate);
-    if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate, gate);
+    if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate);
 }
 
-void PmeKernels::launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, const int* gate) {
+void PmeKernels::launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, RefreshGate gate) {
     fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate, gate);
 }
 
-void PmeKernels::launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, const int* gate) {
+void PmeKernels::launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, RefreshGate gate) {
     fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate, gate);
 }
 
@@ -443,7 +367,7 @@ void PmeKernels::launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int
     require(P->precision == PHB_FP32, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
     fft_forward_impl<float>(P, grid, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
                             P->partials.p, gate);
-    if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate, gate);
+    if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate);
 }
 
 void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,

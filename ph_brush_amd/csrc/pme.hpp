@@ -59,6 +59,112 @@ struct phb_pme_s {
 
 namespace phb {
 
+// Gate of the potential refresh's inverse plane pass: it runs only while
+// accepted charge changes are pending and not yet folded, and its last
+// workgroup to finish marks them folded (null fields: no gate).
+struct RefreshGate {
+    const int* pending_n = nullptr;
+    int* folded = nullptr;
+    int* counter = nullptr;
+};
+
+// Inverse plane pass of z-plane zp (one workgroup): C2R along y then x from the
+// half spectrum kg, stored into or accumulated onto out; nplanes workgroups share
+// the gate's completion count.
+template <typename T>
+__device__ __forceinline__ void fft_xy_c2r_plane(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny,
+                                                 const FftPlan& px, const FftPlan& py, const cplx<T>* __restrict__ twx,
+                                                 const cplx<T>* __restrict__ twy, int accumulate, const RefreshGate& gate,
+                                                 int zp, int nplanes) {
+    if (gate.pending_n && (*gate.folded != 0 || *gate.pending_n == 0)) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* stwy = stwx + nx;
+    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, hs = ny + 1;
+    cplx<T>* A = stwy + ny;
+    cplx<T>* H = A + nyp * nx;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
+    for (int k = tid; k < ny; k += bd) stwy[k] = twy[k];
+    const cplx<T>* in = kg + (size_t)zp * ny * nkx;
+    // all of a thread's loads are issued before the first LDS store (latency overlap)
+    for (int base = tid; base < ny * nkx; base += 8 * bd) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            if (idx < ny * nkx) v[u] = in[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * bd;
+            if (idx < ny * nkx) {
+                const int ky = idx / nkx, kx = idx - ky * nkx;
+                H[kx * hs + ky] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    lds_fft(H, nkx, hs, py, stwy, +1);
+    for (int idx = tid; idx < nyp * nx; idx += bd) {
+        const int yp = idx / nx, k = idx - yp * nx, y0 = 2 * yp;
+        const bool hb = y0 + 1 < ny;
+        cplx<T> a, b;
+        if (k < nkx) {
+            a = H[k * hs + y0];
+            b = hb ? H[k * hs + y0 + 1] : cplx<T>{T(0), T(0)};
+            if (k == 0 || 2 * k == nx) { a.y = T(0); b.y = T(0); }
+        } else {
+            const int kk = nx - k;
+            a = H[kk * hs + y0]; a.y = -a.y;
+            b = hb ? H[kk * hs + y0 + 1] : cplx<T>{T(0), T(0)};
+            b.y = -b.y;
+        }
+        A[idx] = {a.x - b.y, a.y + b.x};
+    }
+    __syncthreads();
+    lds_fft(A, nyp, nx, px, stwx, +1);
+    T* plane = out + (size_t)zp * nx * ny;
+    for (int base = tid; base < nyp * nx; base += 4 * bd) {
+        T o0[4], o1[4];
+        if (accumulate) {   // every load of the chunk first
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = base + u * bd;
+                o0[u] = o1[u] = T(0);
+                if (idx < nyp * nx) {
+                    const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+                    o0[u] = plane[y0 * nx + xx];
+                    if (y0 + 1 < ny) o1[u] = plane[(y0 + 1) * nx + xx];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) o0[u] = o1[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = base + u * bd;
+            if (idx < nyp * nx) {
+                const int yp = idx / nx, xx = idx - yp * nx, y0 = 2 * yp;
+                const cplx<T> v = A[idx];
+                plane[y0 * nx + xx] = o0[u] + v.x;
+                if (y0 + 1 < ny) plane[(y0 + 1) * nx + xx] = o1[u] + v.y;
+            }
+        }
+    }
+    if (gate.counter) {   // the last workgroup marks the pending changes folded
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            if (atomicAdd(gate.counter, 1) == nplanes - 1) {
+                *gate.counter = 0;
+                *gate.folded = 1;
+            }
+        }
+    }
+}
+
 struct PmeKernels {
     static void launch_spread(phb_pme_s* P, const phb_atoms_s* a);
     // forward r2c + z pass (zmode bits: 1 energy, 2 convolve + inverse z),
@@ -69,8 +175,8 @@ struct PmeKernels {
                                bool accumulate, const int* gate = nullptr);
     static void launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int zmode, bool inverse_to_phi,
                                bool accumulate, const int* gate = nullptr);
-    static void launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, const int* gate = nullptr);
-    static void launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, const int* gate = nullptr);
+    static void launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, RefreshGate gate = {});
+    static void launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, RefreshGate gate = {});
     static void launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
                             int mode, double alpha, double kf, int precision, double* partials, int nblocks);
     static void launch_moments(const phb_atoms_s* a, double* partials, int nblocks);

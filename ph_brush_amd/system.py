@@ -491,6 +491,29 @@ class Surrogate(_Handle):
         return r.value, a.value
 
 
+class Batch(_Handle):
+    """Replicas advanced together (phb_batch_*): one workgroup per replica per
+    chain launch, batched refreshes, launch pairs under a hipGraph."""
+    _destroy = "phb_batch_destroy"
+
+    def __init__(self, mcs, graph_pairs: int = 4, use_graph: bool = True):
+        super().__init__()
+        self._keep = list(mcs)
+        arr = (C.c_void_p * len(mcs))(*[m.h.value for m in mcs])
+        d = L.BatchDesc(graph_pairs, int(use_graph), 0, 0)
+        L.call("phb_batch_create", arr, len(mcs), C.byref(d), C.byref(self.h))
+
+    def run(self, sweeps: int) -> dict:
+        st = L.BatchStats()
+        L.call("phb_batch_run", self.h, sweeps, C.byref(st))
+        return st.as_dict()
+
+    def autotune(self, sweeps_per_candidate: int = 1) -> dict:
+        st = L.BatchStats()
+        L.call("phb_batch_autotune", self.h, sweeps_per_candidate, C.byref(st))
+        return st.as_dict()
+
+
 class Replica:
     """One system set up on the device the way main() does (MC/src/main.cu:106-138)."""
 

@@ -134,3 +134,26 @@ def test_rng_state_matches_std_mt19937(gpu):
     for i in range(1, 624):
         x.append((1812433253 * (x[-1] ^ (x[-1] >> 30)) + i) & 0xFFFFFFFF)
     assert np.array_equal(st[:624], np.array(x, np.uint32)) and st[624] == 624
+
+
+@pytest.mark.parametrize("form", ["plain", "da"])
+def test_running_energy_over_uneven_runs(gpu, form):
+    """Runs of uneven length with no intermediate full recompute: launches queued
+    beyond a run's target must leave the potential grid untouched, so the
+    running energy still equals a fresh recompute after many runs."""
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2)
+    mc = S.McParams.at_ph(7.9)
+    r = S.Replica(s, p, mc, seed=11, pending_max=8, recompute_interval=0)
+    if form == "da":
+        r.mc.da_attach(r_desc=6.0)
+        sur = S.Surrogate(0)
+        run = lambda n: r.mc.da_run(sur, n)
+    else:
+        run = r.mc.run
+    for n in (123, 77, 301, 5, 250, 999):
+        run(n)
+    st = r.mc.stats()
+    assert st["attempted"] == 1755 and st["accepted"] > 20
+    e_run = st["E"]
+    e_re = r.mc.recompute()
+    assert abs(e_run - e_re) <= 1e-9 * max(1.0, abs(e_re)), (e_run, e_re)
