@@ -88,12 +88,100 @@ def _timed(run, K, W, T, label):
     return time.perf_counter() - t0, alphas
 
 
+def fp32_form(S, L, SG, args):
+    """The 2,640-atom brush in the FP32 form vs the FP64 form: trial moves/s of
+    both chains (recompute every 1,000 trials), Delta-E error over 10^5 sampled
+    moves against the FP64 path (which equals the oracle's full recompute within
+    1e-10), total-energy error of the FP32 PME plan against the FP64 oracle."""
+    import numpy as np
+    s, p, mc = S.config_small_fp64()
+    res = {}
+    for name, prec in (("fp64", L.PHB_FP64), ("fp32", L.PHB_FP32)):
+        rep = S.Replica(s, p, mc, seed=1, pending_max=8, recompute_interval=1000, precision=prec)
+        rep.mc.run(args.fp32_trials)            # warm-up from the start state
+        t0 = time.perf_counter()
+        rep.mc.run(args.fp32_trials)
+        res[name] = (args.fp32_trials / (time.perf_counter() - t0), rep)
+    _, t = res["fp64"][1].atoms.identity()
+    rng = np.random.default_rng(3)
+    i, j, f = SG.sample_moves(rng, t, s.z, mc, 100000)
+    # both forms evaluated in the same state (a fresh FP32 chain built on the FP64 chain's current identities)
+    q64, t64 = res["fp64"][1].atoms.identity()
+    r32 = S.Replica(S.System(s.x, s.y, s.z, q64, t64, s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data), p, mc, seed=1,
+                    precision=L.PHB_FP32)
+    d64 = res["fp64"][1].mc.delta_e(i, j, f)
+    d32 = r32.mc.delta_e(i, j, f)
+    err = d32 - d64
+    from oracle import pyoracle as O
+    o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
+    o.nlist()
+    e64, _ = o.total_energy()
+    atoms = S.DeviceAtoms(s)
+    box = S.box_of(s)
+    neigh = S.build_half_neighbors_gpu(atoms, box, S.DeviceCSR.from_host(*S.bond_adjacency(s.natoms, s.bonds)), p.rc)
+    params = S.DeviceParams(p)
+    e32, _ = S.total_energy(atoms, params, neigh, S.PMEPlan(params, box, p.rc, precision=L.PHB_FP32), box)
+    print(f"[bench] fp32 form: {res['fp32'][0]:.0f} vs fp64 {res['fp64'][0]:.0f} trial moves/s", file=sys.stderr,
+          flush=True)
+    return {"value": res["fp32"][0], "fp64_value": res["fp64"][0], "unit": "trial moves/s",
+            "workload": "64x20 all-titratable brush, 2,640 atoms, SPME 32x32x128 order 5, pH 7.5",
+            "delta_e_error_vs_fp64": {"moves": len(i), "rms_kT": float(np.sqrt(np.mean(err ** 2))),
+                                      "max_abs_kT": float(np.abs(err).max()),
+                                      "rms_delta_e_kT": float(np.sqrt(np.mean(d64 ** 2)))},
+            "total_energy_rel_error_vs_oracle": abs(e32 - e64) / abs(e64)}
+
+
+def batched_form(S, args):
+    """32 replicas of a pH x salt titration grid (uneven sizes and grids)
+    advanced together; aggregate trial moves/s over timed sweeps after the
+    autotuner has picked the launch shape, alpha per replica, and the 2,640-atom
+    brush's alpha(pH) run serially for comparison."""
+    t0 = time.perf_counter()
+    grid = S.config_titration_grid()
+    reps = [S.Replica(s, p, mc, seed=7 + k, recompute_interval=0) for k, (s, p, mc, _) in enumerate(grid)]
+    b = S.Batch([r.mc for r in reps])
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] batched: {len(reps)} replicas, {sum(r.s.natoms for r in reps)} atoms, setup {t_setup:.1f}s",
+          file=sys.stderr, flush=True)
+    tune = b.autotune(1)
+    print(f"[bench] batched: autotune -> W={tune['spec_window']} pending={tune['pending_max']} "
+          f"pairs={tune['graph_pairs']}", file=sys.stderr, flush=True)
+    b.run(args.batch_warmup)
+    st = b.run(args.batch_sweeps)
+    rate = st["trials_per_s"]
+    alphas = {}
+    for (s, p, mc, label), r in zip(grid, reps):
+        ss = r.mc.stats()
+        alphas[label.split(":")[0]] = ss["nprot"] / ss["nsites"]
+    print(f"[bench] batched: {rate:.0f} trial moves/s aggregate", file=sys.stderr, flush=True)
+    # serial reference curve on the 2,640-atom brush
+    s1, p1, _ = S.config_small_fp64()
+    curve = {}
+    for ph in (5.0, 5.5, 6.0, 6.5, 7.0, 7.5, 8.0, 8.5):
+        rep = S.Replica(s1, p1, S.McParams.at_ph(ph), seed=1, recompute_interval=0)
+        rep.mc.run(args.serial_equil)
+        a = []
+        for _ in range(20):
+            a.append(rep.mc.run(args.serial_trials // 20)["nprot"] / 1280.0)
+        curve[f"pH {ph}"] = sum(a) / len(a)
+        del rep
+    print("[bench] batched: serial alpha(pH) of the 2,640-atom brush done", file=sys.stderr, flush=True)
+    return {"value": rate, "unit": "trial moves/s (aggregate over replicas)", "replicas": len(reps),
+            "atoms": sum(r.s.natoms for r in reps), "sweeps_timed": args.batch_sweeps, "ms": st["ms"],
+            "trials_timed": st["attempted"], "launches": st["launches"], "graph_replays": st["replays"],
+            "autotuned": {"spec_window": tune["spec_window"], "pending_max": tune["pending_max"],
+                          "graph_pairs": tune["graph_pairs"], "trials_per_s": tune["trials_per_s"]},
+            "alpha_by_replica": alphas, "alpha_curve_2640_serial": curve,
+            "note": "alpha after warm-up + timed sweeps from p_prot = 0.5 starts (no equilibration beyond them)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--trials-per-step", type=int, default=40000)
+    ap.add_argument("--trials-per-step", type=int, default=200000,
+                    help="trials per step; the workload's 10^6 trial moves are 5 steps")
     ap.add_argument("--pending-max", type=int, default=8)
     ap.add_argument("--spec-window", type=int, default=8)
     ap.add_argument("--recompute-interval", type=int, default=20000,
@@ -102,6 +190,13 @@ def main():
     ap.add_argument("--da-epochs", type=int, default=20)
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--fp32-trials", type=int, default=100000)
+    ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
+    ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
+    ap.add_argument("--serial-equil", type=int, default=20000)
+    ap.add_argument("--serial-trials", type=int, default=60000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one replica chain on one GPU "
@@ -257,6 +352,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_fp32:
+        out["forms"]["fp32"] = fp32_form(S, L, SG, args)
+    if not args.no_batched:
+        out["forms"]["batched"] = batched_form(S, args)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         n = args.cpu_trials or 150

@@ -264,6 +264,8 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
             phb_mc_s* M = B->mcs[r];
             phb_pme_s* Pm = M->pme;
             require(M->dev.P == B->P, "phb_batch_create: every replica needs the same spline order", PHB_ERR_UNSUPPORTED);
+            require(M->d.precision == PHB_FP64, "phb_batch_create: batched replicas run the FP64 form",
+                    PHB_ERR_UNSUPPORTED);
             for (int q = 0; q < r; ++q) require(B->mcs[q] != M, "phb_batch_create: a replica appears twice");
             require(Pm->z_tx <= 16, "phb_batch_create: z tile wider than 16", PHB_ERR_INTERNAL);
             lds = lds && M->lds_pools;

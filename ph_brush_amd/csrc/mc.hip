@@ -39,13 +39,13 @@
 
 namespace phb {
 
-template <int P, typename PoolT, bool LDS_POOLS>
+template <int P, typename PoolT, bool LDS_POOLS, typename T = double>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_chain(McDev D) {
-    chain_body<P, PoolT, LDS_POOLS>(D);
+    chain_body<P, PoolT, LDS_POOLS, T>(D);
 }
 
 // Delta-E of candidate moves, one block per move, no acceptance.
-template <int P>
+template <int P, typename T = double>
 __global__ __launch_bounds__(MC_THREADS) void k_mc_probe(McDev D, int n, const int* __restrict__ mi,
                                                          const int* __restrict__ mj, const int* __restrict__ mf,
                                                          double* __restrict__ out) {
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_probe(McDev D, int n, const i
     __syncthreads();
     stage_trial_data<P>(D, s_t, tid, blockDim.x);
     __syncthreads();
-    double part = wave_sum(trial_partial<P>(D, s_t, s_pd, pend_n, tid, blockDim.x));
+    double part = wave_sum(trial_partial<P, T>(D, s_t, s_pd, pend_n, tid, blockDim.x));
     if (lane == 0) s_wsum[wave] = part;
     __syncthreads();
     if (tid == 0) {
@@ -212,6 +212,11 @@ __global__ void k_fold_g(int nx, int ny, int nz, const double* __restrict__ gk, 
     }
 }
 
+__global__ void k_cast_f32(const double* __restrict__ in, float* __restrict__ out, long long n) {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        out[k] = (float)in[k];
+}
+
 __global__ void k_delta_grid(double* __restrict__ g) {
     if (threadIdx.x == 0 && blockIdx.x == 0) g[0] = 1.0;
 }
@@ -229,6 +234,18 @@ int launch_batch(const McCtl& c, long long remaining, int pending_max) {
     const double per = std::max(1, pending_max / 2);
     const double n = std::ceil(std::max(acc, 1e-3) * (double)remaining * 1.25 / per) + 1.0;
     return (int)std::max(1.0, std::min(n, 64.0));
+}
+
+void cast_f32(const double* in, float* out, long long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cast_f32, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, stream(),
+                       in, out, n);
+    PHB_HIP(hipGetLastError());
+}
+
+// the FP32 form reads an FP32 copy of the potential grid
+void sync_phi32(phb_mc_s* M) {
+    if (M->d.precision == PHB_FP32) cast_f32(M->phi.p, M->phi32.p, (long long)M->phi.n);
 }
 
 long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr) {
@@ -276,6 +293,7 @@ void full_recompute(phb_mc_s* M) {
     reduce_partials(P->partials.p, nbp, 4, 4, res + 3);
     hipLaunchKernelGGL(k_finish_recompute, dim3(1), dim3(64), 0, stream(), M->dev, res, 0.0);
     PHB_HIP(hipGetLastError());
+    sync_phi32(M);
     if (M->profiling) { M->event(); M->marks.push_back({e0, 1}); }
     ++M->recomputes;
 }
@@ -294,6 +312,7 @@ void refresh_p(phb_mc_s* M) {
     g.counter = &M->ctl.p->refresh_count;
     PmeKernels::launch_c2r_buf(Pm, M->phi.p, true, g);
     PHB_HIP(hipGetLastError());
+    sync_phi32(M);
     ++M->refreshes;
 }
 
@@ -308,19 +327,24 @@ void refresh(phb_mc_s* M) {
     }
 }
 
-template <int P>
+template <int P, typename T = double>
 void launch_chain_p(phb_mc_s* M) {
     if (M->lds_pools) {
-        auto kern = k_mc_chain<P, unsigned short, true>;
+        auto kern = k_mc_chain<P, unsigned short, true, T>;
         PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)M->pool_lds_bytes));
         hipLaunchKernelGGL(kern, dim3(1), dim3(MC_THREADS), M->pool_lds_bytes, stream(), M->dev);
     } else {
-        hipLaunchKernelGGL((k_mc_chain<P, int, false>), dim3(1), dim3(MC_THREADS), 0, stream(), M->dev);
+        hipLaunchKernelGGL((k_mc_chain<P, int, false, T>), dim3(1), dim3(MC_THREADS), 0, stream(), M->dev);
     }
 }
 
 static void launch_chain(phb_mc_s* M) {
+    if (M->d.precision == PHB_FP32) {   // FP32 form: orders 4 and 5 (checked at creation)
+        if (M->dev.P == 4) launch_chain_p<4, float>(M);
+        else launch_chain_p<5, float>(M);
+        return;
+    }
     switch (M->dev.P) {
         case 3: launch_chain_p<3>(M); break;
         case 4: launch_chain_p<4>(M); break;
@@ -335,6 +359,11 @@ void launch_probe(phb_mc_s* M, int n, const int* i, const int* j, const int* f, 
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(n), dim3(MC_THREADS), 0, stream(), M->dev, n, i, j, f, out);
     };
+    if (M->d.precision == PHB_FP32) {
+        if (M->dev.P == 4) go(k_mc_probe<4, float>);
+        else go(k_mc_probe<5, float>);
+        return;
+    }
     switch (M->dev.P) {
         case 3: go(k_mc_probe<3>); break;
         case 4: go(k_mc_probe<4>); break;
@@ -372,8 +401,12 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         require(a && p && neigh && pme && box && d && out, "phb_mc_create: null argument");
         require(p->d.coulombtype == PHB_COUL_PME, "phb_mc_create: the device chain needs coulombtype=pme",
                 PHB_ERR_UNSUPPORTED);
-        require(d->precision == PHB_FP64, "phb_mc_create: only PHB_FP64 is supported by the chain", PHB_ERR_UNSUPPORTED);
-        require(pme->precision == PHB_FP64, "phb_mc_create: PME plan must be FP64", PHB_ERR_UNSUPPORTED);
+        require(d->precision == PHB_FP64 || d->precision == PHB_FP32, "phb_mc_create: precision must be PHB_FP64 or PHB_FP32",
+                PHB_ERR_UNSUPPORTED);
+        require(d->precision == PHB_FP64 || pme->order == 4 || pme->order == 5,
+                "phb_mc_create: the FP32 form supports spline orders 4 and 5", PHB_ERR_UNSUPPORTED);
+        require(pme->precision == PHB_FP64, "phb_mc_create: the PME plan must be FP64 (it keeps the FP64 potential "
+                                            "from which the FP32 form's copy is made)", PHB_ERR_UNSUPPORTED);
         require(neigh->rows == a->n, "phb_mc_create: neighbour list rows != natoms");
         require(pme->nx % 2 == 0, "phb_mc_create: nx must be even (C2R potential grid)", PHB_ERR_UNSUPPORTED);
         require(a->max_type < p->d.ntypes, "phb_mc_create: atom type id outside the LJ table");
@@ -458,6 +491,14 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         M->selfw.alloc(N);
         hipLaunchKernelGGL(k_self_terms, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, P, pme->nx, pme->ny, pme->nz,
                            M->sw.p, M->gk.p, M->selfw.p);
+        // ---- FP32 form: FP32 copies of the real-space coefficients, mesh kernel and potential
+        if (d->precision == PHB_FP32) {
+            M->kreal32.alloc(std::max<long long>(nfull, 1));
+            M->gf32.alloc(GF);
+            M->phi32.alloc(G);
+            cast_f32(M->kreal.p, M->kreal32.p, nfull);
+            cast_f32(M->gf.p, M->gf32.p, (long long)GF);
+        }
         // ---- records
         const long long rc = d->record_cap;
         M->rec_fwd.alloc(std::max<long long>(rc, 1)); M->rec_acc.alloc(std::max<long long>(rc, 1));
@@ -485,6 +526,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         D.fhead = M->fhead.p; D.fnbr = M->fnbr.p; D.kreal = M->kreal.p; D.r2 = M->r2.p;
         D.sbase = M->sbase.p; D.sw = M->sw.p; D.phi = M->phi.p; D.gk = M->gk.p; D.selfw = M->selfw.p;
         D.gf = M->gf.p; D.fnx = pme->nx / 2 + 1; D.fny = pme->ny / 2 + 1;
+        D.kreal32 = M->kreal32.p; D.gf32 = M->gf32.p; D.phi32 = M->phi32.p;
         D.nx = pme->nx; D.ny = pme->ny; D.nz = pme->nz; D.P = P;
         D.T = T; D.eps = p->d_eps.p; D.sig = p->d_sig.p; D.shift = p->d_shift.p;
         D.use_shift = p->d.lj_shift ? 1 : 0; D.use_cut = p->lj_cut_active ? 1 : 0; D.lj_rc2 = p->lj_rc * p->lj_rc;

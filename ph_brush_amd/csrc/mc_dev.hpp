@@ -47,6 +47,7 @@ struct McDev {
     const double* sw;     // 3*P per atom: weights x[P], y[P], z[P]
     const double *phi, *gk;
     const double* gf;     // g folded to |offset| per axis: (nz/2+1) x (ny/2+1) x (nx/2+1)
+    const float *kreal32, *phi32, *gf32;   // FP32 copies for the FP32 form (null otherwise)
     int fnx, fny;         // nx/2+1, ny/2+1
     const double* selfw;  // per atom W_a^T G W_a (positions are fixed)
     int nx, ny, nz, P;
@@ -222,17 +223,32 @@ __device__ __forceinline__ int pool_slot(const McDev& D, int pool, int k) {
 }
 
 // ------------------------------------------------------------ Delta-E
-__device__ __forceinline__ double lj_pair(const McDev& D, int t, int tn, double r2) {
-    if (D.use_cut && r2 > D.lj_rc2) return 0.0;
+template <typename T = double>
+__device__ __forceinline__ T lj_pair(const McDev& D, int t, int tn, double r2) {
+    if (D.use_cut && r2 > D.lj_rc2) return T(0);
     const int k = t * D.T + tn;
-    const double eps = D.eps[k], sig = D.sig[k];
-    if (eps <= 0.0 || sig <= 0.0) return 0.0;
-    const double r = sqrt(r2);
-    const double sr = sig / r, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
-    double e = 4.0 * eps * (sr12 - sr6);
-    if (D.use_shift) e -= D.shift[k];
+    const T eps = (T)D.eps[k], sig = (T)D.sig[k];
+    if (eps <= T(0) || sig <= T(0)) return T(0);
+    const T r = sqrt((T)r2);
+    const T sr = sig / r, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+    T e = T(4) * eps * (sr12 - sr6);
+    if (D.use_shift) e -= (T)D.shift[k];
     return e;
 }
+
+// The per-move arrays in the value type T of a Delta-E evaluation.
+template <typename T>
+struct McArrays;
+template <>
+struct McArrays<double> {
+    const double *kreal, *phi, *gf;
+    __device__ explicit McArrays(const McDev& D) : kreal(D.kreal), phi(D.phi), gf(D.gf) {}
+};
+template <>
+struct McArrays<float> {
+    const float *kreal, *phi, *gf;
+    __device__ explicit McArrays(const McDev& D) : kreal(D.kreal32), phi(D.phi32), gf(D.gf32) {}
+};
 
 // One speculative trial (LDS).
 template <int P>
@@ -265,15 +281,15 @@ __device__ __forceinline__ int fold(int x, int n) {   // x in [0, n) -> periodic
     return 2 * x <= n ? x : n - x;
 }
 
-// Stencil correlation sum_{u - v = d} wa[u] wb[v] for d = dd - (P-1).
-template <int P>
-__device__ __forceinline__ double corr(const double* wa, const double* wb, int dd) {
+// Stencil correlation sum_{u - v = d} wa[u] wb[v] for d = dd - (P-1), in T.
+template <int P, typename T = double>
+__device__ __forceinline__ T corr(const double* wa, const double* wb, int dd) {
     const int d = dd - (P - 1);
-    double c = 0.0;
+    T c = T(0);
 #pragma unroll
     for (int u = 0; u < P; ++u) {
         const int v = u - d;
-        if (v >= 0 && v < P) c += wa[u] * wb[v];
+        if (v >= 0 && v < P) c += (T)wa[u] * (T)wb[v];
     }
     return c;
 }
@@ -285,14 +301,17 @@ __device__ __forceinline__ double corr(const double* wa, const double* wb, int d
 //   items [.., + 2P^2):     one (z, y) row of dq_a W_a . phi
 //   items [.., + (1+2np)CW): one dz plane of w_ab sum_d C(d) g(delta + d) for the
 //                            pair (i,j) and the pairs (i|j, pending atom)
-template <int P>
+// The arithmetic of every item runs in T (per-thread sums in T as well); the
+// caller reduces the per-thread results in FP64.
+template <int P, typename T = double>
 __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<P>& pd, int pend_n, int lt, int nt,
                                 double* win = nullptr) {
     constexpr int CW = 2 * P - 1, PP = P * P;
-    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+    const McArrays<T> arr(D);
+    const T dqi = (T)(t.qi_new - t.qi_old), dqj = (T)(t.qj_new - t.qj_old);
     const int Ri = t.nrow[0], nA = Ri + t.nrow[1], nB = nA + 2 * PP;
     const int total = nB + (1 + 2 * pend_n) * CW * CW;
-    double acc = 0.0;
+    T acc = T(0);
     for (int it = lt; it < total; it += nt) {
         if (it < nA) {
             const bool first = it < Ri;
@@ -302,34 +321,34 @@ __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<
             int tn;
             if (!first && n == t.i) { qn = t.qi_new; tn = t.ti_new; }
             else { qn = D.q[n]; tn = D.type[n]; }
-            double v = (first ? dqi : dqj) * qn * D.kreal[e];
+            T v = (first ? dqi : dqj) * (T)qn * arr.kreal[e];
             const int to = first ? t.ti_old : t.tj_old, tw = first ? t.ti_new : t.tj_new;
             if (to != tw) {
                 const double r2 = D.r2[e];
-                v += lj_pair(D, tw, tn, r2) - lj_pair(D, to, tn, r2);
+                v += lj_pair<T>(D, tw, tn, r2) - lj_pair<T>(D, to, tn, r2);
             }
             acc += v;
         } else if (it < nB) {
             const int s = it - nA, which = s / PP, r = s - which * PP, wz = r / P, wy = r - wz * P;
             const int* sb = t.sb[which];
             const int gz = wrap1(sb[2] + wz, D.nz), gy = wrap1(sb[1] + wy, D.ny);
-            const double* row = D.phi + ((size_t)gz * D.ny + gy) * D.nx;
-            double sx = 0.0;
+            const T* row = arr.phi + ((size_t)gz * D.ny + gy) * D.nx;
+            T sx = T(0);
 #pragma unroll
-            for (int u = 0; u < P; ++u) sx += t.w[which][0][u] * row[wrap1(sb[0] + u, D.nx)];
-            acc += (which ? dqj : dqi) * t.w[which][2][wz] * t.w[which][1][wy] * sx;
+            for (int u = 0; u < P; ++u) sx += (T)t.w[which][0][u] * row[wrap1(sb[0] + u, D.nx)];
+            acc += (which ? dqj : dqi) * (T)t.w[which][2][wz] * (T)t.w[which][1][wy] * sx;
         } else {
             const int s = it - nB, pr = s / (CW * CW), r = s - pr * (CW * CW), dz = r / CW, dy = r - dz * CW;
             int ta;
             const int* sbb;
             const double(*wb)[P];
-            double wgt;
+            T wgt;
             if (pr == 0) { ta = 0; sbb = t.sb[1]; wb = t.w[1]; wgt = dqi * dqj; }
             else {
                 const int m = (pr - 1) >> 1;
                 ta = (pr - 1) & 1;
                 sbb = pd.sb[m]; wb = pd.w[m];
-                wgt = (ta ? dqj : dqi) * pd.dq[m];
+                wgt = (ta ? dqj : dqi) * (T)pd.dq[m];
             }
             const int* sba = t.sb[ta];
             const double(*wa)[P] = t.w[ta];
@@ -337,16 +356,16 @@ __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<
             const int x0 = wrap1(sba[0] - sbb[0], D.nx) - (P - 1);
             const int gy = fold(wrap1(wrap1(sba[1] - sbb[1], D.ny) - (P - 1) + dy, D.ny), D.ny);
             const int gz = fold(wrap1(wrap1(sba[2] - sbb[2], D.nz) - (P - 1) + dz, D.nz), D.nz);
-            const double* row = D.gf + ((size_t)gz * D.fny + gy) * D.fnx;
-            double sx = 0.0;
+            const T* row = arr.gf + ((size_t)gz * D.fny + gy) * D.fnx;
+            T sx = T(0);
 #pragma unroll
-            for (int d = 0; d < CW; ++d) sx += corr<P>(wa[0], wb[0], d) * row[fold(wrap1(x0 + d, D.nx), D.nx)];
-            const double v = wgt * corr<P>(wa[2], wb[2], dz) * corr<P>(wa[1], wb[1], dy) * sx;
+            for (int d = 0; d < CW; ++d) sx += corr<P, T>(wa[0], wb[0], d) * row[fold(wrap1(x0 + d, D.nx), D.nx)];
+            const T v = wgt * corr<P, T>(wa[2], wb[2], dz) * corr<P, T>(wa[1], wb[1], dy) * sx;
             acc += v;
-            if (win) *win += v;   // the mesh-kernel (pair) part on its own
+            if (win) *win += (double)v;   // the mesh-kernel (pair) part on its own
         }
     }
-    return acc;
+    return (double)acc;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -22,6 +22,7 @@ struct phb_mc_s {
     int nsites = 0, nions = 0;
     DevBuf<int> fhead, fnbr, sbase;
     DevBuf<double> kreal, r2, sw, phi, gk, gf, dgrid, selfw;
+    DevBuf<float> kreal32, gf32, phi32;   // the FP32 form's copies
     DevBuf<int> pbuf;
     bool lds_pools = false;
     size_t pool_lds_bytes = 0;

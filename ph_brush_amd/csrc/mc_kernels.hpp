@@ -7,7 +7,7 @@
 namespace phb {
 
 // One persistent workgroup runs the chain of replica D (see mc.hip).
-template <int P, typename PoolT, bool LDS_POOLS>
+template <int P, typename PoolT, bool LDS_POOLS, typename T = double>
 __device__ __forceinline__ void chain_body(const McDev& D) {
     __shared__ RngLds s_rng;
     __shared__ TrialT<P> s_tr[16];
@@ -120,7 +120,7 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
         if (k < ntr) stage_trial_data<P>(D, s_tr[k], lt, nt);
         __syncthreads();
         double part = 0.0;
-        if (k < ntr) part = trial_partial<P>(D, s_tr[k], s_pd, pend_n, lt, nt);
+        if (k < ntr) part = trial_partial<P, T>(D, s_tr[k], s_pd, pend_n, lt, nt);
         part = wave_sum(part);
         if (lane == 0) s_wsum[wave] = part;
         __syncthreads();

@@ -168,6 +168,34 @@ def config_30k_fp64(seed: int = 1):
     return s, Params(grid=(64, 64, 256)), McParams.at_ph(7.0)
 
 
+def config_titration_grid(seed: int = 7, ph_values=(5.0, 5.5, 6.0, 6.5, 7.0, 7.5, 8.0, 8.5),
+                          salts=(200, 1000, 3000, 6000), n_min: int = 5000, n_max: int = 35000):
+    """Replicas of a titration grid: every (pH, salt pairs) combination with its
+    own brush -- chains_x, chains_y in [10, 20] (100-400 chains) on a 2-sigma
+    lattice and chain length in [20, 60], drawn from numpy's default_rng(seed)
+    and redrawn until the replica has n_min..n_max atoms.  Every bead titratable
+    (pKa 7.5, initially protonated with p = 0.5), one counter-particle per bead;
+    box 2*chains_x x 2*chains_y x 2.5*length (x3 slab extension); SPME grid 32 or
+    64 laterally and 128 or 256 along z by box size, order 5.  Returns a list of
+    (System, Params, McParams, label)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for salt in salts:
+        for ph in ph_values:
+            while True:
+                cx, cy, ln = int(rng.integers(10, 21)), int(rng.integers(10, 21)), int(rng.integers(20, 61))
+                n = 2 * cx * cy * ln + 2 * salt
+                if n_min <= n <= n_max:
+                    break
+            Lx, Ly, Lz = 2.0 * cx, 2.0 * cy, 2.5 * ln
+            s = gen_brush(cx, cy, ln, Lx, Ly, Lz, n_perm=0, p_prot=0.5, n_salt=salt, seed=int(rng.integers(1, 2**31)),
+                          zfac=3.0)
+            grid = (32 if Lx <= 24 else 64, 32 if Ly <= 24 else 64, 128 if 3.0 * Lz <= 200 else 256)
+            out.append((s, Params(grid=grid), McParams.at_ph(ph),
+                        f"pH {ph} salt {salt}: {cx}x{cy} chains x {ln}, {s.natoms} atoms, grid {grid}"))
+    return out
+
+
 def config_tiny(seed: int = 3, order: int = 4, grid=(16, 16, 48), chains=(3, 3), chain_len=6, n_perm=0,
                 n_salt=6, box=(6.0, 6.0, 16.0)):
     """A few hundred atoms for quick parity checks."""
