@@ -396,6 +396,7 @@ struct DaDev {
     int M;                        // candidates per round
     int prof;                     // accumulate per-phase times into McCtl::da_prof
     double* rec_fy;               // per-trial record of the reverse surrogate dE*(y->x)
+    const double* phiw;           // W_a . phi per atom, recomputed after every refresh of phi
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -529,6 +530,13 @@ __device__ double stencil_dot(const McDev& D, int a) {
     double s = 0.0;
     for (int r = 0; r < P * P; ++r) s += stencil_row<P>(D, a, r);
     return s;
+}
+
+// W_a . phi of every atom (one thread per atom), after each refresh of phi.
+template <int P>
+__global__ void k_phi_dot(McDev D, double* __restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < D.N) out[a] = stencil_dot<P>(D, a);
 }
 
 // Physics part B = lin + selfq + pr + corr of n moves (one thread per move).
@@ -740,8 +748,8 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         clk.lap(prof, s_prof, 1);
         if (s_dc.stop) break;
         const int ntr = s_dc.ntr;
-        // ---- 2. per-candidate physics terms, W . phi of both atoms (one half-wave per atom),
-        //         and the raw shell-count differences
+        // ---- 2. per-candidate physics terms (W . phi from the per-atom table) and the raw
+        //         shell-count differences
         if (tid < ntr) {
             double r;
             int exA, exB;
@@ -749,33 +757,8 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             pair_terms(A, D, i, j, r, exA, exB);
             s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
             s_c.mt[tid] = move_terms(D, A, i, j, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
-        }
-        {
-            // one half-wave per (candidate, atom); a half-wave's rows of all its pairs are
-            // loaded before any reduction, so the potential-grid reads overlap
-            constexpr int NHW = DA_THREADS / 32, PER = (2 * DA_M + NHW - 1) / NHW;
-            const int hw = tid >> 5, l32 = tid & 31;
-            double v[PER];
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int pidx = hw + q * NHW;
-                v[q] = 0.0;
-                if (pidx < 2 * ntr && l32 < P * P) {
-                    const int k = pidx >> 1, a = (pidx & 1) ? s_c.j[k] : s_c.i[k];
-                    v[q] = stencil_row<P>(D, a, l32);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const int pidx = hw + q * NHW;
-                double x = v[q];
-                x += __shfl_xor(x, 16, 64);
-                x += __shfl_xor(x, 8, 64);
-                x += __shfl_xor(x, 4, 64);
-                x += __shfl_xor(x, 2, 64);
-                x += __shfl_xor(x, 1, 64);
-                if (l32 == 0 && pidx < 2 * ntr) s_c.lin[pidx >> 1][pidx & 1] = x;
-            }
+            s_c.lin[tid][0] = A.phiw[i];
+            s_c.lin[tid][1] = A.phiw[j];
         }
         int raw[DA_FEAT];
 #pragma unroll
@@ -1044,6 +1027,7 @@ struct DaState {
     DevBuf<unsigned short> H;
     DevBuf<double> x, y;       // copies of the positions (fixed)
     DevBuf<double> rec_fy;
+    DevBuf<double> phiw;       // W_a . phi per atom
     DaDev dev{};
     long long nentries = 0;
     long long launches = 0, trials_seen = 0;   // batch sizing of phb_da_run
@@ -1059,6 +1043,23 @@ DaState* da_of(phb_mc_s* M) {
 void rebuild_hist(phb_mc_s* M, DaState* S) {
     const int N = M->atoms->n;
     hipLaunchKernelGGL(k_da_hist, dim3(ceil_div(N, 4)), dim3(256), 0, stream(), N, S->dev, M->atoms->type.p);
+    PHB_HIP(hipGetLastError());
+}
+
+// W_a . phi of every atom from the current potential grid.
+void phi_dot(phb_mc_s* M, DaState* S) {
+    const int N = M->atoms->n;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(ceil_div(N, 128)), dim3(128), 0, stream(), M->dev, S->phiw.p);
+    };
+    switch (M->dev.P) {
+        case 3: go(k_phi_dot<3>); break;
+        case 4: go(k_phi_dot<4>); break;
+        case 5: go(k_phi_dot<5>); break;
+        case 6: go(k_phi_dot<6>); break;
+        case 7: go(k_phi_dot<7>); break;
+        default: go(k_phi_dot<8>); break;
+    }
     PHB_HIP(hipGetLastError());
 }
 
@@ -1420,6 +1421,8 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         A.rc2 = M->params->d.rc * M->params->d.rc;
         S->rec_fy.alloc(std::max<long long>(M->d.record_cap, 1));
         A.rec_fy = S->rec_fy.p;
+        S->phiw.alloc(std::max(N, 1));
+        A.phiw = S->phiw.p;
         if (S->nentries > 0)
             hipLaunchKernelGGL(k_da_row_entries, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
                                M->atoms->y.p, M->atoms->z.p, A, nbr.p, S->dlist.p);
@@ -1505,6 +1508,7 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
         rebuild_hist(M, S);
         upload_views(M, S, s);
         refresh(M);   // the surrogate needs phi to hold every accepted change
+        phi_dot(M, S);
         McCtl c = read_ctl(M);
         const long long target = c.trials_done + ntrials;
         long long done = c.trials_done;
@@ -1524,6 +1528,7 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
                     ++S->launches;
                     if (M->profiling) M->event();
                     refresh(M);
+                    phi_dot(M, S);
                     if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
                 }
                 PHB_HIP(hipGetLastError());
@@ -1538,6 +1543,7 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
             }
             if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
                 full_recompute(M);
+                phi_dot(M, S);
                 M->next_recompute += M->d.recompute_interval;
             }
         }
