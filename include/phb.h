@@ -158,7 +158,17 @@ typedef struct {
     int recompute_interval;    /* full energy recompute (drift reset) every N trials; 0 = never */
     long long record_cap;      /* per-trial records kept (0 = none) */
     int spec_window;           /* trials evaluated concurrently per round: 1, 2, 4, 8, 16 (0 = 16) */
+    int refresh;               /* potential-grid refresh: PHB_REFRESH_AUTO, _DIRECT or _KSPACE */
 } phb_mc_desc;
+/* Refresh of the potential grid by the pending accepted charge changes.
+ * DIRECT: their stencils convolved with the real-space mesh kernel in one pass
+ * over the grid (x extent up to 64 mesh points); KSPACE: structure factor of the
+ * changes times the influence function, inverse FFT onto the grid; AUTO: DIRECT
+ * where the grid allows it and pending_max <= 16, else KSPACE.  The two agree
+ * up to rounding (the same circular convolution). */
+#define PHB_REFRESH_AUTO 0
+#define PHB_REFRESH_DIRECT 1
+#define PHB_REFRESH_KSPACE 2
 typedef struct {
     long long attempted, accepted, accepted_forward, accepted_backward;
     int nprot, nsites, status;         /* status 1: no valid pair to move (MC/src/mc.cu:398-402) */

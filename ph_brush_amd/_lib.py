@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libphb.so"
 PHB_OK = 0
 PHB_FP64, PHB_FP32 = 0, 1
 PHB_COUL_CUT, PHB_COUL_PME = 0, 1
+PHB_REFRESH_AUTO, PHB_REFRESH_DIRECT, PHB_REFRESH_KSPACE = 0, 1, 2
 
 _ERRNAMES = {1: "ARG", 2: "HIP", 3: "ALLOC", 4: "STATE", 5: "UNSUPPORTED", 6: "INTERNAL"}
 
@@ -61,7 +62,7 @@ class McDesc(C.Structure):
         ("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int), ("cl_type", C.c_int),
         ("beta", C.c_double), ("mu_eff", C.c_double), ("w_z_extra", C.c_double),
         ("precision", C.c_int), ("pending_max", C.c_int), ("recompute_interval", C.c_int),
-        ("record_cap", C.c_longlong), ("spec_window", C.c_int),
+        ("record_cap", C.c_longlong), ("spec_window", C.c_int), ("refresh", C.c_int),
     ]
 
 

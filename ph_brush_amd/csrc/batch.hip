@@ -6,7 +6,10 @@
 // chain of replica r (the persistent single-workgroup chain of mc.hip, so each
 // replica's accept/reject sequence is exactly its serial one), and the two
 // refresh launches cover the potential-grid tiles and planes of every replica
-// through a block-offset table.  A sweep gives each replica as many trials as
+// through block-offset tables (the direct refresh's tiles, or the k-space
+// refresh's z pencils and planes, per replica by the rule of its serial
+// refresh, so each replica's potential grid is bit-identical to its serial
+// run's).  A sweep gives each replica as many trials as
 // it has titratable sites; the (chain, refresh) launch pairs of a sweep are
 // captured once in a hipGraph and replayed until every replica has finished
 // its sweep.  The autotuner times the speculation window (waves per trial),
@@ -56,6 +59,14 @@ __global__ __launch_bounds__(512) void k_kspace_batch(const RefreshView* __restr
     refresh_kspace_tile<P>(*w.D, w.kg, w.twx, w.twy, w.twz, w.eta, w.pz, w.tx, blockIdx.x - start[r]);
 }
 
+template <int P>
+__global__ __launch_bounds__(RD_THREADS) void k_direct_batch(const RefreshView* __restrict__ v,
+                                                             const int* __restrict__ start, int R) {
+    const int r = replica_of(start, R, blockIdx.x);
+    const RefreshView& w = v[r];
+    refresh_direct_tile<P>(*w.D, w.phi, nullptr, blockIdx.x - start[r]);
+}
+
 __global__ __launch_bounds__(512) void k_c2r_batch(const RefreshView* __restrict__ v, const int* __restrict__ start,
                                                    int R) {
     const int r = replica_of(start, R, blockIdx.x);
@@ -95,11 +106,12 @@ struct phb_batch_s {
     std::vector<phb_mc_s*> mcs;
     int R = 0, P = 5;
     bool lds_pools = false;
-    size_t pool_lds = 0, klds = 0, plds = 0;
-    int kblocks = 0, pblocks = 0;
+    size_t pool_lds = 0, klds = 0, plds = 0, dlds = 0;
+    int kblocks = 0, pblocks = 0, dblocks = 0, nk = 0, nd = 0;   // k-space / direct refresh replicas
     DevBuf<McDev> views;
-    DevBuf<RefreshView> rviews;
-    DevBuf<int> kstart, pstart;
+    std::vector<RefreshView> hviews;
+    DevBuf<RefreshView> rv_k, rv_d;
+    DevBuf<int> kstart, pstart, dstart;
     DevBuf<long long> add, status;
     std::vector<long long> h_status;
     phb_batch_desc d{};
@@ -122,10 +134,15 @@ void enqueue_pairs_p(phb_batch_s* B, int pairs) {
                                stream(), B->views.p);
         else
             hipLaunchKernelGGL((k_chain_batch<P, int, false>), dim3(B->R), dim3(MC_THREADS), 0, stream(), B->views.p);
-        hipLaunchKernelGGL(k_kspace_batch<P>, dim3(B->kblocks), dim3(512), B->klds, stream(), B->rviews.p,
-                           B->kstart.p, B->R);
-        hipLaunchKernelGGL(k_c2r_batch, dim3(B->pblocks), dim3(512), B->plds, stream(), B->rviews.p, B->pstart.p,
-                           B->R);
+        if (B->nd > 0)
+            hipLaunchKernelGGL(k_direct_batch<P>, dim3(B->dblocks), dim3(RD_THREADS), B->dlds, stream(), B->rv_d.p,
+                               B->dstart.p, B->nd);
+        if (B->nk > 0) {
+            hipLaunchKernelGGL(k_kspace_batch<P>, dim3(B->kblocks), dim3(512), B->klds, stream(), B->rv_k.p,
+                               B->kstart.p, B->nk);
+            hipLaunchKernelGGL(k_c2r_batch, dim3(B->pblocks), dim3(512), B->plds, stream(), B->rv_k.p, B->pstart.p,
+                               B->nk);
+        }
     }
 }
 
@@ -149,6 +166,8 @@ void set_lds_limits_p(phb_batch_s* B) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)B->klds));
     PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_c2r_batch), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)B->plds));
+    PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_direct_batch<P>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)std::max<size_t>(B->dlds, 1)));
 }
 
 void set_lds_limits(phb_batch_s* B) {
@@ -229,6 +248,41 @@ long long sweep(phb_batch_s* B) {
     return n;
 }
 
+// Splits the replicas between the direct and the k-space refresh launches by
+// the rule their serial refresh follows (which depends on pending_max).
+void build_refresh_tables(phb_batch_s* B) {
+    std::vector<RefreshView> vk, vd;
+    std::vector<int> ks{0}, ps{0}, ds{0};
+    for (int r = 0; r < B->R; ++r) {
+        phb_mc_s* M = B->mcs[r];
+        if (use_direct_refresh(M)) {
+            vd.push_back(B->hviews[r]);
+            ds.push_back(ds.back() + refresh_direct_blocks(M->pme->ny, M->pme->nz));
+        } else {
+            vk.push_back(B->hviews[r]);
+            ks.push_back(ks.back() + M->pme->z_blocks);
+            ps.push_back(ps.back() + M->pme->nz);
+        }
+    }
+    B->nd = (int)vd.size();
+    B->nk = (int)vk.size();
+    B->dblocks = ds.back();
+    B->kblocks = ks.back();
+    B->pblocks = ps.back();
+    sync();   // the tables may be in use by launches still queued
+    B->rv_d.alloc(std::max<size_t>(vd.size(), 1));
+    B->rv_k.alloc(std::max<size_t>(vk.size(), 1));
+    B->dstart.alloc(ds.size());
+    B->kstart.alloc(ks.size());
+    B->pstart.alloc(ps.size());
+    if (!vd.empty()) B->rv_d.upload(vd.data(), vd.size());
+    if (!vk.empty()) B->rv_k.upload(vk.data(), vk.size());
+    B->dstart.upload(ds.data(), ds.size());
+    B->kstart.upload(ks.data(), ks.size());
+    B->pstart.upload(ps.data(), ps.size());
+    sync();
+}
+
 void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs) {
     for (phb_mc_s* M : B->mcs) {
         M->dev.W = W;
@@ -236,6 +290,8 @@ void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs) {
         M->d.spec_window = W;
     }
     upload_views(B);
+    if (B->d.use_graph) drop_graph(B);   // the graph holds the old tables
+    build_refresh_tables(B);
     hipLaunchKernelGGL(k_batch_tuning, dim3(ceil_div(B->R, 64)), dim3(64), 0, stream(), B->views.p, B->R, pending_max);
     PHB_HIP(hipGetLastError());
     B->d.graph_pairs = pairs;
@@ -259,7 +315,6 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
         B->P = B->mcs[0]->dev.P;
         bool lds = true;
         std::vector<RefreshView> rv(n);
-        std::vector<int> ks(n + 1, 0), ps(n + 1, 0);
         for (int r = 0; r < n; ++r) {
             phb_mc_s* M = B->mcs[r];
             phb_pme_s* Pm = M->pme;
@@ -272,14 +327,11 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
             B->pool_lds = std::max(B->pool_lds, M->pool_lds_bytes);
             B->klds = std::max(B->klds, Pm->z_lds(sizeof(cplx<double>)) + (size_t)REFRESH_BATCH * Pm->nz * sizeof(cplx<double>));
             B->plds = std::max(B->plds, Pm->plane_lds(sizeof(cplx<double>)));
-            ks[r + 1] = ks[r] + Pm->z_blocks;
-            ps[r + 1] = ps[r] + Pm->nz;
+            if (refresh_direct_allowed(M)) B->dlds = std::max(B->dlds, refresh_direct_lds(B->P));
         }
         B->lds_pools = lds && B->pool_lds <= 96 * 1024;
         require(B->klds <= 160 * 1024 && B->plds <= 160 * 1024, "phb_batch_create: refresh tiles exceed LDS",
                 PHB_ERR_UNSUPPORTED);
-        B->kblocks = ks[n];
-        B->pblocks = ps[n];
         B->views.alloc(n);
         upload_views(B.get());
         for (int r = 0; r < n; ++r) {
@@ -292,18 +344,16 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
             w.tx = Pm->z_tx; w.nx = Pm->nx; w.ny = Pm->ny;
             w.phi = M->phi.p;
             w.gate.pending_n = &M->ctl.p->pending_n;
-            w.gate.folded = &M->ctl.p->pending_folded;
-            w.gate.counter = &M->ctl.p->refresh_count;
+            w.gate.seq = &M->ctl.p->pend_seq;
+            w.gate.applied = M->applied.p;
             require(Pm->kg64.p != nullptr, "phb_batch_create: replicas need FP64 plans", PHB_ERR_UNSUPPORTED);
         }
         // replicas sharing one PME plan would share its k-space buffer
         for (int r = 0; r < n; ++r)
             for (int q = 0; q < r; ++q)
                 require(B->mcs[q]->pme != B->mcs[r]->pme, "phb_batch_create: replicas must not share a PME plan");
-        B->rviews.alloc(n);
-        B->rviews.upload(rv.data(), n);
-        B->kstart.alloc(n + 1); B->kstart.upload(ks.data(), n + 1);
-        B->pstart.alloc(n + 1); B->pstart.upload(ps.data(), n + 1);
+        B->hviews = rv;
+        build_refresh_tables(B.get());
         B->add.alloc(n);
         B->status.alloc(3 * (size_t)n);
         B->h_status.assign(3 * (size_t)n, 0);

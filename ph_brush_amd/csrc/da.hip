@@ -532,11 +532,41 @@ __device__ double stencil_dot(const McDev& D, int a) {
     return s;
 }
 
-// W_a . phi of every atom (one thread per atom), after each refresh of phi.
+// W_a . phi of the atoms a move can pick (the site and ion pools, a fixed set),
+// after each refresh of phi: 8 lanes per atom, lane u < P sums the x column u
+// of the stencil; the list is in stencil-origin order, so neighbouring groups
+// read neighbouring grid rows.
 template <int P>
-__global__ void k_phi_dot(McDev D, double* __restrict__ out) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < D.N) out[a] = stencil_dot<P>(D, a);
+__global__ __launch_bounds__(256) void k_phi_dot(McDev D, const int* __restrict__ list, int n,
+                                                 double* __restrict__ out) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, e = gt >> 3, u = gt & 7;
+    if (e >= n) return;
+    const int a = list[e];
+    const int* sb = D.sbase + 3 * a;
+    const double* w = D.sw + (size_t)a * 3 * P;
+    double t = 0.0;
+    if (u < P) {
+        const int gx = wrap1(sb[0] + u, D.nx);
+        double v[P * P];
+#pragma unroll
+        for (int wz = 0; wz < P; ++wz) {
+            const size_t zrow = (size_t)wrap1(sb[2] + wz, D.nz) * D.ny;
+#pragma unroll
+            for (int wy = 0; wy < P; ++wy) v[wz * P + wy] = D.phi[(zrow + wrap1(sb[1] + wy, D.ny)) * D.nx + gx];
+        }
+#pragma unroll
+        for (int wz = 0; wz < P; ++wz) {
+            double sy = 0.0;
+#pragma unroll
+            for (int wy = 0; wy < P; ++wy) sy += w[P + wy] * v[wz * P + wy];
+            t += w[2 * P + wz] * sy;
+        }
+        t *= w[u];
+    }
+    t += __shfl_xor(t, 4, 8);
+    t += __shfl_xor(t, 2, 8);
+    t += __shfl_xor(t, 1, 8);
+    if (u == 0) out[a] = t;
 }
 
 // Physics part B = lin + selfq + pr + corr of n moves (one thread per move).
@@ -633,7 +663,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     const long long target = C->trials_target, rec_cap = C->rec_cap;
     // a launch queued beyond the target (batches are sized ahead) leaves at once
     if (C->trials_done >= target || C->status != 0) return;
-    const int pn0 = C->pending_folded ? 0 : C->pending_n;
+    const int pn0 = pending_unfolded(D);
     if (pn0 > 0) {   // phi lags the state: the surrogate's potential term would not be a function of the state
         if (tid == 0) C->status = 3;
         return;
@@ -976,7 +1006,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 C->pend_atom[0] = t.i; C->pend_dq[0] = t.qi_new - t.qi_old;
                 C->pend_atom[1] = t.j; C->pend_dq[1] = t.qj_new - t.qj_old;
                 C->pending_n = 2;
-                C->pending_folded = 0;
+                C->pend_seq = C->pend_seq + 1u;   // a new list: no workgroup has folded it
             }
         }
     }
@@ -1027,7 +1057,9 @@ struct DaState {
     DevBuf<unsigned short> H;
     DevBuf<double> x, y;       // copies of the positions (fixed)
     DevBuf<double> rec_fy;
-    DevBuf<double> phiw;       // W_a . phi per atom
+    DevBuf<double> phiw;       // W_a . phi per atom (pool atoms)
+    DevBuf<int> cand;          // the pool atoms, in stencil-origin order
+    int ncand = 0;
     DaDev dev{};
     long long nentries = 0;
     long long launches = 0, trials_seen = 0;   // batch sizing of phb_da_run
@@ -1046,11 +1078,12 @@ void rebuild_hist(phb_mc_s* M, DaState* S) {
     PHB_HIP(hipGetLastError());
 }
 
-// W_a . phi of every atom from the current potential grid.
+// W_a . phi of the pool atoms from the current potential grid.
 void phi_dot(phb_mc_s* M, DaState* S) {
-    const int N = M->atoms->n;
+    if (S->ncand == 0) return;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ceil_div(N, 128)), dim3(128), 0, stream(), M->dev, S->phiw.p);
+        hipLaunchKernelGGL(kern, dim3(ceil_div(8 * S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->cand.p,
+                           S->ncand, S->phiw.p);
     };
     switch (M->dev.P) {
         case 3: go(k_phi_dot<3>); break;
@@ -1423,6 +1456,23 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         A.rec_fy = S->rec_fy.p;
         S->phiw.alloc(std::max(N, 1));
         A.phiw = S->phiw.p;
+        {   // pool atoms (sites and ions: a fixed set) sorted by stencil origin
+            const int np = M->nsites + M->nions;
+            std::vector<int> pool(np), sb(3 * (size_t)N);
+            M->pbuf.download(pool.data(), np);
+            M->sbase.download(sb.data(), 3 * (size_t)N);
+            sync();
+            std::sort(pool.begin(), pool.end(), [&](int x, int y) {
+                const int* a = &sb[3 * (size_t)x];
+                const int* b = &sb[3 * (size_t)y];
+                if (a[2] != b[2]) return a[2] < b[2];
+                if (a[1] != b[1]) return a[1] < b[1];
+                return a[0] != b[0] ? a[0] < b[0] : x < y;
+            });
+            S->ncand = np;
+            S->cand.alloc(std::max(np, 1));
+            S->cand.upload(pool.data(), np);
+        }
         if (S->nentries > 0)
             hipLaunchKernelGGL(k_da_row_entries, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
                                M->atoms->y.p, M->atoms->z.p, A, nbr.p, S->dlist.p);

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_mc_probe(McDev D, int n, const i
     const McCtl* C = D.ctl;
     const int m = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (m >= n) return;
-    const int pend_n = C->pending_folded ? 0 : C->pending_n;
+    const int pend_n = pending_unfolded(D);
     for (int k = wave; k < pend_n; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
     if (tid == 0) stage_trial<P>(D, s_t, mi[m], mj[m], mf[m]);
     __syncthreads();
@@ -86,6 +86,13 @@ __global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* _
     refresh_kspace_tile<P>(D, kg, twx, twy, twz, eta, pz, tx, blockIdx.x);
 }
 
+// Direct refresh (refresh_direct_tile): one workgroup per (y, z) tile of full x rows.
+template <int P>
+__global__ __launch_bounds__(RD_THREADS) void k_refresh_direct(McDev D, double* __restrict__ phi,
+                                                               float* __restrict__ phi32) {
+    refresh_direct_tile<P>(D, phi, phi32, blockIdx.x);
+}
+
 __global__ void k_reset_pending(McCtl* C) {
     if (threadIdx.x == 0) C->pending_n = 0;
 }
@@ -108,7 +115,6 @@ __global__ void k_finish_recompute(McDev D, const double* __restrict__ res, doub
     C->E = E; C->E_recomputed = E;
     C->Q = Q; C->Q2 = Q2; C->Mz = Mz; C->Qz2 = Qz2;
     C->pending_n = 0;
-    C->pending_folded = 0;
 }
 
 // ------------------------------------------------------------ setup kernels
@@ -298,9 +304,33 @@ void full_recompute(phb_mc_s* M) {
     ++M->recomputes;
 }
 
+bool refresh_direct_allowed(const phb_mc_s* M) {
+    return M->pme->nx <= RD_NXMAX && refresh_direct_lds(M->pme->order) <= 64 * 1024;
+}
+
+bool use_direct_refresh(const phb_mc_s* M) {
+    if (M->d.refresh == PHB_REFRESH_KSPACE) return false;
+    if (M->d.refresh == PHB_REFRESH_DIRECT) return true;
+    return refresh_direct_allowed(M) && M->d.pending_max <= 16;
+}
+
 template <int P>
 void refresh_p(phb_mc_s* M) {
     phb_pme_s* Pm = M->pme;
+    if (use_direct_refresh(M)) {
+        static bool lds_set = false;   // per spline order (one instantiation each)
+        if (!lds_set) {
+            PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_refresh_direct<P>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)refresh_direct_lds(P)));
+            lds_set = true;
+        }
+        const int nb = refresh_direct_blocks(Pm->ny, Pm->nz);
+        hipLaunchKernelGGL(k_refresh_direct<P>, dim3(nb), dim3(RD_THREADS), refresh_direct_lds(P), stream(),
+                           M->dev, M->phi.p, M->d.precision == PHB_FP32 ? M->phi32.p : nullptr);
+        PHB_HIP(hipGetLastError());
+        ++M->refreshes;
+        return;
+    }
     require(Pm->z_tx <= 16, "refresh: z tile wider than 16", PHB_ERR_INTERNAL);
     const size_t lds = Pm->z_lds(sizeof(cplx<double>)) + (size_t)REFRESH_BATCH * Pm->nz * sizeof(cplx<double>);
     require(lds <= 160 * 1024, "refresh: z pencil and structure factors exceed LDS", PHB_ERR_UNSUPPORTED);
@@ -308,8 +338,8 @@ void refresh_p(phb_mc_s* M) {
                        Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->eta64.p, Pm->pz, Pm->z_tx);
     RefreshGate g;
     g.pending_n = &M->ctl.p->pending_n;
-    g.folded = &M->ctl.p->pending_folded;
-    g.counter = &M->ctl.p->refresh_count;
+    g.seq = &M->ctl.p->pend_seq;
+    g.applied = M->applied.p;
     PmeKernels::launch_c2r_buf(Pm, M->phi.p, true, g);
     PHB_HIP(hipGetLastError());
     sync_phi32(M);
@@ -416,6 +446,8 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         const int pm = d->pending_max > 0 ? d->pending_max : 32;
         require(pm >= 2 && pm <= MC_KMAX, "phb_mc_create: pending_max must be in [2, 64]");
         require(d->record_cap >= 0, "phb_mc_create: record_cap must be >= 0");
+        require(d->refresh == PHB_REFRESH_AUTO || d->refresh == PHB_REFRESH_DIRECT || d->refresh == PHB_REFRESH_KSPACE,
+                "phb_mc_create: refresh must be PHB_REFRESH_AUTO, _DIRECT or _KSPACE");
         const int sw_ = d->spec_window > 0 ? d->spec_window : 16;
         require(sw_ == 1 || sw_ == 2 || sw_ == 4 || sw_ == 8 || sw_ == 16,
                 "phb_mc_create: spec_window must be 1, 2, 4, 8 or 16");
@@ -423,6 +455,8 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
                 "phb_mc_create: every grid dimension must be >= 2*order-1", PHB_ERR_UNSUPPORTED);
         auto M = std::make_unique<phb_mc_s>();
         M->atoms = a; M->params = p; M->neigh = neigh; M->pme = pme; M->box = *box; M->d = *d;
+        require(d->refresh != PHB_REFRESH_DIRECT || refresh_direct_allowed(M.get()),
+                "phb_mc_create: the direct refresh needs nx <= 64", PHB_ERR_UNSUPPORTED);
         M->d.pending_max = pm;
         const int N = a->n;
         // ---- pools (build_plan, MC/src/mc.cu:67-100 and :332-341)
@@ -520,6 +554,11 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         }
         M->ctl.alloc(1);
         M->ctl.upload(&c, 1);
+        {   // one entry per workgroup of either refresh (direct tiles, or k-space planes)
+            const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz), pme->nz);
+            M->applied.alloc(na);
+            PHB_HIP(hipMemsetAsync(M->applied.p, 0, na * sizeof(unsigned), stream()));
+        }
         // ---- device view
         McDev& D = M->dev;
         D.N = N; D.z = a->z.p; D.q = a->q.p; D.type = a->type.p;
@@ -537,6 +576,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         D.Lz = box->Lz; D.use3dc = pme->use3dc ? 1 : 0;
         D.self_coef = pme->elF * pme->alpha / std::sqrt(M_PI);
         D.ctl = M->ctl.p;
+        D.applied = M->applied.p;
         D.rec_fwd = M->rec_fwd.p; D.rec_i = M->rec_i.p; D.rec_j = M->rec_j.p; D.rec_acc = M->rec_acc.p;
         D.rec_dE = M->rec_dE.p; D.rec_u = M->rec_u.p;
         // ---- initial full recompute (phi, E, moments)

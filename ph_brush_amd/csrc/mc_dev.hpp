@@ -23,8 +23,8 @@ struct McCtl {
     long long accepted, acc_fwd, acc_bwd;
     long long rec_cap;
     int pending_n, pending_max, status, nprot;
-    int pending_folded;           // set by the refresh: phi already holds the pending changes
-    int refresh_count;            // workgroups of the refresh's plane pass that finished (reset by the last)
+    unsigned pend_seq;            // bumped by a chain launch that leaves a new pending list
+    int pad_;
     int sizes[4];                 // nh_n, nh_p, w, cl
     double E, Q, Q2, Mz, Qz2;
     double E_recomputed, max_drift;
@@ -62,11 +62,21 @@ struct McDev {
     double beta, mu, elF, alpha, vol, Lz, self_coef;
     int use3dc;
     McCtl* ctl;
+    // per refresh workgroup: the pending list (pend_seq) it has folded into phi;
+    // entry 0 equal to pend_seq means the whole grid holds the pending changes
+    unsigned* applied;
     signed char* rec_fwd;
     int *rec_i, *rec_j;
     signed char* rec_acc;
     double *rec_dE, *rec_u;
 };
+
+// Pending changes not yet in phi (a refresh kernel runs all its workgroups, so
+// after one has run, entry 0 stands for every workgroup).
+__device__ __forceinline__ int pending_unfolded(const McDev& D) {
+    const McCtl* C = D.ctl;
+    return D.applied[0] == C->pend_seq ? 0 : C->pending_n;
+}
 
 // ------------------------------------------------------------ MT19937 stream
 constexpr unsigned MT_UPPER = 0x80000000u, MT_LOWER = 0x7fffffffu, MT_A = 0x9908b0dfu;

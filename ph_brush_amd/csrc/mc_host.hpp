@@ -27,6 +27,7 @@ struct phb_mc_s {
     bool lds_pools = false;
     size_t pool_lds_bytes = 0;
     DevBuf<McCtl> ctl;
+    DevBuf<unsigned> applied;   // per refresh workgroup: the pending list it folded
     DevBuf<signed char> rec_fwd, rec_acc;
     DevBuf<int> rec_i, rec_j;
     DevBuf<double> rec_dE, rec_u;
@@ -84,6 +85,8 @@ McCtl read_ctl(phb_mc_s* M);
 void full_recompute(phb_mc_s* M);
 // Folds the pending accepted charge changes into phi (k-space refresh).
 void refresh(phb_mc_s* M);
+bool refresh_direct_allowed(const phb_mc_s* M);   // grid fits the direct refresh
+bool use_direct_refresh(const phb_mc_s* M);
 void fill_stats(phb_mc_s* M, const McCtl& c, phb_mc_stats* s);
 void launch_probe(phb_mc_s* M, int n, const int* i, const int* j, const int* f, double* out);
 }  // namespace phb

@@ -30,7 +30,7 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
         s_rng.cur[k] = C->mt[k];
         s_rng.out[0][k] = mt_temper(C->mt[k]);
     }
-    const int pn0 = C->pending_folded ? 0 : C->pending_n;
+    const int pn0 = pending_unfolded(D);
     for (int k = wave; k < pn0; k += 16) stage_pending<P>(D, s_pd, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
     if (tid == 0) {
         s_st.done = C->trials_done;
@@ -207,7 +207,8 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
             for (int k = 0; k < 4; ++k) C->sizes[k] = s_st.sz[k];
             C->trials_done = s_st.done;
             C->accepted = s_st.acc_n; C->acc_fwd = s_st.acc_f; C->acc_bwd = s_st.acc_b;
-            C->pending_n = pend_n; C->pending_folded = 0; C->status = s_st.status; C->nprot = s_st.nprot;
+            if (pend_n > pn0) C->pend_seq = C->pend_seq + 1u;   // a new list: no workgroup has folded it
+            C->pending_n = pend_n; C->status = s_st.status; C->nprot = s_st.nprot;
             C->E = s_st.E; C->Q = s_st.Q; C->Q2 = s_st.Q2; C->Mz = s_st.Mz; C->Qz2 = s_st.Qz2;
         }
     }
@@ -223,8 +224,8 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
                                                     const cplx<double>* __restrict__ twy,
                                                     const cplx<double>* __restrict__ twz, const double* __restrict__ eta,
                                                     const FftPlan& pz, int tx, int blk) {
-    McCtl* C = D.ctl;
-    const int np = C->pending_folded ? 0 : C->pending_n;
+    const McCtl* C = D.ctl;
+    const int np = pending_unfolded(D);   // the plane pass, which records the list, runs after this one
     if (np == 0) return;
     constexpr int NB = REFRESH_BATCH;   // pending atoms per batch
     __shared__ int s_sb[NB][3];
@@ -314,5 +315,194 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
     }
 }
 
+// ------------------------------------------------------------ direct refresh
+// phi(r) += sum_p dq_p sum_{u,v,w} wx_p[u] wy_p[v] wz_p[w] G(r - s_p - (u, v, w)):
+// the pending charge changes convolved with the real-space mesh kernel G (the
+// potential of a unit mesh charge, read from its copy folded to |offset| per
+// axis, which stays in L2).  Equal to the k-space refresh up to rounding (both
+// are the same circular convolution), in one pass over the potential grid.
+//
+// Workgroup blk owns the full x rows of RD_TY y by RD_TZ z mesh points; lane =
+// x (nx <= 64), so every row index is uniform across a wave.  RD_GROUPS groups
+// of four waves take the pending atoms p = g, g + RD_GROUPS, ...: per atom a z
+// pass over the halo rows read into registers, a y pass in place in LDS (rows
+// of pitch 64) and an x pass that applies the x shift, accumulated in
+// registers.  The group sums are added in group order, the grid read (at the
+// start) and written once.  Each workgroup records the pending list's sequence
+// number in its own applied[] entry: no state is shared between workgroups (a
+// device-wide fence or a shared counter would cost more than the convolution).
+constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 2, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
+
+__host__ __device__ constexpr size_t refresh_direct_lds(int P) {
+    return (size_t)RD_GROUPS * RD_TZ * (RD_TY + P - 1) * RD_PITCH * sizeof(double);
+}
+
+__host__ __device__ constexpr int refresh_direct_blocks(int ny, int nz) {
+    return ((ny + RD_TY - 1) / RD_TY) * ((nz + RD_TZ - 1) / RD_TZ);
+}
+
+__device__ __forceinline__ int wrap2(int x, int n) {  // x in [-2n, 3n)
+    x += x < 0 ? n : 0;
+    x += x < 0 ? n : 0;
+    x -= x >= n ? n : 0;
+    return x >= n ? x - n : x;
+}
+
+template <int P>
+__device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __restrict__ phi, float* __restrict__ phi32,
+                                                    int blk) {
+    const McCtl* C = D.ctl;
+    const unsigned seq = C->pend_seq;
+    const int np = D.applied[blk] == seq ? 0 : C->pending_n;
+    if (np == 0) return;
+    constexpr int HY = RD_TY + P - 1, HZ = RD_TZ + P - 1;
+    constexpr int RPW = (HY + 3) / 4, KPW = (RD_TZ + 3) / 4, OPW = (RD_TY * RD_TZ + 3) / 4;
+    constexpr int ABUF = RD_TZ * HY * RD_PITCH;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* A_all = reinterpret_cast<double*>(smem_raw);   // RD_GROUPS x [RD_TZ][HY][RD_PITCH]
+    __shared__ int s_sb[MC_KMAX][3];
+    __shared__ double s_w[MC_KMAX][3 * P];
+    __shared__ double s_dq[MC_KMAX];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wg = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int grp = wg >> 2, wv = wg & 3;
+    double* A = A_all + grp * ABUF;
+    const int nx = D.nx, ny = D.ny, nz = D.nz, fnx = D.fnx, fny = D.fny;
+    const int nty = (ny + RD_TY - 1) / RD_TY;
+    const int tz = blk / nty, y0 = (blk - tz * nty) * RD_TY, z0 = tz * RD_TZ;
+    const bool xin = lane < nx;
+    const int fx = xin ? fold(lane, nx) : 0;
+    for (int e = tid; e < np * (3 * P + 4); e += RD_THREADS) {
+        const int p = e / (3 * P + 4), q = e - p * (3 * P + 4), atom = C->pend_atom[p];
+        if (q < 3 * P) s_w[p][q] = D.sw[(size_t)atom * 3 * P + q];
+        else if (q < 3 * P + 3) s_sb[p][q - 3 * P] = D.sbase[3 * atom + (q - 3 * P)];
+        else s_dq[p] = C->pend_dq[p];
+    }
+    double old[OPW];   // group 0 holds the tile's grid values from the start
+#pragma unroll
+    for (int i = 0; i < OPW; ++i) {
+        const int r = wv * OPW + i, k = r / RD_TY, j = r - k * RD_TY;
+        const bool ok = grp == 0 && xin && r < RD_TY * RD_TZ && y0 + j < ny && z0 + k < nz;
+        old[i] = ok ? phi[((size_t)(z0 + k) * ny + y0 + j) * nx + lane] : 0.0;
+    }
+    double acc[OPW];
+#pragma unroll
+    for (int i = 0; i < OPW; ++i) acc[i] = 0.0;
+    __syncthreads();
+    const int rounds = (np + RD_GROUPS - 1) / RD_GROUPS;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int p = rd * RD_GROUPS + grp;
+        const bool act = p < np;
+        const int pp = act ? p : 0;
+        const int sx = __builtin_amdgcn_readfirstlane(s_sb[pp][0]);
+        const int sy = __builtin_amdgcn_readfirstlane(s_sb[pp][1]);
+        const int sz = __builtin_amdgcn_readfirstlane(s_sb[pp][2]);
+        const double* w = s_w[pp];
+        if (act) {   // z pass: halo row yy holds G at y index y0 - sy - (P-1) + yy, x index = lane
+            const int zb = wrap2(z0 - sz - (P - 1), nz);
+            int zoff[HZ];
+#pragma unroll
+            for (int zz = 0; zz < HZ; ++zz) {
+                int gz = zb + zz;
+                gz -= gz >= nz ? nz : 0;
+                gz -= gz >= nz ? nz : 0;
+                zoff[zz] = fold(gz, nz) * fny * fnx;
+            }
+            double g[RPW][HZ];
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int yy = wv + 4 * r;
+                const double* row = D.gf + (size_t)fold(wrap2(y0 - sy - (P - 1) + yy, ny), ny) * fnx + fx;
+#pragma unroll
+                for (int zz = 0; zz < HZ; ++zz) g[r][zz] = yy < HY ? row[zoff[zz]] : 0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int yy = wv + 4 * r;
+                if (yy < HY) {
+#pragma unroll
+                    for (int k = 0; k < RD_TZ; ++k) {
+                        double a = 0.0;
+#pragma unroll
+                        for (int q = 0; q < P; ++q) a += w[2 * P + q] * g[r][k + P - 1 - q];
+                        A[(k * HY + yy) * RD_PITCH + lane] = a;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {   // y pass, in place: column (k, lane)
+#pragma unroll
+            for (int c = 0; c < KPW; ++c) {
+                const int k = wv + 4 * c;
+                if (k < RD_TZ) {
+                    double v[HY];
+#pragma unroll
+                    for (int yy = 0; yy < HY; ++yy) v[yy] = A[(k * HY + yy) * RD_PITCH + lane];
+#pragma unroll
+                    for (int j = 0; j < RD_TY; ++j) {
+                        double b = 0.0;
+#pragma unroll
+                        for (int u = 0; u < P; ++u) b += w[P + u] * v[j + P - 1 - u];
+                        A[(k * HY + j) * RD_PITCH + lane] = b;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {   // x pass with the x shift
+            const double dq = s_dq[pp];
+            const int xb = xin ? wrap2(lane - sx, nx) : 0;
+            int xo[P];
+            double wx[P];
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                const int xi = xb - u;
+                xo[u] = xi + (xi < 0 ? nx : 0);
+                wx[u] = w[u];
+            }
+#pragma unroll
+            for (int i = 0; i < OPW; ++i) {
+                const int r = wv * OPW + i;
+                if (r < RD_TY * RD_TZ) {
+                    const int k = r / RD_TY, j = r - k * RD_TY;
+                    const double* row = A + (k * HY + j) * RD_PITCH;
+                    double s = 0.0;
+#pragma unroll
+                    for (int u = 0; u < P; ++u) s += wx[u] * row[xo[u]];
+                    acc[i] += dq * s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // groups 1.. hand their sums to group 0 through LDS, added in group order
+    for (int g2 = 1; g2 < RD_GROUPS; ++g2) {
+        if (grp == g2) {
+#pragma unroll
+            for (int i = 0; i < OPW; ++i) A_all[(wv * OPW + i) * RD_PITCH + lane] = acc[i];
+        }
+        __syncthreads();
+        if (grp == 0) {
+#pragma unroll
+            for (int i = 0; i < OPW; ++i) acc[i] += A_all[(wv * OPW + i) * RD_PITCH + lane];
+        }
+        __syncthreads();
+    }
+    if (grp == 0) {
+#pragma unroll
+        for (int i = 0; i < OPW; ++i) {
+            const int r = wv * OPW + i, k = r / RD_TY, j = r - k * RD_TY;
+            if (xin && r < RD_TY * RD_TZ && y0 + j < ny && z0 + k < nz) {
+                const size_t ix = ((size_t)(z0 + k) * ny + y0 + j) * nx + lane;
+                const double v = old[i] + acc[i];
+                phi[ix] = v;
+                if (phi32) phi32[ix] = (float)v;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) D.applied[blk] = seq;
+}
 
 }  // namespace phb

@@ -59,13 +59,14 @@ struct phb_pme_s {
 
 namespace phb {
 
-// Gate of the potential refresh's inverse plane pass: it runs only while
-// accepted charge changes are pending and not yet folded, and its last
-// workgroup to finish marks them folded (null fields: no gate).
+// Gate of the potential refresh's inverse plane pass: plane z runs only while
+// accepted charge changes are pending and applied[z] differs from their list's
+// sequence number, and records the number when done; workgroups share no
+// state, so no fence or atomic is needed (null fields: no gate).
 struct RefreshGate {
     const int* pending_n = nullptr;
-    int* folded = nullptr;
-    int* counter = nullptr;
+    const unsigned* seq = nullptr;
+    unsigned* applied = nullptr;
 };
 
 // Inverse plane pass of z-plane zp (one workgroup): C2R along y then x from the
@@ -76,7 +77,7 @@ __device__ __forceinline__ void fft_xy_c2r_plane(const cplx<T>* __restrict__ kg,
                                                  const FftPlan& px, const FftPlan& py, const cplx<T>* __restrict__ twx,
                                                  const cplx<T>* __restrict__ twy, int accumulate, const RefreshGate& gate,
                                                  int zp, int nplanes) {
-    if (gate.pending_n && (*gate.folded != 0 || *gate.pending_n == 0)) return;
+    if (gate.pending_n && (gate.applied[zp] == *gate.seq || *gate.pending_n == 0)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* stwy = stwx + nx;
@@ -153,15 +154,9 @@ __device__ __forceinline__ void fft_xy_c2r_plane(const cplx<T>* __restrict__ kg,
             }
         }
     }
-    if (gate.counter) {   // the last workgroup marks the pending changes folded
+    if (gate.applied) {   // this plane now holds the pending list
         __syncthreads();
-        if (tid == 0) {
-            __threadfence();
-            if (atomicAdd(gate.counter, 1) == nplanes - 1) {
-                *gate.counter = 0;
-                *gate.folded = 1;
-            }
-        }
+        if (tid == 0) gate.applied[zp] = *gate.seq;
     }
 }
 

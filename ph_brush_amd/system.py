@@ -361,12 +361,13 @@ class MonteCarlo(_Handle):
     _destroy = "phb_mc_destroy"
 
     def __init__(self, atoms, params, neigh, pme, box, mc: McParams, seed: int, *, pending_max=8,
-                 recompute_interval=1000, record_cap=0, precision=L.PHB_FP64, spec_window=8):
+                 recompute_interval=1000, record_cap=0, precision=L.PHB_FP64, spec_window=8,
+                 refresh=L.PHB_REFRESH_AUTO):
         super().__init__()
         self._keep = (atoms, params, neigh, pme)
         self.record_cap = record_cap
         d = L.McDesc(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra,
-                     precision, pending_max, recompute_interval, record_cap, spec_window)
+                     precision, pending_max, recompute_interval, record_cap, spec_window, refresh)
         L.call("phb_mc_create", atoms.h, params.h, neigh.h, pme.h, C.byref(box), C.byref(d), seed & 0xFFFFFFFF,
                C.byref(self.h))
 
