@@ -70,9 +70,23 @@ __device__ __forceinline__ unsigned short f2bf(float f) {      // round to neare
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by the rational approximation of Abramowitz & Stegun 7.1.26 (Hastings):
+// |error| <= 1.5e-7 plus the hardware reciprocal's and exp2's last-bit errors,
+// far below the BF16 rounding of the activations; branch-free, unlike erff.
+__device__ __forceinline__ float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    float p = 1.061405429f;
+    p = p * t - 1.453152027f;
+    p = p * t + 1.421413741f;
+    p = p * t - 0.284496736f;
+    p = p * t + 0.254829592f;
+    const float r = 1.0f - p * t * __expf(-ax * ax);
+    return copysignf(r, x);
+}
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad_f(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+    return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
 }
 __device__ __forceinline__ s4 ld4(const unsigned short* p) { return *reinterpret_cast<const s4*>(p); }
 __device__ __forceinline__ f4 mfma16(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
@@ -128,6 +142,20 @@ __device__ __forceinline__ float xor_sum16(float v) {   // sum over the 16 lanes
     v += __shfl_xor(v, 8, 64);
     return v;
 }
+// Sum over the 16 lanes of a DPP row (lane >> 4) by four row-local DPP adds;
+// the association differs between the row's quads, so the sum may differ in
+// the last bit between lanes: for consumers that read one lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_sum16(float v) {
+    v += dpp_f<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f<0x124>(v);   // row_ror:4
+    v += dpp_f<0x128>(v);   // row_ror:8
+    return v;
+}
 __device__ __forceinline__ float xor_sum_groups(float v) {  // sum over the 4 lanes sharing lane & 15
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
@@ -178,7 +206,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(NetDev n, const short* __re
     const float b3 = n.theta[OFF_B3], om = n.theta[OFF_MEAN], os = n.theta[OFF_STD];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const float v = xor_sum16(yp[e]);
+        const float v = row_sum16(yp[e]);
         const int r = r0 + 4 * g + e;
         if (rr == 0 && r < nr) y[first + r] = om + os * (v + b3);
     }
@@ -811,6 +839,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 s_X[k * XP + c] = f2bf(v);
             }
         }
+        for (int k = tid; k < DA_M * 8; k += DA_THREADS) (&s_y[0][0])[k] = 0.f;
         __syncthreads();
         clk.lap(prof, s_prof, 2);
         // ---- 3. MLP on MFMA: nrb row blocks x 8 column tiles, tile t on wave t mod DA_WAVES
@@ -828,6 +857,9 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 for (int e = 0; e < 4; ++e) s_H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(acc[e] + b));
             }
             __syncthreads();
+            // a wave sums its tiles of one row block in registers and reduces once per
+            // row block into its own slot s_y[row][wave] (the other slots stay zero)
+            float yp[4] = {0.f, 0.f, 0.f, 0.f};
             for (int t = wave; t < nrb * 8; t += DA_WAVES) {
                 const int rb = t % nrb, ct = t / nrb, r0 = rb * 16;
                 const int row = ct * 16 + rr, swz = (2 * row) & 31;
@@ -838,9 +870,15 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     acc = mfma16(ld4(s_H1 + (r0 + rr) * HP + kb * 16 + 4 * g), ld4(wr + (((4 * kb + g) ^ swz) << 2)), acc);
                 const float b = s_b2[row], w = s_w3[row];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float v = xor_sum16(bf2f(f2bf(gelu_f(acc[e] + b))) * w);
-                    if (rr == 0) s_y[r0 + 4 * g + e][ct] = v;
+                for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(acc[e] + b))) * w;
+                const int tn = t + DA_WAVES;
+                if (tn >= nrb * 8 || tn % nrb != rb) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float v = row_sum16(yp[e]);
+                        if (rr == 0) s_y[r0 + 4 * g + e][wave] = v;
+                        yp[e] = 0.f;
+                    }
                 }
             }
         }
