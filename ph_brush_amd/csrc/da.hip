@@ -648,6 +648,7 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
 // ---------------------------------------------------------------- delayed-acceptance chain
 struct CandLds {
     int i[DA_M], j[DA_M], fwd[DA_M], pos[DA_M], exA[DA_M], exB[DA_M];
+    int ki[DA_M], kj[DA_M];       // pool positions the proposal drew for the site and the ion
     double u1[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
     double lin[DA_M][2];          // W_a . phi of the site and the ion
     MoveTerms mt[DA_M];
@@ -755,7 +756,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     const int per = (both ? 2 : 0) + 4;
                     rng_ensure(s_rng, blk, pos + ntr * per, lane);
                     bool ok = true;
-                    int fwd = 0, ii = 0, jj = 0, pp = 0;
+                    int fwd = 0, ii = 0, jj = 0, pp = 0, ki = 0, kj = 0;
                     double u = 0.0;
                     if (lane < ntr) {
                         int q = pos + lane * per;
@@ -764,14 +765,17 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                         unsigned a1, a2;
                         ok = index_at(s_rng, q, (unsigned)(fwd ? sz0 : sz1), a1);
                         ok = index_at(s_rng, q + 1, (unsigned)(fwd ? sz2 : sz3), a2) && ok;
-                        ii = (int)pbuf[pool_slot(D, fwd ? 0 : 1, (int)a1)];
-                        jj = (int)pbuf[pool_slot(D, fwd ? 2 : 3, (int)a2)];
+                        ki = (int)a1;
+                        kj = (int)a2;
+                        ii = (int)pbuf[pool_slot(D, fwd ? 0 : 1, ki)];
+                        jj = (int)pbuf[pool_slot(D, fwd ? 2 : 3, kj)];
                         pp = q + 2;
                         u = canonical_at(s_rng, pp);
                     }
                     if (__all(ok)) {
                         if (lane < ntr) {
                             s_c.i[lane] = ii; s_c.j[lane] = jj; s_c.fwd[lane] = fwd;
+                            s_c.ki[lane] = ki; s_c.kj[lane] = kj;
                             s_c.pos[lane] = pp; s_c.u1[lane] = u;
                         }
                         pos += ntr * per;
@@ -780,18 +784,15 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                             int f2;
                             if (both) f2 = draw_canonical(s_rng, blk, pos, lane) < (double)wf / sum ? 1 : 0;
                             else f2 = wf > 0 ? 1 : 0;
-                            int i2, j2;
-                            if (f2) {
-                                i2 = (int)pbuf[pool_slot(D, 0, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz0))];
-                                j2 = (int)pbuf[pool_slot(D, 2, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz2))];
-                            } else {
-                                i2 = (int)pbuf[pool_slot(D, 1, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz1))];
-                                j2 = (int)pbuf[pool_slot(D, 3, (int)draw_index(s_rng, blk, pos, lane, (unsigned)sz3))];
-                            }
+                            const int ki2 = (int)draw_index(s_rng, blk, pos, lane, (unsigned)(f2 ? sz0 : sz1));
+                            const int kj2 = (int)draw_index(s_rng, blk, pos, lane, (unsigned)(f2 ? sz2 : sz3));
+                            const int i2 = (int)pbuf[pool_slot(D, f2 ? 0 : 1, ki2)];
+                            const int j2 = (int)pbuf[pool_slot(D, f2 ? 2 : 3, kj2)];
                             const int p2 = pos;
                             const double u2 = draw_canonical(s_rng, blk, pos, lane);
                             if (lane == 0) {
                                 s_c.i[k] = i2; s_c.j[k] = j2; s_c.fwd[k] = f2; s_c.pos[k] = p2; s_c.u1[k] = u2;
+                                s_c.ki[k] = ki2; s_c.kj[k] = kj2;
                             }
                         }
                     }
@@ -995,6 +996,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                         s_dc.er_pool[1] = is; s_dc.er_val[1] = t.j; s_dc.er_n[1] = s_st.sz[is];
                         s_dc.pb_pool[0] = pdd; s_dc.pb_val[0] = t.i; s_dc.pb_n[0] = s_st.sz[pdd];
                         s_dc.pb_pool[1] = id; s_dc.pb_val[1] = t.j; s_dc.pb_n[1] = s_st.sz[id];
+                        s_dc.pos[0] = s_c.ki[first]; s_dc.pos[1] = s_c.kj[first];
                         if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
                         else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
                         s_dc.acc_k = 0;
@@ -1006,7 +1008,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
             if (s_dc.acc_k == 0) {
                 const TrialT<P>& t = s_tr[0];
-                pool_update<PoolT>(D, pbuf, s_dc);
+                pool_update_at<PoolT>(D, pbuf, s_dc);
                 for (int side = 0; side < 2; ++side) {
                     const int atom = side ? t.j : t.i;
                     const int so = A.species[side ? t.tj_old : t.ti_old], sn = A.species[side ? t.tj_new : t.ti_new];
@@ -1148,7 +1150,8 @@ template <int P>
 void launch_da_p(phb_mc_s* M, DaState* S) {
     (void)M;
     // Pools stay in HBM: a launch covers a single accepted move, so copying them
-    // into LDS and back each launch would cost more than the proposals' reads save.
+    // into LDS and back each launch costs more than an in-LDS shift saves (the
+    // shift is one merged pass at known positions).
     hipLaunchKernelGGL((k_da_chain<P, int, false>), dim3(1), dim3(DA_THREADS), 0, stream(), S->dmc.p, S->dda.p,
                        S->dnet.p);
 }

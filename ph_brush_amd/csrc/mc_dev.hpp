@@ -506,6 +506,43 @@ __device__ void pool_update(const McDev& D, PoolT* pbuf, DecisionLds& t) {
     __syncthreads();
 }
 
+// pool_update with the erased values' positions already known (t.pos, from the
+// indices their proposal drew: the pools hold each atom once and do not change
+// between that proposal and its acceptance).  The two erasures shift disjoint
+// pool regions, so both shifts run as one range, every load of a chunk issued
+// before its stores.
+template <typename PoolT>
+__device__ void pool_update_at(const McDev& D, PoolT* pbuf, const DecisionLds& t) {
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int p0 = t.er_pool[0], p1 = t.er_pool[1];
+    const int a0 = t.pos[0] + 1, a1 = t.pos[1] + 1;
+    const int L0 = t.er_n[0] - a0, L = L0 + (t.er_n[1] - a1);
+    constexpr int M = 16;
+    for (int base = 0; base < L; base += M * bd) {
+        PoolT v[M];
+        int dst[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int e = base + tid + m * bd;
+            dst[m] = -1;
+            v[m] = PoolT(0);
+            if (e < L) {
+                const bool first = e < L0;
+                const int pool = first ? p0 : p1, k = first ? a0 + e : a1 + (e - L0);
+                v[m] = pbuf[pool_slot(D, pool, k)];
+                dst[m] = pool_slot(D, pool, k - 1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (dst[m] >= 0) pbuf[dst[m]] = v[m];
+        __syncthreads();
+    }
+    if (tid < 2) pbuf[pool_slot(D, t.pb_pool[tid], t.pb_n[tid])] = (PoolT)t.pb_val[tid];
+    __syncthreads();
+}
+
 // One persistent workgroup runs the chain.  Each round, wave 0 draws up to W
 // proposals assuming every one is rejected (a rejected trial always draws its
 // Metropolis uniform), all waves evaluate their Delta-E concurrently (16/W
