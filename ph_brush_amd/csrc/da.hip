@@ -412,6 +412,18 @@ __global__ void k_mlp_adam(float* __restrict__ theta, float* __restrict__ m, flo
 }
 
 // ---------------------------------------------------------------- descriptors
+// What the surrogate's physics terms read about a pool atom, in one 64-byte
+// record (four 16-byte loads): position and self term (fixed), and the
+// potential term, charge, type and species, rewritten after every refresh
+// (a delayed-acceptance launch holds at most one accepted move and ends with
+// it, so the dynamic part is current for the whole launch).
+struct __align__(16) AtomRec {
+    double x, y, z, selfw;
+    double phiw, q;
+    int type, sp;
+    int pad_[2];
+};
+
 struct DaDev {
     const int* dhead;             // r_desc rows (both directions)
     const unsigned* dlist;        // atom | shell << 24
@@ -424,7 +436,7 @@ struct DaDev {
     int M;                        // candidates per round
     int prof;                     // accumulate per-phase times into McCtl::da_prof
     double* rec_fy;               // per-trial record of the reverse surrogate dE*(y->x)
-    const double* phiw;           // W_a . phi per atom, recomputed after every refresh of phi
+    AtomRec* rec;                 // per atom (pool atoms: dynamic part current)
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -539,6 +551,44 @@ __device__ __forceinline__ MoveTerms move_terms(const McDev& D, const DaDev& A, 
     return m;
 }
 
+// pair_terms and move_terms from the atoms' records (same inputs, same results).
+__device__ __forceinline__ void pair_terms_rec(const DaDev& A, const AtomRec& ri, const AtomRec& rj, double& r,
+                                               int& exA, int& exB) {
+    const double dx = min_image_exact(ri.x - rj.x, A.Lx), dy = min_image_exact(ri.y - rj.y, A.Ly),
+                 dz = min_image_exact(ri.z - rj.z, A.Lz);
+    const double r2 = r2_exact(dx, dy, dz);
+    r = sqrt(r2);
+    exA = -1; exB = -1;
+    if (r2 <= A.rd2) {
+        const int sh = shell_of(A, r2);
+        if (rj.sp >= 0) exA = sh * 4 + rj.sp;
+        if (ri.sp >= 0) exB = sh * 4 + ri.sp;
+    }
+}
+
+__device__ __forceinline__ MoveTerms move_terms_rec(const McDev& D, const DaDev& A, const AtomRec& ri,
+                                                    const AtomRec& rj, int fwd, double r, double Q, double Mz,
+                                                    double Qz2) {
+    MoveTerms m;
+    const double qi = ri.q, qj = rj.q;
+    const double qin = fwd ? 1.0 : 0.0, qjn = fwd ? -1.0 : 0.0;
+    m.dqi = qin - qi;
+    m.dqj = qjn - qj;
+    m.selfq = 0.5 * (m.dqi * m.dqi * ri.selfw + m.dqj * m.dqj * rj.selfw);
+    const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
+    m.perfc = r * r <= A.rc2 ? (double)((float)(qin * qjn - qi * qj) * kr * ec) : 0.0;
+    m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
+    m.corr = corr_move(D, qi, qin, qj, qjn, ri.z, rj.z, Q, Mz, Qz2);
+    return m;
+}
+
+__global__ void k_rec_static(int N, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const double* __restrict__ selfw, AtomRec* __restrict__ rec) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    rec[a].x = x[a]; rec[a].y = y[a]; rec[a].z = z[a]; rec[a].selfw = selfw[a];
+}
+
 // One (z, y) row r < P^2 of W_a . phi.
 template <int P>
 __device__ __forceinline__ double stencil_row(const McDev& D, int a, int r) {
@@ -561,12 +611,13 @@ __device__ double stencil_dot(const McDev& D, int a) {
 }
 
 // W_a . phi of the atoms a move can pick (the site and ion pools, a fixed set),
-// after each refresh of phi: 8 lanes per atom, lane u < P sums the x column u
-// of the stencil; the list is in stencil-origin order, so neighbouring groups
-// read neighbouring grid rows.
+// after each refresh of phi, into their records with their charge, type and
+// species: 8 lanes per atom, lane u < P sums the x column u of the stencil; the
+// list is in stencil-origin order, so neighbouring groups read neighbouring
+// grid rows.
 template <int P>
-__global__ __launch_bounds__(256) void k_phi_dot(McDev D, const int* __restrict__ list, int n,
-                                                 double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_phi_dot(McDev D, const signed char* __restrict__ species,
+                                                 const int* __restrict__ list, int n, AtomRec* __restrict__ rec) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x, e = gt >> 3, u = gt & 7;
     if (e >= n) return;
     const int a = list[e];
@@ -594,7 +645,13 @@ __global__ __launch_bounds__(256) void k_phi_dot(McDev D, const int* __restrict_
     t += __shfl_xor(t, 4, 8);
     t += __shfl_xor(t, 2, 8);
     t += __shfl_xor(t, 1, 8);
-    if (u == 0) out[a] = t;
+    if (u == 0) {
+        const int ty = D.type[a];
+        rec[a].phiw = t;
+        rec[a].q = D.q[a];
+        rec[a].type = ty;
+        rec[a].sp = species[ty];
+    }
 }
 
 // Physics part B = lin + selfq + pr + corr of n moves (one thread per move).
@@ -812,12 +869,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         if (tid < ntr) {
             double r;
             int exA, exB;
-            const int i = s_c.i[tid], j = s_c.j[tid];
-            pair_terms(A, D, i, j, r, exA, exB);
+            const AtomRec ri = A.rec[s_c.i[tid]], rj = A.rec[s_c.j[tid]];
+            pair_terms_rec(A, ri, rj, r, exA, exB);
             s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
-            s_c.mt[tid] = move_terms(D, A, i, j, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
-            s_c.lin[tid][0] = A.phiw[i];
-            s_c.lin[tid][1] = A.phiw[j];
+            s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
+            s_c.lin[tid][0] = ri.phiw;
+            s_c.lin[tid][1] = rj.phiw;
         }
         int raw[DA_FEAT];
 #pragma unroll
@@ -1097,7 +1154,7 @@ struct DaState {
     DevBuf<unsigned short> H;
     DevBuf<double> x, y;       // copies of the positions (fixed)
     DevBuf<double> rec_fy;
-    DevBuf<double> phiw;       // W_a . phi per atom (pool atoms)
+    DevBuf<AtomRec> rec;       // per atom: what the chain's physics terms read
     DevBuf<int> cand;          // the pool atoms, in stencil-origin order
     int ncand = 0;
     DaDev dev{};
@@ -1122,8 +1179,8 @@ void rebuild_hist(phb_mc_s* M, DaState* S) {
 void phi_dot(phb_mc_s* M, DaState* S) {
     if (S->ncand == 0) return;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ceil_div(8 * S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->cand.p,
-                           S->ncand, S->phiw.p);
+        hipLaunchKernelGGL(kern, dim3(ceil_div(8 * S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->sp.p,
+                           S->cand.p, S->ncand, S->rec.p);
     };
     switch (M->dev.P) {
         case 3: go(k_phi_dot<3>); break;
@@ -1495,8 +1552,11 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         A.rc2 = M->params->d.rc * M->params->d.rc;
         S->rec_fy.alloc(std::max<long long>(M->d.record_cap, 1));
         A.rec_fy = S->rec_fy.p;
-        S->phiw.alloc(std::max(N, 1));
-        A.phiw = S->phiw.p;
+        S->rec.alloc(std::max(N, 1));
+        A.rec = S->rec.p;
+        if (N > 0)
+            hipLaunchKernelGGL(k_rec_static, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
+                               M->atoms->y.p, M->atoms->z.p, M->selfw.p, S->rec.p);
         {   // pool atoms (sites and ions: a fixed set) sorted by stencil origin
             const int np = M->nsites + M->nions;
             std::vector<int> pool(np), sb(3 * (size_t)N);
