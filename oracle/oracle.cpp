@@ -731,6 +731,9 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
         int i, j;
         if (forward) { i = choose_one(nh_n); j = choose_one(w); }
         else { i = choose_one(nh_p); j = choose_one(cl); }
+        // every trial draws both stages' uniforms, used or not, so the proposal
+        // stream does not depend on the decisions
+        const double u1 = uni(rng), u2 = uni(rng);
         const int sg = forward ? 1 : -1;
         const double sym = sur_pair_sym ? sur_pair_sym[(size_t)i * N + j] : 0.0;
         const double dEs = sur_trial ? sur_trial[t] : (double)sg * sur_pair[(size_t)i * N + j] + sym;
@@ -741,8 +744,7 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
         if (S_before > 0 && S_after > 0) logH = std::log((double)S_before) - std::log((double)S_after);
         double e1 = (double)(-m->beta * (dEs - sg * m->mu_eff)) + logH;
         e1 = std::clamp(e1, -40.0, 40.0);
-        bool acc1 = e1 >= 0.0;
-        if (!acc1) acc1 = std::exp(e1) > uni(rng);
+        const bool acc1 = e1 >= 0.0 || std::exp(e1) > u1;
         int stage = 0;
         double dE = std::numeric_limits<double>::quiet_NaN();
         if (acc1) {
@@ -756,8 +758,7 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
             r1 = std::clamp(r1, -40.0, 40.0);
             const double e2 = (((double)(-m->beta * (dE - sg * m->mu_eff)) + logH) + std::min(0.0, r1)) -
                               std::min(0.0, e1);
-            bool acc2 = e2 >= 0.0;
-            if (!acc2) acc2 = std::exp(e2) > uni(rng);
+            const bool acc2 = e2 >= 0.0 || std::exp(e2) > u2;
             if (acc2) {
                 stage = 1;
                 E = E_trial;

@@ -130,14 +130,17 @@ typedef struct {
     double *dE;                /* exact Delta-E (NaN when stage 1 rejected) */
 } or_da_record;
 
-/* Delayed-acceptance GC chain (Christen & Fox 2005) with the proposal, pools
- * and draw order of or_gc_run.  The surrogate gives dE*(x->y) for the proposed
- * move and dE*(y->x) for its reverse from the proposed state.
+/* Delayed-acceptance GC chain (Christen & Fox 2005) with the proposal and pools
+ * of or_gc_run.  Every trial draws, in order: the direction (when both are
+ * possible), the site and the ion, then two uniforms u1 and u2, used or not, so
+ * the proposal stream does not depend on the decisions.  The surrogate gives
+ * dE*(x->y) for the proposed move and dE*(y->x) for its reverse from the
+ * proposed state.
  *   Stage 1: e1 = clamp(-beta(dE*(x->y) - s mu) + logH, +-40); accept if
- *            e1 >= 0 or exp(e1) > u1 (u1 drawn only when e1 < 0).
+ *            e1 >= 0 or exp(e1) > u1.
  *   Stage 2: r1 = clamp(-beta(dE*(y->x) + s mu) - logH, +-40),
  *            e2 = (-beta(dE - s mu) + logH) + min(0, r1) - min(0, e1);
- *            accept if e2 >= 0 or exp(e2) > u2 (u2 drawn only when e2 < 0).
+ *            accept if e2 >= 0 or exp(e2) > u2.
  * Surrogate sources: per trial, sur_trial[t] and sur_rev_trial[t] (NULL
  * reverse = -sur_trial[t]); else from N x N tables, dE*(x->y) = s A[i,j] +
  * B[i,j] and dE*(y->x) = -s A[i,j] + B[i,j] (sur_pair = A, sur_pair_sym = B or

@@ -704,9 +704,10 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
 
 // ---------------------------------------------------------------- delayed-acceptance chain
 struct CandLds {
-    int i[DA_M], j[DA_M], fwd[DA_M], pos[DA_M], exA[DA_M], exB[DA_M];
+    int i[DA_M], j[DA_M], fwd[DA_M], exA[DA_M], exB[DA_M];
+    int pos[DA_M];                // stream position after the trial's draws
     int ki[DA_M], kj[DA_M];       // pool positions the proposal drew for the site and the ion
-    double u1[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
+    double u1[DA_M], u2[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
     double lin[DA_M][2];          // W_a . phi of the site and the ion
     MoveTerms mt[DA_M];
     float sf[DA_M];               // s f(d)
@@ -737,7 +738,9 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ __align__(16) unsigned short s_X[DA_M * XP];
     __shared__ __align__(16) unsigned short s_H1[DA_M * HP];
     __shared__ float s_y[DA_M][8];
-    __shared__ int s_first;
+    __shared__ int s_first, s_start;
+    __shared__ unsigned long long s_msk;   // stage-1 acceptances of the round
+    __shared__ long long s_done0;          // trials done at the round's start
     __shared__ long long s_s1, s_rounds;
     __shared__ long long s_prof[8];
     extern __shared__ __align__(16) unsigned char s_dyn[];
@@ -760,25 +763,47 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     if (LDS_POOLS)
         for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
-    for (int k = tid; k < 624; k += blockDim.x) {
-        s_rng.cur[k] = C->mt[k];
-        s_rng.out[0][k] = mt_temper(C->mt[k]);
-    }
-    {
+    {   // engine state and network into LDS: every load issued before the first store
+        static_assert(NN_H * NN_IN / 4 == 3 * DA_THREADS && NN_H * NN_H / 4 == 8 * DA_THREADS && NN_H <= DA_THREADS,
+                      "prologue layout");
         const uint2* w1 = reinterpret_cast<const uint2*>(net.w1t);
         const uint2* w2 = reinterpret_cast<const uint2*>(net.w2t);
-        for (int k = tid; k < NN_H * NN_IN / 4; k += blockDim.x) {       // 12 chunks per row
-            const int j = k / (NN_IN / 4), c = k - j * (NN_IN / 4);
-            reinterpret_cast<uint2*>(s_w1t)[j * (XP / 4) + c] = w1[k];
+        uint2 v1[3], v2[8];
+        unsigned mt[2];
+        float b1 = 0.f, b2 = 0.f, w3 = 0.f;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v1[m] = w1[tid + m * DA_THREADS];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v2[m] = w2[tid + m * DA_THREADS];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) mt[m] = tid + m * DA_THREADS < 624 ? C->mt[tid + m * DA_THREADS] : 0u;
+        if (tid < NN_H) {
+            b1 = net.theta[OFF_B1 + tid];
+            b2 = net.theta[OFF_B2 + tid];
+            w3 = bf2f(net.w3[tid]);
         }
-        for (int k = tid; k < NN_H * NN_H / 4; k += blockDim.x) {        // 32 chunks per row
-            const int j = k >> 5, c = k & 31;
-            reinterpret_cast<uint2*>(s_w2t)[j * 32 + (c ^ ((2 * j) & 31))] = w2[k];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {   // 12 chunks per row
+            const int k = tid + m * DA_THREADS, j = k / (NN_IN / 4), c = k - j * (NN_IN / 4);
+            reinterpret_cast<uint2*>(s_w1t)[j * (XP / 4) + c] = v1[m];
         }
-        for (int k = tid; k < NN_H; k += blockDim.x) {
-            s_b1[k] = net.theta[OFF_B1 + k];
-            s_b2[k] = net.theta[OFF_B2 + k];
-            s_w3[k] = bf2f(net.w3[k]);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {   // 32 chunks per row
+            const int k = tid + m * DA_THREADS, j = k >> 5, c = k & 31;
+            reinterpret_cast<uint2*>(s_w2t)[j * 32 + (c ^ ((2 * j) & 31))] = v2[m];
+        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int k = tid + m * DA_THREADS;
+            if (k < 624) {
+                s_rng.cur[k] = mt[m];
+                s_rng.out[0][k] = mt_temper(mt[m]);
+            }
+        }
+        if (tid < NN_H) {
+            s_b1[tid] = b1;
+            s_b2[tid] = b2;
+            s_w3[tid] = w3;
         }
     }
     if (tid == 0) {
@@ -810,11 +835,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     ntr = (int)(room < M ? room : M);
                     const double sum = (double)(wf + wb);
                     const bool both = wf > 0 && wb > 0;
-                    const int per = (both ? 2 : 0) + 4;
+                    // direction (when both are possible), site, ion, u1, u2: fixed per trial
+                    const int per = (both ? 2 : 0) + 6;
                     rng_ensure(s_rng, blk, pos + ntr * per, lane);
                     bool ok = true;
-                    int fwd = 0, ii = 0, jj = 0, pp = 0, ki = 0, kj = 0;
-                    double u = 0.0;
+                    int fwd = 0, ii = 0, jj = 0, ki = 0, kj = 0;
+                    double u = 0.0, uu = 0.0;
                     if (lane < ntr) {
                         int q = pos + lane * per;
                         if (both) { fwd = canonical_at(s_rng, q) < (double)wf / sum ? 1 : 0; q += 2; }
@@ -826,14 +852,14 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                         kj = (int)a2;
                         ii = (int)pbuf[pool_slot(D, fwd ? 0 : 1, ki)];
                         jj = (int)pbuf[pool_slot(D, fwd ? 2 : 3, kj)];
-                        pp = q + 2;
-                        u = canonical_at(s_rng, pp);
+                        u = canonical_at(s_rng, q + 2);
+                        uu = canonical_at(s_rng, q + 4);
                     }
                     if (__all(ok)) {
                         if (lane < ntr) {
                             s_c.i[lane] = ii; s_c.j[lane] = jj; s_c.fwd[lane] = fwd;
                             s_c.ki[lane] = ki; s_c.kj[lane] = kj;
-                            s_c.pos[lane] = pp; s_c.u1[lane] = u;
+                            s_c.pos[lane] = pos + (lane + 1) * per; s_c.u1[lane] = u; s_c.u2[lane] = uu;
                         }
                         pos += ntr * per;
                     } else {
@@ -845,10 +871,11 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                             const int kj2 = (int)draw_index(s_rng, blk, pos, lane, (unsigned)(f2 ? sz2 : sz3));
                             const int i2 = (int)pbuf[pool_slot(D, f2 ? 0 : 1, ki2)];
                             const int j2 = (int)pbuf[pool_slot(D, f2 ? 2 : 3, kj2)];
-                            const int p2 = pos;
-                            const double u2 = draw_canonical(s_rng, blk, pos, lane);
+                            const double v1 = draw_canonical(s_rng, blk, pos, lane);
+                            const double v2 = draw_canonical(s_rng, blk, pos, lane);
                             if (lane == 0) {
-                                s_c.i[k] = i2; s_c.j[k] = j2; s_c.fwd[k] = f2; s_c.pos[k] = p2; s_c.u1[k] = u2;
+                                s_c.i[k] = i2; s_c.j[k] = j2; s_c.fwd[k] = f2; s_c.pos[k] = pos;
+                                s_c.u1[k] = v1; s_c.u2[k] = v2;
                                 s_c.ki[k] = ki2; s_c.kj[k] = kj2;
                             }
                         }
@@ -942,14 +969,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         }
         __syncthreads();
         clk.lap(prof, s_prof, 3);
-        // ---- 4. stage-1 decisions (wave 0): the first stage-1 acceptance ends the round
+        // ---- 4. stage-1 decisions of the round (wave 0)
         if (wave == 0) {
-            const long long done = s_st.done;
             const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
             const long long Sb = a * c + b * d;
             const int k = lane;
             bool acc1 = false;
-            double Fx = 0.0;
             if (k < ntr) {
                 float y = b3;
 #pragma unroll
@@ -958,7 +983,8 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 const int fwd = s_c.fwd[k];
                 const float sf = fwd ? f : -f;
                 const MoveTerms& m = s_c.mt[k];
-                Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr + (double)sf;
+                const double Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr +
+                                  (double)sf;
                 const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
                 const double e1 = mh_exponent(D.beta, Fx, fwd ? 1 : -1, D.mu, Sb, Sa);
                 acc1 = e1 >= 0.0 || exp(e1) > s_c.u1[k];
@@ -967,39 +993,56 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 s_c.Fx[k] = Fx;
             }
             const unsigned long long msk = __ballot(acc1);
-            const int first = msk ? __ffsll((long long)msk) - 1 : ntr;
-            if (k < first && k < ntr && done + k < rec_cap) {
-                const double nan = __longlong_as_double(0x7FF8000000000000LL);
-                D.rec_fwd[done + k] = (signed char)s_c.fwd[k]; D.rec_i[done + k] = s_c.i[k]; D.rec_j[done + k] = s_c.j[k];
-                D.rec_acc[done + k] = 0; D.rec_dE[done + k] = nan; D.rec_u[done + k] = Fx; A.rec_fy[done + k] = nan;
-            }
             if (lane == 0) {
-                s_first = first;
-                s_dc.acc_k = -1;
+                s_msk = msk;
+                s_start = 0;
+                s_done0 = s_st.done;
                 ++s_rounds;
-                if (first >= ntr) s_st.done = done + ntr;
-                else {
-                    ++s_s1;
-                    stage_trial<P>(D, s_tr[0], s_c.i[first], s_c.j[first], s_c.fwd[first]);
-                }
             }
         }
         __syncthreads();
         clk.lap(prof, s_prof, 4);
-        const int first = s_first;
-        if (first < ntr) {
+        // ---- 5./6. the round's stage-1 acceptances in order, until stage 2 accepts one
+        //         (a stage-2 rejection leaves the state, so the later trials stand)
+        for (;;) {
+            if (wave == 0) {
+                const int start = s_start;
+                const unsigned long long m = start < 64 ? (s_msk & (~0ull << start)) : 0ull;
+                const int first = m ? __ffsll((long long)m) - 1 : ntr;
+                const long long done0 = s_done0;
+                const int k = lane;
+                if (k >= start && k < first && k < ntr && done0 + k < rec_cap) {
+                    const double nan = __longlong_as_double(0x7FF8000000000000LL);
+                    D.rec_fwd[done0 + k] = (signed char)s_c.fwd[k]; D.rec_i[done0 + k] = s_c.i[k];
+                    D.rec_j[done0 + k] = s_c.j[k]; D.rec_acc[done0 + k] = 0; D.rec_dE[done0 + k] = nan;
+                    D.rec_u[done0 + k] = s_c.Fx[k]; A.rec_fy[done0 + k] = nan;
+                }
+                if (lane == 0) {
+                    s_first = first;
+                    if (first >= ntr) s_st.done = done0 + ntr;
+                    else {
+                        ++s_s1;
+                        stage_trial<P>(D, s_tr[0], s_c.i[first], s_c.j[first], s_c.fwd[first]);
+                    }
+                }
+            }
+            __syncthreads();
+            const int first = s_first;
+            if (first >= ntr) break;
             // ---- 5. exact Delta-E of the stage-1 acceptance (all waves); its mesh-kernel pair part apart
             stage_trial_data<P>(D, s_tr[0], tid, DA_THREADS);
             __syncthreads();
-            double win = 0.0;
-            const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win));
-            win = wave_sum(win);
-            if (lane == 0) { s_wsum[wave] = part; s_wwin[wave] = win; }
+            {
+                double win = 0.0;
+                const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win));
+                win = wave_sum(win);
+                if (lane == 0) { s_wsum[wave] = part; s_wwin[wave] = win; }
+            }
             __syncthreads();
             clk.lap(prof, s_prof, 5);
-            if (wave == 0) {
+            if (wave == 0 && lane == 0) {
                 const TrialT<P>& t = s_tr[0];
-                const long long done = s_st.done;
+                const long long done0 = s_done0;
                 const double Q = s_st.Q, Q2 = s_st.Q2, Mz = s_st.Mz, Qz2 = s_st.Qz2;
                 const long long a = s_st.sz[0], b = s_st.sz[1], c = s_st.sz[2], d = s_st.sz[3];
                 const long long Sb = a * c + b * d;
@@ -1023,47 +1066,47 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     r1 = fmin(fmax(r1, -40.0), 40.0);
                     e2 = (((double)((-D.beta) * (dE - (double)sg * D.mu)) + logH) + fmin(0.0, r1)) - fmin(0.0, e1);
                 }
-                // roll the stream back to just after this candidate's own draws
-                int blk = s_st.blk, pos = s_c.pos[first] + (e1 >= 0.0 ? 0 : 2);
-                const bool back_ok = pos >= blk * 624 - 624;
-                bool acc2 = e2 >= 0.0;
-                if (!acc2 && back_ok) acc2 = exp(e2) > draw_canonical(s_rng, blk, pos, lane);
-                if (lane == 0) {
-                    if (!back_ok) s_st.status = 2;
-                    s_st.blk = blk; s_st.pos = pos;
-                    const long long tk = done + first;
-                    if (tk < rec_cap) {
-                        D.rec_fwd[tk] = (signed char)fwd; D.rec_i[tk] = t.i; D.rec_j[tk] = t.j;
-                        D.rec_acc[tk] = acc2 ? 1 : 2; D.rec_dE[tk] = dE; D.rec_u[tk] = s_c.Fx[first]; A.rec_fy[tk] = Fy;
-                    }
-                    s_st.done = done + first + 1;
-                    if (acc2) {
-                        const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
-                        s_st.E += dE;
-                        s_st.Q2 = Q2 + ((t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old));
-                        s_st.Q = Q + (dqi + dqj);
-                        s_st.Mz = Mz + (dqi * t.zi + dqj * t.zj);
-                        s_st.Qz2 = Qz2 + (dqi * t.zi * t.zi + dqj * t.zj * t.zj);
-                        ++s_st.acc_n;
-                        if (fwd) { ++s_st.acc_f; ++s_st.nprot; } else { ++s_st.acc_b; --s_st.nprot; }
-                        D.q[t.i] = t.qi_new; D.type[t.i] = t.ti_new;
-                        D.q[t.j] = t.qj_new; D.type[t.j] = t.tj_new;
-                        const int ps = fwd ? 0 : 1, pdd = fwd ? 1 : 0, is = fwd ? 2 : 3, id = fwd ? 3 : 2;
-                        s_dc.er_pool[0] = ps; s_dc.er_val[0] = t.i; s_dc.er_n[0] = s_st.sz[ps];
-                        s_dc.er_pool[1] = is; s_dc.er_val[1] = t.j; s_dc.er_n[1] = s_st.sz[is];
-                        s_dc.pb_pool[0] = pdd; s_dc.pb_val[0] = t.i; s_dc.pb_n[0] = s_st.sz[pdd];
-                        s_dc.pb_pool[1] = id; s_dc.pb_val[1] = t.j; s_dc.pb_n[1] = s_st.sz[id];
-                        s_dc.pos[0] = s_c.ki[first]; s_dc.pos[1] = s_c.kj[first];
-                        if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
-                        else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
-                        s_dc.acc_k = 0;
-                        s_st.pend_n = 2;      // ends the launch: the refresh folds the move into phi
-                    }
+                const bool acc2 = e2 >= 0.0 || exp(e2) > s_c.u2[first];
+                const long long tk = done0 + first;
+                if (tk < rec_cap) {
+                    D.rec_fwd[tk] = (signed char)fwd; D.rec_i[tk] = t.i; D.rec_j[tk] = t.j;
+                    D.rec_acc[tk] = acc2 ? 1 : 2; D.rec_dE[tk] = dE; D.rec_u[tk] = s_c.Fx[first]; A.rec_fy[tk] = Fy;
+                }
+                s_st.done = tk + 1;
+                s_start = first + 1;
+                s_dc.acc_k = -1;
+                if (acc2) {
+                    // the trials after this one are proposed again from the new state:
+                    // the stream resumes right after this trial's draws
+                    const int pos = s_c.pos[first];
+                    if (pos < s_st.blk * 624 - 624) s_st.status = 2;
+                    else s_st.pos = pos;
+                    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+                    s_st.E += dE;
+                    s_st.Q2 = Q2 + ((t.qi_new * t.qi_new - t.qi_old * t.qi_old) + (t.qj_new * t.qj_new - t.qj_old * t.qj_old));
+                    s_st.Q = Q + (dqi + dqj);
+                    s_st.Mz = Mz + (dqi * t.zi + dqj * t.zj);
+                    s_st.Qz2 = Qz2 + (dqi * t.zi * t.zi + dqj * t.zj * t.zj);
+                    ++s_st.acc_n;
+                    if (fwd) { ++s_st.acc_f; ++s_st.nprot; } else { ++s_st.acc_b; --s_st.nprot; }
+                    D.q[t.i] = t.qi_new; D.type[t.i] = t.ti_new;
+                    D.q[t.j] = t.qj_new; D.type[t.j] = t.tj_new;
+                    const int ps = fwd ? 0 : 1, pdd = fwd ? 1 : 0, is = fwd ? 2 : 3, id = fwd ? 3 : 2;
+                    s_dc.er_pool[0] = ps; s_dc.er_val[0] = t.i; s_dc.er_n[0] = s_st.sz[ps];
+                    s_dc.er_pool[1] = is; s_dc.er_val[1] = t.j; s_dc.er_n[1] = s_st.sz[is];
+                    s_dc.pb_pool[0] = pdd; s_dc.pb_val[0] = t.i; s_dc.pb_n[0] = s_st.sz[pdd];
+                    s_dc.pb_pool[1] = id; s_dc.pb_val[1] = t.j; s_dc.pb_n[1] = s_st.sz[id];
+                    s_dc.pos[0] = s_c.ki[first]; s_dc.pos[1] = s_c.kj[first];
+                    if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
+                    else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
+                    s_dc.acc_k = 0;
+                    s_st.pend_n = 2;      // ends the launch: the refresh folds the move into phi
                 }
             }
             __syncthreads();
+            const bool accepted = s_dc.acc_k == 0;
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
-            if (s_dc.acc_k == 0) {
+            if (accepted) {
                 const TrialT<P>& t = s_tr[0];
                 pool_update_at<PoolT>(D, pbuf, s_dc);
                 for (int side = 0; side < 2; ++side) {
@@ -1080,6 +1123,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 }
             }
             clk.lap(prof, s_prof, 6);
+            if (accepted) break;
         }
     }
     // write back

@@ -134,10 +134,10 @@ def _lemire(stream, k, n):
 
 
 def test_da_proposal_stream_when_stage1_rejects_everything():
-    """With dE* = +1e6 on every trial (mu = 0) stage 1 rejects every move and
-    draws u1, so the pools never change and trial t consumes: a direction
-    canonical (2 outputs, only when both directions are possible), two Lemire
-    draws and u1 (2 outputs).  Rebuilt here from the raw mt19937 stream."""
+    """With dE* = +1e6 on every trial (mu = 0) stage 1 rejects every move, so
+    the pools never change and trial t consumes: a direction canonical (2
+    outputs, only when both directions are possible), two Lemire draws, u1 and
+    u2 (2 outputs each).  Rebuilt here from the raw mt19937 stream."""
     s, p, _ = S.config_tiny(order=4, grid=(16, 16, 48))
     mc = S.McParams(beta=1.0, mu_eff=0.0)
     o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
@@ -161,7 +161,7 @@ def test_da_proposal_stream_when_stage1_rejects_everything():
         a, b = (nh_n, w) if fwd else (nh_p, cl)
         ia, k = _lemire(stream, k, len(a))
         ja, k = _lemire(stream, k, len(b))
-        k += 2
+        k += 4
         assert (r["fwd"][m], r["i"][m], r["j"][m]) == (int(fwd), a[ia], b[ja]), m
 
 
