@@ -61,6 +61,11 @@ int phb_atoms_natoms(phb_atoms a, int *natoms);
  * (MC/src/mc.cu:19-41): one transfer for n atoms instead of 2n memcpys. */
 int phb_atoms_set_identity(phb_atoms a, int n, const int *idx, const int *type0, const double *q);
 int phb_atoms_get_identity(phb_atoms a, double *q, int *type0);
+/* New (wrapped) positions for every atom; no counterpart in the reference,
+ * whose positions never change after the upload (MC/src/main.cu:18-41).
+ * Position-dependent state built from the old positions (neighbour lists,
+ * MC objects) is not updated: rebuild it, or use a skin list. */
+int phb_atoms_set_positions(phb_atoms a, const double *x, const double *y, const double *z);
 
 /* -------------------------------------------------------------------- CSR */
 /* Device CSR (rows+1 heads, nnz column ids).  Replaces DeviceCSR
@@ -77,6 +82,22 @@ int phb_csr_download(phb_csr c, int *head, int *list);
  * built from binned cells with wavefront-ballot compaction, bit-exact with
  * the reference's O(N^2) scan. */
 int phb_nlist_build_half(phb_atoms a, const phb_box *box, phb_csr bonds12, double rc, phb_csr *out);
+
+/* Skin (Verlet) list.  No counterpart in the reference, which rebuilds its
+ * O(N^2) list from scratch (build_half_neighbors_gpu, MC/src/neighbor.cu:74-125).
+ * create: the half list at rc + skin from the current positions, which are
+ * snapshotted.  update: the largest min-image displacement since the snapshot;
+ * when it exceeds skin / 2 the outer list and the snapshot are rebuilt
+ * (*rebuilt = 1).  half: the half list at rc (the predicate of
+ * phb_nlist_build_half, rows ascending) filtered from the outer list; while no
+ * atom has moved more than skin / 2 since the last build it equals
+ * phb_nlist_build_half(rc) at the current positions. */
+typedef struct phb_verlet_s *phb_verlet;
+int phb_verlet_create(phb_atoms a, const phb_box *box, phb_csr bonds12, double rc, double skin, phb_verlet *out);
+int phb_verlet_destroy(phb_verlet v);
+int phb_verlet_update(phb_verlet v, phb_atoms a, int *rebuilt, double *max_disp);
+int phb_verlet_half(phb_verlet v, phb_atoms a, phb_csr *out);
+int phb_verlet_stats(phb_verlet v, long long *builds, long long *outer_nnz, double *last_disp);
 
 /* ----------------------------------------------------------------- params */
 /* Replaces DeviceParams (MC/include/device_data.hpp:21-36) and

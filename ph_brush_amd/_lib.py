@@ -145,6 +145,12 @@ SIGNATURES = {
     "phb_csr_shape": [_h, _pi, _pll],
     "phb_csr_download": [_h, _pi, _pi],
     "phb_nlist_build_half": [_h, C.POINTER(Box), _h, C.c_double, C.POINTER(_h)],
+    "phb_atoms_set_positions": [_h, _pd, _pd, _pd],
+    "phb_verlet_create": [_h, C.POINTER(Box), _h, C.c_double, C.c_double, C.POINTER(_h)],
+    "phb_verlet_destroy": [_h],
+    "phb_verlet_update": [_h, _h, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "phb_verlet_half": [_h, _h, C.POINTER(_h)],
+    "phb_verlet_stats": [_h, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
     "phb_params_create": [C.POINTER(ParamsDesc), C.POINTER(_h)],
     "phb_params_destroy": [_h],
     "phb_energy_lj": [_h, _h, _h, C.POINTER(Box), _pd],

@@ -150,6 +150,19 @@ extern "C" int phb_atoms_create(int natoms, const double* x, const double* y, co
     });
 }
 
+extern "C" int phb_atoms_set_positions(phb_atoms a, const double* x, const double* y, const double* z) {
+    return guard([&] {
+        require(a, "phb_atoms_set_positions: null handle");
+        require(a->n == 0 || (x && y && z), "phb_atoms_set_positions: null array");
+        for (int i = 0; i < a->n; ++i)
+            require(std::isfinite(x[i]) && std::isfinite(y[i]) && std::isfinite(z[i]),
+                    "phb_atoms_set_positions: non-finite position");
+        a->x.upload(x, a->n); a->y.upload(y, a->n); a->z.upload(z, a->n);
+        sync();
+        return PHB_OK;
+    });
+}
+
 extern "C" int phb_atoms_destroy(phb_atoms a) {
     return guard([&] { delete a; return PHB_OK; });
 }

@@ -287,6 +287,13 @@ __device__ __forceinline__ int wrap1(int x, int n) {  // x in (-n, 2n)
     return x >= n ? x - n : x;
 }
 
+__device__ __forceinline__ int wrap2(int x, int n) {  // x in [-2n, 3n)
+    x += x < 0 ? n : 0;
+    x += x < 0 ? n : 0;
+    x -= x >= n ? n : 0;
+    return x >= n ? x - n : x;
+}
+
 __device__ __forceinline__ int fold(int x, int n) {   // x in [0, n) -> periodic |x|
     return 2 * x <= n ? x : n - x;
 }

@@ -341,13 +341,6 @@ __host__ __device__ constexpr int refresh_direct_blocks(int ny, int nz) {
     return ((ny + RD_TY - 1) / RD_TY) * ((nz + RD_TZ - 1) / RD_TZ);
 }
 
-__device__ __forceinline__ int wrap2(int x, int n) {  // x in [-2n, 3n)
-    x += x < 0 ? n : 0;
-    x += x < 0 ? n : 0;
-    x -= x >= n ? n : 0;
-    return x >= n ? x - n : x;
-}
-
 template <int P>
 __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __restrict__ phi, float* __restrict__ phi32,
                                                     int blk) {

@@ -134,23 +134,62 @@ int cells_along(double L, double rc) {
 
 }  // namespace
 
-extern "C" int phb_nlist_build_half(phb_atoms a, const phb_box* box, phb_csr bonds12, double rc, phb_csr* out) {
-    return guard([&] {
-        require(a && box && out, "build_half_neighbors_gpu: null argument");
-        require(rc > 0.0, "build_half_neighbors_gpu: rc must be > 0");
-        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "build_half_neighbors_gpu: box must be positive");
-        const int N = a->n;
-        if (bonds12) require(bonds12->rows == N, "build_half_neighbors_gpu: bond CSR rows != natoms");
-        auto c = std::make_unique<phb_csr_s>();
-        c->rows = N;
-        c->head.alloc(N + 1);
-        if (N == 0) {
-            int zero = 0;
-            c->head.upload(&zero, 1);
-            sync();
-            *out = c.release();
-            return PHB_OK;
+namespace {
+
+// Per-block maxima of the squared min-image displacement since the snapshot.
+__global__ __launch_bounds__(256) void k_max_disp(const double* __restrict__ x, const double* __restrict__ y,
+                                                  const double* __restrict__ z, const double* __restrict__ x0,
+                                                  const double* __restrict__ y0, const double* __restrict__ z0, int N,
+                                                  double Lx, double Ly, double Lz, double* __restrict__ part) {
+    __shared__ double s[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double d2 = 0.0;
+    if (i < N) {
+        const double dx = min_image_exact(x[i] - x0[i], Lx), dy = min_image_exact(y[i] - y0[i], Ly),
+                     dz = min_image_exact(z[i] - z0[i], Lz);
+        d2 = r2_exact(dx, dy, dz);
+    }
+    for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+    if (lane == 0) s[w] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+}
+
+// MODE 0: count the entries of the outer row i within rc; MODE 1: copy them (order kept).
+template <int MODE>
+__global__ void k_filter_rows(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                              int N, double Lx, double Ly, double Lz, double rc2, const int* __restrict__ oh,
+                              const int* __restrict__ ol, int* __restrict__ counts, const int* __restrict__ head,
+                              int* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double xi = x[i], yi = y[i], zi = z[i];
+    int n = 0;
+    for (int e = oh[i]; e < oh[i + 1]; ++e) {
+        const int j = ol[e];
+        const double dx = min_image_exact(xi - x[j], Lx), dy = min_image_exact(yi - y[j], Ly),
+                     dz = min_image_exact(zi - z[j], Lz);
+        if (r2_exact(dx, dy, dz) <= rc2) {
+            if (MODE == 1) list[head[i] + n] = j;
+            ++n;
         }
+    }
+    if (MODE == 0) counts[i] = n;
+}
+
+void build_half(phb_atoms_s* a, const phb_box* box, phb_csr_s* bonds12, double rc, phb_csr_s* c) {
+    const int N = a->n;
+    c->rows = N;
+    c->nnz = 0;
+    c->head.alloc(N + 1);
+    c->list.release();
+    if (N == 0) {
+        int zero = 0;
+        c->head.upload(&zero, 1);
+        sync();
+        return;
+    }
+    {
         CellGeom g{cells_along(box->Lx, rc), cells_along(box->Ly, rc), cells_along(box->Lz, rc), box->Lx, box->Ly,
                    box->Lz};
         const long long ncell = (long long)g.ncx * g.ncy * g.ncz;
@@ -187,7 +226,142 @@ extern "C" int phb_nlist_build_half(phb_atoms a, const phb_box* box, phb_csr bon
             PHB_HIP(hipGetLastError());
             sync();
         }
+    }
+}
+
+}  // namespace
+
+extern "C" int phb_nlist_build_half(phb_atoms a, const phb_box* box, phb_csr bonds12, double rc, phb_csr* out) {
+    return guard([&] {
+        require(a && box && out, "build_half_neighbors_gpu: null argument");
+        require(rc > 0.0, "build_half_neighbors_gpu: rc must be > 0");
+        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "build_half_neighbors_gpu: box must be positive");
+        if (bonds12) require(bonds12->rows == a->n, "build_half_neighbors_gpu: bond CSR rows != natoms");
+        auto c = std::make_unique<phb_csr_s>();
+        build_half(a, box, bonds12, rc, c.get());
         *out = c.release();
+        return PHB_OK;
+    });
+}
+
+// ------------------------------------------------------------ skin (Verlet) list
+struct phb_verlet_s {
+    phb_box box{};
+    phb_csr_s* bonds = nullptr;
+    double rc = 0.0, skin = 0.0;
+    phb_csr_s outer;                       // half list at rc + skin from the snapshot
+    DevBuf<double> x0, y0, z0;             // positions at the last build
+    DevBuf<double> part;
+    long long builds = 0;
+    double last_disp = 0.0;                // largest displacement seen by the last update
+};
+
+namespace {
+
+void verlet_build(phb_verlet_s* v, phb_atoms_s* a) {
+    build_half(a, &v->box, v->bonds, v->rc + v->skin, &v->outer);
+    const size_t N = (size_t)a->n;
+    v->x0.alloc(std::max<size_t>(N, 1)); v->y0.alloc(std::max<size_t>(N, 1)); v->z0.alloc(std::max<size_t>(N, 1));
+    if (N) {
+        PHB_HIP(hipMemcpyAsync(v->x0.p, a->x.p, N * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+        PHB_HIP(hipMemcpyAsync(v->y0.p, a->y.p, N * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+        PHB_HIP(hipMemcpyAsync(v->z0.p, a->z.p, N * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+    }
+    sync();
+    ++v->builds;
+}
+
+}  // namespace
+
+extern "C" int phb_verlet_create(phb_atoms a, const phb_box* box, phb_csr bonds12, double rc, double skin,
+                                 phb_verlet* out) {
+    return guard([&] {
+        require(a && box && out, "phb_verlet_create: null argument");
+        require(rc > 0.0 && skin >= 0.0, "phb_verlet_create: need rc > 0 and skin >= 0");
+        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "phb_verlet_create: box must be positive");
+        if (bonds12) require(bonds12->rows == a->n, "phb_verlet_create: bond CSR rows != natoms");
+        auto v = std::make_unique<phb_verlet_s>();
+        v->box = *box; v->bonds = bonds12; v->rc = rc; v->skin = skin;
+        verlet_build(v.get(), a);
+        *out = v.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_verlet_destroy(phb_verlet v) {
+    return guard([&] { delete v; return PHB_OK; });
+}
+
+extern "C" int phb_verlet_update(phb_verlet v, phb_atoms a, int* rebuilt, double* max_disp) {
+    return guard([&] {
+        require(v && a, "phb_verlet_update: null argument");
+        require(a->n == v->outer.rows, "phb_verlet_update: atom count changed");
+        const int N = a->n;
+        double m2 = 0.0;
+        if (N > 0) {
+            const int nb = ceil_div(N, 256);
+            v->part.alloc(nb);
+            hipLaunchKernelGGL(k_max_disp, dim3(nb), dim3(256), 0, stream(), a->x.p, a->y.p, a->z.p, v->x0.p, v->y0.p,
+                               v->z0.p, N, v->box.Lx, v->box.Ly, v->box.Lz, v->part.p);
+            PHB_HIP(hipGetLastError());
+            std::vector<double> h(nb);
+            v->part.download(h.data(), nb);
+            sync();
+            for (double d : h) m2 = std::max(m2, d);
+        }
+        v->last_disp = std::sqrt(m2);
+        // every pair now within rc was within rc + skin at the build while no atom moved more than skin / 2
+        const bool need = 2.0 * v->last_disp > v->skin;
+        if (need) verlet_build(v, a);
+        if (rebuilt) *rebuilt = need ? 1 : 0;
+        if (max_disp) *max_disp = v->last_disp;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_verlet_half(phb_verlet v, phb_atoms a, phb_csr* out) {
+    return guard([&] {
+        require(v && a && out, "phb_verlet_half: null argument");
+        require(a->n == v->outer.rows, "phb_verlet_half: atom count changed");
+        const int N = a->n;
+        auto c = std::make_unique<phb_csr_s>();
+        c->rows = N;
+        c->head.alloc(N + 1);
+        if (N == 0) {
+            int zero = 0;
+            c->head.upload(&zero, 1);
+            sync();
+            *out = c.release();
+            return PHB_OK;
+        }
+        DevBuf<int> counts(N);
+        const double rc2 = v->rc * v->rc;
+        hipLaunchKernelGGL(k_filter_rows<0>, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), a->x.p, a->y.p, a->z.p, N,
+                           v->box.Lx, v->box.Ly, v->box.Lz, rc2, v->outer.head.p, v->outer.list.p, counts.p,
+                           (const int*)nullptr, (int*)nullptr);
+        exclusive_scan(counts.p, c->head.p, N);
+        int total = 0;
+        PHB_HIP(hipMemcpyAsync(&total, c->head.p + N, sizeof(int), hipMemcpyDeviceToHost, stream()));
+        sync();
+        c->nnz = total;
+        c->list.alloc(total);
+        if (total > 0)
+            hipLaunchKernelGGL(k_filter_rows<1>, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), a->x.p, a->y.p,
+                               a->z.p, N, v->box.Lx, v->box.Ly, v->box.Lz, rc2, v->outer.head.p, v->outer.list.p,
+                               (int*)nullptr, c->head.p, c->list.p);
+        PHB_HIP(hipGetLastError());
+        sync();
+        *out = c.release();
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_verlet_stats(phb_verlet v, long long* builds, long long* outer_nnz, double* last_disp) {
+    return guard([&] {
+        require(v, "phb_verlet_stats: null argument");
+        if (builds) *builds = v->builds;
+        if (outer_nnz) *outer_nnz = v->outer.nnz;
+        if (last_disp) *last_disp = v->last_disp;
         return PHB_OK;
     });
 }

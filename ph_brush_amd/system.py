@@ -242,6 +242,12 @@ class DeviceAtoms(_Handle):
         L.call("phb_atoms_get_identity", self.h, L.dptr(q), L.iptr(t))
         return q, t
 
+    def set_positions(self, x, y, z):
+        """New wrapped positions (phb_atoms_set_positions)."""
+        v = [np.ascontiguousarray(a, np.float64) for a in (x, y, z)]
+        assert all(len(a) == self.n for a in v)
+        L.call("phb_atoms_set_positions", self.h, *(L.dptr(a) for a in v))
+
 
 class DeviceCSR(_Handle):
     """Replaces DeviceCSR / copy_csr_to_device (MC/src/neighbor.cu:65-72)."""
@@ -277,6 +283,34 @@ def build_half_neighbors_gpu(atoms: DeviceAtoms, box: L.Box, bonds12: DeviceCSR 
     c = DeviceCSR()
     L.call("phb_nlist_build_half", atoms.h, C.byref(box), bonds12.h if bonds12 else None, rc, C.byref(c.h))
     return c
+
+
+class VerletList(_Handle):
+    """Skin list (phb_verlet_*): the half list at rc + skin, rebuilt only when an
+    atom has moved more than skin / 2 since the last build; half() gives the rc
+    list, equal to build_half_neighbors_gpu at the current positions."""
+    _destroy = "phb_verlet_destroy"
+
+    def __init__(self, atoms: DeviceAtoms, box: L.Box, bonds12: DeviceCSR | None, rc: float, skin: float = 0.3):
+        super().__init__()
+        self._keep = (bonds12,)
+        L.call("phb_verlet_create", atoms.h, C.byref(box), bonds12.h if bonds12 else None, rc, skin,
+               C.byref(self.h))
+
+    def update(self, atoms: DeviceAtoms):
+        rebuilt, disp = C.c_int(0), C.c_double(0)
+        L.call("phb_verlet_update", self.h, atoms.h, C.byref(rebuilt), C.byref(disp))
+        return bool(rebuilt.value), disp.value
+
+    def half(self, atoms: DeviceAtoms) -> DeviceCSR:
+        c = DeviceCSR()
+        L.call("phb_verlet_half", self.h, atoms.h, C.byref(c.h))
+        return c
+
+    def stats(self):
+        b, n, d = C.c_longlong(0), C.c_longlong(0), C.c_double(0)
+        L.call("phb_verlet_stats", self.h, C.byref(b), C.byref(n), C.byref(d))
+        return {"builds": b.value, "outer_nnz": n.value, "last_disp": d.value}
 
 
 class DeviceParams(_Handle):
