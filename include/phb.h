@@ -221,6 +221,34 @@ int phb_mc_delta_e(phb_mc mc, int n, const int *i, const int *j, const int *fwd,
 int phb_mc_recompute(phb_mc mc, double *E);
 /* MT19937 state: 624 words then the index (625 unsigned). */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
+
+/* Phase 2: the 4-point and W-Cl-only move rule of phase2_swap_jitter
+ * (MC/src/mc.cu:159-315) on an MC object, continuing its engine stream.  begin:
+ * the pools rebuilt in ascending atom order from the current identities, as the
+ * phase-1 loop hands them over (MC/src/mc.cu:480-494; W and Cl below the pool z
+ * cut), and the mode chosen as the reference does (MC/src/mc.cu:193-207): the
+ * 4-point move unless NH_N is empty or no 4-point move exists, then W-Cl only,
+ * else no valid proposals.  run: up to max_attempts more trials; a trial is
+ * accepted if dE <= 0 or exp(clamp(-beta dE, +-40)) > u (u drawn only when
+ * dE > 0); stops after `patience` trials without E < best_E - best_eps. */
+typedef struct {
+    int patience;
+    double best_eps;           /* reference: 1e-9 */
+    long long record_cap;      /* per-trial records kept */
+} phb_phase2_desc;
+typedef struct {
+    long long attempted, accepted;
+    int mode;                  /* 4: 4-point, 2: W-Cl only, 0: no valid proposals */
+    int stop;                  /* 0 running, 1 patience reached, 2 no valid proposals */
+    int since_best, status;
+    double E, best_E;
+} phb_phase2_stats;
+int phb_phase2_begin(phb_mc mc, const phb_phase2_desc *d, phb_phase2_stats *out);
+int phb_phase2_run(phb_mc mc, long long max_attempts, phb_phase2_stats *out);
+/* First n trials: atoms (4 per trial in draw order iP, iN, jCl, jW; W-Cl only: jCl, jW, -1, -1),
+ * accepted flag and exact Delta-E. */
+int phb_phase2_record(phb_mc mc, long long n, int *atoms, signed char *acc, double *dE);
+
 /* Time every chain launch, refresh and recompute with HIP events (on = 1). */
 int phb_mc_set_profiling(phb_mc mc, int on);
 

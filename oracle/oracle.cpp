@@ -19,6 +19,7 @@
 #include <random>
 #include <stdexcept>
 #include <vector>
+#include <sstream>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -684,6 +685,119 @@ void or_da_descriptors(const or_system* s, const or_desc_params* dp, int n, cons
         }
         if (rij) rij[m] = std::sqrt(pair_r2(s, a, b));
     }
+}
+
+// phase2_swap_jitter (MC/src/mc.cu:159-315) after the phase-1 loop reached its
+// target: pools rebuilt in ascending atom order from the identities
+// (MC/src/mc.cu:480-494), W and Cl below z_cut (MC/src/mc.cu:177-184); the
+// engine resumes from a saved state (the 624 words and the index, libstdc++'s
+// stream form).  Every trial is recomputed in full, as the reference does.
+int or_phase2_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,
+                  const unsigned* rng_state, long long max_attempts, int patience, double best_eps, int* rec_atoms,
+                  signed char* rec_acc, double* rec_dE, double* q_out, int* type_out, double* E_out,
+                  double* best_out, long long* attempted, long long* accepted, int* mode, int* stop) {
+    const int N = s0->natoms;
+    std::vector<double> q(s0->q, s0->q + N);
+    std::vector<int> type(s0->type, s0->type + N);
+    or_system s = *s0;
+    s.q = q.data(); s.type = type.data();
+    double z_max_site = 0.0;
+    for (int i = 0; i < N; ++i)
+        if (type[i] + 1 == m->nh_n_type || type[i] + 1 == m->nh_p_type) z_max_site = std::max(z_max_site, s.z[i]);
+    const double z_cut = z_max_site + m->w_z_extra;
+    std::vector<int> nh_p, nh_n, w_cand, cl_pool;
+    for (int i = 0; i < N; ++i) {
+        const int t = type[i] + 1;
+        if (t == m->nh_p_type) nh_p.push_back(i);
+        else if (t == m->nh_n_type) nh_n.push_back(i);
+        else if (t == m->w_type) { if (s.z[i] < z_cut) w_cand.push_back(i); }
+        else if (t == m->cl_type) { if (s.z[i] < z_cut) cl_pool.push_back(i); }
+    }
+    std::mt19937 rng;
+    {
+        std::ostringstream os;
+        for (int k = 0; k < 624; ++k) os << rng_state[k] << ' ';
+        os << rng_state[624];
+        std::istringstream is(os.str());
+        is >> rng;
+    }
+    double E = or_total_energy(&s, p, head, list, nullptr);
+    double best_E = E;
+    auto choose_one = [&](const std::vector<int>& pool) {
+        std::uniform_int_distribution<int> d(0, (int)pool.size() - 1);
+        return pool[d(rng)];
+    };
+    auto can_four = [&]() { return !nh_p.empty() && !nh_n.empty() && !w_cand.empty() && !cl_pool.empty(); };
+    auto can_wc = [&]() { return !w_cand.empty() && !cl_pool.empty(); };
+    bool prefer_wc = nh_n.empty();
+    if (!prefer_wc && !can_four() && can_wc()) prefer_wc = true;
+    int md = 0, st = 0;
+    long long att = 0, acc = 0;
+    if ((!prefer_wc && !can_four() && !can_wc()) || (prefer_wc && !can_wc())) st = 2;
+    else {
+        md = prefer_wc ? 2 : 4;
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        int since_best = 0;
+        for (long long step = 0; step < max_attempts; ++step) {
+            const bool mode_four = (!prefer_wc) && can_four();
+            ++att;
+            int at[4] = {-1, -1, -1, -1};
+            int nt[4] = {0, 0, 0, 0};
+            double nq[4] = {0, 0, 0, 0};
+            int K;
+            if (mode_four) {
+                at[0] = choose_one(nh_p); at[1] = choose_one(nh_n); at[2] = choose_one(cl_pool); at[3] = choose_one(w_cand);
+                nt[0] = m->nh_n_type - 1; nq[0] = 0.0;
+                nt[1] = m->nh_p_type - 1; nq[1] = 1.0;
+                nt[2] = m->w_type - 1; nq[2] = 0.0;
+                nt[3] = m->cl_type - 1; nq[3] = -1.0;
+                K = 4;
+            } else {
+                at[0] = choose_one(cl_pool); at[1] = choose_one(w_cand);
+                nt[0] = m->w_type - 1; nq[0] = 0.0;
+                nt[1] = m->cl_type - 1; nq[1] = -1.0;
+                K = 2;
+            }
+            int ot[4];
+            double oq[4];
+            for (int k = 0; k < K; ++k) { ot[k] = type[at[k]]; oq[k] = q[at[k]]; type[at[k]] = nt[k]; q[at[k]] = nq[k]; }
+            const double E_trial = or_total_energy(&s, p, head, list, nullptr);
+            const double dE = E_trial - E;
+            const double expo = std::clamp(-m->beta * dE, -40.0, 40.0);
+            const bool ok = (dE <= 0.0) || (std::exp(expo) > uni(rng));
+            if (rec_atoms) for (int k = 0; k < 4; ++k) rec_atoms[4 * step + k] = at[k];
+            if (rec_acc) rec_acc[step] = ok ? 1 : 0;
+            if (rec_dE) rec_dE[step] = dE;
+            if (ok) {
+                ++acc;
+                E = E_trial;
+                if (E < best_E - best_eps) { best_E = E; since_best = 0; }
+                else ++since_best;
+                if (mode_four) {
+                    nh_p.erase(std::find(nh_p.begin(), nh_p.end(), at[0])); nh_n.push_back(at[0]);
+                    nh_n.erase(std::find(nh_n.begin(), nh_n.end(), at[1])); nh_p.push_back(at[1]);
+                    cl_pool.erase(std::find(cl_pool.begin(), cl_pool.end(), at[2])); w_cand.push_back(at[2]);
+                    w_cand.erase(std::find(w_cand.begin(), w_cand.end(), at[3])); cl_pool.push_back(at[3]);
+                } else {
+                    cl_pool.erase(std::find(cl_pool.begin(), cl_pool.end(), at[0])); w_cand.push_back(at[0]);
+                    w_cand.erase(std::find(w_cand.begin(), w_cand.end(), at[1])); cl_pool.push_back(at[1]);
+                }
+            } else {
+                for (int k = 0; k < K; ++k) { type[at[k]] = ot[k]; q[at[k]] = oq[k]; }
+                ++since_best;
+            }
+            if (since_best >= patience) { st = 1; break; }
+        }
+    }
+    if (q_out) std::copy(q.begin(), q.end(), q_out);
+    if (type_out) std::copy(type.begin(), type.end(), type_out);
+    if (E_out) *E_out = E;
+    if (best_out) *best_out = best_E;
+    if (attempted) *attempted = att;
+    if (accepted) *accepted = acc;
+    if (mode) *mode = md;
+    if (stop) *stop = st;
+    return 0;
 }
 
 int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,

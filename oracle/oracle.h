@@ -98,6 +98,17 @@ int or_gc_run(const or_system *s, const or_params *p, const or_mc_params *m,
               or_mc_record *rec, double *q_out, int *type_out, double *E_out,
               long long *n_done);
 
+/* ---- phase 2: 4-point / W-Cl-only moves with patience (MC/src/mc.cu:159-315) ----
+ * Pools from the identities in ascending atom order (the phase-1 hand-over,
+ * MC/src/mc.cu:480-494); the engine resumes from rng_state (624 words + index).
+ * Records: 4 atoms per trial (draw order; -1 unused), accepted flag, exact dE.
+ * mode: 4, 2 or 0 (no valid proposals); stop: 0 max_attempts, 1 patience, 2 none. */
+int or_phase2_run(const or_system *s, const or_params *p, const or_mc_params *m, const int *head,
+                  const int *list, const unsigned *rng_state, long long max_attempts, int patience,
+                  double best_eps, int *rec_atoms, signed char *rec_acc, double *rec_dE, double *q_out,
+                  int *type_out, double *E_out, double *best_out, long long *attempted,
+                  long long *accepted, int *mode, int *stop);
+
 /* ---- mt19937 + libstdc++ distributions, for known-answer tests ---- */
 void or_mt_draws(unsigned seed, long long n, unsigned *out);
 void or_uniform_int_draws(unsigned seed, long long n, const int *range_hi, int *out);

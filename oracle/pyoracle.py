@@ -94,6 +94,11 @@ def load():
         lib.or_da_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
                                   C.c_longlong, C.c_uint, _pd, _pd, _pd, _pd, C.POINTER(OrDaRecord), _pd, _pi,
                                   _pd, C.POINTER(C.c_longlong)]
+        lib.or_phase2_run.restype = C.c_int
+        lib.or_phase2_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
+                                      _pu, C.c_longlong, C.c_int, C.c_double, _pi, _pb, _pd, _pd, _pi, _pd, _pd,
+                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
         lib.or_desc_thresholds.argtypes = [C.c_double, _pd]
         lib.or_da_descriptors.argtypes = [C.POINTER(OrSystem), C.POINTER(OrDescParams), C.c_int, _pi, _pi, _pi, _pd]
         lib.or_mt_draws.argtypes = [C.c_uint, C.c_longlong, _pu]
@@ -200,6 +205,28 @@ class Oracle:
                                 C.byref(E), C.byref(nd))
         n = nd.value
         return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}
+
+    def phase2_run(self, mc, rng_state, max_attempts, patience, best_eps=1e-9):
+        """phase2_swap_jitter restated (MC/src/mc.cu:159-315) from the current
+        identities, the engine resumed from rng_state (625 words)."""
+        N = self.s.natoms
+        st = np.ascontiguousarray(rng_state, dtype=np.uint32)
+        assert st.shape == (625,)
+        atoms = np.zeros((max_attempts, 4), np.int32)
+        acc = np.zeros(max_attempts, np.int8)
+        dE = np.zeros(max_attempts)
+        m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
+        q, t = np.zeros(N), np.zeros(N, np.int32)
+        E, best = C.c_double(), C.c_double()
+        att, nacc = C.c_longlong(), C.c_longlong()
+        mode, stop = C.c_int(), C.c_int()
+        self.lib.or_phase2_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), st.ctypes.data_as(_pu),
+                               max_attempts, patience, best_eps, atoms.ctypes.data_as(_pi), acc.ctypes.data_as(_pb),
+                               dE.ctypes.data_as(_pd), q.ctypes.data_as(_pd), t.ctypes.data_as(_pi), C.byref(E),
+                               C.byref(best), C.byref(att), C.byref(nacc), C.byref(mode), C.byref(stop))
+        n = att.value
+        return {"n": n, "accepted": nacc.value, "mode": mode.value, "stop": stop.value, "E": E.value,
+                "best_E": best.value, "q": q, "type0": t, "atoms": atoms[:n], "acc": acc[:n], "dE": dE[:n]}
 
     def da_descriptors(self, species_of_type, r_desc, i, j):
         """Brute-force shell-count descriptors d = h_i - h_j (int, n x 48) and r_ij."""

@@ -79,6 +79,18 @@ class McStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Phase2Desc(C.Structure):
+    _fields_ = [("patience", C.c_int), ("best_eps", C.c_double), ("record_cap", C.c_longlong)]
+
+
+class Phase2Stats(C.Structure):
+    _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("mode", C.c_int), ("stop", C.c_int),
+                ("since_best", C.c_int), ("status", C.c_int), ("E", C.c_double), ("best_E", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TrainDesc(C.Structure):
     _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double), ("seed", C.c_uint),
                 ("n_train", C.c_longlong), ("normalize", C.c_int)]
@@ -146,6 +158,9 @@ SIGNATURES = {
     "phb_csr_download": [_h, _pi, _pi],
     "phb_nlist_build_half": [_h, C.POINTER(Box), _h, C.c_double, C.POINTER(_h)],
     "phb_atoms_set_positions": [_h, _pd, _pd, _pd],
+    "phb_phase2_begin": [_h, C.POINTER(Phase2Desc), C.POINTER(Phase2Stats)],
+    "phb_phase2_run": [_h, C.c_longlong, C.POINTER(Phase2Stats)],
+    "phb_phase2_record": [_h, C.c_longlong, _pi, C.POINTER(C.c_byte), _pd],
     "phb_verlet_create": [_h, C.POINTER(Box), _h, C.c_double, C.c_double, C.POINTER(_h)],
     "phb_verlet_destroy": [_h],
     "phb_verlet_update": [_h, _h, C.POINTER(C.c_int), C.POINTER(C.c_double)],

@@ -476,6 +476,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         for (int i : nh_n) z_max_site = std::max(z_max_site, hz[i]);
         for (int i : nh_p) z_max_site = std::max(z_max_site, hz[i]);
         const double z_cut = z_max_site + d->w_z_extra;
+        M->z_cut = z_cut;
         { std::vector<int> f; for (int j : w) if (hz[j] < z_cut) f.push_back(j); w.swap(f); }
         for (int j = 0; j < N; ++j) if (ht[j] + 1 == d->cl_type && hz[j] < z_cut) cl.push_back(j);
         M->nsites = (int)(nh_n.size() + nh_p.size());

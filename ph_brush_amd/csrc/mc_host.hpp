@@ -39,11 +39,15 @@ struct phb_mc_s {
     size_t ev_used = 0;
     std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)
     double chain_ms = 0, refresh_ms = 0, recompute_ms = 0;
-    // delayed-acceptance state (descriptor lists and histograms), owned here
+    double z_cut = 0.0;        // pool z filter of W and Cl (max site z + w_z_extra)
+    // delayed-acceptance state (descriptor lists and histograms) and phase-2 state, owned here
     void* da = nullptr;
     void (*da_free)(void*) = nullptr;
+    void* p2 = nullptr;
+    void (*p2_free)(void*) = nullptr;
     ~phb_mc_s() {
         if (da_free) da_free(da);
+        if (p2_free) p2_free(p2);
         for (auto e : ev) (void)hipEventDestroy(e);
     }
     hipEvent_t event() {

@@ -443,6 +443,25 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_rng_state", self.h, st)
         return np.array(st[:], dtype=np.uint32)
 
+    # ---- phase 2 (phase2_swap_jitter's move rule, MC/src/mc.cu:159-315)
+    def phase2_begin(self, patience: int, best_eps: float = 1e-9, record_cap: int = 0) -> dict:
+        d = L.Phase2Desc(patience, best_eps, record_cap)
+        st = L.Phase2Stats()
+        L.call("phb_phase2_begin", self.h, C.byref(d), C.byref(st))
+        self.p2_record_cap = record_cap
+        return st.as_dict()
+
+    def phase2_run(self, max_attempts: int) -> dict:
+        st = L.Phase2Stats()
+        L.call("phb_phase2_run", self.h, max_attempts, C.byref(st))
+        return st.as_dict()
+
+    def phase2_record(self, n: int) -> dict:
+        out = {"atoms": np.zeros((n, 4), np.int32), "acc": np.zeros(n, np.int8), "dE": np.zeros(n)}
+        L.call("phb_phase2_record", self.h, n, L.iptr(out["atoms"]), out["acc"].ctypes.data_as(C.POINTER(C.c_byte)),
+               L.dptr(out["dE"]))
+        return out
+
 
     # ---- delayed acceptance (no reference counterpart)
     def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64):
