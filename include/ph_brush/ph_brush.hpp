@@ -58,6 +58,8 @@ struct Params {
     real beta{1.0 / (0.008314462 * 300.0)};
     real w_z_extra_nm{1.0};
     real mu_eff{0.0};
+    int patience{3000};            // phase 2 (MC/include/types.hpp:58-59)
+    int max_attempts{1000000};
 };
 
 // build_bond_adjacency (MC/src/io.cpp:148-169)
@@ -239,6 +241,13 @@ struct PhaseGCResult {
     real mu_eff{0.0};
 };
 
+// Phase2Result (MC/include/mc.hpp:29-34).
+struct Phase2Result {
+    long long attempted{0}, accepted{0};
+    real E{0.0};
+    real best_E{0.0};
+};
+
 // Device chain at fixed mu: the move rule of phase_gc_mu_control
 // (MC/src/mc.cu:379-503) with std::mt19937(p.rng_seed) semantics.
 class MonteCarlo : public Handle<phb_mc, phb_mc_destroy> {
@@ -266,6 +275,18 @@ public:
         r.Nh = s.nsites; r.Nprot = s.nprot; r.E = s.E; r.mu_eff = mu_;
         return r;
     }
+    // phase2_swap_jitter's loop (MC/src/mc.cu:159-315) continuing this chain's
+    // engine: pools from the current identities, patience and max_attempts from hp.
+    Phase2Result phase2(const Params& hp) {
+        phb_phase2_desc d{};
+        d.patience = hp.patience; d.best_eps = 1e-9; d.record_cap = 0;
+        phb_phase2_stats s{};
+        check(phb_phase2_begin(h_, &d, &s));
+        if (s.mode != 0) check(phb_phase2_run(h_, hp.max_attempts, &s));
+        Phase2Result r;
+        r.attempted = s.attempted; r.accepted = s.accepted; r.E = s.E; r.best_E = s.best_E;
+        return r;
+    }
 private:
     double mu_ = 0.0;
 };
@@ -275,5 +296,7 @@ inline PhaseGCResult phase_gc_fixed_mu(const DeviceAtoms& a, const DeviceParams&
     MonteCarlo mc(a, p, neigh, plan, box, hp);
     return mc.run(ntrials);
 }
+
+inline Phase2Result phase2_swap_jitter(MonteCarlo& mc, const Params& hp) { return mc.phase2(hp); }
 
 }  // namespace ph_brush

@@ -52,8 +52,14 @@ int main(int argc, char** argv) {
         std::printf("nnz %lld\n", d_neigh.nnz);
         std::printf("E_lj %.17g\nE_coul %.17g\nE_real %.17g\nE_recip %.17g\nE_self %.17g\nE_qcorr %.17g\n", E_lj,
                     E_coul, comps.real_space, comps.recip_space, comps.self_term, comps.qcorr_term);
-        PhaseGCResult r = phase_gc_fixed_mu(d_atoms, d_params, d_neigh, plan, sys.box, p, std::atoll(argv[2]));
+        MonteCarlo mc(d_atoms, d_params, d_neigh, plan, sys.box, p);
+        PhaseGCResult r = mc.run(std::atoll(argv[2]));
         std::printf("attempted %lld\naccepted %lld\nnprot %d\nE_after %.17g\n", r.attempted, r.accepted, r.Nprot, r.E);
+        p.patience = 200;
+        p.max_attempts = 400;
+        Phase2Result r2 = phase2_swap_jitter(mc, p);
+        std::printf("p2_attempted %lld\np2_accepted %lld\np2_E %.17g\np2_best %.17g\n", r2.attempted, r2.accepted,
+                    r2.E, r2.best_E);
         pmeDestroy(&plan);
     } catch (const std::exception& e) {
         std::cerr << "Error: " << e.what() << "\n";  // MC/src/main.cu:224-227

@@ -62,3 +62,7 @@ def test_cpp_api_matches_python_path(gpu, tmp_path):
     assert kv["attempted"][0] == st["attempted"] == 500
     assert kv["accepted"][0] == st["accepted"] and kv["nprot"][0] == st["nprot"]
     assert abs(kv["E_after"][0] - st["E"]) <= 1e-9 * max(1.0, abs(st["E"]))
+    rep.mc.phase2_begin(patience=200)
+    s2 = rep.mc.phase2_run(400)
+    assert kv["p2_attempted"][0] == s2["attempted"] and kv["p2_accepted"][0] == s2["accepted"]
+    assert abs(kv["p2_best"][0] - s2["best_E"]) <= 1e-9 * max(1.0, abs(s2["best_E"]))
