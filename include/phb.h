@@ -222,11 +222,38 @@ int phb_mc_recompute(phb_mc mc, double *E);
 /* MT19937 state: 624 words then the index (625 unsigned). */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
 
+/* Phase 1 as main() runs it (MC/src/main.cu:180-197) on an MC object: build_plan's
+ * checks (MC/src/mc.cu:67-122; errors as PHB_ERR_STATE with the reference's
+ * message), then apply_direct_protonation (MC/src/mc.cu:124-157) when target_pct
+ * > direct_pct_threshold, else phase_gc_mu_control (MC/src/mc.cu:320-546) on the
+ * device chain: mu from the target fraction (clamped to +-10 kT), blocks of
+ * gc_block trials, an epoch ends between blocks after Nacc_target = ceil(nacc_frac
+ * * remaining) acceptances or epoch_attempt_max trials, then mu -= kT eta
+ * logit_diff w_info; the run stops at the trial whose acceptance reaches the
+ * target.  The pools handed to phase 2 are kept on the object (empty unless the
+ * target was reached mid-run or the direct path ran, as in the reference).
+ * max_trials bounds the run (0 = none; the reference's loop has no bound). */
+typedef struct {
+    double target_pct;         /* target_protonation_pct */
+    int direct_pct_threshold;  /* reference default 90 */
+    int gc_block;              /* gc_block_fixed (0 = 2000) */
+    int epoch_attempt_max;     /* sgmc_epoch_attempt_max (0 = 100000) */
+    double nacc_frac;          /* sgmc_nacc_frac_round (<= 0: 0.10) */
+    double mu_eta;             /* (<= 0: 0.3) */
+    long long max_trials;
+} phb_phase1_desc;
+typedef struct {
+    long long attempted, accepted, accepted_forward, accepted_backward;
+    int blocks_done, Nh, Nprot, target_count, reached_target, direct, status;
+    double E, mu_eff;
+} phb_phase1_result;
+int phb_phase1_run(phb_mc mc, const phb_phase1_desc *d, phb_phase1_result *out);
+
 /* Phase 2: the 4-point and W-Cl-only move rule of phase2_swap_jitter
  * (MC/src/mc.cu:159-315) on an MC object, continuing its engine stream.  begin:
- * the pools rebuilt in ascending atom order from the current identities, as the
- * phase-1 loop hands them over (MC/src/mc.cu:480-494; W and Cl below the pool z
- * cut), and the mode chosen as the reference does (MC/src/mc.cu:193-207): the
+ * the pools phb_phase1_run handed over, else rebuilt in ascending atom order from
+ * the current identities as the phase-1 loop hands them over when it reaches its
+ * target (MC/src/mc.cu:480-494); W and Cl below the pool z cut; the mode chosen as the reference does (MC/src/mc.cu:193-207): the
  * 4-point move unless NH_N is empty or no 4-point move exists, then W-Cl only,
  * else no valid proposals.  run: up to max_attempts more trials; a trial is
  * accepted if dE <= 0 or exp(clamp(-beta dE, +-40)) > u (u drawn only when

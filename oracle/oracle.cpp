@@ -687,15 +687,193 @@ void or_da_descriptors(const or_system* s, const or_desc_params* dp, int n, cons
     }
 }
 
+// Phase 1 as main() runs it (MC/src/main.cu:180-197): build_plan's checks
+// (MC/src/mc.cu:67-122), then apply_direct_protonation (MC/src/mc.cu:124-157)
+// when the target exceeds direct_pct_threshold, else phase_gc_mu_control
+// (MC/src/mc.cu:320-546): mu from the target fraction, blocks of gc_block
+// trials, an epoch ends between blocks once Nacc_target moves were accepted or
+// Amax trials made, then mu -= kT eta logit_diff w_info; the run stops at the
+// trial that reaches the target.  Pools handed to phase 2 (nh_n, nh_p, w, cl
+// back to back into handover[N], sizes into handover_sizes): the direct path's
+// lists, the identities in ascending atom order when the target is reached,
+// else empty (the reference fills them only then).  Returns 0, or -1 with *err
+// set when build_plan throws (1 no NH sites, 2 target below current, 3 not
+// enough NH_N, 4 not enough W); status 1 when no valid pair was left.
+int or_phase1_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,
+                  unsigned seed, double target_pct, int direct_pct_threshold, int gc_block, int epoch_attempt_max,
+                  double nacc_frac, double mu_eta, long long max_trials, or_phase1_result* res, double* q_out,
+                  int* type_out, unsigned* rng_state_out, int* handover, int* handover_sizes, int* err) {
+    const int N = s0->natoms;
+    std::vector<double> q(s0->q, s0->q + N);
+    std::vector<int> type(s0->type, s0->type + N);
+    or_system s = *s0;
+    s.q = q.data(); s.type = type.data();
+    *err = 0;
+    *res = or_phase1_result{};
+    std::vector<int> idx_nh_n, idx_nh_p, idx_w;
+    for (int i = 0; i < N; ++i) {
+        const int t = type[i] + 1;
+        if (t == m->nh_n_type) idx_nh_n.push_back(i);
+        else if (t == m->nh_p_type) idx_nh_p.push_back(i);
+        else if (t == m->w_type) idx_w.push_back(i);
+    }
+    const int nh_total = (int)(idx_nh_n.size() + idx_nh_p.size());
+    if (nh_total == 0) { *err = 1; return -1; }
+    const int target_count = (int)std::floor(target_pct / 100.0 * nh_total);
+    const int to_flip = target_count - (int)idx_nh_p.size();
+    if (to_flip < 0) { *err = 2; return -1; }
+    if ((int)idx_nh_n.size() < to_flip) { *err = 3; return -1; }
+    double z_max_site = 0.0;
+    for (int i : idx_nh_n) z_max_site = std::max(z_max_site, s.z[i]);
+    for (int i : idx_nh_p) z_max_site = std::max(z_max_site, s.z[i]);
+    const double z_cut = z_max_site + m->w_z_extra;
+    { std::vector<int> f; for (int j : idx_w) if (s.z[j] < z_cut) f.push_back(j); idx_w.swap(f); }
+    if ((int)idx_w.size() < to_flip) { *err = 4; return -1; }
+    std::mt19937 rng(seed);
+    std::vector<int> h_nn, h_np, h_w, h_cl;
+    res->target_count = target_count;
+    res->Nh = nh_total;
+    if (target_pct > (double)direct_pct_threshold) {
+        res->direct = 1;
+        h_nn = idx_nh_n; h_w = idx_w;
+        for (int k = 0; k < to_flip; ++k) {
+            const int i = h_nn[k], j = h_w[k];
+            type[i] = m->nh_p_type - 1; q[i] = 1.0;
+            type[j] = m->cl_type - 1; q[j] = -1.0;
+            h_np.push_back(i); h_cl.push_back(j);
+        }
+        h_nn.erase(h_nn.begin(), h_nn.begin() + std::max(to_flip, 0));
+        h_w.erase(h_w.begin(), h_w.begin() + std::max(to_flip, 0));
+        res->attempted = to_flip; res->accepted = to_flip;
+        res->Nprot = (int)idx_nh_p.size() + to_flip;
+        res->reached_target = 1;
+        res->E = or_total_energy(&s, p, head, list, nullptr);
+        res->mu_eff = m->mu_eff;
+    } else {
+        std::vector<int> nh_n = idx_nh_n, nh_p = idx_nh_p, w = idx_w, cl;
+        for (int j = 0; j < N; ++j) if (type[j] + 1 == m->cl_type && s.z[j] < z_cut) cl.push_back(j);
+        res->Nprot = (int)nh_p.size();
+        double E = or_total_energy(&s, p, head, list, nullptr);
+        const double beta = m->beta;
+        const double fstar = std::clamp(target_pct / 100.0, 1e-6, 1.0 - 1e-6);
+        double mu = (1.0 / beta) * std::log(fstar / (1.0 - fstar));
+        const double mu_bound = 10.0 / beta;
+        mu = std::clamp(mu, -mu_bound, mu_bound);
+        bool done_all = false;
+        if (res->Nprot == target_count) { res->reached_target = 1; done_all = true; }
+        const int blk = gc_block > 0 ? gc_block : 2000;
+        const int Amax = epoch_attempt_max > 0 ? epoch_attempt_max : 100000;
+        const int R0 = std::max(0, target_count - res->Nprot);
+        const double frac = nacc_frac > 0.0 ? nacc_frac : 0.10;
+        const int Nacc_target = std::max(1, (int)std::ceil(frac * (double)R0));
+        std::uniform_real_distribution<double> uni(0.0, 1.0), u01(0.0, 1.0);
+        auto choose_one = [&](const std::vector<int>& pool) {
+            std::uniform_int_distribution<int> d(0, (int)pool.size() - 1);
+            return pool[d(rng)];
+        };
+        for (int epoch = 0; !done_all; ++epoch) {
+            int acc_since = 0, att_since = 0;
+            while (!done_all && acc_since < Nacc_target && att_since < Amax) {
+                for (int t = 0; t < blk; ++t) {
+                    if (max_trials > 0 && res->attempted >= max_trials) { done_all = true; break; }
+                    const long long a = (long long)nh_n.size(), b = (long long)nh_p.size();
+                    const long long c = (long long)w.size(), d = (long long)cl.size();
+                    const long long w_fwd = a * c, w_bwd = b * d, S_before = a * c + b * d;
+                    bool forward;
+                    if (w_fwd > 0 && w_bwd > 0) forward = u01(rng) < (double)w_fwd / (double)(w_fwd + w_bwd);
+                    else if (w_fwd > 0) forward = true;
+                    else if (w_bwd > 0) forward = false;
+                    else { res->status = 1; done_all = true; break; }
+                    int i, j;
+                    if (forward) { i = choose_one(nh_n); j = choose_one(w); }
+                    else { i = choose_one(nh_p); j = choose_one(cl); }
+                    const int oti = type[i], otj = type[j];
+                    const double oqi = q[i], oqj = q[j];
+                    if (forward) { type[i] = m->nh_p_type - 1; q[i] = 1.0; type[j] = m->cl_type - 1; q[j] = -1.0; }
+                    else { type[i] = m->nh_n_type - 1; q[i] = 0.0; type[j] = m->w_type - 1; q[j] = 0.0; }
+                    const double E_trial = or_total_energy(&s, p, head, list, nullptr);
+                    const double dE = E_trial - E;
+                    const int sg = forward ? 1 : -1;
+                    const long long S_after = forward ? (a - 1) * (c - 1) + (b + 1) * (d + 1)
+                                                      : (a + 1) * (c + 1) + (b - 1) * (d - 1);
+                    double logH = 0.0;
+                    if (S_before > 0 && S_after > 0) logH = std::log((double)S_before) - std::log((double)S_after);
+                    double expo = (double)(-beta * (dE - sg * mu)) + logH;
+                    expo = std::clamp(expo, -40.0, 40.0);
+                    const bool acc = (expo >= 0.0) || (std::exp(expo) > uni(rng));
+                    ++res->attempted; ++att_since;
+                    if (acc) {
+                        ++res->accepted; ++acc_since;
+                        if (forward) ++res->accepted_forward; else ++res->accepted_backward;
+                        E = E_trial;
+                        if (forward) {
+                            nh_n.erase(std::find(nh_n.begin(), nh_n.end(), i)); nh_p.push_back(i);
+                            w.erase(std::find(w.begin(), w.end(), j)); cl.push_back(j);
+                            ++res->Nprot;
+                        } else {
+                            nh_p.erase(std::find(nh_p.begin(), nh_p.end(), i)); nh_n.push_back(i);
+                            cl.erase(std::find(cl.begin(), cl.end(), j)); w.push_back(j);
+                            --res->Nprot;
+                        }
+                        if (res->Nprot == target_count) {
+                            res->reached_target = 1;
+                            for (int k = 0; k < N; ++k) {
+                                const int tk = type[k] + 1;
+                                if (tk == m->nh_p_type) h_np.push_back(k);
+                                else if (tk == m->nh_n_type) h_nn.push_back(k);
+                                else if (tk == m->w_type) { if (s.z[k] < z_cut) h_w.push_back(k); }
+                                else if (tk == m->cl_type) { if (s.z[k] < z_cut) h_cl.push_back(k); }
+                            }
+                            done_all = true;
+                            break;
+                        }
+                    } else {
+                        type[i] = oti; q[i] = oqi; type[j] = otj; q[j] = oqj;
+                    }
+                }
+            }
+            if (done_all) break;
+            const double f = res->Nh > 0 ? (double)res->Nprot / (double)res->Nh : 0.0;
+            const double f0 = std::clamp(fstar, 1e-6, 1.0 - 1e-6), f1 = std::clamp(f, 1e-6, 1.0 - 1e-6);
+            const double logit_diff = std::log(f1 / (1.0 - f1)) - std::log(f0 / (1.0 - f0));
+            const double eta = mu_eta > 0.0 ? mu_eta : 0.3;
+            const double w_info = Nacc_target > 0
+                ? std::clamp((double)acc_since / (double)std::max(1, Nacc_target), 0.0, 1.0) : 1.0;
+            const double mu_step = (1.0 / beta) * eta * logit_diff * w_info;
+            mu -= mu_step;
+            res->blocks_done = epoch + 1;
+        }
+        res->E = E;
+        res->mu_eff = mu;
+    }
+    handover_sizes[0] = (int)h_nn.size(); handover_sizes[1] = (int)h_np.size();
+    handover_sizes[2] = (int)h_w.size(); handover_sizes[3] = (int)h_cl.size();
+    int* o = handover;
+    for (int v : h_nn) *o++ = v;
+    for (int v : h_np) *o++ = v;
+    for (int v : h_w) *o++ = v;
+    for (int v : h_cl) *o++ = v;
+    if (q_out) std::copy(q.begin(), q.end(), q_out);
+    if (type_out) std::copy(type.begin(), type.end(), type_out);
+    if (rng_state_out) {
+        std::ostringstream os;
+        os << rng;
+        std::istringstream is(os.str());
+        for (int k = 0; k < 625; ++k) is >> rng_state_out[k];
+    }
+    return 0;
+}
+
 // phase2_swap_jitter (MC/src/mc.cu:159-315) after the phase-1 loop reached its
 // target: pools rebuilt in ascending atom order from the identities
 // (MC/src/mc.cu:480-494), W and Cl below z_cut (MC/src/mc.cu:177-184); the
 // engine resumes from a saved state (the 624 words and the index, libstdc++'s
 // stream form).  Every trial is recomputed in full, as the reference does.
 int or_phase2_run(const or_system* s0, const or_params* p, const or_mc_params* m, const int* head, const int* list,
-                  const unsigned* rng_state, long long max_attempts, int patience, double best_eps, int* rec_atoms,
-                  signed char* rec_acc, double* rec_dE, double* q_out, int* type_out, double* E_out,
-                  double* best_out, long long* attempted, long long* accepted, int* mode, int* stop) {
+                  const unsigned* rng_state, const int* pools, const int* pool_sizes, long long max_attempts,
+                  int patience, double best_eps, int* rec_atoms, signed char* rec_acc, double* rec_dE,
+                  double* q_out, int* type_out, double* E_out, double* best_out, long long* attempted,
+                  long long* accepted, int* mode, int* stop) {
     const int N = s0->natoms;
     std::vector<double> q(s0->q, s0->q + N);
     std::vector<int> type(s0->type, s0->type + N);
@@ -706,12 +884,21 @@ int or_phase2_run(const or_system* s0, const or_params* p, const or_mc_params* m
         if (type[i] + 1 == m->nh_n_type || type[i] + 1 == m->nh_p_type) z_max_site = std::max(z_max_site, s.z[i]);
     const double z_cut = z_max_site + m->w_z_extra;
     std::vector<int> nh_p, nh_n, w_cand, cl_pool;
-    for (int i = 0; i < N; ++i) {
-        const int t = type[i] + 1;
-        if (t == m->nh_p_type) nh_p.push_back(i);
-        else if (t == m->nh_n_type) nh_n.push_back(i);
-        else if (t == m->w_type) { if (s.z[i] < z_cut) w_cand.push_back(i); }
-        else if (t == m->cl_type) { if (s.z[i] < z_cut) cl_pool.push_back(i); }
+    if (pools) {   // the phase-1 hand-over: nh_n, nh_p, w, cl back to back; W and Cl z-filtered
+        const int* q0 = pools;
+        nh_n.assign(q0, q0 + pool_sizes[0]); q0 += pool_sizes[0];
+        nh_p.assign(q0, q0 + pool_sizes[1]); q0 += pool_sizes[1];
+        for (int k = 0; k < pool_sizes[2]; ++k) if (s.z[q0[k]] < z_cut) w_cand.push_back(q0[k]);
+        q0 += pool_sizes[2];
+        for (int k = 0; k < pool_sizes[3]; ++k) if (s.z[q0[k]] < z_cut) cl_pool.push_back(q0[k]);
+    } else {
+        for (int i = 0; i < N; ++i) {
+            const int t = type[i] + 1;
+            if (t == m->nh_p_type) nh_p.push_back(i);
+            else if (t == m->nh_n_type) nh_n.push_back(i);
+            else if (t == m->w_type) { if (s.z[i] < z_cut) w_cand.push_back(i); }
+            else if (t == m->cl_type) { if (s.z[i] < z_cut) cl_pool.push_back(i); }
+        }
     }
     std::mt19937 rng;
     {

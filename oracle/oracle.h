@@ -99,15 +99,30 @@ int or_gc_run(const or_system *s, const or_params *p, const or_mc_params *m,
               long long *n_done);
 
 /* ---- phase 2: 4-point / W-Cl-only moves with patience (MC/src/mc.cu:159-315) ----
- * Pools from the identities in ascending atom order (the phase-1 hand-over,
- * MC/src/mc.cu:480-494); the engine resumes from rng_state (624 words + index).
+ * Pools: the phase-1 hand-over when given (nh_n, nh_p, w, cl back to back), else the
+ * identities in ascending atom order (MC/src/mc.cu:480-494); W and Cl below the
+ * pool z cut.  The engine resumes from rng_state (624 words + index).
  * Records: 4 atoms per trial (draw order; -1 unused), accepted flag, exact dE.
  * mode: 4, 2 or 0 (no valid proposals); stop: 0 max_attempts, 1 patience, 2 none. */
 int or_phase2_run(const or_system *s, const or_params *p, const or_mc_params *m, const int *head,
-                  const int *list, const unsigned *rng_state, long long max_attempts, int patience,
-                  double best_eps, int *rec_atoms, signed char *rec_acc, double *rec_dE, double *q_out,
-                  int *type_out, double *E_out, double *best_out, long long *attempted,
-                  long long *accepted, int *mode, int *stop);
+                  const int *list, const unsigned *rng_state, const int *pools /* NULL: rebuild */,
+                  const int *pool_sizes, long long max_attempts, int patience, double best_eps,
+                  int *rec_atoms, signed char *rec_acc, double *rec_dE, double *q_out, int *type_out,
+                  double *E_out, double *best_out, long long *attempted, long long *accepted, int *mode,
+                  int *stop);
+
+/* ---- phase 1 as main() runs it: build_plan checks, then direct flips above the
+ * threshold or the mu-control GC loop (MC/src/mc.cu:67-157, :320-546). ---- */
+typedef struct {
+    long long attempted, accepted, accepted_forward, accepted_backward;
+    int blocks_done, Nh, Nprot, target_count, reached_target, direct, status;
+    double E, mu_eff;
+} or_phase1_result;
+int or_phase1_run(const or_system *s, const or_params *p, const or_mc_params *m, const int *head,
+                  const int *list, unsigned seed, double target_pct, int direct_pct_threshold, int gc_block,
+                  int epoch_attempt_max, double nacc_frac, double mu_eta, long long max_trials,
+                  or_phase1_result *res, double *q_out, int *type_out, unsigned *rng_state_out /* 625 */,
+                  int *handover /* natoms */, int *handover_sizes /* 4 */, int *err);
 
 /* ---- mt19937 + libstdc++ distributions, for known-answer tests ---- */
 void or_mt_draws(unsigned seed, long long n, unsigned *out);

@@ -46,6 +46,13 @@ class OrDescParams(C.Structure):
     _fields_ = [("species_of_type", _pb), ("ntypes", C.c_int), ("r_desc", C.c_double)]
 
 
+class OrPhase1Result(C.Structure):
+    _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("accepted_forward", C.c_longlong),
+                ("accepted_backward", C.c_longlong), ("blocks_done", C.c_int), ("Nh", C.c_int), ("Nprot", C.c_int),
+                ("target_count", C.c_int), ("reached_target", C.c_int), ("direct", C.c_int), ("status", C.c_int),
+                ("E", C.c_double), ("mu_eff", C.c_double)]
+
+
 class OrDaRecord(C.Structure):
     _fields_ = [("fwd", _pb), ("i", _pi), ("j", _pi), ("stage", _pb), ("dE_star", _pd), ("dE_star_rev", _pd),
                 ("dE", _pd)]
@@ -96,8 +103,13 @@ def load():
                                   _pd, C.POINTER(C.c_longlong)]
         lib.or_phase2_run.restype = C.c_int
         lib.or_phase2_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
-                                      _pu, C.c_longlong, C.c_int, C.c_double, _pi, _pb, _pd, _pd, _pi, _pd, _pd,
-                                      C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                      _pu, _pi, _pi, C.c_longlong, C.c_int, C.c_double, _pi, _pb, _pd, _pd, _pi,
+                                      _pd, _pd, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.or_phase1_run.restype = C.c_int
+        lib.or_phase1_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
+                                      C.c_uint, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_longlong, C.POINTER(OrPhase1Result), _pd, _pi, _pu, _pi, _pi,
                                       C.POINTER(C.c_int)]
         lib.or_desc_thresholds.argtypes = [C.c_double, _pd]
         lib.or_da_descriptors.argtypes = [C.POINTER(OrSystem), C.POINTER(OrDescParams), C.c_int, _pi, _pi, _pi, _pd]
@@ -206,9 +218,31 @@ class Oracle:
         n = nd.value
         return {"status": st, "n": n, "q": q, "type0": t, "E": E.value, **{k: v[:n] for k, v in rec.items()}}
 
-    def phase2_run(self, mc, rng_state, max_attempts, patience, best_eps=1e-9):
+    def phase1_run(self, mc, seed, target_pct, direct_pct_threshold=90, gc_block=2000, epoch_attempt_max=100000,
+                   nacc_frac=0.10, mu_eta=0.3, max_trials=0):
+        """Phase 1 as main() runs it (build_plan checks, direct flips or the
+        mu-control GC loop).  Returns the result fields, the identities, the
+        engine state after it and the pools it hands to phase 2."""
+        N = self.s.natoms
+        m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
+        res = OrPhase1Result()
+        q, t = np.zeros(N), np.zeros(N, np.int32)
+        rng = np.zeros(625, np.uint32)
+        ho, hs = np.zeros(max(N, 1), np.int32), np.zeros(4, np.int32)
+        err = C.c_int()
+        rc = self.lib.or_phase1_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), seed, target_pct,
+                                    direct_pct_threshold, gc_block, epoch_attempt_max, nacc_frac, mu_eta, max_trials,
+                                    C.byref(res), q.ctypes.data_as(_pd), t.ctypes.data_as(_pi),
+                                    rng.ctypes.data_as(_pu), ho.ctypes.data_as(_pi), hs.ctypes.data_as(_pi),
+                                    C.byref(err))
+        out = {k: getattr(res, k) for k, _ in OrPhase1Result._fields_}
+        out.update(rc=rc, err=err.value, q=q, type0=t, rng_state=rng, handover=ho[:hs.sum()], handover_sizes=hs)
+        return out
+
+    def phase2_run(self, mc, rng_state, max_attempts, patience, best_eps=1e-9, pools=None, pool_sizes=None):
         """phase2_swap_jitter restated (MC/src/mc.cu:159-315) from the current
-        identities, the engine resumed from rng_state (625 words)."""
+        identities (pools: the phase-1 hand-over, else rebuilt in ascending atom
+        order), the engine resumed from rng_state (625 words)."""
         N = self.s.natoms
         st = np.ascontiguousarray(rng_state, dtype=np.uint32)
         assert st.shape == (625,)
@@ -220,8 +254,11 @@ class Oracle:
         E, best = C.c_double(), C.c_double()
         att, nacc = C.c_longlong(), C.c_longlong()
         mode, stop = C.c_int(), C.c_int()
+        pl = np.ascontiguousarray(pools, np.int32) if pools is not None else None
+        ps = np.ascontiguousarray(pool_sizes, np.int32) if pool_sizes is not None else None
         self.lib.or_phase2_run(C.byref(self.sys), C.byref(self.par), C.byref(m), *self._hl(), st.ctypes.data_as(_pu),
-                               max_attempts, patience, best_eps, atoms.ctypes.data_as(_pi), acc.ctypes.data_as(_pb),
+                               pl.ctypes.data_as(_pi) if pl is not None else None,
+                               ps.ctypes.data_as(_pi) if ps is not None else None, max_attempts, patience, best_eps, atoms.ctypes.data_as(_pi), acc.ctypes.data_as(_pb),
                                dE.ctypes.data_as(_pd), q.ctypes.data_as(_pd), t.ctypes.data_as(_pi), C.byref(E),
                                C.byref(best), C.byref(att), C.byref(nacc), C.byref(mode), C.byref(stop))
         n = att.value

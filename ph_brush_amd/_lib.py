@@ -79,6 +79,22 @@ class McStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Phase1Desc(C.Structure):
+    _fields_ = [("target_pct", C.c_double), ("direct_pct_threshold", C.c_int), ("gc_block", C.c_int),
+                ("epoch_attempt_max", C.c_int), ("nacc_frac", C.c_double), ("mu_eta", C.c_double),
+                ("max_trials", C.c_longlong)]
+
+
+class Phase1Result(C.Structure):
+    _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("accepted_forward", C.c_longlong),
+                ("accepted_backward", C.c_longlong), ("blocks_done", C.c_int), ("Nh", C.c_int), ("Nprot", C.c_int),
+                ("target_count", C.c_int), ("reached_target", C.c_int), ("direct", C.c_int), ("status", C.c_int),
+                ("E", C.c_double), ("mu_eff", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Phase2Desc(C.Structure):
     _fields_ = [("patience", C.c_int), ("best_eps", C.c_double), ("record_cap", C.c_longlong)]
 
@@ -158,6 +174,7 @@ SIGNATURES = {
     "phb_csr_download": [_h, _pi, _pi],
     "phb_nlist_build_half": [_h, C.POINTER(Box), _h, C.c_double, C.POINTER(_h)],
     "phb_atoms_set_positions": [_h, _pd, _pd, _pd],
+    "phb_phase1_run": [_h, C.POINTER(Phase1Desc), C.POINTER(Phase1Result)],
     "phb_phase2_begin": [_h, C.POINTER(Phase2Desc), C.POINTER(Phase2Stats)],
     "phb_phase2_run": [_h, C.c_longlong, C.POINTER(Phase2Stats)],
     "phb_phase2_record": [_h, C.c_longlong, _pi, C.POINTER(C.c_byte), _pd],

@@ -553,6 +553,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
             for (int k = 0; k < 624; ++k) ss >> c.mt[k];
             ss >> c.mti;
         }
+        c.nprot_target = -1;
         M->ctl.alloc(1);
         M->ctl.upload(&c, 1);
         {   // one entry per workgroup of either refresh (direct tiles, or k-space planes)
@@ -594,46 +595,54 @@ extern "C" int phb_mc_destroy(phb_mc M) {
     return guard([&] { delete M; return PHB_OK; });
 }
 
+namespace phb {
+
+void run_plain(phb_mc_s* M, long long ntrials) {
+    McCtl c = read_ctl(M);
+    const long long target = c.trials_done + ntrials;
+    long long done = c.trials_done;
+    int status = c.status;
+    while (done < target && status == 0) {
+        long long chunk_end = target;
+        if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
+        hipLaunchKernelGGL(k_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
+        // each launch runs until the chunk ends or pending_max changes accumulate;
+        // batch size from the trials one launch has been covering
+        while (done < chunk_end && status == 0) {
+            const int batch = launch_batch(c, chunk_end - done, M->d.pending_max);
+            const long long before = done;
+            for (int b = 0; b < batch; ++b) {
+                const size_t e0 = M->ev_used;
+                if (M->profiling) M->event();
+                launch_chain(M);
+                ++M->launches;
+                if (M->profiling) M->event();
+                refresh(M);
+                if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
+            }
+            PHB_HIP(hipGetLastError());
+            c = read_ctl(M);
+            require(c.trials_done > done || c.status != 0 || c.trials_done >= chunk_end,
+                    "phb_mc_run: the chain made no progress", PHB_ERR_INTERNAL);
+            require(c.status != 2, "phb_mc_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
+            done = c.trials_done;
+            status = c.status;
+            M->trials_seen += done - before;
+        }
+        if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
+            full_recompute(M);
+            M->next_recompute += M->d.recompute_interval;
+        }
+    }
+    sync();
+}
+
+}  // namespace phb
+
 extern "C" int phb_mc_run(phb_mc M, long long ntrials, phb_mc_stats* out) {
     return guard([&] {
         require(M != nullptr && ntrials >= 0, "phb_mc_run: bad arguments");
-        McCtl c = read_ctl(M);
-        const long long target = c.trials_done + ntrials;
-        long long done = c.trials_done;
-        int status = c.status;
-        while (done < target && status == 0) {
-            long long chunk_end = target;
-            if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
-            hipLaunchKernelGGL(k_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
-            // each launch runs until the chunk ends or pending_max changes accumulate;
-            // batch size from the trials one launch has been covering
-            while (done < chunk_end && status == 0) {
-                const int batch = launch_batch(c, chunk_end - done, M->d.pending_max);
-                const long long before = done;
-                for (int b = 0; b < batch; ++b) {
-                    const size_t e0 = M->ev_used;
-                    if (M->profiling) M->event();
-                    launch_chain(M);
-                    ++M->launches;
-                    if (M->profiling) M->event();
-                    refresh(M);
-                    if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
-                }
-                PHB_HIP(hipGetLastError());
-                c = read_ctl(M);
-                require(c.trials_done > done || c.status != 0 || c.trials_done >= chunk_end,
-                        "phb_mc_run: the chain made no progress", PHB_ERR_INTERNAL);
-                require(c.status != 2, "phb_mc_run: engine stream rolled back past its kept block", PHB_ERR_INTERNAL);
-                done = c.trials_done;
-                status = c.status;
-                M->trials_seen += done - before;
-            }
-            if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
-                full_recompute(M);
-                M->next_recompute += M->d.recompute_interval;
-            }
-        }
-        sync();
+        run_plain(M, ntrials);
         if (out) fill_stats(M, read_ctl(M), out);
         return PHB_OK;
     });

@@ -35,6 +35,7 @@ struct McCtl {
     long long da_stage1, da_rounds;   // delayed-acceptance chain: stage-1 acceptances, rounds
     long long da_prof[8];             // delayed-acceptance phase times (100 MHz ticks) when profiling
     // phase-2 (4-point / W-Cl) chain: trials done and target, accepted, patience counter, stop flag, best E
+    int nprot_target;                 // >= 0: the chain stops (status 4) once nprot equals it
     long long p2_done, p2_target, p2_acc;
     int p2_since, p2_stop;
     double p2_best;

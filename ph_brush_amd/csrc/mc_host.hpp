@@ -40,6 +40,9 @@ struct phb_mc_s {
     std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)
     double chain_ms = 0, refresh_ms = 0, recompute_ms = 0;
     double z_cut = 0.0;        // pool z filter of W and Cl (max site z + w_z_extra)
+    // pools phase 1 hands to phase 2 (nh_n, nh_p, w, cl), when it ran
+    std::vector<int> handover[4];
+    bool has_handover = false;
     // delayed-acceptance state (descriptor lists and histograms) and phase-2 state, owned here
     void* da = nullptr;
     void (*da_free)(void*) = nullptr;
@@ -87,6 +90,9 @@ long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBu
 McCtl read_ctl(phb_mc_s* M);
 // Full recompute: phi = G Q from scratch, total energy, moments, E reset.
 void full_recompute(phb_mc_s* M);
+// ntrials more trials of the plain chain (refreshes, drift resets); ends early
+// at a non-zero status (1 no valid pair, 4 nprot reached its target)
+void run_plain(phb_mc_s* M, long long ntrials);
 // Folds the pending accepted charge changes into phi (k-space refresh).
 void refresh(phb_mc_s* M);
 bool refresh_direct_allowed(const phb_mc_s* M);   // grid fits the direct refresh

@@ -22,7 +22,7 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
     const long long target = C->trials_target, rec_cap = C->rec_cap;
     // a launch queued beyond the target (batches are sized ahead) leaves at once
     if (C->trials_done >= target || C->status != 0) return;
-    const int pend_max = C->pending_max;
+    const int pend_max = C->pending_max, ntg = C->nprot_target;
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     if (LDS_POOLS)
         for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
@@ -51,6 +51,7 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
             const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
             int ntr = 0, stop = 0;
             if (done >= target || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            else if (ntg >= 0 && s_st.nprot == ntg) { status = 4; stop = 1; }
             else {
                 const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
                 if (wf == 0 && wb == 0) { status = 1; stop = 1; }

@@ -485,8 +485,6 @@ extern "C" int phb_phase2_begin(phb_mc M, const phb_phase2_desc* d, phb_phase2_s
         }
         P2State* S = static_cast<P2State*>(M->p2);
         refresh(M);   // phi must hold every accepted change
-        // pools in ascending atom order from the current identities (MC/src/mc.cu:480-494,
-        // then the z filter of MC/src/mc.cu:177-184)
         const int N = M->atoms->n;
         std::vector<int> ht(N);
         std::vector<double> hz(N);
@@ -494,15 +492,26 @@ extern "C" int phb_phase2_begin(phb_mc M, const phb_phase2_desc* d, phb_phase2_s
         M->atoms->z.download(hz.data(), N);
         sync();
         std::vector<int> nh_n, nh_p, w, cl;
-        for (int i = 0; i < N; ++i) {
-            const int t = ht[i] + 1;
-            if (t == M->d.nh_p_type) nh_p.push_back(i);
-            else if (t == M->d.nh_n_type) nh_n.push_back(i);
-            else if (t == M->d.w_type) { if (hz[i] < M->z_cut) w.push_back(i); }
-            else if (t == M->d.cl_type) { if (hz[i] < M->z_cut) cl.push_back(i); }
+        if (M->has_handover) {
+            // the pools phase 1 handed over (W and Cl below the pool z cut, MC/src/mc.cu:177-184)
+            nh_n = M->handover[0];
+            nh_p = M->handover[1];
+            for (int j : M->handover[2]) if (hz[j] < M->z_cut) w.push_back(j);
+            for (int j : M->handover[3]) if (hz[j] < M->z_cut) cl.push_back(j);
+            require((int)(nh_n.size() + nh_p.size()) <= M->nsites && (int)(w.size() + cl.size()) <= M->nions,
+                    "phb_phase2_begin: hand-over pools larger than the chain's pools", PHB_ERR_STATE);
+        } else {
+            // pools in ascending atom order from the current identities (MC/src/mc.cu:480-494)
+            for (int i = 0; i < N; ++i) {
+                const int t = ht[i] + 1;
+                if (t == M->d.nh_p_type) nh_p.push_back(i);
+                else if (t == M->d.nh_n_type) nh_n.push_back(i);
+                else if (t == M->d.w_type) { if (hz[i] < M->z_cut) w.push_back(i); }
+                else if (t == M->d.cl_type) { if (hz[i] < M->z_cut) cl.push_back(i); }
+            }
+            require((int)(nh_n.size() + nh_p.size()) == M->nsites && (int)(w.size() + cl.size()) == M->nions,
+                    "phb_phase2_begin: pool membership differs from the chain's pools", PHB_ERR_STATE);
         }
-        require((int)(nh_n.size() + nh_p.size()) == M->nsites && (int)(w.size() + cl.size()) == M->nions,
-                "phb_phase2_begin: pool membership differs from the chain's pools", PHB_ERR_STATE);
         const int ns = M->nsites, ni = M->nions;
         std::vector<int> h(std::max(ns + ni, 1), -1);
         for (size_t k = 0; k < nh_n.size(); ++k) h[k] = nh_n[k];

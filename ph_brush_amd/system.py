@@ -443,6 +443,15 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_rng_state", self.h, st)
         return np.array(st[:], dtype=np.uint32)
 
+    # ---- phase 1 as main() runs it (MC/src/main.cu:180-197)
+    def phase1_run(self, target_pct: float, direct_pct_threshold: int = 90, gc_block: int = 2000,
+                   epoch_attempt_max: int = 100000, nacc_frac: float = 0.10, mu_eta: float = 0.3,
+                   max_trials: int = 0) -> dict:
+        d = L.Phase1Desc(target_pct, direct_pct_threshold, gc_block, epoch_attempt_max, nacc_frac, mu_eta, max_trials)
+        r = L.Phase1Result()
+        L.call("phb_phase1_run", self.h, C.byref(d), C.byref(r))
+        return r.as_dict()
+
     # ---- phase 2 (phase2_swap_jitter's move rule, MC/src/mc.cu:159-315)
     def phase2_begin(self, patience: int, best_eps: float = 1e-9, record_cap: int = 0) -> dict:
         d = L.Phase2Desc(patience, best_eps, record_cap)
