@@ -60,6 +60,12 @@ struct Params {
     real mu_eff{0.0};
     int patience{3000};            // phase 2 (MC/include/types.hpp:58-59)
     int max_attempts{1000000};
+    real target_protonation_pct{10.0};   // phase 1 (MC/include/types.hpp:55-67)
+    int direct_pct_threshold{90};
+    real mu_eta{0.3};
+    int gc_block_fixed{2000};
+    int sgmc_epoch_attempt_max{100000};
+    real sgmc_nacc_frac_round{0.10};
 };
 
 // build_bond_adjacency (MC/src/io.cpp:148-169)
@@ -233,12 +239,23 @@ inline real total_energy(const DeviceAtoms& a, const DeviceParams& p, const Devi
 }
 
 // ------------------------------------------------------------ Monte Carlo
-// PhaseGCResult-like summary of a fixed-mu run (MC/include/mc.hpp:32-44).
+// PhaseGCResult (MC/include/mc.hpp:36-48); also the summary of a fixed-mu run.
 struct PhaseGCResult {
     long long attempted{0}, accepted{0}, accepted_forward{0}, accepted_backward{0};
+    int blocks_done{0};
     int Nh{0}, Nprot{0};
+    int target_count{0};
+    bool reached_target{false};
     real E{0.0};
     real mu_eff{0.0};
+};
+
+// Phase1Result (MC/include/mc.hpp:19-27): the direct flips and the pools phase 1
+// hands to phase 2.
+struct Phase1Result {
+    long long attempted{0}, accepted{0};
+    real E{0.0};
+    std::vector<int> nh_n_remaining, w_remaining, nh_p_mc, cl_mc;
 };
 
 // Phase2Result (MC/include/mc.hpp:29-34).
@@ -275,6 +292,43 @@ public:
         r.Nh = s.nsites; r.Nprot = s.nprot; r.E = s.E; r.mu_eff = mu_;
         return r;
     }
+    // Phase 1 as main() runs it (MC/src/main.cu:180-197): build_plan's checks,
+    // then apply_direct_protonation when target_protonation_pct exceeds
+    // direct_pct_threshold, else phase_gc_mu_control on this chain (mu from the
+    // target, epochs of gc_block_fixed trials, stop at the target).  max_trials
+    // bounds the run (0: none, as in the reference).  direct_pct_threshold < 0
+    // forces the direct path, > 100 the mu-control loop.
+    PhaseGCResult phase1(const Params& hp, long long max_trials = 0, bool* direct = nullptr) {
+        phb_phase1_desc d{};
+        d.target_pct = hp.target_protonation_pct; d.direct_pct_threshold = hp.direct_pct_threshold;
+        d.gc_block = hp.gc_block_fixed; d.epoch_attempt_max = hp.sgmc_epoch_attempt_max;
+        d.nacc_frac = hp.sgmc_nacc_frac_round; d.mu_eta = hp.mu_eta; d.max_trials = max_trials;
+        phb_phase1_result s{};
+        check(phb_phase1_run(h_, &d, &s));
+        if (s.status == 1) throw std::runtime_error("[sgmc] no valid pairs to flip; stop.");  // MC/src/mc.cu:399
+        PhaseGCResult r;
+        r.attempted = s.attempted; r.accepted = s.accepted;
+        r.accepted_forward = s.accepted_forward; r.accepted_backward = s.accepted_backward;
+        r.blocks_done = s.blocks_done; r.Nh = s.Nh; r.Nprot = s.Nprot; r.target_count = s.target_count;
+        r.reached_target = s.reached_target != 0; r.E = s.E; r.mu_eff = s.mu_eff;
+        mu_ = s.mu_eff;
+        if (direct) *direct = s.direct != 0;
+        return r;
+    }
+    // The pools the last phase-1 run handed over (empty when it did not reach its target).
+    Phase1Result handover() const {
+        int sz[4] = {0, 0, 0, 0};
+        check(phb_phase1_handover(h_, sz, nullptr, 0));
+        std::vector<int> all((size_t)sz[0] + sz[1] + sz[2] + sz[3]);
+        check(phb_phase1_handover(h_, sz, all.data(), (long long)all.size()));
+        Phase1Result r;
+        auto it = all.begin();
+        r.nh_n_remaining.assign(it, it + sz[0]); it += sz[0];
+        r.nh_p_mc.assign(it, it + sz[1]); it += sz[1];
+        r.w_remaining.assign(it, it + sz[2]); it += sz[2];
+        r.cl_mc.assign(it, it + sz[3]);
+        return r;
+    }
     // phase2_swap_jitter's loop (MC/src/mc.cu:159-315) continuing this chain's
     // engine: pools from the current identities, patience and max_attempts from hp.
     Phase2Result phase2(const Params& hp) {
@@ -295,6 +349,28 @@ inline PhaseGCResult phase_gc_fixed_mu(const DeviceAtoms& a, const DeviceParams&
                                        PMEPlan& plan, const Box& box, const Params& hp, long long ntrials) {
     MonteCarlo mc(a, p, neigh, plan, box, hp);
     return mc.run(ntrials);
+}
+
+// apply_direct_protonation (MC/src/mc.cu:124-157): the first to_flip NH_N and W
+// of the plan flipped at once, whatever the threshold.
+inline Phase1Result apply_direct_protonation(MonteCarlo& mc, const Params& hp) {
+    Params q = hp;
+    q.direct_pct_threshold = -1;
+    PhaseGCResult g = mc.phase1(q);
+    Phase1Result r = mc.handover();
+    r.attempted = g.attempted; r.accepted = g.accepted; r.E = g.E;
+    return r;
+}
+
+// phase_gc_mu_control (MC/src/mc.cu:320-546) on the device chain, whatever the
+// threshold; the hand-over for phase 2 stays on mc (and in *seed_out if given).
+inline PhaseGCResult phase_gc_mu_control(MonteCarlo& mc, const Params& hp, Phase1Result* seed_out = nullptr,
+                                         long long max_trials = 0) {
+    Params q = hp;
+    q.direct_pct_threshold = 1000;
+    PhaseGCResult g = mc.phase1(q, max_trials);
+    if (seed_out) *seed_out = mc.handover();
+    return g;
 }
 
 inline Phase2Result phase2_swap_jitter(MonteCarlo& mc, const Params& hp) { return mc.phase2(hp); }

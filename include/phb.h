@@ -248,6 +248,11 @@ typedef struct {
     double E, mu_eff;
 } phb_phase1_result;
 int phb_phase1_run(phb_mc mc, const phb_phase1_desc *d, phb_phase1_result *out);
+/* The lists phase 1 handed to phase 2 (Phase1Result's nh_n_remaining, nh_p_mc,
+ * w_remaining, cl_mc, MC/include/mc.hpp:19-27), in that order: sizes[4] always;
+ * lists (capacity cap ints) when non-null, concatenated.  All empty before a
+ * phase-1 run or when it ended without reaching its target. */
+int phb_phase1_handover(phb_mc mc, int sizes[4], int *lists, long long cap);
 
 /* Phase 2: the 4-point and W-Cl-only move rule of phase2_swap_jitter
  * (MC/src/mc.cu:159-315) on an MC object, continuing its engine stream.  begin:

@@ -175,6 +175,7 @@ SIGNATURES = {
     "phb_nlist_build_half": [_h, C.POINTER(Box), _h, C.c_double, C.POINTER(_h)],
     "phb_atoms_set_positions": [_h, _pd, _pd, _pd],
     "phb_phase1_run": [_h, C.POINTER(Phase1Desc), C.POINTER(Phase1Result)],
+    "phb_phase1_handover": [_h, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_longlong],
     "phb_phase2_begin": [_h, C.POINTER(Phase2Desc), C.POINTER(Phase2Stats)],
     "phb_phase2_run": [_h, C.c_longlong, C.POINTER(Phase2Stats)],
     "phb_phase2_record": [_h, C.c_longlong, _pi, C.POINTER(C.c_byte), _pd],

@@ -229,3 +229,20 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
         return PHB_OK;
     });
 }
+
+extern "C" int phb_phase1_handover(phb_mc M, int sizes[4], int* lists, long long cap) {
+    return guard([&] {
+        require(M && sizes, "phb_phase1_handover: null argument");
+        long long total = 0;
+        for (int k = 0; k < 4; ++k) {
+            sizes[k] = M->has_handover ? (int)M->handover[k].size() : 0;
+            total += sizes[k];
+        }
+        if (!lists) return PHB_OK;
+        require(cap >= total, "phb_phase1_handover: lists capacity below the hand-over size");
+        long long o = 0;
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < sizes[k]; ++j) lists[o++] = M->handover[k][j];
+        return PHB_OK;
+    });
+}

@@ -452,6 +452,15 @@ class MonteCarlo(_Handle):
         L.call("phb_phase1_run", self.h, C.byref(d), C.byref(r))
         return r.as_dict()
 
+    def phase1_handover(self) -> dict:
+        """The pools phase 1 handed to phase 2 (MC/include/mc.hpp:19-27)."""
+        sz = np.zeros(4, np.int32)
+        L.call("phb_phase1_handover", self.h, L.iptr(sz), None, 0)
+        lists = np.zeros(max(int(sz.sum()), 1), np.int32)
+        L.call("phb_phase1_handover", self.h, L.iptr(sz), L.iptr(lists), lists.size)
+        parts = np.split(lists[:int(sz.sum())], np.cumsum(sz)[:3])
+        return dict(zip(("nh_n_remaining", "nh_p_mc", "w_remaining", "cl_mc"), parts))
+
     # ---- phase 2 (phase2_swap_jitter's move rule, MC/src/mc.cu:159-315)
     def phase2_begin(self, patience: int, best_eps: float = 1e-9, record_cap: int = 0) -> dict:
         d = L.Phase2Desc(patience, best_eps, record_cap)

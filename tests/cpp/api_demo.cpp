@@ -1,12 +1,14 @@
 // Exercises the reference-style C++ API (include/ph_brush/ph_brush.hpp) the
 // way main() drives the reference (MC/src/main.cu:102-206): read a system,
-// upload, build the neighbour list, create the PME plan, report energies and
-// run a fixed-mu chain.  Input: a text file written by the test
+// upload, build the neighbour list, create the PME plan, report energies, run a
+// fixed-mu chain (or, with a target percentage, phase 1's mu-control loop with
+// at most ntrials trials) and then phase 2.  Input: a text file written by the test
 //   natoms nbonds Lx Ly Lz
 //   x y z q type1          (natoms lines)
 //   i j                    (nbonds lines, 0-based)
 // Output: "key value" lines.
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 
@@ -15,7 +17,7 @@
 using namespace ph_brush;
 
 int main(int argc, char** argv) {
-    if (argc < 3) { std::cerr << "usage: api_demo system.txt ntrials\n"; return 1; }
+    if (argc < 3) { std::cerr << "usage: api_demo system.txt ntrials [target_pct]\n"; return 1; }
     try {
         std::ifstream in(argv[1]);
         if (!in) throw std::runtime_error(std::string("cannot open ") + argv[1]);
@@ -53,7 +55,19 @@ int main(int argc, char** argv) {
         std::printf("E_lj %.17g\nE_coul %.17g\nE_real %.17g\nE_recip %.17g\nE_self %.17g\nE_qcorr %.17g\n", E_lj,
                     E_coul, comps.real_space, comps.recip_space, comps.self_term, comps.qcorr_term);
         MonteCarlo mc(d_atoms, d_params, d_neigh, plan, sys.box, p);
-        PhaseGCResult r = mc.run(std::atoll(argv[2]));
+        PhaseGCResult r;
+        if (argc > 3) {
+            p.target_protonation_pct = std::atof(argv[3]);
+            p.gc_block_fixed = 200;
+            p.sgmc_epoch_attempt_max = 1000;
+            Phase1Result ho;
+            r = phase_gc_mu_control(mc, p, &ho, std::atoll(argv[2]));
+            std::printf("blocks %d\nreached %d\nmu %.17g\nhandover %zu %zu %zu %zu\n", r.blocks_done,
+                        (int)r.reached_target, r.mu_eff, ho.nh_n_remaining.size(), ho.nh_p_mc.size(),
+                        ho.w_remaining.size(), ho.cl_mc.size());
+        } else {
+            r = mc.run(std::atoll(argv[2]));
+        }
         std::printf("attempted %lld\naccepted %lld\nnprot %d\nE_after %.17g\n", r.attempted, r.accepted, r.Nprot, r.E);
         p.patience = 200;
         p.max_attempts = 400;

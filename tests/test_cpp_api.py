@@ -66,3 +66,32 @@ def test_cpp_api_matches_python_path(gpu, tmp_path):
     s2 = rep.mc.phase2_run(400)
     assert kv["p2_attempted"][0] == s2["attempted"] and kv["p2_accepted"][0] == s2["accepted"]
     assert abs(kv["p2_best"][0] - s2["best_E"]) <= 1e-9 * max(1.0, abs(s2["best_E"]))
+
+
+@pytest.mark.gpu
+def test_cpp_api_phase1_then_phase2_matches_python_path(gpu, tmp_path):
+    """main()'s sequence through the C++ layer: phase_gc_mu_control to a target,
+    the hand-over, phase 2; the same calls from Python give the same numbers."""
+    exe = _build(tmp_path)
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48))
+    _write_system(s, tmp_path / "sys.txt")
+    out = subprocess.run([str(exe), str(tmp_path / "sys.txt"), "100000", "75"], capture_output=True, text=True,
+                         check=True)
+    kv = {}
+    for line in out.stdout.splitlines():
+        k, *v = line.split()
+        kv[k] = [float(x) for x in v]
+    rep = S.Replica(s, p, S.McParams(beta=1.0, mu_eff=0.0), seed=1)
+    g = rep.mc.phase1_run(75.0, direct_pct_threshold=1000, gc_block=200, epoch_attempt_max=1000,
+                          max_trials=100000)
+    ho = rep.mc.phase1_handover()
+    assert g["reached_target"] == 1 and kv["reached"][0] == 1
+    assert kv["attempted"][0] == g["attempted"] and kv["accepted"][0] == g["accepted"]
+    assert kv["blocks"][0] == g["blocks_done"] and kv["nprot"][0] == g["Nprot"] == g["target_count"]
+    assert kv["mu"][0] == g["mu_eff"]
+    assert kv["handover"] == [len(ho[k]) for k in ("nh_n_remaining", "nh_p_mc", "w_remaining", "cl_mc")]
+    assert len(ho["nh_p_mc"]) == g["target_count"]
+    rep.mc.phase2_begin(patience=200)
+    s2 = rep.mc.phase2_run(400)
+    assert kv["p2_attempted"][0] == s2["attempted"] and kv["p2_accepted"][0] == s2["accepted"]
+    assert abs(kv["p2_best"][0] - s2["best_E"]) <= 1e-9 * max(1.0, abs(s2["best_E"]))

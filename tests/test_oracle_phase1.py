@@ -87,3 +87,22 @@ def test_phase1_start_at_target():
     assert r["reached_target"] == 1 and r["attempted"] == 0 and r["handover_sizes"].sum() == 0
     fstar = min(max(pct / 100.0, 1e-6), 1 - 1e-6)
     assert r["mu_eff"] == max(-10.0, min(10.0, math.log(fstar / (1.0 - fstar))))
+
+
+def test_phase1_mu_moves_toward_target():
+    """One epoch (Amax = gc_block = max_trials) below the target raises mu, by at
+    most kT eta |logit f - logit f*| (the information weight is at most 1); a
+    wrong sign or a missing kT factor fails one of the two bounds."""
+    s, p, o = _tiny()
+    mc = S.McParams.at_ph(7.0)
+    pct = 75.0
+    r = o.phase1_run(mc, 3, target_pct=pct, gc_block=150, epoch_attempt_max=150, max_trials=150, mu_eta=0.3)
+    assert r["rc"] == 0 and r["direct"] == 0 and r["reached_target"] == 0
+    assert r["attempted"] == 150 and r["blocks_done"] == 1 and r["accepted"] > 0
+    fstar = pct / 100.0
+    f = r["Nprot"] / r["Nh"]
+    assert f < fstar
+    kT = 1.0 / mc.beta
+    mu0 = kT * math.log(fstar / (1.0 - fstar))
+    bound = kT * 0.3 * abs(math.log(f / (1.0 - f)) - math.log(fstar / (1.0 - fstar)))
+    assert 0.0 < r["mu_eff"] - mu0 <= bound * (1 + 1e-12)
