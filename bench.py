@@ -198,8 +198,8 @@ def main():
     ap.add_argument("--spec-window", type=int, default=8)
     ap.add_argument("--recompute-interval", type=int, default=20000,
                     help="trials between full recomputes of the energy and potential (drift reset)")
-    ap.add_argument("--da-samples-per-ph", type=int, default=65536)
-    ap.add_argument("--da-epochs", type=int, default=20)
+    ap.add_argument("--da-samples-per-ph", type=int, default=131072)
+    ap.add_argument("--da-epochs", type=int, default=60)
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

@@ -437,6 +437,7 @@ struct DaDev {
     int prof;                     // accumulate per-phase times into McCtl::da_prof
     double* rec_fy;               // per-trial record of the reverse surrogate dE*(y->x)
     AtomRec* rec;                 // per atom (pool atoms: dynamic part current)
+    int upd_one;                  // the site's and the ion's species sets are disjoint: one update pass
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -610,16 +611,14 @@ __device__ double stencil_dot(const McDev& D, int a) {
     return s;
 }
 
-// W_a . phi of the atoms a move can pick (the site and ion pools, a fixed set),
-// after each refresh of phi, into their records with their charge, type and
-// species: 8 lanes per atom, lane u < P sums the x column u of the stencil; the
-// list is in stencil-origin order, so neighbouring groups read neighbouring
-// grid rows.
+// W_a . phi of pool atom list[e] (the site and ion pools, a fixed set) into its
+// record with its charge, type and species: 8 lanes per atom (lanes u = 8 e + 0..7
+// of the caller's numbering), lane u < P sums the x column u of the stencil; the
+// list is in stencil-origin order, so neighbouring groups read neighbouring grid
+// rows.
 template <int P>
-__global__ __launch_bounds__(256) void k_phi_dot(McDev D, const signed char* __restrict__ species,
-                                                 const int* __restrict__ list, int n, AtomRec* __restrict__ rec) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x, e = gt >> 3, u = gt & 7;
-    if (e >= n) return;
+__device__ __forceinline__ void phi_dot_atom(const McDev& D, const signed char* __restrict__ species,
+                                             const int* __restrict__ list, int e, int u, AtomRec* __restrict__ rec) {
     const int a = list[e];
     const int* sb = D.sbase + 3 * a;
     const double* w = D.sw + (size_t)a * 3 * P;
@@ -652,6 +651,14 @@ __global__ __launch_bounds__(256) void k_phi_dot(McDev D, const signed char* __r
         rec[a].type = ty;
         rec[a].sp = species[ty];
     }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_phi_dot(McDev D, const signed char* __restrict__ species,
+                                                 const int* __restrict__ list, int n, AtomRec* __restrict__ rec) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, e = gt >> 3;
+    if (e >= n) return;
+    phi_dot_atom<P>(D, species, list, e, gt & 7, rec);
 }
 
 // Physics part B = lin + selfq + pr + corr of n moves (one thread per move).
@@ -703,6 +710,71 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
 }
 
 // ---------------------------------------------------------------- delayed-acceptance chain
+// The pools and the shell histograms after an accepted move, when the species a
+// site's change touches and those an ion's change touches are disjoint (no
+// histogram counter is written by both rows): the pool shifts of pool_update_at
+// and both rows' histogram read-modify-writes go in one load phase and one store
+// phase.  hrow: the rows' bounds in dlist (i's, then j's).
+template <typename PoolT>
+__device__ void da_accept_update(const DaDev& A, const McDev& D, PoolT* pbuf, const DecisionLds& t,
+                                 const int* hrow, int so_i, int sn_i, int so_j, int sn_j) {
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int p0 = t.er_pool[0], p1 = t.er_pool[1];
+    const int a0 = t.pos[0] + 1, a1 = t.pos[1] + 1;
+    const int L0 = t.er_n[0] - a0, L = L0 + (t.er_n[1] - a1);
+    const int hi = hrow[0], ni = so_i != sn_i ? hrow[1] - hrow[0] : 0;
+    const int hj = hrow[2], nj = so_j != sn_j ? hrow[3] - hrow[2] : 0;
+    const int NH = ni + nj;
+    constexpr int M = 16, MH = 4;
+    for (int base = 0, hb = 0; base < L || hb < NH; base += M * bd, hb += MH * bd) {
+        PoolT v[M];
+        int dst[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int e = base + tid + m * bd;
+            dst[m] = -1;
+            v[m] = PoolT(0);
+            if (e < L) {
+                const bool first = e < L0;
+                const int pool = first ? p0 : p1, k = first ? a0 + e : a1 + (e - L0);
+                v[m] = pbuf[pool_slot(D, pool, k)];
+                dst[m] = pool_slot(D, pool, k - 1);
+            }
+        }
+        unsigned short* hp[MH];
+        int ho[MH], hn[MH], so[MH], sn[MH];
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int e = hb + tid + m * bd;
+            hp[m] = nullptr;
+            ho[m] = hn[m] = 0;
+            so[m] = sn[m] = -1;
+            if (e < NH) {
+                const bool first = e < ni;
+                const unsigned en = A.dlist[first ? hi + e : hj + (e - ni)];
+                so[m] = first ? so_i : so_j;
+                sn[m] = first ? sn_i : sn_j;
+                hp[m] = A.H + (size_t)(en & 0xFFFFFFu) * NN_IN + (en >> DA_SHIFT) * 4;
+                if (so[m] >= 0) ho[m] = hp[m][so[m]];
+                if (sn[m] >= 0) hn[m] = hp[m][sn[m]];
+            }
+        }
+        __syncthreads();   // every pool value read before any is moved
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (dst[m] >= 0) pbuf[dst[m]] = v[m];
+#pragma unroll
+        for (int m = 0; m < MH; ++m)
+            if (hp[m]) {
+                if (so[m] >= 0) hp[m][so[m]] = (unsigned short)(ho[m] - 1);
+                if (sn[m] >= 0) hp[m][sn[m]] = (unsigned short)(hn[m] + 1);
+            }
+        __syncthreads();
+    }
+    if (tid < 2) pbuf[pool_slot(D, t.pb_pool[tid], t.pb_n[tid])] = (PoolT)t.pb_val[tid];
+    __syncthreads();
+}
+
 struct CandLds {
     int i[DA_M], j[DA_M], fwd[DA_M], exA[DA_M], exB[DA_M];
     int pos[DA_M];                // stream position after the trial's draws
@@ -739,6 +811,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ __align__(16) unsigned short s_H1[DA_M * HP];
     __shared__ float s_y[DA_M][8];
     __shared__ int s_first, s_start;
+    __shared__ int s_hrow[4];              // dlist bounds of the staged trial's i and j
     __shared__ unsigned long long s_msk;   // stage-1 acceptances of the round
     __shared__ long long s_done0;          // trials done at the round's start
     __shared__ long long s_s1, s_rounds;
@@ -749,21 +822,13 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     clk.start(prof);
     McCtl* C = D.ctl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long long target = C->trials_target, rec_cap = C->rec_cap;
-    // a launch queued beyond the target (batches are sized ahead) leaves at once
-    if (C->trials_done >= target || C->status != 0) return;
-    const int pn0 = pending_unfolded(D);
-    if (pn0 > 0) {   // phi lags the state: the surrogate's potential term would not be a function of the state
-        if (tid == 0) C->status = 3;
-        return;
-    }
     const int M = A.M;
     // pending is empty for the whole launch; trial_partial never reads this view then
     const PendT<P>& s_pd = *reinterpret_cast<const PendT<P>*>(s_wsum);
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
-    if (LDS_POOLS)
-        for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
-    {   // engine state and network into LDS: every load issued before the first store
+    long long target, rec_cap;
+    {   // engine state and network into LDS: every load (with the control words the exit
+        // checks read) issued before the first decision and the first store
         static_assert(NN_H * NN_IN / 4 == 3 * DA_THREADS && NN_H * NN_H / 4 == 8 * DA_THREADS && NN_H <= DA_THREADS,
                       "prologue layout");
         const uint2* w1 = reinterpret_cast<const uint2*>(net.w1t);
@@ -782,6 +847,20 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             b2 = net.theta[OFF_B2 + tid];
             w3 = bf2f(net.w3[tid]);
         }
+        target = C->trials_target;
+        rec_cap = C->rec_cap;
+        const long long done_in = C->trials_done;
+        const int status_in = C->status, pend_in = C->pending_n;
+        const unsigned seq_in = C->pend_seq, applied_in = D.applied[0];
+        // a launch queued beyond the target (batches are sized ahead) leaves at once
+        if (done_in >= target || status_in != 0) return;
+        if (applied_in != seq_in && pend_in > 0) {
+            // phi lags the state: the surrogate's potential term would not be a function of the state
+            if (tid == 0) C->status = 3;
+            return;
+        }
+        if (LDS_POOLS)
+            for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {   // 12 chunks per row
             const int k = tid + m * DA_THREADS, j = k / (NN_IN / 4), c = k - j * (NN_IN / 4);
@@ -893,6 +972,15 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         const int ntr = s_dc.ntr;
         // ---- 2. per-candidate physics terms (W . phi from the per-atom table) and the raw
         //         shell-count differences
+        // (the histogram rows' loads are issued first; the records' follow)
+        int raw[DA_FEAT];
+#pragma unroll
+        for (int q = 0; q < DA_FEAT; ++q) {
+            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
+            raw[q] = 0;
+            if (e < DA_M * NN_IN && k < ntr)
+                raw[q] = (int)A.H[(size_t)s_c.i[k] * NN_IN + c] - (int)A.H[(size_t)s_c.j[k] * NN_IN + c];
+        }
         if (tid < ntr) {
             double r;
             int exA, exB;
@@ -902,14 +990,6 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
             s_c.lin[tid][0] = ri.phiw;
             s_c.lin[tid][1] = rj.phiw;
-        }
-        int raw[DA_FEAT];
-#pragma unroll
-        for (int q = 0; q < DA_FEAT; ++q) {
-            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
-            raw[q] = 0;
-            if (e < DA_M * NN_IN && k < ntr)
-                raw[q] = (int)A.H[(size_t)s_c.i[k] * NN_IN + c] - (int)A.H[(size_t)s_c.j[k] * NN_IN + c];
         }
         __syncthreads();
 #pragma unroll
@@ -1022,7 +1102,10 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     if (first >= ntr) s_st.done = done0 + ntr;
                     else {
                         ++s_s1;
-                        stage_trial<P>(D, s_tr[0], s_c.i[first], s_c.j[first], s_c.fwd[first]);
+                        const int i1 = s_c.i[first], j1 = s_c.j[first];
+                        stage_trial<P>(D, s_tr[0], i1, j1, s_c.fwd[first]);
+                        s_hrow[0] = A.dhead[i1]; s_hrow[1] = A.dhead[i1 + 1];
+                        s_hrow[2] = A.dhead[j1]; s_hrow[3] = A.dhead[j1 + 1];
                     }
                 }
             }
@@ -1106,7 +1189,11 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             const bool accepted = s_dc.acc_k == 0;
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
-            if (accepted) {
+            if (accepted && A.upd_one) {
+                const TrialT<P>& t = s_tr[0];
+                da_accept_update<PoolT>(A, D, pbuf, s_dc, s_hrow, A.species[t.ti_old], A.species[t.ti_new],
+                                        A.species[t.tj_old], A.species[t.tj_new]);
+            } else if (accepted) {
                 const TrialT<P>& t = s_tr[0];
                 pool_update_at<PoolT>(D, pbuf, s_dc);
                 for (int side = 0; side < 2; ++side) {
@@ -1586,6 +1673,12 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         S->H.alloc((size_t)std::max(N, 1) * NN_IN);
         DaDev& A = S->dev;
         A.dhead = S->dhead.p; A.dlist = S->dlist.p; A.H = S->H.p; A.species = S->sp.p;
+        {   // species a site's change touches vs those an ion's change touches
+            auto spc = [&](int t1) { return t1 >= 1 && t1 <= (int)S->species.size() ? (int)S->species[t1 - 1] : -1; };
+            const int a1 = spc(M->d.nh_n_type), a2 = spc(M->d.nh_p_type), b1 = spc(M->d.w_type), b2 = spc(M->d.cl_type);
+            auto meet = [](int x, int y) { return x >= 0 && x == y; };
+            A.upd_one = (meet(a1, b1) || meet(a1, b2) || meet(a2, b1) || meet(a2, b2)) ? 0 : 1;
+        }
         A.x = M->atoms->x.p; A.y = M->atoms->y.p;
         A.Lx = M->box.Lx; A.Ly = M->box.Ly; A.Lz = M->box.Lz; A.rd2 = d->r_desc * d->r_desc;
         for (int k = 0; k < 12; ++k) {

@@ -25,6 +25,7 @@ import pytest
 from oracle import mlp as MLP
 from oracle import pyoracle as O
 from ph_brush_amd import surrogate as SG
+from ph_brush_amd import _lib as L
 from ph_brush_amd import system as S
 
 pytestmark = pytest.mark.gpu
@@ -173,6 +174,61 @@ def test_da_chain_replays_on_oracle(gpu):
         assert np.array_equal(ty, ro["type0"]) and np.array_equal(q, ro["q"])
     e_run = r.mc.stats()["E"]
     assert abs(e_run - r.mc.recompute()) <= 1e-9 * max(1.0, abs(e_run))
+
+
+@pytest.mark.parametrize("refresh", ["auto", "kspace"])
+def test_da_stage1_values_are_functions_of_the_state(gpu, refresh):
+    """With a network that outputs zero, the stage-1 value dE*(x->y) the chain
+    records is the physics part alone; at the state the trial saw (rebuilt from
+    the record, recomputed from scratch) it equals the unchained evaluation
+    (phb_da_predict: W . phi by direct stencil sums over the recomputed grid).
+    For every accepted move the reverse value dE*(y->x) the chain used at stage 2
+    equals the unchained evaluation of the reverse move at y, so stage 2 sees the
+    surrogate as a function of the state (detailed balance).  "auto" takes the
+    direct refresh with the partial stencil sums from the refresh launch;
+    "kspace" the k-space refresh and a separate stencil pass."""
+    s, p = _tiny()
+    mc = S.McParams.at_ph(7.9)
+    n = 1500
+    r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=10 ** 9, record_cap=n,
+                  refresh=L.PHB_REFRESH_AUTO if refresh == "auto" else L.PHB_REFRESH_KSPACE)
+    r.mc.da_attach(r_desc=6.0)
+    sur = S.Surrogate(0)
+    theta = np.zeros_like(sur.get_params())
+    theta[-1] = 1.0          # output std (> 0); every weight zero, so the network outputs exactly 0
+    sur.set_params(theta)
+    q, t = r.atoms.identity()
+    res = r.mc.da_run(sur, n)
+    assert res["attempted"] == n and res["status"] == 0
+    rg = r.mc.da_record(n)
+    acc = np.flatnonzero(rg["stage"] == 1)
+    assert len(acc) >= 10
+    idx = np.arange(s.natoms, dtype=np.int32)
+
+    def at_state(qv, tv, i, j, f):
+        r.atoms.set_identity(idx, tv, qv)
+        r.mc.recompute()
+        return r.mc.da_predict(sur, i, j, f)
+
+    def tol(v):
+        return 1e-9 * np.maximum(1.0, np.abs(v))
+
+    bounds = list(acc) if len(acc) and acc[-1] == n - 1 else list(acc) + [n - 1]
+    start = 0
+    for a in bounds:
+        sl = slice(start, a + 1)
+        pred = at_state(q, t, rg["i"][sl], rg["j"][sl], rg["fwd"][sl])
+        assert np.all(np.abs(pred - rg["dE_star"][sl]) <= tol(pred)), (start, a)
+        if rg["stage"][a] != 1:
+            break
+        i, j, f = int(rg["i"][a]), int(rg["j"][a]), int(rg["fwd"][a])
+        if f:
+            t[i], q[i], t[j], q[j] = mc.nh_p_type - 1, 1.0, mc.cl_type - 1, -1.0
+        else:
+            t[i], q[i], t[j], q[j] = mc.nh_n_type - 1, 0.0, mc.w_type - 1, 0.0
+        rev = at_state(q, t, np.array([i]), np.array([j]), np.array([1 - f]))
+        assert abs(rev[0] - rg["dE_star_rev"][a]) <= tol(rev[0]), a
+        start = a + 1
 
 
 def _alpha_series(run, n_chunks, chunk, nsites):
