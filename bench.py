@@ -330,6 +330,11 @@ def main():
                    "stage1_acceptance": stage1, "stage2_acceptance": da_acc / max(stage1, 1e-12),
                    "trials_per_round": n_att / max(rounds, 1), "alpha": sum(alpha_da) / len(alpha_da)},
             "da_over_plain": value / plain_value,
+            # accepted moves per second: a delayed-acceptance chain accepts fewer of its trials than the
+            # plain Metropolis chain (stage 1 times stage 2), so per accepted move the gain is smaller
+            "plain_accepted_per_s": plain_value * plain_acc,
+            "da_accepted_per_s": value * da_acc,
+            "da_over_plain_accepted": (value * da_acc) / max(plain_value * plain_acc, 1e-12),
             "alpha_note": "alpha of the timed blocks (same start state and seed); equality at fixed pH is "
                           "tested over long runs in tests/test_gpu_da.py",
         },

@@ -438,6 +438,7 @@ struct DaDev {
     double* rec_fy;               // per-trial record of the reverse surrogate dE*(y->x)
     AtomRec* rec;                 // per atom (pool atoms: dynamic part current)
     int upd_one;                  // the site's and the ion's species sets are disjoint: one update pass
+    long long* psi;               // per atom: real-space potential of the listed neighbours (fixed point)
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -533,8 +534,33 @@ __global__ void k_da_desc(DaDev A, McDev D, int n, const int* __restrict__ mi, c
 // and f the MLP of the shell-count descriptor, which learns the rest (the
 // real-space energy of the environment).
 struct MoveTerms {
-    double dqi, dqj, selfq, perfc, perf, corr;
+    double dqi, dqj, selfq, perfc, perf, corr, psi;
 };
+
+// Real-space potential of the listed neighbours at each atom, psi_a = sum over
+// a's neighbour row of q_k elF erfc(alpha r)/r, in fixed point (2^-32 units):
+// each pair contributes llrint(q_k llrint(kreal 2^32)), so psi is an exact
+// integer sum, a function of the charges alone whatever order the changes of an
+// accepted move are added in.
+constexpr double PSI_SCALE = 4294967296.0, PSI_INV = 1.0 / 4294967296.0;
+__device__ __forceinline__ long long psi_k(double kreal) { return __double2ll_rn(kreal * PSI_SCALE); }
+__device__ __forceinline__ long long psi_c(double q, long long K) { return __double2ll_rn(q * (double)K); }
+
+__device__ __forceinline__ long long psi_of(const McDev& D, int a) {
+    long long s = 0;
+    for (int e = D.fhead[a]; e < D.fhead[a + 1]; ++e) s += psi_c(D.q[D.fnbr[e]], psi_k(D.kreal[e]));
+    return s;
+}
+
+// The surrogate's real-space term of a move: dq_i psi_i + dq_j psi_j.
+__device__ __forceinline__ double psi_term(double dqi, long long pi, double dqj, long long pj) {
+    return (dqi * (double)pi + dqj * (double)pj) * PSI_INV;
+}
+
+__global__ void k_psi_all(McDev D, long long* __restrict__ psi) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < D.N) psi[a] = psi_of(D, a);
+}
 
 __device__ __forceinline__ MoveTerms move_terms(const McDev& D, const DaDev& A, int i, int j, int fwd, double r,
                                                 double Q, double Mz, double Qz2) {
@@ -544,11 +570,14 @@ __device__ __forceinline__ MoveTerms move_terms(const McDev& D, const DaDev& A, 
     m.dqi = qin - qi;
     m.dqj = qjn - qj;
     m.selfq = 0.5 * (m.dqi * m.dqi * D.selfw[i] + m.dqj * m.dqj * D.selfw[j]);
-    // the pair term is part of the surrogate only: single precision is enough
+    // the real-space change is dq_i psi_i + dq_j psi_j + dq_i dq_j k_ij (k_ij: the listed
+    // pair's coefficient); the pair terms are part of the surrogate only: single
+    // precision is enough
     const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
-    m.perfc = r * r <= A.rc2 ? (double)((float)(qin * qjn - qi * qj) * kr * ec) : 0.0;
+    m.perfc = r * r <= A.rc2 ? (double)((float)(m.dqi * m.dqj) * kr * ec) : 0.0;
     m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
     m.corr = corr_move(D, qi, qin, qj, qjn, D.z[i], D.z[j], Q, Mz, Qz2);
+    m.psi = psi_term(m.dqi, psi_of(D, i), m.dqj, psi_of(D, j));
     return m;
 }
 
@@ -569,7 +598,7 @@ __device__ __forceinline__ void pair_terms_rec(const DaDev& A, const AtomRec& ri
 
 __device__ __forceinline__ MoveTerms move_terms_rec(const McDev& D, const DaDev& A, const AtomRec& ri,
                                                     const AtomRec& rj, int fwd, double r, double Q, double Mz,
-                                                    double Qz2) {
+                                                    double Qz2, long long pi, long long pj) {
     MoveTerms m;
     const double qi = ri.q, qj = rj.q;
     const double qin = fwd ? 1.0 : 0.0, qjn = fwd ? -1.0 : 0.0;
@@ -577,9 +606,10 @@ __device__ __forceinline__ MoveTerms move_terms_rec(const McDev& D, const DaDev&
     m.dqj = qjn - qj;
     m.selfq = 0.5 * (m.dqi * m.dqi * ri.selfw + m.dqj * m.dqj * rj.selfw);
     const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
-    m.perfc = r * r <= A.rc2 ? (double)((float)(qin * qjn - qi * qj) * kr * ec) : 0.0;
+    m.perfc = r * r <= A.rc2 ? (double)((float)(m.dqi * m.dqj) * kr * ec) : 0.0;
     m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
     m.corr = corr_move(D, qi, qin, qj, qjn, ri.z, rj.z, Q, Mz, Qz2);
+    m.psi = psi_term(m.dqi, pi, m.dqj, pj);
     return m;
 }
 
@@ -674,7 +704,7 @@ __global__ void k_da_base(McDev D, DaDev A, int n, const int* __restrict__ mi, c
     pair_terms(A, D, i, j, r, exA, exB);
     const MoveTerms t = move_terms(D, A, i, j, mf[m], r, C->Q, C->Mz, C->Qz2);
     const double lin = t.dqi * stencil_dot<P>(D, i) + t.dqj * stencil_dot<P>(D, j);
-    base[m] = lin + t.selfq + t.perfc + t.perf + t.corr;
+    base[m] = lin + t.selfq + t.perfc + t.perf + t.corr + t.psi;
 }
 
 // Training target t = s (dE - B).
@@ -775,6 +805,20 @@ __device__ void da_accept_update(const DaDev& A, const McDev& D, PoolT* pbuf, co
     __syncthreads();
 }
 
+// psi of the neighbours of i and j after an accepted move (integer atomics: the
+// order of the additions does not change the sums).
+template <int P>
+__device__ __forceinline__ void psi_update(const McDev& D, const DaDev& A, const TrialT<P>& t, int tid, int bd) {
+    const int ni = t.nrow[0], n = ni + t.nrow[1];
+    for (int it = tid; it < n; it += bd) {
+        const bool first = it < ni;
+        const int e = first ? t.hrow[0] + it : t.hrow[1] + (it - ni);
+        const long long K = psi_k(D.kreal[e]);
+        const long long d = first ? psi_c(t.qi_new, K) - psi_c(t.qi_old, K) : psi_c(t.qj_new, K) - psi_c(t.qj_old, K);
+        if (d != 0) atomicAdd(reinterpret_cast<unsigned long long*>(A.psi + D.fnbr[e]), (unsigned long long)d);
+    }
+}
+
 struct CandLds {
     int i[DA_M], j[DA_M], fwd[DA_M], exA[DA_M], exB[DA_M];
     int pos[DA_M];                // stream position after the trial's draws
@@ -782,6 +826,7 @@ struct CandLds {
     double u1[DA_M], u2[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
     double lin[DA_M][2];          // W_a . phi of the site and the ion
     MoveTerms mt[DA_M];
+    long long pi[DA_M], pj[DA_M]; // psi of the site and the ion
     float sf[DA_M];               // s f(d)
 };
 
@@ -812,6 +857,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ float s_y[DA_M][8];
     __shared__ int s_first, s_start;
     __shared__ int s_hrow[4];              // dlist bounds of the staged trial's i and j
+    __shared__ double s_kij;               // the staged pair's real-space coefficient (0: not listed)
     __shared__ unsigned long long s_msk;   // stage-1 acceptances of the round
     __shared__ long long s_done0;          // trials done at the round's start
     __shared__ long long s_s1, s_rounds;
@@ -985,9 +1031,11 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             double r;
             int exA, exB;
             const AtomRec ri = A.rec[s_c.i[tid]], rj = A.rec[s_c.j[tid]];
+            const long long pi = A.psi[s_c.i[tid]], pj = A.psi[s_c.j[tid]];
             pair_terms_rec(A, ri, rj, r, exA, exB);
             s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
-            s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2);
+            s_c.pi[tid] = pi; s_c.pj[tid] = pj;
+            s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2, pi, pj);
             s_c.lin[tid][0] = ri.phiw;
             s_c.lin[tid][1] = rj.phiw;
         }
@@ -1064,7 +1112,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 const float sf = fwd ? f : -f;
                 const MoveTerms& m = s_c.mt[k];
                 const double Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr +
-                                  (double)sf;
+                                  m.psi + (double)sf;
                 const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
                 const double e1 = mh_exponent(D.beta, Fx, fwd ? 1 : -1, D.mu, Sb, Sa);
                 acc1 = e1 >= 0.0 || exp(e1) > s_c.u1[k];
@@ -1106,6 +1154,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                         stage_trial<P>(D, s_tr[0], i1, j1, s_c.fwd[first]);
                         s_hrow[0] = A.dhead[i1]; s_hrow[1] = A.dhead[i1 + 1];
                         s_hrow[2] = A.dhead[j1]; s_hrow[3] = A.dhead[j1 + 1];
+                        s_kij = 0.0;
                     }
                 }
             }
@@ -1117,7 +1166,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             {
                 double win = 0.0;
-                const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win));
+                const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win, &s_kij));
                 win = wave_sum(win);
                 if (lane == 0) { s_wsum[wave] = part; s_wwin[wave] = win; }
             }
@@ -1139,7 +1188,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 // there is phi + G dQ, so W_a . phi_y = W_a . phi + sum_b dq_b W_a^T G W_b
                 const MoveTerms& m = s_c.mt[first];
                 const double lin = m.dqi * s_c.lin[first][0] + m.dqj * s_c.lin[first][1];
-                const double Fy = -lin - m.selfq - 2.0 * wsum - m.perfc + m.perf - m.corr - (double)s_c.sf[first];
+                // its real-space term from psi at y, formed in fixed point as the table will hold it
+                const long long K = psi_k(s_kij);
+                const long long piy = s_c.pi[first] + (psi_c(t.qj_new, K) - psi_c(t.qj_old, K));
+                const long long pjy = s_c.pj[first] + (psi_c(t.qi_new, K) - psi_c(t.qi_old, K));
+                const double Fy = -lin - m.selfq - 2.0 * wsum + m.perfc + m.perf - m.corr - (double)s_c.sf[first] +
+                                  psi_term(-m.dqi, piy, -m.dqj, pjy);
                 double e2;
                 {
 #pragma clang fp contract(off)
@@ -1189,6 +1243,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             const bool accepted = s_dc.acc_k == 0;
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
+            if (accepted) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
             if (accepted && A.upd_one) {
                 const TrialT<P>& t = s_tr[0];
                 da_accept_update<PoolT>(A, D, pbuf, s_dc, s_hrow, A.species[t.ti_old], A.species[t.ti_new],
@@ -1286,6 +1341,7 @@ struct DaState {
     DevBuf<double> x, y;       // copies of the positions (fixed)
     DevBuf<double> rec_fy;
     DevBuf<AtomRec> rec;       // per atom: what the chain's physics terms read
+    DevBuf<long long> psi;     // per atom: real-space potential of the listed neighbours (fixed point)
     DevBuf<int> cand;          // the pool atoms, in stencil-origin order
     int ncand = 0;
     DaDev dev{};
@@ -1691,6 +1747,8 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         A.rec_fy = S->rec_fy.p;
         S->rec.alloc(std::max(N, 1));
         A.rec = S->rec.p;
+        S->psi.alloc(std::max(N, 1));
+        A.psi = S->psi.p;
         if (N > 0)
             hipLaunchKernelGGL(k_rec_static, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
                                M->atoms->y.p, M->atoms->z.p, M->selfw.p, S->rec.p);
@@ -1797,6 +1855,9 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
         upload_views(M, S, s);
         refresh(M);   // the surrogate needs phi to hold every accepted change
         phi_dot(M, S);
+        // psi from the current charges (identities may have changed since the last run)
+        hipLaunchKernelGGL(k_psi_all, dim3(ceil_div(M->atoms->n, 256)), dim3(256), 0, stream(), M->dev, S->psi.p);
+        PHB_HIP(hipGetLastError());
         McCtl c = read_ctl(M);
         const long long target = c.trials_done + ntrials;
         long long done = c.trials_done;

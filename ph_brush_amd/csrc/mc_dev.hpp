@@ -327,7 +327,7 @@ __device__ __forceinline__ T corr(const double* wa, const double* wb, int dd) {
 // caller reduces the per-thread results in FP64.
 template <int P, typename T = double>
 __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<P>& pd, int pend_n, int lt, int nt,
-                                double* win = nullptr) {
+                                double* win = nullptr, double* kij = nullptr) {
     constexpr int CW = 2 * P - 1, PP = P * P;
     const McArrays<T> arr(D);
     const T dqi = (T)(t.qi_new - t.qi_old), dqj = (T)(t.qj_new - t.qj_old);
@@ -341,6 +341,7 @@ __device__ double trial_partial(const McDev& D, const TrialT<P>& t, const PendT<
             const int n = D.fnbr[e];
             double qn;
             int tn;
+            if (kij && first && n == t.j) *kij = D.kreal[e];   // the (i, j) entry, when the pair is listed
             if (!first && n == t.i) { qn = t.qi_new; tn = t.ti_new; }
             else { qn = D.q[n]; tn = D.type[n]; }
             T v = (first ? dqi : dqj) * (T)qn * arr.kreal[e];
