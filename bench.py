@@ -36,6 +36,18 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
+def _traffic(kernel: str):
+    """HBM bytes per launch of a kernel from the committed counter-pass record
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
+    reads for a device copy of known size); None when there is no record."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def chain_bytes_per_trial(rep) -> float:
     """Algorithmic bytes one trial must read (each byte once), FP64 path:
     two full neighbour rows (4 B index + 8 B coefficient + 8 B neighbour charge
@@ -349,7 +361,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": da_achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": _traffic("k_da_chain"),
+            "traffic_unit": "HBM bytes per launch (FETCH_SIZE and WRITE_SIZE counter passes, copy-calibrated)",
             "avg_launch_us": da_launch_s * 1e6,
             "bytes_per_trial": da_bpt,
             "trials_per_launch": da_trials_per_launch,
