@@ -683,10 +683,15 @@ __device__ __forceinline__ void phi_dot_atom(const McDev& D, const signed char* 
     }
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b on XCD b mod 8), each
+// XCD with its own L2: XCD x takes the x-th eighth of the list (neighbouring grid
+// rows), so that each XCD's L2 holds one slab of the grid instead of all of it.
+// The grid is a multiple of 8 workgroups; logical workgroups past the list leave.
 template <int P>
 __global__ __launch_bounds__(256) void k_phi_dot(McDev D, const signed char* __restrict__ species,
                                                  const int* __restrict__ list, int n, AtomRec* __restrict__ rec) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x, e = gt >> 3;
+    const int per = gridDim.x >> 3, lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int gt = lb * blockDim.x + threadIdx.x, e = gt >> 3;
     if (e >= n) return;
     phi_dot_atom<P>(D, species, list, e, gt & 7, rec);
 }
@@ -1366,8 +1371,8 @@ void rebuild_hist(phb_mc_s* M, DaState* S) {
 void phi_dot(phb_mc_s* M, DaState* S) {
     if (S->ncand == 0) return;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ceil_div(8 * S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->sp.p,
-                           S->cand.p, S->ncand, S->rec.p);
+        hipLaunchKernelGGL(kern, dim3(8 * ceil_div(ceil_div(8 * S->ncand, 256), 8)), dim3(256), 0, stream(), M->dev,
+                           S->sp.p, S->cand.p, S->ncand, S->rec.p);
     };
     switch (M->dev.P) {
         case 3: go(k_phi_dot<3>); break;
