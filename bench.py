@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-da-real-space", action="store_true",
+                    help="skip the delayed-acceptance form whose surrogate includes the real-space potential")
     ap.add_argument("--fp32-trials", type=int, default=100000)
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
     ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
@@ -310,6 +312,36 @@ def main():
     da_bpt = 2 * 48 * 2 + stage1 * bpt
     da_achieved = da_bpt * da_trials_per_launch / da_launch_s / 1e9
     infer_tflops = flop_fwd * da_trials_per_launch / da_launch_s / 1e12
+    del rep
+
+    # ---- the same form with the neighbours' real-space potential in the surrogate's physics part
+    # (its own network: the physics part sets the residual the network learns)
+    da_rs = None
+    if not args.no_da_real_space:
+        spec_rs = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000,
+                                 between=2000, real_space=True)
+        sur_rs = S.Surrogate(len(spec_rs.ph_values) * spec_rs.samples_per_ph)
+        SG.collect(s, p, sur_rs, spec_rs)
+        tr_rs = SG.train_holdout(sur_rs, epochs=args.da_epochs, batch=4096, lr=2e-3)
+        print(f"[bench] real-space surrogate: holdout rmse {tr_rs['rmse_holdout']:.3f} "
+              f"(target std {tr_rs['target_std']:.3f})", file=sys.stderr, flush=True)
+        rep = S.Replica(s_eq, p, mc, seed=1, **kw)
+        rep.mc.da_attach(r_desc=spec_rs.r_desc, real_space=True)
+        e0 = rep.mc.da_run(sur_rs, 0)
+        dt_rs, alpha_rs = _timed(lambda n: rep.mc.da_run(sur_rs, n), K, W, T, "da real-space")
+        e1 = rep.mc.da_run(sur_rs, 0)
+        n_rs = max(e1["attempted"] - e0["attempted"], 1)
+        v_rs = K * T / dt_rs
+        acc_rs = (e1["accepted"] - e0["accepted"]) / n_rs
+        s1_rs = (e1["stage1_accepted"] - e0["stage1_accepted"]) / n_rs
+        da_rs = {"value": v_rs, "ms_per_step": dt_rs / K * 1e3, "acceptance": acc_rs, "stage1_acceptance": s1_rs,
+                 "stage2_acceptance": acc_rs / max(s1_rs, 1e-12),
+                 "trials_per_launch": n_rs / max(e1["launches"] - e0["launches"], 1),
+                 "alpha": sum(alpha_rs) / len(alpha_rs), "over_plain": v_rs / plain_value,
+                 "accepted_per_s": v_rs * acc_rs,
+                 "over_plain_accepted": v_rs * acc_rs / max(plain_value * plain_acc, 1e-12),
+                 "rmse_holdout_kT": tr_rs["rmse_holdout"], "target_std_kT": tr_rs["target_std"]}
+        del rep
 
     out = {
         "metric": "MC trial moves/s at 30k particles; DA-surrogate vs plain at equal alpha(pH)",
@@ -347,6 +379,7 @@ def main():
             "plain_accepted_per_s": plain_value * plain_acc,
             "da_accepted_per_s": value * da_acc,
             "da_over_plain_accepted": (value * da_acc) / max(plain_value * plain_acc, 1e-12),
+            "da_real_space": da_rs,
             "alpha_note": "alpha of the timed blocks (same start state and seed); equality at fixed pH is "
                           "tested over long runs in tests/test_gpu_da.py",
         },

@@ -360,6 +360,10 @@ typedef struct {
     int ntypes;
     double r_desc;                       /* histogram radius */
     int candidates;                      /* surrogate screens per round, 1..64 (0 = 64) */
+    int real_space;                      /* 1: the surrogate's physics part includes the real-space potential
+                                            of the listed neighbours of the site and the ion (exact, kept in
+                                            fixed point); 0: it does not.  A network is trained with the
+                                            physics part of the chain it screens for. */
 } phb_da_desc;
 /* Builds the r_desc neighbour rows and per-atom histograms of a chain. */
 int phb_da_attach(phb_mc mc, const phb_da_desc *d);

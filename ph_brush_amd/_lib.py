@@ -121,7 +121,7 @@ class TrainStats(C.Structure):
 
 class DaDesc(C.Structure):
     _fields_ = [("species_of_type", C.POINTER(C.c_byte)), ("ntypes", C.c_int), ("r_desc", C.c_double),
-                ("candidates", C.c_int)]
+                ("candidates", C.c_int), ("real_space", C.c_int)]
 
 
 class DaStats(C.Structure):

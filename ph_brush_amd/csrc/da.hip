@@ -439,6 +439,7 @@ struct DaDev {
     AtomRec* rec;                 // per atom (pool atoms: dynamic part current)
     int upd_one;                  // the site's and the ion's species sets are disjoint: one update pass
     long long* psi;               // per atom: real-space potential of the listed neighbours (fixed point)
+    int use_psi;                  // the physics part includes dq_i psi_i + dq_j psi_j
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -573,11 +574,13 @@ __device__ __forceinline__ MoveTerms move_terms(const McDev& D, const DaDev& A, 
     // the real-space change is dq_i psi_i + dq_j psi_j + dq_i dq_j k_ij (k_ij: the listed
     // pair's coefficient); the pair terms are part of the surrogate only: single
     // precision is enough
+    // (without psi the pair term is the whole listed pair's change, (q_i' q_j' - q_i q_j) k_ij)
     const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
-    m.perfc = r * r <= A.rc2 ? (double)((float)(m.dqi * m.dqj) * kr * ec) : 0.0;
+    const double pq = A.use_psi ? m.dqi * m.dqj : qin * qjn - qi * qj;
+    m.perfc = r * r <= A.rc2 ? (double)((float)pq * kr * ec) : 0.0;
     m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
     m.corr = corr_move(D, qi, qin, qj, qjn, D.z[i], D.z[j], Q, Mz, Qz2);
-    m.psi = psi_term(m.dqi, psi_of(D, i), m.dqj, psi_of(D, j));
+    m.psi = A.use_psi ? psi_term(m.dqi, psi_of(D, i), m.dqj, psi_of(D, j)) : 0.0;
     return m;
 }
 
@@ -606,10 +609,11 @@ __device__ __forceinline__ MoveTerms move_terms_rec(const McDev& D, const DaDev&
     m.dqj = qjn - qj;
     m.selfq = 0.5 * (m.dqi * m.dqi * ri.selfw + m.dqj * m.dqj * rj.selfw);
     const float ec = erfcf((float)(D.alpha * r)), kr = (float)(D.elF / r);
-    m.perfc = r * r <= A.rc2 ? (double)((float)(m.dqi * m.dqj) * kr * ec) : 0.0;
+    const double pq = A.use_psi ? m.dqi * m.dqj : qin * qjn - qi * qj;
+    m.perfc = r * r <= A.rc2 ? (double)((float)pq * kr * ec) : 0.0;
     m.perf = (double)((float)(m.dqi * m.dqj) * kr * (1.0f - ec));
     m.corr = corr_move(D, qi, qin, qj, qjn, ri.z, rj.z, Q, Mz, Qz2);
-    m.psi = psi_term(m.dqi, pi, m.dqj, pj);
+    m.psi = A.use_psi ? psi_term(m.dqi, pi, m.dqj, pj) : 0.0;
     return m;
 }
 
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             double r;
             int exA, exB;
             const AtomRec ri = A.rec[s_c.i[tid]], rj = A.rec[s_c.j[tid]];
-            const long long pi = A.psi[s_c.i[tid]], pj = A.psi[s_c.j[tid]];
+            const long long pi = A.use_psi ? A.psi[s_c.i[tid]] : 0, pj = A.use_psi ? A.psi[s_c.j[tid]] : 0;
             pair_terms_rec(A, ri, rj, r, exA, exB);
             s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
             s_c.pi[tid] = pi; s_c.pj[tid] = pj;
@@ -1193,12 +1197,18 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 // there is phi + G dQ, so W_a . phi_y = W_a . phi + sum_b dq_b W_a^T G W_b
                 const MoveTerms& m = s_c.mt[first];
                 const double lin = m.dqi * s_c.lin[first][0] + m.dqj * s_c.lin[first][1];
-                // its real-space term from psi at y, formed in fixed point as the table will hold it
-                const long long K = psi_k(s_kij);
-                const long long piy = s_c.pi[first] + (psi_c(t.qj_new, K) - psi_c(t.qj_old, K));
-                const long long pjy = s_c.pj[first] + (psi_c(t.qi_new, K) - psi_c(t.qi_old, K));
-                const double Fy = -lin - m.selfq - 2.0 * wsum + m.perfc + m.perf - m.corr - (double)s_c.sf[first] +
-                                  psi_term(-m.dqi, piy, -m.dqj, pjy);
+                double Fy;
+                if (A.use_psi) {
+                    // its real-space term from psi at y, formed in fixed point as the table will hold it;
+                    // the pair term dq_i dq_j k_ij is even in the move
+                    const long long K = psi_k(s_kij);
+                    const long long piy = s_c.pi[first] + (psi_c(t.qj_new, K) - psi_c(t.qj_old, K));
+                    const long long pjy = s_c.pj[first] + (psi_c(t.qi_new, K) - psi_c(t.qi_old, K));
+                    Fy = -lin - m.selfq - 2.0 * wsum + m.perfc + m.perf - m.corr - (double)s_c.sf[first] +
+                         psi_term(-m.dqi, piy, -m.dqj, pjy);
+                } else {   // the pair term (q_i' q_j' - q_i q_j) k_ij is odd in the move
+                    Fy = -lin - m.selfq - 2.0 * wsum - m.perfc + m.perf - m.corr - (double)s_c.sf[first];
+                }
                 double e2;
                 {
 #pragma clang fp contract(off)
@@ -1248,7 +1258,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             const bool accepted = s_dc.acc_k == 0;
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
-            if (accepted) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
+            if (accepted && A.use_psi) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
             if (accepted && A.upd_one) {
                 const TrialT<P>& t = s_tr[0];
                 da_accept_update<PoolT>(A, D, pbuf, s_dc, s_hrow, A.species[t.ti_old], A.species[t.ti_new],
@@ -1752,8 +1762,11 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         A.rec_fy = S->rec_fy.p;
         S->rec.alloc(std::max(N, 1));
         A.rec = S->rec.p;
-        S->psi.alloc(std::max(N, 1));
-        A.psi = S->psi.p;
+        A.use_psi = d->real_space ? 1 : 0;
+        if (A.use_psi) {
+            S->psi.alloc(std::max(N, 1));
+            A.psi = S->psi.p;
+        }
         if (N > 0)
             hipLaunchKernelGGL(k_rec_static, dim3(ceil_div(N, 256)), dim3(256), 0, stream(), N, M->atoms->x.p,
                                M->atoms->y.p, M->atoms->z.p, M->selfw.p, S->rec.p);
@@ -1860,9 +1873,11 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
         upload_views(M, S, s);
         refresh(M);   // the surrogate needs phi to hold every accepted change
         phi_dot(M, S);
-        // psi from the current charges (identities may have changed since the last run)
-        hipLaunchKernelGGL(k_psi_all, dim3(ceil_div(M->atoms->n, 256)), dim3(256), 0, stream(), M->dev, S->psi.p);
-        PHB_HIP(hipGetLastError());
+        if (S->dev.use_psi) {   // psi from the current charges (identities may have changed since the last run)
+            hipLaunchKernelGGL(k_psi_all, dim3(ceil_div(M->atoms->n, 256)), dim3(256), 0, stream(), M->dev,
+                               S->psi.p);
+            PHB_HIP(hipGetLastError());
+        }
         McCtl c = read_ctl(M);
         const long long target = c.trials_done + ntrials;
         long long done = c.trials_done;

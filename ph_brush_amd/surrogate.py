@@ -53,6 +53,7 @@ class CollectSpec:
     between: int = 2000            # plain trials between sampled states
     seed: int = 1
     r_desc: float = 6.0
+    real_space: bool = False       # physics part with the neighbours' real-space potential (da_attach)
 
 
 def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=None) -> dict:
@@ -64,7 +65,7 @@ def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=N
     for k, ph in enumerate(spec.ph_values):
         mc = S.McParams.at_ph(ph, spec.pka)
         rep = S.Replica(s, p, mc, seed=spec.seed + 1000 * k)
-        rep.mc.da_attach(r_desc=spec.r_desc)
+        rep.mc.da_attach(r_desc=spec.r_desc, real_space=spec.real_space)
         rep.mc.run(spec.equilibrate)
         got = 0
         while got < spec.samples_per_ph:

@@ -482,10 +482,11 @@ class MonteCarlo(_Handle):
 
 
     # ---- delayed acceptance (no reference counterpart)
-    def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64):
+    def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64, real_space: bool = False):
         sp = np.ascontiguousarray(default_species(NTYPES) if species is None else species, np.int8)
         self._species = sp
-        d = L.DaDesc(sp.ctypes.data_as(C.POINTER(C.c_byte)), len(sp), float(r_desc), int(candidates))
+        d = L.DaDesc(sp.ctypes.data_as(C.POINTER(C.c_byte)), len(sp), float(r_desc), int(candidates),
+                     int(bool(real_space)))
         L.call("phb_da_attach", self.h, C.byref(d))
 
     def da_descriptors(self, i, j):

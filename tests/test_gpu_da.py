@@ -176,8 +176,8 @@ def test_da_chain_replays_on_oracle(gpu):
     assert abs(e_run - r.mc.recompute()) <= 1e-9 * max(1.0, abs(e_run))
 
 
-@pytest.mark.parametrize("refresh", ["auto", "kspace"])
-def test_da_stage1_values_are_functions_of_the_state(gpu, refresh):
+@pytest.mark.parametrize("refresh,real_space", [("auto", False), ("kspace", False), ("auto", True)])
+def test_da_stage1_values_are_functions_of_the_state(gpu, refresh, real_space):
     """With a network that outputs zero, the stage-1 value dE*(x->y) the chain
     records is the physics part alone; at the state the trial saw (rebuilt from
     the record, recomputed from scratch) it equals the unchained evaluation
@@ -186,13 +186,15 @@ def test_da_stage1_values_are_functions_of_the_state(gpu, refresh):
     equals the unchained evaluation of the reverse move at y, so stage 2 sees the
     surrogate as a function of the state (detailed balance).  "auto" takes the
     direct refresh with the partial stencil sums from the refresh launch;
-    "kspace" the k-space refresh and a separate stencil pass."""
+    "kspace" the k-space refresh and a separate stencil pass.  With real_space the
+    physics part also holds the neighbours' real-space potential (kept by the chain
+    in fixed point), and the reverse value is formed from it at y."""
     s, p = _tiny()
     mc = S.McParams.at_ph(7.9)
     n = 1500
     r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=10 ** 9, record_cap=n,
                   refresh=L.PHB_REFRESH_AUTO if refresh == "auto" else L.PHB_REFRESH_KSPACE)
-    r.mc.da_attach(r_desc=6.0)
+    r.mc.da_attach(r_desc=6.0, real_space=real_space)
     sur = S.Surrogate(0)
     theta = np.zeros_like(sur.get_params())
     theta[-1] = 1.0          # output std (> 0); every weight zero, so the network outputs exactly 0
