@@ -42,11 +42,14 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_pair_energy(PairArgs A, double* __restrict__ partials) {
     __shared__ double sred[16];
     double e_lj = 0.0, e_c = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.N; i += gridDim.x * blockDim.x) {
+    // 8 lanes per row (lane l takes the entries l, l + 8, ... of the row): enough
+    // independent gathers in flight at every N; the sums keep a fixed order
+    const int l = threadIdx.x & 7;
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; i < A.N; i += (gridDim.x * blockDim.x) >> 3) {
         const double xi = A.x[i], yi = A.y[i], zi = A.z[i], qi = A.q[i];
         const int ti = A.type[i];
         T row_lj = 0, row_c = 0;
-        for (int k = A.head[i], ke = A.head[i + 1]; k < ke; ++k) {
+        for (int k = A.head[i] + l, ke = A.head[i + 1]; k < ke; k += 8) {
             const int j = A.list[k];
             const double dx = min_image_exact(xi - A.x[j], A.Lx);
             const double dy = min_image_exact(yi - A.y[j], A.Ly);
@@ -107,19 +110,21 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 
 // Order-p B-spline spreading (MC/src/pme.cu:140-172 at p = 4) with hardware
 // floating-point atomics; atoms with q = 0 contribute nothing and are skipped.
+// One thread per (atom, x offset a): P^2 atomics each, so that the atomics of
+// many atoms are in flight at once.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ z, const double* __restrict__ q, int N,
                                                 GridGeom g, T* __restrict__ grid) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = g.order, off = stencil_offset(P);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / P, a = t - i * P;
     if (i >= N) return;
     const double qi = q[i];
     if (qi == 0.0) return;
     int i0[3];
     T w[3][8];
     stencil_of(x[i], y[i], z[i], g, i0, w);
-    const int P = g.order, off = stencil_offset(P);
-    for (int a = 0; a < P; ++a) {
+    {
         const int ixa = wrap(i0[0] + a - off, g.nx);
         for (int b = 0; b < P; ++b) {
             const int iyb = wrap(i0[1] + b - off, g.ny);
@@ -308,12 +313,12 @@ void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
     if (P->precision == PHB_FP64) {
         hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, st, P->grid64.p, G, 0.0);
         if (a->n > 0)
-            hipLaunchKernelGGL(k_spread<double>, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
+            hipLaunchKernelGGL(k_spread<double>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
                                a->q.p, a->n, P->geom, P->grid64.p);
     } else {
         hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, st, P->grid32.p, G, 0.0f);
         if (a->n > 0)
-            hipLaunchKernelGGL(k_spread<float>, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
+            hipLaunchKernelGGL(k_spread<float>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
                                a->q.p, a->n, P->geom, P->grid32.p);
     }
     PHB_HIP(hipGetLastError());
@@ -406,7 +411,7 @@ void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const 
     PHB_HIP(hipGetLastError());
 }
 
-int pair_blocks(int N) { return std::max(1, std::min(ceil_div(N, 256), 2048)); }
+int pair_blocks(int N) { return std::max(1, std::min(ceil_div(8 * N, 256), 2048)); }   // 8 lanes per row
 
 void PmeKernels::launch_moments(const phb_atoms_s* a, double* partials, int nblocks) {
     hipLaunchKernelGGL(k_charge_moments, dim3(nblocks), dim3(256), 0, stream(), a->q.p, a->z.p, a->n, partials);
