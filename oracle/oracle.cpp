@@ -1014,6 +1014,13 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
     double E = or_total_energy(&s, p, head, list, nullptr);
     std::mt19937 rng(seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
+    // the delayed-acceptance chain's own pool order (no reference counterpart): an
+    // accepted atom leaves its pool by taking the pool's last entry into its place
+    auto swap_remove = [](std::vector<int>& v, int a) {
+        auto it = std::find(v.begin(), v.end(), a);
+        *it = v.back();
+        v.pop_back();
+    };
     auto choose_one = [&](const std::vector<int>& pool) {
         std::uniform_int_distribution<int> dd(0, (int)pool.size() - 1);
         return pool[dd(rng)];
@@ -1064,11 +1071,11 @@ int or_da_run(const or_system* s0, const or_params* p, const or_mc_params* m, co
                 stage = 1;
                 E = E_trial;
                 if (forward) {
-                    nh_n.erase(std::find(nh_n.begin(), nh_n.end(), i)); nh_p.push_back(i);
-                    w.erase(std::find(w.begin(), w.end(), j)); cl.push_back(j);
+                    swap_remove(nh_n, i); nh_p.push_back(i);
+                    swap_remove(w, j); cl.push_back(j);
                 } else {
-                    nh_p.erase(std::find(nh_p.begin(), nh_p.end(), i)); nh_n.push_back(i);
-                    cl.erase(std::find(cl.begin(), cl.end(), j)); w.push_back(j);
+                    swap_remove(nh_p, i); nh_n.push_back(i);
+                    swap_remove(cl, j); w.push_back(j);
                 }
             } else {
                 stage = 2;

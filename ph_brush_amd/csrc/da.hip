@@ -749,37 +749,34 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
 }
 
 // ---------------------------------------------------------------- delayed-acceptance chain
-// The pools and the shell histograms after an accepted move, when the species a
-// site's change touches and those an ion's change touches are disjoint (no
-// histogram counter is written by both rows): the pool shifts of pool_update_at
-// and both rows' histogram read-modify-writes go in one load phase and one store
-// phase.  hrow: the rows' bounds in dlist (i's, then j's).
+// The accepted move's pool changes, O(1) per pool: the erased entry takes its
+// pool's last entry (the delayed-acceptance chain's own pool order, which the
+// oracle's restatement follows; the plain chain keeps the reference's
+// erase-and-shift order), and the atom joins its partner pool at the end.
+// Thread s < 2 handles the site pools (s = 0) or the ion pools (s = 1): each
+// pair of partner pools shares one region of slots, so one thread orders its
+// erase before its push.
 template <typename PoolT>
-__device__ void da_accept_update(const DaDev& A, const McDev& D, PoolT* pbuf, const DecisionLds& t,
-                                 const int* hrow, int so_i, int sn_i, int so_j, int sn_j) {
+__device__ __forceinline__ void da_pool_swap(const McDev& D, PoolT* pbuf, const DecisionLds& t, int tid) {
+    if (tid < 2) {
+        const int p = t.er_pool[tid], k = t.pos[tid], n = t.er_n[tid];
+        const PoolT last = pbuf[pool_slot(D, p, n - 1)];
+        pbuf[pool_slot(D, p, k)] = last;
+        pbuf[pool_slot(D, t.pb_pool[tid], t.pb_n[tid])] = (PoolT)t.pb_val[tid];
+    }
+}
+
+// The shell histograms after an accepted move, when the species a site's change
+// touches and those an ion's change touches are disjoint (no histogram counter
+// is written by both rows): both rows' read-modify-writes in one load phase and
+// one store phase.  hrow: the rows' bounds in dlist (i's, then j's).
+__device__ void da_hist_update(const DaDev& A, const int* hrow, int so_i, int sn_i, int so_j, int sn_j) {
     const int tid = threadIdx.x, bd = blockDim.x;
-    const int p0 = t.er_pool[0], p1 = t.er_pool[1];
-    const int a0 = t.pos[0] + 1, a1 = t.pos[1] + 1;
-    const int L0 = t.er_n[0] - a0, L = L0 + (t.er_n[1] - a1);
     const int hi = hrow[0], ni = so_i != sn_i ? hrow[1] - hrow[0] : 0;
     const int hj = hrow[2], nj = so_j != sn_j ? hrow[3] - hrow[2] : 0;
     const int NH = ni + nj;
-    constexpr int M = 16, MH = 4;
-    for (int base = 0, hb = 0; base < L || hb < NH; base += M * bd, hb += MH * bd) {
-        PoolT v[M];
-        int dst[M];
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int e = base + tid + m * bd;
-            dst[m] = -1;
-            v[m] = PoolT(0);
-            if (e < L) {
-                const bool first = e < L0;
-                const int pool = first ? p0 : p1, k = first ? a0 + e : a1 + (e - L0);
-                v[m] = pbuf[pool_slot(D, pool, k)];
-                dst[m] = pool_slot(D, pool, k - 1);
-            }
-        }
+    constexpr int MH = 4;
+    for (int hb = 0; hb < NH; hb += MH * bd) {
         unsigned short* hp[MH];
         int ho[MH], hn[MH], so[MH], sn[MH];
 #pragma unroll
@@ -798,20 +795,13 @@ __device__ void da_accept_update(const DaDev& A, const McDev& D, PoolT* pbuf, co
                 if (sn[m] >= 0) hn[m] = hp[m][sn[m]];
             }
         }
-        __syncthreads();   // every pool value read before any is moved
-#pragma unroll
-        for (int m = 0; m < M; ++m)
-            if (dst[m] >= 0) pbuf[dst[m]] = v[m];
 #pragma unroll
         for (int m = 0; m < MH; ++m)
             if (hp[m]) {
                 if (so[m] >= 0) hp[m][so[m]] = (unsigned short)(ho[m] - 1);
                 if (sn[m] >= 0) hp[m][sn[m]] = (unsigned short)(hn[m] + 1);
             }
-        __syncthreads();
     }
-    if (tid < 2) pbuf[pool_slot(D, t.pb_pool[tid], t.pb_n[tid])] = (PoolT)t.pb_val[tid];
-    __syncthreads();
 }
 
 // psi of the neighbours of i and j after an accepted move (integer atomics: the
@@ -1258,14 +1248,16 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             const bool accepted = s_dc.acc_k == 0;
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
-            if (accepted && A.use_psi) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
+            if (accepted) {
+                da_pool_swap<PoolT>(D, pbuf, s_dc, tid);
+                if (A.use_psi) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
+            }
             if (accepted && A.upd_one) {
                 const TrialT<P>& t = s_tr[0];
-                da_accept_update<PoolT>(A, D, pbuf, s_dc, s_hrow, A.species[t.ti_old], A.species[t.ti_new],
-                                        A.species[t.tj_old], A.species[t.tj_new]);
+                da_hist_update(A, s_hrow, A.species[t.ti_old], A.species[t.ti_new], A.species[t.tj_old],
+                               A.species[t.tj_new]);
             } else if (accepted) {
                 const TrialT<P>& t = s_tr[0];
-                pool_update_at<PoolT>(D, pbuf, s_dc);
                 for (int side = 0; side < 2; ++side) {
                     const int atom = side ? t.j : t.i;
                     const int so = A.species[side ? t.tj_old : t.ti_old], sn = A.species[side ? t.tj_new : t.ti_new];
@@ -1279,6 +1271,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     __syncthreads();
                 }
             }
+            if (accepted) __syncthreads();   // the pool swap (LDS pools are copied out below)
             clk.lap(prof, s_prof, 6);
             if (accepted) break;
         }
