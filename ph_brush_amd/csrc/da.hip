@@ -1,8 +1,8 @@
 // da.hip -- learned surrogate for the move's Delta-E and the delayed-acceptance
 // chain.  No counterpart in the reference (it has no learned part); the chain
-// keeps the proposal, pools and draw order of the trial loop of
-// phase_gc_mu_control (MC/src/mc.cu:379-503) so that the exact Delta-E stays
-// the judge of every accepted move.
+// keeps the proposal rule and draw order of the trial loop of
+// phase_gc_mu_control (MC/src/mc.cu:379-503), and the exact Delta-E decides
+// every accepted move.
 //
 // Descriptor of a move (site i, ion j): d = h_i - h_j, 48 shell counts per
 // centre (12 shells of r_desc/12 x 4 species), each centre excluding the other
