@@ -36,20 +36,25 @@ struct PairArgs {
     double lj_rc2, kf, alpha;
 };
 
-// One thread per row i (grid-stride); FP64 per-thread sums (T = double) or
-// FP32 row sums (T = float) folded into FP64, block partials in FP64.
+constexpr int PAIR_LANES = 16;   // lanes per neighbour row in the pair sums (power of 2)
+
+// PAIR_LANES lanes per row i (grid-stride over rows); FP64 per-lane sums
+// (T = double) or FP32 row sums (T = float) folded into FP64, block partials in FP64.
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_pair_energy(PairArgs A, double* __restrict__ partials) {
     __shared__ double sred[16];
     double e_lj = 0.0, e_c = 0.0;
-    // 8 lanes per row (lane l takes the entries l, l + 8, ... of the row): enough
-    // independent gathers in flight at every N; the sums keep a fixed order
-    const int l = threadIdx.x & 7;
-    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; i < A.N; i += (gridDim.x * blockDim.x) >> 3) {
+    // PAIR_LANES lanes per row (lane l takes the entries l, l + PAIR_LANES, ... of
+    // the row): enough independent gathers in flight at every N; the sums keep a
+    // fixed order
+    const int l = threadIdx.x & (PAIR_LANES - 1);
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / PAIR_LANES; i < A.N;
+         i += (gridDim.x * blockDim.x) / PAIR_LANES) {
         const double xi = A.x[i], yi = A.y[i], zi = A.z[i], qi = A.q[i];
         const int ti = A.type[i];
         T row_lj = 0, row_c = 0;
-        for (int k = A.head[i] + l, ke = A.head[i + 1]; k < ke; k += 8) {
+        for (int k = A.head[i] + l, ke = A.head[i + 1]; k < ke; k += PAIR_LANES) {
             const int j = A.list[k];
             const double dx = min_image_exact(xi - A.x[j], A.Lx);
             const double dy = min_image_exact(yi - A.y[j], A.Ly);
@@ -411,7 +416,7 @@ void PmeKernels::launch_pair(const phb_atoms_s* a, const phb_params_s* p, const 
     PHB_HIP(hipGetLastError());
 }
 
-int pair_blocks(int N) { return std::max(1, std::min(ceil_div(8 * N, 256), 2048)); }   // 8 lanes per row
+int pair_blocks(int N) { return std::max(1, std::min(ceil_div(PAIR_LANES * N, 256), 2048)); }
 
 void PmeKernels::launch_moments(const phb_atoms_s* a, double* partials, int nblocks) {
     hipLaunchKernelGGL(k_charge_moments, dim3(nblocks), dim3(256), 0, stream(), a->q.p, a->z.p, a->n, partials);
