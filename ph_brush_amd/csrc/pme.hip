@@ -116,14 +116,18 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 // Order-p B-spline spreading (MC/src/pme.cu:140-172 at p = 4) with hardware
 // floating-point atomics; atoms with q = 0 contribute nothing and are skipped.
 // One thread per (atom, x offset a): P^2 atomics each, so that the atomics of
-// many atoms are in flight at once.
+// many atoms are in flight at once.  Atoms are visited in the order k * stride
+// mod N (stride coprime to N, a bijection): consecutive atoms are bonded
+// neighbours whose stencils overlap, and a wave of them would queue its
+// atomics on the same grid points.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ z, const double* __restrict__ q, int N,
-                                                GridGeom g, T* __restrict__ grid) {
+                                                int stride, GridGeom g, T* __restrict__ grid) {
     const int P = g.order, off = stencil_offset(P);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, i = t / P, a = t - i * P;
-    if (i >= N) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = t / P, a = t - k * P;
+    if (k >= N) return;
+    const int i = (int)(((long long)k * stride) % N);
     const double qi = q[i];
     if (qi == 0.0) return;
     int i0[3];
@@ -311,6 +315,15 @@ phb_pme_s::~phb_pme_s() = default;
 
 namespace phb {
 
+// A stride coprime to n near sqrt(n) * 31 (1 when n is small).
+static int spread_stride(int n) {
+    if (n < 64) return 1;
+    auto gcd = [](long long u, long long v) { while (v) { const long long w = u % v; u = v; v = w; } return u; };
+    long long s = (long long)std::sqrt((double)n) * 31 + 1;
+    while (gcd(s, n) != 1) ++s;
+    return (int)(s % n == 0 ? 1 : s % n);
+}
+
 void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
     hipStream_t st = stream();
     const size_t G = (size_t)P->nx * P->ny * P->nz;
@@ -319,12 +332,12 @@ void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
         hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, st, P->grid64.p, G, 0.0);
         if (a->n > 0)
             hipLaunchKernelGGL(k_spread<double>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
-                               a->q.p, a->n, P->geom, P->grid64.p);
+                               a->q.p, a->n, spread_stride(a->n), P->geom, P->grid64.p);
     } else {
         hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, st, P->grid32.p, G, 0.0f);
         if (a->n > 0)
             hipLaunchKernelGGL(k_spread<float>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
-                               a->q.p, a->n, P->geom, P->grid32.p);
+                               a->q.p, a->n, spread_stride(a->n), P->geom, P->grid32.p);
     }
     PHB_HIP(hipGetLastError());
 }
