@@ -844,7 +844,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     // B operand reads fall into distinct banks.
     __shared__ __align__(16) unsigned short s_w1t[NN_H * XP];
     __shared__ __align__(16) unsigned short s_w2t[NN_H * NN_H];
-    __shared__ float s_b1[NN_H], s_b2[NN_H], s_w3[NN_H];
+    __shared__ float s_b1[NN_H], s_b2[NN_H], s_w3[NN_H], s_scale[NN_IN];
     __shared__ RngLds s_rng;
     __shared__ TrialT<P> s_tr[1];
     __shared__ DecisionLds s_dc;
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         const uint2* w2 = reinterpret_cast<const uint2*>(net.w2t);
         uint2 v1[3], v2[8];
         unsigned mt[2];
-        float b1 = 0.f, b2 = 0.f, w3 = 0.f;
+        float b1 = 0.f, b2 = 0.f, w3 = 0.f, sc = 0.f;
 #pragma unroll
         for (int m = 0; m < 3; ++m) v1[m] = w1[tid + m * DA_THREADS];
 #pragma unroll
@@ -892,6 +892,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             b2 = net.theta[OFF_B2 + tid];
             w3 = bf2f(net.w3[tid]);
         }
+        if (tid < NN_IN) sc = net.theta[OFF_SCALE + tid];
         target = C->trials_target;
         rec_cap = C->rec_cap;
         const long long done_in = C->trials_done;
@@ -929,6 +930,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             s_b2[tid] = b2;
             s_w3[tid] = w3;
         }
+        if (tid < NN_IN) s_scale[tid] = sc;
     }
     if (tid == 0) {
         s_st.done = C->trials_done;
@@ -1046,7 +1048,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 float v = 0.f;
                 if (k < ntr) {
                     const int d = raw[q] - (c == s_c.exA[k] ? 1 : 0) + (c == s_c.exB[k] ? 1 : 0);
-                    v = (float)d * net.theta[OFF_SCALE + c];
+                    v = (float)d * s_scale[c];
                 }
                 s_X[k * XP + c] = f2bf(v);
             }
