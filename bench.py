@@ -167,7 +167,7 @@ def batched_form(S, args):
           file=sys.stderr, flush=True)
     tune = b.autotune(1)
     print(f"[bench] batched: autotune -> W={tune['spec_window']} pending={tune['pending_max']} "
-          f"pairs={tune['graph_pairs']}", file=sys.stderr, flush=True)
+          f"pairs={tune['graph_pairs']} launch_cap={tune['launch_cap']}", file=sys.stderr, flush=True)
     b.run(args.batch_warmup)
     st = b.run(args.batch_sweeps)
     rate = st["trials_per_s"]
@@ -192,7 +192,8 @@ def batched_form(S, args):
             "atoms": sum(r.s.natoms for r in reps), "sweeps_timed": args.batch_sweeps, "ms": st["ms"],
             "trials_timed": st["attempted"], "launches": st["launches"], "graph_replays": st["replays"],
             "autotuned": {"spec_window": tune["spec_window"], "pending_max": tune["pending_max"],
-                          "graph_pairs": tune["graph_pairs"], "trials_per_s": tune["trials_per_s"]},
+                          "graph_pairs": tune["graph_pairs"], "launch_cap": tune["launch_cap"],
+                          "trials_per_s": tune["trials_per_s"]},
             "alpha_by_replica": alphas, "alpha_curve_2640_serial": curve,
             "note": "alpha after warm-up + timed sweeps from p_prot = 0.5 starts (no equilibration beyond them)"}
 

@@ -403,20 +403,26 @@ typedef struct {
     int use_graph;             /* capture the pairs in a hipGraph (1) or launch them directly (0) */
     int spec_window;           /* out: current speculation window */
     int pending_max;           /* out: current pending-change budget */
+    long long launch_cap;      /* out: trials after which a chain launch stops per replica (0: none) */
 } phb_batch_desc;
 typedef struct {
     long long attempted, accepted;   /* totals over the replicas (since creation) */
     long long launches, replays;     /* chain launches and replays of this call */
     double ms, trials_per_s;         /* wall time of this call; autotune: rate of the chosen shape */
     int spec_window, pending_max, graph_pairs;
+    long long launch_cap;            /* a chain launch waits for its slowest replica: the trial cap bounds it */
 } phb_batch_stats;
 int phb_batch_create(const phb_mc *mcs, int n, const phb_batch_desc *d, phb_batch *out);
 int phb_batch_destroy(phb_batch b);
 /* `sweeps` sweeps: every replica attempts as many trials as it has titratable sites per sweep. */
 int phb_batch_run(phb_batch b, int sweeps, phb_batch_stats *st);
 /* Times speculation window x pending budget x graph length on the batch (each candidate runs
- * `sweeps_per_candidate` real sweeps) and keeps the fastest. */
+ * `sweeps_per_candidate` real sweeps), then per-launch trial caps on the fastest shape, and keeps
+ * the fastest.  Launch boundaries do not change any replica's sequence. */
 int phb_batch_autotune(phb_batch b, int sweeps_per_candidate, phb_batch_stats *best);
+/* Sets a launch shape without timing: speculation window (4, 8 or 16), pending budget (2..MC_KMAX, even),
+ * (chain, refresh) pairs per replay (>= 1) and the per-launch trial cap (0: none). */
+int phb_batch_tune(phb_batch b, int spec_window, int pending_max, int graph_pairs, long long launch_cap);
 
 #ifdef __cplusplus
 }

@@ -137,13 +137,15 @@ class DaStats(C.Structure):
 
 
 class BatchDesc(C.Structure):
-    _fields_ = [("graph_pairs", C.c_int), ("use_graph", C.c_int), ("spec_window", C.c_int), ("pending_max", C.c_int)]
+    _fields_ = [("graph_pairs", C.c_int), ("use_graph", C.c_int), ("spec_window", C.c_int), ("pending_max", C.c_int),
+                ("launch_cap", C.c_longlong)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("launches", C.c_longlong),
                 ("replays", C.c_longlong), ("ms", C.c_double), ("trials_per_s", C.c_double),
-                ("spec_window", C.c_int), ("pending_max", C.c_int), ("graph_pairs", C.c_int)]
+                ("spec_window", C.c_int), ("pending_max", C.c_int), ("graph_pairs", C.c_int),
+                ("launch_cap", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -229,6 +231,7 @@ SIGNATURES = {
     "phb_batch_destroy": [_h],
     "phb_batch_run": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_autotune": [_h, C.c_int, C.POINTER(BatchStats)],
+    "phb_batch_tune": [_h, C.c_int, C.c_int, C.c_int, C.c_longlong],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

@@ -93,9 +93,12 @@ __global__ void k_batch_status(const McDev* __restrict__ views, int R, long long
     out[3 * r + 2] = C->accepted;
 }
 
-__global__ void k_batch_tuning(const McDev* __restrict__ views, int R, int pending_max) {
+__global__ void k_batch_tuning(const McDev* __restrict__ views, int R, int pending_max, long long launch_cap) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R) views[r].ctl->pending_max = pending_max;
+    if (r < R) {
+        views[r].ctl->pending_max = pending_max;
+        views[r].ctl->launch_cap = launch_cap;
+    }
 }
 
 }  // namespace phb
@@ -283,7 +286,7 @@ void build_refresh_tables(phb_batch_s* B) {
     sync();
 }
 
-void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs) {
+void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long launch_cap) {
     for (phb_mc_s* M : B->mcs) {
         M->dev.W = W;
         M->d.pending_max = pending_max;
@@ -292,11 +295,13 @@ void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs) {
     upload_views(B);
     if (B->d.use_graph) drop_graph(B);   // the graph holds the old tables
     build_refresh_tables(B);
-    hipLaunchKernelGGL(k_batch_tuning, dim3(ceil_div(B->R, 64)), dim3(64), 0, stream(), B->views.p, B->R, pending_max);
+    hipLaunchKernelGGL(k_batch_tuning, dim3(ceil_div(B->R, 64)), dim3(64), 0, stream(), B->views.p, B->R, pending_max,
+                       launch_cap);
     PHB_HIP(hipGetLastError());
     B->d.graph_pairs = pairs;
     B->d.spec_window = W;
     B->d.pending_max = pending_max;
+    B->d.launch_cap = launch_cap;
     if (B->d.use_graph) capture(B);
     sync();
 }
@@ -397,7 +402,20 @@ extern "C" int phb_batch_run(phb_batch B, int sweeps, phb_batch_stats* st) {
             st->spec_window = B->d.spec_window;
             st->pending_max = B->d.pending_max;
             st->graph_pairs = B->d.graph_pairs;
+            st->launch_cap = B->d.launch_cap;
         }
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_batch_tune(phb_batch B, int spec_window, int pending_max, int graph_pairs, long long launch_cap) {
+    return guard([&] {
+        require(B, "phb_batch_tune: null batch");
+        require(spec_window == 4 || spec_window == 8 || spec_window == 16, "phb_batch_tune: window must be 4, 8 or 16");
+        require(pending_max >= 2 && pending_max <= MC_KMAX && pending_max % 2 == 0,
+                "phb_batch_tune: pending budget must be even, in [2, 64]");
+        require(graph_pairs >= 1 && launch_cap >= 0, "phb_batch_tune: bad graph length or launch cap");
+        apply_tuning(B, spec_window, pending_max, graph_pairs, launch_cap);
         return PHB_OK;
     });
 }
@@ -405,15 +423,19 @@ extern "C" int phb_batch_run(phb_batch B, int sweeps, phb_batch_stats* st) {
 extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_batch_stats* best) {
     return guard([&] {
         require(B && sweeps_per_candidate > 0, "phb_batch_autotune: bad arguments");
-        struct Cand { int W, pm, pairs; };
+        // stage 1: window x budget x graph length without a launch cap; stage 2: the
+        // per-launch trial cap (a launch waits for its slowest replica) with the best of stage 1
+        struct Cand { int W, pm, pairs; long long cap; };
         std::vector<Cand> cands;
         for (int W : {4, 8, 16})
             for (int pm : {4, 8, 16})
-                for (int pairs : {2, 4, 8}) cands.push_back({W, pm, pairs});
+                for (int pairs : {2, 4, 8}) cands.push_back({W, pm, pairs, 0});
         double best_rate = -1.0;
         Cand bc = cands[0];
-        for (const Cand& c : cands) {
-            apply_tuning(B, c.W, c.pm, c.pairs);
+        const size_t n1 = cands.size();
+        for (size_t ci = 0; ci < cands.size(); ++ci) {
+            const Cand c = cands[ci];
+            apply_tuning(B, c.W, c.pm, c.pairs, c.cap);
             read_status(B);
             long long t0 = 0;
             for (int r = 0; r < B->R; ++r) t0 += B->h_status[3 * r + 1];
@@ -426,13 +448,16 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
             for (int r = 0; r < B->R; ++r) t1 += B->h_status[3 * r + 1];
             const double rate = (double)(t1 - t0) / std::max(dt, 1e-9);
             if (rate > best_rate) { best_rate = rate; bc = c; }
+            if (ci + 1 == n1)   // stage 2: the caps, on the best shape of stage 1
+                for (long long cap : {16LL, 32LL, 64LL, 128LL, 256LL}) cands.push_back({bc.W, bc.pm, bc.pairs, cap});
         }
-        apply_tuning(B, bc.W, bc.pm, bc.pairs);
+        apply_tuning(B, bc.W, bc.pm, bc.pairs, bc.cap);
         if (best) {
             *best = phb_batch_stats{};
             best->spec_window = bc.W;
             best->pending_max = bc.pm;
             best->graph_pairs = bc.pairs;
+            best->launch_cap = bc.cap;
             best->trials_per_s = best_rate;
         }
         return PHB_OK;

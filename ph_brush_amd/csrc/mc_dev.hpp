@@ -39,6 +39,7 @@ struct McCtl {
     long long p2_done, p2_target, p2_acc;
     int p2_since, p2_stop;
     double p2_best;
+    long long launch_cap;             // > 0: a chain launch stops after this many trials (batched replicas)
 };
 
 struct McDev {

@@ -23,6 +23,10 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
     // a launch queued beyond the target (batches are sized ahead) leaves at once
     if (C->trials_done >= target || C->status != 0) return;
     const int pend_max = C->pending_max, ntg = C->nprot_target;
+    // the launch's own stop: the target, or launch_cap trials from here (a launch
+    // boundary never changes the sequence: the next launch continues the stream)
+    const long long stop_at = C->launch_cap > 0 && C->trials_done + C->launch_cap < target
+                                  ? C->trials_done + C->launch_cap : target;
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     if (LDS_POOLS)
         for (int k = tid; k < D.nsites + D.nions; k += blockDim.x) pbuf[k] = (PoolT)D.pbuf[k];
@@ -50,13 +54,13 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
             const long long done = s_st.done;
             const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
             int ntr = 0, stop = 0;
-            if (done >= target || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            if (done >= stop_at || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
             else if (ntg >= 0 && s_st.nprot == ntg) { status = 4; stop = 1; }
             else {
                 const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
                 if (wf == 0 && wb == 0) { status = 1; stop = 1; }
                 else {
-                    const long long room = target - done;
+                    const long long room = stop_at - done;
                     ntr = (int)(room < W ? room : W);
                     const double sum = (double)(wf + wb);
                     const bool both = wf > 0 && wb > 0;

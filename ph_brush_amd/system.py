@@ -601,7 +601,7 @@ class Batch(_Handle):
         super().__init__()
         self._keep = list(mcs)
         arr = (C.c_void_p * len(mcs))(*[m.h.value for m in mcs])
-        d = L.BatchDesc(graph_pairs, int(use_graph), 0, 0)
+        d = L.BatchDesc(graph_pairs, int(use_graph), 0, 0, 0)
         L.call("phb_batch_create", arr, len(mcs), C.byref(d), C.byref(self.h))
 
     def run(self, sweeps: int) -> dict:
@@ -613,6 +613,9 @@ class Batch(_Handle):
         st = L.BatchStats()
         L.call("phb_batch_autotune", self.h, sweeps_per_candidate, C.byref(st))
         return st.as_dict()
+
+    def tune(self, spec_window: int, pending_max: int, graph_pairs: int, launch_cap: int = 0):
+        L.call("phb_batch_tune", self.h, spec_window, pending_max, graph_pairs, launch_cap)
 
 
 class Replica:

@@ -66,6 +66,19 @@ def test_batch_equals_serial(gpu, use_graph):
     _check_against_serial(reps, batched, 4000)
 
 
+@pytest.mark.parametrize("launch_cap", [7, 40])
+def test_launch_cap_keeps_every_chain_exact(gpu, launch_cap):
+    """A per-launch trial cap (launches end early, mid-budget, some with no
+    acceptance) leaves every replica's sequence equal to its serial run."""
+    reps = _replicas(6000)
+    batched = _make(reps, 6000)
+    b = S.Batch([r.mc for r in batched], graph_pairs=2)
+    b.tune(8, 8, 2, launch_cap)
+    st = b.run(4)
+    assert st["launch_cap"] == launch_cap and st["launches"] > 0
+    _check_against_serial(reps, batched, 6000)
+
+
 def test_autotune_keeps_every_chain_exact(gpu):
     reps = _replicas(20000)
     batched = _make(reps, 20000)
@@ -73,5 +86,6 @@ def test_autotune_keeps_every_chain_exact(gpu):
     best = b.autotune(1)
     assert best["spec_window"] in (4, 8, 16) and best["pending_max"] in (4, 8, 16)
     assert best["graph_pairs"] in (2, 4, 8) and best["trials_per_s"] > 0
+    assert best["launch_cap"] in (0, 16, 32, 64, 128, 256)
     b.run(2)
     _check_against_serial(reps, batched, 20000)
