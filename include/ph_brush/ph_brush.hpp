@@ -14,6 +14,9 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
+#include <iomanip>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -26,7 +29,7 @@ namespace ph_brush {
 using real = double;
 
 inline void check(int rc) {
-    if (rc != PHB_OK) throw std::runtime_error(std::string("ph_brush: ") + phb_last_error());
+    if (rc != PHB_OK) throw std::runtime_error(phb_last_error());
 }
 
 // ---------------------------------------------------------------- host types
@@ -66,6 +69,9 @@ struct Params {
     int gc_block_fixed{2000};
     int sgmc_epoch_attempt_max{100000};
     real sgmc_nacc_frac_round{0.10};
+    int energy_interval{100};      // phase-2 report interval (MC/include/types.hpp:57)
+    bool phase2_wc{true};          // run phase 2 (MC/include/types.hpp:68)
+    std::vector<real> lj_shift_tbl;  // LJ(rc) per type pair when lj_shift (MC/include/types.hpp:40)
 };
 
 // build_bond_adjacency (MC/src/io.cpp:148-169)
@@ -329,21 +335,105 @@ public:
         r.cl_mc.assign(it, it + sz[3]);
         return r;
     }
+    // What the last phase1() did, block by block (the events its reference
+    // counterpart reports: direct flips, chunks, mu updates, target reached).
+    std::vector<phb_phase1_event> phase1_events() const {
+        int n = 0;
+        check(phb_phase1_events(h_, nullptr, 0, &n));
+        std::vector<phb_phase1_event> ev(n);
+        check(phb_phase1_events(h_, ev.data(), n, &n));
+        return ev;
+    }
     // phase2_swap_jitter's loop (MC/src/mc.cu:159-315) continuing this chain's
-    // engine: pools from the current identities, patience and max_attempts from hp.
-    Phase2Result phase2(const Params& hp) {
+    // engine: pools from the phase-1 hand-over (else the current identities),
+    // patience and max_attempts from hp.  With a trace, the first record_cap
+    // trials' outcomes are returned too (for per-interval reporting).
+    struct Phase2Trace {
+        int mode{0};                    // 4: 4-point, 2: W-Cl only, 0: no valid proposals
+        int stop{0};                    // 1: patience reached, 2: no valid proposals
+        int since_best{0};
+        real E0{0.0};                   // energy phase 2 started from
+        std::vector<int> atoms;         // 4 per trial (W-Cl only: jCl, jW, -1, -1)
+        std::vector<signed char> acc;
+        std::vector<double> dE;
+    };
+    Phase2Result phase2(const Params& hp, Phase2Trace* trace = nullptr, long long record_cap = 1LL << 22) {
         phb_phase2_desc d{};
-        d.patience = hp.patience; d.best_eps = 1e-9; d.record_cap = 0;
+        d.patience = hp.patience; d.best_eps = 1e-9;
+        d.record_cap = trace ? std::max(0LL, std::min<long long>(hp.max_attempts, record_cap)) : 0;
         phb_phase2_stats s{};
         check(phb_phase2_begin(h_, &d, &s));
+        const double E0 = s.E;
+        const int mode = s.mode;
         if (s.mode != 0) check(phb_phase2_run(h_, hp.max_attempts, &s));
         Phase2Result r;
         r.attempted = s.attempted; r.accepted = s.accepted; r.E = s.E; r.best_E = s.best_E;
+        if (trace) {
+            trace->mode = mode;
+            trace->stop = s.stop;
+            trace->since_best = s.since_best;
+            trace->E0 = E0;
+            const long long n = std::min(s.attempted, d.record_cap);
+            trace->atoms.assign(4 * (size_t)n, -1);
+            trace->acc.assign((size_t)n, 0);
+            trace->dE.assign((size_t)n, 0.0);
+            check(phb_phase2_record(h_, n, trace->atoms.data(), trace->acc.data(), trace->dE.data()));
+        }
         return r;
     }
 private:
     double mu_ = 0.0;
 };
+
+// MCPlan + build_plan (MC/include/mc.hpp:11-17, MC/src/mc.cu:67-122): the site
+// and W pools in atom order (W below z_cut = max site z + w_z_extra_nm), the
+// flips to the target, and the counts its summary reports.  Same errors.
+struct MCPlan {
+    std::vector<int> idx_nh_n, idx_nh_p, idx_w;
+    int to_flip{0};
+    real z_cut{0.0};
+    int nh_total{0}, cl_total{0};
+    real z_max_site{0.0};
+};
+
+inline MCPlan build_plan(const System& sys, const Params& p) {
+    MCPlan out;
+    for (int i = 0; i < sys.natoms; ++i) {
+        const int t = sys.atoms[i].type;
+        if (t == p.NH_N_type) out.idx_nh_n.push_back(i);
+        else if (t == p.NH_P_type) out.idx_nh_p.push_back(i);
+        else if (t == p.W_type) out.idx_w.push_back(i);
+        else if (t == p.Cl_type) ++out.cl_total;
+    }
+    out.nh_total = (int)(out.idx_nh_n.size() + out.idx_nh_p.size());
+    if (out.nh_total == 0) throw std::runtime_error("[plan] No NH sites found.");
+    const int target_count = (int)std::floor(p.target_protonation_pct / 100.0 * out.nh_total);
+    out.to_flip = target_count - (int)out.idx_nh_p.size();
+    if (out.to_flip < 0) throw std::runtime_error("[plan] Target < current; de-protonation not supported.");
+    if ((int)out.idx_nh_n.size() < out.to_flip) throw std::runtime_error("[plan] Not enough NH_N to flip.");
+    for (int i : out.idx_nh_n) out.z_max_site = std::max(out.z_max_site, sys.atoms[i].z);
+    for (int i : out.idx_nh_p) out.z_max_site = std::max(out.z_max_site, sys.atoms[i].z);
+    out.z_cut = out.z_max_site + p.w_z_extra_nm;
+    std::vector<int> below;
+    for (int j : out.idx_w) if (sys.atoms[j].z < out.z_cut) below.push_back(j);
+    out.idx_w.swap(below);
+    if ((int)out.idx_w.size() < out.to_flip) {
+        std::ostringstream m;
+        m << "[plan] Not enough W below z<" << std::setprecision(3) << out.z_cut << " nm: need " << out.to_flip
+          << ", have " << out.idx_w.size();
+        throw std::runtime_error(m.str());
+    }
+    return out;
+}
+
+// The device identities (type, charge) copied back into the host System, as
+// the reference's set_atom_identity keeps its host mirror (MC/src/mc.cu:19-31).
+inline void download_identities(const DeviceAtoms& a, System& sys) {
+    std::vector<double> q(sys.natoms);
+    std::vector<int> t0(sys.natoms);
+    check(phb_atoms_get_identity(a.get(), q.data(), t0.data()));
+    for (int i = 0; i < sys.natoms; ++i) { sys.atoms[i].q = q[i]; sys.atoms[i].type = t0[i] + 1; }
+}
 
 inline PhaseGCResult phase_gc_fixed_mu(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh,
                                        PMEPlan& plan, const Box& box, const Params& hp, long long ntrials) {

@@ -40,7 +40,7 @@ enum {
 };
 
 enum { PHB_FP64 = 0, PHB_FP32 = 1 };
-enum { PHB_COUL_CUT = 0, PHB_COUL_PME = 1 };
+enum { PHB_COUL_CUT = 0, PHB_COUL_PME = 1, PHB_COUL_UNKNOWN = 2 /* params.in value not recognised */ };
 
 const char *phb_last_error(void);
 int phb_abi_version(void);
@@ -253,6 +253,33 @@ int phb_phase1_run(phb_mc mc, const phb_phase1_desc *d, phb_phase1_result *out);
  * lists (capacity cap ints) when non-null, concatenated.  All empty before a
  * phase-1 run or when it ended without reaching its target. */
 int phb_phase1_handover(phb_mc mc, int sizes[4], int *lists, long long cap);
+/* What the last phb_phase1_run did, block by block: the events its reference
+ * counterpart reports as it runs (MC/src/mc.cu:153-155, :356-358, :399, :471-479,
+ * :508-516, :534-545), so that a caller can write the same report lines.
+ * Counts are of that run; mu_over_kT is beta * mu_eff at the event (after the
+ * update for PHB_P1_MU_UPDATE); E the chain energy at the event. */
+enum {
+    PHB_P1_DIRECT = 1,      /* direct flips applied: attempted = flips */
+    PHB_P1_AT_TARGET = 2,   /* already at the target: no trial made */
+    PHB_P1_CHUNK = 3,       /* a block of gc_block trials ended short of the target */
+    PHB_P1_MU_UPDATE = 4,   /* an epoch ended: mu -= kT eta logit_diff w_info */
+    PHB_P1_REACHED = 5,     /* the target was reached at trial step_in_block of a block */
+    PHB_P1_NO_PAIRS = 6     /* no valid pair to flip: the run stopped */
+};
+typedef struct {
+    int kind;
+    int epoch;                          /* 0-based epoch */
+    long long attempted, accepted;      /* of the run so far */
+    long long fwd_total, bwd_total;     /* accepted forward / backward moves of the run */
+    long long acc_since, att_since;     /* accepted / attempted since the last mu update */
+    long long block_attempted, block_accepted;
+    int step_in_block;                  /* PHB_P1_REACHED: trials of the block up to the target */
+    int early_stop;                     /* PHB_P1_MU_UPDATE: the epoch ended on epoch_attempt_max */
+    int nprot, nh, target_count, remaining, r0, nacc_target, amax, gc_block;
+    double mu_over_kT, E, f, w_info;
+} phb_phase1_event;
+/* n = events recorded (always); ev filled with the first min(n, cap) when non-null. */
+int phb_phase1_events(phb_mc mc, phb_phase1_event *ev, int cap, int *n);
 
 /* Phase 2: the 4-point and W-Cl-only move rule of phase2_swap_jitter
  * (MC/src/mc.cu:159-315) on an MC object, continuing its engine stream.  begin:
@@ -283,6 +310,72 @@ int phb_phase2_record(phb_mc mc, long long n, int *atoms, signed char *acc, doub
 
 /* Time every chain launch, refresh and recompute with HIP events (on = 1). */
 int phb_mc_set_profiling(phb_mc mc, int on);
+
+/* ------------------------------------------------------------ file formats
+ * The reference's on-disk inputs and outputs, host code (MC/include/io.hpp,
+ * MC/src/io.cpp).  A file the reference reads gives the same values here, and a
+ * file written here is byte-identical to the reference's from the same system.
+ * Errors carry the reference's messages (PHB_ERR_ARG). */
+typedef struct phb_data_s *phb_data;
+typedef struct {
+    int natoms, nbonds, ntypes;     /* header counts ("N atoms", "N bonds", "N atom types") */
+    int nbond_rows;                 /* Bonds rows kept: both atom ids known */
+    phb_box box;                    /* xhi-xlo, yhi-ylo, zhi-zlo */
+} phb_data_info;
+/* read_lammps_full (MC/include/io.hpp:7, MC/src/io.cpp:79-146): the header
+ * before "Atoms", the Atoms rows (id mol type q x y z, file order, types
+ * 1-based) and the Bonds rows (atom ids mapped to 0-based row indices). */
+int phb_data_read(const char *path, phb_data *out, phb_data_info *info);
+/* Copies the rows out: natoms entries per atom column, nbond_rows per bond
+ * column; a NULL column is skipped. */
+int phb_data_get(phb_data d, int *id, int *mol, int *type, double *q, double *x, double *y, double *z,
+                 int *bond_id, int *bond_type, int *bond_i, int *bond_j);
+int phb_data_destroy(phb_data d);
+typedef struct {
+    int natoms, nbonds, ntypes;     /* header counts written */
+    int natoms_rows, nbond_rows;    /* rows written */
+    const int *id, *mol, *type;     /* 1-based types */
+    const double *q, *x, *y, *z;
+    const int *bond_id, *bond_type, *bond_i, *bond_j;   /* bond_i/j: 0-based rows of the atom columns */
+} phb_data_view;
+/* write_lammps_full (MC/include/io.hpp:17, MC/src/io.cpp:326-368): box from 0 to
+ * box->L per axis (the data-file box, not one extended for the slab correction). */
+int phb_data_write(const char *path, const phb_data_view *v, const phb_box *box);
+/* wrap_positions (MC/include/io.hpp:14, MC/src/io.cpp:303-316): into [0, L) per axis. */
+int phb_wrap_positions(int n, double *x, double *y, double *z, const phb_box *box);
+/* Ewald alpha from a real-space tolerance as read_params derives it
+ * (MC/src/io.cpp:46-77): erfc^-1(clamp(rtol, 1e-12, 0.5)) / rc, rtol <= 0 read as 1e-5. */
+int phb_ewald_alpha_from_rtol(double rc, double rtol, double *alpha);
+
+/* params.in (read_params, MC/include/io.hpp:11, MC/src/io.cpp:171-301): "key =
+ * value" lines ('-' in a key read as '_', unknown keys ignored, '#' starts a
+ * comment line) and "lj ti tj eps sig" lines; then the reference's defaults,
+ * alpha from ewald_rtol when PME has no ewald_alpha, the LJ(rc) shift table and
+ * its range checks.  Two keys are this library's: lj_cutoff_nm (a separate LJ
+ * cutoff, 0: rc) and pme_grid = nx ny nz (0: from pme_spacing). */
+typedef struct {
+    double rc;
+    int ntypes;
+    int lj_shift;
+    int coulombtype;                /* PHB_COUL_CUT, PHB_COUL_PME or PHB_COUL_UNKNOWN */
+    double epsilon_r, ewald_rtol, ewald_alpha, pme_spacing;
+    int pme_order, pme_3dc;
+    double pme_3dc_zfac;
+    int nh_n_type, nh_p_type, w_type, cl_type;   /* 1-based */
+    double target_protonation_pct;
+    int rng_seed, energy_interval, patience, max_attempts;
+    double beta, w_z_extra_nm;
+    int direct_pct_threshold;
+    double mu_eta;
+    int gc_block_fixed, sgmc_epoch_attempt_max;
+    double sgmc_nacc_frac_round;
+    int phase2_wc;
+    double lj_rc;                   /* extension: lj_cutoff_nm */
+    int nx, ny, nz;                 /* extension: pme_grid */
+} phb_params_file;
+/* lj_eps, lj_sig, lj_shift_tbl: ntypes*ntypes row-major (ti-1)*ntypes+(tj-1), or NULL. */
+int phb_params_read(const char *path, int ntypes, phb_params_file *out, double *lj_eps, double *lj_sig,
+                    double *lj_shift_tbl);
 
 /* ----------------------------------------------------------- generator */
 /* Synthetic brush (stands in for the missing LAMMPS inputs). */

@@ -61,9 +61,16 @@ class OrDaRecord(C.Structure):
 _lib = None
 
 
+REF_SOURCES = Path("/root/reference")
+REF_IO_DUMP = HERE / "_ref" / "ref_io_dump"
+
+
 def build() -> Path:
-    """Compile liboracle.so with the committed Makefile (g++)."""
+    """Compile liboracle.so with the committed Makefile (g++), and, where the
+    reference sources are present, oracle/_ref/ref_io_dump from them."""
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    if (REF_SOURCES / "MC" / "src" / "io.cpp").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "-B", "ref", f"REF={REF_SOURCES}"], check=True)
     return LIB_PATH
 
 

@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libphb.so"
 
 PHB_OK = 0
 PHB_FP64, PHB_FP32 = 0, 1
-PHB_COUL_CUT, PHB_COUL_PME = 0, 1
+PHB_COUL_CUT, PHB_COUL_PME, PHB_COUL_UNKNOWN = 0, 1, 2
 PHB_REFRESH_AUTO, PHB_REFRESH_DIRECT, PHB_REFRESH_KSPACE = 0, 1, 2
 
 _ERRNAMES = {1: "ARG", 2: "HIP", 3: "ALLOC", 4: "STATE", 5: "UNSUPPORTED", 6: "INTERNAL"}
@@ -160,6 +160,43 @@ _pll = C.POINTER(C.c_longlong)
 _pf = C.POINTER(C.c_float)
 _ps = C.POINTER(C.c_short)
 
+class DataInfo(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("nbonds", C.c_int), ("ntypes", C.c_int), ("nbond_rows", C.c_int),
+                ("box", Box)]
+
+
+class DataView(C.Structure):
+    _fields_ = [("natoms", C.c_int), ("nbonds", C.c_int), ("ntypes", C.c_int), ("natoms_rows", C.c_int),
+                ("nbond_rows", C.c_int), ("id", _pi), ("mol", _pi), ("type", _pi), ("q", _pd), ("x", _pd),
+                ("y", _pd), ("z", _pd), ("bond_id", _pi), ("bond_type", _pi), ("bond_i", _pi), ("bond_j", _pi)]
+
+
+class ParamsFile(C.Structure):
+    _fields_ = [("rc", C.c_double), ("ntypes", C.c_int), ("lj_shift", C.c_int), ("coulombtype", C.c_int),
+                ("epsilon_r", C.c_double), ("ewald_rtol", C.c_double), ("ewald_alpha", C.c_double),
+                ("pme_spacing", C.c_double), ("pme_order", C.c_int), ("pme_3dc", C.c_int),
+                ("pme_3dc_zfac", C.c_double), ("nh_n_type", C.c_int), ("nh_p_type", C.c_int), ("w_type", C.c_int),
+                ("cl_type", C.c_int), ("target_protonation_pct", C.c_double), ("rng_seed", C.c_int),
+                ("energy_interval", C.c_int), ("patience", C.c_int), ("max_attempts", C.c_int),
+                ("beta", C.c_double), ("w_z_extra_nm", C.c_double), ("direct_pct_threshold", C.c_int),
+                ("mu_eta", C.c_double), ("gc_block_fixed", C.c_int), ("sgmc_epoch_attempt_max", C.c_int),
+                ("sgmc_nacc_frac_round", C.c_double), ("phase2_wc", C.c_int), ("lj_rc", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)]
+
+
+PHB_P1_DIRECT, PHB_P1_AT_TARGET, PHB_P1_CHUNK, PHB_P1_MU_UPDATE, PHB_P1_REACHED, PHB_P1_NO_PAIRS = 1, 2, 3, 4, 5, 6
+
+
+class Phase1Event(C.Structure):
+    _fields_ = [("kind", C.c_int), ("epoch", C.c_int), ("attempted", C.c_longlong), ("accepted", C.c_longlong),
+                ("fwd_total", C.c_longlong), ("bwd_total", C.c_longlong), ("acc_since", C.c_longlong),
+                ("att_since", C.c_longlong), ("block_attempted", C.c_longlong), ("block_accepted", C.c_longlong),
+                ("step_in_block", C.c_int), ("early_stop", C.c_int), ("nprot", C.c_int), ("nh", C.c_int),
+                ("target_count", C.c_int), ("remaining", C.c_int), ("r0", C.c_int), ("nacc_target", C.c_int),
+                ("amax", C.c_int), ("gc_block", C.c_int), ("mu_over_kT", C.c_double), ("E", C.c_double),
+                ("f", C.c_double), ("w_info", C.c_double)]
+
+
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 SIGNATURES = {
     "phb_last_error": [],
@@ -232,6 +269,14 @@ SIGNATURES = {
     "phb_batch_run": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_autotune": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_tune": [_h, C.c_int, C.c_int, C.c_int, C.c_longlong],
+    "phb_data_read": [C.c_char_p, C.POINTER(_h), C.POINTER(DataInfo)],
+    "phb_data_get": [_h, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi],
+    "phb_data_destroy": [_h],
+    "phb_data_write": [C.c_char_p, C.POINTER(DataView), C.POINTER(Box)],
+    "phb_wrap_positions": [C.c_int, _pd, _pd, _pd, C.POINTER(Box)],
+    "phb_ewald_alpha_from_rtol": [C.c_double, C.c_double, _pd],
+    "phb_params_read": [C.c_char_p, C.c_int, C.POINTER(ParamsFile), _pd, _pd, _pd],
+    "phb_phase1_events": [_h, C.POINTER(Phase1Event), C.c_int, _pi],
     "phb_gen_brush": [C.POINTER(BrushSpec), C.c_int, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi, C.c_int, _pi, _pi],
 }
 _RESTYPE = {"phb_last_error": C.c_char_p}

@@ -18,6 +18,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libphb.so"
+CLI = PKG / "bin" / "MC"
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
 
 ARCH = os.environ.get("PHB_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -64,9 +66,23 @@ def build(verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"[build] {LIB} ({len(objs)} objects, {ARCH})")
+        print(f"[build] {LIB} ({len(objs)} objects, {ARCH}); {CLI}")
     return LIB
+
+
+def build_cli() -> Path:
+    """The command-line program (host code over libphb.so): ph_brush_amd/bin/MC."""
+    CLI.parent.mkdir(exist_ok=True)
+    tmp = CLI.with_name(CLI.name + ".tmp")
+    cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(PKG / "cli" / "mc_main.cpp"),
+           f"-L{PKG}", "-lphb", "-Wl,-rpath,$ORIGIN/..", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

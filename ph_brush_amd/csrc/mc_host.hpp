@@ -43,6 +43,7 @@ struct phb_mc_s {
     // pools phase 1 hands to phase 2 (nh_n, nh_p, w, cl), when it ran
     std::vector<int> handover[4];
     bool has_handover = false;
+    std::vector<phb_phase1_event> p1_events;   // what the last phase-1 run did, block by block
     // delayed-acceptance state (descriptor lists and histograms) and phase-2 state, owned here
     void* da = nullptr;
     void (*da_free)(void*) = nullptr;

@@ -95,12 +95,31 @@ void handover_from_identities(phb_mc_s* M, const Ident& h) {
 
 }  // namespace
 
+// The counters and state every phase-1 event carries, relative to the run's start.
+static phb_phase1_event event_of(const McCtl& c, long long a0, long long acc0, long long f0, long long b0, int epoch,
+                                 int nh, int target_count, double mu_over_kT) {
+    phb_phase1_event e{};
+    e.epoch = epoch;
+    e.attempted = c.trials_done - a0;
+    e.accepted = c.accepted - acc0;
+    e.fwd_total = c.acc_fwd - f0;
+    e.bwd_total = c.acc_bwd - b0;
+    e.nprot = c.nprot;
+    e.nh = nh;
+    e.target_count = target_count;
+    e.remaining = std::max(0, target_count - c.nprot);
+    e.mu_over_kT = mu_over_kT;
+    e.E = c.E;
+    return e;
+}
+
 extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_result* out) {
     return guard([&] {
         require(M && d && out, "phb_phase1_run: null argument");
         McCtl c = read_ctl(M);
         require(c.status == 0, "phb_phase1_run: the chain is stopped (status != 0)", PHB_ERR_STATE);
         *out = phb_phase1_result{};
+        M->p1_events.clear();
         // ---- build_plan (MC/src/mc.cu:67-122)
         Ident h = identities(M);
         const int N = M->atoms->n;
@@ -151,6 +170,15 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
             full_recompute(M);   // potential grid, moments and energy of the new identities
             c = read_ctl(M);
             out->direct = 1;
+            if (to_flip > 0) {
+                phb_phase1_event e{};
+                e.kind = PHB_P1_DIRECT;
+                e.attempted = e.accepted = to_flip;
+                e.nprot = c.nprot; e.nh = nh_total; e.target_count = target_count;
+                e.mu_over_kT = M->d.beta * M->dev.mu;
+                e.E = c.E;
+                M->p1_events.push_back(e);
+            }
             out->attempted = to_flip;
             out->accepted = to_flip;
             out->Nprot = c.nprot;
@@ -191,6 +219,16 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
                     c = read_ctl(M);
                     acc_since += c.accepted - before.accepted;
                     att_since += c.trials_done - before.trials_done;
+                    {
+                        phb_phase1_event e = event_of(c, a0, acc0, f0, b0, epoch, nh_total, target_count, beta * mu);
+                        e.kind = c.status == 4 ? PHB_P1_REACHED : c.status != 0 ? PHB_P1_NO_PAIRS : PHB_P1_CHUNK;
+                        e.acc_since = acc_since; e.att_since = att_since;
+                        e.block_attempted = c.trials_done - before.trials_done;
+                        e.block_accepted = c.accepted - before.accepted;
+                        e.step_in_block = (int)e.block_attempted;
+                        e.r0 = R0; e.nacc_target = Nacc_target; e.amax = Amax; e.gc_block = blk;
+                        M->p1_events.push_back(e);
+                    }
                     if (c.status == 4) {
                         out->reached_target = 1;
                         handover_from_identities(M, identities(M));
@@ -210,6 +248,16 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
                 M->dev.mu = mu;
                 M->d.mu_eff = mu;
                 out->blocks_done = epoch + 1;
+                {
+                    phb_phase1_event e = event_of(c, a0, acc0, f0, b0, epoch, nh_total, target_count, beta * mu);
+                    e.kind = PHB_P1_MU_UPDATE;
+                    e.acc_since = acc_since; e.att_since = att_since;
+                    e.early_stop = att_since >= Amax && acc_since < Nacc_target;
+                    e.r0 = R0; e.nacc_target = Nacc_target; e.amax = Amax; e.gc_block = blk;
+                    e.f = f;
+                    e.w_info = w_info;
+                    M->p1_events.push_back(e);
+                }
             }
             hipLaunchKernelGGL(k_p1_control, dim3(1), dim3(64), 0, stream(), M->ctl.p, -1, 1);
             PHB_HIP(hipGetLastError());
@@ -217,6 +265,9 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
             c = read_ctl(M);
         } else {
             out->reached_target = 1;  // the reference returns before filling the hand-over (MC/src/mc.cu:346-353)
+            phb_phase1_event e = event_of(c, a0, acc0, f0, b0, 0, nh_total, target_count, beta * mu);
+            e.kind = PHB_P1_AT_TARGET;
+            M->p1_events.push_back(e);
         }
         out->attempted = c.trials_done - a0;
         out->accepted = c.accepted - acc0;
@@ -243,6 +294,15 @@ extern "C" int phb_phase1_handover(phb_mc M, int sizes[4], int* lists, long long
         long long o = 0;
         for (int k = 0; k < 4; ++k)
             for (int j = 0; j < sizes[k]; ++j) lists[o++] = M->handover[k][j];
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_phase1_events(phb_mc M, phb_phase1_event* ev, int cap, int* n) {
+    return guard([&] {
+        require(M && n && cap >= 0, "phb_phase1_events: bad arguments");
+        *n = (int)M->p1_events.size();
+        if (ev) std::copy(M->p1_events.begin(), M->p1_events.begin() + std::min(cap, *n), ev);
         return PHB_OK;
     });
 }

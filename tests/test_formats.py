@@ -1,0 +1,286 @@
+"""The reference's file formats: LAMMPS "full" data files and params.in
+(MC/src/io.cpp).  CPU only.
+
+Parity: the same canonical dump (oracle/io_dump.hpp) is made by this library's
+C++ layer (tests/cpp/io_dump.cpp over libphb.so) and by the reference's own
+io.cpp compiled into oracle/_ref/ref_io_dump (FP64 build; built where the
+reference sources are present, skipped otherwise); the dumps, exit codes and
+written data files must be byte-identical on every case below.  Independent
+pins: alpha against math.erfc, write -> read round trips, the wrap range, the
+Python mirror against the C++ dump.
+"""
+from __future__ import annotations
+
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from ph_brush_amd import _lib as L
+from ph_brush_amd import formats as F
+from ph_brush_amd import system as S
+
+ROOT = Path(__file__).resolve().parents[1]
+REF_DUMP = ROOT / "oracle" / "_ref" / "ref_io_dump"
+
+HEADER = """LAMMPS data file via write_data, version 2 Aug 2023, timestep = 0, units = real
+
+{natoms} atoms
+8 atom types
+{nbonds} bonds
+1 bond types
+
+-1.5 14.5 xlo xhi
+0 16 ylo yhi
+0.0 40.0 zlo zhi
+
+Masses
+
+1 1.0
+2 1.0
+
+"""
+
+ATOMS = """Atoms # full
+
+1 1 4 0.0 1.0 2.0 3.0
+2 1 5 1.0 -0.5 17.2 3.9 0 1 0
+3 1 4 0 1.5e0 2.25 41.5
+5 2 7 0.0 0.1 0.2 0.3
+4 2 8 -1.0 14.4999999 15.99999999999 39.99999999999999
+"""
+
+BONDS = """
+Velocities
+
+1 0 0 0
+
+Bonds
+
+1 1 1 2
+2 1 2 3
+3 1 3 9
+"""
+
+DATA = {
+    "mixed": HEADER.format(natoms=5, nbonds=3) + ATOMS + BONDS,
+    "crlf": (HEADER.format(natoms=5, nbonds=3) + ATOMS + BONDS).replace("\n", "\r\n"),
+    "tabs_in_rows": HEADER.format(natoms=5, nbonds=3) + (ATOMS + BONDS).replace(" ", "\t").replace("Atoms\t#", "Atoms #"),
+    "tabs_in_header": (HEADER.format(natoms=5, nbonds=3) + ATOMS + BONDS).replace(" ", "\t"),
+    "title_with_atoms": "1000 atoms is not a count here\n" + HEADER.format(natoms=5, nbonds=0) + ATOMS,
+    "no_types_header": HEADER.format(natoms=5, nbonds=0).replace("8 atom types\n", "") + ATOMS,
+    "angles_after_bonds": HEADER.format(natoms=5, nbonds=2) + ATOMS + "\nBonds\n\n1 1 1 2\n\nAngles\n\n1 1 1 2 3\n",
+    "err_no_atoms": HEADER.format(natoms=0, nbonds=0) + ATOMS,
+    "err_count": HEADER.format(natoms=6, nbonds=0) + ATOMS,
+    "err_stoi": HEADER.format(natoms=5, nbonds=0) + ATOMS.replace("5 2 7", "five 2 7"),
+    "err_stod": HEADER.format(natoms=5, nbonds=0) + ATOMS.replace("0.1 0.2 0.3", "x 0.2 0.3"),
+}
+
+PME = "cutoff_nm = 3.5\ncoulombtype = pme\nepsilon_r = 46.3118\nlj 1 1 1.0 1.0\nlj 4 5 0.5 0.9\nlj 8 8 1.0 1.0\n"
+
+PARAMS = {
+    "pme_rtol": PME + "ewald_rtol = 1e-5\npme_order = 5\npme_3dc = yes 3\ntarget_pct = 40\nphase2_WC = Off\n",
+    "pme_rtol_small": PME + "ewald_rtol = 3e-11\n",
+    "pme_rtol_large": PME + "ewald_rtol = 0.9\n",
+    "pme_rtol_zero": PME + "ewald_rtol = 0\n",
+    "pme_alpha": PME + "ewald_alpha = 0.9\npme_spacing = -1\npme_order = 0\n",
+    "coul_cut": "cutoff_nm = 2.5\ncoulombtype = Coul_Cut\nlj_shift = no\nlj 2 2 1 1\nbeta = 0.4\n",
+    "coul_unknown": PME.replace("pme", "reaction-field") + "ewald_alpha = 1\n",
+    "inline_comment": "cutoff_nm = 3.5 # nm\ncoulombtype = pme # keeps the comment\npatience = 12 # trials\n",
+    "keys_dashes": PME + "w-type = 6\nwater_type = 7\ndirect-protonation-pct-threshold = 50\nepoch_attempt_max = -4\n"
+                   "nacc_frac_round = 0\nmu_eta = -1\ngc_block_fixed = 0\nrng_seed = 77\nenergy-interval = 5\n",
+    "pme3dc_bare": PME + "pme_3dc =\n",
+    "pme3dc_no": PME + "pme_3dc = no\n",
+    "pme3dc_small": PME + "pme_3dc = on 1.5\n",
+    "pme3dc_word": PME + "pme_3dc = yes abc\n",
+    "lj_shift_word": PME + "lj_shift = maybe\nphase2_wc = 1\n",
+    "extension_keys": PME + "lj_cutoff_nm = 1.122462048309373\npme_grid = 32 32 128\n",
+    "unknown_keys": PME + "frobnicate = 3\n= 4\nlj_comment 1 2 = 3\n",
+    "err_bool": PME + "phase2_wc = maybe\n",
+    "err_lj_bounds": PME + "lj 9 1 1.0 1.0\n",
+    "err_direct": PME + "direct_pct_threshold = 5\n",
+    "err_type": PME + "nh_n_type = 9\n",
+    "err_cutoff": "coulombtype = pme\n",
+    "err_stoi": PME + "patience = abc\n",
+    "err_stod": PME + "beta = x\n",
+}
+
+
+@pytest.fixture(scope="module")
+def lib_dump(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("io") / "io_dump"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}",
+           f"-I{ROOT / 'oracle'}", str(ROOT / "tests" / "cpp" / "io_dump.cpp"), f"-L{ROOT / 'ph_brush_amd'}", "-lphb",
+           f"-Wl,-rpath,{ROOT / 'ph_brush_amd'}", "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def _run(exe, data, params, out):
+    r = subprocess.run([str(exe), str(data), str(params), str(out)], capture_output=True, text=True, timeout=60)
+    written = out.read_bytes() if out.exists() else None
+    return r.returncode, r.stdout, written
+
+
+def _files(tmp_path, dname, pname):
+    d = tmp_path / f"{dname}.data"
+    p = tmp_path / f"{pname}.params"
+    d.write_bytes(DATA[dname].encode())
+    p.write_bytes(PARAMS[pname].encode())
+    return d, p
+
+
+CASES = [(d, "pme_rtol") for d in DATA] + [("mixed", p) for p in PARAMS]
+
+
+@pytest.mark.skipif(not REF_DUMP.exists(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("dname,pname", CASES, ids=[f"{d}-{p}" for d, p in CASES])
+def test_dump_matches_reference_io(tmp_path, lib_dump, dname, pname):
+    d, p = _files(tmp_path, dname, pname)
+    mine = _run(lib_dump, d, p, tmp_path / "mine.out")
+    ref = _run(REF_DUMP, d, p, tmp_path / "ref.out")
+    assert mine[0] == ref[0]
+    assert mine[1] == ref[1]
+    assert mine[2] == ref[2]
+
+
+@pytest.mark.skipif(not REF_DUMP.exists(), reason="oracle/_ref not built (reference sources absent)")
+def test_generated_brush_round_trip_matches_reference_io(tmp_path, lib_dump):
+    """A 9-chain brush written by this library, read back by both readers."""
+    s = S.gen_brush(3, 3, 8, 6.0, 6.0, 30.0, n_perm=2, p_prot=0.5, n_salt=5, seed=4)
+    d = _brush_datafile(s)
+    path = tmp_path / "brush.data"
+    F.write_lammps_full(path, d)
+    prm = tmp_path / "p.params"
+    prm.write_text(PARAMS["pme_rtol"])
+    mine = _run(lib_dump, path, prm, tmp_path / "mine.out")
+    ref = _run(REF_DUMP, path, prm, tmp_path / "ref.out")
+    assert mine == ref
+    assert mine[0] == 0
+
+
+def _brush_datafile(s: S.System) -> F.DataFile:
+    n = s.natoms
+    nb = s.bonds.shape[0]
+    return F.DataFile(n, nb, 8, s.Lx, s.Ly, s.Lz_data, np.arange(1, n + 1, dtype=np.int32), s.mol.astype(np.int32),
+                      (s.type0 + 1).astype(np.int32), s.q.copy(), s.x.copy(), s.y.copy(), s.z.copy(),
+                      np.arange(1, nb + 1, dtype=np.int32), np.ones(nb, np.int32), s.bonds[:, 0].copy(),
+                      s.bonds[:, 1].copy())
+
+
+def test_round_trip_keeps_rows(tmp_path):
+    s = S.gen_brush(3, 3, 8, 6.0, 6.0, 30.0, n_perm=2, p_prot=0.5, n_salt=5, seed=4)
+    d = _brush_datafile(s)
+    F.write_lammps_full(tmp_path / "a.data", d)
+    e = F.read_lammps_full(tmp_path / "a.data")
+    assert (e.natoms, e.nbonds, e.ntypes) == (d.natoms, d.nbonds, 8)
+    assert (e.Lx, e.Ly, e.Lz) == (d.Lx, d.Ly, d.Lz)
+    for k in ("id", "mol", "type", "bond_id", "bond_type", "bond_i", "bond_j"):
+        np.testing.assert_array_equal(getattr(e, k), getattr(d, k))
+    for k in ("q", "x", "y", "z"):
+        np.testing.assert_allclose(getattr(e, k), getattr(d, k), rtol=0, atol=5e-9)   # 8 decimals written
+    # a second write of what was read is byte-identical (a fixed point of write o read)
+    F.write_lammps_full(tmp_path / "b.data", e)
+    F.write_lammps_full(tmp_path / "c.data", F.read_lammps_full(tmp_path / "b.data"))
+    assert (tmp_path / "b.data").read_bytes() == (tmp_path / "c.data").read_bytes()
+
+
+@pytest.mark.parametrize("rtol", [1e-3, 1e-5, 1e-8, 3e-11, 0.2])
+def test_alpha_solves_erfc(rtol):
+    rc = 3.5
+    a = F.ewald_alpha_from_rtol(rc, rtol)
+    assert math.erfc(a * rc) == pytest.approx(rtol, rel=1e-12)
+
+
+def test_alpha_clamps_and_defaults():
+    rc = 2.0
+    assert F.ewald_alpha_from_rtol(rc, 0.0) == F.ewald_alpha_from_rtol(rc, 1e-5)
+    assert F.ewald_alpha_from_rtol(rc, 0.9) == F.ewald_alpha_from_rtol(rc, 0.5)
+    assert F.ewald_alpha_from_rtol(rc, 1e-30) == F.ewald_alpha_from_rtol(rc, 1e-12)
+    assert math.erfc(F.ewald_alpha_from_rtol(rc, 0.9) * rc) == pytest.approx(0.5, rel=1e-12)
+
+
+def test_wrap_range_and_congruence(tmp_path):
+    rng = np.random.default_rng(5)
+    n = 2000
+    L_ = (7.0, 3.25, 40.0)
+    x, y, z = (rng.uniform(-3 * l, 4 * l, n) for l in L_)
+    x[:4] = [0.0, -0.0, L_[0], -L_[0]]
+    d = F.DataFile(n, 0, 1, *L_, np.arange(1, n + 1, dtype=np.int32), np.ones(n, np.int32), np.ones(n, np.int32),
+                   np.zeros(n), x.copy(), y.copy(), z.copy(), *(np.zeros(0, np.int32) for _ in range(4)))
+    F.wrap_positions(d)
+    for w, v, l in ((d.x, x, L_[0]), (d.y, y, L_[1]), (d.z, z, L_[2])):
+        assert np.all(w >= 0) and np.all(w < l)
+        k = (v - w) / l
+        np.testing.assert_allclose(k, np.round(k), atol=1e-9)
+        np.testing.assert_array_equal(w, S.wrap_positions(v, l))   # the numpy restatement used by the generator
+
+
+def test_python_mirror_matches_cpp_dump(tmp_path, lib_dump):
+    d, p = _files(tmp_path, "mixed", "pme_rtol")
+    code, out, _ = _run(lib_dump, d, p, tmp_path / "o.data")
+    assert code == 0
+    kv = {}
+    for line in out.splitlines():
+        k, _, v = line.partition(" ")
+        kv.setdefault(k, v)
+    df = F.read_lammps_full(d)
+    pf = F.read_params(p, df.ntypes)
+    assert float(kv["ewald_alpha"]) == pf.ewald_alpha
+    assert kv["pme_3dc"] == f"{pf.pme_3dc} {pf.pme_3dc_zfac:.17g}"
+    assert int(kv["phase2_wc"]) == pf.phase2_wc == 0
+    assert float(kv["target_protonation_pct"]) == pf.target_protonation_pct == 40.0
+    assert pf.coulombtype == L.PHB_COUL_PME
+    assert pf.lj_eps.reshape(8, 8)[3, 4] == pf.lj_eps.reshape(8, 8)[4, 3] == 0.5
+
+
+def test_extension_keys_and_load_run(tmp_path):
+    d, p = _files(tmp_path, "mixed", "extension_keys")
+    dfile, pf, s, prm, mc = F.load_run(d, p)
+    assert (pf.nx, pf.ny, pf.nz) == (32, 32, 128) and prm.grid == (32, 32, 128)
+    assert pf.lj_rc == prm.lj_rc == 1.122462048309373
+    assert s.Lz == dfile.Lz and s.Lz_data == 40.0          # no slab correction in this file
+    np.testing.assert_array_equal(s.type0, dfile.type - 1)
+    assert np.all(s.y < 16.0) and np.all(s.y >= 0)
+    assert mc.w_z_extra == 1.0 and mc.mu_eff == 0.0
+    d2, p2 = _files(tmp_path, "mixed", "pme_rtol")
+    _, pf2, s2, prm2, _ = F.load_run(d2, p2)
+    assert s2.Lz == 120.0 and prm2.grid is None and prm2.pme_order == 5
+
+
+@pytest.mark.parametrize("pname,msg", [
+    ("err_bool", "invalid boolean: maybe"),
+    ("err_lj_bounds", "LJ type index out of bounds"),
+    ("err_direct", "direct_pct_threshold must be between 10 and 100"),
+    ("err_type", "nh_n_type (9) must be between 1 and 8"),
+    ("err_cutoff", "cutoff_nm must be > 0 in params"),
+    ("err_stoi", "stoi"),
+    ("err_stod", "stod"),
+])
+def test_params_errors(tmp_path, pname, msg):
+    _, p = _files(tmp_path, "mixed", pname)
+    with pytest.raises(L.PhbError) as e:
+        F.read_params(p, 8)
+    assert e.value.code == 1 and str(e.value).endswith("] " + msg)
+
+
+@pytest.mark.parametrize("dname,msg", [
+    ("err_no_atoms", "No atoms in header"),
+    ("err_count", "Atom count mismatch in Atoms section"),
+    ("err_stoi", "stoi"),
+    ("err_stod", "stod"),
+])
+def test_data_errors(tmp_path, dname, msg):
+    d, _ = _files(tmp_path, dname, "pme_rtol")
+    with pytest.raises(L.PhbError) as e:
+        F.read_lammps_full(d)
+    assert str(e.value).endswith("] " + msg)
+
+
+def test_missing_files(tmp_path):
+    with pytest.raises(L.PhbError, match="Failed to open LAMMPS data: "):
+        F.read_lammps_full(tmp_path / "none.data")
+    with pytest.raises(L.PhbError, match="Failed to open params: "):
+        F.read_params(tmp_path / "none.params", 8)

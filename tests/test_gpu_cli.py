@@ -1,0 +1,139 @@
+"""The command-line program ph_brush_amd/bin/MC (`MC in.data params.in
+[--write-data out.data]`, MC/src/main.cu:68-229) end to end on the GPU,
+against the oracle's phase 1 and phase 2 on the same file contents.
+
+Bar: exit 0; phase1.data and phase2.data hold the oracle's identities after
+each phase (types and charges exact) and the input's wrapped positions;
+--write-data writes phase2.data's bytes; report.txt has the reference's lines,
+with the neighbour-pair count, plan counts, epochs and phase-2 step lines
+equal to the oracle's and the [summary] energies within the printed 6
+decimals.  Also the usage errors (exit 1) and a file error (exit 2).
+"""
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from ph_brush_amd import formats as F
+from ph_brush_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+MC = ROOT / "ph_brush_amd" / "bin" / "MC"
+
+
+def _mc_exe():
+    if not MC.exists():
+        pytest.fail(f"{MC} is missing: build with `python -m ph_brush_amd.build` (no fallback exists)")
+    return MC
+
+
+def _inputs(tmp_path, target_pct, patience=300, max_attempts=600, interval=50):
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
+    n, nb = s.natoms, s.bonds.shape[0]
+    d = F.DataFile(n, nb, 8, s.Lx, s.Ly, s.Lz_data, np.arange(1, n + 1, dtype=np.int32), s.mol.astype(np.int32),
+                   (s.type0 + 1).astype(np.int32), s.q.copy(), s.x.copy(), s.y.copy(), s.z.copy(),
+                   np.arange(1, nb + 1, dtype=np.int32), np.ones(nb, np.int32), s.bonds[:, 0].copy(),
+                   s.bonds[:, 1].copy())
+    data = tmp_path / "in.data"
+    F.write_lammps_full(data, d)
+    prm = tmp_path / "params.in"
+    lines = [f"cutoff_nm = {p.rc!r}", f"lj_cutoff_nm = {p.lj_rc!r}", "lj_shift = yes", "coulombtype = pme",
+             f"epsilon_r = {p.epsilon_r!r}", f"ewald_alpha = {p.ewald_alpha!r}", "pme_order = 5",
+             "pme_grid = 16 16 48", "pme_3dc = yes 3", "beta = 1.0", "w_z_extra_nm = 1000000",
+             f"target_pct = {target_pct}", "gc_block_fixed = 200", "sgmc_epoch_attempt_max = 1000",
+             "rng_seed = 5", f"patience = {patience}", f"max_attempts = {max_attempts}",
+             f"energy_interval = {interval}"]
+    lines += [f"lj {a} {b} 1.0 1.0" for a in range(1, 9) for b in range(a, 9)]
+    prm.write_text("\n".join(lines) + "\n")
+    return data, prm
+
+
+def _run(tmp_path, *args):
+    print(f"[cli] MC {' '.join(map(str, args))}", flush=True)
+    return subprocess.run([str(_mc_exe()), *map(str, args)], cwd=tmp_path, capture_output=True, text=True,
+                          timeout=300)
+
+
+def _oracle(data, prm, target_pct, patience, max_attempts):
+    _, pf, s, p, mc = F.load_run(data, prm)
+    bonds = S.bond_adjacency(s.natoms, s.bonds)
+    o = O.Oracle(s, p, bonds)
+    nnz = len(o.nlist()[1])
+    e0 = o.total_energy()[0]
+    ro = o.phase1_run(mc, pf.rng_seed, target_pct, gc_block=pf.gc_block_fixed,
+                      epoch_attempt_max=pf.sgmc_epoch_attempt_max, nacc_frac=pf.sgmc_nacc_frac_round,
+                      mu_eta=pf.mu_eta)
+    s1 = S.System(s.x, s.y, s.z, ro["q"], ro["type0"], s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data)
+    o2 = O.Oracle(s1, p, bonds)
+    o2.nlist()
+    r2 = o2.phase2_run(mc, ro["rng_state"], max_attempts, patience, pools=ro["handover"],
+                       pool_sizes=ro["handover_sizes"])
+    return s, nnz, e0, ro, r2
+
+
+def _identities(path):
+    d = F.read_lammps_full(path)
+    return d, (d.type - 1).astype(np.int32), d.q
+
+
+@pytest.mark.parametrize("target_pct", [75.0, 95.0], ids=["mu_control", "direct"])
+def test_mc_program_matches_oracle(tmp_path, gpu, target_pct):
+    data, prm = _inputs(tmp_path, target_pct)
+    r = _run(tmp_path, data, prm, "--write-data", "out.data")
+    assert r.returncode == 0, r.stderr
+    s, nnz, e0, ro, r2 = _oracle(data, prm, target_pct, 300, 600)
+    assert ro["rc"] == 0 and ro["reached_target"] == 1
+
+    d1, t1, q1 = _identities(tmp_path / "phase1.data")
+    assert np.array_equal(t1, ro["type0"]) and np.array_equal(q1, ro["q"])
+    d2, t2, q2 = _identities(tmp_path / "phase2.data")
+    assert np.array_equal(t2, r2["type0"]) and np.array_equal(q2, r2["q"])
+    for d in (d1, d2):
+        np.testing.assert_allclose(d.z, s.z, rtol=0, atol=5e-9)     # wrapped input positions, 8 decimals
+        assert (d.Lx, d.Ly, d.Lz) == (s.Lx, s.Ly, s.Lz_data)          # the data-file box, not the extended one
+    assert (tmp_path / "out.data").read_bytes() == (tmp_path / "phase2.data").read_bytes()
+
+    rep = (tmp_path / "report.txt").read_text()
+    assert "[3dc] Enabling slab correction: zfac=3, Lz: 16 -> 48 (nm)\n" in rep
+    assert f"Atoms: {s.natoms}\n" in rep
+    assert f"Neighbor pairs (half): {nnz}\n" in rep
+    assert re.search(r"PME grid: 16 x 16 x 48 order=5 alpha=", rep)
+    assert f"[NH] total={ro['Nh']}," in rep
+    assert "[write] phase1.data written\n" in rep and "[write] phase2.data written\n" in rep
+    assert "[write] Wrote updated data file to out.data\n" in rep
+    if ro["direct"]:
+        assert re.search(r"\[direct\] done: applied direct flips=\d+ E=", rep)
+    else:
+        assert len(re.findall(r"\[sgmc\] mu-update#", rep)) == ro["blocks_done"]
+        m = re.search(r"\[sgmc\] reached target in epoch=(\d+) step_in_block=\d+ Nprot=(\d+)/(\d+)", rep)
+        assert m and int(m.group(1)) == ro["blocks_done"] and int(m.group(2)) == ro["Nprot"]
+    steps = re.findall(r"\[phase2\] step=(\d+) E=(\S+) accepted=(\d+)", rep)
+    assert len(steps) == r2["n"] // 50
+    acc_cum = np.cumsum(r2["acc"])
+    e_cum = ro["E"] + np.cumsum(np.where(r2["acc"] != 0, r2["dE"], 0.0))
+    for st, e, a in steps:
+        assert int(a) == acc_cum[int(st) - 1]
+        assert abs(float(e) - e_cum[int(st) - 1]) <= 1e-6 + 1e-9 * abs(e_cum[int(st) - 1])
+    if r2["stop"] == 1:
+        assert "[phase2] patience reached: 300/300. Stop.\n" in rep
+    m = re.search(r"\[summary\] E0/after phase1/after phase2 = (\S+) / (\S+) / (\S+)", rep)
+    assert m
+    for got, want in zip(map(float, m.groups()), (e0, ro["E"], r2["E"])):
+        assert abs(got - want) <= 1e-6 + 1e-9 * abs(want)
+
+
+def test_mc_program_usage_and_errors(tmp_path, gpu):
+    r = _run(tmp_path)
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    data, prm = _inputs(tmp_path, 75.0)
+    r = _run(tmp_path, data, prm, "--write-data")
+    assert r.returncode == 1 and r.stderr == "--write-data requires a path\n"
+    r = _run(tmp_path, data, prm, "--bogus")
+    assert r.returncode == 1 and r.stderr == "Unrecognized argument: --bogus\n"
+    r = _run(tmp_path, tmp_path / "missing.data", prm)
+    assert r.returncode == 2 and r.stderr == f"Error: Failed to open LAMMPS data: {tmp_path / 'missing.data'}\n"
