@@ -132,7 +132,7 @@ def _files(tmp_path, dname, pname):
     return d, p
 
 
-CASES = [(d, "pme_rtol") for d in DATA] + [("mixed", p) for p in PARAMS]
+CASES = [(d, "pme_rtol") for d in DATA] + [("mixed", p) for p in PARAMS if p != "pme_rtol"]
 
 
 @pytest.mark.skipif(not REF_DUMP.exists(), reason="oracle/_ref not built (reference sources absent)")
