@@ -81,20 +81,43 @@ def refresh_bytes(rep, pending_max: int) -> float:
     return (16 * Gk + 8 * Gk) + (16 * Gk + 16 * G)
 
 
-def cpu_baseline(cfg, trials: int, threads: int):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, trials: int, threads: int, trials_one: int):
+    """The oracle's restatement of the reference algorithm (a full LJ + SPME
+    recompute per trial) on the host cores: `trials` trials on `threads`
+    threads, then `trials_one` on one thread."""
     from oracle import pyoracle as O
     from ph_brush_amd import system as S
     s, p, mc = cfg()
     o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
-    O.set_threads(threads)
     o.nlist()
-    o.total_energy()
-    t0 = time.perf_counter()
-    r = o.gc_run(mc, trials, 1, record=False)
-    dt = time.perf_counter() - t0
-    return {"value": r["n"] / dt, "unit": "trial moves/s", "cores": threads, "kind": "port",
-            "sample": f"{r['n']} GC trials of the same 44,000-atom workload, seed 1, reference algorithm "
-                      f"(full LJ + SPME recompute per trial, FP64, OpenMP) in {dt:.1f} s"}
+
+    def leg(n, nthreads):
+        O.set_threads(nthreads)
+        o.total_energy()
+        t0 = time.perf_counter()
+        r = o.gc_run(mc, n, 1, record=False)
+        return r["n"], time.perf_counter() - t0
+
+    n, dt = leg(trials, threads)
+    out = {"value": n / dt, "unit": "trial moves/s", "cores": threads, "kind": "port",
+           "sample": f"{n} GC trials of the same 44,000-atom workload, seed 1, reference algorithm "
+                     f"(full LJ + SPME recompute per trial, FP64, OpenMP) in {dt:.1f} s",
+           "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
+    if trials_one > 0:
+        n1, dt1 = leg(trials_one, 1)
+        out["one_thread"] = {"value": n1 / dt1, "sample": f"{n1} trials of the same chain on 1 thread in {dt1:.1f} s"}
+    return out
 
 
 def _timed(run, K, W, T, label):
@@ -434,7 +457,7 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         n = args.cpu_trials or 150
         print(f"[bench] cpu baseline: {n} trials on {threads} threads", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cfg, n, threads)
+        out["cpu_baseline"] = cpu_baseline(cfg, n, threads, max(1, n // 12))
     print(json.dumps(out))
 
 
