@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
+#include <functional>
 #include <iomanip>
 #include <sstream>
 #include <stdexcept>
@@ -304,13 +306,29 @@ public:
     // target, epochs of gc_block_fixed trials, stop at the target).  max_trials
     // bounds the run (0: none, as in the reference).  direct_pct_threshold < 0
     // forces the direct path, > 100 the mu-control loop.
-    PhaseGCResult phase1(const Params& hp, long long max_trials = 0, bool* direct = nullptr) {
+    // on_event, when given, sees each phase-1 event as it happens (phase1_events()
+    // lists them afterwards); an exception it throws is rethrown once the run returns.
+    PhaseGCResult phase1(const Params& hp, long long max_trials = 0, bool* direct = nullptr,
+                         const std::function<void(const phb_phase1_event&)>& on_event = {}) {
         phb_phase1_desc d{};
         d.target_pct = hp.target_protonation_pct; d.direct_pct_threshold = hp.direct_pct_threshold;
         d.gc_block = hp.gc_block_fixed; d.epoch_attempt_max = hp.sgmc_epoch_attempt_max;
         d.nacc_frac = hp.sgmc_nacc_frac_round; d.mu_eta = hp.mu_eta; d.max_trials = max_trials;
+        struct Observer {
+            const std::function<void(const phb_phase1_event&)>* f;
+            std::exception_ptr error;
+        } obs{&on_event, nullptr};
+        if (on_event) {
+            d.on_event = [](const phb_phase1_event* e, void* u) {
+                auto* o = static_cast<Observer*>(u);
+                if (o->error) return;
+                try { (*o->f)(*e); } catch (...) { o->error = std::current_exception(); }
+            };
+            d.user = &obs;
+        }
         phb_phase1_result s{};
         check(phb_phase1_run(h_, &d, &s));
+        if (obs.error) std::rethrow_exception(obs.error);
         if (s.status == 1) throw std::runtime_error("[sgmc] no valid pairs to flip; stop.");  // MC/src/mc.cu:399
         PhaseGCResult r;
         r.attempted = s.attempted; r.accepted = s.accepted;

@@ -222,37 +222,6 @@ int phb_mc_recompute(phb_mc mc, double *E);
 /* MT19937 state: 624 words then the index (625 unsigned). */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
 
-/* Phase 1 as main() runs it (MC/src/main.cu:180-197) on an MC object: build_plan's
- * checks (MC/src/mc.cu:67-122; errors as PHB_ERR_STATE with the reference's
- * message), then apply_direct_protonation (MC/src/mc.cu:124-157) when target_pct
- * > direct_pct_threshold, else phase_gc_mu_control (MC/src/mc.cu:320-546) on the
- * device chain: mu from the target fraction (clamped to +-10 kT), blocks of
- * gc_block trials, an epoch ends between blocks after Nacc_target = ceil(nacc_frac
- * * remaining) acceptances or epoch_attempt_max trials, then mu -= kT eta
- * logit_diff w_info; the run stops at the trial whose acceptance reaches the
- * target.  The pools handed to phase 2 are kept on the object (empty unless the
- * target was reached mid-run or the direct path ran, as in the reference).
- * max_trials bounds the run (0 = none; the reference's loop has no bound). */
-typedef struct {
-    double target_pct;         /* target_protonation_pct */
-    int direct_pct_threshold;  /* reference default 90 */
-    int gc_block;              /* gc_block_fixed (0 = 2000) */
-    int epoch_attempt_max;     /* sgmc_epoch_attempt_max (0 = 100000) */
-    double nacc_frac;          /* sgmc_nacc_frac_round (<= 0: 0.10) */
-    double mu_eta;             /* (<= 0: 0.3) */
-    long long max_trials;
-} phb_phase1_desc;
-typedef struct {
-    long long attempted, accepted, accepted_forward, accepted_backward;
-    int blocks_done, Nh, Nprot, target_count, reached_target, direct, status;
-    double E, mu_eff;
-} phb_phase1_result;
-int phb_phase1_run(phb_mc mc, const phb_phase1_desc *d, phb_phase1_result *out);
-/* The lists phase 1 handed to phase 2 (Phase1Result's nh_n_remaining, nh_p_mc,
- * w_remaining, cl_mc, MC/include/mc.hpp:19-27), in that order: sizes[4] always;
- * lists (capacity cap ints) when non-null, concatenated.  All empty before a
- * phase-1 run or when it ended without reaching its target. */
-int phb_phase1_handover(phb_mc mc, int sizes[4], int *lists, long long cap);
 /* What the last phb_phase1_run did, block by block: the events its reference
  * counterpart reports as it runs (MC/src/mc.cu:153-155, :356-358, :399, :471-479,
  * :508-516, :534-545), so that a caller can write the same report lines.
@@ -278,6 +247,41 @@ typedef struct {
     int nprot, nh, target_count, remaining, r0, nacc_target, amax, gc_block;
     double mu_over_kT, E, f, w_info;
 } phb_phase1_event;
+/* Phase 1 as main() runs it (MC/src/main.cu:180-197) on an MC object: build_plan's
+ * checks (MC/src/mc.cu:67-122; errors as PHB_ERR_STATE with the reference's
+ * message), then apply_direct_protonation (MC/src/mc.cu:124-157) when target_pct
+ * > direct_pct_threshold, else phase_gc_mu_control (MC/src/mc.cu:320-546) on the
+ * device chain: mu from the target fraction (clamped to +-10 kT), blocks of
+ * gc_block trials, an epoch ends between blocks after Nacc_target = ceil(nacc_frac
+ * * remaining) acceptances or epoch_attempt_max trials, then mu -= kT eta
+ * logit_diff w_info; the run stops at the trial whose acceptance reaches the
+ * target.  The pools handed to phase 2 are kept on the object (empty unless the
+ * target was reached mid-run or the direct path ran, as in the reference).
+ * max_trials bounds the run (0 = none; the reference's loop has no bound). */
+typedef struct {
+    double target_pct;         /* target_protonation_pct */
+    int direct_pct_threshold;  /* reference default 90 */
+    int gc_block;              /* gc_block_fixed (0 = 2000) */
+    int epoch_attempt_max;     /* sgmc_epoch_attempt_max (0 = 100000) */
+    double nacc_frac;          /* sgmc_nacc_frac_round (<= 0: 0.10) */
+    double mu_eta;             /* (<= 0: 0.3) */
+    long long max_trials;
+    /* optional: called on the host with each event as it happens (the events of
+     * phb_phase1_events, in order), e.g. to write report lines while the run goes on */
+    void (*on_event)(const phb_phase1_event *ev, void *user);
+    void *user;
+} phb_phase1_desc;
+typedef struct {
+    long long attempted, accepted, accepted_forward, accepted_backward;
+    int blocks_done, Nh, Nprot, target_count, reached_target, direct, status;
+    double E, mu_eff;
+} phb_phase1_result;
+int phb_phase1_run(phb_mc mc, const phb_phase1_desc *d, phb_phase1_result *out);
+/* The lists phase 1 handed to phase 2 (Phase1Result's nh_n_remaining, nh_p_mc,
+ * w_remaining, cl_mc, MC/include/mc.hpp:19-27), in that order: sizes[4] always;
+ * lists (capacity cap ints) when non-null, concatenated.  All empty before a
+ * phase-1 run or when it ended without reaching its target. */
+int phb_phase1_handover(phb_mc mc, int sizes[4], int *lists, long long cap);
 /* n = events recorded (always); ev filled with the first min(n, cap) when non-null. */
 int phb_phase1_events(phb_mc mc, phb_phase1_event *ev, int cap, int *n);
 

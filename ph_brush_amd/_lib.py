@@ -82,7 +82,7 @@ class McStats(C.Structure):
 class Phase1Desc(C.Structure):
     _fields_ = [("target_pct", C.c_double), ("direct_pct_threshold", C.c_int), ("gc_block", C.c_int),
                 ("epoch_attempt_max", C.c_int), ("nacc_frac", C.c_double), ("mu_eta", C.c_double),
-                ("max_trials", C.c_longlong)]
+                ("max_trials", C.c_longlong), ("on_event", C.c_void_p), ("user", C.c_void_p)]
 
 
 class Phase1Result(C.Structure):

@@ -51,46 +51,45 @@ void plan_summary(std::ostream& r, const MCPlan& plan, const Params& p) {
       << std::endl;
 }
 
-// The phase-1 lines from the chain's events.
-void phase1_lines(std::ostream& r, const std::vector<phb_phase1_event>& events) {
-    for (const phb_phase1_event& e : events) {
-        switch (e.kind) {
-            case PHB_P1_DIRECT:
-                r << std::setprecision(6) << "[direct] done: applied direct flips=" << e.attempted << " E=" << e.E
-                  << " remaining=" << 0 << std::endl;
-                break;
-            case PHB_P1_AT_TARGET:
-                r << "[sgmc] already at target: Nprot=" << e.nprot << "/" << e.nh << " target=" << e.target_count
-                  << " mu/kT=" << e.mu_over_kT << " E=" << e.E << " remaining=0" << std::endl;
-                break;
-            case PHB_P1_CHUNK: {
-                const double rate =
-                    e.block_attempted > 0 ? (double)e.block_accepted / (double)e.block_attempted : 0.0;
-                r << std::setprecision(6) << "[sgmc] chunk acc=" << e.acc_since << "/" << e.nacc_target
-                  << " attempts=" << e.att_since << " acc_rate=" << rate << " fwd_total=" << e.fwd_total
-                  << " bwd_total=" << e.bwd_total << " gc_block=" << e.gc_block << " remaining=" << e.remaining
-                  << std::endl;
-                break;
-            }
-            case PHB_P1_MU_UPDATE:
-                r << std::setprecision(6) << "[sgmc] mu-update#" << (e.epoch + 1)
-                  << " cause=" << (e.early_stop ? "Amax" : "Nacc") << " mu/kT=" << e.mu_over_kT << " f=" << e.f
-                  << " acc=" << e.acc_since << "/" << e.nacc_target << " attempts=" << e.att_since << "/" << e.amax
-                  << " fwd_total=" << e.fwd_total << " bwd_total=" << e.bwd_total << " remaining=" << e.remaining
-                  << " R0=" << e.r0 << " w_info=" << e.w_info << std::endl;
-                break;
-            case PHB_P1_REACHED:
-                r << std::setprecision(6) << "[sgmc] reached target in epoch=" << e.epoch
-                  << " step_in_block=" << e.step_in_block << " Nprot=" << e.nprot << "/" << e.nh
-                  << " mu/kT=" << e.mu_over_kT << " E=" << e.E << " fwd_acc=" << e.fwd_total
-                  << " bwd_acc=" << e.bwd_total << " remaining=0" << std::endl;
-                break;
-            case PHB_P1_NO_PAIRS:
-                r << "[sgmc] no valid pairs to flip; stop." << std::endl;
-                break;
-            default:
-                break;
+// The phase-1 line of one chain event (each ends with std::endl, as in the
+// reference, so report.txt follows the run).
+void phase1_line(std::ostream& r, const phb_phase1_event& e) {
+    switch (e.kind) {
+        case PHB_P1_DIRECT:
+            r << std::setprecision(6) << "[direct] done: applied direct flips=" << e.attempted << " E=" << e.E
+              << " remaining=" << 0 << std::endl;
+            break;
+        case PHB_P1_AT_TARGET:
+            r << "[sgmc] already at target: Nprot=" << e.nprot << "/" << e.nh << " target=" << e.target_count
+              << " mu/kT=" << e.mu_over_kT << " E=" << e.E << " remaining=0" << std::endl;
+            break;
+        case PHB_P1_CHUNK: {
+            const double rate =
+                e.block_attempted > 0 ? (double)e.block_accepted / (double)e.block_attempted : 0.0;
+            r << std::setprecision(6) << "[sgmc] chunk acc=" << e.acc_since << "/" << e.nacc_target
+              << " attempts=" << e.att_since << " acc_rate=" << rate << " fwd_total=" << e.fwd_total
+              << " bwd_total=" << e.bwd_total << " gc_block=" << e.gc_block << " remaining=" << e.remaining
+              << std::endl;
+            break;
         }
+        case PHB_P1_MU_UPDATE:
+            r << std::setprecision(6) << "[sgmc] mu-update#" << (e.epoch + 1)
+              << " cause=" << (e.early_stop ? "Amax" : "Nacc") << " mu/kT=" << e.mu_over_kT << " f=" << e.f
+              << " acc=" << e.acc_since << "/" << e.nacc_target << " attempts=" << e.att_since << "/" << e.amax
+              << " fwd_total=" << e.fwd_total << " bwd_total=" << e.bwd_total << " remaining=" << e.remaining
+              << " R0=" << e.r0 << " w_info=" << e.w_info << std::endl;
+            break;
+        case PHB_P1_REACHED:
+            r << std::setprecision(6) << "[sgmc] reached target in epoch=" << e.epoch
+              << " step_in_block=" << e.step_in_block << " Nprot=" << e.nprot << "/" << e.nh
+              << " mu/kT=" << e.mu_over_kT << " E=" << e.E << " fwd_acc=" << e.fwd_total
+              << " bwd_acc=" << e.bwd_total << " remaining=0" << std::endl;
+            break;
+        case PHB_P1_NO_PAIRS:
+            r << "[sgmc] no valid pairs to flip; stop." << std::endl;
+            break;
+        default:
+            break;
     }
 }
 
@@ -228,8 +227,7 @@ int main(int argc, char** argv) {
         const MCPlan mcplan = build_plan(sys, p);
         plan_summary(rep, mcplan, p);
         MonteCarlo mc(d_atoms, d_params, d_neigh, plan, sys.box, p);
-        const PhaseGCResult r1 = mc.phase1(p);
-        phase1_lines(rep, mc.phase1_events());
+        const PhaseGCResult r1 = mc.phase1(p, 0, nullptr, [&rep](const phb_phase1_event& e) { phase1_line(rep, e); });
         const Phase1Result ho = mc.handover();
         download_identities(d_atoms, sys);
         write_lammps_full("phase1.data", sys, write_box);

@@ -96,6 +96,13 @@ void handover_from_identities(phb_mc_s* M, const Ident& h) {
 }  // namespace
 
 // The counters and state every phase-1 event carries, relative to the run's start.
+// Records a phase-1 event and hands it to the caller's callback, if any (the
+// callback runs on this thread, between blocks, and must not throw).
+static void emit(phb_mc_s* M, const phb_phase1_desc* d, const phb_phase1_event& e) {
+    M->p1_events.push_back(e);
+    if (d->on_event) d->on_event(&e, d->user);
+}
+
 static phb_phase1_event event_of(const McCtl& c, long long a0, long long acc0, long long f0, long long b0, int epoch,
                                  int nh, int target_count, double mu_over_kT) {
     phb_phase1_event e{};
@@ -177,7 +184,7 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
                 e.nprot = c.nprot; e.nh = nh_total; e.target_count = target_count;
                 e.mu_over_kT = M->d.beta * M->dev.mu;
                 e.E = c.E;
-                M->p1_events.push_back(e);
+                emit(M, d, e);
             }
             out->attempted = to_flip;
             out->accepted = to_flip;
@@ -227,7 +234,7 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
                         e.block_accepted = c.accepted - before.accepted;
                         e.step_in_block = (int)e.block_attempted;
                         e.r0 = R0; e.nacc_target = Nacc_target; e.amax = Amax; e.gc_block = blk;
-                        M->p1_events.push_back(e);
+                        emit(M, d, e);
                     }
                     if (c.status == 4) {
                         out->reached_target = 1;
@@ -256,7 +263,7 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
                     e.r0 = R0; e.nacc_target = Nacc_target; e.amax = Amax; e.gc_block = blk;
                     e.f = f;
                     e.w_info = w_info;
-                    M->p1_events.push_back(e);
+                    emit(M, d, e);
                 }
             }
             hipLaunchKernelGGL(k_p1_control, dim3(1), dim3(64), 0, stream(), M->ctl.p, -1, 1);
@@ -267,7 +274,7 @@ extern "C" int phb_phase1_run(phb_mc M, const phb_phase1_desc* d, phb_phase1_res
             out->reached_target = 1;  // the reference returns before filling the hand-over (MC/src/mc.cu:346-353)
             phb_phase1_event e = event_of(c, a0, acc0, f0, b0, 0, nh_total, target_count, beta * mu);
             e.kind = PHB_P1_AT_TARGET;
-            M->p1_events.push_back(e);
+            emit(M, d, e);
         }
         out->attempted = c.trials_done - a0;
         out->accepted = c.accepted - acc0;

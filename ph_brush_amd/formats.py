@@ -171,3 +171,74 @@ def load_run(data_path, params_path):
     s = S.System(d.x.copy(), d.y.copy(), d.z.copy(), d.q.copy(), (d.type - 1).astype(np.int32), bonds, d.Lx, d.Ly,
                  d.Lz * zfac, d.Lz, d.mol.copy())
     return d, pf, s, physics_params(pf), move_params(pf)
+
+
+# ------------------------------------------------------------ workloads as input files
+def data_file_of(s: S.System, ntypes: int = S.NTYPES) -> DataFile:
+    """A System as the rows of a LAMMPS "full" data file: ids 1..N in row order,
+    1-based types, bond type 1, the data-file box (Lz before the slab extension)."""
+    n, nb = s.natoms, int(s.bonds.shape[0])
+    mol = s.mol.astype(np.int32) if s.mol is not None else np.ones(n, np.int32)
+    return DataFile(n, nb, ntypes, s.Lx, s.Ly, s.Lz_data or s.Lz, np.arange(1, n + 1, dtype=np.int32), mol,
+                    (s.type0 + 1).astype(np.int32), s.q.copy(), s.x.copy(), s.y.copy(), s.z.copy(),
+                    np.arange(1, nb + 1, dtype=np.int32), np.ones(nb, np.int32),
+                    s.bonds[:, 0].astype(np.int32), s.bonds[:, 1].astype(np.int32))
+
+
+def params_text(s: S.System, p: S.Params, mc: S.McParams, target_pct: float, rng_seed: int = 1, **keys) -> str:
+    """params.in for a workload: the physics of p (explicit grid and LJ cutoff as
+    the extension keys), the move types, beta and W filter of mc, the phase-1
+    target and seed; `keys` adds or overrides entries."""
+    zfac = s.Lz / s.Lz_data if s.Lz_data else 1.0
+    entries = {
+        "cutoff_nm": repr(float(p.rc)), "lj_cutoff_nm": repr(float(p.lj_rc)),
+        "lj_shift": "yes" if p.lj_shift else "no",
+        "coulombtype": "pme" if p.coulombtype == L.PHB_COUL_PME else "coul_cut",
+        "epsilon_r": repr(float(p.epsilon_r)), "ewald_alpha": repr(float(p.ewald_alpha)),
+        "pme_order": str(int(p.pme_order)), "pme_3dc": f"yes {float(zfac)!r}" if p.pme_3dc else "no",
+        "nh_n_type": str(mc.nh_n_type), "nh_p_type": str(mc.nh_p_type), "w_type": str(mc.w_type),
+        "cl_type": str(mc.cl_type), "beta": repr(float(mc.beta)), "w_z_extra_nm": repr(float(mc.w_z_extra)),
+        "target_pct": repr(float(target_pct)),
+        "rng_seed": str(rng_seed),
+    }
+    if p.grid:
+        entries["pme_grid"] = " ".join(map(str, p.grid))
+    entries.update({k: str(v) for k, v in keys.items()})
+    T = p.ntypes
+    eps, sig = np.asarray(p.lj_eps).reshape(T, T), np.asarray(p.lj_sig).reshape(T, T)
+    lines = [f"{k} = {v}" for k, v in entries.items()]
+    lines += [f"lj {a + 1} {b + 1} {float(eps[a, b])!r} {float(sig[a, b])!r}" for a in range(T) for b in range(a, T)]
+    return "\n".join(lines) + "\n"
+
+
+WORKLOADS = {
+    "brush_2640": (S.config_small_fp64, 60.0),      # 64 chains x 20 beads, every bead titratable
+    "diblock_44000": (S.config_30k_fp64, 60.0),     # 400 chains x 50 beads, beads 26-50 titratable
+}
+
+
+def write_workload(name: str, out_dir, target_pct: float | None = None, rng_seed: int = 1, **keys):
+    """Writes <out_dir>/in.data and <out_dir>/params.in for a named workload
+    (the generated brushes standing in for the reference's undistributed inputs);
+    returns their paths."""
+    from pathlib import Path
+    cfg, default_target = WORKLOADS[name]
+    s, p, mc = cfg()
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    write_lammps_full(out / "in.data", data_file_of(s))
+    (out / "params.in").write_text(params_text(s, p, mc, default_target if target_pct is None else target_pct,
+                                               rng_seed, **keys))
+    return out / "in.data", out / "params.in"
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="write a workload as a LAMMPS data file and params.in")
+    ap.add_argument("workload", choices=sorted(WORKLOADS))
+    ap.add_argument("out_dir")
+    ap.add_argument("--target-pct", type=float, default=None)
+    ap.add_argument("--rng-seed", type=int, default=1)
+    a = ap.parse_args()
+    for path in write_workload(a.workload, a.out_dir, a.target_pct, a.rng_seed):
+        print(path)

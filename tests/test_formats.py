@@ -284,3 +284,23 @@ def test_missing_files(tmp_path):
         F.read_lammps_full(tmp_path / "none.data")
     with pytest.raises(L.PhbError, match="Failed to open params: "):
         F.read_params(tmp_path / "none.params", 8)
+
+
+def test_workload_files_read_back_to_the_generated_workload(tmp_path):
+    """python -m ph_brush_amd.formats brush_2640 DIR: the files describe the
+    generator's system (8-decimal positions) and physics exactly."""
+    data, prm = F.write_workload("brush_2640", tmp_path, target_pct=55.0, rng_seed=9)
+    _, pf, s, p, mc = F.load_run(data, prm)
+    s0, p0, mc0 = S.config_small_fp64()
+    assert s.natoms == s0.natoms and (s.Lx, s.Ly, s.Lz, s.Lz_data) == (s0.Lx, s0.Ly, s0.Lz, s0.Lz_data)
+    for k in ("x", "y", "z"):
+        np.testing.assert_allclose(getattr(s, k), getattr(s0, k), rtol=0, atol=5e-9)
+    np.testing.assert_array_equal(s.type0, s0.type0)
+    np.testing.assert_array_equal(s.q, s0.q)
+    np.testing.assert_array_equal(s.bonds, s0.bonds)
+    for k in ("rc", "lj_rc", "lj_shift", "epsilon_r", "coulombtype", "ewald_alpha", "pme_order", "grid", "pme_3dc",
+              "ntypes"):
+        assert getattr(p, k) == getattr(p0, k), k
+    np.testing.assert_array_equal(p.lj_eps, p0.lj_eps)
+    assert (mc.beta, mc.w_z_extra, mc.nh_n_type, mc.cl_type) == (mc0.beta, mc0.w_z_extra, mc0.nh_n_type, mc0.cl_type)
+    assert (pf.target_protonation_pct, pf.rng_seed) == (55.0, 9)
