@@ -339,7 +339,7 @@ const std::unordered_map<std::string, Setter>& setters() {
 
 // read_params (MC/src/io.cpp:171-301).
 void read_params_file(const std::string& path, int ntypes, ParamsOut& out) {
-    require(ntypes > 0, "read_params: ntypes must be positive");
+    require(ntypes >= 0, "read_params: ntypes must not be negative");   // 0 fails the type checks below
     phb_params_file& P = out.P;
     param_defaults(P, ntypes);
     const size_t T2 = (size_t)ntypes * ntypes;

@@ -76,6 +76,8 @@ DATA = {
     "err_count": HEADER.format(natoms=6, nbonds=0) + ATOMS,
     "err_stoi": HEADER.format(natoms=5, nbonds=0) + ATOMS.replace("5 2 7", "five 2 7"),
     "err_stod": HEADER.format(natoms=5, nbonds=0) + ATOMS.replace("0.1 0.2 0.3", "x 0.2 0.3"),
+    "err_zero_types": HEADER.format(natoms=1, nbonds=0).replace("8 atom types\n", "")
+                      + "Atoms # full\n\n1 1 0 0.0 1.0 2.0 3.0\n",
 }
 
 PME = "cutoff_nm = 3.5\ncoulombtype = pme\nepsilon_r = 46.3118\nlj 1 1 1.0 1.0\nlj 4 5 0.5 0.9\nlj 8 8 1.0 1.0\n"
