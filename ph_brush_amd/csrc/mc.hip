@@ -428,9 +428,10 @@ void fill_stats(phb_mc_s* M, const McCtl& c, phb_mc_stats* s) {
 extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box* box,
                              const phb_mc_desc* d, unsigned seed, phb_mc* out) {
     return guard([&] {
-        require(a && p && neigh && pme && box && d && out, "phb_mc_create: null argument");
+        require(a && p && neigh && box && d && out, "phb_mc_create: null argument");
         require(p->d.coulombtype == PHB_COUL_PME, "phb_mc_create: the device chain needs coulombtype=pme",
                 PHB_ERR_UNSUPPORTED);
+        require(pme != nullptr, "phb_mc_create: null PME plan");
         require(d->precision == PHB_FP64 || d->precision == PHB_FP32, "phb_mc_create: precision must be PHB_FP64 or PHB_FP32",
                 PHB_ERR_UNSUPPORTED);
         require(d->precision == PHB_FP64 || pme->order == 4 || pme->order == 5,

@@ -137,3 +137,45 @@ def test_mc_program_usage_and_errors(tmp_path, gpu):
     assert r.returncode == 1 and r.stderr == "Unrecognized argument: --bogus\n"
     r = _run(tmp_path, tmp_path / "missing.data", prm)
     assert r.returncode == 2 and r.stderr == f"Error: Failed to open LAMMPS data: {tmp_path / 'missing.data'}\n"
+
+
+def _rewrite(prm, **keys):
+    lines = [l for l in prm.read_text().splitlines() if l.split("=")[0].strip() not in keys]
+    prm.write_text("\n".join(lines + [f"{k} = {v}" for k, v in keys.items()]) + "\n")
+
+
+def test_mc_program_already_at_target_and_phase2_skipped(tmp_path, gpu):
+    data, prm = _inputs(tmp_path, 75.0)
+    d = F.read_lammps_full(data)
+    nh = int(np.sum((d.type == 4) | (d.type == 5)))
+    nprot = int(np.sum(d.type == 5))
+    pct = 100.0 * (nprot + 0.5) / nh            # floor(pct/100 * Nh) == Nprot: no flip needed
+    _rewrite(prm, target_pct=repr(pct))
+    r = _run(tmp_path, data, prm)
+    assert r.returncode == 0, r.stderr
+    rep = (tmp_path / "report.txt").read_text()
+    assert f"[sgmc] already at target: Nprot={nprot}/{nh} target={nprot} " in rep
+    # the hand-over is empty after an early return, as in the reference: phase 2 has no W-Cl pools
+    assert "[phase2] pools: NH_P=0 NH_N=0 Cl=0 W=0" in rep
+    assert "[phase2] wc-only requested but pools empty; Stop.\n" in rep
+    _, t1, q1 = _identities(tmp_path / "phase1.data")
+    np.testing.assert_array_equal(t1, d.type - 1)
+    _rewrite(prm, target_pct="75.0", phase2_wc="no")
+    r = _run(tmp_path, data, prm)
+    assert r.returncode == 0, r.stderr
+    rep = (tmp_path / "report.txt").read_text()
+    assert "[phase2] skipped (phase2_wc=false)\n" in rep
+    assert (tmp_path / "phase1.data").read_bytes() == (tmp_path / "phase2.data").read_bytes()
+    m = re.search(r"\[summary\] E0/after phase1/after phase2 = (\S+) / (\S+) / (\S+)", rep)
+    assert m and m.group(2) == m.group(3)
+
+
+def test_mc_program_coul_cut_reports_energies_then_stops(tmp_path, gpu):
+    data, prm = _inputs(tmp_path, 75.0)
+    _rewrite(prm, coulombtype="coul_cut")
+    r = _run(tmp_path, data, prm)
+    assert r.returncode == 2
+    assert r.stderr.startswith("Error: ") and "coulombtype=pme" in r.stderr
+    rep = (tmp_path / "report.txt").read_text()
+    assert "E_Coulomb (SR, cutoff) = " in rep and "E_total = " in rep
+    assert "PME grid" not in rep
