@@ -204,7 +204,11 @@ typedef struct phb_mc_s *phb_mc;
 /* Builds pools (build_plan, MC/src/mc.cu:67-122), full neighbour rows with
  * real-space coefficients, spline stencils, the mesh kernel and the potential
  * grid; seeds the device MT19937 like std::mt19937(seed).  The atoms handle is
- * updated in place by every accepted move and must outlive the MC object. */
+ * updated in place by every accepted move and must outlive the MC object.
+ * coulombtype pme: pme is the plan (FP64).  coulombtype coul_cut
+ * (total_energy's other branch, MC/src/mc.cu:52-56): pme may be NULL and is not
+ * used; the chain runs on a plan of its own with alpha 0 (pair coefficient
+ * elF/r), no reciprocal part and no slab term. */
 int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box *box,
                   const phb_mc_desc *d, unsigned seed, phb_mc *out);
 int phb_mc_destroy(phb_mc mc);

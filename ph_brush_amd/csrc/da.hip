@@ -1711,6 +1711,8 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         require(d->ntypes == M->params->d.ntypes, "phb_da_attach: ntypes must match the parameter set");
         require(M->d.precision == PHB_FP64, "phb_da_attach: the delayed-acceptance chain runs the FP64 form",
                 PHB_ERR_UNSUPPORTED);
+        require(M->params->d.coulombtype == PHB_COUL_PME,
+                "phb_da_attach: the surrogate's physics part needs coulombtype=pme", PHB_ERR_UNSUPPORTED);
         require(d->r_desc > 0.0, "phb_da_attach: r_desc must be > 0");
         const int cand = d->candidates > 0 ? d->candidates : DA_M;
         require(cand >= 1 && cand <= DA_M, "phb_da_attach: candidates must be in [1, 64]");

@@ -108,7 +108,7 @@ __global__ void k_finish_recompute(McDev D, const double* __restrict__ res, doub
     McCtl* C = D.ctl;
     const double Q = res[3], Q2 = res[4], Mz = res[5], Qz2 = res[6];
     const double e_self = -D.self_coef * Q2;
-    const double e_q = -(M_PI * D.elF) * (Q * Q) / (2.0 * D.vol * D.alpha * D.alpha);
+    const double e_q = D.alpha > 0.0 ? -(M_PI * D.elF) * (Q * Q) / (2.0 * D.vol * D.alpha * D.alpha) : 0.0;
     const double E = res[0] + (res[1] + res[2] + e_self + e_q + slab_energy(D, Mz, Q, Qz2));
     if (C->E_recomputed == C->E_recomputed)  // not the first recompute
         C->max_drift = fmax(C->max_drift, fabs(C->E - E));
@@ -429,8 +429,15 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
                              const phb_mc_desc* d, unsigned seed, phb_mc* out) {
     return guard([&] {
         require(a && p && neigh && box && d && out, "phb_mc_create: null argument");
-        require(p->d.coulombtype == PHB_COUL_PME, "phb_mc_create: the device chain needs coulombtype=pme",
-                PHB_ERR_UNSUPPORTED);
+        require(p->d.coulombtype == PHB_COUL_PME || p->d.coulombtype == PHB_COUL_CUT,
+                "phb_mc_create: coulombtype must be pme or coul_cut", PHB_ERR_UNSUPPORTED);
+        // coul_cut (total_energy's other branch, MC/src/mc.cu:52-56): the chain runs on a plan of its
+        // own with alpha 0 and no reciprocal part; a plan passed in is not used
+        std::unique_ptr<phb_pme_s> cut_plan;
+        if (p->d.coulombtype == PHB_COUL_CUT) {
+            cut_plan.reset(pme_create_cut(p->d, *box));
+            pme = cut_plan.get();
+        }
         require(pme != nullptr, "phb_mc_create: null PME plan");
         require(d->precision == PHB_FP64 || d->precision == PHB_FP32, "phb_mc_create: precision must be PHB_FP64 or PHB_FP32",
                 PHB_ERR_UNSUPPORTED);
@@ -456,6 +463,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
                 "phb_mc_create: every grid dimension must be >= 2*order-1", PHB_ERR_UNSUPPORTED);
         auto M = std::make_unique<phb_mc_s>();
         M->atoms = a; M->params = p; M->neigh = neigh; M->pme = pme; M->box = *box; M->d = *d;
+        M->own_pme = cut_plan.release();
         require(d->refresh != PHB_REFRESH_DIRECT || refresh_direct_allowed(M.get()),
                 "phb_mc_create: the direct refresh needs nx <= 64", PHB_ERR_UNSUPPORTED);
         M->d.pending_max = pm;

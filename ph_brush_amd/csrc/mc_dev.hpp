@@ -415,7 +415,8 @@ __device__ __forceinline__ double corr_move(const McDev& D, double qi, double qi
     const double Mzn = Mz + dqi * zi + dqj * zj;
     const double Qz2n = Qz2 + dqi * zi * zi + dqj * zj * zj;
     const double e_self = -D.self_coef * dQ2;
-    const double e_q = -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q);
+    // alpha = 0 is the cut-off Coulomb form: no Ewald net-charge term
+    const double e_q = D.alpha > 0.0 ? -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q) : 0.0;
     return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2));
 }
 

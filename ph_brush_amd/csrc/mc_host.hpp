@@ -17,6 +17,7 @@ struct phb_mc_s {
     phb_params_s* params = nullptr;
     phb_csr_s* neigh = nullptr;
     phb_pme_s* pme = nullptr;
+    phb_pme_s* own_pme = nullptr;   // the coul_cut plan this object made for itself
     phb_box box{};
     phb_mc_desc d{};
     int nsites = 0, nions = 0;
@@ -52,6 +53,7 @@ struct phb_mc_s {
     ~phb_mc_s() {
         if (da_free) da_free(da);
         if (p2_free) p2_free(p2);
+        delete own_pme;
         for (auto e : ev) (void)hipEventDestroy(e);
     }
     hipEvent_t event() {

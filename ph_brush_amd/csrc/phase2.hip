@@ -159,7 +159,7 @@ __device__ double correction4(const McDev& D, const Trial4<P>& t, int K, double 
     }
     const double Qn = Q + dQ, Mzn = Mz + dMz, Qz2n = Qz2 + dQz2;
     const double e_self = -D.self_coef * dQ2;
-    const double e_q = -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q);
+    const double e_q = D.alpha > 0.0 ? -(M_PI * D.elF) / (2.0 * D.vol * D.alpha * D.alpha) * (Qn * Qn - Q * Q) : 0.0;
     return e_self + e_q + (slab_energy(D, Mzn, Qn, Qz2n) - slab_energy(D, Mz, Q, Qz2)) + mesh;
 }
 

@@ -442,7 +442,7 @@ void corrections(const phb_pme_s* P, const phb_box* box, const double mom[4], do
     const double elF = P->elF, alpha = P->alpha, V = P->vol;
     const double Q = mom[0], Q2 = mom[1];
     *e_self = -elF * alpha / std::sqrt(M_PI) * Q2;
-    *e_qcorr = -(M_PI * elF) * (Q * Q) / (2.0 * V * alpha * alpha);
+    *e_qcorr = alpha > 0.0 ? -(M_PI * elF) * (Q * Q) / (2.0 * V * alpha * alpha) : 0.0;
     double E_slab = 0.0;
     if (P->use3dc) {
         const double dc = (2.0 * M_PI / V) * elF;
@@ -455,87 +455,105 @@ void corrections(const phb_pme_s* P, const phb_box* box, const double mom[4], do
 }  // namespace phb
 
 // ----------------------------------------------------------------------- ABI
+namespace phb {
+
+// The plan behind pmeCreate (MC/src/pme.cu:291-335).  cut: the plan of the
+// cut-off Coulomb form for the Monte Carlo chain -- alpha 0 (real-space
+// coefficient elF/r exactly, no self or net-charge term), a zero influence
+// function (no reciprocal part: zero mesh kernel and potential), no slab term,
+// and the smallest grid the chain's stencils accept (8^3, order 4).
+std::unique_ptr<phb_pme_s> build_pme(const phb_params_desc& d, const phb_box* box, double rc, int precision,
+                                     bool cut) {
+    require(precision == PHB_FP64 || precision == PHB_FP32, "pmeCreate: precision must be PHB_FP64 or PHB_FP32");
+    require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "pmeCreate: box must be positive");
+    require(cut || (d.pme_order >= 3 && d.pme_order <= 8), "pmeCreate: pme_order must be in [3, 8]",
+            PHB_ERR_UNSUPPORTED);
+    require_device();
+    auto P = std::make_unique<phb_pme_s>();
+    P->precision = precision;
+    P->order = cut ? 4 : d.pme_order;
+    P->use3dc = !cut && d.pme_3dc != 0;
+    // MC/src/pme.cu:299-306
+    P->alpha = cut ? 0.0 : d.ewald_alpha > 0 ? d.ewald_alpha : alpha_from_rtol(rc, d.ewald_rtol > 0 ? d.ewald_rtol : 1e-5);
+    P->ewF = cut ? 0.0 : (M_PI / P->alpha) * (M_PI / P->alpha);
+    P->elF = kKe / d.epsilon_r;
+    P->Lx = box->Lx; P->Ly = box->Ly; P->Lz = box->Lz;
+    P->vol = box->Lx * box->Ly * box->Lz;
+    // MC/src/pme.cu:312-318, EXT explicit dims
+    const double sp = d.pme_spacing > 0 ? d.pme_spacing : 0.12;
+    int nx = cut ? 8 : d.nx > 0 ? d.nx : good_fft_dim((int)std::ceil(box->Lx / sp));
+    int ny = cut ? 8 : d.ny > 0 ? d.ny : good_fft_dim((int)std::ceil(box->Ly / sp));
+    int nz = cut ? 8 : d.nz > 0 ? d.nz : good_fft_dim((int)std::ceil(box->Lz / sp));
+    nx = std::max(nx, P->order); ny = std::max(ny, P->order); nz = std::max(nz, P->order);
+    P->nx = nx; P->ny = ny; P->nz = nz;
+    P->px = make_fft_plan(nx); P->py = make_fft_plan(ny); P->pz = make_fft_plan(nz);
+    P->geom = GridGeom{nx, ny, nz, P->order, box->Lx, box->Ly, box->Lz};
+    const int nkx = nx / 2 + 1;
+    const size_t G = (size_t)nx * ny * nz, Gk = (size_t)nkx * ny * nz;
+    // launch shapes
+    // planes of >= 1024 elements get 8 waves (their LDS stages are latency-bound)
+    const long long plane_elems = std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny);
+    P->plane_threads = fft_threads(plane_elems, plane_elems >= 1024 ? 512 : 256);
+    const size_t csz = precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+    require(P->plane_lds(csz) <= 160 * 1024, "pmeCreate: xy plane too large for LDS", PHB_ERR_UNSUPPORTED);
+    const int ntile0 = (nkx + 7) / 8;
+    P->z_tx = (nkx + ntile0 - 1) / ntile0;
+    P->z_blocks = ny * ((nkx + P->z_tx - 1) / P->z_tx);
+    P->z_threads = fft_threads((long long)P->z_tx * nz);
+    require(P->z_lds(csz) <= 160 * 1024, "pmeCreate: z pencil too large for LDS", PHB_ERR_UNSUPPORTED);
+    // influence function (MC/src/pme.cu:196-223), half spectrum, no corner weight
+    std::vector<double> mX = spme_moduli(nx, P->order), mY = spme_moduli(ny, P->order), mZ = spme_moduli(nz, P->order);
+    std::vector<double> eta(Gk, 0.0);
+    const double Hxx = 1.0 / box->Lx, Hyy = 1.0 / box->Ly, Hzz = 1.0 / box->Lz;
+    for (int kz = 0; kz < (cut ? 0 : nz); ++kz)
+        for (int ky = 0; ky < ny; ++ky)
+            for (int kx = 0; kx < nkx; ++kx) {
+                size_t o = ((size_t)kz * ny + ky) * nkx + kx;
+                if (kx == 0 && ky == 0 && kz == 0) continue;
+                int mx = kx, my = (ky <= ny / 2) ? ky : ky - ny, mz = (kz <= nz / 2) ? kz : kz - nz;
+                double kxv = mx * Hxx, kyv = my * Hyy, kzv = mz * Hzz;
+                double m2k = kxv * kxv + kyv * kyv + kzv * kzv;
+                if (m2k == 0.0) continue;
+                double denom = m2k * M_PI * P->vol * mX[kx] * mY[ky] * mZ[kz];
+                if (!(denom > 0.0)) continue;
+                eta[o] = P->elF * std::exp(-P->ewF * m2k) / denom;
+            }
+    P->eta_host = eta;
+    auto tx = twiddles(nx), ty = twiddles(ny), tz = twiddles(nz);
+    if (precision == PHB_FP64) {
+        P->grid64.alloc(G); P->phi64.alloc(G); P->kg64.alloc(Gk); P->eta64.alloc(Gk);
+        P->eta64.upload(eta.data(), Gk);
+        P->twx64.alloc(nx); P->twy64.alloc(ny); P->twz64.alloc(nz);
+        P->twx64.upload(tx.data(), nx); P->twy64.upload(ty.data(), ny); P->twz64.upload(tz.data(), nz);
+    } else {
+        std::vector<float> e32(eta.begin(), eta.end());
+        auto cv = [](const std::vector<cplx<double>>& v) {
+            std::vector<cplx<float>> o(v.size());
+            for (size_t k = 0; k < v.size(); ++k) o[k] = {(float)v[k].x, (float)v[k].y};
+            return o;
+        };
+        auto fx = cv(tx), fy = cv(ty), fz = cv(tz);
+        P->grid32.alloc(G); P->phi32.alloc(G); P->kg32.alloc(Gk); P->eta32.alloc(Gk);
+        P->eta32.upload(e32.data(), Gk);
+        P->twx32.alloc(nx); P->twy32.alloc(ny); P->twz32.alloc(nz);
+        P->twx32.upload(fx.data(), nx); P->twy32.upload(fy.data(), ny); P->twz32.upload(fz.data(), nz);
+    }
+    P->partials.alloc(std::max<size_t>(4 * 2048, (size_t)P->z_blocks));
+    P->result.alloc(16);
+    sync();
+    return P;
+}
+
+phb_pme_s* pme_create_cut(const phb_params_desc& d, const phb_box& box) {
+    return build_pme(d, &box, 0.0, PHB_FP64, true).release();
+}
+
+}  // namespace phb
+
 extern "C" int phb_pme_create(phb_params p, const phb_box* box, double rc, int precision, phb_pme* out) {
     return guard([&] {
         require(p && box && out, "pmeCreate: null plan");
-        require(precision == PHB_FP64 || precision == PHB_FP32, "pmeCreate: precision must be PHB_FP64 or PHB_FP32");
-        require(box->Lx > 0 && box->Ly > 0 && box->Lz > 0, "pmeCreate: box must be positive");
-        const phb_params_desc& d = p->d;
-        require(d.pme_order >= 3 && d.pme_order <= 8, "pmeCreate: pme_order must be in [3, 8]", PHB_ERR_UNSUPPORTED);
-        require_device();
-        auto P = std::make_unique<phb_pme_s>();
-        P->precision = precision;
-        P->order = d.pme_order;
-        P->use3dc = d.pme_3dc != 0;
-        // MC/src/pme.cu:299-306
-        P->alpha = d.ewald_alpha > 0 ? d.ewald_alpha : alpha_from_rtol(rc, d.ewald_rtol > 0 ? d.ewald_rtol : 1e-5);
-        P->ewF = (M_PI / P->alpha) * (M_PI / P->alpha);
-        P->elF = kKe / d.epsilon_r;
-        P->Lx = box->Lx; P->Ly = box->Ly; P->Lz = box->Lz;
-        P->vol = box->Lx * box->Ly * box->Lz;
-        // MC/src/pme.cu:312-318, EXT explicit dims
-        const double sp = d.pme_spacing > 0 ? d.pme_spacing : 0.12;
-        int nx = d.nx > 0 ? d.nx : good_fft_dim((int)std::ceil(box->Lx / sp));
-        int ny = d.ny > 0 ? d.ny : good_fft_dim((int)std::ceil(box->Ly / sp));
-        int nz = d.nz > 0 ? d.nz : good_fft_dim((int)std::ceil(box->Lz / sp));
-        nx = std::max(nx, P->order); ny = std::max(ny, P->order); nz = std::max(nz, P->order);
-        P->nx = nx; P->ny = ny; P->nz = nz;
-        P->px = make_fft_plan(nx); P->py = make_fft_plan(ny); P->pz = make_fft_plan(nz);
-        P->geom = GridGeom{nx, ny, nz, P->order, box->Lx, box->Ly, box->Lz};
-        const int nkx = nx / 2 + 1;
-        const size_t G = (size_t)nx * ny * nz, Gk = (size_t)nkx * ny * nz;
-        // launch shapes
-        // planes of >= 1024 elements get 8 waves (their LDS stages are latency-bound)
-        const long long plane_elems = std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny);
-        P->plane_threads = fft_threads(plane_elems, plane_elems >= 1024 ? 512 : 256);
-        const size_t csz = precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
-        require(P->plane_lds(csz) <= 160 * 1024, "pmeCreate: xy plane too large for LDS", PHB_ERR_UNSUPPORTED);
-        const int ntile0 = (nkx + 7) / 8;
-        P->z_tx = (nkx + ntile0 - 1) / ntile0;
-        P->z_blocks = ny * ((nkx + P->z_tx - 1) / P->z_tx);
-        P->z_threads = fft_threads((long long)P->z_tx * nz);
-        require(P->z_lds(csz) <= 160 * 1024, "pmeCreate: z pencil too large for LDS", PHB_ERR_UNSUPPORTED);
-        // influence function (MC/src/pme.cu:196-223), half spectrum, no corner weight
-        std::vector<double> mX = spme_moduli(nx, P->order), mY = spme_moduli(ny, P->order), mZ = spme_moduli(nz, P->order);
-        std::vector<double> eta(Gk, 0.0);
-        const double Hxx = 1.0 / box->Lx, Hyy = 1.0 / box->Ly, Hzz = 1.0 / box->Lz;
-        for (int kz = 0; kz < nz; ++kz)
-            for (int ky = 0; ky < ny; ++ky)
-                for (int kx = 0; kx < nkx; ++kx) {
-                    size_t o = ((size_t)kz * ny + ky) * nkx + kx;
-                    if (kx == 0 && ky == 0 && kz == 0) continue;
-                    int mx = kx, my = (ky <= ny / 2) ? ky : ky - ny, mz = (kz <= nz / 2) ? kz : kz - nz;
-                    double kxv = mx * Hxx, kyv = my * Hyy, kzv = mz * Hzz;
-                    double m2k = kxv * kxv + kyv * kyv + kzv * kzv;
-                    if (m2k == 0.0) continue;
-                    double denom = m2k * M_PI * P->vol * mX[kx] * mY[ky] * mZ[kz];
-                    if (!(denom > 0.0)) continue;
-                    eta[o] = P->elF * std::exp(-P->ewF * m2k) / denom;
-                }
-        P->eta_host = eta;
-        auto tx = twiddles(nx), ty = twiddles(ny), tz = twiddles(nz);
-        if (precision == PHB_FP64) {
-            P->grid64.alloc(G); P->phi64.alloc(G); P->kg64.alloc(Gk); P->eta64.alloc(Gk);
-            P->eta64.upload(eta.data(), Gk);
-            P->twx64.alloc(nx); P->twy64.alloc(ny); P->twz64.alloc(nz);
-            P->twx64.upload(tx.data(), nx); P->twy64.upload(ty.data(), ny); P->twz64.upload(tz.data(), nz);
-        } else {
-            std::vector<float> e32(eta.begin(), eta.end());
-            auto cv = [](const std::vector<cplx<double>>& v) {
-                std::vector<cplx<float>> o(v.size());
-                for (size_t k = 0; k < v.size(); ++k) o[k] = {(float)v[k].x, (float)v[k].y};
-                return o;
-            };
-            auto fx = cv(tx), fy = cv(ty), fz = cv(tz);
-            P->grid32.alloc(G); P->phi32.alloc(G); P->kg32.alloc(Gk); P->eta32.alloc(Gk);
-            P->eta32.upload(e32.data(), Gk);
-            P->twx32.alloc(nx); P->twy32.alloc(ny); P->twz32.alloc(nz);
-            P->twx32.upload(fx.data(), nx); P->twy32.upload(fy.data(), ny); P->twz32.upload(fz.data(), nz);
-        }
-        P->partials.alloc(std::max<size_t>(4 * 2048, (size_t)P->z_blocks));
-        P->result.alloc(16);
-        sync();
-        *out = P.release();
+        *out = phb::build_pme(p->d, box, rc, precision, false).release();
         return PHB_OK;
     });
 }

@@ -59,6 +59,14 @@ struct phb_pme_s {
 
 namespace phb {
 
+// The chain's plan for coulombtype=coul_cut: alpha 0, no reciprocal part, no
+// slab term, an 8^3 order-4 grid (owned by the caller; delete when done).
+phb_pme_s* pme_create_cut(const phb_params_desc& d, const phb_box& box);
+
+}  // namespace phb
+
+namespace phb {
+
 // Gate of the potential refresh's inverse plane pass: plane z runs only while
 // accepted charge changes are pending and applied[z] differs from their list's
 // sequence number, and records the number when done; workgroups share no

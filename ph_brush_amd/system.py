@@ -402,8 +402,9 @@ class MonteCarlo(_Handle):
         self.record_cap = record_cap
         d = L.McDesc(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra,
                      precision, pending_max, recompute_interval, record_cap, spec_window, refresh)
-        L.call("phb_mc_create", atoms.h, params.h, neigh.h, pme.h, C.byref(box), C.byref(d), seed & 0xFFFFFFFF,
-               C.byref(self.h))
+        # coulombtype=coul_cut: no plan (the chain makes its own alpha-0 plan)
+        L.call("phb_mc_create", atoms.h, params.h, neigh.h, pme.h if pme is not None else None, C.byref(box),
+               C.byref(d), seed & 0xFFFFFFFF, C.byref(self.h))
 
     def run(self, ntrials: int) -> dict:
         st = L.McStats()

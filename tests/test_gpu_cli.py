@@ -7,7 +7,8 @@ each phase (types and charges exact) and the input's wrapped positions;
 --write-data writes phase2.data's bytes; report.txt has the reference's lines,
 with the neighbour-pair count, plan counts, epochs and phase-2 step lines
 equal to the oracle's and the [summary] energies within the printed 6
-decimals.  Also the usage errors (exit 1) and a file error (exit 2).
+decimals; for coulombtype pme and coul_cut.  Also the usage errors (exit 1), a
+file error (exit 2), the already-at-target path and phase 2 switched off.
 """
 import re
 import subprocess
@@ -81,9 +82,12 @@ def _identities(path):
     return d, (d.type - 1).astype(np.int32), d.q
 
 
-@pytest.mark.parametrize("target_pct", [75.0, 95.0], ids=["mu_control", "direct"])
-def test_mc_program_matches_oracle(tmp_path, gpu, target_pct):
+@pytest.mark.parametrize("target_pct,coulomb", [(75.0, "pme"), (95.0, "pme"), (75.0, "coul_cut")],
+                         ids=["mu_control", "direct", "coul_cut"])
+def test_mc_program_matches_oracle(tmp_path, gpu, target_pct, coulomb):
     data, prm = _inputs(tmp_path, target_pct)
+    if coulomb != "pme":
+        _rewrite(prm, coulombtype=coulomb)
     r = _run(tmp_path, data, prm, "--write-data", "out.data")
     assert r.returncode == 0, r.stderr
     s, nnz, e0, ro, r2 = _oracle(data, prm, target_pct, 300, 600)
@@ -102,7 +106,11 @@ def test_mc_program_matches_oracle(tmp_path, gpu, target_pct):
     assert "[3dc] Enabling slab correction: zfac=3, Lz: 16 -> 48 (nm)\n" in rep
     assert f"Atoms: {s.natoms}\n" in rep
     assert f"Neighbor pairs (half): {nnz}\n" in rep
-    assert re.search(r"PME grid: 16 x 16 x 48 order=5 alpha=", rep)
+    if coulomb == "pme":
+        assert re.search(r"PME grid: 16 x 16 x 48 order=5 alpha=", rep)
+        assert "E_Coulomb (PME) = " in rep
+    else:
+        assert "PME grid" not in rep and "E_Coulomb (SR, cutoff) = " in rep
     assert f"[NH] total={ro['Nh']}," in rep
     assert "[write] phase1.data written\n" in rep and "[write] phase2.data written\n" in rep
     assert "[write] Wrote updated data file to out.data\n" in rep
@@ -168,14 +176,3 @@ def test_mc_program_already_at_target_and_phase2_skipped(tmp_path, gpu):
     assert (tmp_path / "phase1.data").read_bytes() == (tmp_path / "phase2.data").read_bytes()
     m = re.search(r"\[summary\] E0/after phase1/after phase2 = (\S+) / (\S+) / (\S+)", rep)
     assert m and m.group(2) == m.group(3)
-
-
-def test_mc_program_coul_cut_reports_energies_then_stops(tmp_path, gpu):
-    data, prm = _inputs(tmp_path, 75.0)
-    _rewrite(prm, coulombtype="coul_cut")
-    r = _run(tmp_path, data, prm)
-    assert r.returncode == 2
-    assert r.stderr.startswith("Error: ") and "coulombtype=pme" in r.stderr
-    rep = (tmp_path / "report.txt").read_text()
-    assert "E_Coulomb (SR, cutoff) = " in rep and "E_total = " in rep
-    assert "PME grid" not in rep

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
+from ph_brush_amd import _lib as L
 from ph_brush_amd import system as S
 
 pytestmark = pytest.mark.gpu
@@ -89,12 +90,17 @@ def _compare_chains(rec_g, rec_o, n):
     return n
 
 
-@pytest.mark.parametrize("case", [("tiny", 3000, 4), ("tiny", 3000, 32), ("small", 1200, 32)])
+@pytest.mark.parametrize("case", [("tiny", 3000, 4), ("tiny", 3000, 32), ("small", 1200, 32), ("tiny_coul_cut", 3000, 8)])
 def test_chain_matches_reference_sequence(gpu, case):
+    """tiny_coul_cut: total_energy's coul_cut branch (MC/src/mc.cu:52-56) -- LJ plus
+    elF q q / r over the listed pairs, no Ewald parts and no slab term although the
+    box is extended -- on the chain's own alpha-0 plan."""
     name, n, pend = case
-    if name == "tiny":
+    if name.startswith("tiny"):
         s, p, mc = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2)
         mc = S.McParams.at_ph(7.9)
+        if name == "tiny_coul_cut":
+            p.coulombtype = L.PHB_COUL_CUT
     else:
         s, p, mc = S.config_small_fp64()
     seed = 1
