@@ -176,6 +176,39 @@ def fp32_form(S, L, SG, args):
             "total_energy_rel_error_vs_oracle": abs(e32 - e64) / abs(e64)}
 
 
+def program_form():
+    """The command-line program (ph_brush_amd/bin/MC, the reference's `MC in.data
+    params.in`) end to end on the 44,000-atom workload written as input files:
+    file reading, set-up, phase 1 to 60 % protonation, phase 2 to patience 3000,
+    three data files and report.txt written.  Wall time of the child process."""
+    import re
+    import subprocess
+    import tempfile
+    from ph_brush_amd import formats as F
+    exe = Path(__file__).resolve().parent / "ph_brush_amd" / "bin" / "MC"
+    if not exe.exists():
+        raise RuntimeError(f"{exe} is missing: build with `python -m ph_brush_amd.build`")
+    with tempfile.TemporaryDirectory() as tmp:
+        data, prm = F.write_workload("diblock_44000", tmp)
+        print("[bench] program: MC in.data params.in on the 44,000-atom workload", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        r = subprocess.run([str(exe), str(data), str(prm)], cwd=tmp, capture_output=True, text=True, timeout=300)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise RuntimeError(f"MC failed ({r.returncode}): {r.stderr}")
+        rep = (Path(tmp) / "report.txt").read_text()
+    updates = re.findall(r"\[sgmc\] mu-update#\d+ .*? attempts=(\d+)/", rep)
+    reached = re.search(r"\[sgmc\] reached target in epoch=\d+ step_in_block=(\d+)", rep)
+    chunks_last_epoch = re.findall(r"\[sgmc\] chunk acc=\d+/\d+ attempts=(\d+) ", rep.split("mu-update#")[-1])
+    p1 = sum(map(int, updates)) + (int(chunks_last_epoch[-1]) if chunks_last_epoch else 0) \
+        + (int(reached.group(1)) if reached else 0)
+    steps = re.findall(r"\[phase2\] step=(\d+) ", rep)
+    return {"wall_s": wall, "atoms": 44000, "phase1_trials": p1,
+            "phase2_trials_at_last_report_line": int(steps[-1]) if steps else 0,
+            "note": "one process: read files, neighbour list, energies, phase 1 (mu control to 60 %), phase 2 "
+                    "(4-point moves to patience 3000), write phase1/phase2 data and report.txt"}
+
+
 def batched_form(S, args):
     """32 replicas of a pH x salt titration grid (uneven sizes and grids)
     advanced together; aggregate trial moves/s over timed sweeps after the
@@ -239,6 +272,7 @@ def main():
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-program", action="store_true", help="skip timing the MC command-line program")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-da-real-space", action="store_true",
                     help="skip the delayed-acceptance form whose surrogate includes the real-space potential")
@@ -453,6 +487,8 @@ def main():
         out["forms"]["fp32"] = fp32_form(S, L, SG, args)
     if not args.no_batched:
         out["forms"]["batched"] = batched_form(S, args)
+    if not args.no_program:
+        out["forms"]["program"] = program_form()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         n = args.cpu_trials or 150
