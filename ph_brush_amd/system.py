@@ -453,6 +453,15 @@ class MonteCarlo(_Handle):
         L.call("phb_phase1_run", self.h, C.byref(d), C.byref(r))
         return r.as_dict()
 
+    def phase1_events(self) -> list:
+        """What the last phase1_run did, block by block (phb_phase1_events): one dict per
+        direct-flip / already-at-target / chunk / mu-update / target-reached / no-pairs event."""
+        n = C.c_int()
+        L.call("phb_phase1_events", self.h, None, 0, C.byref(n))
+        ev = (L.Phase1Event * max(n.value, 1))()
+        L.call("phb_phase1_events", self.h, ev, n.value, C.byref(n))
+        return [{k: getattr(e, k) for k, _ in L.Phase1Event._fields_} for e in ev[:n.value]]
+
     def phase1_handover(self) -> dict:
         """The pools phase 1 handed to phase 2 (MC/include/mc.hpp:19-27)."""
         sz = np.zeros(4, np.int32)

@@ -56,18 +56,41 @@ def _check_phase2(s, p, mc, r, ro, n2=600, patience=10 ** 6):
     return st
 
 
+def _check_events(r, g, ro, mc):
+    """The event log behind the report's [sgmc] lines against the oracle's run."""
+    from ph_brush_amd import _lib as L
+    ev = r.mc.phase1_events()
+    ups = [e for e in ev if e["kind"] == L.PHB_P1_MU_UPDATE]
+    assert len(ups) == ro["blocks_done"]
+    assert [e["epoch"] for e in ups] == list(range(len(ups)))
+    last = ev[-1]
+    assert last["kind"] == L.PHB_P1_REACHED and last["epoch"] == ro["blocks_done"]
+    assert last["nprot"] == ro["Nprot"] and last["remaining"] == 0
+    assert (last["attempted"], last["accepted"]) == (ro["attempted"], ro["accepted"])
+    assert (last["fwd_total"], last["bwd_total"]) == (ro["accepted_forward"], ro["accepted_backward"])
+    assert last["mu_over_kT"] == mc.beta * ro["mu_eff"]           # the same FP64 updates, bit for bit
+    blocks = [e for e in ev if e["kind"] in (L.PHB_P1_CHUNK, L.PHB_P1_REACHED)]
+    assert sum(e["block_attempted"] for e in blocks) == ro["attempted"]
+    assert sum(e["block_accepted"] for e in blocks) == ro["accepted"]
+    assert last["step_in_block"] == last["block_attempted"] <= 200
+
+
 def test_phase1_mu_control_then_phase2(gpu):
     s, p, mc, r, g, o, ro = _both(75.0, gc_block=200, epoch_attempt_max=1000, max_trials=200000)
     assert g["reached_target"] == 1 and g["direct"] == 0 and g["blocks_done"] >= 1
     _check_phase1(r, g, ro)
+    _check_events(r, g, ro, mc)
     st = _check_phase2(s, p, mc, r, ro)
     assert st["mode"] == 4 and st["accepted"] > 0
 
 
 def test_phase1_direct_then_phase2(gpu):
+    from ph_brush_amd import _lib as L
     s, p, mc, r, g, o, ro = _both(95.0)
     assert g["direct"] == 1
     _check_phase1(r, g, ro)
+    ev = r.mc.phase1_events()
+    assert len(ev) == 1 and ev[0]["kind"] == L.PHB_P1_DIRECT and ev[0]["attempted"] == ro["attempted"] > 0
     e_re = r.mc.recompute()
     assert abs(g["E"] - e_re) <= 1e-9 * max(1.0, abs(e_re))
     _check_phase2(s, p, mc, r, ro)
