@@ -1,7 +1,8 @@
 """Writes tests/golden/oracle_tiny.npz: golden vectors of the oracle (the FP64
 CPU restatement of MC/src/neighbor.cu:74-125, MC/src/pme.cu:291-457,
 MC/src/mc.cu:43-60 and the trial loop MC/src/mc.cu:379-503) on two small
-brushes, with every input stored beside the outputs:
+brushes with PME and one with coulombtype coul_cut, with every input stored
+beside the outputs:
 
   - system: positions, charges, 0-based types, bonds, box (data-file height);
   - parameters: cutoffs, epsilon_r, Ewald alpha, spline order, grid, slab flag;
@@ -9,6 +10,9 @@ brushes, with every input stored beside the outputs:
     components, and a 400-trial fixed-mu GC chain (seed 5): direction, site,
     ion, decision, Delta-E, Metropolis exponent and uniform per trial, final identities
     and energy.
+
+The oracle's energies are OpenMP sums, so a rerun reproduces the floating-point
+vectors to about 1e-13 (the tests allow 1e-12) and the integer ones exactly.
 
 Run from the repository root: python tests/golden/make_golden.py
 """
@@ -22,18 +26,22 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 
 from oracle import pyoracle as O  # noqa: E402
+from ph_brush_amd import _lib as L  # noqa: E402
 from ph_brush_amd import system as S  # noqa: E402
 
-CASES = [("o4", dict(order=4, grid=(16, 16, 48), n_perm=2, n_salt=8), 7.9),
-         ("o5", dict(order=5, grid=(16, 20, 45), n_perm=0, n_salt=6), 7.5)]
+CASES = [("o4", dict(order=4, grid=(16, 16, 48), n_perm=2, n_salt=8), 7.9, "pme"),
+         ("o5", dict(order=5, grid=(16, 20, 45), n_perm=0, n_salt=6), 7.5, "pme"),
+         ("c4", dict(order=4, grid=(16, 16, 48), n_perm=2, n_salt=8), 7.9, "coul_cut")]
 NTRIALS, SEED = 400, 5
 
 
 def main():
     out = {}
     meta = {}
-    for name, kw, ph in CASES:
+    for name, kw, ph, coulomb in CASES:
         s, p, _ = S.config_tiny(**kw)
+        if coulomb == "coul_cut":
+            p.coulombtype = L.PHB_COUL_CUT
         mc = S.McParams.at_ph(ph)
         o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
         head, lst = o.nlist()
@@ -50,7 +58,7 @@ def main():
         meta[name] = {"box": [s.Lx, s.Ly, s.Lz, s.Lz_data],
                       "params": {"rc": p.rc, "lj_rc": p.lj_rc, "lj_shift": p.lj_shift, "epsilon_r": p.epsilon_r,
                                  "ewald_alpha": p.ewald_alpha, "pme_order": p.pme_order, "grid": list(p.grid),
-                                 "pme_3dc": p.pme_3dc, "ntypes": p.ntypes},
+                                 "pme_3dc": p.pme_3dc, "ntypes": p.ntypes, "coulombtype": p.coulombtype},
                       "mc": {"nh_n_type": mc.nh_n_type, "nh_p_type": mc.nh_p_type, "w_type": mc.w_type,
                              "cl_type": mc.cl_type, "beta": mc.beta, "mu_eff": mc.mu_eff, "w_z_extra": mc.w_z_extra},
                       "ntrials": NTRIALS, "seed": SEED}

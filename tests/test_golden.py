@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
+from ph_brush_amd import _lib as L
 from ph_brush_amd import system as S
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_tiny.npz")
@@ -37,12 +38,12 @@ def _case(z, meta, name):
     pp = m["params"]
     p = S.Params(rc=pp["rc"], lj_rc=pp["lj_rc"], lj_shift=pp["lj_shift"], epsilon_r=pp["epsilon_r"],
                  ewald_alpha=pp["ewald_alpha"], pme_order=pp["pme_order"], grid=tuple(pp["grid"]),
-                 pme_3dc=pp["pme_3dc"], ntypes=pp["ntypes"])
+                 pme_3dc=pp["pme_3dc"], ntypes=pp["ntypes"], coulombtype=pp.get("coulombtype", L.PHB_COUL_PME))
     mc = S.McParams(**m["mc"])
     return s, p, mc, g
 
 
-CASES = ["o4", "o5"]
+CASES = ["o4", "o5", "c4"]   # c4: coulombtype coul_cut
 
 
 def _rel(a, b):
