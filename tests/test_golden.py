@@ -98,3 +98,57 @@ def test_device_matches_golden_vectors(gpu, name):
         q, ty = r.atoms.identity()
         assert np.array_equal(ty, g("type0_end")) and np.array_equal(q, g("q_end"))
         assert _rel(st["E"], g("E_end")[0]) <= 1e-9
+
+
+def _phases_case():
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
+    return s, p, S.McParams.at_ph(7.0)
+
+
+P1_KEYS = ("attempted", "accepted", "accepted_forward", "accepted_backward", "blocks_done", "Nh", "Nprot",
+           "target_count")
+
+
+def test_oracle_reproduces_golden_phases():
+    z, meta = _load()
+    ph = meta["phases"]
+    s, p, mc = _phases_case()
+    o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
+    o.nlist()
+    ro = o.phase1_run(mc, ph["seed"], ph["target_pct"], gc_block=ph["gc_block"],
+                      epoch_attempt_max=ph["epoch_attempt_max"])
+    assert [ro[k] for k in P1_KEYS] == list(z["p1_counts"])
+    assert ro["mu_eff"] == z["p1_mu"][0]
+    assert np.array_equal(ro["type0"], z["p1_type0"]) and np.array_equal(ro["rng_state"], z["p1_rng"])
+    assert np.array_equal(ro["handover"], z["p1_handover"])
+    s1 = S.System(s.x, s.y, s.z, ro["q"], ro["type0"], s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data)
+    o2 = O.Oracle(s1, p, S.bond_adjacency(s.natoms, s.bonds))
+    o2.nlist()
+    r2 = o2.phase2_run(mc, ro["rng_state"], ph["phase2_trials"], ph["patience"], pools=ro["handover"],
+                       pool_sizes=ro["handover_sizes"])
+    assert np.array_equal(r2["atoms"], z["p2_atoms"]) and np.array_equal(r2["acc"], z["p2_acc"])
+    assert np.all(np.abs(r2["dE"] - z["p2_dE"]) <= 1e-12 * np.maximum(1.0, np.abs(z["p2_dE"])))
+    assert np.array_equal(r2["type0"], z["p2_type0"]) and r2["mode"] == z["p2_mode"][0]
+
+
+@pytest.mark.gpu
+def test_device_matches_golden_phases(gpu):
+    z, meta = _load()
+    ph = meta["phases"]
+    s, p, mc = _phases_case()
+    r = S.Replica(s, p, mc, seed=ph["seed"], pending_max=8, recompute_interval=400)
+    g = r.mc.phase1_run(ph["target_pct"], gc_block=ph["gc_block"], epoch_attempt_max=ph["epoch_attempt_max"])
+    assert [g[k] for k in P1_KEYS] == list(z["p1_counts"])
+    assert g["mu_eff"] == z["p1_mu"][0]
+    q, t = r.atoms.identity()
+    assert np.array_equal(t, z["p1_type0"]) and np.array_equal(r.mc.rng_state(), z["p1_rng"])
+    n2 = ph["phase2_trials"]
+    r.mc.phase2_begin(patience=ph["patience"], record_cap=n2)
+    st = r.mc.phase2_run(n2)
+    rec = r.mc.phase2_record(st["attempted"])
+    assert st["attempted"] == n2 and st["mode"] == z["p2_mode"][0]
+    assert np.array_equal(rec["atoms"], z["p2_atoms"]) and np.array_equal(rec["acc"], z["p2_acc"])
+    dE = z["p2_dE"]
+    assert np.all(np.abs(rec["dE"] - dE) <= 1e-10 * np.maximum(1.0, np.abs(dE)) + 1e-14 * abs(z["p2_E"][0]))
+    q, t = r.atoms.identity()
+    assert np.array_equal(t, z["p2_type0"])
