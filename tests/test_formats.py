@@ -164,12 +164,7 @@ def test_generated_brush_round_trip_matches_reference_io(tmp_path, lib_dump):
 
 
 def _brush_datafile(s: S.System) -> F.DataFile:
-    n = s.natoms
-    nb = s.bonds.shape[0]
-    return F.DataFile(n, nb, 8, s.Lx, s.Ly, s.Lz_data, np.arange(1, n + 1, dtype=np.int32), s.mol.astype(np.int32),
-                      (s.type0 + 1).astype(np.int32), s.q.copy(), s.x.copy(), s.y.copy(), s.z.copy(),
-                      np.arange(1, nb + 1, dtype=np.int32), np.ones(nb, np.int32), s.bonds[:, 0].copy(),
-                      s.bonds[:, 1].copy())
+    return F.data_file_of(s, 8)
 
 
 def test_round_trip_keeps_rows(tmp_path):

@@ -35,13 +35,8 @@ def _mc_exe():
 
 def _inputs(tmp_path, target_pct, patience=300, max_attempts=600, interval=50):
     s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
-    n, nb = s.natoms, s.bonds.shape[0]
-    d = F.DataFile(n, nb, 8, s.Lx, s.Ly, s.Lz_data, np.arange(1, n + 1, dtype=np.int32), s.mol.astype(np.int32),
-                   (s.type0 + 1).astype(np.int32), s.q.copy(), s.x.copy(), s.y.copy(), s.z.copy(),
-                   np.arange(1, nb + 1, dtype=np.int32), np.ones(nb, np.int32), s.bonds[:, 0].copy(),
-                   s.bonds[:, 1].copy())
     data = tmp_path / "in.data"
-    F.write_lammps_full(data, d)
+    F.write_lammps_full(data, F.data_file_of(s, 8))
     prm = tmp_path / "params.in"
     lines = [f"cutoff_nm = {p.rc!r}", f"lj_cutoff_nm = {p.lj_rc!r}", "lj_shift = yes", "coulombtype = pme",
              f"epsilon_r = {p.epsilon_r!r}", f"ewald_alpha = {p.ewald_alpha!r}", "pme_order = 5",
