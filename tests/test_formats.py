@@ -301,3 +301,32 @@ def test_workload_files_read_back_to_the_generated_workload(tmp_path):
     np.testing.assert_array_equal(p.lj_eps, p0.lj_eps)
     assert (mc.beta, mc.w_z_extra, mc.nh_n_type, mc.cl_type) == (mc0.beta, mc0.w_z_extra, mc0.nh_n_type, mc0.cl_type)
     assert (pf.target_protonation_pct, pf.rng_seed) == (55.0, 9)
+
+
+MC = ROOT / "ph_brush_amd" / "bin" / "MC"
+
+
+@pytest.mark.parametrize("pname,msg", [
+    ("err_direct", "direct_pct_threshold must be between 10 and 100"),
+    ("err_bool", "invalid boolean: maybe"),
+    ("err_stoi", "stoi"),
+])
+def test_mc_program_file_errors_before_any_device_work(tmp_path, pname, msg):
+    """MC/src/main.cu:224-227: a file error prints "Error: <message>" and exits 2;
+    these stop before the first device call, so they run without a GPU."""
+    d, p = _files(tmp_path, "mixed", pname)
+    r = subprocess.run([str(MC), str(d), str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and r.stderr == f"Error: {msg}\n"
+    r = subprocess.run([str(MC), str(tmp_path / "none.data"), str(p)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and r.stderr == f"Error: Failed to open LAMMPS data: {tmp_path / 'none.data'}\n"
+
+
+def test_mc_program_usage(tmp_path):
+    """MC/src/main.cu:69-89: usage errors exit 1."""
+    r = subprocess.run([str(MC)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and r.stderr == f"Usage: {MC} in.data params.in [--write-data out.data]\n"
+    r = subprocess.run([str(MC), "a", "b", "--write-data"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and r.stderr == "--write-data requires a path\n"
+    r = subprocess.run([str(MC), "a", "b", "-x"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and r.stderr == "Unrecognized argument: -x\n"
