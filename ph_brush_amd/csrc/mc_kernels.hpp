@@ -54,8 +54,12 @@ __device__ __forceinline__ void chain_body(const McDev& D) {
             const long long done = s_st.done;
             const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
             int ntr = 0, stop = 0;
-            if (done >= stop_at || s_st.pend_n + 2 > pend_max || status != 0) stop = 1;
+            // the target comes first: when the trial that reaches it is also the last
+            // of the launch's trials, the run still ends there (status 4), as the
+            // reference returns from inside its block loop (MC/src/mc.cu:456-468)
+            if (status != 0) stop = 1;
             else if (ntg >= 0 && s_st.nprot == ntg) { status = 4; stop = 1; }
+            else if (done >= stop_at || s_st.pend_n + 2 > pend_max) stop = 1;
             else {
                 const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
                 if (wf == 0 && wb == 0) { status = 1; stop = 1; }

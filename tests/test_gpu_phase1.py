@@ -17,10 +17,9 @@ from ph_brush_amd import system as S
 pytestmark = pytest.mark.gpu
 
 
-def _both(target_pct, **kw):
+def _both(target_pct, seed=5, **kw):
     s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
     mc = S.McParams.at_ph(7.0)
-    seed = 5
     r = S.Replica(s, p, mc, seed=seed, pending_max=8, recompute_interval=400)
     g = r.mc.phase1_run(target_pct, **kw)
     o = O.Oracle(s, p, r.bonds_host)
@@ -72,7 +71,7 @@ def _check_events(r, g, ro, mc):
     blocks = [e for e in ev if e["kind"] in (L.PHB_P1_CHUNK, L.PHB_P1_REACHED)]
     assert sum(e["block_attempted"] for e in blocks) == ro["attempted"]
     assert sum(e["block_accepted"] for e in blocks) == ro["accepted"]
-    assert last["step_in_block"] == last["block_attempted"] <= 200
+    assert last["step_in_block"] == last["block_attempted"] <= last["gc_block"]
 
 
 def test_phase1_mu_control_then_phase2(gpu):
@@ -94,3 +93,22 @@ def test_phase1_direct_then_phase2(gpu):
     e_re = r.mc.recompute()
     assert abs(g["E"] - e_re) <= 1e-9 * max(1.0, abs(e_re))
     _check_phase2(s, p, mc, r, ro)
+
+
+@pytest.mark.parametrize("seed,gc_block,amax,nacc_frac,epochs", [
+    (6, 2211, 10 ** 7, 1.0, 0),   # the target lands on the last trial of the first block
+    (6, 36, 300, 0.10, 3),        # ... of a block after three mu updates
+])
+def test_phase1_target_on_last_trial_of_a_block(gpu, seed, gc_block, amax, nacc_frac, epochs):
+    """The trial that reaches the target is the last of its block: the run ends
+    there (step_in_block = gc_block), with no chunk line and no mu update after it
+    (MC/src/mc.cu:456-468 returns from inside the block loop)."""
+    from ph_brush_amd import _lib as L
+    s, p, mc, r, g, o, ro = _both(75.0, seed=seed, gc_block=gc_block, epoch_attempt_max=amax,
+                                  nacc_frac=nacc_frac, max_trials=200000)
+    assert ro["reached_target"] == 1 and ro["attempted"] % gc_block == 0 and ro["blocks_done"] == epochs
+    _check_phase1(r, g, ro)
+    _check_events(r, g, ro, mc)
+    ev = r.mc.phase1_events()
+    assert ev[-1]["step_in_block"] == gc_block
+    assert sum(e["kind"] == L.PHB_P1_REACHED for e in ev) == 1

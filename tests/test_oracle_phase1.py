@@ -106,3 +106,21 @@ def test_phase1_mu_moves_toward_target():
     mu0 = kT * math.log(fstar / (1.0 - fstar))
     bound = kT * 0.3 * abs(math.log(f / (1.0 - f)) - math.log(fstar / (1.0 - fstar)))
     assert 0.0 < r["mu_eff"] - mu0 <= bound * (1 + 1e-12)
+
+
+def test_phase1_target_on_a_block_end_returns_there():
+    """Within one epoch the chain does not depend on gc_block, so with gc_block equal
+    to the trial count at which the target is reached, that trial is the block's last:
+    the run returns there with no mu update (MC/src/mc.cu:456-468 returns from inside
+    the block loop, before the chunk line and the epoch's mu step)."""
+    s, p, o = _tiny()
+    mc = S.McParams.at_ph(7.0)
+    kw = dict(epoch_attempt_max=10 ** 7, nacc_frac=1.0, max_trials=10 ** 6)
+    r0 = o.phase1_run(mc, 6, target_pct=75.0, gc_block=10 ** 6, **kw)
+    assert r0["rc"] == 0 and r0["reached_target"] == 1 and r0["blocks_done"] == 0
+    T = r0["attempted"]
+    r1 = o.phase1_run(mc, 6, target_pct=75.0, gc_block=T, **kw)
+    assert r1["rc"] == 0 and r1["reached_target"] == 1
+    assert (r1["attempted"], r1["accepted"], r1["blocks_done"]) == (T, r0["accepted"], 0)
+    assert r1["mu_eff"] == r0["mu_eff"]
+    assert np.array_equal(r1["type0"], r0["type0"]) and np.array_equal(r1["rng_state"], r0["rng_state"])
