@@ -348,3 +348,47 @@ def test_gc_chain_samples_exact_distribution():
         counts[states.index(tuple(cur))] += 1
     f = counts / n
     assert np.abs(f - pi).max() < 0.015, (f, pi)
+
+
+def test_mixed_draws_from_a_state():
+    """The proposal stream's mix (ints with Lemire rejections, canonical doubles)
+    from an engine state, against the raw stream and the published algorithms;
+    the state form round-trips (a state taken mid-stream resumes the stream)."""
+    rng = np.random.default_rng(3)
+    n = 3000
+    his = rng.integers(0, 5000, size=n).astype(np.int32)
+    his[rng.random(n) < 0.3] = -1                      # canonical doubles
+    his[rng.random(n) < 0.2] = 1431655765              # 2^32/3 + 1 values: rejection ~1/3 of draws
+    his[:4] = [2 ** 31 - 1, -1, 0, 1431655765]
+    st0 = O.seed_state(19)
+    got, after = O.mixed_draws(st0, his)
+    stream = O.mt_draws(19, 3 * n + 100)
+    pos = 0
+    nrej = 0
+    for h, v in zip(his, got):
+        if h >= 0:
+            r, used = _lemire(stream[pos:], int(h) + 1)
+            nrej += used - 1
+            pos += used
+            assert r == v
+        else:
+            g1, g2 = float(stream[pos]), float(stream[pos + 1])
+            pos += 2
+            e = (g1 + g2 * 4294967296.0) / 18446744073709551616.0
+            e = min(e, np.nextafter(1.0, 0.0))
+            assert np.float64(e).view(np.uint64) == v
+    assert nrej > 100
+    # resuming from the state after the draws continues the raw stream
+    more, _ = O.mixed_draws(after, np.full(5, 2 ** 31 - 1, np.int32))
+    exp = [_lemire(stream[pos + k:], 2 ** 31)[0] for k in range(5)]
+    assert list(more) == exp
+
+
+def test_seed_state_is_the_standard_seeding():
+    """std::mt19937(seed)'s stream form: the standard's seeding recurrence, index 624;
+    the C ABI's host helper gives the same words."""
+    from ph_brush_amd import system as S
+    st = O.seed_state(5489)
+    assert st[624] == 624 and st[0] == 5489 and st[1] == 1301868182
+    assert np.array_equal(S.rng_seed_state(5489), st)
+    assert np.array_equal(S.rng_seed_state(2025), O.seed_state(2025))
