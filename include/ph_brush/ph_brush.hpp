@@ -5,12 +5,26 @@
 // std::runtime_error carrying the library's message (the reference's CUDA
 // checks throw the same type, e.g. MC/src/energy_gpu.cu:7-10).
 //
+// The move API keeps the reference's free functions with their exact
+// signatures (MC/include/mc.hpp:50-81): build_plan, apply_direct_protonation,
+// phase_gc_mu_control (mutating p.mu_eff and advancing the caller's
+// std::mt19937) and phase2_swap_jitter, plus main's upload_atoms_to_device /
+// upload_params_to_device (MC/src/main.cu:18-66).  Each move function runs its
+// chain on the device and writes the identities back into the host System, as
+// the reference's set_atom_identity keeps its host mirror.
+//
 // Differences a caller sees:
 //   - device objects own opaque handles instead of thrust vectors;
 //   - `real` is double (the FP64 plain path); the FP32 form is selected per
 //     PME plan at run time;
-//   - the Monte Carlo entry point advances a fixed-mu chain on the device
-//     (phase_gc_fixed_mu) instead of the host loop of phase_gc_mu_control.
+//   - the move functions write no report.txt lines (the MC program writes them
+//     from the phase-1 events and phase-2 records); result counters are 64-bit;
+//   - the MCPlan given to phase_gc_mu_control must describe the system's current
+//     identities (build_plan(sys, p) on the same system, as main does), since the
+//     device chain builds the same pools from them; a different plan is refused;
+//   - the MonteCarlo class keeps one device chain across phases (its engine,
+//     potential grid and pools), for callers that run many blocks or a fixed-mu
+//     titration (phase_gc_fixed_mu).
 #pragma once
 
 #include <algorithm>
@@ -18,6 +32,7 @@
 #include <exception>
 #include <functional>
 #include <iomanip>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -162,8 +177,10 @@ inline DeviceCSR build_half_neighbors_gpu(const DeviceAtoms& atoms, const Box& b
 struct DeviceParams : Handle<phb_params, phb_params_destroy> {
     int ntypes{0};
     std::string coulombtype;
+    bool pme_3dc_enabled{false};
     DeviceParams() = default;
-    explicit DeviceParams(const Params& p) : ntypes(p.ntypes), coulombtype(p.coulombtype) {
+    explicit DeviceParams(const Params& p) : ntypes(p.ntypes), coulombtype(p.coulombtype),
+                                             pme_3dc_enabled(p.pme_3dc_enabled) {
         phb_params_desc d{};
         d.ntypes = p.ntypes;
         d.lj_eps = p.lj_eps.data();
@@ -186,6 +203,30 @@ struct DeviceParams : Handle<phb_params, phb_params_destroy> {
         h_ = h;
     }
 };
+
+// upload_atoms_to_device / upload_params_to_device (static in MC/src/main.cu:18-66):
+// SoA positions, charges and 0-based types; the parameter tables.
+inline DeviceAtoms upload_atoms_to_device(const System& sys) {
+    if (sys.natoms <= 0) return DeviceAtoms();
+    return DeviceAtoms(sys);
+}
+inline DeviceParams upload_params_to_device(const Params& p) { return DeviceParams(p); }
+
+// std::mt19937 <-> the library's engine state: the engine's stream form
+// (624 words, then the index), which operator<< / operator>> define.
+inline std::vector<unsigned> engine_words(const std::mt19937& rng) {
+    std::stringstream ss;
+    ss << rng;
+    std::vector<unsigned> w(625);
+    for (auto& x : w) ss >> x;
+    return w;
+}
+inline void engine_from_words(std::mt19937& rng, const unsigned* w) {
+    std::stringstream ss;
+    for (int k = 0; k < 625; ++k) ss << w[k] << ' ';
+    ss >> rng;
+    if (!ss) throw std::runtime_error("engine state: not a std::mt19937 stream form");
+}
 
 // compute_lj_sr_energy_gpu / compute_coul_sr_energy_gpu (MC/include/energy.hpp:7-18)
 inline real compute_lj_sr_energy_gpu(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, const Box& box) {
@@ -218,6 +259,7 @@ inline void pmeCreate(PMEPlan* plan, const DeviceParams& params, const Box& box,
     phb_box b{box.Lx, box.Ly, box.Lz};
     check(phb_pme_create(params.get(), &b, rc, precision, &plan->handle));
     check(phb_pme_info(plan->handle, &plan->nx, &plan->ny, &plan->nz, &plan->order, &plan->alpha));
+    plan->use3dc = params.pme_3dc_enabled;   // MC/src/pme.cu:296
 }
 
 inline void pmeDestroy(PMEPlan* plan) {
@@ -278,7 +320,12 @@ struct Phase2Result {
 class MonteCarlo : public Handle<phb_mc, phb_mc_destroy> {
 public:
     MonteCarlo(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, PMEPlan& plan, const Box& box,
+               const Params& hp, int pending_max = 8, int spec_window = 8, int recompute_interval = 1000)
+        : MonteCarlo(a, p, neigh, &plan, box, hp, pending_max, spec_window, recompute_interval) {}
+    // plan may be null for coulombtype coul_cut (total_energy's other branch, MC/src/mc.cu:52-56)
+    MonteCarlo(const DeviceAtoms& a, const DeviceParams& p, const DeviceCSR& neigh, PMEPlan* plan, const Box& box,
                const Params& hp, int pending_max = 8, int spec_window = 8, int recompute_interval = 1000) {
+        if (p.coulombtype == "pme" && (!plan || !plan->handle)) throw std::runtime_error("total_energy: PME plan is null");
         phb_box b{box.Lx, box.Ly, box.Lz};
         phb_mc_desc d{};
         d.nh_n_type = hp.NH_N_type; d.nh_p_type = hp.NH_P_type; d.w_type = hp.W_type; d.cl_type = hp.Cl_type;
@@ -287,8 +334,29 @@ public:
         d.record_cap = 0; d.spec_window = spec_window;
         mu_ = hp.mu_eff;
         phb_mc h = nullptr;
-        check(phb_mc_create(a.get(), p.get(), neigh.get(), plan.handle, &b, &d, (unsigned)hp.rng_seed, &h));
+        check(phb_mc_create(a.get(), p.get(), neigh.get(), plan ? plan->handle : nullptr, &b, &d,
+                            (unsigned)hp.rng_seed, &h));
         h_ = h;
+    }
+    // The chain continues from (set_engine) or hands back (engine_to) a caller's engine.
+    void set_engine(const std::mt19937& rng) {
+        const std::vector<unsigned> w = engine_words(rng);
+        check(phb_mc_set_rng_state(h_, w.data()));
+    }
+    void engine_to(std::mt19937& rng) const {
+        unsigned w[625];
+        check(phb_mc_rng_state(h_, w));
+        engine_from_words(rng, w);
+    }
+    // Pools phase 2 starts from (a Phase1Result's lists, MC/src/mc.cu:172-184).
+    void set_phase2_pools(const Phase1Result& ph1) {
+        const int sz[4] = {(int)ph1.nh_n_remaining.size(), (int)ph1.nh_p_mc.size(), (int)ph1.w_remaining.size(),
+                           (int)ph1.cl_mc.size()};
+        std::vector<int> all;
+        all.reserve((size_t)sz[0] + sz[1] + sz[2] + sz[3]);
+        for (const auto* v : {&ph1.nh_n_remaining, &ph1.nh_p_mc, &ph1.w_remaining, &ph1.cl_mc})
+            all.insert(all.end(), v->begin(), v->end());
+        check(phb_phase2_set_pools(h_, sz, all.data()));
     }
     PhaseGCResult run(long long ntrials) {
         phb_mc_stats s{};
@@ -329,7 +397,7 @@ public:
         phb_phase1_result s{};
         check(phb_phase1_run(h_, &d, &s));
         if (obs.error) std::rethrow_exception(obs.error);
-        if (s.status == 1) throw std::runtime_error("[sgmc] no valid pairs to flip; stop.");  // MC/src/mc.cu:399
+        // status 1 (no valid pair to flip) ends the run and returns, as MC/src/mc.cu:397-401 does
         PhaseGCResult r;
         r.attempted = s.attempted; r.accepted = s.accepted;
         r.accepted_forward = s.accepted_forward; r.accepted_backward = s.accepted_backward;
@@ -482,5 +550,122 @@ inline PhaseGCResult phase_gc_mu_control(MonteCarlo& mc, const Params& hp, Phase
 }
 
 inline Phase2Result phase2_swap_jitter(MonteCarlo& mc, const Params& hp) { return mc.phase2(hp); }
+
+// ---------------------------------------------------------------------------
+// The reference's move functions with their exact signatures (MC/include/mc.hpp:50-81).
+
+namespace detail {
+// set_atom_identity's checks (MC/src/mc.cu:33-35)
+inline void check_identity(const System& sys, int i, int type1, int ntypes) {
+    if (i < 0 || i >= sys.natoms) throw std::runtime_error("set_atom_identity: atom index out of range");
+    if (type1 < 1 || type1 > ntypes) throw std::runtime_error("set_atom_identity: type out of range");
+}
+// The pools the device chain builds from sys (MC/src/mc.cu:67-100, :332-341) against the caller's plan.
+inline void check_plan_matches(const System& sys, const Params& p, const MCPlan& mcplan, const char* who) {
+    std::vector<int> nh_n, nh_p, w;
+    real z_max = 0.0;
+    for (int i = 0; i < sys.natoms; ++i) {
+        const int t = sys.atoms[i].type;
+        if (t == p.NH_N_type) nh_n.push_back(i);
+        else if (t == p.NH_P_type) nh_p.push_back(i);
+    }
+    for (int i : nh_n) z_max = std::max(z_max, sys.atoms[i].z);
+    for (int i : nh_p) z_max = std::max(z_max, sys.atoms[i].z);
+    const real z_cut = z_max + p.w_z_extra_nm;
+    for (int i = 0; i < sys.natoms; ++i)
+        if (sys.atoms[i].type == p.W_type && sys.atoms[i].z < z_cut) w.push_back(i);
+    if (nh_n != mcplan.idx_nh_n || nh_p != mcplan.idx_nh_p || w != mcplan.idx_w || z_cut != mcplan.z_cut)
+        throw std::runtime_error(std::string(who) +
+                                 ": mcplan does not describe the system's identities (use build_plan(sys, p))");
+}
+}  // namespace detail
+
+// apply_direct_protonation (MC/include/mc.hpp:52-59, MC/src/mc.cu:124-157): the
+// first to_flip NH_N and W of the plan become NH_P (+1) and Cl (-1) in one
+// batched upload; E is the total energy after the flips.
+inline Phase1Result apply_direct_protonation(System& sys, DeviceAtoms& d_atoms, const DeviceParams& d_params,
+                                             const DeviceCSR& d_neigh, const Box& box, PMEPlan* plan,
+                                             const Params& p, const MCPlan& mcplan) {
+    Phase1Result R;
+    R.nh_n_remaining = mcplan.idx_nh_n;
+    R.w_remaining = mcplan.idx_w;
+    const int need = mcplan.to_flip;
+    if (need <= 0) {
+        R.E = total_energy(d_atoms, d_params, d_neigh, box, plan);
+        return R;
+    }
+    if ((int)R.nh_n_remaining.size() < need || (int)R.w_remaining.size() < need)
+        throw std::runtime_error("[direct] pools smaller than to_flip");
+    std::vector<int> idx, ty0;
+    std::vector<double> q;
+    for (int k = 0; k < need; ++k) {
+        const int i_nh = R.nh_n_remaining[k], j_w = R.w_remaining[k];
+        detail::check_identity(sys, i_nh, p.NH_P_type, d_params.ntypes);
+        detail::check_identity(sys, j_w, p.Cl_type, d_params.ntypes);
+        idx.push_back(i_nh); ty0.push_back(p.NH_P_type - 1); q.push_back(+1.0);
+        idx.push_back(j_w); ty0.push_back(p.Cl_type - 1); q.push_back(-1.0);
+        R.nh_p_mc.push_back(i_nh);
+        R.cl_mc.push_back(j_w);
+    }
+    check(phb_atoms_set_identity(d_atoms.get(), (int)idx.size(), idx.data(), ty0.data(), q.data()));
+    for (size_t k = 0; k < idx.size(); ++k) { sys.atoms[idx[k]].type = ty0[k] + 1; sys.atoms[idx[k]].q = q[k]; }
+    R.nh_n_remaining.erase(R.nh_n_remaining.begin(), R.nh_n_remaining.begin() + need);
+    R.w_remaining.erase(R.w_remaining.begin(), R.w_remaining.begin() + need);
+    R.attempted = need;
+    R.accepted = need;
+    R.E = total_energy(d_atoms, d_params, d_neigh, box, plan);
+    return R;
+}
+
+// phase_gc_mu_control (MC/include/mc.hpp:72-81, MC/src/mc.cu:320-550): the
+// mu-control GC loop on a device chain seeded from `rng`; afterwards rng has
+// advanced exactly as the reference's engine, p.mu_eff holds the final mu, sys
+// the final identities, and *seed_out the ascending hand-over (with E) when the
+// target was reached by a move.
+inline PhaseGCResult phase_gc_mu_control(System& sys, DeviceAtoms& d_atoms, const DeviceParams& d_params,
+                                         const DeviceCSR& d_neigh, const Box& box, PMEPlan* plan, Params& p,
+                                         const MCPlan& mcplan, std::mt19937& rng, Phase1Result* seed_out = nullptr) {
+    detail::check_plan_matches(sys, p, mcplan, "phase_gc_mu_control");
+    MonteCarlo mc(d_atoms, d_params, d_neigh, plan, box, p);
+    mc.set_engine(rng);
+    Params q = p;
+    q.direct_pct_threshold = 1000;   // the loop itself, whatever the threshold
+    const PhaseGCResult r = mc.phase1(q);
+    mc.engine_to(rng);
+    p.mu_eff = r.mu_eff;
+    download_identities(d_atoms, sys);
+    if (seed_out && r.reached_target && r.attempted > 0) {
+        const Phase1Result h = mc.handover();
+        seed_out->nh_p_mc = h.nh_p_mc;
+        seed_out->nh_n_remaining = h.nh_n_remaining;
+        seed_out->w_remaining = h.w_remaining;
+        seed_out->cl_mc = h.cl_mc;
+        seed_out->E = r.E;
+    }
+    return r;
+}
+
+// phase2_swap_jitter (MC/include/mc.hpp:61-70, MC/src/mc.cu:159-318): 4-point or
+// W-Cl-only moves from ph1's pools (W and Cl below mcplan.z_cut), patience and
+// max_attempts from p, continuing `rng`; sys gets the final identities.
+inline Phase2Result phase2_swap_jitter(System& sys, DeviceAtoms& d_atoms, const DeviceParams& d_params,
+                                       const DeviceCSR& d_neigh, const Box& box, PMEPlan* plan, const Params& p,
+                                       const MCPlan& mcplan, const Phase1Result& ph1, std::mt19937& rng) {
+    Phase1Result pools = ph1;
+    auto filter_under_z = [&](std::vector<int>& pool) {
+        pool.erase(std::remove_if(pool.begin(), pool.end(),
+                                  [&](int idx) { return idx < 0 || idx >= sys.natoms || sys.atoms[idx].z >= mcplan.z_cut; }),
+                   pool.end());
+    };
+    filter_under_z(pools.w_remaining);
+    filter_under_z(pools.cl_mc);
+    MonteCarlo mc(d_atoms, d_params, d_neigh, plan, box, p);
+    mc.set_engine(rng);
+    mc.set_phase2_pools(pools);
+    const Phase2Result r = mc.phase2(p);
+    mc.engine_to(rng);
+    download_identities(d_atoms, sys);
+    return r;
+}
 
 }  // namespace ph_brush

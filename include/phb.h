@@ -223,8 +223,27 @@ int phb_mc_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, sign
 int phb_mc_delta_e(phb_mc mc, int n, const int *i, const int *j, const int *fwd, double *dE);
 /* Full recompute now (drift reset); returns the recomputed total energy. */
 int phb_mc_recompute(phb_mc mc, double *E);
-/* MT19937 state: 624 words then the index (625 unsigned). */
+/* MT19937 state: 624 words then the index (625 unsigned), the stream form of
+ * std::mt19937 (operator<< / operator>>).  get: the engine after the last run;
+ * set: the next run continues from that engine, so a caller's std::mt19937 is
+ * handed in and out as the reference passes `std::mt19937& rng` through
+ * phase_gc_mu_control and phase2_swap_jitter (MC/include/mc.hpp:58-81,
+ * MC/src/main.cu:180-206).  set rejects an index above 624. */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
+int phb_mc_set_rng_state(phb_mc mc, const unsigned *state);
+/* std::mt19937(seed) in the stream form above (host only). */
+int phb_rng_seed_state(unsigned seed, unsigned *state);
+/* The chains' device engine on its own (no reference counterpart; it checks the
+ * device form of std::mt19937, uniform_int_distribution<int> and
+ * uniform_real_distribution<double>, MC/src/mc.cu:62-65, :368-369, draw by
+ * draw).  From an engine state (625 words), draw k is
+ * uniform_int_distribution<int>(0, hi[k]) when hi[k] >= 0, else a uniform double
+ * in [0, 1) (out[k] = its IEEE bits).  Draws go in rounds of 64 read at fixed
+ * stream positions, a round with a Lemire rejection redrawn serially, as the
+ * chains propose.  state_after (may be NULL): the engine afterwards;
+ * serial_rounds (may be NULL): rounds that took the serial path. */
+int phb_rng_draws(const unsigned *state, long long n, const int *hi, unsigned long long *out,
+                  unsigned *state_after, long long *serial_rounds);
 
 /* What the last phb_phase1_run did, block by block: the events its reference
  * counterpart reports as it runs (MC/src/mc.cu:153-155, :356-358, :399, :471-479,
@@ -310,6 +329,12 @@ typedef struct {
     int since_best, status;
     double E, best_E;
 } phb_phase2_stats;
+/* The pools phase 2 starts from, given by the caller in phb_phase1_handover's
+ * order (nh_n, nh_p, w, cl; sizes[4], lists concatenated): the Phase1Result
+ * lists phase2_swap_jitter receives (MC/include/mc.hpp:61-70, MC/src/mc.cu:172-184).
+ * They replace any hand-over of phb_phase1_run; begin then filters W and Cl by
+ * the pool z cut.  Indices are checked against the atom count. */
+int phb_phase2_set_pools(phb_mc mc, const int sizes[4], const int *lists);
 int phb_phase2_begin(phb_mc mc, const phb_phase2_desc *d, phb_phase2_stats *out);
 int phb_phase2_run(phb_mc mc, long long max_attempts, phb_phase2_stats *out);
 /* First n trials: atoms (4 per trial in draw order iP, iN, jCl, jW; W-Cl only: jCl, jW, -1, -1),

@@ -17,9 +17,9 @@
 #include <cstring>
 #include <limits>
 #include <random>
+#include <sstream>
 #include <stdexcept>
 #include <vector>
-#include <sstream>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -650,6 +650,33 @@ void or_canonical_draws(unsigned seed, long long n, double* out) {
     for (long long k = 0; k < n; ++k) out[k] = d(rng);
 }
 
+// The proposal stream's mix: hi[k] >= 0 draws uniform_int_distribution<int>(0, hi[k])
+// (MC/src/mc.cu:62-65), hi[k] < 0 draws uniform_real_distribution<double>(0, 1)
+// (MC/src/mc.cu:368-369), from one std::mt19937 resumed from its stream form
+// (624 words + index, as main's engine between phases, MC/src/main.cu:180-206);
+// state_after: the engine in the same form after the draws.
+void or_mixed_draws(const unsigned* state, long long n, const int* hi, unsigned long long* out, unsigned* state_after) {
+    std::mt19937 rng;
+    {
+        std::stringstream ss;
+        for (int k = 0; k < 625; ++k) ss << state[k] << ' ';
+        ss >> rng;
+    }
+    std::uniform_real_distribution<double> u01(0.0, 1.0);
+    for (long long k = 0; k < n; ++k) {
+        if (hi[k] >= 0) {
+            std::uniform_int_distribution<int> d(0, hi[k]);
+            out[k] = (unsigned long long)(unsigned)d(rng);
+        } else {
+            const double v = u01(rng);
+            std::memcpy(&out[k], &v, sizeof v);
+        }
+    }
+    std::stringstream ss;
+    ss << rng;
+    for (int k = 0; k < 625; ++k) ss >> state_after[k];
+}
+
 // ---------------------------------------------------------------------------
 // Delayed-acceptance surrogate (EXTENSION; no reference counterpart).
 void or_desc_thresholds(double r_desc, double* th) {
@@ -873,7 +900,7 @@ int or_phase2_run(const or_system* s0, const or_params* p, const or_mc_params* m
                   const unsigned* rng_state, const int* pools, const int* pool_sizes, long long max_attempts,
                   int patience, double best_eps, int* rec_atoms, signed char* rec_acc, double* rec_dE,
                   double* q_out, int* type_out, double* E_out, double* best_out, long long* attempted,
-                  long long* accepted, int* mode, int* stop) {
+                  long long* accepted, int* mode, int* stop, unsigned* rng_after) {
     const int N = s0->natoms;
     std::vector<double> q(s0->q, s0->q + N);
     std::vector<int> type(s0->type, s0->type + N);
@@ -975,6 +1002,11 @@ int or_phase2_run(const or_system* s0, const or_params* p, const or_mc_params* m
             }
             if (since_best >= patience) { st = 1; break; }
         }
+    }
+    if (rng_after) {   // the engine after phase 2, in its stream form
+        std::stringstream ss;
+        ss << rng;
+        for (int k = 0; k < 625; ++k) ss >> rng_after[k];
     }
     if (q_out) std::copy(q.begin(), q.end(), q_out);
     if (type_out) std::copy(type.begin(), type.end(), type_out);

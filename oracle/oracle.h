@@ -109,7 +109,7 @@ int or_phase2_run(const or_system *s, const or_params *p, const or_mc_params *m,
                   const int *pool_sizes, long long max_attempts, int patience, double best_eps,
                   int *rec_atoms, signed char *rec_acc, double *rec_dE, double *q_out, int *type_out,
                   double *E_out, double *best_out, long long *attempted, long long *accepted, int *mode,
-                  int *stop);
+                  int *stop, unsigned *rng_after);
 
 /* ---- phase 1 as main() runs it: build_plan checks, then direct flips above the
  * threshold or the mu-control GC loop (MC/src/mc.cu:67-157, :320-546). ---- */
@@ -128,6 +128,8 @@ int or_phase1_run(const or_system *s, const or_params *p, const or_mc_params *m,
 void or_mt_draws(unsigned seed, long long n, unsigned *out);
 void or_uniform_int_draws(unsigned seed, long long n, const int *range_hi, int *out);
 void or_canonical_draws(unsigned seed, long long n, double *out);
+void or_mixed_draws(const unsigned *state, long long n, const int *hi, unsigned long long *out,
+                    unsigned *state_after);
 
 /* ---- delayed-acceptance surrogate (EXTENSION: the reference has no learned part) ----
  * Descriptor of a candidate move (site i, ion j): d = h_i - h_j with 48 counts

@@ -112,7 +112,7 @@ def load():
         lib.or_phase2_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
                                       _pu, _pi, _pi, C.c_longlong, C.c_int, C.c_double, _pi, _pb, _pd, _pd, _pi,
                                       _pd, _pd, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
-                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), _pu]
         lib.or_phase1_run.restype = C.c_int
         lib.or_phase1_run.argtypes = [C.POINTER(OrSystem), C.POINTER(OrParams), C.POINTER(OrMcParams), _pi, _pi,
                                       C.c_uint, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -123,6 +123,7 @@ def load():
         lib.or_mt_draws.argtypes = [C.c_uint, C.c_longlong, _pu]
         lib.or_uniform_int_draws.argtypes = [C.c_uint, C.c_longlong, _pi, _pi]
         lib.or_canonical_draws.argtypes = [C.c_uint, C.c_longlong, _pd]
+        lib.or_mixed_draws.argtypes = [_pu, C.c_longlong, _pi, C.POINTER(C.c_ulonglong), _pu]
         lib.or_set_threads.argtypes = [C.c_int]
         _lib = lib
     return _lib
@@ -259,6 +260,7 @@ class Oracle:
         m = OrMcParams(mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type, mc.beta, mc.mu_eff, mc.w_z_extra)
         q, t = np.zeros(N), np.zeros(N, np.int32)
         E, best = C.c_double(), C.c_double()
+        after = np.zeros(625, np.uint32)
         att, nacc = C.c_longlong(), C.c_longlong()
         mode, stop = C.c_int(), C.c_int()
         pl = np.ascontiguousarray(pools, np.int32) if pools is not None else None
@@ -267,10 +269,12 @@ class Oracle:
                                pl.ctypes.data_as(_pi) if pl is not None else None,
                                ps.ctypes.data_as(_pi) if ps is not None else None, max_attempts, patience, best_eps, atoms.ctypes.data_as(_pi), acc.ctypes.data_as(_pb),
                                dE.ctypes.data_as(_pd), q.ctypes.data_as(_pd), t.ctypes.data_as(_pi), C.byref(E),
-                               C.byref(best), C.byref(att), C.byref(nacc), C.byref(mode), C.byref(stop))
+                               C.byref(best), C.byref(att), C.byref(nacc), C.byref(mode), C.byref(stop),
+                               after.ctypes.data_as(_pu))
         n = att.value
         return {"n": n, "accepted": nacc.value, "mode": mode.value, "stop": stop.value, "E": E.value,
-                "best_E": best.value, "q": q, "type0": t, "atoms": atoms[:n], "acc": acc[:n], "dE": dE[:n]}
+                "best_E": best.value, "q": q, "type0": t, "atoms": atoms[:n], "acc": acc[:n], "dE": dE[:n],
+                "rng_state": after}
 
     def da_descriptors(self, species_of_type, r_desc, i, j):
         """Brute-force shell-count descriptors d = h_i - h_j (int, n x 48) and r_ij."""
@@ -363,6 +367,38 @@ def canonical_draws(seed, n):
     out = np.zeros(n)
     load().or_canonical_draws(seed, n, out.ctypes.data_as(_pd))
     return out
+
+
+def mixed_draws(state, his):
+    """The proposal stream's mix from an engine state (625 words): his[k] >= 0 draws
+    uniform_int_distribution<int>(0, his[k]), his[k] < 0 a canonical double (its
+    bits).  Returns the draws (uint64) and the engine state after them."""
+    st = np.ascontiguousarray(state, np.uint32)
+    assert st.shape == (625,)
+    his = np.ascontiguousarray(his, np.int32)
+    out = np.zeros(len(his), np.uint64)
+    after = np.zeros(625, np.uint32)
+    load().or_mixed_draws(st.ctypes.data_as(_pu), len(his), his.ctypes.data_as(_pi),
+                          out.ctypes.data_as(C.POINTER(C.c_ulonglong)), after.ctypes.data_as(_pu))
+    return out, after
+
+
+def seed_state(seed):
+    """std::mt19937(seed) in its stream form: its 624 words, then the index (624)."""
+    st0 = np.zeros(625, np.uint32)
+    st0[:624] = _seeded_words(seed)
+    st0[624] = 624
+    return st0
+
+
+def _seeded_words(seed):
+    # the standard's seeding recurrence x_i = 1812433253 (x_{i-1} ^ (x_{i-1} >> 30)) + i
+    w = np.zeros(624, np.uint64)
+    w[0] = seed & 0xFFFFFFFF
+    for i in range(1, 624):
+        x = int(w[i - 1])
+        w[i] = (1812433253 * (x ^ (x >> 30)) + i) & 0xFFFFFFFF
+    return w.astype(np.uint32)
 
 
 def set_threads(n):

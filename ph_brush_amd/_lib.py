@@ -215,6 +215,7 @@ SIGNATURES = {
     "phb_atoms_set_positions": [_h, _pd, _pd, _pd],
     "phb_phase1_run": [_h, C.POINTER(Phase1Desc), C.POINTER(Phase1Result)],
     "phb_phase1_handover": [_h, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_longlong],
+    "phb_phase2_set_pools": [_h, _pi, _pi],
     "phb_phase2_begin": [_h, C.POINTER(Phase2Desc), C.POINTER(Phase2Stats)],
     "phb_phase2_run": [_h, C.c_longlong, C.POINTER(Phase2Stats)],
     "phb_phase2_record": [_h, C.c_longlong, _pi, C.POINTER(C.c_byte), _pd],
@@ -244,6 +245,10 @@ SIGNATURES = {
     "phb_mc_delta_e": [_h, C.c_int, _pi, _pi, _pi, _pd],
     "phb_mc_recompute": [_h, _pd],
     "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
+    "phb_mc_set_rng_state": [_h, C.POINTER(C.c_uint)],
+    "phb_rng_seed_state": [C.c_uint, C.POINTER(C.c_uint)],
+    "phb_rng_draws": [C.POINTER(C.c_uint), C.c_longlong, _pi, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint),
+                      C.POINTER(C.c_longlong)],
     "phb_mc_set_profiling": [_h, C.c_int],
     "phb_surrogate_create": [C.c_longlong, C.c_uint, C.POINTER(_h)],
     "phb_surrogate_destroy": [_h],

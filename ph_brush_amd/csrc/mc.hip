@@ -723,6 +723,20 @@ extern "C" int phb_mc_set_profiling(phb_mc M, int on) {
     });
 }
 
+extern "C" int phb_mc_set_rng_state(phb_mc M, const unsigned* state) {
+    return guard([&] {
+        require(M && state, "phb_mc_set_rng_state: null");
+        require(state[624] <= 624u, "phb_mc_set_rng_state: engine index beyond 624");
+        sync();
+        McCtl c = read_ctl(M);
+        for (int k = 0; k < 624; ++k) c.mt[k] = state[k];
+        c.mti = (int)state[624];
+        M->ctl.upload(&c, 1);
+        sync();
+        return PHB_OK;
+    });
+}
+
 extern "C" int phb_mc_rng_state(phb_mc M, unsigned* state) {
     return guard([&] {
         require(M && state, "phb_mc_rng_state: null");

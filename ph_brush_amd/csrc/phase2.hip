@@ -474,6 +474,30 @@ void fill_p2(phb_mc_s* M, const P2State* S, phb_phase2_stats* st) {
 
 }  // namespace
 
+extern "C" int phb_phase2_set_pools(phb_mc M, const int sizes[4], const int* lists) {
+    return guard([&] {
+        require(M && sizes, "phb_phase2_set_pools: null argument");
+        long long total = 0;
+        for (int k = 0; k < 4; ++k) {
+            require(sizes[k] >= 0, "phb_phase2_set_pools: negative pool size");
+            total += sizes[k];
+        }
+        require(total == 0 || lists != nullptr, "phb_phase2_set_pools: null lists");
+        const int N = M->atoms->n;
+        for (long long k = 0; k < total; ++k)
+            require(lists[k] >= 0 && lists[k] < N, "phb_phase2_set_pools: atom index out of range");
+        require(sizes[0] + sizes[1] <= M->nsites && sizes[2] + sizes[3] <= M->nions,
+                "phb_phase2_set_pools: pools larger than the chain's site and ion pools", PHB_ERR_STATE);
+        long long o = 0;
+        for (int k = 0; k < 4; ++k) {
+            M->handover[k].assign(lists + o, lists + o + sizes[k]);
+            o += sizes[k];
+        }
+        M->has_handover = true;
+        return PHB_OK;
+    });
+}
+
 extern "C" int phb_phase2_begin(phb_mc M, const phb_phase2_desc* d, phb_phase2_stats* out) {
     return guard([&] {
         require(M && d, "phb_phase2_begin: null argument");

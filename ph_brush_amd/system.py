@@ -444,6 +444,13 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_rng_state", self.h, st)
         return np.array(st[:], dtype=np.uint32)
 
+    def set_rng_state(self, state) -> None:
+        """Continue from an engine state (625 words, std::mt19937's stream form)."""
+        st = np.ascontiguousarray(state, np.uint32)
+        if st.shape != (625,):
+            raise ValueError("engine state must hold 625 words")
+        L.call("phb_mc_set_rng_state", self.h, st.ctypes.data_as(C.POINTER(C.c_uint)))
+
     # ---- phase 1 as main() runs it (MC/src/main.cu:180-197)
     def phase1_run(self, target_pct: float, direct_pct_threshold: int = 90, gc_block: int = 2000,
                    epoch_attempt_max: int = 100000, nacc_frac: float = 0.10, mu_eta: float = 0.3,
@@ -644,3 +651,24 @@ class Replica:
 
     def total_energy(self):
         return total_energy(self.atoms, self.params, self.neigh, self.pme, self.box)
+
+
+def rng_seed_state(seed: int) -> np.ndarray:
+    """std::mt19937(seed) in its stream form (624 words, then the index)."""
+    st = np.zeros(625, np.uint32)
+    L.call("phb_rng_seed_state", int(seed) & 0xFFFFFFFF, st.ctypes.data_as(C.POINTER(C.c_uint)))
+    return st
+
+
+def rng_draws(state, his):
+    """The chains' device engine from `state`: his[k] >= 0 draws
+    uniform_int_distribution<int>(0, his[k]), his[k] < 0 a uniform double (its
+    bits).  Returns (draws as uint64, engine state after, serially redrawn rounds)."""
+    st = np.ascontiguousarray(state, np.uint32)
+    his = np.ascontiguousarray(his, np.int32)
+    out = np.zeros(len(his), np.uint64)
+    after = np.zeros(625, np.uint32)
+    nser = C.c_longlong()
+    L.call("phb_rng_draws", st.ctypes.data_as(C.POINTER(C.c_uint)), len(his), his.ctypes.data_as(C.POINTER(C.c_int)),
+           out.ctypes.data_as(C.POINTER(C.c_ulonglong)), after.ctypes.data_as(C.POINTER(C.c_uint)), C.byref(nser))
+    return out, after, nser.value

@@ -95,3 +95,73 @@ def test_cpp_api_phase1_then_phase2_matches_python_path(gpu, tmp_path):
     s2 = rep.mc.phase2_run(400)
     assert kv["p2_attempted"][0] == s2["attempted"] and kv["p2_accepted"][0] == s2["accepted"]
     assert abs(kv["p2_best"][0] - s2["best_E"]) <= 1e-9 * max(1.0, abs(s2["best_E"]))
+
+
+def _build_dropin(tmp_path):
+    exe = tmp_path / "dropin_demo"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}",
+           str(ROOT / "tests" / "cpp" / "dropin_demo.cpp"), f"-L{ROOT / 'ph_brush_amd'}", "-lphb",
+           f"-Wl,-rpath,{ROOT / 'ph_brush_amd'}", "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def test_dropin_caller_compiles_against_reference_signatures(tmp_path):
+    """A caller typed with the reference's move signatures (MC/include/mc.hpp:50-81,
+    std::mt19937& and Params& included) compiles and, without a GPU, fails the way
+    main does."""
+    exe = _build_dropin(tmp_path)
+    if L.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    s, _, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
+    _write_system(s, tmp_path / "sys.txt")
+    r = subprocess.run([str(exe), str(tmp_path / "sys.txt"), "75", "90"], capture_output=True, text=True)
+    assert r.returncode == 2 and "Error:" in r.stderr and "no CPU fallback" in r.stderr
+
+
+def _kv(text):
+    kv = {}
+    for line in text.splitlines():
+        k, *v = line.split()
+        kv[k] = [float(x) for x in v]
+    return kv
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("target,threshold", [(75.0, 90), (95.0, 90)])
+def test_dropin_caller_matches_oracle(gpu, tmp_path, target, threshold):
+    """main()'s sequence through the reference signatures against the oracle's
+    phase 1 and phase 2 (MC/src/main.cu:180-206): counts, epochs, bit-identical
+    mu written back into Params, identities written back into System, the
+    caller's std::mt19937 after each phase equal to the oracle's engine, and
+    phase 2 from the Phase1Result lists."""
+    from oracle import pyoracle as O
+    exe = _build_dropin(tmp_path)
+    s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2, n_salt=10)
+    _write_system(s, tmp_path / "sys.txt")
+    out = subprocess.run([str(exe), str(tmp_path / "sys.txt"), str(target), str(threshold)], capture_output=True,
+                         text=True, check=True)
+    kv = _kv(out.stdout)
+    mc = S.McParams(beta=1.0, mu_eff=0.0)
+    o = O.Oracle(s, p, S.bond_adjacency(s.natoms, s.bonds))
+    o.nlist()
+    ro = o.phase1_run(mc, 5, target, direct_pct_threshold=threshold, gc_block=200, epoch_attempt_max=1000)
+    assert ro["rc"] == 0 and kv["use3dc"] == [1.0]
+    assert kv["direct"][0] == ro["direct"]
+    assert (kv["p1_attempted"][0], kv["p1_accepted"][0]) == (ro["attempted"], ro["accepted"])
+    assert abs(kv["p1_E"][0] - ro["E"]) <= 1e-9 * max(1.0, abs(ro["E"]))
+    if not ro["direct"]:
+        assert kv["blocks"][0] == ro["blocks_done"] and kv["reached"][0] == ro["reached_target"] == 1
+        assert (kv["fwd"][0], kv["bwd"][0]) == (ro["accepted_forward"], ro["accepted_backward"])
+        assert kv["mu_eff"][0] == ro["mu_eff"]                # Params& p.mu_eff, bit for bit
+    assert kv["handover"] == [float(x) for x in ro["handover_sizes"]]
+    assert np.array_equal(np.array(kv["types1"], np.int64), ro["type0"] + 1)
+    assert np.array_equal(np.array(kv["engine1"], np.uint64), ro["rng_state"].astype(np.uint64))
+    s1 = S.System(s.x, s.y, s.z, ro["q"], ro["type0"], s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data)
+    o2 = O.Oracle(s1, p, S.bond_adjacency(s.natoms, s.bonds))
+    o2.nlist()
+    r2 = o2.phase2_run(mc, ro["rng_state"], 600, 100000, pools=ro["handover"], pool_sizes=ro["handover_sizes"])
+    assert (kv["p2_attempted"][0], kv["p2_accepted"][0]) == (r2["n"], r2["accepted"]) and r2["accepted"] > 0
+    assert abs(kv["p2_best"][0] - r2["best_E"]) <= 1e-9 * max(1.0, abs(r2["best_E"]))
+    assert np.array_equal(np.array(kv["types2"], np.int64), r2["type0"] + 1)
+    assert np.array_equal(np.array(kv["engine2"], np.uint64), r2["rng_state"].astype(np.uint64))
