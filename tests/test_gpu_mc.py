@@ -130,6 +130,31 @@ def test_chain_matches_reference_sequence(gpu, case):
     assert abs(e_tot - e_re) <= 1e-10 * max(1.0, abs(e_re))
 
 
+def test_chain_matches_reference_sequence_44k(gpu):
+    """The chain the benchmark times, at its size: the 44,000-atom diblock brush
+    (SPME 64x64x256, order 5, pH 7), pending budget 8 and window 8 as in the
+    benchmark, 400 trials against the oracle's full-recompute chain (the same
+    bar as above; several refreshes of the potential grid happen in between)."""
+    s, p, mc = S.config_30k_fp64()
+    n, seed = 400, 1
+    r = S.Replica(s, p, mc, seed=seed, pending_max=8, spec_window=8, recompute_interval=0, record_cap=n)
+    st = r.mc.run(n)
+    assert st["attempted"] == n and st["status"] == 0 and st["accepted"] >= 4
+    rg = r.mc.record(n)
+    o = O.Oracle(s, p, r.bonds_host)
+    o.nlist()
+    ro = o.gc_run(mc, n, seed)
+    k = _compare_chains(rg, ro, n)
+    for t in range(k):
+        assert abs(rg["dE"][t] - ro["dE"][t]) <= _tol(ro["dE"][t], ro["E"]), t
+    if k == n:
+        q, ty = r.atoms.identity()
+        assert np.array_equal(q, ro["q"]) and np.array_equal(ty, ro["type0"])
+        assert abs(st["E"] - ro["E"]) <= 1e-9 * max(1.0, abs(ro["E"]))
+    e_re = r.mc.recompute()
+    assert abs(st["E"] - e_re) <= 1e-9 * max(1.0, abs(e_re))
+
+
 def test_rng_state_matches_std_mt19937(gpu):
     """The device engine starts from std::mt19937(seed): the C++ standard's seeding
     x[0] = seed, x[i] = 1812433253 * (x[i-1] ^ (x[i-1] >> 30)) + i, index 624."""
