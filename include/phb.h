@@ -148,6 +148,17 @@ int phb_pme_energy(phb_pme pme, phb_atoms a, phb_csr neigh, const phb_box *box,
 int phb_total_energy(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box *box,
                      double *comps, double *E);
 
+/* No counterpart in the reference: the same computation as phb_total_energy for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Its partial sums and
+ * results live in a fixed device array of the library (8,208 doubles, 64 KiB,
+ * one copy per GPU, held until the process ends) instead of per-call buffers;
+ * the PME plan's grids are the plan's own.  Limit: plans whose z pass needs more
+ * than 8,192 partial sums are refused (PHB_ERR_ARG).  Calls on one GPU share that
+ * array: calls that overlap on one GPU (several host threads) race. */
+int phb_total_energy_noalloc(phb_atoms a, phb_params p, phb_csr neigh, phb_pme pme, const phb_box *box,
+                             double *comps, double *E);
+
 /* Stage-level access for parity tests and tooling (no reference counterpart;
  * the reference keeps these buffers inside PMEPlan). Host buffers in/out. */
 int phb_pme_spread(phb_pme pme, phb_atoms a, double *grid);                 /* nx*ny*nz */

@@ -233,6 +233,7 @@ SIGNATURES = {
     "phb_pme_info": [_h, _pi, _pi, _pi, _pi, _pd],
     "phb_pme_energy": [_h, _h, _h, C.POINTER(Box), C.POINTER(PmeComponents), _pd],
     "phb_total_energy": [_h, _h, _h, _h, C.POINTER(Box), _pd, _pd],
+    "phb_total_energy_noalloc": [_h, _h, _h, _h, C.POINTER(Box), _pd, _pd],
     "phb_pme_spread": [_h, _h, _pd],
     "phb_pme_fft_r2c": [_h, _pd, _pd],
     "phb_pme_influence": [_h, _pd],

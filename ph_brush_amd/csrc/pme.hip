@@ -15,9 +15,12 @@
 // partials and then summed in a fixed order (deterministic).
 #include <algorithm>
 #include <memory>
+#include <mutex>
+#include <vector>
 
 #include "device.hpp"
 #include "handles.hpp"
+
 #include "pme.hpp"
 
 namespace phb {
@@ -364,16 +367,17 @@ static void fft_inverse_impl(phb_pme_s* P, const cplx<T>* kg, T* out, const cplx
 }
 
 // grid -> kgrid (-> eta, energy partials) [-> phi]
-void PmeKernels::launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate) {
-    if (P->precision == PHB_FP64) launch_fft_buf(P, P->grid64.p, P->phi64.p, zmode, inverse_to_phi, accumulate);
-    else launch_fft_buf(P, P->grid32.p, P->phi32.p, zmode, inverse_to_phi, accumulate);
+void PmeKernels::launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate, double* partials) {
+    if (P->precision == PHB_FP64)
+        launch_fft_buf(P, P->grid64.p, P->phi64.p, zmode, inverse_to_phi, accumulate, nullptr, partials);
+    else launch_fft_buf(P, P->grid32.p, P->phi32.p, zmode, inverse_to_phi, accumulate, nullptr, partials);
 }
 
 void PmeKernels::launch_fft_buf(phb_pme_s* P, const double* grid, double* phi, int zmode, bool inverse_to_phi,
-                                bool accumulate, const int* gate) {
+                                bool accumulate, const int* gate, double* partials) {
     require(P->precision == PHB_FP64, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
     fft_forward_impl<double>(P, grid, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
-                             P->partials.p, gate);
+                             partials ? partials : P->partials.p, gate);
     if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate);
 }
 
@@ -386,10 +390,10 @@ void PmeKernels::launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, Refre
 }
 
 void PmeKernels::launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int zmode, bool inverse_to_phi,
-                                bool accumulate, const int* gate) {
+                                bool accumulate, const int* gate, double* partials) {
     require(P->precision == PHB_FP32, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
     fft_forward_impl<float>(P, grid, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
-                            P->partials.p, gate);
+                            partials ? partials : P->partials.p, gate);
     if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate);
 }
 
@@ -577,13 +581,13 @@ extern "C" int phb_pme_info(phb_pme P, int* nx, int* ny, int* nz, int* order, do
 namespace {
 
 // E_real (slot 0 of the returned pair) and the reciprocal + correction terms.
+// part: max(4 * pair_blocks, z_blocks) doubles, res: 8 doubles (the plan's own
+// buffers, or the scratch-free entry point's).
 void pme_energy_core(phb_pme_s* P, const phb_atoms_s* a, const phb_csr_s* nl, const phb_params_s* par,
-                     const phb_box* box, int pair_mode, double out[8]) {
+                     const phb_box* box, int pair_mode, double out[8], double* part, double* res) {
     // out: [pair_lj, pair_coul, recip, Q, Q2, Mz, Qz2, -]
     hipStream_t st = stream();
     const int nbp = pair_blocks(a->n);
-    double* part = P->partials.p;
-    double* res = P->result.p;
     if (pair_mode) {
         PmeKernels::launch_pair(a, par, nl, box, pair_mode, P->alpha, P->elF, P->precision, part, nbp);
         reduce_partials(part, nbp, 2, 2, res + 0);
@@ -592,7 +596,7 @@ void pme_energy_core(phb_pme_s* P, const phb_atoms_s* a, const phb_csr_s* nl, co
     }
     PmeKernels::launch_spread(P, a);
     // pair partials are reduced already (stream order); reuse the buffer
-    PmeKernels::launch_fft(P, 1, false, false);
+    PmeKernels::launch_fft(P, 1, false, false, part);
     reduce_partials(part, P->z_blocks, 1, 1, res + 2);
     PmeKernels::launch_moments(a, part, nbp);
     reduce_partials(part, nbp, 4, 4, res + 3);
@@ -650,7 +654,7 @@ extern "C" int phb_pme_energy(phb_pme P, phb_atoms a, phb_csr nl, const phb_box*
         phb_params_s dummy;
         dummy.d.ntypes = 1; dummy.d.lj_shift = 0;
         double o[8];
-        pme_energy_core(P, a, nl, &dummy, box, PAIR_EWALD, o);
+        pme_energy_core(P, a, nl, &dummy, box, PAIR_EWALD, o, P->partials.p, P->result.p);
         double es, eq, esl;
         corrections(P, box, o + 3, &es, &eq, &esl);
         if (comps) *comps = phb_pme_components{o[1], o[2], es, eq + esl, esl};
@@ -669,7 +673,8 @@ extern "C" int phb_total_energy(phb_atoms a, phb_params p, phb_csr nl, phb_pme P
             require(P != nullptr, "total_energy: PME plan is null");  // MC/src/mc.cu:54
             if (a->n > 0) {
                 double o[8];
-                pme_energy_core(P, a, nl, p, box, nl->nnz > 0 ? (PAIR_LJ | PAIR_EWALD) : 0, o);
+                pme_energy_core(P, a, nl, p, box, nl->nnz > 0 ? (PAIR_LJ | PAIR_EWALD) : 0, o, P->partials.p,
+                                P->result.p);
                 double es, eq, esl;
                 corrections(P, box, o + 3, &es, &eq, &esl);
                 c[0] = o[0]; c[1] = o[1]; c[2] = o[2]; c[3] = es; c[4] = eq; c[5] = esl;
@@ -685,6 +690,74 @@ extern "C" int phb_total_energy(phb_atoms a, phb_params p, phb_csr nl, phb_pme P
                 sync();
                 c[0] = h[0]; c[1] = h[1];
             }
+        }
+        if (comps) for (int k = 0; k < 6; ++k) comps[k] = c[k];
+        *E = c[0] + (c[1] + c[2] + c[3] + c[4] + c[5]);
+        return PHB_OK;
+    });
+}
+
+// ------------------------------------------------------------ scratch-free total energy
+// The partial sums and results of phb_total_energy_noalloc live in this array
+// (one copy per GPU, held until the process ends) instead of a per-call or
+// per-plan allocation.  Host code never passes its name to a kernel: it takes
+// the device address from hipGetSymbolAddress, once per device.
+namespace {
+constexpr size_t kNoallocPartials = 4 * 2048;   // >= 4 * pair_blocks(N) for every N, and the z-pass blocks
+constexpr size_t kNoallocDoubles = kNoallocPartials + 16;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;   // per device: the address of that device's copy of `scratch`
+
+double* scratch_of_current_device() {
+    int dev = 0;
+    PHB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        PHB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "phb_total_energy_noalloc: no device address for the scratch array", PHB_ERR_HIP);
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int phb_total_energy_noalloc(phb_atoms a, phb_params p, phb_csr nl, phb_pme P, const phb_box* box,
+                                        double* comps, double* E) {
+    return guard([&] {
+        require(a && p && nl && box && E, "total_energy: null argument");
+        require(nl->rows == a->n, "total_energy: neighbour list rows != natoms");
+        require_device();
+        double* part = scratch_of_current_device();
+        double* res = part + kNoallocPartials;
+        double c[6] = {0, 0, 0, 0, 0, 0};
+        if (p->d.coulombtype == PHB_COUL_PME) {
+            require(P != nullptr, "total_energy: PME plan is null");  // MC/src/mc.cu:54
+            require((size_t)P->z_blocks <= kNoallocPartials,
+                    "phb_total_energy_noalloc: the plan's grid needs more z-pass partials than the scratch array "
+                    "holds (8192); use phb_total_energy");
+            if (a->n > 0) {
+                double o[8];
+                pme_energy_core(P, a, nl, p, box, nl->nnz > 0 ? (PAIR_LJ | PAIR_EWALD) : 0, o, part, res);
+                double es, eq, esl;
+                corrections(P, box, o + 3, &es, &eq, &esl);
+                c[0] = o[0]; c[1] = o[1]; c[2] = o[2]; c[3] = es; c[4] = eq; c[5] = esl;
+            }
+        } else if (a->n > 0 && nl->nnz > 0) {
+            const int nb = pair_blocks(a->n);
+            PmeKernels::launch_pair(a, p, nl, box, PAIR_LJ | PAIR_COUL_CUT, 0.0, kKe / p->d.epsilon_r, PHB_FP64, part,
+                                    nb);
+            reduce_partials(part, nb, 2, 2, res);
+            double h[2];
+            PHB_HIP(hipMemcpyAsync(h, res, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+            PHB_HIP(hipStreamSynchronize(stream()));
+            c[0] = h[0]; c[1] = h[1];
         }
         if (comps) for (int k = 0; k < 6; ++k) comps[k] = c[k];
         *E = c[0] + (c[1] + c[2] + c[3] + c[4] + c[5]);

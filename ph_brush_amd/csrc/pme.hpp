@@ -172,12 +172,13 @@ struct PmeKernels {
     static void launch_spread(phb_pme_s* P, const phb_atoms_s* a);
     // forward r2c + z pass (zmode bits: 1 energy, 2 convolve + inverse z),
     // optionally the inverse plane pass into phi (store or accumulate)
-    static void launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate);
+    // partials: where the z pass writes its energy partials (z_blocks doubles; null: the plan's own)
+    static void launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool accumulate, double* partials = nullptr);
     // explicit buffers; a non-null gate makes every FFT workgroup return at once while *gate == 0
     static void launch_fft_buf(phb_pme_s* P, const double* grid, double* phi, int zmode, bool inverse_to_phi,
-                               bool accumulate, const int* gate = nullptr);
+                               bool accumulate, const int* gate = nullptr, double* partials = nullptr);
     static void launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int zmode, bool inverse_to_phi,
-                               bool accumulate, const int* gate = nullptr);
+                               bool accumulate, const int* gate = nullptr, double* partials = nullptr);
     static void launch_c2r_buf(phb_pme_s* P, double* phi, bool accumulate, RefreshGate gate = {});
     static void launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, RefreshGate gate = {});
     static void launch_pair(const phb_atoms_s* a, const phb_params_s* p, const phb_csr_s* nl, const phb_box* box,
