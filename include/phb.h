@@ -140,6 +140,13 @@ int phb_pme_create(phb_params p, const phb_box *box, double rc, int precision, p
 int phb_pme_destroy(phb_pme pme);
 /* The PMEPlan fields main reads (MC/src/main.cu:134-137). */
 int phb_pme_info(phb_pme pme, int *nx, int *ny, int *nz, int *order, double *alpha);
+/* The xy-plane transform: one workgroup per plane where the plane fits in LDS
+ * (default), else row and column passes through global memory (planes of the
+ * reference's pme_spacing on boxes wider than ~11 nm).  mode 1 selects the passes
+ * for any plan (same results up to rounding), 0 the one-workgroup form where it
+ * fits, -1 only reports; *split (may be NULL) gets the form in use.  Set it
+ * before MC objects are created on the plan (their refresh gates are sized by it). */
+int phb_pme_plane_mode(phb_pme pme, int mode, int *split);
 /* Replaces pmeEnergy (MC/include/pme.hpp:45-49). */
 int phb_pme_energy(phb_pme pme, phb_atoms a, phb_csr neigh, const phb_box *box,
                    phb_pme_components *comps, double *E);

@@ -230,6 +230,7 @@ SIGNATURES = {
     "phb_energy_coul_cut": [_h, _h, _h, C.POINTER(Box), _pd],
     "phb_pme_create": [_h, C.POINTER(Box), C.c_double, C.c_int, C.POINTER(_h)],
     "phb_pme_destroy": [_h],
+    "phb_pme_plane_mode": [_h, C.c_int, _pi],
     "phb_pme_info": [_h, _pi, _pi, _pi, _pi, _pd],
     "phb_pme_energy": [_h, _h, _h, C.POINTER(Box), C.POINTER(PmeComponents), _pd],
     "phb_total_energy": [_h, _h, _h, _h, C.POINTER(Box), _pd, _pd],

@@ -328,6 +328,8 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
                     PHB_ERR_UNSUPPORTED);
             for (int q = 0; q < r; ++q) require(B->mcs[q] != M, "phb_batch_create: a replica appears twice");
             require(Pm->z_tx <= 16, "phb_batch_create: z tile wider than 16", PHB_ERR_INTERNAL);
+            require(!Pm->split, "phb_batch_create: every replica's xy plane must fit one workgroup (the batched "
+                                "refresh transforms whole planes)", PHB_ERR_UNSUPPORTED);
             lds = lds && M->lds_pools;
             B->pool_lds = std::max(B->pool_lds, M->pool_lds_bytes);
             B->klds = std::max(B->klds, Pm->z_lds(sizeof(cplx<double>)) + (size_t)REFRESH_BATCH * Pm->nz * sizeof(cplx<double>));
@@ -344,7 +346,7 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
             phb_pme_s* Pm = M->pme;
             RefreshView& w = rv[r];
             w.D = B->views.p + r;
-            w.kg = Pm->kg64.p; w.twx = Pm->twx64.p; w.twy = Pm->twy64.p; w.twz = Pm->twz64.p; w.eta = Pm->eta64.p;
+            w.kg = Pm->kg64.p; w.twx = Pm->twx64.p; w.twy = Pm->twy64.p; w.twz = Pm->twz64.p; w.eta = Pm->conv_eta64();
             w.px = Pm->px; w.py = Pm->py; w.pz = Pm->pz;
             w.tx = Pm->z_tx; w.nx = Pm->nx; w.ny = Pm->ny;
             w.phi = M->phi.p;

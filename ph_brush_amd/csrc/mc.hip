@@ -334,8 +334,10 @@ void refresh_p(phb_mc_s* M) {
     require(Pm->z_tx <= 16, "refresh: z tile wider than 16", PHB_ERR_INTERNAL);
     const size_t lds = Pm->z_lds(sizeof(cplx<double>)) + (size_t)REFRESH_BATCH * Pm->nz * sizeof(cplx<double>);
     require(lds <= 160 * 1024, "refresh: z pencil and structure factors exceed LDS", PHB_ERR_UNSUPPORTED);
+    require(M->applied.n >= (size_t)Pm->plane_gate_blocks(),
+            "refresh: the plan's plane transform changed after the MC object was created", PHB_ERR_STATE);
     hipLaunchKernelGGL(k_refresh_kspace<P>, dim3(Pm->z_blocks), dim3(Pm->z_threads), lds, stream(), M->dev, Pm->kg64.p,
-                       Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->eta64.p, Pm->pz, Pm->z_tx);
+                       Pm->twx64.p, Pm->twy64.p, Pm->twz64.p, Pm->conv_eta64(), Pm->pz, Pm->z_tx);
     RefreshGate g;
     g.pending_n = &M->ctl.p->pending_n;
     g.seq = &M->ctl.p->pend_seq;
@@ -446,7 +448,6 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         require(pme->precision == PHB_FP64, "phb_mc_create: the PME plan must be FP64 (it keeps the FP64 potential "
                                             "from which the FP32 form's copy is made)", PHB_ERR_UNSUPPORTED);
         require(neigh->rows == a->n, "phb_mc_create: neighbour list rows != natoms");
-        require(pme->nx % 2 == 0, "phb_mc_create: nx must be even (C2R potential grid)", PHB_ERR_UNSUPPORTED);
         require(a->max_type < p->d.ntypes, "phb_mc_create: atom type id outside the LJ table");
         const int T = p->d.ntypes;
         for (int t : {d->nh_n_type, d->nh_p_type, d->w_type, d->cl_type})
@@ -565,8 +566,8 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         c.nprot_target = -1;
         M->ctl.alloc(1);
         M->ctl.upload(&c, 1);
-        {   // one entry per workgroup of either refresh (direct tiles, or k-space planes)
-            const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz), pme->nz);
+        {   // one entry per workgroup of either refresh (direct tiles, or the inverse plane pass)
+            const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz), pme->plane_gate_blocks());
             M->applied.alloc(na);
             PHB_HIP(hipMemsetAsync(M->applied.p, 0, na * sizeof(unsigned), stream()));
         }

@@ -195,8 +195,8 @@ __global__ __launch_bounds__(512) void k_fft_xy_r2c(const T* __restrict__ grid, 
 // mode bit 4: store the forward transform (no eta) back.
 template <typename T>
 __global__ __launch_bounds__(512) void k_fft_z(cplx<T>* __restrict__ kg, int nx, int ny, int nz, FftPlan pz,
-                        const cplx<T>* __restrict__ twz, const T* __restrict__ eta, int tx, int mode,
-                        double* __restrict__ partials, const int* __restrict__ gate) {
+                        const cplx<T>* __restrict__ twz, const T* __restrict__ eta, const T* __restrict__ etac,
+                        int tx, int mode, double* __restrict__ partials, const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double sred[16];
@@ -218,12 +218,16 @@ __global__ __launch_bounds__(512) void k_fft_z(cplx<T>* __restrict__ kg, int nx,
     for (int idx = tid; idx < cnt * nz; idx += bd) {
         const int c = idx / nz, kz = idx - c * nz, kx = kx0 + c;
         const cplx<T> v = C[c * hs + kz];
-        const T et = eta[((size_t)kz * ny + ky) * nkx + kx];
+        const size_t o = ((size_t)kz * ny + ky) * nkx + kx;
         if (mode & 1) {
+            const T et = eta[o];
             const double corner = (kx == 0 || kx == nx / 2) ? 0.5 : 1.0;  // MC/src/pme.cu:234
             e += corner * (double)et * ((double)v.x * v.x + (double)v.y * v.y);
         }
-        if (mode & 2) C[c * hs + kz] = {v.x * et, v.y * et};
+        if (mode & 2) {
+            const T ec = etac[o];
+            C[c * hs + kz] = {v.x * ec, v.y * ec};
+        }
     }
     if (mode & 1) {
         e = block_sum(e, sred);
@@ -247,6 +251,122 @@ __global__ __launch_bounds__(512) void k_fft_xy_c2r(const cplx<T>* __restrict__ 
                              FftPlan py, const cplx<T>* __restrict__ twx, const cplx<T>* __restrict__ twy,
                              int accumulate, RefreshGate gate) {
     fft_xy_c2r_plane<T>(kg, out, nx, ny, px, py, twx, twy, accumulate, gate, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------------ split plane passes
+// For xy planes larger than one workgroup's LDS (the reference's pme_spacing on
+// boxes wider than ~11 nm gives planes of 100^2 to 180^2 and more), the plane
+// transform runs as two passes through global memory: x rows, then y columns.
+// Rows are taken in pairs (y0, y0 + 1) of one z plane, packed as one complex row.
+
+// Forward x pass: rpb row pairs per workgroup -> half-spectrum rows kg[z][y][kx].
+template <typename T>
+__global__ __launch_bounds__(512) void k_fft_x_r2c(const T* __restrict__ grid, cplx<T>* __restrict__ kg, int nx, int ny,
+                                                   int nz, FftPlan px, const cplx<T>* __restrict__ twx, int rpb,
+                                                   const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* A = stwx + nx;
+    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, npairs = nz * nyp;
+    const int p0 = blockIdx.x * rpb, cnt = min(rpb, npairs - p0);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
+    for (int idx = tid; idx < cnt * nx; idx += bd) {
+        const int r = idx / nx, xx = idx - r * nx, pr = p0 + r, z = pr / nyp, y0 = 2 * (pr - z * nyp);
+        const T* row = grid + ((size_t)z * ny + y0) * nx;
+        A[idx] = {row[xx], y0 + 1 < ny ? row[nx + xx] : T(0)};
+    }
+    __syncthreads();
+    lds_fft(A, cnt, nx, px, stwx, -1);
+    for (int idx = tid; idx < cnt * nkx; idx += bd) {
+        const int r = idx / nkx, k = idx - r * nkx, pr = p0 + r, z = pr / nyp, y0 = 2 * (pr - z * nyp);
+        const cplx<T> Zk = A[r * nx + k], Zn = A[r * nx + (k == 0 ? 0 : nx - k)];
+        cplx<T>* out = kg + ((size_t)z * ny + y0) * nkx;
+        out[k] = {(Zk.x + Zn.x) * T(0.5), (Zk.y - Zn.y) * T(0.5)};
+        if (y0 + 1 < ny) out[nkx + k] = {(Zk.y + Zn.y) * T(0.5), -(Zk.x - Zn.x) * T(0.5)};
+    }
+}
+
+// y pass over kg[z][y][kx] in place: workgroup = (z, tile of tx kx columns);
+// sign -1 forward, +1 inverse.  With a gate (the potential refresh's inverse
+// pass) it runs only while a pending list is not yet folded (applied[0]).
+template <typename T>
+__global__ __launch_bounds__(512) void k_fft_y(cplx<T>* __restrict__ kg, int nx, int ny, FftPlan py,
+                                               const cplx<T>* __restrict__ twy, int tx, int sign, const int* __restrict__ gate,
+                                               RefreshGate rg) {
+    if (gate && *gate == 0) return;
+    if (rg.pending_n && (rg.applied[0] == *rg.seq || *rg.pending_n == 0)) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stw = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* C = stw + ny;
+    const int nkx = nx / 2 + 1, hs = ny + 1, ntile = (nkx + tx - 1) / tx;
+    const int z = blockIdx.x / ntile, kx0 = (blockIdx.x - z * ntile) * tx, cnt = min(tx, nkx - kx0);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int k = tid; k < ny; k += bd) stw[k] = twy[k];
+    cplx<T>* base = kg + (size_t)z * ny * nkx + kx0;
+    for (int idx = tid; idx < ny * cnt; idx += bd) {
+        const int y = idx / cnt, c = idx - y * cnt;
+        C[c * hs + y] = base[(size_t)y * nkx + c];
+    }
+    __syncthreads();
+    lds_fft(C, cnt, hs, py, stw, sign);
+    for (int idx = tid; idx < ny * cnt; idx += bd) {
+        const int y = idx / cnt, c = idx - y * cnt;
+        base[(size_t)y * nkx + c] = C[c * hs + y];
+    }
+}
+
+// Inverse x pass: half-spectrum rows (y0, y0 + 1) -> one complex row -> the two
+// real rows, stored into or accumulated onto out.  With a gate each workgroup
+// applies a pending list once and records it in applied[blockIdx.x].
+template <typename T>
+__global__ __launch_bounds__(512) void k_fft_x_c2r(const cplx<T>* __restrict__ kg, T* __restrict__ out, int nx, int ny,
+                                                   int nz, FftPlan px, const cplx<T>* __restrict__ twx, int rpb,
+                                                   int accumulate, RefreshGate gate) {
+    if (gate.pending_n && (gate.applied[blockIdx.x] == *gate.seq || *gate.pending_n == 0)) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T>* stwx = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* A = stwx + nx;
+    const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1, npairs = nz * nyp;
+    const int p0 = blockIdx.x * rpb, cnt = min(rpb, npairs - p0);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int k = tid; k < nx; k += bd) stwx[k] = twx[k];
+    for (int idx = tid; idx < cnt * nx; idx += bd) {
+        const int r = idx / nx, k = idx - r * nx, pr = p0 + r, z = pr / nyp, y0 = 2 * (pr - z * nyp);
+        const bool hb = y0 + 1 < ny;
+        const cplx<T>* in = kg + ((size_t)z * ny + y0) * nkx;
+        cplx<T> a, b;
+        if (k < nkx) {
+            a = in[k];
+            b = hb ? in[nkx + k] : cplx<T>{T(0), T(0)};
+            if (k == 0 || 2 * k == nx) { a.y = T(0); b.y = T(0); }
+        } else {
+            const int kk = nx - k;
+            a = in[kk]; a.y = -a.y;
+            b = hb ? in[nkx + kk] : cplx<T>{T(0), T(0)};
+            b.y = -b.y;
+        }
+        A[idx] = {a.x - b.y, a.y + b.x};
+    }
+    __syncthreads();
+    lds_fft(A, cnt, nx, px, stwx, +1);
+    for (int idx = tid; idx < cnt * nx; idx += bd) {
+        const int r = idx / nx, xx = idx - r * nx, pr = p0 + r, z = pr / nyp, y0 = 2 * (pr - z * nyp);
+        T* row = out + ((size_t)z * ny + y0) * nx;
+        const cplx<T> v = A[idx];
+        if (accumulate) {
+            row[xx] += v.x;
+            if (y0 + 1 < ny) row[nx + xx] += v.y;
+        } else {
+            row[xx] = v.x;
+            if (y0 + 1 < ny) row[nx + xx] = v.y;
+        }
+    }
+    if (gate.applied) {
+        __syncthreads();
+        if (tid == 0) gate.applied[blockIdx.x] = *gate.seq;
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -276,6 +396,50 @@ int fft_threads(long long elems, int min_threads = 256) {
     while (b < t && b < 512) b *= 2;
     require(t <= 512, "FFT plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
     return b;
+}
+
+size_t plan_csz(const phb_pme_s* P) {
+    return P->precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+}
+
+// One workgroup holds the whole xy plane: at most 7,680 complex elements per LDS
+// transform and 160 KiB of LDS.
+bool plane_fits(const phb_pme_s* P) {
+    const int nx = P->nx, ny = P->ny, nkx = nx / 2 + 1;
+    const long long plane_elems = std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny);
+    return plane_elems <= 7680 && P->plane_lds(plan_csz(P)) <= 160 * 1024;
+}
+
+// Launch shapes of the plane transform, fused (one workgroup per plane) or split
+// (x rows, then y columns, through global memory).
+void set_plane_mode(phb_pme_s* P, bool split) {
+    const int nx = P->nx, ny = P->ny, nz = P->nz, nkx = nx / 2 + 1, nyp = (ny + 1) / 2;
+    const size_t csz = plan_csz(P);
+    if (!split) {
+        require(plane_fits(P), "pmeCreate: xy plane too large for one workgroup", PHB_ERR_UNSUPPORTED);
+        // planes of >= 1024 elements get 8 waves (their LDS stages are latency-bound)
+        const long long plane_elems = std::max<long long>((long long)nyp * nx, (long long)nkx * ny);
+        P->plane_threads = fft_threads(plane_elems, plane_elems >= 1024 ? 512 : 256);
+        P->split = false;
+        return;
+    }
+    const size_t lds_cap = 64 * 1024 / csz;   // complex elements per workgroup (occupancy)
+    require(nx <= 4096 && ny <= 4096 && (size_t)2 * nx <= lds_cap && (size_t)2 * ny + 1 <= lds_cap,
+            "pmeCreate: grid lines longer than the row and column passes take", PHB_ERR_UNSUPPORTED);
+    P->split = true;
+    // x pass: row pairs per workgroup up to 4,096 complex elements
+    int rpb = (int)std::min<size_t>(4096 / nx, (lds_cap - nx) / nx);
+    rpb = std::max(1, std::min(rpb, nz * nyp));
+    P->x_rpb = rpb;
+    P->x_blocks = (nz * nyp + rpb - 1) / rpb;
+    P->x_threads = fft_threads((long long)rpb * nx);
+    // y pass: up to 16 kx columns per workgroup (256-byte row segments in FP64)
+    int ytx = std::min(nkx, 16);
+    ytx = std::min<int>(ytx, (int)std::min<size_t>(4096 / ny, (lds_cap - ny) / (ny + 1)));
+    ytx = std::max(1, ytx);
+    P->y_tx = ytx;
+    P->y_blocks = nz * ((nkx + ytx - 1) / ytx);
+    P->y_threads = fft_threads((long long)ytx * ny);
 }
 
 }  // namespace
@@ -347,22 +511,36 @@ void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
 
 template <typename T>
 static void fft_forward_impl(phb_pme_s* P, const T* grid, cplx<T>* kg, const cplx<T>* twx, const cplx<T>* twy,
-                             const cplx<T>* twz, const T* eta, int zmode, double* partials,
+                             const cplx<T>* twz, const T* eta, const T* etac, int zmode, double* partials,
                              const int* gate = nullptr) {
     hipStream_t st = stream();
     const int nx = P->nx, ny = P->ny, nz = P->nz;
-    hipLaunchKernelGGL(k_fft_xy_r2c<T>, dim3(nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), st, grid, kg,
-                       nx, ny, P->px, P->py, twx, twy, gate);
+    if (P->split) {
+        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(P->x_blocks), dim3(P->x_threads), P->x_lds(sizeof(cplx<T>)), st, grid,
+                           kg, nx, ny, nz, P->px, twx, P->x_rpb, gate);
+        hipLaunchKernelGGL(k_fft_y<T>, dim3(P->y_blocks), dim3(P->y_threads), P->y_lds(sizeof(cplx<T>)), st, kg, nx,
+                           ny, P->py, twy, P->y_tx, -1, gate, RefreshGate{});
+    } else {
+        hipLaunchKernelGGL(k_fft_xy_r2c<T>, dim3(nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), st, grid,
+                           kg, nx, ny, P->px, P->py, twx, twy, gate);
+    }
     hipLaunchKernelGGL(k_fft_z<T>, dim3(P->z_blocks), dim3(P->z_threads), P->z_lds(sizeof(cplx<T>)), st, kg, nx, ny,
-                       nz, P->pz, twz, eta, P->z_tx, zmode, partials, gate);
+                       nz, P->pz, twz, eta, etac, P->z_tx, zmode, partials, gate);
     PHB_HIP(hipGetLastError());
 }
 
 template <typename T>
-static void fft_inverse_impl(phb_pme_s* P, const cplx<T>* kg, T* out, const cplx<T>* twx, const cplx<T>* twy,
+static void fft_inverse_impl(phb_pme_s* P, cplx<T>* kg, T* out, const cplx<T>* twx, const cplx<T>* twy,
                              int accumulate, RefreshGate gate = {}) {
-    hipLaunchKernelGGL(k_fft_xy_c2r<T>, dim3(P->nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)), stream(),
-                       kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate, gate);
+    if (P->split) {   // y columns in place on kg, then x rows into out
+        hipLaunchKernelGGL(k_fft_y<T>, dim3(P->y_blocks), dim3(P->y_threads), P->y_lds(sizeof(cplx<T>)), stream(), kg,
+                           P->nx, P->ny, P->py, twy, P->y_tx, +1, nullptr, gate);
+        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(P->x_blocks), dim3(P->x_threads), P->x_lds(sizeof(cplx<T>)), stream(),
+                           kg, out, P->nx, P->ny, P->nz, P->px, twx, P->x_rpb, accumulate, gate);
+    } else {
+        hipLaunchKernelGGL(k_fft_xy_c2r<T>, dim3(P->nz), dim3(P->plane_threads), P->plane_lds(sizeof(cplx<T>)),
+                           stream(), kg, out, P->nx, P->ny, P->px, P->py, twx, twy, accumulate, gate);
+    }
     PHB_HIP(hipGetLastError());
 }
 
@@ -376,8 +554,8 @@ void PmeKernels::launch_fft(phb_pme_s* P, int zmode, bool inverse_to_phi, bool a
 void PmeKernels::launch_fft_buf(phb_pme_s* P, const double* grid, double* phi, int zmode, bool inverse_to_phi,
                                 bool accumulate, const int* gate, double* partials) {
     require(P->precision == PHB_FP64, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
-    fft_forward_impl<double>(P, grid, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, zmode,
-                             partials ? partials : P->partials.p, gate);
+    fft_forward_impl<double>(P, grid, P->kg64.p, P->twx64.p, P->twy64.p, P->twz64.p, P->eta64.p, P->conv_eta64(),
+                             zmode, partials ? partials : P->partials.p, gate);
     if (inverse_to_phi) fft_inverse_impl<double>(P, P->kg64.p, phi, P->twx64.p, P->twy64.p, accumulate);
 }
 
@@ -392,8 +570,8 @@ void PmeKernels::launch_c2r_buf(phb_pme_s* P, float* phi, bool accumulate, Refre
 void PmeKernels::launch_fft_buf(phb_pme_s* P, const float* grid, float* phi, int zmode, bool inverse_to_phi,
                                 bool accumulate, const int* gate, double* partials) {
     require(P->precision == PHB_FP32, "launch_fft_buf: precision mismatch", PHB_ERR_INTERNAL);
-    fft_forward_impl<float>(P, grid, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, zmode,
-                            partials ? partials : P->partials.p, gate);
+    fft_forward_impl<float>(P, grid, P->kg32.p, P->twx32.p, P->twy32.p, P->twz32.p, P->eta32.p, P->conv_eta32(),
+                            zmode, partials ? partials : P->partials.p, gate);
     if (inverse_to_phi) fft_inverse_impl<float>(P, P->kg32.p, phi, P->twx32.p, P->twy32.p, accumulate);
 }
 
@@ -494,12 +672,9 @@ std::unique_ptr<phb_pme_s> build_pme(const phb_params_desc& d, const phb_box* bo
     P->geom = GridGeom{nx, ny, nz, P->order, box->Lx, box->Ly, box->Lz};
     const int nkx = nx / 2 + 1;
     const size_t G = (size_t)nx * ny * nz, Gk = (size_t)nkx * ny * nz;
-    // launch shapes
-    // planes of >= 1024 elements get 8 waves (their LDS stages are latency-bound)
-    const long long plane_elems = std::max<long long>((long long)((ny + 1) / 2) * nx, (long long)nkx * ny);
-    P->plane_threads = fft_threads(plane_elems, plane_elems >= 1024 ? 512 : 256);
+    // launch shapes: one workgroup per xy plane where the plane fits, else row and column passes
+    set_plane_mode(P.get(), !plane_fits(P.get()));
     const size_t csz = precision == PHB_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
-    require(P->plane_lds(csz) <= 160 * 1024, "pmeCreate: xy plane too large for LDS", PHB_ERR_UNSUPPORTED);
     const int ntile0 = (nkx + 7) / 8;
     P->z_tx = (nkx + ntile0 - 1) / ntile0;
     P->z_blocks = ny * ((nkx + P->z_tx - 1) / P->z_tx);
@@ -523,10 +698,17 @@ std::unique_ptr<phb_pme_s> build_pme(const phb_params_desc& d, const phb_box* bo
                 eta[o] = P->elF * std::exp(-P->ewF * m2k) / denom;
             }
     P->eta_host = eta;
+    // the convolutions' copy: the reference's half weight at kx = nx/2 for odd nx (MC/src/pme.cu:234)
+    std::vector<double> etac;
+    if (nx % 2 == 1) {
+        etac = eta;
+        for (size_t o = (size_t)(nx / 2); o < Gk; o += nkx) etac[o] *= 0.5;
+    }
     auto tx = twiddles(nx), ty = twiddles(ny), tz = twiddles(nz);
     if (precision == PHB_FP64) {
         P->grid64.alloc(G); P->phi64.alloc(G); P->kg64.alloc(Gk); P->eta64.alloc(Gk);
         P->eta64.upload(eta.data(), Gk);
+        if (!etac.empty()) { P->etac64.alloc(Gk); P->etac64.upload(etac.data(), Gk); }
         P->twx64.alloc(nx); P->twy64.alloc(ny); P->twz64.alloc(nz);
         P->twx64.upload(tx.data(), nx); P->twy64.upload(ty.data(), ny); P->twz64.upload(tz.data(), nz);
     } else {
@@ -539,6 +721,11 @@ std::unique_ptr<phb_pme_s> build_pme(const phb_params_desc& d, const phb_box* bo
         auto fx = cv(tx), fy = cv(ty), fz = cv(tz);
         P->grid32.alloc(G); P->phi32.alloc(G); P->kg32.alloc(Gk); P->eta32.alloc(Gk);
         P->eta32.upload(e32.data(), Gk);
+        if (!etac.empty()) {
+            std::vector<float> c32(etac.begin(), etac.end());
+            P->etac32.alloc(Gk);
+            P->etac32.upload(c32.data(), Gk);
+        }
         P->twx32.alloc(nx); P->twy32.alloc(ny); P->twz32.alloc(nz);
         P->twx32.upload(fx.data(), nx); P->twy32.upload(fy.data(), ny); P->twz32.upload(fz.data(), nz);
     }
@@ -564,6 +751,20 @@ extern "C" int phb_pme_create(phb_params p, const phb_box* box, double rc, int p
 
 extern "C" int phb_pme_destroy(phb_pme pme) {
     return guard([&] { delete pme; return PHB_OK; });
+}
+
+extern "C" int phb_pme_plane_mode(phb_pme P, int mode, int* split) {
+    return guard([&] {
+        require(P != nullptr, "phb_pme_plane_mode: null plan");
+        require(mode >= -1 && mode <= 1, "phb_pme_plane_mode: mode must be -1 (query), 0 (one workgroup per plane) "
+                                         "or 1 (row and column passes)");
+        if (mode >= 0) {
+            sync();
+            set_plane_mode(P, mode == 1);
+        }
+        if (split) *split = P->split ? 1 : 0;
+        return PHB_OK;
+    });
 }
 
 extern "C" int phb_pme_info(phb_pme P, int* nx, int* ny, int* nz, int* order, double* alpha) {

@@ -43,17 +43,33 @@ struct phb_pme_s {
     phb::FftPlan px, py, pz;
     phb::GridGeom geom{};
     int plane_threads = 256, z_tx = 8, z_blocks = 0, z_threads = 256;
+    // split plane transform (planes beyond one workgroup's LDS): x pass of x_rpb
+    // row pairs per workgroup, y pass of y_tx kx columns per workgroup
+    bool split = false;
+    int x_rpb = 1, x_blocks = 0, x_threads = 256, y_tx = 1, y_blocks = 0, y_threads = 256;
     std::vector<double> eta_host;
     phb::DevBuf<double> grid64, phi64, eta64;
     phb::DevBuf<phb::cplx<double>> kg64, twx64, twy64, twz64;
     phb::DevBuf<float> grid32, phi32, eta32;
     phb::DevBuf<phb::cplx<float>> kg32, twx32, twy32, twz32;
+    // influence function of the convolutions (potential grids, mesh kernel): eta,
+    // except that for odd nx the kx = nx/2 column carries half weight, as the
+    // reference's energy sum counts it (MC/src/pme.cu:234), so that 0.5 Q.phi
+    // equals that energy; null for even nx (eta itself)
+    phb::DevBuf<double> etac64;
+    phb::DevBuf<float> etac32;
     phb::DevBuf<double> partials, result;
+    const double* conv_eta64() const { return etac64.p ? etac64.p : eta64.p; }
+    const float* conv_eta32() const { return etac32.p ? etac32.p : eta32.p; }
     size_t plane_lds(size_t csz) const {
         const int nyp = (ny + 1) / 2, nkx = nx / 2 + 1;
         return ((size_t)nx + ny + (size_t)nyp * nx + (size_t)nkx * (ny + 1)) * csz;
     }
     size_t z_lds(size_t csz) const { return ((size_t)nz + (size_t)z_tx * (nz + 1)) * csz; }
+    size_t x_lds(size_t csz) const { return ((size_t)nx + (size_t)x_rpb * nx) * csz; }
+    size_t y_lds(size_t csz) const { return ((size_t)ny + (size_t)y_tx * (ny + 1)) * csz; }
+    // entries of the per-workgroup refresh gate an inverse plane pass needs
+    int plane_gate_blocks() const { return split ? x_blocks : nz; }
     ~phb_pme_s();
 };
 

@@ -354,6 +354,13 @@ class PMEPlan(_Handle):
         L.call("phb_pme_energy", self.h, atoms.h, neigh.h, C.byref(box), C.byref(comps), C.byref(e))
         return e.value, comps
 
+    def plane_mode(self, mode: int = -1) -> int:
+        """-1: report; 0: one workgroup per xy plane; 1: row and column passes.
+        Returns 1 when the row and column passes are in use."""
+        s = C.c_int()
+        L.call("phb_pme_plane_mode", self.h, mode, C.byref(s))
+        return s.value
+
     def spread(self, atoms):
         g = np.zeros(self.nx * self.ny * self.nz)
         L.call("phb_pme_spread", self.h, atoms.h, L.dptr(g))
@@ -638,7 +645,8 @@ class Batch(_Handle):
 class Replica:
     """One system set up on the device the way main() does (MC/src/main.cu:106-138)."""
 
-    def __init__(self, s: System, p: Params, mc: McParams | None = None, seed: int = 1, **mc_kw):
+    def __init__(self, s: System, p: Params, mc: McParams | None = None, seed: int = 1, plane_mode: int = -1,
+                 **mc_kw):
         self.s, self.p = s, p
         self.bonds_host = bond_adjacency(s.natoms, s.bonds)
         self.atoms = DeviceAtoms(s)
@@ -647,6 +655,8 @@ class Replica:
         self.neigh = build_half_neighbors_gpu(self.atoms, self.box, self.bonds, p.rc)
         self.params = DeviceParams(p)
         self.pme = PMEPlan(self.params, self.box, p.rc) if p.coulombtype == L.PHB_COUL_PME else None
+        if self.pme is not None and plane_mode >= 0:
+            self.pme.plane_mode(plane_mode)
         self.mc = MonteCarlo(self.atoms, self.params, self.neigh, self.pme, self.box, mc, seed, **mc_kw) if mc else None
 
     def total_energy(self):
