@@ -217,6 +217,70 @@ WORKLOADS = {
 }
 
 
+# ------------------------------------------------------------ presets in the reference's units
+# The reference's inputs (sim_inputs/PDMAEMA_C4.zip, PDMAEMA_N1.zip,
+# PMETAC_b_PDMAEMA_C4.zip, listed in .MISSING_LARGE_BLOBS) are not distributed.
+# These presets stand in for them in the reference's units and conventions: nm,
+# kJ/mol, e; Martini-style type ids NH_N = 4, NH_P = 5, W = 7, Cl = 8 with the
+# reference's default beta = 1/(0.008314462 * 300) (MC/include/types.hpp:51-60);
+# one cutoff for LJ and real-space Ewald (1.2 nm, shifted LJ), epsilon_r = 15,
+# alpha from ewald_rtol, cubic splines, grid from pme_spacing 0.12 nm, slab
+# correction with zfac 3.  Chains are grown by the reduced-unit generator and
+# scaled by the bead diameter SIGMA_NM.  Types: 2 Na+, 3 PMETAC's quaternary
+# ammonium (+1, not titratable), 4/5 DMAEMA's amine neutral/protonated, 7 the
+# neutral water bead (the ion-swap partner), 8 Cl-; 1 and 6 take no atoms.
+SIGMA_NM = 0.47
+PRESET_TYPES = (3, 2, 4, 5, 7, 8)   # perm, cation, nh_n, nh_p, w, cl (1-based)
+
+PRESETS = {
+    # 10 x 10 PDMAEMA chains of 30 beads at 1.41 nm, 14.1 x 14.1 x 19.74 nm box: grid 120 x 120 x 500
+    "pdmaema": dict(chains=(10, 10), spacing_nm=1.41, chain_len=30, n_perm=0, p_prot=0.10, n_salt=200,
+                    box_nm=(14.1, 14.1, 19.74), target_pct=30.0, seed=11),
+    # 11 x 11 PMETAC(15)-b-PDMAEMA(15) chains at 1.4636 nm, 16.1 x 16.1 x 19.74 nm: grid 135 x 135 x 500
+    "pmetac_b_pdmaema": dict(chains=(11, 11), spacing_nm=16.1 / 11, chain_len=30, n_perm=15, p_prot=0.10,
+                             n_salt=200, box_nm=(16.1, 16.1, 19.74), target_pct=30.0, seed=12),
+}
+
+
+def preset_system(name: str) -> S.System:
+    """The preset's brush in nm (data-file box; Lz extended x3 for the slab correction)."""
+    c = PRESETS[name]
+    Lx, Ly, Lz = (v / SIGMA_NM for v in c["box_nm"])
+    s = S.gen_brush(c["chains"][0], c["chains"][1], c["chain_len"], Lx, Ly, Lz, spacing=c["spacing_nm"] / SIGMA_NM,
+                    n_perm=c["n_perm"], p_prot=c["p_prot"], n_salt=c["n_salt"], seed=c["seed"], zfac=3.0,
+                    types=PRESET_TYPES)
+    k = SIGMA_NM
+    return S.System(s.x * k, s.y * k, s.z * k, s.q, s.type0, s.bonds, s.Lx * k, s.Ly * k, s.Lz * k, s.Lz_data * k,
+                    s.mol)
+
+
+def preset_params_text(name: str, rng_seed: int = 2025, **keys) -> str:
+    """params.in of a preset (reference keys only; `keys` adds or overrides entries)."""
+    c = PRESETS[name]
+    entries = {"cutoff_nm": "1.2", "lj_shift": "yes", "coulombtype": "pme", "epsilon_r": "15",
+               "ewald_rtol": "1e-5", "pme_spacing": "0.12", "pme_order": "4", "pme_3dc": "yes 3",
+               "nh_n_type": "4", "nh_p_type": "5", "w_type": "7", "cl_type": "8",
+               "target_pct": repr(float(c["target_pct"])), "rng_seed": str(rng_seed)}
+    entries.update({k: str(v) for k, v in keys.items()})
+    lines = [f"{k} = {v}" for k, v in entries.items()]
+    for a in range(1, 9):
+        for b in range(a, 9):
+            eps = 5.0 if a == b == 7 else 3.5          # Martini-style levels: water-water stronger
+            lines.append(f"lj {a} {b} {eps!r} {SIGMA_NM!r}")
+    return "\n".join(lines) + "\n"
+
+
+def write_preset(name: str, out_dir, rng_seed: int = 2025, **keys):
+    """Writes <out_dir>/in.data and <out_dir>/params.in for a preset; returns their paths."""
+    from pathlib import Path
+    s = preset_system(name)
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    write_lammps_full(out / "in.data", data_file_of(s, 8))
+    (out / "params.in").write_text(preset_params_text(name, rng_seed, **keys))
+    return out / "in.data", out / "params.in"
+
+
 def write_workload(name: str, out_dir, target_pct: float | None = None, rng_seed: int = 1, **keys):
     """Writes <out_dir>/in.data and <out_dir>/params.in for a named workload
     (the generated brushes standing in for the reference's undistributed inputs);
@@ -235,10 +299,15 @@ def write_workload(name: str, out_dir, target_pct: float | None = None, rng_seed
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="write a workload as a LAMMPS data file and params.in")
-    ap.add_argument("workload", choices=sorted(WORKLOADS))
+    ap.add_argument("workload", choices=sorted(WORKLOADS) + sorted(PRESETS))
     ap.add_argument("out_dir")
     ap.add_argument("--target-pct", type=float, default=None)
-    ap.add_argument("--rng-seed", type=int, default=1)
+    ap.add_argument("--rng-seed", type=int, default=None)
     a = ap.parse_args()
-    for path in write_workload(a.workload, a.out_dir, a.target_pct, a.rng_seed):
+    if a.workload in PRESETS:
+        kw = {} if a.target_pct is None else {"target_pct": a.target_pct}
+        paths = write_preset(a.workload, a.out_dir, 2025 if a.rng_seed is None else a.rng_seed, **kw)
+    else:
+        paths = write_workload(a.workload, a.out_dir, a.target_pct, 1 if a.rng_seed is None else a.rng_seed)
+    for path in paths:
         print(path)

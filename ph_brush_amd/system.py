@@ -86,10 +86,12 @@ def wrap_positions(v: np.ndarray, L: float) -> np.ndarray:
 
 
 def gen_brush(chains_x, chains_y, chain_len, Lx, Ly, Lz, *, spacing=2.0, n_perm=0, p_prot=0.5, n_salt=0,
-              bond=0.97, min_dist=0.8, seed=1, graft_z=0.0, zfac=None) -> System:
-    """Synthetic brush from the C generator (phb_gen_brush)."""
+              bond=0.97, min_dist=0.8, seed=1, graft_z=0.0, zfac=None, types=None) -> System:
+    """Synthetic brush from the C generator (phb_gen_brush).  types: 1-based ids
+    (perm, cation, nh_n, nh_p, w, cl); default the synthetic brushes' TYPE_*."""
+    tp = types or (TYPE_PERM, TYPE_CATION, TYPE_NH_N, TYPE_NH_P, TYPE_W, TYPE_CL)
     spec = L.BrushSpec(chains_x, chains_y, spacing, graft_z, chain_len, n_perm, p_prot, n_salt, Lx, Ly, Lz, bond,
-                       min_dist, seed, TYPE_PERM, TYPE_CATION, TYPE_NH_N, TYPE_NH_P, TYPE_W, TYPE_CL)
+                       min_dist, seed, *tp)
     n = C.c_int(0)
     nb = C.c_int(0)
     null_d = C.POINTER(C.c_double)()

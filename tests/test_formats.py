@@ -330,3 +330,30 @@ def test_mc_program_usage(tmp_path):
     assert r.returncode == 1 and r.stderr == "--write-data requires a path\n"
     r = subprocess.run([str(MC), "a", "b", "-x"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and r.stderr == "Unrecognized argument: -x\n"
+
+
+@pytest.mark.parametrize("name,grid", [("pdmaema", (120, 120, 500)), ("pmetac_b_pdmaema", (135, 135, 500))])
+def test_presets_in_reference_units(tmp_path, lib_dump, name, grid):
+    """The presets' files: read by the reference's own io.cpp exactly as here
+    (where oracle/_ref is built), nm / kJ/mol with the reference's defaults (beta,
+    types 4/5/7/8, pme_order 4, spacing 0.12), the grid good_fft_dim gives on the
+    box (planes beyond one workgroup), a neutral system whose W placeholders pair
+    the neutral sites."""
+    from oracle import pyoracle as O
+    data, prm = F.write_preset(name, tmp_path)
+    if REF_DUMP.exists():
+        mine = _run(lib_dump, data, prm, tmp_path / "mine.out")
+        ref = _run(REF_DUMP, data, prm, tmp_path / "ref.out")
+        assert mine == ref and mine[0] == 0
+    d, pf, s, p, mc = F.load_run(data, prm)
+    assert (pf.rc, pf.epsilon_r, pf.pme_order, pf.pme_spacing, pf.coulombtype) == (1.2, 15.0, 4, 0.12, L.PHB_COUL_PME)
+    assert pf.beta == 1.0 / (0.008314462 * 300.0) and p.grid is None and pf.pme_3dc == 1
+    assert (mc.nh_n_type, mc.nh_p_type, mc.w_type, mc.cl_type) == (4, 5, 7, 8)
+    assert tuple(O.good_fft_dim(math.ceil(v / 0.12)) for v in (s.Lx, s.Ly, s.Lz)) == grid
+    t = s.type0 + 1
+    assert abs(float(s.q.sum())) < 1e-12 and np.sum(t == 7) == np.sum(t == 4) > 0 and np.sum(t == 5) > 0
+    assert np.all(s.z >= 0) and s.z.max() < s.Lz_data
+    bonded = np.linalg.norm(np.stack([s.x[s.bonds[:, 0]] - s.x[s.bonds[:, 1]], s.y[s.bonds[:, 0]] - s.y[s.bonds[:, 1]],
+                                      s.z[s.bonds[:, 0]] - s.z[s.bonds[:, 1]]]), axis=0)
+    bonded = bonded[bonded < 5.0]    # bonds across the periodic x/y boundary are wrapped
+    np.testing.assert_allclose(bonded, 0.97 * F.SIGMA_NM, rtol=1e-6)   # 8 written decimals

@@ -171,3 +171,40 @@ def test_mc_program_already_at_target_and_phase2_skipped(tmp_path, gpu):
     assert (tmp_path / "phase1.data").read_bytes() == (tmp_path / "phase2.data").read_bytes()
     m = re.search(r"\[summary\] E0/after phase1/after phase2 = (\S+) / (\S+) / (\S+)", rep)
     assert m and m.group(2) == m.group(3)
+
+
+@pytest.mark.parametrize("name,grid", [("pdmaema", (120, 120, 500)), ("pmetac_b_pdmaema", (135, 135, 500))])
+def test_mc_program_on_reference_unit_presets(tmp_path, gpu, name, grid):
+    """The presets in nm / kJ/mol (Martini-style types 4/5/7/8, default beta,
+    alpha from ewald_rtol, cubic splines, grid from pme_spacing 0.12 on 14-16 nm
+    boxes: planes transformed by row and column passes, odd nx for the second)
+    through the program: the grid line, the pair count and the energies of the
+    input and of phase1.data / phase2.data against the oracle, the phase-1 target
+    met exactly, and the pools of the Martini types."""
+    data, prm = F.write_preset(name, tmp_path, target_pct=15.0, max_attempts=2000, patience=500)
+    r = _run(tmp_path, data, prm)
+    assert r.returncode == 0, r.stderr
+    rep = (tmp_path / "report.txt").read_text()
+    assert re.search(rf"PME grid: {grid[0]} x {grid[1]} x {grid[2]} order=4 alpha=", rep)
+    _, pf, s, p, mc = F.load_run(data, prm)
+    assert abs(pf.beta - 1.0 / (0.008314462 * 300.0)) < 1e-15 and (mc.nh_n_type, mc.nh_p_type) == (4, 5)
+    p.grid = grid
+    bonds = S.bond_adjacency(s.natoms, s.bonds)
+    o = O.Oracle(s, p, bonds)
+    nnz = len(o.nlist()[1])
+    assert f"Neighbor pairs (half): {nnz}\n" in rep
+    e0 = o.total_energy()[0]
+    m = re.search(r"\[summary\] E0/after phase1/after phase2 = (\S+) / (\S+) / (\S+)", rep)
+    assert m
+    es = [float(v) for v in m.groups()]
+    assert abs(es[0] - e0) <= 1e-6 + 1e-9 * abs(e0)
+    nh = int(np.sum((s.type0 == 3) | (s.type0 == 4)))
+    target = int(np.floor(15.0 / 100.0 * nh))
+    for k, fname in ((1, "phase1.data"), (2, "phase2.data")):
+        _, t, q = _identities(tmp_path / fname)
+        assert int(np.sum(t == 4)) == target                      # NH_P count: phase 2 keeps it
+        assert abs(float(np.sum(q))) < 1e-9                        # neutral: protonation paired with Cl
+        o.set_charges(q, t)
+        e = o.total_energy()[0]
+        assert abs(es[k] - e) <= 1e-6 + 1e-9 * abs(e), (fname, es[k], e)
+    assert re.search(r"\[phase2\] step=\d+ E=", rep)
