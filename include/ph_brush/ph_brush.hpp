@@ -443,15 +443,47 @@ public:
         std::vector<signed char> acc;
         std::vector<double> dE;
     };
+    // With a trace, the device keeps the records of record_cap trials at a time and
+    // phase 2 runs window by window, so every trial of the run is traced.
     Phase2Result phase2(const Params& hp, Phase2Trace* trace = nullptr, long long record_cap = 1LL << 22) {
         phb_phase2_desc d{};
         d.patience = hp.patience; d.best_eps = 1e-9;
-        d.record_cap = trace ? std::max(0LL, std::min<long long>(hp.max_attempts, record_cap)) : 0;
+        const long long window = trace ? std::max(1LL, std::min<long long>(hp.max_attempts, record_cap)) : 0;
+        d.record_cap = window;
         phb_phase2_stats s{};
         check(phb_phase2_begin(h_, &d, &s));
         const double E0 = s.E;
         const int mode = s.mode;
-        if (s.mode != 0) check(phb_phase2_run(h_, hp.max_attempts, &s));
+        if (trace) {
+            trace->atoms.clear();
+            trace->acc.clear();
+            trace->dE.clear();
+        }
+        if (s.mode != 0) {
+            if (!trace) {
+                check(phb_phase2_run(h_, hp.max_attempts, &s));
+            } else {
+                long long left = hp.max_attempts;
+                std::vector<int> at;
+                std::vector<signed char> ac;
+                std::vector<double> de;
+                while (left > 0) {
+                    const long long first = s.attempted;
+                    check(phb_phase2_record_from(h_, first));
+                    check(phb_phase2_run(h_, std::min(left, window), &s));
+                    const long long n = s.attempted - first;
+                    at.assign(4 * (size_t)n, -1);
+                    ac.assign((size_t)n, 0);
+                    de.assign((size_t)n, 0.0);
+                    check(phb_phase2_record(h_, n, at.data(), ac.data(), de.data()));
+                    trace->atoms.insert(trace->atoms.end(), at.begin(), at.end());
+                    trace->acc.insert(trace->acc.end(), ac.begin(), ac.end());
+                    trace->dE.insert(trace->dE.end(), de.begin(), de.end());
+                    if (s.stop != 0 || n == 0) break;
+                    left -= n;
+                }
+            }
+        }
         Phase2Result r;
         r.attempted = s.attempted; r.accepted = s.accepted; r.E = s.E; r.best_E = s.best_E;
         if (trace) {
@@ -459,11 +491,6 @@ public:
             trace->stop = s.stop;
             trace->since_best = s.since_best;
             trace->E0 = E0;
-            const long long n = std::min(s.attempted, d.record_cap);
-            trace->atoms.assign(4 * (size_t)n, -1);
-            trace->acc.assign((size_t)n, 0);
-            trace->dE.assign((size_t)n, 0.0);
-            check(phb_phase2_record(h_, n, trace->atoms.data(), trace->acc.data(), trace->dE.data()));
         }
         return r;
     }

@@ -355,9 +355,12 @@ typedef struct {
 int phb_phase2_set_pools(phb_mc mc, const int sizes[4], const int *lists);
 int phb_phase2_begin(phb_mc mc, const phb_phase2_desc *d, phb_phase2_stats *out);
 int phb_phase2_run(phb_mc mc, long long max_attempts, phb_phase2_stats *out);
-/* First n trials: atoms (4 per trial in draw order iP, iN, jCl, jW; W-Cl only: jCl, jW, -1, -1),
- * accepted flag and exact Delta-E. */
+/* First n records: atoms (4 per trial in draw order iP, iN, jCl, jW; W-Cl only: jCl, jW, -1, -1),
+ * accepted flag and exact Delta-E.  Records are kept for trials first .. first + record_cap - 1
+ * of the phase (first = 0 after begin); record_from moves that window for the trials still to
+ * run, so that a long phase 2 is recorded window by window. */
 int phb_phase2_record(phb_mc mc, long long n, int *atoms, signed char *acc, double *dE);
+int phb_phase2_record_from(phb_mc mc, long long first);
 
 /* Time every chain launch, refresh and recompute with HIP events (on = 1). */
 int phb_mc_set_profiling(phb_mc mc, int on);

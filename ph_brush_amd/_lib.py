@@ -218,6 +218,7 @@ SIGNATURES = {
     "phb_phase2_set_pools": [_h, _pi, _pi],
     "phb_phase2_begin": [_h, C.POINTER(Phase2Desc), C.POINTER(Phase2Stats)],
     "phb_phase2_run": [_h, C.c_longlong, C.POINTER(Phase2Stats)],
+    "phb_phase2_record_from": [_h, C.c_longlong],
     "phb_phase2_record": [_h, C.c_longlong, _pi, C.POINTER(C.c_byte), _pd],
     "phb_verlet_create": [_h, C.POINTER(Box), _h, C.c_double, C.c_double, C.POINTER(_h)],
     "phb_verlet_destroy": [_h],

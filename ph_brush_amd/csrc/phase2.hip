@@ -44,6 +44,7 @@ struct P2Dev {
     int patience;
     double best_eps;
     long long rec_cap;
+    long long rec_base;        // trial index of record 0 (records rec_base .. rec_base + rec_cap - 1 are kept)
     int* rec_atoms;            // K per trial (4 slots)
     signed char* rec_acc;
     double* rec_dE;
@@ -299,11 +300,12 @@ __global__ __launch_bounds__(P2_THREADS) void k_mc_phase2(McDev D, P2Dev Z) {
             const int kstop = Z.patience - since0 - 1;   // last trial if every trial is rejected
             const bool pat_first = kstop < first && kstop < ntr;
             const int last = pat_first ? kstop : (first < ntr ? first : ntr - 1);
-            if (kk <= last && kk < ntr && done + kk < Z.rec_cap) {
+            const long long ri = done + kk - Z.rec_base;
+            if (kk <= last && kk < ntr && ri >= 0 && ri < Z.rec_cap) {
                 const Trial4<P>& t = s_tr[kk];
-                for (int k = 0; k < 4; ++k) Z.rec_atoms[4 * (done + kk) + k] = k < K ? t.atom[k] : -1;
-                Z.rec_acc[done + kk] = acc ? 1 : 0;
-                Z.rec_dE[done + kk] = dE;
+                for (int k = 0; k < 4; ++k) Z.rec_atoms[4 * ri + k] = k < K ? t.atom[k] : -1;
+                Z.rec_acc[ri] = acc ? 1 : 0;
+                Z.rec_dE[ri] = dE;
             }
             if (!pat_first && first < ntr && kk == first) {
                 // accepted: state update; the stream resumes right after this trial's draws
@@ -431,7 +433,7 @@ struct P2State {
     int K = 0;                 // 4, 2, or 0 (no valid proposals)
     int patience = 0;
     double best_eps = 1e-9;
-    long long rec_cap = 0;
+    long long rec_cap = 0, rec_base = 0;
     DevBuf<int> rec_atoms;
     DevBuf<signed char> rec_acc;
     DevBuf<double> rec_dE;
@@ -445,7 +447,7 @@ P2State* p2_of(phb_mc_s* M) {
 }
 
 void launch_p2(phb_mc_s* M, const P2State* S) {
-    P2Dev Z{S->K, S->patience, S->best_eps, S->rec_cap, S->rec_atoms.p, S->rec_acc.p, S->rec_dE.p};
+    P2Dev Z{S->K, S->patience, S->best_eps, S->rec_cap, S->rec_base, S->rec_atoms.p, S->rec_acc.p, S->rec_dE.p};
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(1), dim3(P2_THREADS), 0, stream(), M->dev, Z);
     };
@@ -555,6 +557,7 @@ extern "C" int phb_phase2_begin(phb_mc M, const phb_phase2_desc* d, phb_phase2_s
         S->patience = d->patience;
         S->best_eps = d->best_eps;
         S->rec_cap = d->record_cap;
+        S->rec_base = 0;
         S->rec_atoms.alloc(std::max<long long>(4 * d->record_cap, 1));
         S->rec_acc.alloc(std::max<long long>(d->record_cap, 1));
         S->rec_dE.alloc(std::max<long long>(d->record_cap, 1));
@@ -609,6 +612,15 @@ extern "C" int phb_phase2_run(phb_mc M, long long max_attempts, phb_phase2_stats
         }
         sync();
         if (out) fill_p2(M, S, out);
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_phase2_record_from(phb_mc M, long long first) {
+    return guard([&] {
+        require(M && first >= 0, "phb_phase2_record_from: bad arguments");
+        P2State* S = p2_of(M);
+        S->rec_base = first;
         return PHB_OK;
     });
 }

@@ -500,6 +500,10 @@ class MonteCarlo(_Handle):
         L.call("phb_phase2_run", self.h, max_attempts, C.byref(st))
         return st.as_dict()
 
+    def phase2_record_from(self, first: int) -> None:
+        """Keep the records of trials first .. first + record_cap - 1 of phase 2."""
+        L.call("phb_phase2_record_from", self.h, first)
+
     def phase2_record(self, n: int) -> dict:
         out = {"atoms": np.zeros((n, 4), np.int32), "acc": np.zeros(n, np.int8), "dE": np.zeros(n)}
         L.call("phb_phase2_record", self.h, n, L.iptr(out["atoms"]), out["acc"].ctypes.data_as(C.POINTER(C.c_byte)),

@@ -71,9 +71,15 @@ int main(int argc, char** argv) {
         std::printf("attempted %lld\naccepted %lld\nnprot %d\nE_after %.17g\n", r.attempted, r.accepted, r.Nprot, r.E);
         p.patience = 200;
         p.max_attempts = 400;
-        Phase2Result r2 = phase2_swap_jitter(mc, p);
+        // traced in windows of 97 device records (the MC program's per-interval lines read the trace)
+        MonteCarlo::Phase2Trace trace;
+        Phase2Result r2 = mc.phase2(p, &trace, 97);
         std::printf("p2_attempted %lld\np2_accepted %lld\np2_E %.17g\np2_best %.17g\n", r2.attempted, r2.accepted,
                     r2.E, r2.best_E);
+        long long tacc = 0;
+        double tde = 0.0;
+        for (size_t k = 0; k < trace.acc.size(); ++k) if (trace.acc[k]) { ++tacc; tde += trace.dE[k]; }
+        std::printf("trace_n %zu\ntrace_acc %lld\ntrace_E %.17g\n", trace.acc.size(), tacc, trace.E0 + tde);
         pmeDestroy(&plan);
     } catch (const std::exception& e) {
         std::cerr << "Error: " << e.what() << "\n";  // MC/src/main.cu:224-227

@@ -66,6 +66,9 @@ def test_cpp_api_matches_python_path(gpu, tmp_path):
     s2 = rep.mc.phase2_run(400)
     assert kv["p2_attempted"][0] == s2["attempted"] and kv["p2_accepted"][0] == s2["accepted"]
     assert abs(kv["p2_best"][0] - s2["best_E"]) <= 1e-9 * max(1.0, abs(s2["best_E"]))
+    # the trace, collected window by window (97 records at a time), covers every trial
+    assert kv["trace_n"][0] == s2["attempted"] and kv["trace_acc"][0] == s2["accepted"]
+    assert abs(kv["trace_E"][0] - s2["E"]) <= 1e-8 * max(1.0, abs(s2["E"]))
 
 
 @pytest.mark.gpu

@@ -91,3 +91,28 @@ def test_phase2_patience_stop_matches_oracle(gpu):
     # a further run does nothing once the patience is reached
     st2 = r.mc.phase2_run(100)
     assert st2["attempted"] == st["attempted"]
+
+
+def test_phase2_records_in_windows(gpu):
+    """Records kept window by window (phb_phase2_record_from) equal one record of
+    the whole run: the same trials, atoms, decisions and Delta-E."""
+    def setup():
+        s, p, _ = S.config_tiny(order=5, grid=(16, 16, 48), n_perm=2)
+        return S.Replica(s, p, S.McParams.at_ph(7.5), seed=9, pending_max=8)
+    r1 = setup()
+    r1.mc.phase2_begin(patience=10 ** 6, record_cap=700)
+    st1 = r1.mc.phase2_run(700)
+    whole = r1.mc.phase2_record(st1["attempted"])
+    r2 = setup()
+    r2.mc.phase2_begin(patience=10 ** 6, record_cap=64)
+    parts = []
+    done = 0
+    while done < 700:
+        r2.mc.phase2_record_from(done)
+        st2 = r2.mc.phase2_run(min(64, 700 - done))
+        n = st2["attempted"] - done
+        parts.append(r2.mc.phase2_record(n))
+        done = st2["attempted"]
+    for key in ("atoms", "acc"):
+        assert np.array_equal(np.concatenate([p[key] for p in parts]), whole[key]), key
+    np.testing.assert_allclose(np.concatenate([p["dE"] for p in parts]), whole["dE"], rtol=1e-10, atol=1e-10)
