@@ -48,17 +48,59 @@ def _traffic(kernel: str):
         return None
 
 
-def chain_bytes_per_trial(rep) -> float:
-    """Algorithmic bytes one trial must read (each byte once), FP64 path:
-    two full neighbour rows (4 B index + 8 B coefficient + 8 B neighbour charge
-    + 4 B neighbour type per entry), two p^3 potential stencils (8 B each) and
-    the 3 stencil-pair mesh-kernel windows ((2p-1)^3 x 8 B each)."""
-    import numpy as np
+def chain_bytes_per_trial(rep, fp32: bool = False) -> float:
+    """Algorithmic bytes one trial must read (each byte once): two full
+    neighbour rows (4 B index, the real-space coefficient, 8 B r^2 for the LJ
+    change, 8 B neighbour charge, 4 B neighbour type per entry), two p^3
+    potential stencils and the 3 stencil-pair mesh-kernel windows ((2p-1)^3
+    values each).  Coefficients, potential and mesh kernel are 8 B in the FP64
+    form and 4 B in the FP32 form."""
     s, p = rep.s, rep.p
     h, _ = rep.neigh.download()
     d_full = 2.0 * h[-1] / s.natoms
     P = p.pme_order
-    return 2 * d_full * 24 + 2 * P ** 3 * 8 + 3 * (2 * P - 1) ** 3 * 8
+    w = 4 if fp32 else 8
+    return 2 * d_full * (4 + w + 8 + 8 + 4) + 2 * P ** 3 * w + 3 * (2 * P - 1) ** 3 * w
+
+
+def integrated_autocorr(x, c: float = 5.0):
+    """Integrated autocorrelation time tau = 1/2 + sum_{t>=1} rho(t) of a series
+    (in samples), summed up to the first window M >= c tau(M) (Sokal's automatic
+    window); the effective sample size of the series is len(x) / (2 tau).
+    Returns (tau, window)."""
+    import numpy as np
+    x = np.asarray(x, dtype=np.float64)
+    x = x - x.mean()
+    n = len(x)
+    if n < 4 or not np.any(x):
+        return float("nan"), 0
+    f = np.fft.rfft(x, 2 * n)
+    acf = np.fft.irfft(f * np.conj(f))[:n]
+    acf /= acf[0]
+    taus = 0.5 + np.cumsum(acf[1:])
+    for m in range(1, n):
+        if m >= c * taus[m - 1]:
+            return float(taus[m - 1]), m
+    return float(taus[-1]), n - 1
+
+
+def ess_of_chain(nprot0, E0, fwd, acc, dE, rate, label):
+    """Effective samples of N_prot and of E per trial and per second, from a
+    chain's per-trial record (direction, accepted flag, exact Delta-E of the
+    accepted trials) and its timed trial rate."""
+    import numpy as np
+    acc = np.asarray(acc) != 0
+    step = np.where(acc, np.where(np.asarray(fwd) != 0, 1, -1), 0)
+    nprot = nprot0 + np.cumsum(step)
+    energy = E0 + np.cumsum(np.where(acc, np.nan_to_num(np.asarray(dE)), 0.0))
+    out = {"trials": int(len(step))}
+    for name, series in (("nprot", nprot), ("E", energy)):
+        tau, m = integrated_autocorr(series)
+        out[name] = {"tau_int_trials": tau, "window": m, "ess": len(series) / (2 * tau),
+                     "ess_per_s": rate / (2 * tau)}
+    print(f"[bench] {label} ESS: tau(N_prot) {out['nprot']['tau_int_trials']:.0f} trials, "
+          f"tau(E) {out['E']['tau_int_trials']:.0f} trials", file=sys.stderr, flush=True)
+    return out
 
 
 def refresh_uses_direct(rep, pending_max: int) -> bool:
@@ -176,6 +218,69 @@ def fp32_form(S, L, SG, args):
             "total_energy_rel_error_vs_oracle": abs(e32 - e64) / abs(e64)}
 
 
+def fp32_44k_form(S, L, s_eq, p, mc, kw, K, T, fp64_value, fp64_bpt):
+    """The FP32 form on the benchmark's own workload (44,000 atoms, from the same
+    equilibrated state): trial moves/s over K steps of T trials, and the live
+    launch time of its chain kernel against its algorithmic bytes."""
+    rep = S.Replica(s_eq, p, mc, seed=1, precision=L.PHB_FP32, **kw)
+    rep.mc.run(T)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        rep.mc.run(T)
+    dt = time.perf_counter() - t0
+    rep.mc.set_profiling(True)
+    a = rep.mc.stats()
+    rep.mc.run(T)
+    b = rep.mc.stats()
+    rep.mc.set_profiling(False)
+    launches = max(b["launches"] - a["launches"], 1)
+    bpt = chain_bytes_per_trial(rep, fp32=True)
+    launch_s = b["chain_ms"] / 1e3 / launches
+    tpl = (b["attempted"] - a["attempted"]) / launches
+    achieved = bpt * tpl / launch_s / 1e9
+    value = K * T / dt
+    print(f"[bench] fp32 form on the 44k brush: {value:.0f} trial moves/s", file=sys.stderr, flush=True)
+    return {"value": value, "fp64_value": fp64_value, "over_fp64": value / fp64_value, "unit": "trial moves/s",
+            "workload": "the benchmark's 44,000-atom diblock brush, same equilibrated state and seed",
+            "roofline": {"kernel": "k_mc_chain (FP32 form)", "bound": "hbm", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "avg_launch_us": launch_s * 1e6, "bytes_per_trial": bpt, "trials_per_launch": tpl,
+                         "fp64_bytes_per_trial": fp64_bpt}}
+
+
+def setup_form(S, reps: int = 5):
+    """The set-up path on the 44,000-atom workload, each step timed on the host
+    around synchronous calls: the half neighbour list (cells, counts, scan, fill,
+    row sort) against its algorithmic bytes (FP64 positions 24 N, cell index and
+    cell-sorted atom ids 8 N, heads 4 (N + 1), entries 4 nnz written), and the
+    MC object's set-up (full rows, coefficients, stencils, mesh kernel, potential)."""
+    s, p, mc = S.config_30k_fp64()
+    atoms, box = S.DeviceAtoms(s), S.box_of(s)
+    bonds = S.DeviceCSR.from_host(*S.bond_adjacency(s.natoms, s.bonds))
+    S.build_half_neighbors_gpu(atoms, box, bonds, p.rc)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        neigh = S.build_half_neighbors_gpu(atoms, box, bonds, p.rc)
+        ts.append(time.perf_counter() - t0)
+    h, _ = neigh.download()
+    nnz, N = int(h[-1]), s.natoms
+    nb_bytes = 24 * N + 8 * N + 4 * (N + 1) + 4 * nnz
+    params = S.DeviceParams(p)
+    pme = S.PMEPlan(params, box, p.rc)
+    tm = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        m = S.MonteCarlo(atoms, params, neigh, pme, box, mc, 1)
+        tm.append(time.perf_counter() - t0)
+        del m
+    best = min(ts)
+    return {"natoms": N, "nnz_half": nnz, "nlist_ms": best * 1e3, "nlist_bytes": nb_bytes,
+            "nlist_GBs": nb_bytes / best / 1e9, "nlist_frac": nb_bytes / best / 1e9 / HBM_PEAK_GBS,
+            "mc_create_ms": min(tm) * 1e3,
+            "note": "host wall time of synchronous calls (allocation included); kernel times in the rocprof summary"}
+
+
 def program_form():
     """The command-line program (ph_brush_amd/bin/MC, the reference's `MC in.data
     params.in`) end to end on the 44,000-atom workload written as input files:
@@ -274,9 +379,13 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-program", action="store_true", help="skip timing the MC command-line program")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="time the set-up path (neighbour list, MC object) on the 44k workload and exit")
     ap.add_argument("--no-da-real-space", action="store_true",
                     help="skip the delayed-acceptance form whose surrogate includes the real-space potential")
     ap.add_argument("--fp32-trials", type=int, default=100000)
+    ap.add_argument("--ess-trials", type=int, default=2000000,
+                    help="recorded trials per form for the autocorrelation times of N_prot and E (0: skip)")
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
     ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
     ap.add_argument("--serial-equil", type=int, default=20000)
@@ -294,6 +403,9 @@ def main():
     if L.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.setup_only:
+        print(json.dumps({"setup": setup_form(S)}))
+        return
     cfg = S.config_30k_fp64
     s, p, mc = cfg()
     K, W, T = args.steps, args.warmup, args.trials_per_step
@@ -333,6 +445,14 @@ def main():
     refresh_method = "direct" if refresh_uses_direct(rep, args.pending_max) else "kspace"
     refresh_gbs = rb / (stp1["refresh_ms"] / 1e3 / max(refreshes, 1)) / 1e9
     del rep
+    ess = {}
+    if args.ess_trials > 0:
+        rep = S.Replica(s_eq, p, mc, seed=3, record_cap=args.ess_trials, **kw)
+        st_e = rep.mc.stats()
+        rep.mc.run(args.ess_trials)
+        r = rep.mc.record(args.ess_trials)
+        ess["plain"] = ess_of_chain(st_e["nprot"], st_e["E"], r["fwd"], r["acc"], r["dE"], plain_value, "plain")
+        del rep, r
 
     # ---- surrogate: data from exact chains at pH 6-9, GPU training (untimed)
     t_sur = time.perf_counter()
@@ -371,6 +491,20 @@ def main():
     da_achieved = da_bpt * da_trials_per_launch / da_launch_s / 1e9
     infer_tflops = flop_fwd * da_trials_per_launch / da_launch_s / 1e12
     del rep
+    if args.ess_trials > 0:
+        rep = S.Replica(s_eq, p, mc, seed=3, record_cap=args.ess_trials, **kw)
+        rep.mc.da_attach(r_desc=spec.r_desc)
+        st_e = rep.mc.stats()
+        rep.mc.da_run(sur, args.ess_trials)
+        r = rep.mc.da_record(args.ess_trials)
+        ess["da"] = ess_of_chain(st_e["nprot"], st_e["E"], r["fwd"], r["stage"] == 1, r["dE"], value, "da")
+        for k in ("nprot", "E"):
+            ess[f"da_over_plain_{k}"] = ess["da"][k]["ess_per_s"] / ess["plain"][k]["ess_per_s"]
+        del rep, r
+    # surrogate cost against the workload's 10^6 moves: time to train, trials after which DA has paid it back
+    train_total_s = col["seconds"] + tr["ms"] / 1e3
+    gain_per_trial_s = 1.0 / plain_value - 1.0 / value
+    break_even = train_total_s / gain_per_trial_s if gain_per_trial_s > 0 else float("inf")
 
     # ---- the same form with the neighbours' real-space potential in the surrogate's physics part
     # (its own network: the physics part sets the residual the network learns)
@@ -441,6 +575,12 @@ def main():
             "alpha_note": "alpha of the timed blocks (same start state and seed); equality at fixed pH is "
                           "tested over long runs in tests/test_gpu_da.py",
         },
+        "ess": ess or None,
+        "surrogate_cost": {"collect_s": col["seconds"], "train_s": tr["ms"] / 1e3, "total_s": train_total_s,
+                           "break_even_trials": break_even,
+                           "workload_1e6_moves_s": {"plain": 1e6 / plain_value,
+                                                    "da_with_training": train_total_s + 1e6 / value,
+                                                    "da_without_training": 1e6 / value}},
         "surrogate": {"samples": col["samples"], "collect_s": col["seconds"], "train_steps": tr["steps"],
                       "train_ms": tr["ms"], "rmse_holdout_kT": tr["rmse_holdout"],
                       "mae_holdout_kT": tr["mae_holdout"], "target_std_kT": tr["target_std"],
@@ -485,10 +625,12 @@ def main():
     }
     if not args.no_fp32:
         out["forms"]["fp32"] = fp32_form(S, L, SG, args)
+        out["forms"]["fp32_44k"] = fp32_44k_form(S, L, s_eq, p, mc, kw, K, T, plain_value, bpt)
     if not args.no_batched:
         out["forms"]["batched"] = batched_form(S, args)
     if not args.no_program:
         out["forms"]["program"] = program_form()
+    out["setup"] = setup_form(S)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         n = args.cpu_trials or 150

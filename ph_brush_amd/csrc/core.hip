@@ -81,29 +81,129 @@ void reduce_partials(const double* d_partials, int n, int stride, int nsum, doub
     PHB_HIP(hipGetLastError());
 }
 
-// One block of 1024 threads: each thread owns a contiguous segment.
-__global__ void k_exclusive_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+// Exclusive scan in three passes (deterministic: fixed segments, fixed order):
+// block sums of SCAN_TILE-element tiles, one block scanning the tile sums, then
+// each tile's scan from its offset.  Each thread owns SCAN_PER consecutive
+// elements of its tile.
+constexpr int SCAN_TPB = 256, SCAN_PER = 16, SCAN_TILE = SCAN_TPB * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_tile_sums(const int* __restrict__ in, int n,
+                                                             long long* __restrict__ tsum) {
+    __shared__ long long s[SCAN_TPB / 64];
+    const int b0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    long long v = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) v += b0 + k < n ? in[b0 + k] : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot = 0;
+        for (int w = 0; w < SCAN_TPB / 64; ++w) tot += s[w];
+        tsum[blockIdx.x] = tot;
+    }
+}
+
+// One block: exclusive scan of the tile sums in place (each thread a segment of
+// consecutive tiles, the segment sums scanned across the block); out[n] = total.
+__global__ __launch_bounds__(1024) void k_scan_tile_offsets(long long* __restrict__ tsum, int ntiles, int* __restrict__ out,
+                                                            int n) {
     __shared__ long long part[1024];
     const int T = blockDim.x, t = threadIdx.x;
-    const int seg = (n + T - 1) / T;
-    const int b = min(n, t * seg), e = min(n, b + seg);
+    const int seg = (ntiles + T - 1) / T;
+    const int b = min(ntiles, t * seg), e = min(ntiles, b + seg);
     long long s = 0;
-    for (int k = b; k < e; ++k) s += in[k];
+    for (int k = b; k < e; ++k) s += tsum[k];
     part[t] = s;
     __syncthreads();
-    if (t == 0) {
-        long long run = 0;
-        for (int k = 0; k < T; ++k) { long long v = part[k]; part[k] = run; run += v; }
-        out[n] = (int)run;
+    for (int o = 1; o < T; o <<= 1) {
+        const long long add = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
     }
+    if (t == T - 1) out[n] = (int)part[t];
+    long long run = part[t] - s;
+    for (int k = b; k < e; ++k) { const long long v = tsum[k]; tsum[k] = run; run += v; }
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                         const long long* __restrict__ toff) {
+    __shared__ long long s[SCAN_TPB];
+    const int b0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    int v[SCAN_PER];
+    long long mine = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        v[k] = b0 + k < n ? in[b0 + k] : 0;
+        mine += v[k];
+    }
+    s[threadIdx.x] = mine;
     __syncthreads();
-    long long run = part[t];
-    for (int k = b; k < e; ++k) { out[k] = (int)run; run += in[k]; }
+    // Hillis-Steele inclusive scan of the thread sums (256 entries)
+    for (int o = 1; o < SCAN_TPB; o <<= 1) {
+        const long long add = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    long long run = toff[blockIdx.x] + s[threadIdx.x] - mine;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        if (b0 + k < n) out[b0 + k] = (int)run;
+        run += v[k];
+    }
+}
+
+// Rows sorted ascending: one wave per row.  The row is staged in the wave's LDS
+// slice and each lane ranks its entries against it (every lane reads the same
+// LDS word at a time: a broadcast); entries are distinct, so ranks are unique.
+// Rows longer than the slice are ranked against global memory.
+constexpr int SORT_TPB = 256, SORT_CAP = 512;
+
+__global__ __launch_bounds__(SORT_TPB) void k_sort_rows_lds(int N, const int* __restrict__ head,
+                                                            const int* __restrict__ tmp, int* __restrict__ out) {
+    __shared__ int buf[SORT_TPB / 64][SORT_CAP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * (SORT_TPB / 64) + w;
+    if (i >= N) return;   // wave-uniform
+    const int s = head[i], e = head[i + 1], len = e - s;
+    if (len <= SORT_CAP) {
+        int* row = buf[w];
+        for (int a = lane; a < len; a += 64) row[a] = tmp[s + a];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0);
+        for (int a = lane; a < len; a += 64) {
+            const int v = row[a];
+            int rank = 0;
+            for (int b = 0; b < len; ++b) rank += row[b] < v ? 1 : 0;
+            out[s + rank] = v;
+        }
+    } else {
+        for (int a = s + lane; a < e; a += 64) {
+            const int v = tmp[a];
+            int rank = 0;
+            for (int b = s; b < e; ++b) rank += tmp[b] < v ? 1 : 0;
+            out[s + rank] = v;
+        }
+    }
+}
+
+void sort_rows(int N, const int* d_head, const int* d_tmp, int* d_out) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_sort_rows_lds, dim3(ceil_div(N, SORT_TPB / 64)), dim3(SORT_TPB), 0, stream(), N, d_head,
+                       d_tmp, d_out);
+    PHB_HIP(hipGetLastError());
 }
 
 void exclusive_scan(const int* d_in, int* d_out, int n) {
-    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, stream(), d_in, d_out, n);
+    const int ntiles = std::max(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+    DevBuf<long long> tsum(ntiles);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(ntiles), dim3(SCAN_TPB), 0, stream(), d_in, n, tsum.p);
+    hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, stream(), tsum.p, ntiles, d_out, n);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_TPB), 0, stream(), d_in, d_out, n, tsum.p);
     PHB_HIP(hipGetLastError());
+    sync();   // tsum is freed on return
 }
 
 }  // namespace phb

@@ -229,6 +229,9 @@ void reduce_partials(const double* d_partials, int n, int stride, int nsum, doub
 // Exclusive scan of n ints into out[0..n] (out[n] = total), single block,
 // deterministic; n up to a few million.
 void exclusive_scan(const int* d_in, int* d_out, int n);
+// Rows i < N of a CSR (head) sorted ascending from the unsorted entries tmp into out
+// (entries of a row distinct).
+void sort_rows(int N, const int* d_head, const int* d_tmp, int* d_out);
 
 // ------------------------------------------------------------ geometry
 // Minimum image and r^2 with the arithmetic rounded exactly as written (no

@@ -118,49 +118,42 @@ __global__ void k_finish_recompute(McDev D, const double* __restrict__ res, doub
 }
 
 // ------------------------------------------------------------ setup kernels
-__global__ void k_full_degree(int N, const int* __restrict__ hh, const int* __restrict__ hl, int* __restrict__ deg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Full rows from the half list, one wave per half row (lanes over its entries).
+// Row i of the full list holds i's own half row first, then the atoms whose half
+// rows hold i (in arrival order; rows are sorted afterwards).
+__global__ __launch_bounds__(256) void k_full_degree(int N, const int* __restrict__ hh, const int* __restrict__ hl,
+                                                     int* __restrict__ deg) {
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= N) return;
     const int s = hh[i], e = hh[i + 1];
-    atomicAdd(&deg[i], e - s);
-    for (int k = s; k < e; ++k) atomicAdd(&deg[hl[k]], 1);
+    if (lane == 0) atomicAdd(&deg[i], e - s);
+    for (int k = s + lane; k < e; k += 64) atomicAdd(&deg[hl[k]], 1);
 }
 
-__global__ void k_full_fill(int N, const int* __restrict__ hh, const int* __restrict__ hl, const int* __restrict__ fh,
-                            int* __restrict__ cursor, int* __restrict__ tmp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_full_fill(int N, const int* __restrict__ hh, const int* __restrict__ hl,
+                                                   const int* __restrict__ fh, int* __restrict__ cursor,
+                                                   int* __restrict__ tmp) {
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= N) return;
-    for (int k = hh[i]; k < hh[i + 1]; ++k) {
+    const int s = hh[i], e = hh[i + 1];
+    for (int k = s + lane; k < e; k += 64) {
         const int j = hl[k];
-        tmp[fh[i] + atomicAdd(&cursor[i], 1)] = j;
-        tmp[fh[j] + atomicAdd(&cursor[j], 1)] = i;
+        tmp[fh[i] + (k - s)] = j;
+        tmp[fh[j] + (hh[j + 1] - hh[j]) + atomicAdd(&cursor[j], 1)] = i;
     }
 }
 
-__global__ __launch_bounds__(256) void k_full_sort(int N, const int* __restrict__ fh, const int* __restrict__ tmp,
-                                                   int* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= N) return;
-    const int s = fh[i], e = fh[i + 1];
-    for (int a = s + lane; a < e; a += 64) {
-        const int v = tmp[a];
-        int rank = 0;
-        for (int b = s; b < e; ++b) rank += tmp[b] < v ? 1 : 0;
-        out[s + rank] = v;
-    }
-}
-
-__global__ void k_full_coeffs(int N, const double* __restrict__ x, const double* __restrict__ y,
+__global__ __launch_bounds__(256) void k_full_coeffs(int N, const double* __restrict__ x, const double* __restrict__ y,
                               const double* __restrict__ z, double Lx, double Ly, double Lz, double elF, double alpha,
                               const int* __restrict__ fh, const int* __restrict__ fn, double* __restrict__ kreal,
                               double* __restrict__ r2o) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);   // one wave per row
     if (i >= N) return;
-    for (int e = fh[i]; e < fh[i + 1]; ++e) {
+    const double xi = x[i], yi = y[i], zi = z[i];
+    for (int e = fh[i] + lane; e < fh[i + 1]; e += 64) {
         const int j = fn[e];
-        const double dx = min_image_exact(x[i] - x[j], Lx), dy = min_image_exact(y[i] - y[j], Ly),
-                     dz = min_image_exact(z[i] - z[j], Lz);
+        const double dx = min_image_exact(xi - x[j], Lx), dy = min_image_exact(yi - y[j], Ly),
+                     dz = min_image_exact(zi - z[j], Lz);
         const double r2 = r2_exact(dx, dy, dz), r = sqrt(r2);
         r2o[e] = r2;
         kreal[e] = elF * (erfc(alpha * r) / r);
@@ -260,15 +253,15 @@ long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBu
     PHB_HIP(hipMemsetAsync(deg.p, 0, N * sizeof(int), st));
     PHB_HIP(hipMemsetAsync(cursor.p, 0, N * sizeof(int), st));
     head.alloc(N + 1);
-    hipLaunchKernelGGL(k_full_degree, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, half->head.p, half->list.p, deg.p);
+    hipLaunchKernelGGL(k_full_degree, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, half->head.p, half->list.p, deg.p);
     exclusive_scan(deg.p, head.p, N);
     const long long nfull = 2 * half->nnz;
     nbr.alloc(std::max<long long>(nfull, 1));
     if (nfull > 0) {
         DevBuf<int> tmp(nfull);
-        hipLaunchKernelGGL(k_full_fill, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, half->head.p, half->list.p, head.p,
+        hipLaunchKernelGGL(k_full_fill, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, half->head.p, half->list.p, head.p,
                            cursor.p, tmp.p);
-        hipLaunchKernelGGL(k_full_sort, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, head.p, tmp.p, nbr.p);
+        sort_rows(N, head.p, tmp.p, nbr.p);
         PHB_HIP(hipGetLastError());
         sync();
     }
@@ -510,7 +503,7 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         M->kreal.alloc(std::max<long long>(nfull, 1));
         M->r2.alloc(std::max<long long>(nfull, 1));
         if (nfull > 0) {
-            hipLaunchKernelGGL(k_full_coeffs, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, a->x.p, a->y.p, a->z.p,
+            hipLaunchKernelGGL(k_full_coeffs, dim3(ceil_div(N, 4)), dim3(256), 0, st, N, a->x.p, a->y.p, a->z.p,
                                box->Lx, box->Ly, box->Lz, pme->elF, pme->alpha, M->fhead.p, M->fnbr.p, M->kreal.p,
                                M->r2.p);
             PHB_HIP(hipGetLastError());

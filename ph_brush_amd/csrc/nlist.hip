@@ -23,7 +23,17 @@ using namespace phb;
 struct CellGeom {
     int ncx, ncy, ncz;
     double Lx, Ly, Lz;
+    double ix, iy, iz;   // 1/L along axes with >= 3 cells, else 0 (divide)
 };
+
+// dx - L rint(dx/L).  Along an axis with >= 3 cells (L >= 3 rc) the quotient is
+// taken as dx * (1/L): it can round differently from dx / L only when dx is
+// within an ulp of +-L/2, where either image lies at |dx| ~ L/2 > rc and the
+// pair fails the cutoff whichever is taken, so the list is unchanged.
+__device__ __forceinline__ double min_image_cell(double dx, double L, double inv) {
+#pragma clang fp contract(off)
+    return inv != 0.0 ? dx - L * rint(dx * inv) : dx - L * rint(dx / L);
+}
 
 __device__ __forceinline__ int cell_of(double v, double L, int nc) {
     int c = (int)floor(v / L * nc);
@@ -49,6 +59,17 @@ __global__ void k_cell_fill(const int* __restrict__ cell, int N, const int* __re
     catoms[cstart[c] + slot] = i;
 }
 
+// Positions in cell order, so that a wave reading a cell's candidates loads
+// consecutive addresses.
+__global__ void k_cell_coords(const int* __restrict__ catoms, int N, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ xs,
+                              double* __restrict__ ys, double* __restrict__ zs) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = catoms[k];
+    xs[k] = x[i]; ys[k] = y[i]; zs[k] = z[i];
+}
+
 __device__ __forceinline__ bool bonded12(int i, int j, const int* __restrict__ bh, const int* __restrict__ bl) {
     if (!bh) return false;
     int lo = bh[i], hi = bh[i + 1] - 1;
@@ -64,7 +85,9 @@ __device__ __forceinline__ bool bonded12(int i, int j, const int* __restrict__ b
 // MODE 0: count into counts[i]; MODE 1: write unsorted row into tmp at head[i].
 template <int MODE>
 __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, const double* __restrict__ y,
-                                               const double* __restrict__ z, int N, CellGeom g, double rc2,
+                                               const double* __restrict__ z, const double* __restrict__ xs,
+                                               const double* __restrict__ ys, const double* __restrict__ zs, int N,
+                                               CellGeom g, double rc2,
                                                const int* __restrict__ cell, const int* __restrict__ cstart,
                                                const int* __restrict__ catoms, const int* __restrict__ bh,
                                                const int* __restrict__ bl, int* __restrict__ counts,
@@ -92,11 +115,11 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, con
                     int j = -1;
                     if (idx < e) {
                         j = catoms[idx];
-                        if (j > i && !bonded12(i, j, bh, bl)) {
-                            double ddx = min_image_exact(xi - x[j], g.Lx);
-                            double ddy = min_image_exact(yi - y[j], g.Ly);
-                            double ddz = min_image_exact(zi - z[j], g.Lz);
-                            pass = r2_exact(ddx, ddy, ddz) <= rc2;
+                        if (j > i) {
+                            const double ddx = min_image_cell(xi - xs[idx], g.Lx, g.ix);
+                            const double ddy = min_image_cell(yi - ys[idx], g.Ly, g.iy);
+                            const double ddz = min_image_cell(zi - zs[idx], g.Lz, g.iz);
+                            pass = r2_exact(ddx, ddy, ddz) <= rc2 && !bonded12(i, j, bh, bl);
                         }
                     }
                     const unsigned long long m = __ballot(pass);
@@ -110,21 +133,6 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, con
         }
     }
     if (MODE == 0 && lane == 0) counts[i] = found;
-}
-
-// One wave per row: rank sort (entries are distinct).
-__global__ __launch_bounds__(256) void k_sort_rows(int N, const int* __restrict__ head, const int* __restrict__ tmp,
-                                                   int* __restrict__ list) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (i >= N) return;
-    const int s = head[i], e = head[i + 1];
-    for (int a = s + lane; a < e; a += 64) {
-        const int v = tmp[a];
-        int rank = 0;
-        for (int b = s; b < e; ++b) rank += tmp[b] < v ? 1 : 0;
-        list[s + rank] = v;
-    }
 }
 
 int cells_along(double L, double rc) {
@@ -191,11 +199,15 @@ void build_half(phb_atoms_s* a, const phb_box* box, phb_csr_s* bonds12, double r
     }
     {
         CellGeom g{cells_along(box->Lx, rc), cells_along(box->Ly, rc), cells_along(box->Lz, rc), box->Lx, box->Ly,
-                   box->Lz};
+                   box->Lz, 0.0, 0.0, 0.0};
+        g.ix = g.ncx >= 3 ? 1.0 / box->Lx : 0.0;
+        g.iy = g.ncy >= 3 ? 1.0 / box->Ly : 0.0;
+        g.iz = g.ncz >= 3 ? 1.0 / box->Lz : 0.0;
         const long long ncell = (long long)g.ncx * g.ncy * g.ncz;
         require(ncell < (1LL << 30), "build_half_neighbors_gpu: too many cells");
         hipStream_t st = stream();
         DevBuf<int> cell(N), ccount(ncell), cstart(ncell + 1), cursor(ncell), catoms(N), counts(N);
+        DevBuf<double> xs(N), ys(N), zs(N);
         PHB_HIP(hipMemsetAsync(ccount.p, 0, ncell * sizeof(int), st));
         PHB_HIP(hipMemsetAsync(cursor.p, 0, ncell * sizeof(int), st));
         const int tb = 256;
@@ -204,11 +216,14 @@ void build_half(phb_atoms_s* a, const phb_box* box, phb_csr_s* bonds12, double r
         exclusive_scan(ccount.p, cstart.p, (int)ncell);
         hipLaunchKernelGGL(k_cell_fill, dim3(ceil_div(N, tb)), dim3(tb), 0, st, cell.p, N, cstart.p, cursor.p,
                            catoms.p);
+        hipLaunchKernelGGL(k_cell_coords, dim3(ceil_div(N, tb)), dim3(tb), 0, st, catoms.p, N, a->x.p, a->y.p, a->z.p,
+                           xs.p, ys.p, zs.p);
         const int* bh = bonds12 && bonds12->nnz > 0 ? bonds12->head.p : nullptr;
         const int* bl = bonds12 && bonds12->nnz > 0 ? bonds12->list.p : nullptr;
         const double rc2 = rc * rc;
         const int wpb = tb / 64;
-        hipLaunchKernelGGL(k_pairs<0>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, N, g, rc2,
+        hipLaunchKernelGGL(k_pairs<0>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, xs.p, ys.p,
+                           zs.p, N, g, rc2,
                            cell.p, cstart.p, catoms.p, bh, bl, counts.p, (const int*)nullptr, (int*)nullptr);
         PHB_HIP(hipGetLastError());
         exclusive_scan(counts.p, c->head.p, N);
@@ -220,9 +235,10 @@ void build_half(phb_atoms_s* a, const phb_box* box, phb_csr_s* bonds12, double r
         c->list.alloc(total);
         if (total > 0) {
             DevBuf<int> tmp(total);
-            hipLaunchKernelGGL(k_pairs<1>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, N, g,
-                               rc2, cell.p, cstart.p, catoms.p, bh, bl, (int*)nullptr, c->head.p, tmp.p);
-            hipLaunchKernelGGL(k_sort_rows, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, N, c->head.p, tmp.p, c->list.p);
+            hipLaunchKernelGGL(k_pairs<1>, dim3(ceil_div(N, wpb)), dim3(tb), 0, st, a->x.p, a->y.p, a->z.p, xs.p,
+                               ys.p, zs.p, N, g, rc2, cell.p, cstart.p, catoms.p, bh, bl, (int*)nullptr, c->head.p,
+                               tmp.p);
+            sort_rows(N, c->head.p, tmp.p, c->list.p);
             PHB_HIP(hipGetLastError());
             sync();
         }
