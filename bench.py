@@ -379,6 +379,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-program", action="store_true", help="skip timing the MC command-line program")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--train-only", action="store_true",
+                    help="collect the surrogate's data on the 44k workload, train it, report the training rate and exit")
     ap.add_argument("--setup-only", action="store_true",
                     help="time the set-up path (neighbour list, MC object) on the 44k workload and exit")
     ap.add_argument("--no-da-real-space", action="store_true",
@@ -405,6 +407,19 @@ def main():
         sys.exit(1)
     if args.setup_only:
         print(json.dumps({"setup": setup_form(S)}))
+        return
+    if args.train_only:
+        s, p, mc = S.config_30k_fp64()
+        spec = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000, between=2000)
+        sur = S.Surrogate(len(spec.ph_values) * spec.samples_per_ph)
+        col = SG.collect(s, p, sur, spec)
+        tr = SG.train_holdout(sur, epochs=args.da_epochs, batch=4096, lr=2e-3)
+        flop = 3 * 2 * (48 * 128 + 128 * 128 + 128)
+        tflops = tr["steps"] * 4096 * flop / (tr["ms"] / 1e3) / 1e12
+        print(json.dumps({"train": {"samples": col["samples"], "collect_s": col["seconds"], "steps": tr["steps"],
+                                    "ms": tr["ms"], "us_per_step": tr["ms"] * 1e3 / max(tr["steps"], 1),
+                                    "tflops": tflops, "mfma_frac": tflops / 2500.0,
+                                    "rmse_holdout_kT": tr["rmse_holdout"], "target_std_kT": tr["target_std"]}}))
         return
     cfg = S.config_30k_fp64
     s, p, mc = cfg()

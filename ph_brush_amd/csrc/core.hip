@@ -159,7 +159,7 @@ __global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int* __restrict__
 // slice and each lane ranks its entries against it (every lane reads the same
 // LDS word at a time: a broadcast); entries are distinct, so ranks are unique.
 // Rows longer than the slice are ranked against global memory.
-constexpr int SORT_TPB = 256, SORT_CAP = 512;
+constexpr int SORT_TPB = 256, SORT_CAP = 2048;
 
 __global__ __launch_bounds__(SORT_TPB) void k_sort_rows_lds(int N, const int* __restrict__ head,
                                                             const int* __restrict__ tmp, int* __restrict__ out) {

@@ -162,18 +162,23 @@ __device__ __forceinline__ float xor_sum_groups(float v) {  // sum over the 4 la
     return v;
 }
 
-// Loads 64 descriptor rows (dataset indices from idx, or rows first.. when idx is null) into X.
+// Loads 64 descriptor rows (dataset indices from idx, or rows first.. when idx is null) into X,
+// scaled by in_scale and rounded to BF16: four threads per row, each with three 8-byte loads
+// of 12 features issued together (a row is 96 bytes, 8-byte aligned).
 __device__ __forceinline__ void load_rows(const short* __restrict__ dd, const int* __restrict__ idx, long long first,
                                           int nrows, const float* __restrict__ theta, unsigned short* X, int tid,
                                           int nthr) {
-    for (int e = tid; e < 64 * NN_IN; e += nthr) {
-        const int r = e / NN_IN, c = e - r * NN_IN;
-        float v = 0.f;
+    typedef short sh4 __attribute__((ext_vector_type(4)));
+    for (int u = tid; u < 64 * 4; u += nthr) {
+        const int r = u >> 2, c0 = 12 * (u & 3);
+        sh4 v[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
         if (r < nrows) {
             const long long s = idx ? (long long)idx[r] : first + r;
-            v = (float)dd[s * NN_IN + c] * theta[OFF_SCALE + c];
+            const sh4* src = reinterpret_cast<const sh4*>(dd + s * NN_IN + c0);
+            v[0] = src[0]; v[1] = src[1]; v[2] = src[2];
         }
-        X[r * XP + c] = f2bf(v);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) X[r * XP + c0 + k] = f2bf((float)v[k >> 2][k & 3] * theta[OFF_SCALE + c0 + k]);
     }
 }
 
@@ -212,11 +217,40 @@ __global__ __launch_bounds__(256) void k_mlp_forward(NetDev n, const short* __re
     }
 }
 
-// Forward + backward over 64 rows per workgroup (4 waves, wave w owns rows 16w..16w+15);
-// writes the workgroup's gradient of sum_rows (y - t_hat)^2 / B and its sum of squares.
-__global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restrict__ dd, const float* __restrict__ dt,
-                                                  const int* __restrict__ idx, long long first, int B,
-                                                  float* __restrict__ part) {
+// Training step of the loss sum_rows (y - t_hat)^2 / B, in three launches:
+//   k_mlp_act   forward and backward activations, 64 rows per workgroup (4 waves, wave w owns
+//               rows 16w..16w+15), stored transposed in BF16 (row index fastest) for the weight
+//               products, and the workgroup's sums for the vector parameters (db1, db2, dw3, db3)
+//               and the loss;
+//   k_mlp_wgrad dW2 = H1^T DZ2 and dW1 = X^T DZ1 over the whole batch, one workgroup per 16x16
+//               output tile (16x16x32 BF16 MFMA along the rows, the four waves' sums added in a
+//               fixed order), plus one workgroup adding the vector sums in workgroup order;
+//   k_mlp_adam  Adam on the finished gradient.
+// Every sum has a fixed order: the gradient does not depend on scheduling.
+constexpr int VP = 3 * NN_H + 2;   // vector-parameter record per workgroup: db1, db2, dw3, db3, loss
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf8 ld8(const unsigned short* p) { return *reinterpret_cast<const bf8*>(p); }
+// 16x16x32: A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8); D as mfma16.
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// LDS copies of the weights in the training step (pitches in BF16 elements; the padding spreads
+// the 16 rows an MFMA operand reads over the LDS banks)
+constexpr int W1P = NN_IN + 8, W2P = NN_H + 8;
+constexpr size_t ACT_WLDS = (size_t)(NN_H * W1P + 2 * NN_H * W2P) * sizeof(unsigned short);
+
+struct TrainBufs {
+    unsigned short *xt, *h1t, *dz1t, *dz2t;   // [48][ld], [128][ld] x 3: column-major activations
+    float* vpart;                             // [nparts][VP]
+    int ld;                                   // rows of the batch
+};
+
+__global__ __launch_bounds__(256) void k_mlp_act(NetDev n, const short* __restrict__ dd, const float* __restrict__ dt,
+                                                 const int* __restrict__ idx, long long first, int B, TrainBufs tb) {
+    extern __shared__ __align__(16) unsigned char s_wraw[];
+    unsigned short* sW1 = reinterpret_cast<unsigned short*>(s_wraw);   // [128 out][W1P]
+    unsigned short* sW2t = sW1 + NN_H * W1P;                            // [128 out][W2P]
+    unsigned short* sW2 = sW2t + NN_H * W2P;                            // [128 in][W2P]
     __shared__ __align__(16) unsigned short X[64 * XP];
     __shared__ __align__(16) unsigned short H1[64 * HP];
     __shared__ __align__(16) unsigned short DZ1[64 * HP];
@@ -228,6 +262,17 @@ __global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restr
     const int b = blockIdx.x;
     const int* bidx = idx ? idx + (long long)b * 64 : nullptr;
     const long long bfirst = first + (long long)b * 64;
+    // weights into LDS (8-byte chunks), every load of the copy issued with the rows' loads
+    typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+    for (int c = tid; c < NN_H * (NN_IN / 4); c += 256) {
+        const int row = c / (NN_IN / 4), k = c - row * (NN_IN / 4);
+        *reinterpret_cast<us4*>(sW1 + row * W1P + 4 * k) = *reinterpret_cast<const us4*>(n.w1t + row * NN_IN + 4 * k);
+    }
+    for (int c = tid; c < NN_H * (NN_H / 4); c += 256) {
+        const int row = c / (NN_H / 4), k = c - row * (NN_H / 4);
+        *reinterpret_cast<us4*>(sW2t + row * W2P + 4 * k) = *reinterpret_cast<const us4*>(n.w2t + row * NN_H + 4 * k);
+        *reinterpret_cast<us4*>(sW2 + row * W2P + 4 * k) = *reinterpret_cast<const us4*>(n.w2 + row * NN_H + 4 * k);
+    }
     load_rows(dd, bidx, bfirst, 64, n.theta, X, tid, 256);
     const float om = n.theta[OFF_MEAN], os = n.theta[OFF_STD];
     if (tid < 64) {
@@ -237,13 +282,37 @@ __global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restr
     __syncthreads();
     const int r0 = wave * 16;
     f4 z1[8], z2[8];
-    layer1(X, r0, n, z1, lane);
+    {   // layer 1: 16x16x16 over the 48 inputs
+        s4 a[3];
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) a[kb] = ld4(X + (r0 + rr) * XP + kb * 16 + 4 * g);
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            const unsigned short* wr = sW1 + (ct * 16 + rr) * W1P + 4 * g;
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+            z1[ct] = acc + n.theta[OFF_B1 + ct * 16 + rr];
+        }
+    }
 #pragma unroll
     for (int ct = 0; ct < 8; ++ct)
 #pragma unroll
         for (int e = 0; e < 4; ++e) H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(z1[ct][e]));
     __syncthreads();
-    layer2(H1, r0, n, z2, lane);
+    {   // layer 2: 16x16x32 over the 128 hidden units
+        bf8 a[4];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) a[kb] = ld8(H1 + (r0 + rr) * HP + kb * 32 + 8 * g);
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            const unsigned short* wr = sW2t + (ct * 16 + rr) * W2P + 8 * g;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) acc = mfma32(a[kb], ld8(wr + kb * 32), acc);
+            z2[ct] = acc + n.theta[OFF_B2 + ct * 16 + rr];
+        }
+    }
     float yp[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ct = 0; ct < 8; ++ct) {
@@ -281,17 +350,17 @@ __global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restr
     }
     if (lane == 0) { s_red2[wave][0] = 0.f; s_red2[wave][1] = 0.f; }
     __syncthreads();
-    // dh1 = dz2 W2^T (this wave's rows), then GELU 1
+    // dh1 = dz2 W2^T (this wave's rows; 16x16x32), then GELU 1
     {
-        s4 a[8];
+        bf8 a[4];
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb) a[kb] = ld4(DZ2 + (r0 + rr) * HP + kb * 16 + 4 * g);
+        for (int kb = 0; kb < 4; ++kb) a[kb] = ld8(DZ2 + (r0 + rr) * HP + kb * 32 + 8 * g);
 #pragma unroll
         for (int ct = 0; ct < 8; ++ct) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
-            const unsigned short* wr = n.w2 + (ct * 16 + rr) * NN_H + 4 * g;
+            const unsigned short* wr = sW2 + (ct * 16 + rr) * W2P + 8 * g;
 #pragma unroll
-            for (int kb = 0; kb < 8; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+            for (int kb = 0; kb < 4; ++kb) acc = mfma32(a[kb], ld8(wr + kb * 32), acc);
             const int col = ct * 16 + rr;
             float db1 = 0.f;
 #pragma unroll
@@ -312,60 +381,94 @@ __global__ __launch_bounds__(256) void k_mlp_grad(NetDev n, const short* __restr
         if (lane == 0) { s_red2[wave][0] = gs; s_red2[wave][1] = l; }
     }
     __syncthreads();
-    float* out = part + (long long)b * NTP;
-    // dW2[in][out] = H1^T DZ2 over the 64 rows: 64 tiles, 16 per wave
-    for (int t = wave * 16; t < wave * 16 + 16; ++t) {
-        const int ti = t >> 3, tj = t & 7;
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            s4 a, bb;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int row = kb * 16 + 4 * g + e;
-                a[e] = (short)H1[row * HP + ti * 16 + rr];
-                bb[e] = (short)DZ2[row * HP + tj * 16 + rr];
-            }
-            acc = mfma16(a, bb, acc);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) out[OFF_W2 + (ti * 16 + 4 * g + e) * NN_H + tj * 16 + rr] = acc[e];
+    // the 64 rows' activations, transposed (row index fastest): 4 rows per 8-byte store
+    const int r0g = b * 64;
+    for (int e = tid; e < 16 * NN_IN; e += 256) {
+        const int c = e >> 4, r = 4 * (e & 15);
+        const us4 v = {X[r * XP + c], X[(r + 1) * XP + c], X[(r + 2) * XP + c], X[(r + 3) * XP + c]};
+        *reinterpret_cast<us4*>(tb.xt + (size_t)c * tb.ld + r0g + r) = v;
     }
-    // dW1[in][out] = X^T DZ1: 3 x 8 tiles, 6 per wave
-    for (int t = wave * 6; t < wave * 6 + 6; ++t) {
-        const int ti = t >> 3, tj = t & 7;
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            s4 a, bb;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int row = kb * 16 + 4 * g + e;
-                a[e] = (short)X[row * XP + ti * 16 + rr];
-                bb[e] = (short)DZ1[row * HP + tj * 16 + rr];
-            }
-            acc = mfma16(a, bb, acc);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) out[OFF_W1 + (ti * 16 + 4 * g + e) * NN_H + tj * 16 + rr] = acc[e];
+    for (int e = tid; e < 16 * NN_H; e += 256) {
+        const int c = e >> 4, r = 4 * (e & 15);
+        const size_t o = (size_t)c * tb.ld + r0g + r;
+        const us4 h = {H1[r * HP + c], H1[(r + 1) * HP + c], H1[(r + 2) * HP + c], H1[(r + 3) * HP + c]};
+        const us4 d1 = {DZ1[r * HP + c], DZ1[(r + 1) * HP + c], DZ1[(r + 2) * HP + c], DZ1[(r + 3) * HP + c]};
+        const us4 d2 = {DZ2[r * HP + c], DZ2[(r + 1) * HP + c], DZ2[(r + 2) * HP + c], DZ2[(r + 3) * HP + c]};
+        *reinterpret_cast<us4*>(tb.h1t + o) = h;
+        *reinterpret_cast<us4*>(tb.dz1t + o) = d1;
+        *reinterpret_cast<us4*>(tb.dz2t + o) = d2;
     }
+    float* vp = tb.vpart + (size_t)b * VP;
     for (int c = tid; c < NN_H; c += 256) {
-        out[OFF_B1 + c] = ((s_red[0][c] + s_red[1][c]) + s_red[2][c]) + s_red[3][c];
-        out[OFF_B2 + c] = ((s_red[0][NN_H + c] + s_red[1][NN_H + c]) + s_red[2][NN_H + c]) + s_red[3][NN_H + c];
-        out[OFF_W3 + c] =
+        vp[c] = ((s_red[0][c] + s_red[1][c]) + s_red[2][c]) + s_red[3][c];
+        vp[NN_H + c] = ((s_red[0][NN_H + c] + s_red[1][NN_H + c]) + s_red[2][NN_H + c]) + s_red[3][NN_H + c];
+        vp[2 * NN_H + c] =
             ((s_red[0][2 * NN_H + c] + s_red[1][2 * NN_H + c]) + s_red[2][2 * NN_H + c]) + s_red[3][2 * NN_H + c];
     }
     if (tid == 0) {
-        out[OFF_B3] = ((s_red2[0][0] + s_red2[1][0]) + s_red2[2][0]) + s_red2[3][0];
-        out[NN_TRAIN] = ((s_red2[0][1] + s_red2[1][1]) + s_red2[2][1]) + s_red2[3][1];
+        vp[3 * NN_H] = ((s_red2[0][0] + s_red2[1][0]) + s_red2[2][0]) + s_red2[3][0];
+        vp[3 * NN_H + 1] = ((s_red2[0][1] + s_red2[1][1]) + s_red2[2][1]) + s_red2[3][1];
     }
 }
 
-// Sum of the partial records in workgroup order (deterministic).
-__device__ __forceinline__ float sum_parts(const float* __restrict__ part, int nparts, int k) {
-    float s = 0.f;
-    for (int p = 0; p < nparts; ++p) s += part[(long long)p * NTP + k];
-    return s;
+constexpr int WG_TILES = 64 + 24;   // dW2: 8 x 8 tiles, dW1: 3 x 8 tiles of 16 x 16
+
+constexpr int WG_THREADS = 1024, WG_WAVES = WG_THREADS / 64, WG_UNROLL = 4;
+
+// g[W2 / W1 entries] and, by the last workgroup, g[vector parameters] and g[NN_TRAIN] (loss sum).
+// Wave w of a tile's workgroup takes the 32-row chunks w, w + 16, ...; each wave issues the loads
+// of WG_UNROLL chunks before their MFMAs.
+__global__ __launch_bounds__(WG_THREADS) void k_mlp_wgrad(TrainBufs tb, int nparts, float* __restrict__ g) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, q = lane >> 4;
+    const int t = blockIdx.x;
+    if (t == WG_TILES) {
+        for (int k = tid; k < VP; k += WG_THREADS) {
+            // the partials in workgroup order, loaded 16 at a time ahead of their (ordered) additions
+            float s = 0.f;
+            int p = 0;
+            for (; p + 16 <= nparts; p += 16) {
+                float v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = tb.vpart[(size_t)(p + u) * VP + k];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += v[u];
+            }
+            for (; p < nparts; ++p) s += tb.vpart[(size_t)p * VP + k];
+            const int o = k < NN_H ? OFF_B1 + k : k < 2 * NN_H ? OFF_B2 + (k - NN_H)
+                        : k < 3 * NN_H ? OFF_W3 + (k - 2 * NN_H) : k == 3 * NN_H ? OFF_B3 : NN_TRAIN;
+            g[o] = s;
+        }
+        return;
+    }
+    const bool w2 = t < 64;
+    const int tt = w2 ? t : t - 64, ti = tt >> 3, tj = tt & 7;
+    const unsigned short* At = (w2 ? tb.h1t : tb.xt) + (size_t)(ti * 16 + r) * tb.ld + 8 * q;
+    const unsigned short* Bt = (w2 ? tb.dz2t : tb.dz1t) + (size_t)(tj * 16 + r) * tb.ld + 8 * q;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int nchunk = tb.ld / 32;
+    int c = wave;
+    for (; c + (WG_UNROLL - 1) * WG_WAVES < nchunk; c += WG_UNROLL * WG_WAVES) {
+        bf8 a[WG_UNROLL], b[WG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < WG_UNROLL; ++u) {
+            a[u] = ld8(At + 32 * (c + u * WG_WAVES));
+            b[u] = ld8(Bt + 32 * (c + u * WG_WAVES));
+        }
+#pragma unroll
+        for (int u = 0; u < WG_UNROLL; ++u) acc = mfma32(a[u], b[u], acc);
+    }
+    for (; c < nchunk; c += WG_WAVES) acc = mfma32(ld8(At + 32 * c), ld8(Bt + 32 * c), acc);
+    __shared__ f4 red[WG_WAVES][64];
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0) {
+        f4 s = red[0][lane];
+#pragma unroll
+        for (int w = 1; w < WG_WAVES; ++w) s += red[w][lane];
+        const int off = w2 ? OFF_W2 : OFF_W1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g[off + (ti * 16 + 4 * q + e) * NN_H + tj * 16 + r] = s[e];
+    }
 }
 
 __device__ __forceinline__ void cast_param(int k, float v, unsigned short* w1t, unsigned short* w2t,
@@ -382,26 +485,21 @@ __device__ __forceinline__ void cast_param(int k, float v, unsigned short* w1t, 
     }
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ part, int nparts, float* __restrict__ g) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < NTP) g[k] = sum_parts(part, nparts, k);
-}
-
 __global__ void k_mlp_cast(const float* __restrict__ theta, unsigned short* w1t, unsigned short* w2t,
                            unsigned short* w2, unsigned short* w3) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < NN_TRAIN) cast_param(k, theta[k], w1t, w2t, w2, w3);
 }
 
-// Reduce the partial gradients, Adam (Kingma & Ba 2015, Algorithm 1), refresh the BF16 copies.
+// Adam (Kingma & Ba 2015, Algorithm 1) on the gradient g, then the BF16 copies.
 __global__ void k_mlp_adam(float* __restrict__ theta, float* __restrict__ m, float* __restrict__ v,
-                           const float* __restrict__ part, int nparts, float lr, float b1, float b2, float eps,
-                           float bc1, float bc2, unsigned short* w1t, unsigned short* w2t, unsigned short* w2,
-                           unsigned short* w3, float* __restrict__ loss_out, float invB) {
+                           const float* __restrict__ g, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                           unsigned short* w1t, unsigned short* w2t, unsigned short* w2, unsigned short* w3,
+                           float* __restrict__ loss_out, float invB) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == NN_TRAIN && loss_out) *loss_out = sum_parts(part, nparts, NN_TRAIN) * invB;
+    if (k == NN_TRAIN && loss_out) *loss_out = g[NN_TRAIN] * invB;
     if (k >= NN_TRAIN) return;
-    const float gk = sum_parts(part, nparts, k);
+    const float gk = g[k];
     const float mk = b1 * m[k] + (1.0f - b1) * gk;
     const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
     m[k] = mk;
@@ -1315,24 +1413,45 @@ using namespace phb;
 
 // ----------------------------------------------------------------------- host
 struct phb_surrogate_s {
-    DevBuf<float> theta, m, v, part, grad, loss_hist;
+    DevBuf<float> theta, m, v, vpart, grad, loss_hist;
+    DevBuf<unsigned short> act;   // transposed training activations of one batch: X^T, H1^T, DZ1^T, DZ2^T
     DevBuf<unsigned short> w1t, w2t, w2, w3;
     DevBuf<short> dd;          // dataset descriptors [cap][48]
     DevBuf<float> dt;          // dataset targets [cap]
     DevBuf<int> idx;
     long long cap = 0, n = 0, steps = 0;
-    int part_cap = 0;
+    int rows_cap = 0;
     NetDev net() const { return NetDev{w1t.p, w2t.p, w2.p, w3.p, theta.p}; }
     void cast() {
         hipLaunchKernelGGL(k_mlp_cast, dim3(ceil_div(NN_TRAIN, 256)), dim3(256), 0, stream(), theta.p, w1t.p, w2t.p,
                            w2.p, w3.p);
         PHB_HIP(hipGetLastError());
     }
-    void ensure_parts(int nparts) {
-        if (nparts > part_cap) {
-            part.alloc((size_t)nparts * NTP);
-            part_cap = nparts;
+    // buffers of a training batch of `rows` rows (a multiple of 64)
+    TrainBufs bufs(int rows) {
+        if (rows > rows_cap) {
+            act.alloc((size_t)(NN_IN + 3 * NN_H) * rows);
+            vpart.alloc((size_t)(rows / 64) * VP);
+            rows_cap = rows;
         }
+        unsigned short* a = act.p;
+        return TrainBufs{a, a + (size_t)NN_IN * rows, a + (size_t)(NN_IN + NN_H) * rows,
+                         a + (size_t)(NN_IN + 2 * NN_H) * rows, vpart.p, rows};
+    }
+    // gradient of one batch into grad (W1, W2 from the tile products, the rest and the loss sum from
+    // the workgroups' vector sums)
+    void batch_grad(const short* dd_, const float* dt_, const int* idx_, int rows) {
+        const TrainBufs tb = bufs(rows);
+        const int nb = rows / 64;
+        static bool lds_set = false;
+        if (!lds_set) {
+            PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_act),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ACT_WLDS));
+            lds_set = true;
+        }
+        hipLaunchKernelGGL(k_mlp_act, dim3(nb), dim3(256), ACT_WLDS, stream(), net(), dd_, dt_, idx_, 0LL, rows, tb);
+        hipLaunchKernelGGL(k_mlp_wgrad, dim3(WG_TILES + 1), dim3(WG_THREADS), 0, stream(), tb, nb, grad.p);
+        PHB_HIP(hipGetLastError());
     }
 };
 
@@ -1535,12 +1654,7 @@ extern "C" int phb_surrogate_grad(phb_surrogate s, int n, const short* d, const 
         DevBuf<float> dt(n);
         dd.upload(d, (size_t)n * NN_IN);
         dt.upload(t, n);
-        const int nb = n / 64;
-        s->ensure_parts(nb);
-        hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, stream(), s->net(), dd.p, dt.p, (const int*)nullptr, 0LL,
-                           n, s->part.p);
-        hipLaunchKernelGGL(k_mlp_reduce, dim3(ceil_div(NTP, 256)), dim3(256), 0, stream(), s->part.p, nb, s->grad.p);
-        PHB_HIP(hipGetLastError());
+        s->batch_grad(dd.p, dt.p, nullptr, n);
         std::vector<float> g(NTP);
         s->grad.download(g.data(), NTP);
         sync();
@@ -1637,8 +1751,7 @@ extern "C" int phb_surrogate_train(phb_surrogate s, const phb_train_desc* td, ph
         const long long total = spe * td->epochs;
         s->idx.alloc(std::max<long long>(spe * B, 1));
         s->loss_hist.alloc(std::max<long long>(total, 1));
-        const int nb = B / 64;
-        s->ensure_parts(nb);
+        s->bufs(B);
         std::vector<int> perm(ntr);
         std::mt19937_64 g(td->seed);
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -1651,10 +1764,9 @@ extern "C" int phb_surrogate_train(phb_surrogate s, const phb_train_desc* td, ph
                 ++s->steps;
                 const double bc1 = 1.0 - std::pow((double)b1, (double)s->steps);
                 const double bc2 = 1.0 - std::pow((double)b2, (double)s->steps);
-                hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, stream(), s->net(), s->dd.p, s->dt.p,
-                                   s->idx.p + b * B, 0LL, B, s->part.p);
+                s->batch_grad(s->dd.p, s->dt.p, s->idx.p + b * B, B);
                 hipLaunchKernelGGL(k_mlp_adam, dim3(ceil_div(NTP, 256)), dim3(256), 0, stream(), s->theta.p, s->m.p,
-                                   s->v.p, s->part.p, nb, (float)td->lr, b1, b2, eps, (float)bc1, (float)bc2, s->w1t.p,
+                                   s->v.p, s->grad.p, (float)td->lr, b1, b2, eps, (float)bc1, (float)bc2, s->w1t.p,
                                    s->w2t.p, s->w2.p, s->w3.p, s->loss_hist.p + k, 1.0f / B);
             }
             PHB_HIP(hipGetLastError());
