@@ -317,8 +317,12 @@ def program_form():
 def batched_form(S, args):
     """32 replicas of a pH x salt titration grid (uneven sizes and grids)
     advanced together; aggregate trial moves/s over timed sweeps after the
-    autotuner has picked the launch shape, alpha per replica, and the 2,640-atom
-    brush's alpha(pH) run serially for comparison."""
+    autotuner has picked the form (graph replays or one persistent launch per
+    sweep) and its shape; alpha per replica averaged over the timed sweeps; the
+    dominant kernel's roofline; per-replica rates against serial runs of the
+    same replicas (the same chains: a serial copy run for the same number of
+    trials reaches the same state)."""
+    import numpy as np
     t0 = time.perf_counter()
     grid = S.config_titration_grid()
     reps = [S.Replica(s, p, mc, seed=7 + k, recompute_interval=0) for k, (s, p, mc, _) in enumerate(grid)]
@@ -327,36 +331,73 @@ def batched_form(S, args):
     print(f"[bench] batched: {len(reps)} replicas, {sum(r.s.natoms for r in reps)} atoms, setup {t_setup:.1f}s",
           file=sys.stderr, flush=True)
     tune = b.autotune(1)
-    print(f"[bench] batched: autotune -> W={tune['spec_window']} pending={tune['pending_max']} "
-          f"pairs={tune['graph_pairs']} launch_cap={tune['launch_cap']}", file=sys.stderr, flush=True)
+    print(f"[bench] batched: autotune -> persistent={tune['persistent']} W={tune['spec_window']} "
+          f"pending={tune['pending_max']} pairs={tune['graph_pairs']} launch_cap={tune['launch_cap']}",
+          file=sys.stderr, flush=True)
     b.run(args.batch_warmup)
-    st = b.run(args.batch_sweeps)
-    rate = st["trials_per_s"]
-    alphas = {}
-    for (s, p, mc, label), r in zip(grid, reps):
-        ss = r.mc.stats()
-        alphas[label.split(":")[0]] = ss["nprot"] / ss["nsites"]
+    st0 = [r.mc.stats() for r in reps]
+    nsw, ms, launches, replays = args.batch_sweeps, 0.0, 0, 0
+    alpha = np.zeros((nsw, len(reps)))
+    for k in range(nsw):
+        st = b.run(1)
+        ms += st["ms"]
+        launches += st["launches"]
+        replays += st["replays"]
+        for q, r in enumerate(reps):
+            ss = r.mc.stats()
+            alpha[k, q] = ss["nprot"] / ss["nsites"]
+    st1 = [r.mc.stats() for r in reps]
+    trials = sum(b1["attempted"] - a1["attempted"] for a1, b1 in zip(st0, st1))
+    rate = trials / (ms / 1e3)
     print(f"[bench] batched: {rate:.0f} trial moves/s aggregate", file=sys.stderr, flush=True)
-    # serial reference curve on the 2,640-atom brush
-    s1, p1, _ = S.config_small_fp64()
-    curve = {}
-    for ph in (5.0, 5.5, 6.0, 6.5, 7.0, 7.5, 8.0, 8.5):
-        rep = S.Replica(s1, p1, S.McParams.at_ph(ph), seed=1, recompute_interval=0)
-        rep.mc.run(args.serial_equil)
-        a = []
-        for _ in range(20):
-            a.append(rep.mc.run(args.serial_trials // 20)["nprot"] / 1280.0)
-        curve[f"pH {ph}"] = sum(a) / len(a)
-        del rep
-    print("[bench] batched: serial alpha(pH) of the 2,640-atom brush done", file=sys.stderr, flush=True)
+    # roofline of the dominant kernel: every trial's algorithmic bytes and every refresh's
+    pm = tune["pending_max"]
+    byts = 0.0
+    for r, a1, b1 in zip(reps, st0, st1):
+        byts += (b1["attempted"] - a1["attempted"]) * chain_bytes_per_trial(r)
+        byts += (b1["accepted"] - a1["accepted"]) / max(pm // 2, 1) * refresh_bytes(r, pm)
+    achieved = byts / (ms / 1e3) / 1e9
+    kern = "k_sweep_persistent" if tune["persistent"] else "k_chain_batch + refresh launches"
+    per_rep = None
+    if tune["persistent"]:
+        run_ms, ref_ms = b.replica_times()
+        nat = [g[0].natoms for g in grid]
+        order = sorted(range(len(grid)), key=lambda k: nat[k])
+        per_rep = {"run_ms_max": float(run_ms.max()), "run_ms_mean": float(run_ms.mean()),
+                   "refresh_share": float(ref_ms.sum() / max(run_ms.sum(), 1e-9)), "replicas": []}
+        for k in (order[0], order[len(order) // 2], order[-1]):
+            s, p, mc, lab = grid[k]
+            n_run = st1[k]["attempted"]
+            ns = st1[k]["nsites"]
+            r = S.Replica(s, p, mc, seed=7 + k, recompute_interval=0, pending_max=pm,
+                          spec_window=tune["spec_window"])
+            r.mc.run(n_run - ns)
+            tq = time.perf_counter()
+            r.mc.run(ns)
+            dts = time.perf_counter() - tq
+            same = r.mc.stats()["nprot"] == st1[k]["nprot"] and r.mc.stats()["accepted"] == st1[k]["accepted"]
+            per_rep["replicas"].append({"label": lab, "natoms": nat[k], "sites": ns,
+                                        "batched_per_s": ns / (run_ms[k] / 1e3), "serial_per_s": ns / dts,
+                                        "ratio": (ns / (run_ms[k] / 1e3)) / (ns / dts),
+                                        "serial_state_equal": bool(same)})
+            del r
+        print(f"[bench] batched: per-replica rates {per_rep['replicas']}", file=sys.stderr, flush=True)
+    alphas = {lab.split(":")[0]: float(alpha[:, q].mean()) for q, (s, p, mc, lab) in enumerate(grid)}
     return {"value": rate, "unit": "trial moves/s (aggregate over replicas)", "replicas": len(reps),
-            "atoms": sum(r.s.natoms for r in reps), "sweeps_timed": args.batch_sweeps, "ms": st["ms"],
-            "trials_timed": st["attempted"], "launches": st["launches"], "graph_replays": st["replays"],
-            "autotuned": {"spec_window": tune["spec_window"], "pending_max": tune["pending_max"],
-                          "graph_pairs": tune["graph_pairs"], "launch_cap": tune["launch_cap"],
-                          "trials_per_s": tune["trials_per_s"]},
-            "alpha_by_replica": alphas, "alpha_curve_2640_serial": curve,
-            "note": "alpha after warm-up + timed sweeps from p_prot = 0.5 starts (no equilibration beyond them)"}
+            "atoms": sum(r.s.natoms for r in reps), "sweeps_timed": nsw, "ms": ms,
+            "trials_timed": trials, "launches": launches, "graph_replays": replays,
+            "autotuned": {"persistent": tune["persistent"], "spec_window": tune["spec_window"],
+                          "pending_max": tune["pending_max"], "graph_pairs": tune["graph_pairs"],
+                          "launch_cap": tune["launch_cap"], "trials_per_s": tune["trials_per_s"]},
+            "roofline": {"kernel": kern, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_sweep": byts / nsw,
+                         "note": "algorithmic bytes: each trial's rows, stencils and kernel windows, and each "
+                                 "refresh's grid pass, over the timed sweeps' wall time"},
+            "per_replica": per_rep,
+            "alpha_by_replica": alphas,
+            "note": f"alpha: mean over the {nsw} timed sweeps after {args.batch_warmup} warm-up sweeps and the "
+                    "autotuner's sweeps from p_prot = 0.5 starts; a serial copy of a replica run for the same "
+                    "trials reaches the same state (serial_state_equal)"}
 
 
 def main():
@@ -390,8 +431,6 @@ def main():
                     help="recorded trials per form for the autocorrelation times of N_prot and E (0: skip)")
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
     ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
-    ap.add_argument("--serial-equil", type=int, default=20000)
-    ap.add_argument("--serial-trials", type=int, default=60000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one replica chain on one GPU "

@@ -542,8 +542,12 @@ int phb_da_record(phb_mc mc, long long n, signed char *fwd, int *i, int *j, sign
  * a titration grid (own system, PME plan and chain each; same spline order,
  * FP64) advance together: one workgroup per replica in each chain launch, the
  * potential refreshes of all replicas in two launches, the (chain, refresh)
- * launch pairs captured in a hipGraph.  Each replica's accept/reject sequence
- * is the one phb_mc_run would give it. */
+ * launch pairs captured in a hipGraph; or, in the persistent form (every replica
+ * on the direct refresh), one cooperative launch per sweep in which each
+ * replica's workgroup runs its chain to the sweep target and the potential
+ * refreshes are folded tile by tile by that workgroup and the launch's other
+ * workgroups.  Each replica's accept/reject sequence is the one phb_mc_run would
+ * give it. */
 typedef struct phb_batch_s *phb_batch;
 typedef struct {
     int graph_pairs;           /* (chain, refresh) launch pairs per replay; 0 = 4 */
@@ -551,6 +555,7 @@ typedef struct {
     int spec_window;           /* out: current speculation window */
     int pending_max;           /* out: current pending-change budget */
     long long launch_cap;      /* out: trials after which a chain launch stops per replica (0: none) */
+    int persistent;            /* in: start in the persistent form; out: the current form */
 } phb_batch_desc;
 typedef struct {
     long long attempted, accepted;   /* totals over the replicas (since creation) */
@@ -558,18 +563,27 @@ typedef struct {
     double ms, trials_per_s;         /* wall time of this call; autotune: rate of the chosen shape */
     int spec_window, pending_max, graph_pairs;
     long long launch_cap;            /* a chain launch waits for its slowest replica: the trial cap bounds it */
+    int persistent;                  /* 1: one persistent launch per sweep */
 } phb_batch_stats;
 int phb_batch_create(const phb_mc *mcs, int n, const phb_batch_desc *d, phb_batch *out);
 int phb_batch_destroy(phb_batch b);
 /* `sweeps` sweeps: every replica attempts as many trials as it has titratable sites per sweep. */
 int phb_batch_run(phb_batch b, int sweeps, phb_batch_stats *st);
 /* Times speculation window x pending budget x graph length on the batch (each candidate runs
- * `sweeps_per_candidate` real sweeps), then per-launch trial caps on the fastest shape, and keeps
- * the fastest.  Launch boundaries do not change any replica's sequence. */
+ * `sweeps_per_candidate` real sweeps), the persistent form at each window x budget where every
+ * replica qualifies, then per-launch trial caps on the fastest graph shape, and keeps the fastest.
+ * Launch boundaries do not change any replica's sequence. */
 int phb_batch_autotune(phb_batch b, int sweeps_per_candidate, phb_batch_stats *best);
 /* Sets a launch shape without timing: speculation window (4, 8 or 16), pending budget (2..MC_KMAX, even),
  * (chain, refresh) pairs per replay (>= 1) and the per-launch trial cap (0: none). */
 int phb_batch_tune(phb_batch b, int spec_window, int pending_max, int graph_pairs, long long launch_cap);
+/* 1: the persistent form (one cooperative launch per sweep; PHB_ERR_UNSUPPORTED unless every replica
+ * uses the direct refresh and the device holds a workgroup per replica at once), 0: graph replays.
+ * A persistent sweep whose waits pass 30 s of wall clock stops with an error. */
+int phb_batch_mode(phb_batch b, int persistent);
+/* Per replica, of the last persistent sweep: wall time from the launch's start to the replica's
+ * finish and the part of it its chain spent on refreshes (waiting for and folding tiles), in ms. */
+int phb_batch_replica_times(phb_batch b, double *run_ms, double *refresh_ms);
 
 #ifdef __cplusplus
 }

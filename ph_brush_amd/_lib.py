@@ -138,14 +138,14 @@ class DaStats(C.Structure):
 
 class BatchDesc(C.Structure):
     _fields_ = [("graph_pairs", C.c_int), ("use_graph", C.c_int), ("spec_window", C.c_int), ("pending_max", C.c_int),
-                ("launch_cap", C.c_longlong)]
+                ("launch_cap", C.c_longlong), ("persistent", C.c_int)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("launches", C.c_longlong),
                 ("replays", C.c_longlong), ("ms", C.c_double), ("trials_per_s", C.c_double),
                 ("spec_window", C.c_int), ("pending_max", C.c_int), ("graph_pairs", C.c_int),
-                ("launch_cap", C.c_longlong)]
+                ("launch_cap", C.c_longlong), ("persistent", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -277,6 +277,8 @@ SIGNATURES = {
     "phb_batch_run": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_autotune": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_tune": [_h, C.c_int, C.c_int, C.c_int, C.c_longlong],
+    "phb_batch_mode": [_h, C.c_int],
+    "phb_batch_replica_times": [_h, _pd, _pd],
     "phb_data_read": [C.c_char_p, C.POINTER(_h), C.POINTER(DataInfo)],
     "phb_data_get": [_h, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi],
     "phb_data_destroy": [_h],

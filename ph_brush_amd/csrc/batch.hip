@@ -15,6 +15,19 @@
 // its sweep.  The autotuner times the speculation window (waves per trial),
 // the pending-change budget between refreshes and the graph length on the
 // batch itself and keeps the fastest.
+//
+// Persistent form (every replica on the direct refresh): one cooperative launch
+// per sweep.  Workgroup r < R runs replica r's chain to its sweep target without
+// waiting for the other replicas: whenever its pending budget fills it posts a
+// refresh request (a device-scope release of a per-replica request count) and
+// folds the request's tiles itself, tile by tile, while the other workgroups of
+// the launch take tiles of posted requests as well; a tile is handed out by a
+// compare-and-swap on the replica's tile counter, whose bound moves with the
+// request count, and each finished tile is counted with a release.  The chain
+// resumes once every tile of its request is counted, so its potential grid and
+// accept/reject sequence are those of its serial run.  A chain never waits for
+// a tile no workgroup has taken (it takes them itself), and every wait is
+// bounded by a wall-clock limit that stops the launch with an error status.
 #include <algorithm>
 #include <chrono>
 #include <memory>
@@ -48,7 +61,8 @@ __device__ __forceinline__ int replica_of(const int* __restrict__ start, int R, 
 
 template <int P, typename PoolT, bool LDS_POOLS>
 __global__ __launch_bounds__(MC_THREADS) void k_chain_batch(const McDev* __restrict__ views) {
-    chain_body<P, PoolT, LDS_POOLS>(views[blockIdx.x]);
+    const McDev D = views[blockIdx.x];
+    chain_body<P, PoolT, LDS_POOLS>(D);
 }
 
 template <int P>
@@ -73,6 +87,174 @@ __global__ __launch_bounds__(512) void k_c2r_batch(const RefreshView* __restrict
     const RefreshView& w = v[r];
     fft_xy_c2r_plane<double>(w.kg, w.phi, w.nx, w.ny, w.px, w.py, w.twx, w.twy, 1, w.gate, blockIdx.x - start[r],
                              start[r + 1] - start[r]);
+}
+
+// ---- persistent sweep
+struct PersistView {
+    double* phi;
+    int nt;     // direct-refresh tiles of the replica's grid
+    int prio;   // the replica helpers look at k-th: most trials per sweep first
+};
+
+// layout of the sweep's synchronisation words (zeroed before each launch)
+struct SweepSync {
+    unsigned long long* req;    // [R] refresh requests posted
+    unsigned long long* next;   // [R] tiles handed out (all requests, in order)
+    unsigned long long* done;   // [R] tiles folded
+    unsigned long long* fin;    // replicas finished
+    unsigned long long* abort;  // a wait ran past the limit
+    unsigned long long* tfin;   // [R] wall-clock ticks from the launch's start to the replica's finish
+    unsigned long long* twait;  // [R] ticks the replica's chain spent on its refreshes
+};
+
+// Polling reads are relaxed device-scope atomic loads (coherent across the XCDs'
+// L2s without invalidating them); the acquire is one fence once the awaited
+// value is seen, so that spinning workgroups do not keep emptying the caches
+// the chains work from.
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void acquire_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// Up to `chunk` consecutive tiles of the posted requests of one replica (tiles
+// below `goal`, never past the end of a request): the first tile's index within
+// its request, the count in *cnt; -1 when none is left.
+__device__ __forceinline__ int grab_tiles(unsigned long long* next, unsigned long long goal, int nt, int chunk,
+                                          int* cnt) {
+    unsigned long long t = __hip_atomic_load(next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (t < goal) {
+        const unsigned long long end = (t / (unsigned long long)nt + 1ull) * (unsigned long long)nt;
+        const unsigned long long c = min(min((unsigned long long)chunk, goal - t), end - t);
+        if (__hip_atomic_compare_exchange_strong(next, &t, t + c, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            *cnt = (int)c;
+            return (int)(t % (unsigned long long)nt);
+        }
+    }
+    return -1;
+}
+
+// tiles handed out per grab: a request's tiles go out in about 32 grabs
+__device__ __forceinline__ int tile_chunk(int nt) { return max(1, nt / 32); }
+
+__device__ __forceinline__ bool past_limit(long long t0, long long limit) {
+    return (long long)__builtin_amdgcn_s_memrealtime() - t0 > limit;
+}
+
+// One sweep of every replica (see the file comment).  limit: wall-clock ticks
+// (100 MHz) any wait may last before the launch stops with status 6.
+template <int P, typename PoolT, bool LDS_POOLS>
+__global__ __launch_bounds__(MC_THREADS) void k_sweep_persistent(const McDev* __restrict__ views,
+                                                                 const PersistView* __restrict__ pv, int R,
+                                                                 SweepSync sy, long long limit) {
+    __shared__ int s_cmd[4];
+    const int tid = threadIdx.x;
+    const long long t0 = (long long)__builtin_amdgcn_s_memrealtime();
+    if ((int)blockIdx.x < R) {   // ---- chain of replica r
+        const int r = blockIdx.x;
+        const McDev D = views[r];
+        const int nt = pv[r].nt;
+        unsigned long long goal = 0;
+        bool failed = false;
+        for (;;) {
+            chain_body<P, PoolT, LDS_POOLS>(D);
+            __syncthreads();
+            if (tid == 0) {
+                const McCtl* C = D.ctl;
+                const bool fin = C->trials_done >= C->trials_target || C->status != 0;
+                const bool pend = pending_unfolded(D) > 0;
+                s_cmd[0] = (fin ? 1 : 0) | (pend ? 2 : 0);
+            }
+            __syncthreads();
+            const int st = s_cmd[0];
+            __syncthreads();
+            if (st & 2) {   // post the request, then fold tiles until all of them are counted
+                const long long tw0 = (long long)__builtin_amdgcn_s_memrealtime();
+                goal += (unsigned long long)nt;
+                if (tid == 0) __hip_atomic_fetch_add(sy.req + r, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                for (;;) {
+                    if (tid == 0) {
+                        int cnt = 0;
+                        int cmd = grab_tiles(sy.next + r, goal, nt, tile_chunk(nt), &cnt);   // >= 0: tiles to fold
+                        s_cmd[2] = cnt;
+                        if (cmd < 0) {
+                            if (ld_rlx(sy.done + r) >= goal) { acquire_fence(); cmd = -1; }   // every tile counted
+                            else if (ld_rlx(sy.abort) != 0ull || past_limit(t0, limit)) cmd = -3;
+                            else cmd = -2;   // tiles still being folded elsewhere
+                        }
+                        s_cmd[1] = cmd;
+                    }
+                    __syncthreads();
+                    const int cmd = s_cmd[1], cnt = s_cmd[2];
+                    if (cmd >= 0) {
+                        for (int q = 0; q < cnt; ++q) {
+                            refresh_direct_tile<P>(D, pv[r].phi, nullptr, cmd + q);
+                            __syncthreads();
+                        }
+                        if (tid == 0)
+                            __hip_atomic_fetch_add(sy.done + r, (unsigned long long)cnt, __ATOMIC_RELEASE,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                    } else if (cmd == -2) {
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    __syncthreads();
+                    if (cmd == -1) break;
+                    if (cmd == -3) { failed = true; break; }
+                }
+                if (tid == 0) sy.twait[r] += (unsigned long long)((long long)__builtin_amdgcn_s_memrealtime() - tw0);
+            }
+            if (failed || (st & 1)) break;
+            if (!(st & 2)) { failed = true; break; }   // neither finished nor pending: no progress possible
+        }
+        if (tid == 0) {
+            sy.tfin[r] = (unsigned long long)((long long)__builtin_amdgcn_s_memrealtime() - t0);
+            if (failed) {
+                D.ctl->status = 6;
+                __hip_atomic_store(sy.abort, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_fetch_add(sy.fin, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    // ---- helper: tiles of any replica's posted requests, until every chain has finished;
+    // the replicas with the longest sweeps first, so that the last chain to finish
+    // waits least for its refreshes
+    for (;;) {
+        if (tid == 0) {
+            int rr = -1, tt = -1, ex = 0, cnt = 0;
+            for (int k = 0; k < R && tt < 0; ++k) {
+                const int r = pv[k].prio;
+                const int nt = pv[r].nt;
+                const unsigned long long goal = ld_rlx(sy.req + r) * (unsigned long long)nt;
+                tt = grab_tiles(sy.next + r, goal, nt, tile_chunk(nt), &cnt);
+                if (tt >= 0) rr = r;
+            }
+            if (tt >= 0) acquire_fence();   // the request's pending list and grid, as posted
+            else if (ld_rlx(sy.fin) >= (unsigned long long)R || ld_rlx(sy.abort) != 0ull || past_limit(t0, limit))
+                ex = 1;
+            s_cmd[0] = rr; s_cmd[1] = tt; s_cmd[2] = ex; s_cmd[3] = cnt;
+        }
+        __syncthreads();
+        const int rr = s_cmd[0], tt = s_cmd[1], ex = s_cmd[2], cnt = s_cmd[3];
+        if (ex) return;
+        if (tt >= 0) {
+            for (int q = 0; q < cnt; ++q) {
+                refresh_direct_tile<P>(views[rr], pv[rr].phi, nullptr, tt + q);
+                __syncthreads();
+            }
+            if (tid == 0)
+                __hip_atomic_fetch_add(sy.done + rr, (unsigned long long)cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __builtin_amdgcn_s_sleep(16);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0ull;
 }
 
 // Next sweep: each replica's target moves `add[r]` trials past where it stands.
@@ -121,6 +303,13 @@ struct phb_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long long replays = 0, launches = 0;
+    // persistent form: one cooperative launch per sweep
+    bool persistent = false;
+    DevBuf<PersistView> pv;
+    DevBuf<unsigned long long> syncw;   // 5R + 2 words (SweepSync)
+    int pgrid = 0;                      // workgroups of the launch (R chains, the rest helpers)
+    size_t sweep_lds = 0;               // its dynamic LDS
+    double limit_s = 30.0;
     ~phb_batch_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -200,6 +389,84 @@ void capture(phb_batch_s* B) {
     PHB_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
 }
 
+template <int P>
+const void* sweep_kernel_p(bool lds_pools) {
+    return lds_pools ? reinterpret_cast<const void*>(k_sweep_persistent<P, unsigned short, true>)
+                     : reinterpret_cast<const void*>(k_sweep_persistent<P, int, false>);
+}
+
+const void* sweep_kernel(const phb_batch_s* B) {
+    switch (B->P) {
+        case 3: return sweep_kernel_p<3>(B->lds_pools);
+        case 4: return sweep_kernel_p<4>(B->lds_pools);
+        case 5: return sweep_kernel_p<5>(B->lds_pools);
+        case 6: return sweep_kernel_p<6>(B->lds_pools);
+        case 7: return sweep_kernel_p<7>(B->lds_pools);
+        default: return sweep_kernel_p<8>(B->lds_pools);
+    }
+}
+
+// Empty when the persistent form can run with the current tuning; else the reason.
+const char* persistent_blocker(phb_batch_s* B) {
+    for (phb_mc_s* M : B->mcs)
+        if (!use_direct_refresh(M))
+            return "phb_batch_mode: the persistent sweep needs every replica on the direct refresh (grid x extent "
+                   "at most 64, pending budget at most 16)";
+    const void* f = sweep_kernel(B);
+    const size_t lds = std::max(B->lds_pools ? B->pool_lds : (size_t)0, refresh_direct_lds(B->P));
+    PHB_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, dev = 0, ncu = 0;
+    PHB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, MC_THREADS, lds));
+    PHB_HIP(hipGetDevice(&dev));
+    PHB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (per_cu * ncu < B->R)
+        return "phb_batch_mode: more replicas than workgroups the device holds at once";
+    B->sweep_lds = lds;
+    B->pgrid = per_cu * ncu;
+    return "";
+}
+
+void setup_persistent(phb_batch_s* B) {
+    std::vector<PersistView> v(B->R);
+    std::vector<int> order(B->R);
+    for (int r = 0; r < B->R; ++r) {
+        phb_mc_s* M = B->mcs[r];
+        v[r].phi = M->phi.p;
+        v[r].nt = refresh_direct_blocks(M->pme->ny, M->pme->nz);
+        require(M->applied.n >= (size_t)v[r].nt, "phb_batch_mode: refresh gate smaller than the tile count",
+                PHB_ERR_INTERNAL);
+        order[r] = r;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->mcs[a]->nsites > B->mcs[b]->nsites; });
+    for (int k = 0; k < B->R; ++k) v[k].prio = order[k];
+    sync();
+    B->pv.alloc(B->R);
+    B->pv.upload(v.data(), B->R);
+    B->syncw.alloc(5 * (size_t)B->R + 2);
+    sync();
+}
+
+void launch_persistent(phb_batch_s* B) {
+    const int R = B->R;
+    hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(5 * R + 2, 256)), dim3(256), 0, stream(), B->syncw.p, 5 * R + 2);
+    PHB_HIP(hipGetLastError());
+    SweepSync sy;
+    sy.req = B->syncw.p;
+    sy.next = sy.req + R;
+    sy.done = sy.next + R;
+    sy.fin = sy.done + R;
+    sy.abort = sy.fin + 1;
+    sy.tfin = sy.abort + 1;
+    sy.twait = sy.tfin + R;
+    const McDev* views = B->views.p;
+    const PersistView* pv = B->pv.p;
+    int Rv = R;
+    long long limit = (long long)(B->limit_s * 1e8);   // s_memrealtime runs at 100 MHz
+    void* args[] = {&views, &pv, &Rv, &sy, &limit};
+    PHB_HIP(hipLaunchCooperativeKernel(sweep_kernel(B), dim3(B->pgrid), dim3(MC_THREADS), args,
+                                       (unsigned)B->sweep_lds, stream()));
+}
+
 // reads the per-replica status block; true when every replica finished its sweep
 bool read_status(phb_batch_s* B) {
     hipLaunchKernelGGL(k_batch_status, dim3(ceil_div(B->R, 64)), dim3(64), 0, stream(), B->views.p, B->R, B->status.p);
@@ -226,14 +493,26 @@ long long sweep(phb_batch_s* B) {
     hipLaunchKernelGGL(k_batch_targets, dim3(ceil_div(B->R, 64)), dim3(64), 0, stream(), B->views.p, B->R, B->add.p);
     PHB_HIP(hipGetLastError());
     long long n = 0;
-    for (;;) {
-        if (B->d.use_graph) PHB_HIP(hipGraphLaunch(B->exec, stream()));
-        else enqueue_pairs(B, B->d.graph_pairs);
-        PHB_HIP(hipGetLastError());
-        B->launches += B->d.graph_pairs;
-        ++n;
-        if (read_status(B)) break;
-        require(n < 1000000, "phb_batch_run: a sweep made no progress", PHB_ERR_INTERNAL);
+    if (B->persistent) {   // one launch: every replica to its target, refreshes inside
+        launch_persistent(B);
+        B->launches += 1;
+        n = 1;
+        const bool all = read_status(B);
+        for (int r = 0; r < B->R; ++r)
+            require(read_ctl(B->mcs[r]).status != 6, "phb_batch_run: a persistent sweep ran past its time limit",
+                    PHB_ERR_INTERNAL);
+        require(all, "phb_batch_run: a persistent sweep ended before every replica reached its target",
+                PHB_ERR_INTERNAL);
+    } else {
+        for (;;) {
+            if (B->d.use_graph) PHB_HIP(hipGraphLaunch(B->exec, stream()));
+            else enqueue_pairs(B, B->d.graph_pairs);
+            PHB_HIP(hipGetLastError());
+            B->launches += B->d.graph_pairs;
+            ++n;
+            if (read_status(B)) break;
+            require(n < 1000000, "phb_batch_run: a sweep made no progress", PHB_ERR_INTERNAL);
+        }
     }
     for (int r = 0; r < B->R; ++r) {
         const McCtl c = read_ctl(B->mcs[r]);
@@ -286,7 +565,8 @@ void build_refresh_tables(phb_batch_s* B) {
     sync();
 }
 
-void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long launch_cap) {
+void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long launch_cap, bool persistent) {
+    if (persistent) launch_cap = 0;   // a persistent chain runs to its sweep target
     for (phb_mc_s* M : B->mcs) {
         M->dev.W = W;
         M->d.pending_max = pending_max;
@@ -302,7 +582,16 @@ void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long l
     B->d.spec_window = W;
     B->d.pending_max = pending_max;
     B->d.launch_cap = launch_cap;
-    if (B->d.use_graph) capture(B);
+    B->persistent = false;
+    if (persistent) {
+        const char* why = persistent_blocker(B);
+        require(why[0] == 0, why, PHB_ERR_UNSUPPORTED);
+        setup_persistent(B);
+        B->persistent = true;
+    } else if (B->d.use_graph) {
+        capture(B);
+    }
+    B->d.persistent = B->persistent ? 1 : 0;
     sync();
 }
 
@@ -369,7 +658,12 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
         B->d.pending_max = B->mcs[0]->d.pending_max;
         // every replica's potential grid must hold its accepted changes before the first launch
         for (phb_mc_s* M : B->mcs) refresh(M);
-        if (B->d.use_graph) capture(B.get());
+        const bool want_persistent = B->d.persistent != 0;
+        B->d.persistent = 0;
+        if (want_persistent)
+            apply_tuning(B.get(), B->d.spec_window, B->d.pending_max, B->d.graph_pairs, 0, true);
+        else if (B->d.use_graph)
+            capture(B.get());
         sync();
         *out = B.release();
         return PHB_OK;
@@ -405,6 +699,7 @@ extern "C" int phb_batch_run(phb_batch B, int sweeps, phb_batch_stats* st) {
             st->pending_max = B->d.pending_max;
             st->graph_pairs = B->d.graph_pairs;
             st->launch_cap = B->d.launch_cap;
+            st->persistent = B->persistent ? 1 : 0;
         }
         return PHB_OK;
     });
@@ -417,7 +712,33 @@ extern "C" int phb_batch_tune(phb_batch B, int spec_window, int pending_max, int
         require(pending_max >= 2 && pending_max <= MC_KMAX && pending_max % 2 == 0,
                 "phb_batch_tune: pending budget must be even, in [2, 64]");
         require(graph_pairs >= 1 && launch_cap >= 0, "phb_batch_tune: bad graph length or launch cap");
-        apply_tuning(B, spec_window, pending_max, graph_pairs, launch_cap);
+        apply_tuning(B, spec_window, pending_max, graph_pairs, launch_cap, B->persistent);
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_batch_replica_times(phb_batch B, double* run_ms, double* refresh_ms) {
+    return guard([&] {
+        require(B && run_ms && refresh_ms, "phb_batch_replica_times: bad arguments");
+        require(B->persistent && B->syncw.p != nullptr, "phb_batch_replica_times: no persistent sweep has run",
+                PHB_ERR_STATE);
+        const int R = B->R;
+        std::vector<unsigned long long> w(5 * (size_t)R + 2);
+        sync();
+        B->syncw.download(w.data(), w.size());
+        sync();
+        for (int r = 0; r < R; ++r) {   // 100 MHz ticks
+            run_ms[r] = (double)w[3 * (size_t)R + 2 + r] * 1e-5;
+            refresh_ms[r] = (double)w[4 * (size_t)R + 2 + r] * 1e-5;
+        }
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_batch_mode(phb_batch B, int persistent) {
+    return guard([&] {
+        require(B, "phb_batch_mode: null batch");
+        apply_tuning(B, B->d.spec_window, B->d.pending_max, B->d.graph_pairs, B->d.launch_cap, persistent != 0);
         return PHB_OK;
     });
 }
@@ -427,17 +748,26 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
         require(B && sweeps_per_candidate > 0, "phb_batch_autotune: bad arguments");
         // stage 1: window x budget x graph length without a launch cap; stage 2: the
         // per-launch trial cap (a launch waits for its slowest replica) with the best of stage 1
-        struct Cand { int W, pm, pairs; long long cap; };
+        // (the persistent form, where every replica qualifies: window x budget, one launch per sweep)
+        struct Cand { int W, pm, pairs; long long cap; bool pers; };
         std::vector<Cand> cands;
         for (int W : {4, 8, 16})
             for (int pm : {4, 8, 16})
-                for (int pairs : {2, 4, 8}) cands.push_back({W, pm, pairs, 0});
+                for (int pairs : {2, 4, 8}) cands.push_back({W, pm, pairs, 0, false});
+        const bool pers_ok = B->persistent || [&] {   // probe at the current shape
+            for (phb_mc_s* M : B->mcs)
+                if (!refresh_direct_allowed(M) || M->d.refresh == PHB_REFRESH_KSPACE) return false;
+            return true;
+        }();
+        if (pers_ok)
+            for (int W : {4, 8, 16})
+                for (int pm : {4, 8, 16}) cands.push_back({W, pm, B->d.graph_pairs, 0, true});
         double best_rate = -1.0;
         Cand bc = cands[0];
         const size_t n1 = cands.size();
         for (size_t ci = 0; ci < cands.size(); ++ci) {
             const Cand c = cands[ci];
-            apply_tuning(B, c.W, c.pm, c.pairs, c.cap);
+            apply_tuning(B, c.W, c.pm, c.pairs, c.cap, c.pers);
             read_status(B);
             long long t0 = 0;
             for (int r = 0; r < B->R; ++r) t0 += B->h_status[3 * r + 1];
@@ -450,16 +780,18 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
             for (int r = 0; r < B->R; ++r) t1 += B->h_status[3 * r + 1];
             const double rate = (double)(t1 - t0) / std::max(dt, 1e-9);
             if (rate > best_rate) { best_rate = rate; bc = c; }
-            if (ci + 1 == n1)   // stage 2: the caps, on the best shape of stage 1
-                for (long long cap : {16LL, 32LL, 64LL, 128LL, 256LL}) cands.push_back({bc.W, bc.pm, bc.pairs, cap});
+            if (ci + 1 == n1 && !bc.pers)   // stage 2: the caps, on the best graph shape of stage 1
+                for (long long cap : {16LL, 32LL, 64LL, 128LL, 256LL})
+                    cands.push_back({bc.W, bc.pm, bc.pairs, cap, false});
         }
-        apply_tuning(B, bc.W, bc.pm, bc.pairs, bc.cap);
+        apply_tuning(B, bc.W, bc.pm, bc.pairs, bc.cap, bc.pers);
         if (best) {
             *best = phb_batch_stats{};
             best->spec_window = bc.W;
             best->pending_max = bc.pm;
             best->graph_pairs = bc.pairs;
             best->launch_cap = bc.cap;
+            best->persistent = bc.pers ? 1 : 0;
             best->trials_per_s = best_rate;
         }
         return PHB_OK;

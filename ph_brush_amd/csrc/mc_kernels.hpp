@@ -340,7 +340,10 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
 // start) and written once.  Each workgroup records the pending list's sequence
 // number in its own applied[] entry: no state is shared between workgroups (a
 // device-wide fence or a shared counter would cost more than the convolution).
-constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 2, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
+// A workgroup wider than RD_THREADS (the persistent sweep's) runs a tile with its
+// first RD_THREADS threads; the others only meet the barriers, so the
+// arithmetic is the same as in the RD_THREADS launches.
+constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 4, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
 
 __host__ __device__ constexpr size_t refresh_direct_lds(int P) {
     return (size_t)RD_GROUPS * RD_TZ * (RD_TY + P - 1) * RD_PITCH * sizeof(double);
@@ -368,13 +371,14 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
     const int tid = threadIdx.x, lane = tid & 63;
     const int wg = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wg >> 2, wv = wg & 3;
-    double* A = A_all + grp * ABUF;
+    const bool active = tid < RD_THREADS;   // groups grp >= RD_GROUPS only meet the barriers
+    double* A = A_all + (active ? grp : 0) * ABUF;
     const int nx = D.nx, ny = D.ny, nz = D.nz, fnx = D.fnx, fny = D.fny;
     const int nty = (ny + RD_TY - 1) / RD_TY;
     const int tz = blk / nty, y0 = (blk - tz * nty) * RD_TY, z0 = tz * RD_TZ;
     const bool xin = lane < nx;
     const int fx = xin ? fold(lane, nx) : 0;
-    for (int e = tid; e < np * (3 * P + 4); e += RD_THREADS) {
+    for (int e = tid; active && e < np * (3 * P + 4); e += RD_THREADS) {
         const int p = e / (3 * P + 4), q = e - p * (3 * P + 4), atom = C->pend_atom[p];
         if (q < 3 * P) s_w[p][q] = D.sw[(size_t)atom * 3 * P + q];
         else if (q < 3 * P + 3) s_sb[p][q - 3 * P] = D.sbase[3 * atom + (q - 3 * P)];
@@ -394,7 +398,7 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
     const int rounds = (np + RD_GROUPS - 1) / RD_GROUPS;
     for (int rd = 0; rd < rounds; ++rd) {
         const int p = rd * RD_GROUPS + grp;
-        const bool act = p < np;
+        const bool act = active && p < np;
         const int pp = act ? p : 0;
         const int sx = __builtin_amdgcn_readfirstlane(s_sb[pp][0]);
         const int sy = __builtin_amdgcn_readfirstlane(s_sb[pp][1]);

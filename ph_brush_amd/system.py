@@ -624,15 +624,26 @@ class Surrogate(_Handle):
 
 class Batch(_Handle):
     """Replicas advanced together (phb_batch_*): one workgroup per replica per
-    chain launch, batched refreshes, launch pairs under a hipGraph."""
+    chain launch, batched refreshes, launch pairs under a hipGraph; or one
+    persistent launch per sweep (persistent=True)."""
     _destroy = "phb_batch_destroy"
 
-    def __init__(self, mcs, graph_pairs: int = 4, use_graph: bool = True):
+    def __init__(self, mcs, graph_pairs: int = 4, use_graph: bool = True, persistent: bool = False):
         super().__init__()
         self._keep = list(mcs)
         arr = (C.c_void_p * len(mcs))(*[m.h.value for m in mcs])
-        d = L.BatchDesc(graph_pairs, int(use_graph), 0, 0, 0)
+        d = L.BatchDesc(graph_pairs, int(use_graph), 0, 0, 0, int(persistent))
         L.call("phb_batch_create", arr, len(mcs), C.byref(d), C.byref(self.h))
+
+    def mode(self, persistent: bool):
+        L.call("phb_batch_mode", self.h, int(bool(persistent)))
+
+    def replica_times(self):
+        """Last persistent sweep, per replica: (run ms to its finish, ms of it on refreshes)."""
+        n = len(self._keep)
+        run, ref = np.zeros(n), np.zeros(n)
+        L.call("phb_batch_replica_times", self.h, L.dptr(run), L.dptr(ref))
+        return run, ref
 
     def run(self, sweeps: int) -> dict:
         st = L.BatchStats()

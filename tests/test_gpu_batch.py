@@ -2,8 +2,8 @@
 
 Bar: per replica, the trial sequence (direction, site, ion, accept) and the
 final identities of a batched run -- uneven system sizes and grids, launches
-captured in a hipGraph or issued directly, with or without autotuning in
-between -- equal those of phb_mc_run on a fresh copy with the same seed and the
+captured in a hipGraph or issued directly, one persistent launch per sweep,
+with or without autotuning in between -- equal those of phb_mc_run on a fresh copy with the same seed and the
 same number of trials; the running energy equals a full recompute.
 """
 import numpy as np
@@ -66,6 +66,26 @@ def test_batch_equals_serial(gpu, use_graph):
     _check_against_serial(reps, batched, 4000)
 
 
+def test_persistent_sweep_equals_serial(gpu):
+    """The persistent form: one cooperative launch per sweep, each replica's chain
+    running to its target and its refreshes folded tile by tile inside the
+    launch; a switch to graph replays and back between sweeps leaves every
+    sequence exact as well."""
+    reps = _replicas(6000)
+    batched = _make(reps, 6000)
+    b = S.Batch([r.mc for r in batched], graph_pairs=2, persistent=True)
+    st = b.run(3)
+    assert st["persistent"] == 1 and st["launches"] == 3 and st["replays"] == 3
+    b.mode(False)
+    st = b.run(2)
+    assert st["persistent"] == 0 and st["launches"] > 0
+    b.mode(True)
+    b.tune(16, 4, 2, 0)
+    st = b.run(2)
+    assert st["persistent"] == 1 and st["launches"] == 2 and st["pending_max"] == 4
+    _check_against_serial(reps, batched, 6000)
+
+
 @pytest.mark.parametrize("launch_cap", [7, 40])
 def test_launch_cap_keeps_every_chain_exact(gpu, launch_cap):
     """A per-launch trial cap (launches end early, mid-budget, some with no
@@ -84,6 +104,7 @@ def test_autotune_keeps_every_chain_exact(gpu):
     batched = _make(reps, 20000)
     b = S.Batch([r.mc for r in batched])
     best = b.autotune(1)
+    print("autotune:", best)
     assert best["spec_window"] in (4, 8, 16) and best["pending_max"] in (4, 8, 16)
     assert best["graph_pairs"] in (2, 4, 8) and best["trials_per_s"] > 0
     assert best["launch_cap"] in (0, 16, 32, 64, 128, 256)

@@ -3,7 +3,8 @@ levels, seed 7), 5,000-35,000 atoms each, SPME grids 32-64 x 32-64 x 128-256,
 advanced together by phb_batch_run with the launch shape the autotuner picks on
 this grid (window 16, pending budget 4, 8 launch pairs per graph, launch cap 32).
 A quarter of the replicas refresh their potential grid in k-space, the rest
-directly.
+directly; in the persistent form (one cooperative launch per sweep, which needs
+the direct refresh) every replica refreshes directly.
 
 Bars: every replica's trial sequence (direction, site, ion, decision), final
 identities, counts and running energy equal those of phb_mc_run on a separate
@@ -34,25 +35,28 @@ def grid():
     return g
 
 
-def _make(grid):
+def _make(grid, persistent=False):
     reps = []
     for k, (s, p, mc, _) in enumerate(grid):
-        refresh = L.PHB_REFRESH_KSPACE if k % 4 == 1 else L.PHB_REFRESH_AUTO
+        refresh = L.PHB_REFRESH_KSPACE if k % 4 == 1 and not persistent else L.PHB_REFRESH_AUTO
         reps.append(S.Replica(s, p, mc, seed=1000 + k, pending_max=8, recompute_interval=0, record_cap=REC,
                               refresh=refresh))
     return reps
 
 
-def test_titration_grid_batch_equals_serial_and_oracle(gpu, grid):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_titration_grid_batch_equals_serial_and_oracle(gpu, grid, persistent):
     natoms = [g[0].natoms for g in grid]
     assert min(natoms) >= 5000 and max(natoms) <= 35000
     assert len({g[1].grid for g in grid}) >= 6     # 32 and 64 laterally, 128 and 256 along z
-    batched = _make(grid)
-    b = S.Batch([r.mc for r in batched])
+    batched = _make(grid, persistent)
+    b = S.Batch([r.mc for r in batched], persistent=persistent)
     b.tune(16, 4, 8, 32)
     st = b.run(2)
     assert st["launches"] > 0 and st["replays"] > 0
-    serial = _make(grid)
+    if persistent:
+        assert st["persistent"] == 1 and st["launches"] == 2
+    serial = _make(grid, persistent)
     for k, (rb, rs) in enumerate(zip(batched, serial)):
         sb = rb.mc.stats()
         n = sb["attempted"]
