@@ -298,7 +298,7 @@ void full_recompute(phb_mc_s* M) {
 }
 
 bool refresh_direct_allowed(const phb_mc_s* M) {
-    return M->pme->nx <= RD_NXMAX && refresh_direct_lds(M->pme->order) <= 128 * 1024;
+    return M->pme->nx <= RD_NXMAX && refresh_direct_lds(M->pme->order) <= 64 * 1024;
 }
 
 bool use_direct_refresh(const phb_mc_s* M) {

@@ -343,7 +343,7 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
 // A workgroup wider than RD_THREADS (the persistent sweep's) runs a tile with its
 // first RD_THREADS threads; the others only meet the barriers, so the
 // arithmetic is the same as in the RD_THREADS launches.
-constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 4, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
+constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 2, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
 
 __host__ __device__ constexpr size_t refresh_direct_lds(int P) {
     return (size_t)RD_GROUPS * RD_TZ * (RD_TY + P - 1) * RD_PITCH * sizeof(double);

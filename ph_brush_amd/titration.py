@@ -60,12 +60,14 @@ def titration(cfg=S.config_30k_fp64, ph_values=(6.0, 6.5, 7.0, 7.5, 8.0, 8.5, 9.
         td = time.perf_counter() - td
         del da
         row = {"pH": ph, "alpha_plain": a_plain, "sem_plain": e_plain, "alpha_da": a_da, "sem_da": e_da,
-               "diff": a_da - a_plain, "plain_s": tp, "da_s": td}
+               "diff": a_da - a_plain, "plain_s": tp, "da_s": td, "speedup": tp / td}
         rows.append(row)
         log(f"pH {ph:4.1f}: alpha plain {a_plain:.4f} +- {e_plain:.4f}, DA {a_da:.4f} +- {e_da:.4f}, "
-            f"diff {a_da - a_plain:+.4f}; {blocks * block_trials} trials each, plain {tp:.1f} s, DA {td:.1f} s")
+            f"diff {a_da - a_plain:+.4f}; {blocks * block_trials} trials each, plain {tp:.1f} s, DA {td:.1f} s "
+            f"({tp / td:.2f}x)")
     return {"workload": f"{s.natoms} atoms", "burn": burn, "blocks": blocks, "block_trials": block_trials,
-            "max_abs_diff": max(abs(r["diff"]) for r in rows), "rows": rows}
+            "max_abs_diff": max(abs(r["diff"]) for r in rows), "min_speedup": min(r["speedup"] for r in rows),
+            "rows": rows}
 
 
 def main():
