@@ -511,9 +511,25 @@ typedef struct {
                                             of the listed neighbours of the site and the ion (exact, kept in
                                             fixed point); 0: it does not.  A network is trained with the
                                             physics part of the chain it screens for. */
+    int local_pairs;                     /* 1: local-pair form.  The physics part takes the potential at the site
+                                            and the ion as far_a + q_a s_a + sum over the pool atoms b within
+                                            r_local of q_b (W_a^T G W_b + k_ab), the sum kept exactly in fixed
+                                            point as moves are accepted and far_a (everything else) frozen at the
+                                            state phb_da_freeze saw; the surrogate then needs no potential refresh
+                                            after an accepted move, and a launch holds up to pending_max/2 of them
+                                            (the exact stage takes the pending changes as the plain chain does).
+                                            real_space is ignored.  0: the potential term is W_a . phi of the
+                                            current state, one accepted move per launch. */
+    double r_local;                      /* local-pair radius: 0 = the real-space cutoff rc; rc <= r_local <= r_desc */
 } phb_da_desc;
-/* Builds the r_desc neighbour rows and per-atom histograms of a chain. */
+/* Builds the r_desc neighbour rows and per-atom histograms of a chain (and, in the
+ * local-pair form, the local pair rows and their coefficients). */
 int phb_da_attach(phb_mc mc, const phb_da_desc *d);
+/* Local-pair form: freezes far_a at the current state.  The first run, collection or
+ * prediction after phb_da_attach freezes by itself; later runs keep the frozen field
+ * (a chain is exact for a field fixed before it starts), so call this at the start
+ * of each new chain.  PHB_ERR_STATE without the local-pair form. */
+int phb_da_freeze(phb_mc mc);
 /* Descriptors (n x 48) and pair distances of moves in the chain's current state. */
 int phb_da_descriptors(phb_mc mc, int n, const int *i, const int *j, short *d, double *rij);
 /* Appends (descriptor, s (dE - B)) of n moves, dE exact, to the dataset. */

@@ -121,7 +121,8 @@ class TrainStats(C.Structure):
 
 class DaDesc(C.Structure):
     _fields_ = [("species_of_type", C.POINTER(C.c_byte)), ("ntypes", C.c_int), ("r_desc", C.c_double),
-                ("candidates", C.c_int), ("real_space", C.c_int)]
+                ("candidates", C.c_int), ("real_space", C.c_int), ("local_pairs", C.c_int),
+                ("r_local", C.c_double)]
 
 
 class DaStats(C.Structure):
@@ -268,6 +269,7 @@ SIGNATURES = {
     "phb_surrogate_eval": [_h, C.c_longlong, C.c_longlong, _pd, _pd],
     "phb_da_attach": [_h, C.POINTER(DaDesc)],
     "phb_da_descriptors": [_h, C.c_int, _pi, _pi, _ps, _pd],
+    "phb_da_freeze": [_h],
     "phb_da_collect": [_h, _h, C.c_int, _pi, _pi, _pi],
     "phb_da_predict": [_h, _h, C.c_int, _pi, _pi, _pi, _pd],
     "phb_da_run": [_h, _h, C.c_longlong, C.POINTER(DaStats)],

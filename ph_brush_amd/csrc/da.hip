@@ -538,6 +538,22 @@ struct DaDev {
     int upd_one;                  // the site's and the ion's species sets are disjoint: one update pass
     long long* psi;               // per atom: real-space potential of the listed neighbours (fixed point)
     int use_psi;                  // the physics part includes dq_i psi_i + dq_j psi_j
+    // local-pair form: rows of the pool atoms within r_local of each pool atom
+    // (ascending), their coefficients C_ab in fixed point, and per atom
+    // psil_a = sum_b q_b C_ab in fixed point (rewritten in place by accepted moves)
+    int local;
+    double rl2;                   // r_local^2
+    const int* lhead;
+    const int* latom;
+    const long long* lcoef;
+    long long* psil;
+    const double* q0;             // charges at the freeze
+    double hx, hy, hz;            // mesh spacings
+    double cw;                    // half-width of the lateral column that L also covers
+    const double* gs;             // mesh kernel smoothed by two grid-point stencils, folded as gf
+    double* rqn;                  // per pending entry: the atom's charge after the change (written at launch end)
+    int* fold_from;               // first pending entry a fold still has to add to psil
+    unsigned* fapplied;           // per fold workgroup: the pending list (pend_seq) it has added
 };
 
 // Phase clock of the chain (100 MHz real-time counter), thread 0 only.
@@ -846,6 +862,395 @@ __global__ void k_da_set_target(McCtl* C, long long target) {
     if (threadIdx.x == 0) C->trials_target = target;
 }
 
+// ---------------------------------------------------------------- local-pair form
+// The potential at a pool atom a (mesh and real space, its own mesh self term
+// included) is written as
+//   pot_a = q_a s_a + sum_{b in L(a)} q_b C_ab + sum_{b pool, not in L(a)} q_b G_ab + rest_a
+// with L(a) the pool atoms within r_local of a or in its lateral column (see
+// local_member), C_ab = W_a^T G W_b + k_ab (k_ab the
+// real-space coefficient when a and b are listed neighbours), G_ab = W_a^T G W_b,
+// rest_a the potential of the fixed charges.  The surrogate freezes everything
+// at a state x0 and tracks the change since then in fixed point (2^-32 units,
+// each pair rounded on its own, so that the sums are exact integers that do not
+// depend on the order or time at which changes are added):
+//   T_a(x) = sum_{b in L(a)} c(q_b(x), C_ab) + sum_{b not in L(a)} [c(q_b(x), Gt_ab) - c(q_b(x0), Gt_ab)]
+//   P_a(x) = far_a + q_a(x) s_a + T_a(x),   far_a = pot_a(x0) - q_a(x0) s_a - T_a(x0)
+// with c(q, K) = rint(q K) and Gt_ab the mesh kernel at the pair's displacement
+// (trilinear in the folded table, |offset| per axis: the same bits for (a, b)
+// and (b, a)) in place of W_a^T G W_b beyond r_local.  P_a is a function of the
+// state for x0 fixed before the chain starts, equals pot_a at x0, and differs
+// from it later only by the difference of Gt and W^T G W over the pool atoms
+// that changed beyond r_local.  The move's physics part is
+//   dq_i P_i + dq_j P_j + 0.5 (dq_i^2 s_i + dq_j^2 s_j) + dq_i dq_j K_ij + corr,
+// K_ij = C_ij or Gt_ij (the same coefficient T carries), so the reverse move at y
+// is its negative up to the fixed-point rounding.  The chain adds the L rows of an
+// accepted move to T at once; the far part of its changes is added to every pool
+// atom by k_loc_fold after the launch, and until then the chain adds it for the
+// atoms it evaluates.  No contraction in these sums: the chain, the unchained
+// evaluation and the reverse value form the same ones.
+
+// W_a^T G W_b of two stencils (one thread; the separable correlations times the folded mesh kernel).
+template <int P>
+__device__ double mesh_pair(const McDev& D, int a, int b) {
+    constexpr int CW = 2 * P - 1;
+    const int* sba = D.sbase + 3 * a;
+    const int* sbb = D.sbase + 3 * b;
+    const double* wa = D.sw + (size_t)a * 3 * P;
+    const double* wb = D.sw + (size_t)b * 3 * P;
+    double cx[CW], cy[CW], cz[CW];
+#pragma unroll
+    for (int d = 0; d < CW; ++d) {
+        cx[d] = corr<P>(wa, wb, d);
+        cy[d] = corr<P>(wa + P, wb + P, d);
+        cz[d] = corr<P>(wa + 2 * P, wb + 2 * P, d);
+    }
+    const int x0 = wrap1(sba[0] - sbb[0], D.nx) - (P - 1);
+    const int y0 = wrap1(sba[1] - sbb[1], D.ny) - (P - 1);
+    const int z0 = wrap1(sba[2] - sbb[2], D.nz) - (P - 1);
+    int gx[CW];
+#pragma unroll
+    for (int d = 0; d < CW; ++d) gx[d] = fold(wrap1(x0 + d, D.nx), D.nx);
+    double s = 0.0;
+    for (int dz = 0; dz < CW; ++dz) {
+        const int gz = fold(wrap1(z0 + dz, D.nz), D.nz);
+        double sz = 0.0;
+        for (int dy = 0; dy < CW; ++dy) {
+            const int gy = fold(wrap1(y0 + dy, D.ny), D.ny);
+            const double* row = D.gf + ((size_t)gz * D.fny + gy) * D.fnx;
+            double sx = 0.0;
+#pragma unroll
+            for (int d = 0; d < CW; ++d) sx += cx[d] * row[gx[d]];
+            sz += cy[dy] * sx;
+        }
+        s += cz[dz] * sz;
+    }
+    return s;
+}
+
+// Minimum-image displacement and its square (the arithmetic of the neighbour lists).
+__device__ __forceinline__ double pair_disp(const DaDev& A, double xa, double ya, double za, double xb, double yb,
+                                            double zb, double& dx, double& dy, double& dz) {
+    dx = min_image_exact(xa - xb, A.Lx);
+    dy = min_image_exact(ya - yb, A.Ly);
+    dz = min_image_exact(za - zb, A.Lz);
+    return r2_exact(dx, dy, dz);
+}
+
+__device__ __forceinline__ double pair_r2(const DaDev& A, const McDev& D, int a, int b) {
+    double dx, dy, dz;
+    return pair_disp(A, A.x[a], A.y[a], D.z[a], A.x[b], A.y[b], D.z[b], dx, dy, dz);
+}
+
+__device__ __forceinline__ int fold_index(int k, int n) { return fold(k < n ? k : k - n, n); }
+
+// Gt of a displacement: trilinear at |offset| per axis in the mesh kernel smoothed
+// by the stencils of two atoms on grid points (the kernel itself oscillates at the
+// grid scale: the spline moduli are divided out of it).
+__device__ __forceinline__ long long far_coef(const McDev& D, const DaDev& A, double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    const double ax = fabs(dx) / A.hx, ay = fabs(dy) / A.hy, az = fabs(dz) / A.hz;
+    const int ix = (int)ax, iy = (int)ay, iz = (int)az;
+    const double tx = ax - (double)ix, ty = ay - (double)iy, tz = az - (double)iz;
+    const int x0 = fold_index(ix, D.nx), x1 = fold_index(ix + 1, D.nx);
+    const int y0 = fold_index(iy, D.ny), y1 = fold_index(iy + 1, D.ny);
+    const int z0 = fold_index(iz, D.nz), z1 = fold_index(iz + 1, D.nz);
+    const double* g = A.gs;
+    const size_t fx = (size_t)D.fnx, fxy = (size_t)D.fnx * D.fny;
+    auto at = [&](int z, int y, int x) { return g[(size_t)z * fxy + (size_t)y * fx + x]; };
+    const double c00 = at(z0, y0, x0) + tx * (at(z0, y0, x1) - at(z0, y0, x0));
+    const double c01 = at(z0, y1, x0) + tx * (at(z0, y1, x1) - at(z0, y1, x0));
+    const double c10 = at(z1, y0, x0) + tx * (at(z1, y0, x1) - at(z1, y0, x0));
+    const double c11 = at(z1, y1, x0) + tx * (at(z1, y1, x1) - at(z1, y1, x0));
+    const double c0 = c00 + ty * (c01 - c00), c1 = c10 + ty * (c11 - c10);
+    return psi_k(c0 + tz * (c1 - c0));
+}
+
+// Pool pairs that L carries exactly: within r_local, or in the same lateral
+// column (|dx|, |dy| <= cw, any dz).  The column holds what the interpolated
+// kernel misses: the spline moduli divided out of the mesh kernel amplify its
+// near-Nyquist lateral modes, and those with kz = 0 couple atoms at nearly the
+// same (x, y) at any height.  A function of the displacement, the same for (a, b)
+// and (b, a).
+__device__ __forceinline__ bool local_member(const DaDev& A, double r2, double dx, double dy) {
+    return r2 <= A.rl2 || (fabs(dx) <= A.cw && fabs(dy) <= A.cw);
+}
+
+// The change of T_a from atom p's change qo -> qn when L(a) does not hold p (0
+// when it does, and for p == a).
+__device__ __forceinline__ long long far_delta(const McDev& D, const DaDev& A, int a, double xa, double ya,
+                                               double za, int p, double xp, double yp, double zp, double qo,
+                                               double qn) {
+    if (p == a) return 0;
+    double dx, dy, dz;
+    const double r2 = pair_disp(A, xa, ya, za, xp, yp, zp, dx, dy, dz);
+    if (local_member(A, r2, dx, dy)) return 0;
+    const long long K = far_coef(D, A, dx, dy, dz);
+    return psi_c(qn, K) - psi_c(qo, K);
+}
+
+__device__ __forceinline__ double local_pot(double far, double q, double s, long long T) {
+#pragma clang fp contract(off)
+    return far + q * s + (double)T * PSI_INV;
+}
+
+__device__ __forceinline__ double local_selfq(double dqi, double si, double dqj, double sj) {
+#pragma clang fp contract(off)
+    return 0.5 * (dqi * dqi * si + dqj * dqj * sj);
+}
+
+__device__ __forceinline__ double local_pair(double dqi, double dqj, long long K) {
+#pragma clang fp contract(off)
+    return (dqi * dqj) * ((double)K * PSI_INV);
+}
+
+__device__ __forceinline__ double local_phys(double dqi, double Pi, double dqj, double Pj, double selfq, double pair,
+                                             double corr) {
+#pragma clang fp contract(off)
+    return dqi * Pi + dqj * Pj + selfq + pair + corr;
+}
+
+// Whether L(i) holds j (then K is the row's C_ij), else K = Gt_ij.  Without a row
+// search: the caller finds C_ij (wave_find_coef).
+__device__ __forceinline__ int local_pair_kind(const McDev& D, const DaDev& A, double xi, double yi, double zi,
+                                               double xj, double yj, double zj, long long& K) {
+    double dx, dy, dz;
+    const double r2 = pair_disp(A, xi, yi, zi, xj, yj, zj, dx, dy, dz);
+    if (local_member(A, r2, dx, dy)) { K = 0; return 1; }
+    K = far_coef(D, A, dx, dy, dz);
+    return 0;
+}
+
+// C_ij from i's ascending L row by one wave: 64 samples at a stride, then the
+// segment that must hold j (two dependent loads instead of a binary search's
+// chain).  All lanes of the wave call it with the same i and j.
+__device__ __forceinline__ long long wave_find_coef(const DaDev& A, int i, int j, int lane) {
+    const int h = A.lhead[i], n = A.lhead[i + 1] - h;
+    const int stride = (n + 63) >> 6;
+    const int k = lane * stride;
+    const bool le = k < n && A.latom[h + k] <= j;
+    const unsigned long long m = __ballot(le);
+    const int seg = m ? 63 - __clzll((long long)m) : 0;
+    const int e = seg * stride + lane;
+    const bool hit = lane < stride && e < n && A.latom[h + e] == j;
+    const unsigned long long mh = __ballot(hit);
+    const int src = mh ? __ffsll((long long)mh) - 1 : 0;
+    long long K = 0;
+    if (hit) K = A.lcoef[h + e];
+    return __shfl(K, src, 64);
+}
+
+// K_ij of a move's pair: C_ij from i's L row (ascending: binary search) when L(i)
+// holds j, else Gt_ij.
+__device__ __forceinline__ long long local_pair_coef(const McDev& D, const DaDev& A, int i, int j, double xi,
+                                                     double yi, double zi, double xj, double yj, double zj) {
+    double dx, dy, dz;
+    const double r2 = pair_disp(A, xi, yi, zi, xj, yj, zj, dx, dy, dz);
+    if (local_member(A, r2, dx, dy)) {
+        int l = A.lhead[i], h = A.lhead[i + 1];
+        while (l < h) {
+            const int m = (l + h) >> 1;
+            if (A.latom[m] < j) l = m + 1;
+            else h = m;
+        }
+        if (l < A.lhead[i + 1] && A.latom[l] == j) return A.lcoef[l];
+    }
+    return far_coef(D, A, dx, dy, dz);
+}
+
+// T of the atoms in L(i) and L(j) after a move (charges qio -> qin, qjo -> qjn):
+// integer atomics without return, both rows in one pass, each thread's row
+// entries loaded before any atomic is issued.  The caller fences before anything
+// reads T again; the chain reads T with agent-scope loads, which see the atomics
+// (performed in L2) past the L1.
+__device__ __forceinline__ void local_update(const DaDev& A, int i, double qio, double qin, int j, double qjo,
+                                             double qjn, int tid, int bd) {
+    constexpr int U = 4;
+    const int hi = A.lhead[i], ni = A.lhead[i + 1] - hi, hj = A.lhead[j], nt = ni + (A.lhead[j + 1] - hj);
+    for (int base = 0; base < nt; base += U * bd) {
+        long long K[U];
+        int b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + tid + u * bd;
+            b[u] = -1;
+            K[u] = 0;
+            if (e < nt) {
+                const int x = e < ni ? hi + e : hj + (e - ni);
+                K[u] = A.lcoef[x];
+                b[u] = A.latom[x];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + tid + u * bd;
+            if (b[u] >= 0) {
+                const bool first = e < ni;
+                const long long d = first ? psi_c(qin, K[u]) - psi_c(qio, K[u]) : psi_c(qjn, K[u]) - psi_c(qjo, K[u]);
+                if (d != 0) atomicAdd(reinterpret_cast<unsigned long long*>(A.psil + b[u]), (unsigned long long)d);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ long long load_T(const DaDev& A, int a) {
+    return __hip_atomic_load(A.psil + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// L rows, one wave per pool atom a (list, n atoms): the pool atoms b (ps, np of
+// them in ascending order) that local_member admits, counted (out = cnt[a]) or
+// written in ascending order from lhead[a] (fill).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_loc_rows(McDev D, DaDev A, const int* __restrict__ list, int n,
+                                                  const int* __restrict__ ps, int np, int* __restrict__ cnt,
+                                                  int* __restrict__ latom, int* __restrict__ lown) {
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= n) return;
+    const int a = list[w];
+    const double xa = A.x[a], ya = A.y[a], za = D.z[a];
+    int o = FILL ? A.lhead[a] : 0;
+    for (int k0 = 0; k0 < np; k0 += 64) {
+        const int k = k0 + lane;
+        bool in = false;
+        int b = -1;
+        if (k < np) {
+            b = ps[k];
+            double dx, dy, dz;
+            const double r2 = pair_disp(A, xa, ya, za, A.x[b], A.y[b], D.z[b], dx, dy, dz);
+            in = b != a && local_member(A, r2, dx, dy);
+        }
+        const unsigned long long m = __ballot(in);
+        if (FILL && in) {
+            const int pos = o + __popcll(m & ((1ull << lane) - 1ull));
+            latom[pos] = b;
+            lown[pos] = a;
+        }
+        o += __popcll(m);
+    }
+    if (!FILL && lane == 0) cnt[a] = o;
+}
+
+// C_ab per L entry, formed from the lower-numbered atom's side (the same bits
+// for (a, b) and (b, a)): mesh pair plus the real-space coefficient of the
+// listed pair (a binary search of the ascending full row).
+template <int P>
+__global__ void k_loc_coef(McDev D, long long n, const int* __restrict__ latom, const int* __restrict__ lown,
+                           long long* __restrict__ lcoef) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int a = lown[e], b = latom[e];
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    double c = mesh_pair<P>(D, lo, hi);
+    int l = D.fhead[lo], h = D.fhead[lo + 1];
+    while (l < h) {
+        const int m = (l + h) >> 1;
+        if (D.fnbr[m] < hi) l = m + 1;
+        else h = m;
+    }
+    if (l < D.fhead[lo + 1] && D.fnbr[l] == hi) c += D.kreal[l];
+    lcoef[e] = psi_k(c);
+}
+
+// T of every atom from the current charges: the L rows, plus the far part of the
+// pool atoms whose charge differs from the freeze's (list chg, nchg entries).
+// Exact integers: equal to what the chain and the folds keep.
+__global__ void k_psil_all(McDev D, DaDev A, const int* __restrict__ list, int n, const int* __restrict__ chg,
+                           int nchg) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int a = list[k];
+    long long s = 0;
+    for (int e = A.lhead[a]; e < A.lhead[a + 1]; ++e) s += psi_c(D.q[A.latom[e]], A.lcoef[e]);
+    const double xa = A.x[a], ya = A.y[a], za = D.z[a];
+    for (int c = 0; c < nchg; ++c) {
+        const int p = chg[c];
+        s += far_delta(D, A, a, xa, ya, za, p, A.x[p], A.y[p], D.z[p], A.q0[p], D.q[p]);
+    }
+    A.psil[a] = s;
+}
+
+// far_a = W_a . phi + psi_real_a - q_a s_a - T_a at the freeze (the record's
+// potential slot holds W_a . phi on entry; T_a holds only its L rows then).
+__global__ void k_loc_far(McDev D, DaDev A, const int* __restrict__ list, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int a = list[k];
+    double pr = 0.0;
+    for (int e = D.fhead[a]; e < D.fhead[a + 1]; ++e) pr += D.q[D.fnbr[e]] * D.kreal[e];
+    AtomRec& r = A.rec[a];
+    r.phiw = ((r.phiw + pr) - D.q[a] * D.selfw[a]) - (double)A.psil[a] * PSI_INV;
+}
+
+// Adds the far part of a chain launch's accepted changes (pending entries
+// fold_from .. pending_n, new charges in rqn) to T of every pool atom, four lanes
+// per atom (entries dealt in turn, summed by shuffles).  Gate as the refresh's: a
+// workgroup adds a list only if its own fapplied differs from pend_seq, then
+// records pend_seq (a launch that accepted nothing leaves pend_seq).
+constexpr int FOLD_LANES = 4;
+__global__ __launch_bounds__(256) void k_loc_fold(McDev D, DaDev A, const int* __restrict__ list, int n) {
+    const McCtl* C = D.ctl;
+    const unsigned seq = C->pend_seq;
+    if (A.fapplied[blockIdx.x] == seq) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, k = gt / FOLD_LANES, g = gt - k * FOLD_LANES;
+    const int f0 = *A.fold_from, pn = C->pending_n;
+    long long s = 0;
+    int a = -1;
+    if (k < n) {
+        a = list[k];
+        const double xa = A.x[a], ya = A.y[a], za = D.z[a];
+        for (int e = f0 + g; e < pn; e += FOLD_LANES) {
+            const int p = C->pend_atom[e];
+            const double qn = A.rqn[e];
+            s += far_delta(D, A, a, xa, ya, za, p, A.x[p], A.y[p], D.z[p], qn - C->pend_dq[e], qn);
+        }
+    }
+    s += __shfl_xor(s, 1, FOLD_LANES);
+    s += __shfl_xor(s, 2, FOLD_LANES);
+    if (a >= 0 && g == 0 && s != 0) A.psil[a] += s;
+    __syncthreads();
+    if (threadIdx.x == 0) A.fapplied[blockIdx.x] = seq;
+}
+
+__global__ void k_loc_fold_init(McDev D, DaDev A, int nblk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nblk) A.fapplied[k] = D.ctl->pend_seq;
+}
+
+// 1 in *flag when a pool atom's record holds another charge than the atom (the
+// identities changed outside the delayed-acceptance chain).
+__global__ void k_loc_dirty(McDev D, DaDev A, const int* __restrict__ list, int n, int* __restrict__ flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && A.rec[list[k]].q != D.q[list[k]]) *flag = 1;
+}
+
+// The dynamic part of the pool atoms' records from the current identities.
+__global__ void k_rec_dyn(McDev D, DaDev A, const int* __restrict__ list, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int a = list[k], ty = D.type[a];
+    A.rec[a].q = D.q[a];
+    A.rec[a].type = ty;
+    A.rec[a].sp = A.species[ty];
+}
+
+// Physics part of n moves in the local-pair form (T current, far frozen).
+template <int P>
+__global__ void k_da_base_local(McDev D, DaDev A, int n, const int* __restrict__ mi, const int* __restrict__ mj,
+                                const int* __restrict__ mf, double* __restrict__ base) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const McCtl* C = D.ctl;
+    const int i = mi[m], j = mj[m], fwd = mf[m];
+    const double qi = D.q[i], qj = D.q[j];
+    const double qin = fwd ? 1.0 : 0.0, qjn = fwd ? -1.0 : 0.0;
+    const double dqi = qin - qi, dqj = qjn - qj;
+    const long long K = local_pair_coef(D, A, i, j, A.x[i], A.y[i], D.z[i], A.x[j], A.y[j], D.z[j]);
+    const double Pi = local_pot(A.rec[i].phiw, qi, D.selfw[i], A.psil[i]);
+    const double Pj = local_pot(A.rec[j].phiw, qj, D.selfw[j], A.psil[j]);
+    const double corr = corr_move(D, qi, qin, qj, qjn, D.z[i], D.z[j], C->Q, C->Mz, C->Qz2);
+    base[m] = local_phys(dqi, Pi, dqj, Pj, local_selfq(dqi, D.selfw[i], dqj, D.selfw[j]), local_pair(dqi, dqj, K),
+                         corr);
+}
+
 // ---------------------------------------------------------------- delayed-acceptance chain
 // The accepted move's pool changes, O(1) per pool: the erased entry takes its
 // pool's last entry (the delayed-acceptance chain's own pool order, which the
@@ -923,13 +1328,23 @@ struct CandLds {
     double u1[DA_M], u2[DA_M], rij[DA_M], e1[DA_M], Fx[DA_M];
     double lin[DA_M][2];          // W_a . phi of the site and the ion
     MoveTerms mt[DA_M];
-    long long pi[DA_M], pj[DA_M]; // psi of the site and the ion
+    long long pi[DA_M], pj[DA_M]; // psi (local-pair form: psil) of the site and the ion
     float sf[DA_M];               // s f(d)
+    // local-pair form (pi, pj: T_i, T_j)
+    double far[DA_M][2];          // far_i, far_j
+    double cq[DA_M][2], cs[DA_M][2];             // charges, self terms
+    double cx[DA_M][2], cy[DA_M][2], cz[DA_M][2];  // positions
+    long long kq[DA_M];           // K_ij in fixed point
+    int kin[DA_M];                // L(i) holds j: kq is found by a row search after the physics pass
 };
 
-// The chain of one replica.  Each launch ends after one accepted move, so that
-// the refresh between launches keeps phi exact for the surrogate's potential term.
-template <int P, typename PoolT, bool LDS_POOLS>
+// The chain of one replica.  W . phi form (LOCAL false): each launch ends after
+// one accepted move, so that the refresh between launches keeps phi exact for
+// the surrogate's potential term.  Local-pair form: the surrogate reads only the
+// frozen field, the record and psil, which the chain updates itself, so a launch
+// runs until its pending budget is full; the exact Delta-E takes the pending
+// changes as the plain chain does.
+template <int P, typename PoolT, bool LDS_POOLS, bool LOCAL>
 __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict__ Dp, const DaDev* __restrict__ Ap,
                                                          const NetDev* __restrict__ Np) {
     // the views live in device memory: read field by field instead of holding every field in registers
@@ -966,8 +1381,19 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     McCtl* C = D.ctl;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int M = A.M;
-    // pending is empty for the whole launch; trial_partial never reads this view then
-    const PendT<P>& s_pd = *reinterpret_cast<const PendT<P>*>(s_wsum);
+    // W . phi form: pending is empty for the whole launch and trial_partial never
+    // reads this view; local-pair form: the changes accepted since the last refresh
+    __shared__ PendT<LOCAL ? P : 1> s_pdl;
+    // local-pair form: position and charges of each pending entry (the far part of the
+    // entries this launch adds is not yet in T)
+    constexpr int RK = LOCAL ? MC_KMAX : 1;
+    __shared__ double s_rx[RK], s_ry[RK], s_rz[RK], s_rqo[RK], s_rqn[RK];
+    const PendT<P>* pdp;
+    if constexpr (LOCAL) pdp = &s_pdl;
+    else pdp = reinterpret_cast<const PendT<P>*>(s_wsum);
+    const PendT<P>& s_pd = *pdp;
+    const int pend_max = C->pending_max;
+    int pn0 = 0;   // local-pair form: changes pending (not in phi) at the launch's start
     PoolT* pbuf = LDS_POOLS ? reinterpret_cast<PoolT*>(s_dyn) : reinterpret_cast<PoolT*>(D.pbuf);
     long long target, rec_cap;
     {   // engine state and network into LDS: every load (with the control words the exit
@@ -998,7 +1424,9 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         const unsigned seq_in = C->pend_seq, applied_in = D.applied[0];
         // a launch queued beyond the target (batches are sized ahead) leaves at once
         if (done_in >= target || status_in != 0) return;
-        if (applied_in != seq_in && pend_in > 0) {
+        if constexpr (LOCAL) {
+            pn0 = applied_in == seq_in ? 0 : pend_in;
+        } else if (applied_in != seq_in && pend_in > 0) {
             // phi lags the state: the surrogate's potential term would not be a function of the state
             if (tid == 0) C->status = 3;
             return;
@@ -1030,12 +1458,14 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         }
         if (tid < NN_IN) s_scale[tid] = sc;
     }
+    if constexpr (LOCAL)
+        for (int k = wave; k < pn0; k += DA_WAVES) stage_pending<P>(D, s_pdl, k, C->pend_atom[k], C->pend_dq[k], lane, 64);
     if (tid == 0) {
         s_st.done = C->trials_done;
         s_st.acc_n = C->accepted; s_st.acc_f = C->acc_fwd; s_st.acc_b = C->acc_bwd;
         for (int k = 0; k < 4; ++k) s_st.sz[k] = C->sizes[k];
         s_st.status = C->status; s_st.nprot = C->nprot;
-        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = 0;
+        s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = pn0;
         s_st.E = C->E; s_st.Q = C->Q; s_st.Q2 = C->Q2; s_st.Mz = C->Mz; s_st.Qz2 = C->Qz2;
         s_s1 = C->da_stage1; s_rounds = C->da_rounds;
         for (int k = 0; k < 8; ++k) s_prof[k] = C->da_prof[k];
@@ -1050,7 +1480,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             const long long done = s_st.done;
             const int sz0 = s_st.sz[0], sz1 = s_st.sz[1], sz2 = s_st.sz[2], sz3 = s_st.sz[3];
             int ntr = 0, stop = 0;
-            if (done >= target || s_st.pend_n > 0 || status != 0) stop = 1;
+            if (done >= target || status != 0 || (LOCAL ? s_st.pend_n + 2 > pend_max : s_st.pend_n > 0)) stop = 1;
             else {
                 const long long wf = (long long)sz0 * sz2, wb = (long long)sz1 * sz3;
                 if (wf == 0 && wb == 0) { status = 1; stop = 1; }
@@ -1129,14 +1559,31 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         if (tid < ntr) {
             double r;
             int exA, exB;
-            const AtomRec ri = A.rec[s_c.i[tid]], rj = A.rec[s_c.j[tid]];
-            const long long pi = A.use_psi ? A.psi[s_c.i[tid]] : 0, pj = A.use_psi ? A.psi[s_c.j[tid]] : 0;
-            pair_terms_rec(A, ri, rj, r, exA, exB);
-            s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
-            s_c.pi[tid] = pi; s_c.pj[tid] = pj;
-            s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2, pi, pj);
-            s_c.lin[tid][0] = ri.phiw;
-            s_c.lin[tid][1] = rj.phiw;
+            const int ci = s_c.i[tid], cj = s_c.j[tid];
+            const AtomRec ri = A.rec[ci], rj = A.rec[cj];
+            if constexpr (LOCAL) {
+                const long long pi = load_T(A, ci), pj = load_T(A, cj);
+                pair_terms_rec(A, ri, rj, r, exA, exB);
+                s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
+                s_c.pi[tid] = pi; s_c.pj[tid] = pj;   // the unfolded changes are added below
+                long long K;
+                s_c.kin[tid] = local_pair_kind(D, A, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, K);
+                s_c.kq[tid] = K;
+                s_c.far[tid][0] = ri.phiw; s_c.far[tid][1] = rj.phiw;
+                s_c.cq[tid][0] = ri.q; s_c.cq[tid][1] = rj.q;
+                s_c.cs[tid][0] = ri.selfw; s_c.cs[tid][1] = rj.selfw;
+                s_c.cx[tid][0] = ri.x; s_c.cx[tid][1] = rj.x;
+                s_c.cy[tid][0] = ri.y; s_c.cy[tid][1] = rj.y;
+                s_c.cz[tid][0] = ri.z; s_c.cz[tid][1] = rj.z;
+            } else {
+                const long long pi = A.use_psi ? A.psi[ci] : 0, pj = A.use_psi ? A.psi[cj] : 0;
+                pair_terms_rec(A, ri, rj, r, exA, exB);
+                s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
+                s_c.pi[tid] = pi; s_c.pj[tid] = pj;
+                s_c.mt[tid] = move_terms_rec(D, A, ri, rj, s_c.fwd[tid], r, s_st.Q, s_st.Mz, s_st.Qz2, pi, pj);
+                s_c.lin[tid][0] = ri.phiw;
+                s_c.lin[tid][1] = rj.phiw;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -1152,6 +1599,26 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             }
         }
         for (int k = tid; k < DA_M * 8; k += DA_THREADS) (&s_y[0][0])[k] = 0.f;
+        if constexpr (LOCAL) {
+            // C_ij of the pairs L holds: one wave per such candidate
+            for (int c = wave; c < ntr; c += DA_WAVES)
+                if (s_c.kin[c]) {
+                    const long long K = wave_find_coef(A, s_c.i[c], s_c.j[c], lane);
+                    if (lane == 0) s_c.kq[c] = K;
+                }
+            // T of the site and the ion: the far part of this launch's accepted changes
+            const int nrec = s_st.pend_n - pn0;
+            for (int idx = tid; idx < ntr * 2 * nrec; idx += DA_THREADS) {
+                const int c = idx / (2 * nrec), rem = idx - c * 2 * nrec, side = rem / nrec;
+                const int e = pn0 + (rem - side * nrec);
+                const int a = side ? s_c.j[c] : s_c.i[c];
+                const long long d = far_delta(D, A, a, s_c.cx[c][side], s_c.cy[c][side], s_c.cz[c][side],
+                                              s_pdl.atom[e], s_rx[e], s_ry[e], s_rz[e], s_rqo[e], s_rqn[e]);
+                if (d != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(side ? &s_c.pj[c] : &s_c.pi[c]),
+                              (unsigned long long)d);
+            }
+        }
         __syncthreads();
         clk.lap(prof, s_prof, 2);
         // ---- 3. MLP on MFMA: nrb row blocks x 8 column tiles, tile t on wave t mod DA_WAVES
@@ -1209,9 +1676,23 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 const float f = om + osd * y;
                 const int fwd = s_c.fwd[k];
                 const float sf = fwd ? f : -f;
-                const MoveTerms& m = s_c.mt[k];
-                const double Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr +
-                                  m.psi + (double)sf;
+                double Fx;
+                if constexpr (LOCAL) {
+                    const double qi = s_c.cq[k][0], qj = s_c.cq[k][1];
+                    const double qin = fwd ? 1.0 : 0.0, qjn = fwd ? -1.0 : 0.0;
+                    const double dqi = qin - qi, dqj = qjn - qj;
+                    const double Pi = local_pot(s_c.far[k][0], qi, s_c.cs[k][0], s_c.pi[k]);
+                    const double Pj = local_pot(s_c.far[k][1], qj, s_c.cs[k][1], s_c.pj[k]);
+                    const double corr = corr_move(D, qi, qin, qj, qjn, s_c.cz[k][0], s_c.cz[k][1], s_st.Q, s_st.Mz,
+                                                  s_st.Qz2);
+                    Fx = local_phys(dqi, Pi, dqj, Pj, local_selfq(dqi, s_c.cs[k][0], dqj, s_c.cs[k][1]),
+                                    local_pair(dqi, dqj, s_c.kq[k]), corr) +
+                         (double)sf;
+                } else {
+                    const MoveTerms& m = s_c.mt[k];
+                    Fx = m.dqi * s_c.lin[k][0] + m.dqj * s_c.lin[k][1] + m.selfq + m.perfc + m.perf + m.corr + m.psi +
+                         (double)sf;
+                }
                 const long long Sa = fwd ? (a - 1) * (c - 1) + (b + 1) * (d + 1) : (a + 1) * (c + 1) + (b - 1) * (d - 1);
                 const double e1 = mh_exponent(D.beta, Fx, fwd ? 1 : -1, D.mu, Sb, Sa);
                 acc1 = e1 >= 0.0 || exp(e1) > s_c.u1[k];
@@ -1265,7 +1746,8 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             __syncthreads();
             {
                 double win = 0.0;
-                const double part = wave_sum(trial_partial<P>(D, s_tr[0], s_pd, 0, tid, DA_THREADS, &win, &s_kij));
+                const double part = wave_sum(
+                    trial_partial<P>(D, s_tr[0], s_pd, LOCAL ? s_st.pend_n : 0, tid, DA_THREADS, &win, &s_kij));
                 win = wave_sum(win);
                 if (lane == 0) { s_wsum[wave] = part; s_wwin[wave] = win; }
             }
@@ -1288,7 +1770,25 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 const MoveTerms& m = s_c.mt[first];
                 const double lin = m.dqi * s_c.lin[first][0] + m.dqj * s_c.lin[first][1];
                 double Fy;
-                if (A.use_psi) {
+                if constexpr (LOCAL) {
+                    // the chain's own stage-1 value of the reverse move at y: psil of i and j at y
+                    // (the pair's change when j is in L(i)), the records' charges at y, the
+                    // moments as the update below forms them
+                    const long long K = s_c.kq[first];
+                    const long long piy = s_c.pi[first] + (psi_c(t.qj_new, K) - psi_c(t.qj_old, K));
+                    const long long pjy = s_c.pj[first] + (psi_c(t.qi_new, K) - psi_c(t.qi_old, K));
+                    const double dqi = t.qi_new - t.qi_old, dqj = t.qj_new - t.qj_old;
+                    const double Qy = Q + (dqi + dqj);
+                    const double Mzy = Mz + (dqi * t.zi + dqj * t.zj);
+                    const double Qz2y = Qz2 + (dqi * t.zi * t.zi + dqj * t.zj * t.zj);
+                    const double rdqi = t.qi_old - t.qi_new, rdqj = t.qj_old - t.qj_new;
+                    const double Pi = local_pot(s_c.far[first][0], t.qi_new, t.self[0], piy);
+                    const double Pj = local_pot(s_c.far[first][1], t.qj_new, t.self[1], pjy);
+                    const double corr = corr_move(D, t.qi_new, t.qi_old, t.qj_new, t.qj_old, t.zi, t.zj, Qy, Mzy, Qz2y);
+                    Fy = local_phys(rdqi, Pi, rdqj, Pj, local_selfq(rdqi, t.self[0], rdqj, t.self[1]),
+                                    local_pair(rdqi, rdqj, K), corr) +
+                         (double)(-s_c.sf[first]);
+                } else if (A.use_psi) {
                     // its real-space term from psi at y, formed in fixed point as the table will hold it;
                     // the pair term dq_i dq_j k_ij is even in the move
                     const long long K = psi_k(s_kij);
@@ -1342,7 +1842,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     if (fwd) { --s_st.sz[0]; ++s_st.sz[1]; --s_st.sz[2]; ++s_st.sz[3]; }
                     else { ++s_st.sz[0]; --s_st.sz[1]; ++s_st.sz[2]; --s_st.sz[3]; }
                     s_dc.acc_k = 0;
-                    s_st.pend_n = 2;      // ends the launch: the refresh folds the move into phi
+                    if (!LOCAL) s_st.pend_n = 2;   // ends the launch: the refresh folds the move into phi
                 }
             }
             __syncthreads();
@@ -1350,7 +1850,29 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             // ---- 6. accepted: pools and the histograms of the atoms around i and j
             if (accepted) {
                 da_pool_swap<PoolT>(D, pbuf, s_dc, tid);
-                if (A.use_psi) psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
+                if constexpr (LOCAL) {
+                    // the move joins the pending list; the records and psil of the atoms
+                    // within r_local of i, then of j (a row has no repeated atom)
+                    const TrialT<P>& t = s_tr[0];
+                    const int pn = s_st.pend_n;
+                    if (wave == 1) stage_pending<P>(D, s_pdl, pn, t.i, t.qi_new - t.qi_old, lane, 64);
+                    if (wave == 2) stage_pending<P>(D, s_pdl, pn + 1, t.j, t.qj_new - t.qj_old, lane, 64);
+                    if (tid == 64 || tid == 65) {
+                        const int side = tid - 64, a = side ? t.j : t.i, e = pn + side;
+                        s_rx[e] = A.x[a]; s_ry[e] = A.y[a]; s_rz[e] = side ? t.zj : t.zi;
+                        s_rqo[e] = side ? t.qj_old : t.qi_old;
+                        s_rqn[e] = side ? t.qj_new : t.qi_new;
+                    }
+                    if (tid == 0) {
+                        A.rec[t.i].q = t.qi_new; A.rec[t.i].type = t.ti_new; A.rec[t.i].sp = A.species[t.ti_new];
+                    } else if (tid == 1) {
+                        A.rec[t.j].q = t.qj_new; A.rec[t.j].type = t.tj_new; A.rec[t.j].sp = A.species[t.tj_new];
+                    }
+                    local_update(A, t.i, t.qi_old, t.qi_new, t.j, t.qj_old, t.qj_new, tid, DA_THREADS);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the atomics complete before the barrier
+                } else if (A.use_psi) {
+                    psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
+                }
             }
             if (accepted && A.upd_one) {
                 const TrialT<P>& t = s_tr[0];
@@ -1372,6 +1894,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 }
             }
             if (accepted) __syncthreads();   // the pool swap (LDS pools are copied out below)
+            if (LOCAL && accepted && tid == 0) s_st.pend_n += 2;
             clk.lap(prof, s_prof, 6);
             if (accepted) break;
         }
@@ -1392,7 +1915,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             C->status = s_st.status; C->nprot = s_st.nprot;
             C->E = s_st.E; C->Q = s_st.Q; C->Q2 = s_st.Q2; C->Mz = s_st.Mz; C->Qz2 = s_st.Qz2;
             C->da_stage1 = s_s1; C->da_rounds = s_rounds;
-            if (s_st.pend_n > 0) {
+            if (!LOCAL && s_st.pend_n > 0) {
                 const TrialT<P>& t = s_tr[0];
                 C->pend_atom[0] = t.i; C->pend_dq[0] = t.qi_new - t.qi_old;
                 C->pend_atom[1] = t.j; C->pend_dq[1] = t.qj_new - t.qj_old;
@@ -1400,6 +1923,16 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 C->pend_seq = C->pend_seq + 1u;   // a new list: no workgroup has folded it
             }
         }
+    }
+    if constexpr (LOCAL) {
+        const int pend_n = s_st.pend_n;
+        if (tid == 0) {
+            if (pend_n > pn0) C->pend_seq = C->pend_seq + 1u;   // a new list: no workgroup has folded it
+            C->pending_n = pend_n;
+            *A.fold_from = pn0;
+        }
+        for (int k = tid; k < pend_n; k += DA_THREADS) { C->pend_atom[k] = s_pdl.atom[k]; C->pend_dq[k] = s_pdl.dq[k]; }
+        for (int k = pn0 + tid; k < pend_n; k += DA_THREADS) A.rqn[k] = s_rqn[k];
     }
     if (prof) {
         clk.lap(prof, s_prof, 7);
@@ -1473,6 +2006,14 @@ struct DaState {
     DevBuf<long long> psi;     // per atom: real-space potential of the listed neighbours (fixed point)
     DevBuf<int> cand;          // the pool atoms, in stencil-origin order
     int ncand = 0;
+    // local-pair form: L rows, coefficients, psil; far frozen in the records' potential slot
+    DevBuf<int> lhead, latom;
+    DevBuf<long long> lcoef, psil;
+    DevBuf<double> q0, rqn, gs;
+    DevBuf<int> fold_from, flag;
+    DevBuf<unsigned> fapplied;
+    long long nlocal = 0;
+    bool frozen = false;
     DaDev dev{};
     long long nentries = 0;
     long long launches = 0, trials_seen = 0;   // batch sizing of phb_da_run
@@ -1509,6 +2050,215 @@ void phi_dot(phb_mc_s* M, DaState* S) {
     PHB_HIP(hipGetLastError());
 }
 
+// psil of every atom from the current charges (exact integers: equal to what the chain keeps).
+void local_psil(phb_mc_s* M, DaState* S, const int* chg, int nchg) {
+    if (S->ncand > 0)
+        hipLaunchKernelGGL(k_psil_all, dim3(ceil_div(S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->dev,
+                           S->cand.p, S->ncand, chg, nchg);
+    PHB_HIP(hipGetLastError());
+}
+
+// The records' charge, type and species from the current identities (the far field stays).
+void local_rec_dyn(phb_mc_s* M, DaState* S) {
+    if (S->ncand > 0)
+        hipLaunchKernelGGL(k_rec_dyn, dim3(ceil_div(S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->dev,
+                           S->cand.p, S->ncand);
+    PHB_HIP(hipGetLastError());
+}
+
+int fold_blocks(const DaState* S) { return std::max(ceil_div(S->ncand * FOLD_LANES, 256), 1); }
+
+// Every fold workgroup takes the current pending list as added (T is current).
+void local_fold_init(phb_mc_s* M, DaState* S) {
+    const int nb = fold_blocks(S);
+    hipLaunchKernelGGL(k_loc_fold_init, dim3(ceil_div(nb, 256)), dim3(256), 0, stream(), M->dev, S->dev, nb);
+    PHB_HIP(hipGetLastError());
+}
+
+// After a chain launch: the far part of its accepted changes into T of every pool atom.
+void local_fold(phb_mc_s* M, DaState* S) {
+    if (S->ncand > 0)
+        hipLaunchKernelGGL(k_loc_fold, dim3(fold_blocks(S)), dim3(256), 0, stream(), M->dev, S->dev, S->cand.p,
+                           S->ncand);
+    PHB_HIP(hipGetLastError());
+}
+
+// far_a at the current state: phi with every accepted change, W . phi into the
+// records, the charges kept as the freeze's, T (its L rows), then
+// far = W . phi + psi_real - q s - T.
+void local_freeze(phb_mc_s* M, DaState* S) {
+    const int N = M->atoms->n;
+    refresh(M);
+    phi_dot(M, S);
+    if (N > 0) PHB_HIP(hipMemcpyAsync(S->q0.p, M->atoms->q.p, sizeof(double) * N, hipMemcpyDeviceToDevice, stream()));
+    local_psil(M, S, nullptr, 0);
+    if (S->ncand > 0)
+        hipLaunchKernelGGL(k_loc_far, dim3(ceil_div(S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->dev,
+                           S->cand.p, S->ncand);
+    PHB_HIP(hipGetLastError());
+    local_fold_init(M, S);
+    S->frozen = true;
+}
+
+// Run start of the local-pair form: freeze once; later, when the identities
+// changed outside the chain (plain runs, uploads), the records' dynamic part and
+// T from the charges (the far part over the pool atoms changed since the freeze).
+void local_start(phb_mc_s* M, DaState* S) {
+    if (!S->frozen) {
+        local_freeze(M, S);
+        return;
+    }
+    refresh(M);
+    int dirty = 0;
+    if (S->ncand > 0) {
+        PHB_HIP(hipMemsetAsync(S->flag.p, 0, sizeof(int), stream()));
+        hipLaunchKernelGGL(k_loc_dirty, dim3(ceil_div(S->ncand, 256)), dim3(256), 0, stream(), M->dev, S->dev,
+                           S->cand.p, S->ncand, S->flag.p);
+        PHB_HIP(hipGetLastError());
+        S->flag.download(&dirty, 1);
+        sync();
+    }
+    if (dirty) {
+        const int N = M->atoms->n;
+        std::vector<double> q(N), q0(N);
+        std::vector<int> pool(S->ncand), chg;
+        M->atoms->q.download(q.data(), N);
+        S->q0.download(q0.data(), N);
+        S->cand.download(pool.data(), S->ncand);
+        sync();
+        for (int a : pool)
+            if (q[a] != q0[a]) chg.push_back(a);
+        DevBuf<int> dchg(std::max<size_t>(chg.size(), 1));
+        if (!chg.empty()) dchg.upload(chg.data(), chg.size());
+        local_rec_dyn(M, S);
+        local_psil(M, S, dchg.p, (int)chg.size());
+        sync();   // dchg is freed on return
+    }
+    local_fold_init(M, S);
+}
+
+template <int P>
+void build_local_p(phb_mc_s* M, DaState* S, const int* lown) {
+    hipLaunchKernelGGL(k_loc_coef<P>, dim3((unsigned)((S->nlocal + 255) / 256)), dim3(256), 0, stream(), M->dev,
+                       S->nlocal, S->latom.p, lown, S->lcoef.p);
+}
+
+// Cardinal B-spline M_n at x (recursion over the order).
+double cardinal_bspline(int n, double x) {
+    if (n == 1) return x >= 0.0 && x < 1.0 ? 1.0 : 0.0;
+    return (x * cardinal_bspline(n - 1, x) + (n - x) * cardinal_bspline(n - 1, x - 1.0)) / (n - 1);
+}
+
+// The mesh kernel smoothed by the stencils of two atoms on grid points,
+// sum_{u,v} w(u) w(v) g(r + u - v): along each axis a convolution with the
+// discrete autocorrelation of the stencil samples M_P(1..P-1), whose DFT is the
+// squared spline modulus that g has divided out, so the table is the continuum
+// reciprocal-space pair potential sampled at the grid offsets.  Folded to |offset|
+// per axis like gf (the result is even).
+void build_smoothed_kernel(phb_mc_s* M, DaState* S) {
+    const int nx = M->dev.nx, ny = M->dev.ny, nz = M->dev.nz, P = M->dev.P;
+    const size_t G = (size_t)nx * ny * nz;
+    std::vector<double> g(G), t(G);
+    M->gk.download(g.data(), G);
+    sync();
+    std::vector<double> sm(P + 1, 0.0), a(2 * P - 1, 0.0);
+    for (int k = 1; k < P; ++k) sm[k] = cardinal_bspline(P, (double)k);
+    for (int d = -(P - 1); d <= P - 1; ++d)
+        for (int k = 1; k < P; ++k) {
+            const int k2 = k + d;
+            if (k2 >= 1 && k2 < P) a[d + P - 1] += sm[k] * sm[k2];
+        }
+    auto pass = [&](std::vector<double>& in, std::vector<double>& out, int axis) {
+        const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
+        const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)nx : (size_t)nx * ny;
+        for (size_t k = 0; k < G; ++k) {
+            const int c = (int)((k / stride) % n);
+            double v = 0.0;
+            for (int d = -(P - 1); d <= P - 1; ++d) {
+                int cc = c + d;
+                cc += cc < 0 ? n : 0;
+                cc -= cc >= n ? n : 0;
+                v += a[d + P - 1] * in[k + ((long long)cc - c) * (long long)stride];
+            }
+            out[k] = v;
+        }
+    };
+    pass(g, t, 0);
+    pass(t, g, 1);
+    pass(g, t, 2);
+    const int fx = nx / 2 + 1, fy = ny / 2 + 1, fz = nz / 2 + 1;
+    std::vector<double> f((size_t)fx * fy * fz);
+    for (int z = 0; z < fz; ++z)
+        for (int y = 0; y < fy; ++y)
+            for (int x = 0; x < fx; ++x) f[((size_t)z * fy + y) * fx + x] = t[((size_t)z * ny + y) * nx + x];
+    S->gs.alloc(f.size());
+    S->gs.upload(f.data(), f.size());
+    S->dev.gs = S->gs.p;
+    sync();
+}
+
+// L rows: the pool atoms of each pool atom's r_desc row within r_local, and their C_ab.
+void build_local(phb_mc_s* M, DaState* S) {
+    build_smoothed_kernel(M, S);
+    const int N = M->atoms->n, np = S->ncand;
+    DaDev& A = S->dev;
+    A.hx = M->box.Lx / M->dev.nx; A.hy = M->box.Ly / M->dev.ny; A.hz = M->box.Lz / M->dev.nz;
+    A.cw = 5.0 * std::max(A.hx, A.hy);
+    std::vector<int> ps(np);
+    S->cand.download(ps.data(), np);
+    sync();
+    std::sort(ps.begin(), ps.end());
+    DevBuf<int> dps(std::max(np, 1));
+    if (np > 0) dps.upload(ps.data(), np);
+    DevBuf<int> cnt(std::max(N, 1));
+    PHB_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * std::max(N, 1), stream()));
+    const unsigned rows_grid = (unsigned)ceil_div(np, 4);   // four waves per workgroup
+    if (np > 0)
+        hipLaunchKernelGGL(k_loc_rows<false>, dim3(rows_grid), dim3(256), 0, stream(), M->dev, A, S->cand.p, np, dps.p,
+                           np, cnt.p, nullptr, nullptr);
+    PHB_HIP(hipGetLastError());
+    std::vector<int> c(N), head(N + 1, 0);
+    cnt.download(c.data(), N);
+    sync();
+    long long tot = 0;
+    for (int a = 0; a < N; ++a) {
+        head[a] = (int)tot;
+        tot += c[a];
+    }
+    require(tot < (1LL << 31), "phb_da_attach: too many local pairs");
+    head[N] = (int)tot;
+    S->nlocal = tot;
+    S->lhead.alloc(N + 1);
+    S->lhead.upload(head.data(), N + 1);
+    S->latom.alloc(std::max<long long>(tot, 1));
+    S->lcoef.alloc(std::max<long long>(tot, 1));
+    S->psil.alloc(std::max(N, 1));
+    S->q0.alloc(std::max(N, 1));
+    S->rqn.alloc(MC_KMAX);
+    S->fold_from.alloc(1);
+    S->flag.alloc(1);
+    S->fapplied.alloc(fold_blocks(S));
+    A.lhead = S->lhead.p; A.latom = S->latom.p; A.lcoef = S->lcoef.p; A.psil = S->psil.p;
+    A.q0 = S->q0.p; A.rqn = S->rqn.p; A.fold_from = S->fold_from.p; A.fapplied = S->fapplied.p;
+    PHB_HIP(hipMemsetAsync(S->fold_from.p, 0, sizeof(int), stream()));
+    DevBuf<int> lown(std::max<long long>(tot, 1));
+    if (np > 0)
+        hipLaunchKernelGGL(k_loc_rows<true>, dim3(rows_grid), dim3(256), 0, stream(), M->dev, A, S->cand.p, np, dps.p,
+                           np, nullptr, S->latom.p, lown.p);
+    if (tot > 0) {
+        switch (M->dev.P) {
+            case 3: build_local_p<3>(M, S, lown.p); break;
+            case 4: build_local_p<4>(M, S, lown.p); break;
+            case 5: build_local_p<5>(M, S, lown.p); break;
+            case 6: build_local_p<6>(M, S, lown.p); break;
+            case 7: build_local_p<7>(M, S, lown.p); break;
+            default: build_local_p<8>(M, S, lown.p); break;
+        }
+    }
+    PHB_HIP(hipGetLastError());
+    sync();   // dps, cnt and lown are freed on return
+}
+
 // Uploads the views the chain kernel reads by pointer (before a run; the stream orders it).
 void upload_views(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
     S->dev.prof = M->profiling ? 1 : 0;
@@ -1522,11 +2272,15 @@ void upload_views(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
 template <int P>
 void launch_da_p(phb_mc_s* M, DaState* S) {
     (void)M;
-    // Pools stay in HBM: a launch covers a single accepted move, so copying them
+    // Pools stay in HBM: a launch covers few accepted moves, so copying them
     // into LDS and back each launch costs more than an in-LDS shift saves (the
     // shift is one merged pass at known positions).
-    hipLaunchKernelGGL((k_da_chain<P, int, false>), dim3(1), dim3(DA_THREADS), 0, stream(), S->dmc.p, S->dda.p,
-                       S->dnet.p);
+    if (S->dev.local)
+        hipLaunchKernelGGL((k_da_chain<P, int, false, true>), dim3(1), dim3(DA_THREADS), 0, stream(), S->dmc.p,
+                           S->dda.p, S->dnet.p);
+    else
+        hipLaunchKernelGGL((k_da_chain<P, int, false, false>), dim3(1), dim3(DA_THREADS), 0, stream(), S->dmc.p,
+                           S->dda.p, S->dnet.p);
 }
 
 void launch_da(phb_mc_s* M, DaState* S) {
@@ -1549,6 +2303,23 @@ void check_moves(phb_mc_s* M, int n, const int* i, const int* j, const int* f) {
 
 // physics part B of n moves (device buffers); phi must hold every accepted change
 void base_terms(phb_mc_s* M, DaState* S, int n, const int* di, const int* dj, const int* df, double* out) {
+    if (S->dev.local) {
+        local_start(M, S);   // freezes on first use
+        auto gl = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(ceil_div(n, 128)), dim3(128), 0, stream(), M->dev, S->dev, n, di, dj, df,
+                               out);
+        };
+        switch (M->dev.P) {
+            case 3: gl(k_da_base_local<3>); break;
+            case 4: gl(k_da_base_local<4>); break;
+            case 5: gl(k_da_base_local<5>); break;
+            case 6: gl(k_da_base_local<6>); break;
+            case 7: gl(k_da_base_local<7>); break;
+            default: gl(k_da_base_local<8>); break;
+        }
+        PHB_HIP(hipGetLastError());
+        return;
+    }
     refresh(M);   // folds pending accepted changes into phi (no-op when none are pending)
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(ceil_div(n, 128)), dim3(128), 0, stream(), M->dev, S->dev, n, di, dj, df, out);
@@ -1826,6 +2597,9 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         require(M->params->d.coulombtype == PHB_COUL_PME,
                 "phb_da_attach: the surrogate's physics part needs coulombtype=pme", PHB_ERR_UNSUPPORTED);
         require(d->r_desc > 0.0, "phb_da_attach: r_desc must be > 0");
+        const double rl = d->r_local > 0.0 ? d->r_local : d->r_desc;
+        require(!d->local_pairs || (rl >= M->params->d.rc && rl <= d->r_desc),
+                "phb_da_attach: r_local must lie in [rc, r_desc]");
         const int cand = d->candidates > 0 ? d->candidates : DA_M;
         require(cand >= 1 && cand <= DA_M, "phb_da_attach: candidates must be in [1, 64]");
         const int N = M->atoms->n;
@@ -1902,9 +2676,23 @@ extern "C" int phb_da_attach(phb_mc M, const phb_da_desc* d) {
         PHB_HIP(hipGetLastError());
         rebuild_hist(M, S.get());
         sync();
+        A.local = d->local_pairs ? 1 : 0;
+        A.rl2 = rl * rl;
+        if (A.local) build_local(M, S.get());
         if (M->da_free) M->da_free(M->da);
         M->da = S.release();
         M->da_free = da_free;
+        return PHB_OK;
+    });
+}
+
+extern "C" int phb_da_freeze(phb_mc M) {
+    return guard([&] {
+        require(M != nullptr, "phb_da_freeze: null argument");
+        DaState* S = da_of(M);
+        require(S->dev.local != 0, "phb_da_freeze: the chain is not in the local-pair form", PHB_ERR_STATE);
+        local_freeze(M, S);
+        sync();
         return PHB_OK;
     });
 }
@@ -1978,11 +2766,15 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
     return guard([&] {
         require(M && s && ntrials >= 0, "phb_da_run: bad arguments");
         DaState* S = da_of(M);
+        const bool local = S->dev.local != 0;
         rebuild_hist(M, S);
         upload_views(M, S, s);
-        refresh(M);   // the surrogate needs phi to hold every accepted change
-        phi_dot(M, S);
-        if (S->dev.use_psi) {   // psi from the current charges (identities may have changed since the last run)
+        if (local) local_start(M, S);   // far frozen once; records and psil to the current identities
+        else {
+            refresh(M);   // the surrogate needs phi to hold every accepted change
+            phi_dot(M, S);
+        }
+        if (!local && S->dev.use_psi) {   // psi from the current charges (identities may have changed since the last run)
             hipLaunchKernelGGL(k_psi_all, dim3(ceil_div(M->atoms->n, 256)), dim3(256), 0, stream(), M->dev,
                                S->psi.p);
             PHB_HIP(hipGetLastError());
@@ -1996,7 +2788,8 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
             if (M->next_recompute > 0) chunk_end = std::min(chunk_end, M->next_recompute);
             hipLaunchKernelGGL(k_da_set_target, dim3(1), dim3(64), 0, stream(), M->ctl.p, chunk_end);
             while (done < chunk_end && status == 0) {
-                const int batch = launch_batch(c, chunk_end - done, 2);   // a launch ends at its accepted move
+                // a launch ends at its accepted move (local-pair form: when its pending budget is full)
+                const int batch = launch_batch(c, chunk_end - done, local ? M->d.pending_max : 2);
                 const long long before = done;
                 for (int b = 0; b < batch; ++b) {
                     const size_t e0 = M->ev_used;
@@ -2004,9 +2797,10 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
                     launch_da(M, S);
                     ++M->launches;
                     ++S->launches;
+                    if (local) local_fold(M, S);
                     if (M->profiling) M->event();
                     refresh(M);
-                    phi_dot(M, S);
+                    if (!local) phi_dot(M, S);
                     if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
                 }
                 PHB_HIP(hipGetLastError());
@@ -2021,7 +2815,7 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
             }
             if (done >= chunk_end && M->next_recompute > 0 && chunk_end == M->next_recompute) {
                 full_recompute(M);
-                phi_dot(M, S);
+                if (!local) phi_dot(M, S);
                 M->next_recompute += M->d.recompute_interval;
             }
         }

@@ -54,6 +54,7 @@ class CollectSpec:
     seed: int = 1
     r_desc: float = 6.0
     real_space: bool = False       # physics part with the neighbours' real-space potential (da_attach)
+    local_pairs: bool = False      # local-pair physics part (da_attach); the far field frozen after equilibration
 
 
 def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=None) -> dict:
@@ -65,8 +66,10 @@ def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=N
     for k, ph in enumerate(spec.ph_values):
         mc = S.McParams.at_ph(ph, spec.pka)
         rep = S.Replica(s, p, mc, seed=spec.seed + 1000 * k)
-        rep.mc.da_attach(r_desc=spec.r_desc, real_space=spec.real_space)
+        rep.mc.da_attach(r_desc=spec.r_desc, real_space=spec.real_space, local_pairs=spec.local_pairs)
         rep.mc.run(spec.equilibrate)
+        if spec.local_pairs:
+            rep.mc.da_freeze()   # the states sampled below see this field, as a chain sees its start's
         got = 0
         while got < spec.samples_per_ph:
             rep.mc.run(spec.between)

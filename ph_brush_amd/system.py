@@ -512,12 +512,21 @@ class MonteCarlo(_Handle):
 
 
     # ---- delayed acceptance (no reference counterpart)
-    def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64, real_space: bool = False):
+    def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64, real_space: bool = False,
+                  local_pairs: bool = False, r_local: float = 0.0):
+        """local_pairs: the local-pair form of the surrogate's physics part (the
+        potential of the pool atoms within r_local kept exactly, the rest frozen
+        by da_freeze; several accepted moves per launch)."""
         sp = np.ascontiguousarray(default_species(NTYPES) if species is None else species, np.int8)
         self._species = sp
         d = L.DaDesc(sp.ctypes.data_as(C.POINTER(C.c_byte)), len(sp), float(r_desc), int(candidates),
-                     int(bool(real_space)))
+                     int(bool(real_space)), int(bool(local_pairs)), float(r_local))
         L.call("phb_da_attach", self.h, C.byref(d))
+
+    def da_freeze(self):
+        """Local-pair form: freezes the far field at the current state (call at
+        the start of each new chain; the first use freezes by itself)."""
+        L.call("phb_da_freeze", self.h)
 
     def da_descriptors(self, i, j):
         i = np.ascontiguousarray(i, np.int32); j = np.ascontiguousarray(j, np.int32)

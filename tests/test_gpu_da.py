@@ -128,9 +128,10 @@ def test_descriptors_match_brute_force_before_and_after_chain(gpu):
             assert st["accepted"] > 10
 
 
-def _trained_surrogate(s, p, n_per_ph=8192, ph=(6.5, 7.5, 8.5), epochs=30):
+def _trained_surrogate(s, p, n_per_ph=8192, ph=(6.5, 7.5, 8.5), epochs=30, local_pairs=False):
     sur = S.Surrogate(n_per_ph * len(ph))
-    spec = SG.CollectSpec(ph_values=ph, samples_per_ph=n_per_ph, per_state=1024, equilibrate=2000, between=500)
+    spec = SG.CollectSpec(ph_values=ph, samples_per_ph=n_per_ph, per_state=1024, equilibrate=2000, between=500,
+                          local_pairs=local_pairs)
     SG.collect(s, p, sur, spec)
     st = SG.train_holdout(sur, epochs=epochs, batch=1024, lr=2e-3)
     return sur, st
@@ -147,13 +148,16 @@ def _compare_da(rg, ro, n):
     return n
 
 
-def test_da_chain_replays_on_oracle(gpu):
+@pytest.mark.parametrize("local_pairs", [False, True])
+def test_da_chain_replays_on_oracle(gpu, local_pairs):
+    """local_pairs: several accepted moves per launch, the exact Delta-E taking
+    the pending changes (pending budget 8: up to four moves per launch)."""
     s, p = _tiny()
-    sur, st = _trained_surrogate(s, p, n_per_ph=4096, epochs=20)
+    sur, st = _trained_surrogate(s, p, n_per_ph=4096, epochs=20, local_pairs=local_pairs)
     mc = S.McParams.at_ph(7.9)
     n = 3000
     r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=250, record_cap=n)
-    r.mc.da_attach(r_desc=6.0)
+    r.mc.da_attach(r_desc=6.0, local_pairs=local_pairs)
     res = r.mc.da_run(sur, n)
     assert res["attempted"] == n and res["status"] == 0
     rg = r.mc.da_record(n)
@@ -176,8 +180,10 @@ def test_da_chain_replays_on_oracle(gpu):
     assert abs(e_run - r.mc.recompute()) <= 1e-9 * max(1.0, abs(e_run))
 
 
-@pytest.mark.parametrize("refresh,real_space", [("auto", False), ("kspace", False), ("auto", True)])
-def test_da_stage1_values_are_functions_of_the_state(gpu, refresh, real_space):
+@pytest.mark.parametrize("refresh,real_space,local_pairs", [("auto", False, False), ("kspace", False, False),
+                                                           ("auto", True, False), ("auto", False, True),
+                                                           ("kspace", False, True)])
+def test_da_stage1_values_are_functions_of_the_state(gpu, refresh, real_space, local_pairs):
     """With a network that outputs zero, the stage-1 value dE*(x->y) the chain
     records is the physics part alone; at the state the trial saw (rebuilt from
     the record, recomputed from scratch) it equals the unchained evaluation
@@ -188,13 +194,16 @@ def test_da_stage1_values_are_functions_of_the_state(gpu, refresh, real_space):
     direct refresh with the partial stencil sums from the refresh launch;
     "kspace" the k-space refresh and a separate stencil pass.  With real_space the
     physics part also holds the neighbours' real-space potential (kept by the chain
-    in fixed point), and the reverse value is formed from it at y."""
+    in fixed point), and the reverse value is formed from it at y.  In the
+    local-pair form the chain keeps psil itself across the accepted moves of a
+    launch; the unchained evaluation rebuilds it from the charges, with the far
+    field frozen at the run's start."""
     s, p = _tiny()
     mc = S.McParams.at_ph(7.9)
     n = 1500
     r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=10 ** 9, record_cap=n,
                   refresh=L.PHB_REFRESH_AUTO if refresh == "auto" else L.PHB_REFRESH_KSPACE)
-    r.mc.da_attach(r_desc=6.0, real_space=real_space)
+    r.mc.da_attach(r_desc=6.0, real_space=real_space, local_pairs=local_pairs)
     sur = S.Surrogate(0)
     theta = np.zeros_like(sur.get_params())
     theta[-1] = 1.0          # output std (> 0); every weight zero, so the network outputs exactly 0
@@ -231,6 +240,83 @@ def test_da_stage1_values_are_functions_of_the_state(gpu, refresh, real_space):
         rev = at_state(q, t, np.array([i]), np.array([j]), np.array([1 - f]))
         assert abs(rev[0] - rg["dE_star_rev"][a]) <= tol(rev[0]), a
         start = a + 1
+
+
+def _move_identity(mc, fw):
+    site = (1.0, mc.nh_p_type - 1) if fw else (0.0, mc.nh_n_type - 1)
+    ion = (-1.0, mc.cl_type - 1) if fw else (0.0, mc.w_type - 1)
+    return site, ion
+
+
+def _exact_moves(o, s, p, mc, q, t, i, j, f, r_near):
+    """Exact Delta-E of each move from oracle energies, its pair term dq_i dq_j
+    (W_i^T G W_j + k_ij) and whether the ion lies within r_near of the site.  The
+    pair term is the cross difference E(i and j changed) - E(i) - E(j) + E(neither)
+    of the real-space and reciprocal components alone (the net-charge and slab
+    terms are quadratic in the moments and have cross terms of their own)."""
+    def energy(qq, tt):
+        o.set_charges(qq, tt)
+        e, c = o.total_energy()
+        return np.array([e, c[1] + c[2]])
+
+    e00 = energy(q, t)
+    dE, pair, near = np.zeros(len(i)), np.zeros(len(i)), np.zeros(len(i), bool)
+    for k in range(len(i)):
+        a, b, fw = int(i[k]), int(j[k]), int(f[k])
+        (qa, ta), (qb, tb) = _move_identity(mc, fw)
+        q1, t1 = q.copy(), t.copy(); q1[a], t1[a] = qa, ta
+        q2, t2 = q.copy(), t.copy(); q2[b], t2[b] = qb, tb
+        q3, t3 = q1.copy(), t1.copy(); q3[b], t3[b] = qb, tb
+        e10, e01, e11 = energy(q1, t1), energy(q2, t2), energy(q3, t3)
+        dE[k], pair[k] = e11[0] - e00[0], e11[1] - e10[1] - e01[1] + e00[1]
+        dx = np.array([s.x[a] - s.x[b], s.y[a] - s.y[b], s.z[a] - s.z[b]])
+        dx[0] -= s.Lx * np.rint(dx[0] / s.Lx)
+        dx[1] -= s.Ly * np.rint(dx[1] / s.Ly)
+        near[k] = np.sqrt((dx ** 2).sum()) <= r_near
+    return dE, pair, near
+
+
+def test_local_pairs_physics_matches_exact_delta_e(gpu):
+    """Local-pair form, network output zero.  At the state the field was frozen
+    at, the physics part equals the oracle's exact Delta-E (to the 2^-32 fixed
+    point per pair) when the ion lies within r_local (= r_desc) of the site; beyond
+    it only the pair coefficient is the interpolated mesh kernel, within 0.02 kT of
+    the exact pair term.  After 3,000 plain trials (T rebuilt from the charges,
+    the changes beyond r_local through the interpolated kernel) the physics part
+    still follows the exact Delta-E, within 0.05 kT, where the frozen field alone
+    is off by several kT on this brush."""
+    s, p = _tiny()
+    mc = S.McParams.at_ph(7.9)
+    r_desc = 6.0
+    r = S.Replica(s, p, mc, seed=5, pending_max=8, recompute_interval=10 ** 9)
+    r.mc.da_attach(r_desc=r_desc, local_pairs=True)
+    r.mc.run(400)
+    r.mc.da_freeze()
+    sur = S.Surrogate(0)
+    theta = np.zeros_like(sur.get_params())
+    theta[-1] = 1.0
+    sur.set_params(theta)
+    q, t = r.atoms.identity()
+    rng = np.random.default_rng(11)
+    i, j, f = SG.sample_moves(rng, t, s.z, mc, 160)
+    pred = r.mc.da_predict(sur, i, j, f)
+    o = O.Oracle(s, p, r.bonds_host)
+    o.nlist()
+    dE, pair, near = _exact_moves(o, s, p, mc, q, t, i, j, f, r_desc)
+    assert near.sum() >= 10 and (~near).sum() >= 10, (near.sum(), len(near))
+    err = np.abs(pred - dE)
+    tol = 1e-7 * np.maximum(1.0, np.abs(dE))
+    assert np.all(err[near] <= tol[near]), (err[near].max(), np.argmax(err[near]))
+    assert np.all(err[~near] <= 0.02), err[~near].max()
+    assert np.max(err[~near]) > 1e-9   # the interpolated kernel is not the exact pair term
+    # a chain run away from the frozen state
+    r.mc.run(3000)
+    q2, t2 = r.atoms.identity()
+    assert (t2 != t).sum() >= 20
+    i2, j2, f2 = SG.sample_moves(rng, t2, s.z, mc, 60)
+    p2 = r.mc.da_predict(sur, i2, j2, f2)
+    dE2, _, _ = _exact_moves(o, s, p, mc, q2, t2, i2, j2, f2, r_desc)
+    assert np.max(np.abs(p2 - dE2)) <= 0.05, np.max(np.abs(p2 - dE2))
 
 
 def _alpha_series(run, n_chunks, chunk, nsites):
