@@ -8,16 +8,20 @@ counter-particle per charged/titratable bead (44,000 atoms realised); SPME
 64x64x256, order 5, alpha 0.9, rc 3.5, WCA LJ; FP64 exact Delta-E (full
 recompute of energy and potential every 20,000 trials).
 
-Two forms run in the same process, each timed over K steps of
+The forms run in the same process, each timed over K steps of
 --trials-per-step GC trials after W warm-up steps:
   plain: every trial evaluates the exact Delta-E;
-  da:    delayed acceptance -- a BF16-MFMA MLP screens each trial and only
-         stage-1 acceptances get the exact Delta-E; every trial is an
-         exact-equivalent trial (the second stage keeps the distribution).
-The surrogate is trained first (untimed) on (descriptor, exact Delta-E) pairs
-collected on the GPU from exact chains at pH 6-9.  `value` is the DA form's
-exact-equivalent trial moves/s; the plain form, the ratio and alpha of both
-forms are reported beside it.  Inputs are resident on the GPU before timing.
+  da:    delayed acceptance, local-pair surrogate -- a physics part (frozen
+         field plus exactly tracked pair sums) and a BF16-MFMA MLP screen each
+         trial and only stage-1 acceptances get the exact Delta-E; the second
+         stage keeps the exact chain's distribution, so every trial is an
+         exact-equivalent trial;
+  da_wphi: the same with the W . phi surrogate (one accepted move per launch).
+Each surrogate is trained first (untimed, reported) on (descriptor, exact
+Delta-E) pairs collected on the GPU from exact chains at pH 6-9.  `value` is
+the local-pair DA form's trial moves/s; the plain form, both ratios (per trial
+and per accepted move), effective sample sizes per second and alpha of every
+form are reported beside it.  Inputs are resident on the GPU before timing.
 --gpus above 1 is refused: one replica's chain runs on one GPU (replicas would
 shard across GPUs with no communication).
 """
@@ -424,10 +428,13 @@ def main():
                     help="collect the surrogate's data on the 44k workload, train it, report the training rate and exit")
     ap.add_argument("--setup-only", action="store_true",
                     help="time the set-up path (neighbour list, MC object) on the 44k workload and exit")
-    ap.add_argument("--no-da-real-space", action="store_true",
-                    help="skip the delayed-acceptance form whose surrogate includes the real-space potential")
+    ap.add_argument("--da-real-space", action="store_true",
+                    help="also run the W . phi form whose surrogate includes the real-space potential")
+    ap.add_argument("--no-da-wphi", action="store_true", help="skip the W . phi form of the surrogate")
+    ap.add_argument("--da-pending-max", type=int, default=8,
+                    help="pending budget of the local-pair delayed-acceptance chain (accepted moves per launch x 2)")
     ap.add_argument("--fp32-trials", type=int, default=100000)
-    ap.add_argument("--ess-trials", type=int, default=2000000,
+    ap.add_argument("--ess-trials", type=int, default=6000000,
                     help="recorded trials per form for the autocorrelation times of N_prot and E (0: skip)")
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
     ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
@@ -508,86 +515,105 @@ def main():
         ess["plain"] = ess_of_chain(st_e["nprot"], st_e["E"], r["fwd"], r["acc"], r["dE"], plain_value, "plain")
         del rep, r
 
-    # ---- surrogate: data from exact chains at pH 6-9, GPU training (untimed)
-    t_sur = time.perf_counter()
-    spec = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000, between=2000)
-    sur = S.Surrogate(len(spec.ph_values) * spec.samples_per_ph)
-    col = SG.collect(s, p, sur, spec, log=lambda m: print(m, file=sys.stderr, flush=True))
-    tr = SG.train_holdout(sur, epochs=args.da_epochs, batch=4096, lr=2e-3)
-    t_sur = time.perf_counter() - t_sur
-    print(f"[bench] surrogate: {col['samples']} samples, holdout rmse {tr['rmse_holdout']:.3f} "
-          f"(target std {tr['target_std']:.3f}), {t_sur:.1f}s", file=sys.stderr, flush=True)
+    # ---- surrogates: data from exact chains at pH 6-9, GPU training (untimed); one network per
+    # physics part (it sets the residual the network learns)
     flop_fwd = 2 * (48 * 128 + 128 * 128 + 128)
     flop_train = 3 * flop_fwd
-    train_tflops = tr["steps"] * 4096 * flop_train / (tr["ms"] / 1e3) / 1e12
 
-    # ---- delayed-acceptance form, same workload, start state and seed
-    rep = S.Replica(s_eq, p, mc, seed=1, **kw)
-    rep.mc.da_attach(r_desc=spec.r_desc)
-    d0 = rep.mc.da_run(sur, 0)
-    dt_da, alpha_da = _timed(lambda n: rep.mc.da_run(sur, n), K, W, T, "da")
-    d1 = rep.mc.da_run(sur, 0)
-    da_trials = K * T
-    value = da_trials / dt_da
-    n_att = max(d1["attempted"] - d0["attempted"], 1)
-    stage1 = (d1["stage1_accepted"] - d0["stage1_accepted"]) / n_att
-    da_acc = (d1["accepted"] - d0["accepted"]) / n_att
-    rounds = d1["rounds"] - d0["rounds"]
-    # live DA chain launch time
-    rep.mc.set_profiling(True)
-    p0 = rep.mc.da_run(sur, 0)
-    p1 = rep.mc.da_run(sur, T)
-    rep.mc.set_profiling(False)
-    da_launch_s = p1["chain_ms"] / 1e3 / max(p1["launches"] - p0["launches"], 1)
-    da_trials_per_launch = (p1["attempted"] - p0["attempted"]) / max(p1["launches"] - p0["launches"], 1)
-    # algorithmic bytes per DA trial: two 96-byte histograms and the stage-1 share of an exact trial
-    da_bpt = 2 * 48 * 2 + stage1 * bpt
+    def train_surrogate(local_pairs, real_space=False):
+        t0 = time.perf_counter()
+        spec = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000,
+                              between=2000, local_pairs=local_pairs, real_space=real_space)
+        sur = S.Surrogate(len(spec.ph_values) * spec.samples_per_ph)
+        col = SG.collect(s, p, sur, spec, log=lambda m: print(m, file=sys.stderr, flush=True))
+        tr = SG.train_holdout(sur, epochs=args.da_epochs, batch=4096, lr=2e-3)
+        print(f"[bench] surrogate (local pairs {local_pairs}, real space {real_space}): {col['samples']} samples, "
+              f"holdout rmse {tr['rmse_holdout']:.3f} (target std {tr['target_std']:.3f}), "
+              f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        return spec, sur, col, tr
+
+    def da_form(spec, sur, label, **attach):
+        """Times the delayed-acceptance chain (same workload, start state and seed as the plain
+        form), then one profiled step for the live launch time, then an ESS run."""
+        kw_da = dict(kw)
+        kw_da["pending_max"] = args.da_pending_max if attach.get("local_pairs") else args.pending_max
+        rep = S.Replica(s_eq, p, mc, seed=1, **kw_da)
+        t_att = time.perf_counter()
+        rep.mc.da_attach(r_desc=spec.r_desc, **attach)
+        t_att = time.perf_counter() - t_att
+        d0 = rep.mc.da_run(sur, 0)
+        dt, alpha = _timed(lambda n: rep.mc.da_run(sur, n), K, W, T, label)
+        d1 = rep.mc.da_run(sur, 0)
+        n_att = max(d1["attempted"] - d0["attempted"], 1)
+        v = K * T / dt
+        st1 = (d1["stage1_accepted"] - d0["stage1_accepted"]) / n_att
+        acc = (d1["accepted"] - d0["accepted"]) / n_att
+        rep.mc.set_profiling(True)
+        p0 = rep.mc.da_run(sur, 0)
+        p1 = rep.mc.da_run(sur, T)
+        rep.mc.set_profiling(False)
+        nl = max(p1["launches"] - p0["launches"], 1)
+        launch_s = (p1["chain_ms"] - p0["chain_ms"]) / 1e3 / nl
+        tpl = (p1["attempted"] - p0["attempted"]) / nl
+        acc_p = max(p1["accepted"] - p0["accepted"], 1)
+        phases = ["prologue", "proposals", "descriptors_physics", "mlp", "stage1", "exact_dE", "stage2_updates",
+                  "epilogue"]
+        per_acc = {k: (p1["phase_us"][i] - p0["phase_us"][i]) / acc_p for i, k in enumerate(phases)}
+        del rep
+        form = {"value": v, "ms_per_step": dt / K * 1e3, "acceptance": acc, "stage1_acceptance": st1,
+                "stage2_acceptance": acc / max(st1, 1e-12), "alpha": sum(alpha) / len(alpha),
+                "over_plain": v / plain_value, "accepted_per_s": v * acc,
+                "over_plain_accepted": v * acc / max(plain_value * plain_acc, 1e-12),
+                "trials_per_round": n_att / max(d1["rounds"] - d0["rounds"], 1), "trials_per_launch": tpl,
+                "accepted_per_launch": acc_p / nl, "launch_us": launch_s * 1e6,
+                "refresh_us_per_accepted": (p1["refresh_ms"] - p0["refresh_ms"]) * 1e3 / acc_p,
+                "chain_us_per_accepted_by_phase": per_acc, "attach_s": t_att,
+                "pending_max": kw_da["pending_max"]}
+        if args.ess_trials > 0:
+            rep = S.Replica(s_eq, p, mc, seed=3, record_cap=args.ess_trials, **kw_da)
+            rep.mc.da_attach(r_desc=spec.r_desc, **attach)
+            st_e = rep.mc.stats()
+            rep.mc.da_run(sur, args.ess_trials)
+            r = rep.mc.da_record(args.ess_trials)
+            e = ess_of_chain(st_e["nprot"], st_e["E"], r["fwd"], r["stage"] == 1, r["dE"], v, label)
+            form["ess"] = e
+            form["ess_over_plain"] = {k: e[k]["ess_per_s"] / ess["plain"][k]["ess_per_s"] for k in ("nprot", "E")}
+            del rep, r
+        return form
+
+    spec, sur, col, tr = train_surrogate(local_pairs=True)
+    train_tflops = tr["steps"] * 4096 * flop_train / (tr["ms"] / 1e3) / 1e12
+    da = da_form(spec, sur, "da", local_pairs=True)
+    value, dt_da = da["value"], da["ms_per_step"] * K / 1e3
+    stage1 = da["stage1_acceptance"]
+    # algorithmic bytes per delayed-acceptance trial: two 96-byte histogram rows, two 64-byte records,
+    # two 8-byte T values, and the stage-1 share of an exact trial
+    da_bpt = 2 * 48 * 2 + 2 * 64 + 2 * 8 + stage1 * bpt
+    da_launch_s = da["launch_us"] / 1e6
+    da_trials_per_launch = da["trials_per_launch"]
     da_achieved = da_bpt * da_trials_per_launch / da_launch_s / 1e9
     infer_tflops = flop_fwd * da_trials_per_launch / da_launch_s / 1e12
-    del rep
-    if args.ess_trials > 0:
-        rep = S.Replica(s_eq, p, mc, seed=3, record_cap=args.ess_trials, **kw)
-        rep.mc.da_attach(r_desc=spec.r_desc)
-        st_e = rep.mc.stats()
-        rep.mc.da_run(sur, args.ess_trials)
-        r = rep.mc.da_record(args.ess_trials)
-        ess["da"] = ess_of_chain(st_e["nprot"], st_e["E"], r["fwd"], r["stage"] == 1, r["dE"], value, "da")
-        for k in ("nprot", "E"):
-            ess[f"da_over_plain_{k}"] = ess["da"][k]["ess_per_s"] / ess["plain"][k]["ess_per_s"]
-        del rep, r
     # surrogate cost against the workload's 10^6 moves: time to train, trials after which DA has paid it back
     train_total_s = col["seconds"] + tr["ms"] / 1e3
     gain_per_trial_s = 1.0 / plain_value - 1.0 / value
     break_even = train_total_s / gain_per_trial_s if gain_per_trial_s > 0 else float("inf")
+    ess["da"] = da.get("ess")
+    if ess.get("da"):
+        for k in ("nprot", "E"):
+            ess[f"da_over_plain_{k}"] = da["ess_over_plain"][k]
 
-    # ---- the same form with the neighbours' real-space potential in the surrogate's physics part
-    # (its own network: the physics part sets the residual the network learns)
+    # ---- the W . phi form of the surrogate (one accepted move per launch), its own network
+    da_wphi = None
+    if not args.no_da_wphi:
+        spec_w, sur_w, col_w, tr_w = train_surrogate(local_pairs=False)
+        da_wphi = da_form(spec_w, sur_w, "da W.phi")
+        da_wphi.update({"rmse_holdout_kT": tr_w["rmse_holdout"], "target_std_kT": tr_w["target_std"],
+                        "collect_s": col_w["seconds"], "train_s": tr_w["ms"] / 1e3})
     da_rs = None
-    if not args.no_da_real_space:
-        spec_rs = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000,
-                                 between=2000, real_space=True)
-        sur_rs = S.Surrogate(len(spec_rs.ph_values) * spec_rs.samples_per_ph)
-        SG.collect(s, p, sur_rs, spec_rs)
-        tr_rs = SG.train_holdout(sur_rs, epochs=args.da_epochs, batch=4096, lr=2e-3)
-        print(f"[bench] real-space surrogate: holdout rmse {tr_rs['rmse_holdout']:.3f} "
-              f"(target std {tr_rs['target_std']:.3f})", file=sys.stderr, flush=True)
-        rep = S.Replica(s_eq, p, mc, seed=1, **kw)
-        rep.mc.da_attach(r_desc=spec_rs.r_desc, real_space=True)
-        e0 = rep.mc.da_run(sur_rs, 0)
-        dt_rs, alpha_rs = _timed(lambda n: rep.mc.da_run(sur_rs, n), K, W, T, "da real-space")
-        e1 = rep.mc.da_run(sur_rs, 0)
-        n_rs = max(e1["attempted"] - e0["attempted"], 1)
-        v_rs = K * T / dt_rs
-        acc_rs = (e1["accepted"] - e0["accepted"]) / n_rs
-        s1_rs = (e1["stage1_accepted"] - e0["stage1_accepted"]) / n_rs
-        da_rs = {"value": v_rs, "ms_per_step": dt_rs / K * 1e3, "acceptance": acc_rs, "stage1_acceptance": s1_rs,
-                 "stage2_acceptance": acc_rs / max(s1_rs, 1e-12),
-                 "trials_per_launch": n_rs / max(e1["launches"] - e0["launches"], 1),
-                 "alpha": sum(alpha_rs) / len(alpha_rs), "over_plain": v_rs / plain_value,
-                 "accepted_per_s": v_rs * acc_rs,
-                 "over_plain_accepted": v_rs * acc_rs / max(plain_value * plain_acc, 1e-12),
-                 "rmse_holdout_kT": tr_rs["rmse_holdout"], "target_std_kT": tr_rs["target_std"]}
-        del rep
+    if args.da_real_space:
+        spec_r, sur_r, _, tr_r = train_surrogate(local_pairs=False, real_space=True)
+        da_rs = da_form(spec_r, sur_r, "da W.phi real-space", real_space=True)
+        da_rs.update({"rmse_holdout_kT": tr_r["rmse_holdout"], "target_std_kT": tr_r["target_std"]})
 
     out = {
         "metric": "MC trial moves/s at 30k particles; DA-surrogate vs plain at equal alpha(pH)",
@@ -605,26 +631,26 @@ def main():
         "config": {
             "workload": "diblock brush 400x50 on 20x20 lattice, 40x40x150 box x3 slab, 2000 salt pairs, "
                         "SPME 64x64x256 order 5, fixed-mu GC protonation + ion-swap moves, pH 7.0 pKa 7.5",
-            "form": "delayed acceptance (BF16-MFMA MLP stage 1, exact FP64 stage 2)",
+            "form": "delayed acceptance, local-pair surrogate (BF16-MFMA MLP stage 1, exact FP64 stage 2)",
             "natoms": s.natoms,
             "trials_per_step": T,
             "equil_trials": args.equil_trials,
             "pending_max": args.pending_max,
+            "da_pending_max": args.da_pending_max,
             "spec_window": args.spec_window,
             "parallelism": "1 chain / 1 GPU",
         },
         "forms": {
             "plain": {"value": plain_value, "ms_per_step": dt_plain / K * 1e3, "acceptance": plain_acc,
                       "alpha": sum(alpha_plain) / len(alpha_plain)},
-            "da": {"value": value, "ms_per_step": dt_da / K * 1e3, "acceptance": da_acc,
-                   "stage1_acceptance": stage1, "stage2_acceptance": da_acc / max(stage1, 1e-12),
-                   "trials_per_round": n_att / max(rounds, 1), "alpha": sum(alpha_da) / len(alpha_da)},
+            "da": da,
             "da_over_plain": value / plain_value,
-            # accepted moves per second: a delayed-acceptance chain accepts fewer of its trials than the
-            # plain Metropolis chain (stage 1 times stage 2), so per accepted move the gain is smaller
+            # accepted moves per second: a delayed-acceptance chain accepts a smaller share of its trials
+            # than the plain Metropolis chain (stage 1 times stage 2), so per accepted move the gain differs
             "plain_accepted_per_s": plain_value * plain_acc,
-            "da_accepted_per_s": value * da_acc,
-            "da_over_plain_accepted": (value * da_acc) / max(plain_value * plain_acc, 1e-12),
+            "da_accepted_per_s": da["accepted_per_s"],
+            "da_over_plain_accepted": da["over_plain_accepted"],
+            "da_wphi": da_wphi,
             "da_real_space": da_rs,
             "alpha_note": "alpha of the timed blocks (same start state and seed); equality at fixed pH is "
                           "tested over long runs in tests/test_gpu_da.py",
@@ -646,7 +672,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": da_achieved / HBM_PEAK_GBS,
-            "traffic": _traffic("k_da_chain"),
+            "traffic": _traffic("k_da_chain_local"),
             "traffic_unit": "HBM bytes per launch (FETCH_SIZE and WRITE_SIZE counter passes, copy-calibrated)",
             "avg_launch_us": da_launch_s * 1e6,
             "bytes_per_trial": da_bpt,

@@ -1009,17 +1009,6 @@ __device__ __forceinline__ double local_phys(double dqi, double Pi, double dqj, 
     return dqi * Pi + dqj * Pj + selfq + pair + corr;
 }
 
-// Whether L(i) holds j (then K is the row's C_ij), else K = Gt_ij.  Without a row
-// search: the caller finds C_ij (wave_find_coef).
-__device__ __forceinline__ int local_pair_kind(const McDev& D, const DaDev& A, double xi, double yi, double zi,
-                                               double xj, double yj, double zj, long long& K) {
-    double dx, dy, dz;
-    const double r2 = pair_disp(A, xi, yi, zi, xj, yj, zj, dx, dy, dz);
-    if (local_member(A, r2, dx, dy)) { K = 0; return 1; }
-    K = far_coef(D, A, dx, dy, dz);
-    return 0;
-}
-
 // C_ij from i's ascending L row by one wave: 64 samples at a stride, then the
 // segment that must hold j (two dependent loads instead of a binary search's
 // chain).  All lanes of the wave call it with the same i and j.
@@ -1566,9 +1555,9 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                 pair_terms_rec(A, ri, rj, r, exA, exB);
                 s_c.rij[tid] = r; s_c.exA[tid] = exA; s_c.exB[tid] = exB;
                 s_c.pi[tid] = pi; s_c.pj[tid] = pj;   // the unfolded changes are added below
-                long long K;
-                s_c.kin[tid] = local_pair_kind(D, A, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, K);
-                s_c.kq[tid] = K;
+                double dx, dy, dz;
+                const double r2 = pair_disp(A, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, dx, dy, dz);
+                s_c.kin[tid] = local_member(A, r2, dx, dy) ? 1 : 0;   // K_ij after the barrier
                 s_c.far[tid][0] = ri.phiw; s_c.far[tid][1] = rj.phiw;
                 s_c.cq[tid][0] = ri.q; s_c.cq[tid][1] = rj.q;
                 s_c.cs[tid][0] = ri.selfw; s_c.cs[tid][1] = rj.selfw;
@@ -1600,7 +1589,17 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         }
         for (int k = tid; k < DA_M * 8; k += DA_THREADS) (&s_y[0][0])[k] = 0.f;
         if constexpr (LOCAL) {
-            // C_ij of the pairs L holds: one wave per such candidate
+            // K_ij: Gt_ij (one thread per candidate), or C_ij when L holds the pair (one
+            // wave per such candidate)
+            if (tid >= DA_THREADS - DA_M) {
+                const int c = tid - (DA_THREADS - DA_M);
+                if (c < ntr && !s_c.kin[c]) {
+                    double dx, dy, dz;
+                    pair_disp(A, s_c.cx[c][0], s_c.cy[c][0], s_c.cz[c][0], s_c.cx[c][1], s_c.cy[c][1], s_c.cz[c][1],
+                              dx, dy, dz);
+                    s_c.kq[c] = far_coef(D, A, dx, dy, dz);
+                }
+            }
             for (int c = wave; c < ntr; c += DA_WAVES)
                 if (s_c.kin[c]) {
                     const long long K = wave_find_coef(A, s_c.i[c], s_c.j[c], lane);
