@@ -431,6 +431,9 @@ def main():
     ap.add_argument("--da-real-space", action="store_true",
                     help="also run the W . phi form whose surrogate includes the real-space potential")
     ap.add_argument("--no-da-wphi", action="store_true", help="skip the W . phi form of the surrogate")
+    ap.add_argument("--da-only", action="store_true",
+                    help="the plain and local-pair forms only (no W . phi, FP32, batched, program, set-up or CPU "
+                         "legs): a short command for kernel profiles")
     ap.add_argument("--da-pending-max", type=int, default=8,
                     help="pending budget of the local-pair delayed-acceptance chain (accepted moves per launch x 2)")
     ap.add_argument("--fp32-trials", type=int, default=100000)
@@ -439,6 +442,8 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
     ap.add_argument("--batch-sweeps", type=int, default=5, help="timed sweeps of the batched replicas")
     args = ap.parse_args()
+    if args.da_only:
+        args.no_da_wphi = args.no_fp32 = args.no_batched = args.no_program = args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one replica chain on one GPU "
               "(independent replicas would shard across GPUs without communication)", file=sys.stderr)
@@ -710,7 +715,8 @@ def main():
         out["forms"]["batched"] = batched_form(S, args)
     if not args.no_program:
         out["forms"]["program"] = program_form()
-    out["setup"] = setup_form(S)
+    if not args.da_only:
+        out["setup"] = setup_form(S)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         n = args.cpu_trials or 150
