@@ -222,6 +222,85 @@ def fp32_form(S, L, SG, args):
             "total_energy_rel_error_vs_oracle": abs(e32 - e64) / abs(e64)}
 
 
+def learned_form(S, SG, args):
+    """The learned surrogate on the 2,640-atom brush (64 chains x 20 titratable
+    beads): trained on about 2x10^6 (descriptor, exact Delta-E) pairs from exact
+    chains at pH 6-9 with the W . phi physics part (where the network carries the
+    real-space energy of the environment).  At pH 6, 7.5 and 9, from a state
+    equilibrated by the plain chain: alpha and trial / accepted moves/s of the
+    plain chain, of delayed acceptance with the trained network, of delayed
+    acceptance with a network that outputs zero (the method without it), and of
+    the local-pair form with its own network (same number of pairs)."""
+    import math
+    import numpy as np
+    s, p, _ = S.config_small_fp64()
+    n_per_ph = max(4096, (args.learned_pairs // 7 // 4096) * 4096)
+    spec = SG.CollectSpec(samples_per_ph=n_per_ph, per_state=4096, equilibrate=20000, between=2000)
+    sur = S.Surrogate(len(spec.ph_values) * n_per_ph)
+    col = SG.collect(s, p, sur, spec)
+    tr = SG.train_holdout(sur, epochs=args.learned_epochs, batch=4096, lr=2e-3)
+    spec_l = SG.CollectSpec(samples_per_ph=n_per_ph, per_state=4096, equilibrate=20000, between=2000,
+                            local_pairs=True)
+    sur_l = S.Surrogate(len(spec_l.ph_values) * n_per_ph)
+    col_l = SG.collect(s, p, sur_l, spec_l)
+    tr_l = SG.train_holdout(sur_l, epochs=args.learned_epochs, batch=4096, lr=2e-3)
+    zero = S.Surrogate(0)
+    theta = np.zeros_like(zero.get_params())
+    theta[-1] = 1.0
+    zero.set_params(theta)
+    blocks, bt = 10, args.learned_trials // 10
+
+    def sample(run, stats_of):
+        a = []
+        s0 = stats_of()
+        t0 = time.perf_counter()
+        for _ in range(blocks):
+            st = run(bt)
+            a.append(st["nprot"] / st["nsites"])
+        dt = time.perf_counter() - t0
+        s1 = stats_of()
+        acc = (s1["accepted"] - s0["accepted"]) / max(s1["attempted"] - s0["attempted"], 1)
+        st1 = (s1.get("stage1_accepted", 0) - s0.get("stage1_accepted", 0)) / max(s1["attempted"] - s0["attempted"], 1)
+        m = sum(a) / len(a)
+        sem = math.sqrt(sum((x - m) ** 2 for x in a) / (len(a) - 1) / len(a))
+        return {"alpha": m, "sem": sem, "trials_per_s": blocks * bt / dt, "acceptance": acc,
+                "accepted_per_s": blocks * bt * acc / dt, "stage1_acceptance": st1 if st1 > 0 else None}
+
+    rows = []
+    for ph in (6.0, 7.5, 9.0):
+        mc = S.McParams.at_ph(ph)
+        rep = S.Replica(s, p, mc, seed=1, pending_max=8, recompute_interval=20000)
+        rep.mc.run(args.learned_burn)
+        q, ty = rep.atoms.identity()
+        s_eq = S.System(s.x, s.y, s.z, q, ty, s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data)
+        row = {"pH": ph, "plain": sample(rep.mc.run, rep.mc.stats)}
+        del rep
+        for name, net, local in (("da", sur, False), ("da_zero_network", zero, False), ("da_local_pairs", sur_l, True)):
+            da = S.Replica(s_eq, p, mc, seed=2, pending_max=8, recompute_interval=20000)
+            da.mc.da_attach(r_desc=spec.r_desc, local_pairs=local)
+            r = sample(lambda n: da.mc.da_run(net, n), lambda: da.mc.da_run(net, 0))
+            r["stage2_acceptance"] = r["acceptance"] / max(r["stage1_acceptance"] or 1e-12, 1e-12)
+            r["over_plain"] = r["trials_per_s"] / row["plain"]["trials_per_s"]
+            r["over_plain_accepted"] = r["accepted_per_s"] / max(row["plain"]["accepted_per_s"], 1e-12)
+            r["alpha_diff"] = r["alpha"] - row["plain"]["alpha"]
+            row[name] = r
+            del da
+        rows.append(row)
+        print(f"[bench] learned surrogate, 2,640 atoms, pH {ph}: alpha plain {row['plain']['alpha']:.4f}, "
+              f"DA {row['da']['alpha']:.4f}; DA {row['da']['over_plain']:.2f}x trials, "
+              f"{row['da']['over_plain_accepted']:.2f}x accepted (zero network "
+              f"{row['da_zero_network']['over_plain_accepted']:.2f}x; local pairs "
+              f"{row['da_local_pairs']['over_plain_accepted']:.2f}x)", file=sys.stderr, flush=True)
+    return {"workload": "64x20 all-titratable brush, 2,640 atoms, SPME 32x32x128 order 5, W . phi surrogate",
+            "training_pairs": col["samples"], "collect_s": col["seconds"], "train_s": tr["ms"] / 1e3,
+            "rmse_holdout_kT": tr["rmse_holdout"], "mae_holdout_kT": tr["mae_holdout"],
+            "target_std_kT": tr["target_std"], "trials_per_block": bt, "blocks": blocks,
+            "local_pairs_network": {"training_pairs": col_l["samples"], "rmse_holdout_kT": tr_l["rmse_holdout"],
+                                    "target_std_kT": tr_l["target_std"]},
+            "note": "candidates are screened 64 per round; a launch of this form ends at its accepted move",
+            "rows": rows}
+
+
 def fp32_44k_form(S, L, s_eq, p, mc, kw, K, T, fp64_value, fp64_bpt):
     """The FP32 form on the benchmark's own workload (44,000 atoms, from the same
     equilibrated state): trial moves/s over K steps of T trials, and the live
@@ -437,6 +516,12 @@ def main():
     ap.add_argument("--da-pending-max", type=int, default=8,
                     help="pending budget of the local-pair delayed-acceptance chain (accepted moves per launch x 2)")
     ap.add_argument("--fp32-trials", type=int, default=100000)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-surrogate form on the 2,640-atom brush")
+    ap.add_argument("--learned-pairs", type=int, default=2000000,
+                    help="training pairs of the learned-surrogate form (pH 6-9, 2,640-atom brush)")
+    ap.add_argument("--learned-epochs", type=int, default=40)
+    ap.add_argument("--learned-burn", type=int, default=200000)
+    ap.add_argument("--learned-trials", type=int, default=500000, help="trials per chain and pH (10 blocks)")
     ap.add_argument("--ess-trials", type=int, default=6000000,
                     help="recorded trials per form for the autocorrelation times of N_prot and E (0: skip)")
     ap.add_argument("--batch-warmup", type=int, default=2, help="untimed sweeps after autotuning")
@@ -444,6 +529,7 @@ def main():
     args = ap.parse_args()
     if args.da_only:
         args.no_da_wphi = args.no_fp32 = args.no_batched = args.no_program = args.no_cpu_baseline = True
+        args.no_learned = True
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs one replica chain on one GPU "
               "(independent replicas would shard across GPUs without communication)", file=sys.stderr)
@@ -713,6 +799,8 @@ def main():
         out["forms"]["fp32_44k"] = fp32_44k_form(S, L, s_eq, p, mc, kw, K, T, plain_value, bpt)
     if not args.no_batched:
         out["forms"]["batched"] = batched_form(S, args)
+    if not args.no_learned:
+        out["forms"]["learned_2640"] = learned_form(S, SG, args)
     if not args.no_program:
         out["forms"]["program"] = program_form()
     if not args.da_only:
