@@ -1262,8 +1262,8 @@ __device__ __forceinline__ void da_pool_swap(const McDev& D, PoolT* pbuf, const 
 // touches and those an ion's change touches are disjoint (no histogram counter
 // is written by both rows): both rows' read-modify-writes in one load phase and
 // one store phase.  hrow: the rows' bounds in dlist (i's, then j's).
-__device__ void da_hist_update(const DaDev& A, const int* hrow, int so_i, int sn_i, int so_j, int sn_j) {
-    const int tid = threadIdx.x, bd = blockDim.x;
+__device__ void da_hist_update(const DaDev& A, const int* hrow, int so_i, int sn_i, int so_j, int sn_j, int tid,
+                               int bd) {
     const int hi = hrow[0], ni = so_i != sn_i ? hrow[1] - hrow[0] : 0;
     const int hj = hrow[2], nj = so_j != sn_j ? hrow[3] - hrow[2] : 0;
     const int NH = ni + nj;
@@ -1357,6 +1357,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ __align__(16) unsigned short s_H1[DA_M * HP];
     __shared__ float s_y[DA_M][8];
     __shared__ int s_first, s_start;
+    __shared__ int s_defer;                // local-pair form: an accepted move's T and histogram updates are due
     __shared__ int s_hrow[4];              // dlist bounds of the staged trial's i and j
     __shared__ double s_kij;               // the staged pair's real-space coefficient (0: not listed)
     __shared__ unsigned long long s_msk;   // stage-1 acceptances of the round
@@ -1457,13 +1458,22 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         s_st.blk = 0; s_st.pos = C->mti; s_st.pend_n = pn0;
         s_st.E = C->E; s_st.Q = C->Q; s_st.Q2 = C->Q2; s_st.Mz = C->Mz; s_st.Qz2 = C->Qz2;
         s_s1 = C->da_stage1; s_rounds = C->da_rounds;
+        s_defer = 0;
         for (int k = 0; k < 8; ++k) s_prof[k] = C->da_prof[k];
     }
     const float om = net.theta[OFF_MEAN], osd = net.theta[OFF_STD], b3 = net.theta[OFF_B3];
     __syncthreads();
     clk.lap(prof, s_prof, 0);
     for (;;) {
-        // ---- 1. proposals (wave 0), each assumed rejected at stage 1
+        // ---- 1. proposals (wave 0), each assumed rejected at stage 1; the other waves
+        //         meanwhile apply the last accepted move's T-row and histogram updates
+        if (LOCAL && wave != 0 && s_defer) {
+            const TrialT<P>& t = s_tr[0];
+            local_update(A, t.i, t.qi_old, t.qi_new, t.j, t.qj_old, t.qj_new, tid - 64, DA_THREADS - 64);
+            da_hist_update(A, s_hrow, A.species[t.ti_old], A.species[t.ti_new], A.species[t.tj_old],
+                           A.species[t.tj_new], tid - 64, DA_THREADS - 64);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // complete before the barrier
+        }
         if (wave == 0) {
             int blk = s_st.blk, pos = s_st.pos, status = s_st.status;
             const long long done = s_st.done;
@@ -1532,6 +1542,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         }
         __syncthreads();
         clk.lap(prof, s_prof, 1);
+        if (LOCAL && tid == 0) s_defer = 0;   // every reader passed the barrier
         if (s_dc.stop) break;
         const int ntr = s_dc.ntr;
         // ---- 2. per-candidate physics terms (W . phi from the per-atom table) and the raw
@@ -1867,16 +1878,21 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                     } else if (tid == 1) {
                         A.rec[t.j].q = t.qj_new; A.rec[t.j].type = t.tj_new; A.rec[t.j].sp = A.species[t.tj_new];
                     }
-                    local_update(A, t.i, t.qi_old, t.qi_new, t.j, t.qj_old, t.qj_new, tid, DA_THREADS);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the atomics complete before the barrier
+                    if (!A.upd_one) {
+                        local_update(A, t.i, t.qi_old, t.qi_new, t.j, t.qj_old, t.qj_new, tid, DA_THREADS);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // the atomics complete before the barrier
+                    }
                 } else if (A.use_psi) {
                     psi_update<P>(D, A, s_tr[0], tid, DA_THREADS);
                 }
             }
-            if (accepted && A.upd_one) {
+            if (accepted && A.upd_one && LOCAL) {
+                // T rows and histograms: waves 1-7 during the next round's proposals
+                if (tid == 0) s_defer = 1;
+            } else if (accepted && A.upd_one) {
                 const TrialT<P>& t = s_tr[0];
                 da_hist_update(A, s_hrow, A.species[t.ti_old], A.species[t.ti_new], A.species[t.tj_old],
-                               A.species[t.tj_new]);
+                               A.species[t.tj_new], tid, DA_THREADS);
             } else if (accepted) {
                 const TrialT<P>& t = s_tr[0];
                 for (int side = 0; side < 2; ++side) {
