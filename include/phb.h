@@ -512,15 +512,20 @@ typedef struct {
                                             fixed point); 0: it does not.  A network is trained with the
                                             physics part of the chain it screens for. */
     int local_pairs;                     /* 1: local-pair form.  The physics part takes the potential at the site
-                                            and the ion as far_a + q_a s_a + sum over the pool atoms b within
-                                            r_local of q_b (W_a^T G W_b + k_ab), the sum kept exactly in fixed
-                                            point as moves are accepted and far_a (everything else) frozen at the
-                                            state phb_da_freeze saw; the surrogate then needs no potential refresh
-                                            after an accepted move, and a launch holds up to pending_max/2 of them
-                                            (the exact stage takes the pending changes as the plain chain does).
-                                            real_space is ignored.  0: the potential term is W_a . phi of the
-                                            current state, one accepted move per launch. */
-    double r_local;                      /* local-pair radius: 0 = the real-space cutoff rc; rc <= r_local <= r_desc */
+                                            and the ion as far_a + q_a s_a + T_a, T_a = sum over the pool atoms b
+                                            of L(a) of q_b C_ab plus, for the pool atoms outside L(a), their
+                                            charge changes since phb_da_freeze times a smoothed continuum
+                                            reciprocal pair kernel; T is kept exactly in fixed point as moves are
+                                            accepted and far_a (everything else) is frozen at the state
+                                            phb_da_freeze saw.  L(a) holds the pool atoms within r_local of a and
+                                            those in its lateral column (|dx|, |dy| <= 5 mesh spacings, where the
+                                            order-P mesh couples z-aligned atoms), C_ab = W_a^T G W_b + k_ab.
+                                            The surrogate then needs no potential refresh after an accepted move,
+                                            and a launch holds up to pending_max/2 of them (the exact stage takes
+                                            the pending changes as the plain chain does).  real_space is ignored.
+                                            0: the potential term is W_a . phi of the current state, one accepted
+                                            move per launch. */
+    double r_local;                      /* local-pair radius: 0 = r_desc; rc <= r_local <= r_desc */
 } phb_da_desc;
 /* Builds the r_desc neighbour rows and per-atom histograms of a chain (and, in the
  * local-pair form, the local pair rows and their coefficients). */

@@ -2212,7 +2212,8 @@ void build_smoothed_kernel(phb_mc_s* M, DaState* S) {
     sync();
 }
 
-// L rows: the pool atoms of each pool atom's r_desc row within r_local, and their C_ab.
+// L rows: for each pool atom the pool atoms within r_local of it or in its lateral
+// column, ascending, and their C_ab.
 void build_local(phb_mc_s* M, DaState* S) {
     build_smoothed_kernel(M, S);
     const int N = M->atoms->n, np = S->ncand;

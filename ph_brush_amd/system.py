@@ -515,8 +515,10 @@ class MonteCarlo(_Handle):
     def da_attach(self, species=None, r_desc: float = 6.0, candidates: int = 64, real_space: bool = False,
                   local_pairs: bool = False, r_local: float = 0.0):
         """local_pairs: the local-pair form of the surrogate's physics part (the
-        potential of the pool atoms within r_local kept exactly, the rest frozen
-        by da_freeze; several accepted moves per launch)."""
+        pair potential of the pool atoms within r_local (0: r_desc) or in the
+        same lateral column kept exactly, the changes of the other pool atoms
+        through a smoothed continuum kernel, the rest frozen by da_freeze;
+        several accepted moves per launch)."""
         sp = np.ascontiguousarray(default_species(NTYPES) if species is None else species, np.int8)
         self._species = sp
         d = L.DaDesc(sp.ctypes.data_as(C.POINTER(C.c_byte)), len(sp), float(r_desc), int(candidates),
