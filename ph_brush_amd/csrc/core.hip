@@ -242,6 +242,7 @@ extern "C" int phb_atoms_create(int natoms, const double* x, const double* y, co
         a->max_type = tmax;
         a->x.alloc(natoms); a->y.alloc(natoms); a->z.alloc(natoms);
         a->q.alloc(natoms); a->type.alloc(natoms);
+        a->sp_slot.alloc(natoms); a->sp_list.alloc(natoms);
         a->x.upload(x, natoms); a->y.upload(y, natoms); a->z.upload(z, natoms);
         a->q.upload(q, natoms); a->type.upload(type0, natoms);
         sync();

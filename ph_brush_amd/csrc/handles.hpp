@@ -10,6 +10,7 @@ struct phb_atoms_s {
     int max_type = 0;                 // largest type id uploaded (host-side bound check)
     phb::DevBuf<double> x, y, z, q;   // SoA, wrapped positions
     phb::DevBuf<int> type;            // 0-based
+    phb::DevBuf<int> sp_slot, sp_list;   // spreading: each atom's slot in its tile, the tiles' atom lists
 };
 
 struct phb_csr_s {

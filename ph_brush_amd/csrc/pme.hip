@@ -40,6 +40,9 @@ struct PairArgs {
 };
 
 constexpr int PAIR_LANES = 16;   // lanes per neighbour row in the pair sums (power of 2)
+constexpr int SPT_MAX = 8;        // largest spread tile edge (grid points)
+constexpr int SP_THREADS = 512;   // threads of a spread tile's workgroup (one per atom)
+constexpr int SP_SCAN = 1024;     // threads of the one-block tile scan
 
 // PAIR_LANES lanes per row i (grid-stride over rows); FP64 per-lane sums
 // (T = double) or FP32 row sums (T = float) folded into FP64, block partials in FP64.
@@ -116,36 +119,205 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
 }
 
-// Order-p B-spline spreading (MC/src/pme.cu:140-172 at p = 4) with hardware
-// floating-point atomics; atoms with q = 0 contribute nothing and are skipped.
-// One thread per (atom, x offset a): P^2 atomics each, so that the atomics of
-// many atoms are in flight at once.  Atoms are visited in the order k * stride
-// mod N (stride coprime to N, a bijection): consecutive atoms are bonded
-// neighbours whose stencils overlap, and a wave of them would queue its
-// atomics on the same grid points.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spread(const double* __restrict__ x, const double* __restrict__ y,
-                                                const double* __restrict__ z, const double* __restrict__ q, int N,
-                                                int stride, GridGeom g, T* __restrict__ grid) {
-    const int P = g.order, off = stencil_offset(P);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = t / P, a = t - k * P;
-    if (k >= N) return;
-    const int i = (int)(((long long)k * stride) % N);
-    const double qi = q[i];
-    if (qi == 0.0) return;
-    int i0[3];
-    T w[3][8];
-    stencil_of(x[i], y[i], z[i], g, i0, w);
-    {
-        const int ixa = wrap(i0[0] + a - off, g.nx);
-        for (int b = 0; b < P; ++b) {
-            const int iyb = wrap(i0[1] + b - off, g.ny);
-            const T wxy = (T)qi * w[0][a] * w[1][b];
-            for (int c = 0; c < P; ++c) {
-                const int izc = wrap(i0[2] + c - off, g.nz);
-                unsafeAtomicAdd(grid + ((size_t)izc * g.ny + iyb) * g.nx + ixa, wxy * w[2][c]);
+// Order-p B-spline spreading (MC/src/pme.cu:140-172 at p = 4) through
+// LDS-privatised tiles, summed exactly in 64-bit fixed point.  The grid is cut
+// into tiles of SPT_MAX points a side (fewer along an axis shorter than that);
+// an atom belongs to the tile that holds its stencil origin
+// (the lowest grid index its stencil touches along each axis), and the tile's
+// workgroup sums its atoms' P^3 contributions into an LDS copy of the
+// (tx + P - 1)(ty + P - 1)(tz + P - 1) points they reach, then adds the non-zero
+// points to the grid with global integer atomics.  Every contribution is
+// rounded once to units of 2^-S, S = 62 - ceil(log2(max|q| * fullest tile *
+// tiles that reach a point)), so no partial sum can overflow and every sum is
+// exact: the grid does not depend on the order in which atoms or tiles arrive
+// and is the same bit for bit on every run.  Atoms with q = 0 contribute
+// nothing and are skipped.  Five kernels: bin (tile counts, each atom's slot,
+// max|q|), one-block scan of the counts (tile offsets, S; clears the counts and
+// max|q| for the next call), fill (tile lists), tile sums, and the conversion
+// to T that also clears the fixed-point grid.
+
+// stencil origin of an atom along each axis (wrapped), as stencil_of places it
+__device__ __forceinline__ void spread_origin(double x, double y, double z, const GridGeom& g, int o[3]) {
+    const int off = stencil_offset(g.order);
+    o[0] = wrap((int)floor((x / g.Lx) * g.nx) - off, g.nx);
+    o[1] = wrap((int)floor((y / g.Ly) * g.ny) - off, g.ny);
+    o[2] = wrap((int)floor((z / g.Lz) * g.nz) - off, g.nz);
+}
+
+struct SpreadTiles {
+    int tx, ty, tz;      // tile edges (grid points)
+    int tnx, tny, tnz;   // tiles along each axis
+    __device__ __forceinline__ int tile_of(const int o[3]) const {
+        return ((o[2] / tz) * tny + o[1] / ty) * tnx + o[0] / tx;
+    }
+};
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long u = __shfl_xor(v, d);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// bound[0]: bits of max|q| (non-negative doubles order as their bit patterns),
+// bound[2]: S of the call.  Lanes of a wave that share a tile take their slots
+// with one atomic (consecutive atoms are bonded neighbours, mostly in one tile).
+__global__ __launch_bounds__(256) void k_spread_bin(const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const double* __restrict__ q, int N,
+                                                    GridGeom g, SpreadTiles st, int* __restrict__ count,
+                                                    int* __restrict__ slot, unsigned long long* __restrict__ bound) {
+    __shared__ unsigned long long s_q[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+    int t = -1;
+    double aq = 0.0;
+    if (i < N) {
+        const double qi = q[i];
+        if (qi != 0.0) {
+            int o[3];
+            spread_origin(x[i], y[i], z[i], g, o);
+            t = st.tile_of(o);
+            aq = fabs(qi);
+        }
+    }
+    // group the wave's lanes by tile (one ballot per distinct tile), then every
+    // group's leader takes the group's slots with one atomic, all at once
+    int leader = lane, rank = 0, gsize = 0;
+    unsigned long long todo = __ballot(t >= 0);
+    while (todo) {
+        const int l = __ffsll((unsigned long long)todo) - 1;
+        const int tl = __shfl(t, l);
+        const unsigned long long m = __ballot(t == tl);
+        if (t == tl) { leader = l; rank = __popcll(m & ((1ull << lane) - 1ull)); }
+        if (lane == l) gsize = __popcll(m);
+        todo &= ~m;
+    }
+    int base = 0;
+    if (gsize > 0) base = atomicAdd(&count[t], gsize);
+    base = __shfl(base, leader);
+    if (t >= 0) slot[i] = base + rank;
+    const unsigned long long qb = wave_max_u64((unsigned long long)__double_as_longlong(aq));
+    if (lane == 0) s_q[threadIdx.x >> 6] = qb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long v = s_q[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = s_q[w] > v ? s_q[w] : v;
+        if (v > 0ull) atomicMax(&bound[0], v);
+    }
+}
+
+// One block: exclusive scan of the tile counts into off[0..nt] (each thread a
+// run of consecutive tiles), the fullest tile, and S; clears the counts and
+// max|q| for the next call.
+__global__ __launch_bounds__(SP_SCAN) void k_spread_scan(int* __restrict__ count, int nt, int* __restrict__ off,
+                                                         double reach, unsigned long long* __restrict__ bound) {
+    __shared__ int s_sum[SP_SCAN], s_max[SP_SCAN];
+    const int tid = threadIdx.x, per = (nt + SP_SCAN - 1) / SP_SCAN;
+    const int b = min(nt, tid * per), e = min(nt, b + per);
+    int sum = 0, mx = 0;
+    for (int k = b; k < e; ++k) {
+        const int c = count[k];
+        sum += c;
+        mx = max(mx, c);
+    }
+    s_sum[tid] = sum;
+    s_max[tid] = mx;
+    __syncthreads();
+    for (int d = 1; d < SP_SCAN; d <<= 1) {   // inclusive Hillis-Steele scan and running max
+        const int vs = tid >= d ? s_sum[tid - d] : 0, vm = tid >= d ? s_max[tid - d] : 0;
+        __syncthreads();
+        s_sum[tid] += vs;
+        s_max[tid] = max(s_max[tid], vm);
+        __syncthreads();
+    }
+    int run = s_sum[tid] - sum;
+    for (int k = b; k < e; ++k) {
+        const int c = count[k];
+        off[k] = run;
+        run += c;
+        count[k] = 0;
+    }
+    if (tid == SP_SCAN - 1) {
+        off[nt] = s_sum[tid];
+        int ex;
+        (void)frexp(__longlong_as_double((long long)bound[0]) * (double)s_max[tid] * reach, &ex);
+        bound[2] = (unsigned long long)(long long)(62 - ex);
+        bound[0] = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spread_fill(const double* __restrict__ x, const double* __restrict__ y,
+                                                     const double* __restrict__ z, const double* __restrict__ q, int N,
+                                                     GridGeom g, SpreadTiles st, const int* __restrict__ off,
+                                                     const int* __restrict__ slot, int* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || q[i] == 0.0) return;
+    int o[3];
+    spread_origin(x[i], y[i], z[i], g, o);
+    list[off[st.tile_of(o)] + slot[i]] = i;
+}
+
+// One workgroup per tile, one thread per atom (weights in registers).
+template <typename T, int P>
+__global__ __launch_bounds__(SP_THREADS) void k_spread_tile(const double* __restrict__ x, const double* __restrict__ y,
+                                                           const double* __restrict__ z, const double* __restrict__ q,
+                                                           GridGeom g, SpreadTiles st, const int* __restrict__ off,
+                                                           const int* __restrict__ list,
+                                                           const unsigned long long* __restrict__ bound,
+                                                           unsigned long long* __restrict__ gfx) {
+    constexpr int RMAX = SPT_MAX + P - 1;
+    __shared__ unsigned long long acc[RMAX * RMAX * RMAX];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int a0 = off[t], cnt = off[t + 1] - a0;
+    if (cnt == 0) return;
+    const int S = (int)(long long)bound[2];
+    const int rx = st.tx + P - 1, ry = st.ty + P - 1, rz = st.tz + P - 1, r3 = rx * ry * rz;
+    const int x0 = (t % st.tnx) * st.tx, y0 = ((t / st.tnx) % st.tny) * st.ty, z0 = (t / (st.tnx * st.tny)) * st.tz;
+    for (int k = tid; k < r3; k += SP_THREADS) acc[k] = 0ull;
+    __syncthreads();
+    for (int k = tid; k < cnt; k += SP_THREADS) {
+        const int i = list[a0 + k];
+        const double qi = q[i];
+        int i0[3];
+        T w[3][8];
+        stencil_of(x[i], y[i], z[i], g, i0, w);
+        const int so = stencil_offset(P);
+        const int ox = wrap(i0[0] - so, g.nx) - x0, oy = wrap(i0[1] - so, g.ny) - y0, oz = wrap(i0[2] - so, g.nz) - z0;
+        // a listed atom's origin lies in its tile by construction (the same arithmetic)
+        if (ox < 0 || ox >= st.tx || oy < 0 || oy >= st.ty || oz < 0 || oz >= st.tz) continue;
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+#pragma unroll
+            for (int b = 0; b < P; ++b) {
+                const T wbc = (T)qi * w[1][b] * w[2][c];
+                unsigned long long* row = acc + ((size_t)(oz + c) * ry + oy + b) * rx + ox;
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    const long long v = llrint(ldexp((double)(wbc * w[0][a]), S));
+                    if (v != 0) atomicAdd(row + a, (unsigned long long)v);
+                }
             }
         }
+    }
+    __syncthreads();
+    for (int k = tid; k < r3; k += SP_THREADS) {
+        const unsigned long long v = acc[k];
+        if (v == 0ull) continue;
+        const int lz = k / (rx * ry), ly = (k / rx) % ry, lx = k % rx;
+        const size_t gi = ((size_t)wrap(z0 + lz, g.nz) * g.ny + wrap(y0 + ly, g.ny)) * g.nx + wrap(x0 + lx, g.nx);
+        atomicAdd(gfx + gi, v);
+    }
+}
+
+// grid = fixed-point sums * 2^-S; clears the sums for the next call.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_fix(unsigned long long* __restrict__ gfx, size_t G,
+                                                    const unsigned long long* __restrict__ bound, T* __restrict__ grid) {
+    const int S = (int)(long long)bound[2];
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < G; k += (size_t)gridDim.x * blockDim.x) {
+        const long long v = (long long)gfx[k];
+        grid[k] = (T)ldexp((double)v, -S);
+        gfx[k] = 0ull;
     }
 }
 
@@ -482,31 +654,46 @@ phb_pme_s::~phb_pme_s() = default;
 
 namespace phb {
 
-// A stride coprime to n near sqrt(n) * 31 (1 when n is small).
-static int spread_stride(int n) {
-    if (n < 64) return 1;
-    auto gcd = [](long long u, long long v) { while (v) { const long long w = u % v; u = v; v = w; } return u; };
-    long long s = (long long)std::sqrt((double)n) * 31 + 1;
-    while (gcd(s, n) != 1) ++s;
-    return (int)(s % n == 0 ? 1 : s % n);
+static SpreadTiles spread_tiles_of(const phb_pme_s* P) {
+    return SpreadTiles{P->sp_t[0], P->sp_t[1], P->sp_t[2], P->sp_tn[0], P->sp_tn[1], P->sp_tn[2]};
+}
+
+template <typename T, int P>
+static void spread_tiles(phb_pme_s* Pl, const phb_atoms_s* a) {
+    hipLaunchKernelGGL((k_spread_tile<T, P>), dim3(Pl->sp_nt), dim3(SP_THREADS), 0, stream(), a->x.p, a->y.p, a->z.p,
+                       a->q.p, Pl->geom, spread_tiles_of(Pl), Pl->sp_off.p, a->sp_list.p, Pl->sp_bound.p, Pl->sp_fix.p);
+}
+
+template <typename T>
+static void spread_impl(phb_pme_s* P, const phb_atoms_s* a, T* grid) {
+    hipStream_t st = stream();
+    const size_t G = (size_t)P->nx * P->ny * P->nz;
+    const SpreadTiles tl = spread_tiles_of(P);
+    if (a->n > 0)
+        hipLaunchKernelGGL(k_spread_bin, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p, a->q.p,
+                           a->n, P->geom, tl, P->sp_count.p, a->sp_slot.p, P->sp_bound.p);
+    hipLaunchKernelGGL(k_spread_scan, dim3(1), dim3(SP_SCAN), 0, st, P->sp_count.p, P->sp_nt, P->sp_off.p, P->sp_reach,
+                       P->sp_bound.p);
+    if (a->n > 0) {
+        hipLaunchKernelGGL(k_spread_fill, dim3(ceil_div(a->n, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p, a->q.p,
+                           a->n, P->geom, tl, P->sp_off.p, a->sp_slot.p, a->sp_list.p);
+        switch (P->order) {
+            case 3: spread_tiles<T, 3>(P, a); break;
+            case 4: spread_tiles<T, 4>(P, a); break;
+            case 5: spread_tiles<T, 5>(P, a); break;
+            case 6: spread_tiles<T, 6>(P, a); break;
+            case 7: spread_tiles<T, 7>(P, a); break;
+            default: spread_tiles<T, 8>(P, a); break;
+        }
+    }
+    const int nb = (int)std::min<size_t>((G + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_spread_fix<T>, dim3(nb), dim3(256), 0, st, P->sp_fix.p, G, P->sp_bound.p, grid);
+    PHB_HIP(hipGetLastError());
 }
 
 void PmeKernels::launch_spread(phb_pme_s* P, const phb_atoms_s* a) {
-    hipStream_t st = stream();
-    const size_t G = (size_t)P->nx * P->ny * P->nz;
-    const int nb = (int)std::min<size_t>((G + 255) / 256, 4096);
-    if (P->precision == PHB_FP64) {
-        hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, st, P->grid64.p, G, 0.0);
-        if (a->n > 0)
-            hipLaunchKernelGGL(k_spread<double>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
-                               a->q.p, a->n, spread_stride(a->n), P->geom, P->grid64.p);
-    } else {
-        hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, st, P->grid32.p, G, 0.0f);
-        if (a->n > 0)
-            hipLaunchKernelGGL(k_spread<float>, dim3(ceil_div(a->n * P->order, 256)), dim3(256), 0, st, a->x.p, a->y.p, a->z.p,
-                               a->q.p, a->n, spread_stride(a->n), P->geom, P->grid32.p);
-    }
-    PHB_HIP(hipGetLastError());
+    if (P->precision == PHB_FP64) spread_impl<double>(P, a, P->grid64.p);
+    else spread_impl<float>(P, a, P->grid32.p);
 }
 
 template <typename T>
@@ -731,6 +918,32 @@ std::unique_ptr<phb_pme_s> build_pme(const phb_params_desc& d, const phb_box* bo
     }
     P->partials.alloc(std::max<size_t>(4 * 2048, (size_t)P->z_blocks));
     P->result.alloc(16);
+    {   // spread tiles; counts, fixed-point sums and bounds start cleared and each call leaves them so
+        const int n3[3] = {nx, ny, nz};
+        double reach = 1.0;
+        for (int ax = 0; ax < 3; ++ax) {
+            P->sp_t[ax] = std::min(SPT_MAX, n3[ax]);
+            P->sp_tn[ax] = ceil_div(n3[ax], P->sp_t[ax]);
+            // tiles whose atoms reach one point along the axis (a narrow last tile adds one; a
+            // stencil longer than the axis wraps onto itself)
+            const int d = ceil_div(P->order - 1, P->sp_t[ax]) + 1 + (n3[ax] % P->sp_t[ax] != 0);
+            reach *= (double)d * std::max(1, ceil_div(P->order, n3[ax]));
+        }
+        P->sp_reach = reach;
+        P->sp_nt = P->sp_tn[0] * P->sp_tn[1] * P->sp_tn[2];
+        const size_t nt = (size_t)P->sp_nt;
+        P->sp_count.alloc(nt);
+        P->sp_off.alloc(nt + 1);
+        P->sp_fix.alloc(G);
+        P->sp_bound.alloc(4);
+        hipStream_t st = stream();
+        hipLaunchKernelGGL(k_fill<int>, dim3((unsigned)std::min<size_t>((nt + 255) / 256, 4096)), dim3(256), 0, st,
+                           P->sp_count.p, nt, 0);
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3((unsigned)std::min<size_t>((G + 255) / 256, 4096)),
+                           dim3(256), 0, st, P->sp_fix.p, G, 0ull);
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, st, P->sp_bound.p, (size_t)4, 0ull);
+        PHB_HIP(hipGetLastError());
+    }
     sync();
     return P;
 }

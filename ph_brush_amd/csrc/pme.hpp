@@ -59,6 +59,12 @@ struct phb_pme_s {
     phb::DevBuf<double> etac64;
     phb::DevBuf<float> etac32;
     phb::DevBuf<double> partials, result;
+    // spreading tiles: edges and counts along each axis, tiles whose atoms reach
+    // one point, count and offset per tile, fixed-point grid, bounds of the call
+    int sp_t[3] = {1, 1, 1}, sp_tn[3] = {0, 0, 0}, sp_nt = 0;
+    double sp_reach = 1.0;
+    phb::DevBuf<int> sp_count, sp_off;
+    phb::DevBuf<unsigned long long> sp_fix, sp_bound;
     const double* conv_eta64() const { return etac64.p ? etac64.p : eta64.p; }
     const float* conv_eta32() const { return etac32.p ? etac32.p : eta32.p; }
     size_t plane_lds(size_t csz) const {
