@@ -498,6 +498,9 @@ def main():
                     help="trials between full recomputes of the energy and potential (drift reset)")
     ap.add_argument("--da-samples-per-ph", type=int, default=131072)
     ap.add_argument("--da-epochs", type=int, default=60)
+    ap.add_argument("--da-local-samples-per-ph", type=int, default=16384,
+                    help="training samples per pH of the local-pair form's network (its residual target is "
+                         "small, so fewer samples than the W.phi form's)")
     ap.add_argument("--cpu-trials", type=int, default=0, help="0 = auto (about 10-20 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -613,10 +616,11 @@ def main():
 
     def train_surrogate(local_pairs, real_space=False):
         t0 = time.perf_counter()
-        spec = SG.CollectSpec(samples_per_ph=args.da_samples_per_ph, per_state=4096, equilibrate=20000,
+        n_ph = args.da_local_samples_per_ph if local_pairs else args.da_samples_per_ph
+        spec = SG.CollectSpec(samples_per_ph=n_ph, per_state=4096, equilibrate=20000,
                               between=2000, local_pairs=local_pairs, real_space=real_space)
         sur = S.Surrogate(len(spec.ph_values) * spec.samples_per_ph)
-        col = SG.collect(s, p, sur, spec, log=lambda m: print(m, file=sys.stderr, flush=True))
+        col = SG.collect(s_eq, p, sur, spec, log=lambda m: print(m, file=sys.stderr, flush=True))
         tr = SG.train_holdout(sur, epochs=args.da_epochs, batch=4096, lr=2e-3)
         print(f"[bench] surrogate (local pairs {local_pairs}, real space {real_space}): {col['samples']} samples, "
               f"holdout rmse {tr['rmse_holdout']:.3f} (target std {tr['target_std']:.3f}), "

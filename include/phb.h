@@ -249,6 +249,11 @@ int phb_mc_recompute(phb_mc mc, double *E);
  * MC/src/main.cu:180-206).  set rejects an index above 624. */
 int phb_mc_rng_state(phb_mc mc, unsigned *state);
 int phb_mc_set_rng_state(phb_mc mc, const unsigned *state);
+/* Sets the fixed mu_eff of the chain's next runs (plain and delayed-acceptance):
+ * a titration moves one chain from pH to pH, as phase_gc_mu_control moves
+ * Params::mu_eff between blocks (MC/src/mc.cu:349, :530).  A batch built from
+ * the chain keeps the mu_eff it was built with. */
+int phb_mc_set_mu(phb_mc mc, double mu_eff);
 /* std::mt19937(seed) in the stream form above (host only). */
 int phb_rng_seed_state(unsigned seed, unsigned *state);
 /* The chains' device engine on its own (no reference counterpart; it checks the

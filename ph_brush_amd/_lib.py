@@ -250,6 +250,7 @@ SIGNATURES = {
     "phb_mc_recompute": [_h, _pd],
     "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
     "phb_mc_set_rng_state": [_h, C.POINTER(C.c_uint)],
+    "phb_mc_set_mu": [_h, C.c_double],
     "phb_rng_seed_state": [C.c_uint, C.POINTER(C.c_uint)],
     "phb_rng_draws": [C.POINTER(C.c_uint), C.c_longlong, _pi, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint),
                       C.POINTER(C.c_longlong)],

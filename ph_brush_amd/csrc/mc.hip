@@ -731,6 +731,16 @@ extern "C" int phb_mc_set_rng_state(phb_mc M, const unsigned* state) {
     });
 }
 
+extern "C" int phb_mc_set_mu(phb_mc M, double mu_eff) {
+    return guard([&] {
+        require(M, "phb_mc_set_mu: null");
+        require(std::isfinite(mu_eff), "phb_mc_set_mu: mu_eff must be finite");
+        M->dev.mu = mu_eff;
+        M->d.mu_eff = mu_eff;
+        return PHB_OK;
+    });
+}
+
 extern "C" int phb_mc_rng_state(phb_mc M, unsigned* state) {
     return guard([&] {
         require(M && state, "phb_mc_rng_state: null");

@@ -63,10 +63,12 @@ def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=N
     rng = np.random.default_rng(spec.seed)
     t0 = time.perf_counter()
     n0 = sur.size()
+    # one chain visits the pH values in turn (its mu set per pH), equilibrating at each
+    rep = S.Replica(s, p, S.McParams.at_ph(spec.ph_values[0], spec.pka), seed=spec.seed)
+    rep.mc.da_attach(r_desc=spec.r_desc, real_space=spec.real_space, local_pairs=spec.local_pairs)
     for k, ph in enumerate(spec.ph_values):
         mc = S.McParams.at_ph(ph, spec.pka)
-        rep = S.Replica(s, p, mc, seed=spec.seed + 1000 * k)
-        rep.mc.da_attach(r_desc=spec.r_desc, real_space=spec.real_space, local_pairs=spec.local_pairs)
+        rep.mc.set_mu(mc.mu_eff)
         rep.mc.run(spec.equilibrate)
         if spec.local_pairs:
             rep.mc.da_freeze()   # the states sampled below see this field, as a chain sees its start's
@@ -80,7 +82,7 @@ def collect(s: S.System, p: S.Params, sur: S.Surrogate, spec: CollectSpec, log=N
             got += n
         if log:
             log(f"[surrogate] pH {ph}: {got} samples ({time.perf_counter() - t0:.1f}s)")
-        del rep
+    del rep
     return {"samples": sur.size() - n0, "seconds": time.perf_counter() - t0}
 
 

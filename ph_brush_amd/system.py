@@ -453,6 +453,10 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_rng_state", self.h, st)
         return np.array(st[:], dtype=np.uint32)
 
+    def set_mu(self, mu_eff: float) -> None:
+        """Fixed mu_eff of the next runs (a titration moving one chain from pH to pH)."""
+        L.call("phb_mc_set_mu", self.h, float(mu_eff))
+
     def set_rng_state(self, state) -> None:
         """Continue from an engine state (625 words, std::mt19937's stream form)."""
         st = np.ascontiguousarray(state, np.uint32)

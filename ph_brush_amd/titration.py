@@ -37,8 +37,12 @@ def _sample(run, blocks, block_trials):
 
 
 def titration(cfg=S.config_30k_fp64, ph_values=(6.0, 6.5, 7.0, 7.5, 8.0, 8.5, 9.0), burn=300000, blocks=20,
-              block_trials=100000, samples_per_ph=131072, epochs=60, seed=1, log=print, local_pairs=True):
+              block_trials=100000, samples_per_ph=0, epochs=60, seed=1, log=print, local_pairs=True):
+    """samples_per_ph 0: 16,384 for the local-pair form (its network's residual
+    target is small), 131,072 for the W . phi form."""
     s, p, _ = cfg()
+    if samples_per_ph <= 0:
+        samples_per_ph = 16384 if local_pairs else 131072
     t0 = time.perf_counter()
     spec = SG.CollectSpec(samples_per_ph=samples_per_ph, per_state=4096, equilibrate=20000, between=2000,
                           local_pairs=local_pairs)
@@ -90,7 +94,8 @@ def main():
     ap.add_argument("--burn", type=int, default=300000)
     ap.add_argument("--blocks", type=int, default=20)
     ap.add_argument("--block-trials", type=int, default=100000)
-    ap.add_argument("--samples-per-ph", type=int, default=131072)
+    ap.add_argument("--samples-per-ph", type=int, default=0,
+                    help="0: 16,384 (local pairs) or 131,072 (W . phi)")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--wphi", action="store_true", help="the W . phi form of the surrogate instead of local pairs")
     a = ap.parse_args()

@@ -177,6 +177,24 @@ def test_chain_bit_reproducible_44k(gpu):
         assert np.float64(s1[k]).view(np.int64) == np.float64(s2[k]).view(np.int64), k
 
 
+def test_set_mu_equals_a_chain_built_at_that_mu(gpu):
+    """A chain moved to another pH by phb_mc_set_mu runs the trial sequence of a
+    chain created at that pH (same state and seed)."""
+    s, p, _ = S.config_small_fp64()
+    n = 2000
+    mc8 = S.McParams.at_ph(8.0)
+    a = S.Replica(s, p, S.McParams.at_ph(6.0), seed=9, record_cap=n)
+    a.mc.set_mu(mc8.mu_eff)
+    b = S.Replica(s, p, mc8, seed=9, record_cap=n)
+    sa, sb = a.mc.run(n), b.mc.run(n)
+    ra, rb = a.mc.record(n), b.mc.record(n)
+    for k in ("acc", "i", "j", "fwd"):
+        assert np.array_equal(ra[k], rb[k]), k
+    assert sa["E"] == sb["E"] and sa["nprot"] == sb["nprot"]
+    with pytest.raises(L.PhbError):
+        a.mc.set_mu(float("nan"))
+
+
 def test_rng_state_matches_std_mt19937(gpu):
     """The device engine starts from std::mt19937(seed): the C++ standard's seeding
     x[0] = seed, x[i] = 1812433253 * (x[i-1] ^ (x[i-1] >> 30)) + i, index 624."""
