@@ -92,6 +92,10 @@ __device__ __forceinline__ s4 ld4(const unsigned short* p) { return *reinterpret
 __device__ __forceinline__ f4 mfma16(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 // a 16x16x16 tile: A[i][k] in lane i + 16 (k/4), element k%4; B[k][j] in lane j + 16 (k/4);
 // D[i][j] in lane j + 16 (i/4), element i%4.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf8 ld8(const unsigned short* p) { return *reinterpret_cast<const bf8*>(p); }
+// 16x16x32: A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8); D as mfma16.
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 struct NetDev {
     const unsigned short *w1t;   // [128 out][48 in]   bf16 (B operand of the forward pass)
@@ -103,6 +107,9 @@ struct NetDev {
 
 // ---------------------------------------------------------------- 4-wave tile passes (64 rows)
 // Layer 1 for rows r0..r0+15 of X: z[ct] holds D-layout tiles (rows r0 + 4g + e, column 16 ct + rr).
+// Three 16x16x16 steps over the 48 inputs (a 16x16x32 step feeding a 16x16x16 step's
+// accumulator gave wrong values in two of the four accumulator registers, so the two
+// shapes are not chained); the delayed-acceptance chain issues the same steps.
 __device__ __forceinline__ void layer1(const unsigned short* X, int r0, const NetDev& n, f4 (&z)[8], int lane) {
     const int rr = lane & 15, g = lane >> 4;
     s4 a[3];
@@ -119,17 +126,18 @@ __device__ __forceinline__ void layer1(const unsigned short* X, int r0, const Ne
     }
 }
 
+// Layer 2: four 16x16x32 steps over the 128 hidden units.
 __device__ __forceinline__ void layer2(const unsigned short* H1, int r0, const NetDev& n, f4 (&z)[8], int lane) {
     const int rr = lane & 15, g = lane >> 4;
-    s4 a[8];
+    bf8 a[4];
 #pragma unroll
-    for (int kb = 0; kb < 8; ++kb) a[kb] = ld4(H1 + (r0 + rr) * HP + kb * 16 + 4 * g);
+    for (int kb = 0; kb < 4; ++kb) a[kb] = ld8(H1 + (r0 + rr) * HP + 32 * kb + 8 * g);
 #pragma unroll
     for (int ct = 0; ct < 8; ++ct) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
-        const unsigned short* wr = n.w2t + (ct * 16 + rr) * NN_H + 4 * g;
+        const unsigned short* wr = n.w2t + (ct * 16 + rr) * NN_H + 8 * g;
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb) acc = mfma16(a[kb], ld4(wr + kb * 16), acc);
+        for (int kb = 0; kb < 4; ++kb) acc = mfma32(a[kb], ld8(wr + 32 * kb), acc);
         const float b = n.theta[OFF_B2 + ct * 16 + rr];
         z[ct] = acc + b;
     }
@@ -229,10 +237,6 @@ __global__ __launch_bounds__(256) void k_mlp_forward(NetDev n, const short* __re
 // Every sum has a fixed order: the gradient does not depend on scheduling.
 constexpr int VP = 3 * NN_H + 2;   // vector-parameter record per workgroup: db1, db2, dw3, db3, loss
 
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-__device__ __forceinline__ bf8 ld8(const unsigned short* p) { return *reinterpret_cast<const bf8*>(p); }
-// 16x16x32: A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane j + 16 (k/8); D as mfma16.
-__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 // LDS copies of the weights in the training step (pitches in BF16 elements; the padding spreads
 // the 16 rows an MFMA operand reads over the LDS banks)
@@ -1631,43 +1635,62 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         }
         __syncthreads();
         clk.lap(prof, s_prof, 2);
-        // ---- 3. MLP on MFMA: nrb row blocks x 8 column tiles, tile t on wave t mod DA_WAVES
-        const int nrb = (ntr + 15) >> 4;
+        // ---- 3. MLP on MFMA (the steps of layer1/layer2): wave w owns the 16 candidates of row
+        //      block w & 3 and the column tiles (w >> 2) + 2u, u = 0..3, all four in flight at once
         {
-            const int rr = lane & 15, g = lane >> 4;
-            for (int t = wave; t < nrb * 8; t += DA_WAVES) {
-                const int rb = t % nrb, ct = t / nrb, r0 = rb * 16;
-                f4 acc = {0.f, 0.f, 0.f, 0.f};
-                const unsigned short* wr = s_w1t + (ct * 16 + rr) * XP + 4 * g;
+            const int rr = lane & 15, g = lane >> 4, r0 = (wave & 3) * 16, c0 = wave >> 2;
+            {
+                s4 a0[3];
 #pragma unroll
-                for (int kb = 0; kb < 3; ++kb) acc = mfma16(ld4(s_X + (r0 + rr) * XP + kb * 16 + 4 * g), ld4(wr + kb * 16), acc);
-                const float b = s_b1[ct * 16 + rr];
+                for (int kb = 0; kb < 3; ++kb) a0[kb] = ld4(s_X + (r0 + rr) * XP + kb * 16 + 4 * g);
+                f4 acc[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) s_H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(acc[e] + b));
+                for (int u = 0; u < 4; ++u) {
+                    const unsigned short* wr = s_w1t + ((c0 + 2 * u) * 16 + rr) * XP + 4 * g;
+                    acc[u] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kb = 0; kb < 3; ++kb) acc[u] = mfma16(a0[kb], ld4(wr + kb * 16), acc[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int ct = c0 + 2 * u;
+                    const float b = s_b1[ct * 16 + rr];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        s_H1[(r0 + 4 * g + e) * HP + ct * 16 + rr] = f2bf(gelu_f(acc[u][e] + b));
+                }
             }
             __syncthreads();
-            // a wave sums its tiles of one row block in registers and reduces once per
-            // row block into its own slot s_y[row][wave] (the other slots stay zero)
-            float yp[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int t = wave; t < nrb * 8; t += DA_WAVES) {
-                const int rb = t % nrb, ct = t / nrb, r0 = rb * 16;
-                const int row = ct * 16 + rr, swz = (2 * row) & 31;
-                const unsigned short* wr = s_w2t + row * NN_H;
-                f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-                for (int kb = 0; kb < 8; ++kb)
-                    acc = mfma16(ld4(s_H1 + (r0 + rr) * HP + kb * 16 + 4 * g), ld4(wr + (((4 * kb + g) ^ swz) << 2)), acc);
-                const float b = s_b2[row], w = s_w3[row];
+            {
+                bf8 a[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(acc[e] + b))) * w;
-                const int tn = t + DA_WAVES;
-                if (tn >= nrb * 8 || tn % nrb != rb) {
+                for (int kb = 0; kb < 4; ++kb) a[kb] = ld8(s_H1 + (r0 + rr) * HP + 32 * kb + 8 * g);
+                f4 acc[4];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float v = row_sum16(yp[e]);
-                        if (rr == 0) s_y[r0 + 4 * g + e][wave] = v;
-                        yp[e] = 0.f;
-                    }
+                for (int u = 0; u < 4; ++u) {
+                    const int row = (c0 + 2 * u) * 16 + rr, swz = (2 * row) & 31;
+                    const unsigned short* wr = s_w2t + row * NN_H;
+                    acc[u] = f4{0.f, 0.f, 0.f, 0.f};
+                    // k 32kb + 8g .. +7 are the swizzled chunks 8kb + 2g and 8kb + 2g + 1: swz is
+                    // even, so they stay adjacent (one 16-byte load)
+#pragma unroll
+                    for (int kb = 0; kb < 4; ++kb)
+                        acc[u] = mfma32(a[kb], ld8(wr + (((8 * kb + 2 * g) ^ swz) << 2)), acc[u]);
+                }
+                // the row's four tiles summed in registers, one reduction into this wave's
+                // slot s_y[row][wave] (the other slots stay zero)
+                float yp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int row = (c0 + 2 * u) * 16 + rr;
+                    const float b = s_b2[row], w = s_w3[row];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) yp[e] += bf2f(f2bf(gelu_f(acc[u][e] + b))) * w;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = row_sum16(yp[e]);
+                    if (rr == 0) s_y[r0 + 4 * g + e][wave] = v;
                 }
             }
         }
