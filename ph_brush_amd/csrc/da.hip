@@ -1300,6 +1300,74 @@ __device__ void da_hist_update(const DaDev& A, const int* hrow, int so_i, int sn
     }
 }
 
+// local_update and da_hist_update of an accepted move in one pass: the L-row heads
+// (lrow) and the histogram rows' bounds (hrow) are known, so the L entries and the
+// neighbour list are loaded together, then the counts; the T additions and the
+// count stores follow (about three dependent memory trips instead of six).
+__device__ __forceinline__ void local_hist_update(const DaDev& A, const int* lrow, const int* hrow, double qio,
+                                                  double qin, double qjo, double qjn, int so_i, int sn_i, int so_j,
+                                                  int sn_j, int tid, int bd) {
+    constexpr int U = 6, MH = 4;
+    const int hi = lrow[0], ni = lrow[1] - lrow[0], hj = lrow[2], nt = ni + (lrow[3] - lrow[2]);
+    const int di = hrow[0], dni = so_i != sn_i ? hrow[1] - hrow[0] : 0;
+    const int dj = hrow[2], NH = dni + (so_j != sn_j ? hrow[3] - hrow[2] : 0);
+    for (int base = 0, hb = 0; base < nt || hb < NH; base += U * bd, hb += MH * bd) {
+        long long K[U];
+        int b[U];
+        unsigned en[MH];
+        bool hv[MH];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + tid + u * bd;
+            b[u] = -1;
+            K[u] = 0;
+            if (e < nt) {
+                const int x = e < ni ? hi + e : hj + (e - ni);
+                K[u] = A.lcoef[x];
+                b[u] = A.latom[x];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int e = hb + tid + m * bd;
+            hv[m] = e < NH;
+            en[m] = hv[m] ? A.dlist[e < dni ? di + e : dj + (e - dni)] : 0u;
+        }
+        unsigned short* hp[MH];
+        int ho[MH], hn[MH], so[MH], sn[MH];
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int e = hb + tid + m * bd;
+            hp[m] = nullptr;
+            ho[m] = hn[m] = 0;
+            so[m] = sn[m] = -1;
+            if (hv[m]) {
+                const bool first = e < dni;
+                so[m] = first ? so_i : so_j;
+                sn[m] = first ? sn_i : sn_j;
+                hp[m] = A.H + (size_t)(en[m] & 0xFFFFFFu) * NN_IN + (en[m] >> DA_SHIFT) * 4;
+                if (so[m] >= 0) ho[m] = hp[m][so[m]];
+                if (sn[m] >= 0) hn[m] = hp[m][sn[m]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + tid + u * bd;
+            if (b[u] >= 0) {
+                const bool first = e < ni;
+                const long long d = first ? psi_c(qin, K[u]) - psi_c(qio, K[u]) : psi_c(qjn, K[u]) - psi_c(qjo, K[u]);
+                if (d != 0) atomicAdd(reinterpret_cast<unsigned long long*>(A.psil + b[u]), (unsigned long long)d);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MH; ++m)
+            if (hp[m]) {
+                if (so[m] >= 0) hp[m][so[m]] = (unsigned short)(ho[m] - 1);
+                if (sn[m] >= 0) hp[m][sn[m]] = (unsigned short)(hn[m] + 1);
+            }
+    }
+}
+
 // psi of the neighbours of i and j after an accepted move (integer atomics: the
 // order of the additions does not change the sums).
 template <int P>
@@ -1363,6 +1431,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ int s_first, s_start;
     __shared__ int s_defer;                // local-pair form: an accepted move's T and histogram updates are due
     __shared__ int s_hrow[4];              // dlist bounds of the staged trial's i and j
+    __shared__ int s_lrow[4];              // local-pair form: L-row bounds of the staged trial's i and j
     __shared__ double s_kij;               // the staged pair's real-space coefficient (0: not listed)
     __shared__ unsigned long long s_msk;   // stage-1 acceptances of the round
     __shared__ long long s_done0;          // trials done at the round's start
@@ -1473,9 +1542,9 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         //         meanwhile apply the last accepted move's T-row and histogram updates
         if (LOCAL && wave != 0 && s_defer) {
             const TrialT<P>& t = s_tr[0];
-            local_update(A, t.i, t.qi_old, t.qi_new, t.j, t.qj_old, t.qj_new, tid - 64, DA_THREADS - 64);
-            da_hist_update(A, s_hrow, A.species[t.ti_old], A.species[t.ti_new], A.species[t.tj_old],
-                           A.species[t.tj_new], tid - 64, DA_THREADS - 64);
+            local_hist_update(A, s_lrow, s_hrow, t.qi_old, t.qi_new, t.qj_old, t.qj_new, A.species[t.ti_old],
+                              A.species[t.ti_new], A.species[t.tj_old], A.species[t.tj_new], tid - 64,
+                              DA_THREADS - 64);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // complete before the barrier
         }
         if (wave == 0) {
@@ -1767,6 +1836,10 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
                         stage_trial<P>(D, s_tr[0], i1, j1, s_c.fwd[first]);
                         s_hrow[0] = A.dhead[i1]; s_hrow[1] = A.dhead[i1 + 1];
                         s_hrow[2] = A.dhead[j1]; s_hrow[3] = A.dhead[j1 + 1];
+                        if constexpr (LOCAL) {
+                            s_lrow[0] = A.lhead[i1]; s_lrow[1] = A.lhead[i1 + 1];
+                            s_lrow[2] = A.lhead[j1]; s_lrow[3] = A.lhead[j1 + 1];
+                        }
                         s_kij = 0.0;
                     }
                 }
