@@ -367,7 +367,9 @@ int phb_phase2_run(phb_mc mc, long long max_attempts, phb_phase2_stats *out);
 int phb_phase2_record(phb_mc mc, long long n, int *atoms, signed char *acc, double *dE);
 int phb_phase2_record_from(phb_mc mc, long long first);
 
-/* Time every chain launch, refresh and recompute with HIP events (on = 1). */
+/* Time every chain launch, refresh and recompute with HIP events: on = 1 also runs the
+ * delayed-acceptance chain's in-kernel phase clocks (phb_da_stats.phase_us), on = 2 the
+ * events alone (the kernels run as unprofiled), 0 off. */
 int phb_mc_set_profiling(phb_mc mc, int on);
 
 /* ------------------------------------------------------------ file formats

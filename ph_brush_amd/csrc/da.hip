@@ -2373,7 +2373,7 @@ void build_local(phb_mc_s* M, DaState* S) {
 
 // Uploads the views the chain kernel reads by pointer (before a run; the stream orders it).
 void upload_views(phb_mc_s* M, DaState* S, phb_surrogate_s* sur) {
-    S->dev.prof = M->profiling ? 1 : 0;
+    S->dev.prof = M->profiling && M->phase_clock ? 1 : 0;
     const NetDev net = sur->net();
     S->dmc.upload(&M->dev, 1);
     S->dda.upload(&S->dev, 1);

@@ -707,9 +707,11 @@ extern "C" int phb_mc_recompute(phb_mc M, double* E) {
 extern "C" int phb_mc_set_profiling(phb_mc M, int on) {
     return guard([&] {
         require(M != nullptr, "phb_mc_set_profiling: null");
+        require(on >= 0 && on <= 2, "phb_mc_set_profiling: on must be 0, 1 or 2");
         sync();
         if (M->profiling) M->collect();
         M->profiling = on != 0;
+        M->phase_clock = on == 1;
         M->chain_ms = M->refresh_ms = M->recompute_ms = 0.0;
         PHB_HIP(hipMemsetAsync(M->ctl.p->da_prof, 0, sizeof(M->ctl.p->da_prof), stream()));
         sync();

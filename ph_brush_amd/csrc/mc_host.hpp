@@ -36,6 +36,7 @@ struct phb_mc_s {
     long long refreshes = 0, recomputes = 0, launches = 0, next_recompute = 0, trials_seen = 0;
     // profiling: event triples around (chain launch, refresh) pairs and recomputes
     bool profiling = false;
+    bool phase_clock = false;   // with profiling: the delayed-acceptance chain's in-kernel phase clocks
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
     std::vector<std::pair<size_t, int>> marks;   // (first event index, kind: 0 chain+refresh, 1 recompute)

@@ -445,8 +445,10 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_recompute", self.h, C.byref(e))
         return e.value
 
-    def set_profiling(self, on: bool = True):
-        L.call("phb_mc_set_profiling", self.h, int(on))
+    def set_profiling(self, on: bool = True, phase_clock: bool = True):
+        """HIP-event times of every launch; phase_clock also runs the delayed-acceptance
+        chain's in-kernel phase clocks (which slow it slightly)."""
+        L.call("phb_mc_set_profiling", self.h, 0 if not on else (1 if phase_clock else 2))
 
     def rng_state(self) -> np.ndarray:
         st = (C.c_uint * 625)()
