@@ -86,11 +86,11 @@ __global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* _
     refresh_kspace_tile<P>(D, kg, twx, twy, twz, eta, pz, tx, blockIdx.x);
 }
 
-// Direct refresh (refresh_direct_tile): one workgroup per (y, z) tile of full x rows.
+// Direct refresh (refresh_direct_tile): one workgroup per listed (y, z) tile of full x rows.
 template <int P>
 __global__ __launch_bounds__(RD_THREADS) void k_refresh_direct(McDev D, double* __restrict__ phi,
-                                                               float* __restrict__ phi32) {
-    refresh_direct_tile<P>(D, phi, phi32, blockIdx.x);
+                                                               float* __restrict__ phi32, const int* __restrict__ tiles) {
+    refresh_direct_tile<P>(D, phi, phi32, tiles[blockIdx.x]);
 }
 
 __global__ void k_reset_pending(McCtl* C) {
@@ -317,9 +317,8 @@ void refresh_p(phb_mc_s* M) {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)refresh_direct_lds(P)));
             lds_set = true;
         }
-        const int nb = refresh_direct_blocks(Pm->ny, Pm->nz);
-        hipLaunchKernelGGL(k_refresh_direct<P>, dim3(nb), dim3(RD_THREADS), refresh_direct_lds(P), stream(),
-                           M->dev, M->phi.p, M->d.precision == PHB_FP32 ? M->phi32.p : nullptr);
+        hipLaunchKernelGGL(k_refresh_direct<P>, dim3(M->rd_ntiles), dim3(RD_THREADS), refresh_direct_lds(P), stream(),
+                           M->dev, M->phi.p, M->d.precision == PHB_FP32 ? M->phi32.p : nullptr, M->rd_tiles.p);
         PHB_HIP(hipGetLastError());
         ++M->refreshes;
         return;
@@ -563,6 +562,28 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
             const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz), pme->plane_gate_blocks());
             M->applied.alloc(na);
             PHB_HIP(hipMemsetAsync(M->applied.p, 0, na * sizeof(unsigned), stream()));
+        }
+        {   // Direct refresh tiles: the potential is read only at atoms' stencil points (trial
+            // atoms, W . phi, the exact stage), and positions are fixed, so a tile that no atom's
+            // stencil reaches (the vacuum of a slab geometry) is never read and is left as is.
+            // Tile 0 is always listed: its gate entry tells the chains that a refresh has run.
+            const int nty = (pme->ny + RD_TY - 1) / RD_TY, ntz = (pme->nz + RD_TZ - 1) / RD_TZ;
+            std::vector<int> sb(3 * (size_t)N);
+            if (N > 0) M->sbase.download(sb.data(), 3 * (size_t)N);
+            sync();
+            std::vector<char> need((size_t)nty * ntz, 0);
+            need[0] = 1;
+            for (int i = 0; i < N; ++i)
+                for (int c = 0; c < P; ++c) {
+                    const int tz = ((sb[3 * (size_t)i + 2] + c) % pme->nz) / RD_TZ;
+                    for (int b = 0; b < P; ++b) need[(size_t)tz * nty + ((sb[3 * (size_t)i + 1] + b) % pme->ny) / RD_TY] = 1;
+                }
+            std::vector<int> tiles;
+            for (int t = 0; t < nty * ntz; ++t)
+                if (need[t]) tiles.push_back(t);
+            M->rd_ntiles = (int)tiles.size();
+            M->rd_tiles.alloc(tiles.size());
+            M->rd_tiles.upload(tiles.data(), tiles.size());
         }
         // ---- device view
         McDev& D = M->dev;

@@ -29,6 +29,8 @@ struct phb_mc_s {
     size_t pool_lds_bytes = 0;
     DevBuf<McCtl> ctl;
     DevBuf<unsigned> applied;   // per refresh workgroup: the pending list it folded
+    DevBuf<int> rd_tiles;       // direct refresh: the tiles some atom's stencil reaches (tile 0 always)
+    int rd_ntiles = 0;
     DevBuf<signed char> rec_fwd, rec_acc;
     DevBuf<int> rec_i, rec_j;
     DevBuf<double> rec_dE, rec_u;
