@@ -47,6 +47,7 @@ struct RefreshView {
     int tx, nx, ny;
     double* phi;
     RefreshGate gate;
+    const int* tiles;                    // direct refresh: the replica's listed tiles
 };
 
 // Largest r with start[r] <= b (start ascending, R entries).
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(RD_THREADS) void k_direct_batch(const RefreshView* 
                                                              const int* __restrict__ start, int R) {
     const int r = replica_of(start, R, blockIdx.x);
     const RefreshView& w = v[r];
-    refresh_direct_tile<P>(*w.D, w.phi, nullptr, blockIdx.x - start[r]);
+    refresh_direct_tile<P>(*w.D, w.phi, nullptr, w.tiles[blockIdx.x - start[r]]);
 }
 
 __global__ __launch_bounds__(512) void k_c2r_batch(const RefreshView* __restrict__ v, const int* __restrict__ start,
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(512) void k_c2r_batch(const RefreshView* __restrict
 // ---- persistent sweep
 struct PersistView {
     double* phi;
-    int nt;     // direct-refresh tiles of the replica's grid
+    const int* tiles;   // the replica's listed direct-refresh tiles
+    int nt;     // how many
     int prio;   // the replica helpers look at k-th: most trials per sweep first
 };
 
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_sweep_persistent(const McDev* __
                     const int cmd = s_cmd[1], cnt = s_cmd[2];
                     if (cmd >= 0) {
                         for (int q = 0; q < cnt; ++q) {
-                            refresh_direct_tile<P>(D, pv[r].phi, nullptr, cmd + q);
+                            refresh_direct_tile<P>(D, pv[r].phi, nullptr, pv[r].tiles[cmd + q]);
                             __syncthreads();
                         }
                         if (tid == 0)
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(MC_THREADS) void k_sweep_persistent(const McDev* __
         if (ex) return;
         if (tt >= 0) {
             for (int q = 0; q < cnt; ++q) {
-                refresh_direct_tile<P>(views[rr], pv[rr].phi, nullptr, tt + q);
+                refresh_direct_tile<P>(views[rr], pv[rr].phi, nullptr, pv[rr].tiles[tt + q]);
                 __syncthreads();
             }
             if (tid == 0)
@@ -432,7 +434,8 @@ void setup_persistent(phb_batch_s* B) {
     for (int r = 0; r < B->R; ++r) {
         phb_mc_s* M = B->mcs[r];
         v[r].phi = M->phi.p;
-        v[r].nt = refresh_direct_blocks(M->pme->ny, M->pme->nz);
+        v[r].tiles = M->rd_tiles.p;
+        v[r].nt = M->rd_ntiles;
         require(M->applied.n >= (size_t)v[r].nt, "phb_batch_mode: refresh gate smaller than the tile count",
                 PHB_ERR_INTERNAL);
         order[r] = r;
@@ -539,7 +542,7 @@ void build_refresh_tables(phb_batch_s* B) {
         phb_mc_s* M = B->mcs[r];
         if (use_direct_refresh(M)) {
             vd.push_back(B->hviews[r]);
-            ds.push_back(ds.back() + refresh_direct_blocks(M->pme->ny, M->pme->nz));
+            ds.push_back(ds.back() + M->rd_ntiles);
         } else {
             vk.push_back(B->hviews[r]);
             ks.push_back(ks.back() + M->pme->z_blocks);
@@ -642,6 +645,7 @@ extern "C" int phb_batch_create(const phb_mc* mcs, int n, const phb_batch_desc* 
             w.gate.pending_n = &M->ctl.p->pending_n;
             w.gate.seq = &M->ctl.p->pend_seq;
             w.gate.applied = M->applied.p;
+            w.tiles = M->rd_tiles.p;
             require(Pm->kg64.p != nullptr, "phb_batch_create: replicas need FP64 plans", PHB_ERR_UNSUPPORTED);
         }
         // replicas sharing one PME plan would share its k-space buffer
