@@ -101,14 +101,30 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, con
     const int ox = g.ncx >= 3 ? 1 : 0, oy = g.ncy >= 3 ? 1 : 0, oz = g.ncz >= 3 ? 1 : 0;
     int found = 0;
     const int base = MODE == 1 ? head[i] : 0;
+    // i's 1-2 bond partners in registers (rows of up to four; longer rows are searched)
+    int bn = 0, b0 = -1, b1 = -1, b2 = -1, b3 = -1;
+    if (bh) {
+        const int blo = bh[i];
+        bn = bh[i + 1] - blo;
+        if (bn > 0) b0 = bl[blo];
+        if (bn > 1) b1 = bl[blo + 1];
+        if (bn > 2) b2 = bl[blo + 2];
+        if (bn > 3) b3 = bl[blo + 3];
+    }
+    // the x-adjacent cells of one (dy, dz) row are adjacent in the cell order: scan them as
+    // one range (two where the row wraps), so a round of 64 candidates spans cells
+    const int xa = cx - ox, xb = cx + ox;   // cells xa..xb along x, wrapped
     for (int dz = -oz; dz <= oz; ++dz) {
         const int kz = (cz + dz + g.ncz) % g.ncz;
         for (int dy = -oy; dy <= oy; ++dy) {
             const int ky = (cy + dy + g.ncy) % g.ncy;
-            for (int dx = -ox; dx <= ox; ++dx) {
-                const int kx = (cx + dx + g.ncx) % g.ncx;
-                const int kc = (kz * g.ncy + ky) * g.ncx + kx;
-                const int s = cstart[kc], e = cstart[kc + 1];
+            const int rowc = (kz * g.ncy + ky) * g.ncx;
+            for (int part = 0; part < 2; ++part) {
+                int lo, hi;   // cell range [lo, hi] of this part along x
+                if (xa >= 0 && xb < g.ncx) { if (part) break; lo = xa; hi = xb; }
+                else if (xa < 0) { if (part) { lo = 0; hi = xb; } else { lo = xa + g.ncx; hi = g.ncx - 1; } }
+                else { if (part) { lo = 0; hi = xb - g.ncx; } else { lo = xa; hi = g.ncx - 1; } }
+                const int s = cstart[rowc + lo], e = cstart[rowc + hi + 1];
                 for (int b = s; b < e; b += 64) {
                     const int idx = b + lane;
                     bool pass = false;
@@ -119,7 +135,8 @@ __global__ __launch_bounds__(256) void k_pairs(const double* __restrict__ x, con
                             const double ddx = min_image_cell(xi - xs[idx], g.Lx, g.ix);
                             const double ddy = min_image_cell(yi - ys[idx], g.Ly, g.iy);
                             const double ddz = min_image_cell(zi - zs[idx], g.Lz, g.iz);
-                            pass = r2_exact(ddx, ddy, ddz) <= rc2 && !bonded12(i, j, bh, bl);
+                            pass = r2_exact(ddx, ddy, ddz) <= rc2 &&
+                                   !(bn <= 4 ? (j == b0 || j == b1 || j == b2 || j == b3) : bonded12(i, j, bh, bl));
                         }
                     }
                     const unsigned long long m = __ballot(pass);
