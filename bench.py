@@ -415,8 +415,8 @@ def batched_form(S, args):
           file=sys.stderr, flush=True)
     tune = b.autotune(1)
     print(f"[bench] batched: autotune -> persistent={tune['persistent']} W={tune['spec_window']} "
-          f"pending={tune['pending_max']} pairs={tune['graph_pairs']} launch_cap={tune['launch_cap']}",
-          file=sys.stderr, flush=True)
+          f"pending={tune['pending_max']} pairs={tune['graph_pairs']} launch_cap={tune['launch_cap']} "
+          f"refresh tile={tune['refresh_ty']}x{tune['refresh_tz']}", file=sys.stderr, flush=True)
     b.run(args.batch_warmup)
     st0 = [r.mc.stats() for r in reps]
     nsw, ms, launches, replays = args.batch_sweeps, 0.0, 0, 0
@@ -471,7 +471,8 @@ def batched_form(S, args):
             "trials_timed": trials, "launches": launches, "graph_replays": replays,
             "autotuned": {"persistent": tune["persistent"], "spec_window": tune["spec_window"],
                           "pending_max": tune["pending_max"], "graph_pairs": tune["graph_pairs"],
-                          "launch_cap": tune["launch_cap"], "trials_per_s": tune["trials_per_s"]},
+                          "launch_cap": tune["launch_cap"], "trials_per_s": tune["trials_per_s"],
+                          "refresh_tile": [tune["refresh_ty"], tune["refresh_tz"]]},
             "roofline": {"kernel": kern, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_sweep": byts / nsw,
                          "note": "algorithmic bytes: each trial's rows, stencils and kernel windows, and each "

@@ -254,6 +254,10 @@ int phb_mc_set_rng_state(phb_mc mc, const unsigned *state);
  * Params::mu_eff between blocks (MC/src/mc.cu:349, :530).  A batch built from
  * the chain keeps the mu_eff it was built with. */
 int phb_mc_set_mu(phb_mc mc, double mu_eff);
+/* Tile shape of the chain's direct potential refresh: ty rows along y (8 or 4) by tz planes along
+ * z (4 or 2); default 8 x 4.  Pending changes are folded in first.  The refresh has no counterpart
+ * in the reference, which recomputes the potential per trial. */
+int phb_mc_set_refresh_tile(phb_mc mc, int ty, int tz);
 /* std::mt19937(seed) in the stream form above (host only). */
 int phb_rng_seed_state(unsigned seed, unsigned *state);
 /* The chains' device engine on its own (no reference counterpart; it checks the
@@ -592,15 +596,18 @@ typedef struct {
     int spec_window, pending_max, graph_pairs;
     long long launch_cap;            /* a chain launch waits for its slowest replica: the trial cap bounds it */
     int persistent;                  /* 1: one persistent launch per sweep */
+    int refresh_ty, refresh_tz;      /* direct-refresh tile: rows along y, planes along z */
 } phb_batch_stats;
 int phb_batch_create(const phb_mc *mcs, int n, const phb_batch_desc *d, phb_batch *out);
 int phb_batch_destroy(phb_batch b);
 /* `sweeps` sweeps: every replica attempts as many trials as it has titratable sites per sweep. */
 int phb_batch_run(phb_batch b, int sweeps, phb_batch_stats *st);
-/* Times speculation window x pending budget x graph length on the batch (each candidate runs
- * `sweeps_per_candidate` real sweeps), the persistent form at each window x budget where every
- * replica qualifies, then per-launch trial caps on the fastest graph shape, and keeps the fastest.
- * Launch boundaries do not change any replica's sequence. */
+/* After 2 x `sweeps_per_candidate` warm-up sweeps, times speculation window x pending budget x
+ * graph length on the batch (each candidate runs `sweeps_per_candidate` real sweeps), the
+ * persistent form at each window x budget where every replica qualifies, per-launch trial caps
+ * on the fastest graph shape, then the two forms' fastest shapes again back to back (twice the
+ * sweeps), then the direct refresh's tile shape (8 or 4 rows by 4 or 2 planes) on the faster,
+ * and keeps the fastest.  Launch boundaries and tile shapes do not change any replica's sequence. */
 int phb_batch_autotune(phb_batch b, int sweeps_per_candidate, phb_batch_stats *best);
 /* Sets a launch shape without timing: speculation window (4, 8 or 16), pending budget (2..MC_KMAX, even),
  * (chain, refresh) pairs per replay (>= 1) and the per-launch trial cap (0: none). */
@@ -609,6 +616,9 @@ int phb_batch_tune(phb_batch b, int spec_window, int pending_max, int graph_pair
  * uses the direct refresh and the device holds a workgroup per replica at once), 0: graph replays.
  * A persistent sweep whose waits pass 30 s of wall clock stops with an error. */
 int phb_batch_mode(phb_batch b, int persistent);
+/* Direct-refresh tile shape of every replica (ty rows along y: 8 or 4; tz planes along z: 4 or 2);
+ * no counterpart in the reference, whose potential is recomputed per trial. */
+int phb_batch_set_refresh_tile(phb_batch b, int ty, int tz);
 /* Per replica, of the last persistent sweep: wall time from the launch's start to the replica's
  * finish and the part of it its chain spent on refreshes (waiting for and folding tiles), in ms. */
 int phb_batch_replica_times(phb_batch b, double *run_ms, double *refresh_ms);

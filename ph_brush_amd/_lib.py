@@ -146,7 +146,8 @@ class BatchStats(C.Structure):
     _fields_ = [("attempted", C.c_longlong), ("accepted", C.c_longlong), ("launches", C.c_longlong),
                 ("replays", C.c_longlong), ("ms", C.c_double), ("trials_per_s", C.c_double),
                 ("spec_window", C.c_int), ("pending_max", C.c_int), ("graph_pairs", C.c_int),
-                ("launch_cap", C.c_longlong), ("persistent", C.c_int)]
+                ("launch_cap", C.c_longlong), ("persistent", C.c_int), ("refresh_ty", C.c_int),
+                ("refresh_tz", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -251,6 +252,7 @@ SIGNATURES = {
     "phb_mc_rng_state": [_h, C.POINTER(C.c_uint)],
     "phb_mc_set_rng_state": [_h, C.POINTER(C.c_uint)],
     "phb_mc_set_mu": [_h, C.c_double],
+    "phb_mc_set_refresh_tile": [_h, C.c_int, C.c_int],
     "phb_rng_seed_state": [C.c_uint, C.POINTER(C.c_uint)],
     "phb_rng_draws": [C.POINTER(C.c_uint), C.c_longlong, _pi, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint),
                       C.POINTER(C.c_longlong)],
@@ -281,6 +283,7 @@ SIGNATURES = {
     "phb_batch_autotune": [_h, C.c_int, C.POINTER(BatchStats)],
     "phb_batch_tune": [_h, C.c_int, C.c_int, C.c_int, C.c_longlong],
     "phb_batch_mode": [_h, C.c_int],
+    "phb_batch_set_refresh_tile": [_h, C.c_int, C.c_int],
     "phb_batch_replica_times": [_h, _pd, _pd],
     "phb_data_read": [C.c_char_p, C.POINTER(_h), C.POINTER(DataInfo)],
     "phb_data_get": [_h, _pi, _pi, _pi, _pd, _pd, _pd, _pd, _pi, _pi, _pi, _pi],

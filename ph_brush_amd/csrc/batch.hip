@@ -436,6 +436,8 @@ void setup_persistent(phb_batch_s* B) {
         v[r].phi = M->phi.p;
         v[r].tiles = M->rd_tiles.p;
         v[r].nt = M->rd_ntiles;
+        require(M->rd_tiles.p != nullptr && (size_t)M->rd_ntiles <= M->rd_tiles.n,
+                "phb_batch_mode: tile list out of step with its count", PHB_ERR_INTERNAL);
         require(M->applied.n >= (size_t)v[r].nt, "phb_batch_mode: refresh gate smaller than the tile count",
                 PHB_ERR_INTERNAL);
         order[r] = r;
@@ -540,7 +542,11 @@ void build_refresh_tables(phb_batch_s* B) {
     std::vector<int> ks{0}, ps{0}, ds{0};
     for (int r = 0; r < B->R; ++r) {
         phb_mc_s* M = B->mcs[r];
+        // the replica's current tile list (a new tile shape reallocates it)
+        B->hviews[r].tiles = M->rd_tiles.p;
         if (use_direct_refresh(M)) {
+            require(M->rd_tiles.p != nullptr && M->rd_ntiles > 0 && (size_t)M->rd_ntiles <= M->rd_tiles.n,
+                    "batch refresh tables: tile list out of step with its count", PHB_ERR_INTERNAL);
             vd.push_back(B->hviews[r]);
             ds.push_back(ds.back() + M->rd_ntiles);
         } else {
@@ -568,9 +574,11 @@ void build_refresh_tables(phb_batch_s* B) {
     sync();
 }
 
-void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long launch_cap, bool persistent) {
+void apply_tuning(phb_batch_s* B, int W, int pending_max, int pairs, long long launch_cap, bool persistent, int ty = 0,
+                  int tz = 0) {
     if (persistent) launch_cap = 0;   // a persistent chain runs to its sweep target
     for (phb_mc_s* M : B->mcs) {
+        if (ty > 0) set_refresh_tile(M, ty, tz);
         M->dev.W = W;
         M->d.pending_max = pending_max;
         M->d.spec_window = W;
@@ -704,6 +712,8 @@ extern "C" int phb_batch_run(phb_batch B, int sweeps, phb_batch_stats* st) {
             st->graph_pairs = B->d.graph_pairs;
             st->launch_cap = B->d.launch_cap;
             st->persistent = B->persistent ? 1 : 0;
+            st->refresh_ty = B->mcs.empty() ? 0 : B->mcs[0]->dev.rd_ty;
+            st->refresh_tz = B->mcs.empty() ? 0 : B->mcs[0]->dev.rd_tz;
         }
         return PHB_OK;
     });
@@ -739,6 +749,15 @@ extern "C" int phb_batch_replica_times(phb_batch B, double* run_ms, double* refr
     });
 }
 
+extern "C" int phb_batch_set_refresh_tile(phb_batch B, int ty, int tz) {
+    return guard([&] {
+        require(B, "phb_batch_set_refresh_tile: null batch");
+        require(refresh_tile_ok(ty, tz), "phb_batch_set_refresh_tile: 8 or 4 rows along y by 4 or 2 planes along z");
+        apply_tuning(B, B->d.spec_window, B->d.pending_max, B->d.graph_pairs, B->d.launch_cap, B->persistent, ty, tz);
+        return PHB_OK;
+    });
+}
+
 extern "C" int phb_batch_mode(phb_batch B, int persistent) {
     return guard([&] {
         require(B, "phb_batch_mode: null batch");
@@ -750,45 +769,85 @@ extern "C" int phb_batch_mode(phb_batch B, int persistent) {
 extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_batch_stats* best) {
     return guard([&] {
         require(B && sweeps_per_candidate > 0, "phb_batch_autotune: bad arguments");
-        // stage 1: window x budget x graph length without a launch cap; stage 2: the
-        // per-launch trial cap (a launch waits for its slowest replica) with the best of stage 1
-        // (the persistent form, where every replica qualifies: window x budget, one launch per sweep)
-        struct Cand { int W, pm, pairs; long long cap; bool pers; };
-        std::vector<Cand> cands;
-        for (int W : {4, 8, 16})
-            for (int pm : {4, 8, 16})
-                for (int pairs : {2, 4, 8}) cands.push_back({W, pm, pairs, 0, false});
-        const bool pers_ok = B->persistent || [&] {   // probe at the current shape
-            for (phb_mc_s* M : B->mcs)
-                if (!refresh_direct_allowed(M) || M->d.refresh == PHB_REFRESH_KSPACE) return false;
-            return true;
-        }();
-        if (pers_ok)
-            for (int W : {4, 8, 16})
-                for (int pm : {4, 8, 16}) cands.push_back({W, pm, B->d.graph_pairs, 0, true});
-        double best_rate = -1.0;
-        Cand bc = cands[0];
-        const size_t n1 = cands.size();
-        for (size_t ci = 0; ci < cands.size(); ++ci) {
-            const Cand c = cands[ci];
-            apply_tuning(B, c.W, c.pm, c.pairs, c.cap, c.pers);
+        // The replicas move towards equilibrium while the candidates run (fewer accepted moves,
+        // faster sweeps), so later candidates would look faster: warm-up sweeps first, and the
+        // best graph shape and the best persistent shape timed again back to back before the
+        // form is chosen.
+        // stage 1: window x budget x graph length (graph form), window x budget (persistent form,
+        // where every replica qualifies); stage 2: the per-launch trial cap on the best graph
+        // shape; stage 3: the two forms' best shapes again; stage 4: the refresh tile shape.
+        struct Cand { int W, pm, pairs; long long cap; bool pers; int ty, tz; };
+        const int spc = sweeps_per_candidate;
+        auto rate_of = [&](const Cand& c, int sweeps) {
+            apply_tuning(B, c.W, c.pm, c.pairs, c.cap, c.pers, c.ty, c.tz);
             read_status(B);
             long long t0 = 0;
             for (int r = 0; r < B->R; ++r) t0 += B->h_status[3 * r + 1];
             const auto c0 = std::chrono::steady_clock::now();
-            for (int s = 0; s < sweeps_per_candidate; ++s) sweep(B);
+            for (int s = 0; s < sweeps; ++s) sweep(B);
             sync();
             const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
             read_status(B);
             long long t1 = 0;
             for (int r = 0; r < B->R; ++r) t1 += B->h_status[3 * r + 1];
-            const double rate = (double)(t1 - t0) / std::max(dt, 1e-9);
-            if (rate > best_rate) { best_rate = rate; bc = c; }
-            if (ci + 1 == n1 && !bc.pers)   // stage 2: the caps, on the best graph shape of stage 1
-                for (long long cap : {16LL, 32LL, 64LL, 128LL, 256LL})
-                    cands.push_back({bc.W, bc.pm, bc.pairs, cap, false});
+            return (double)(t1 - t0) / std::max(dt, 1e-9);
+        };
+        const int ty0 = B->mcs.empty() ? RD_TY : B->mcs[0]->dev.rd_ty, tz0 = B->mcs.empty() ? RD_TZ : B->mcs[0]->dev.rd_tz;
+        for (int s = 0; s < 2 * spc; ++s) sweep(B);   // warm-up at the current shape
+        sync();
+        const bool pers_ok = B->persistent || [&] {
+            for (phb_mc_s* M : B->mcs)
+                if (!refresh_direct_allowed(M) || M->d.refresh == PHB_REFRESH_KSPACE) return false;
+            return true;
+        }();
+        Cand bg{8, 8, 2, 0, false, ty0, tz0}, bp{8, 8, 2, 0, true, ty0, tz0};
+        double rg = -1.0, rp = -1.0;
+        for (int W : {4, 8, 16})
+            for (int pm : {4, 8, 16})
+                for (int pairs : {2, 4, 8}) {
+                    const Cand c{W, pm, pairs, 0, false, ty0, tz0};
+                    const double r = rate_of(c, spc);
+                    if (r > rg) { rg = r; bg = c; }
+                }
+        if (pers_ok)
+            for (int W : {4, 8, 16})
+                for (int pm : {4, 8, 16}) {
+                    const Cand c{W, pm, B->d.graph_pairs, 0, true, ty0, tz0};
+                    const double r = rate_of(c, spc);
+                    if (r > rp) { rp = r; bp = c; }
+                }
+        for (long long cap : {16LL, 32LL, 64LL, 128LL, 256LL}) {
+            Cand c = bg;
+            c.cap = cap;
+            const double r = rate_of(c, spc);
+            if (r > rg) { rg = r; bg = c; }
         }
-        apply_tuning(B, bc.W, bc.pm, bc.pairs, bc.cap, bc.pers);
+        Cand bc = bg;
+        double best_rate = rate_of(bg, 2 * spc);
+        if (pers_ok) {
+            const double r = rate_of(bp, 2 * spc);
+            if (r > best_rate) { best_rate = r; bc = bp; }
+        }
+        const bool tiles_ok = [&] {   // the tile shape matters where a replica refreshes directly
+            for (phb_mc_s* M : B->mcs)
+                if (use_direct_refresh(M)) return true;
+            return false;
+        }();
+        if (tiles_ok) {
+            const int shapes[4][2] = {{8, 4}, {8, 2}, {4, 4}, {4, 2}};
+            double rt = -1.0;
+            Cand bt = bc;
+            for (const auto& t : shapes) {
+                Cand c = bc;
+                c.ty = t[0];
+                c.tz = t[1];
+                const double r = rate_of(c, spc);
+                if (r > rt) { rt = r; bt = c; }
+            }
+            bc = bt;
+            best_rate = rt;
+        }
+        apply_tuning(B, bc.W, bc.pm, bc.pairs, bc.cap, bc.pers, bc.ty, bc.tz);
         if (best) {
             *best = phb_batch_stats{};
             best->spec_window = bc.W;
@@ -796,6 +855,8 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
             best->graph_pairs = bc.pairs;
             best->launch_cap = bc.cap;
             best->persistent = bc.pers ? 1 : 0;
+            best->refresh_ty = bc.ty;
+            best->refresh_tz = bc.tz;
             best->trials_per_s = best_rate;
         }
         return PHB_OK;

@@ -86,11 +86,24 @@ __global__ __launch_bounds__(512) void k_refresh_kspace(McDev D, cplx<double>* _
     refresh_kspace_tile<P>(D, kg, twx, twy, twz, eta, pz, tx, blockIdx.x);
 }
 
-// Direct refresh (refresh_direct_tile): one workgroup per listed (y, z) tile of full x rows.
-template <int P>
+// Direct refresh (refresh_direct_tile): one workgroup per listed (y, z) tile of full x rows,
+// one instantiation per tile shape.
+template <int P, int TY, int TZ>
 __global__ __launch_bounds__(RD_THREADS) void k_refresh_direct(McDev D, double* __restrict__ phi,
                                                                float* __restrict__ phi32, const int* __restrict__ tiles) {
-    refresh_direct_tile<P>(D, phi, phi32, tiles[blockIdx.x]);
+    refresh_direct_tile_fixed<P, TY, TZ>(D, phi, phi32, tiles[blockIdx.x]);
+}
+
+template <int P, int TY, int TZ>
+void launch_refresh_direct(phb_mc_s* M) {
+    static bool lds_set = false;   // per instantiation
+    if (!lds_set) {
+        PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_refresh_direct<P, TY, TZ>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)refresh_direct_lds(P)));
+        lds_set = true;
+    }
+    hipLaunchKernelGGL((k_refresh_direct<P, TY, TZ>), dim3(M->rd_ntiles), dim3(RD_THREADS), refresh_direct_lds(P),
+                       stream(), M->dev, M->phi.p, M->d.precision == PHB_FP32 ? M->phi32.p : nullptr, M->rd_tiles.p);
 }
 
 __global__ void k_reset_pending(McCtl* C) {
@@ -297,6 +310,36 @@ void full_recompute(phb_mc_s* M) {
     ++M->recomputes;
 }
 
+// Direct refresh tiles of the current shape: the potential is read only at atoms'
+// stencil points (trial atoms, W . phi, the exact stage), and positions are fixed,
+// so a tile that no atom's stencil reaches (the vacuum of a slab geometry) is never
+// read and is left as is.  Tile 0 is always listed: its gate entry tells the chains
+// that a refresh has run.
+void build_rd_tiles(phb_mc_s* M) {
+    const phb_pme_s* pme = M->pme;
+    const int N = M->atoms->n, P = pme->order, ty = M->dev.rd_ty, tz = M->dev.rd_tz;
+    const int nty = (pme->ny + ty - 1) / ty, ntz = (pme->nz + tz - 1) / tz;
+    std::vector<int> sb(3 * (size_t)N);
+    if (N > 0) M->sbase.download(sb.data(), 3 * (size_t)N);
+    sync();
+    std::vector<char> need((size_t)nty * ntz, 0);
+    need[0] = 1;
+    for (int i = 0; i < N; ++i)
+        for (int c = 0; c < P; ++c) {
+            const int tzi = ((sb[3 * (size_t)i + 2] + c) % pme->nz) / tz;
+            for (int b = 0; b < P; ++b) need[(size_t)tzi * nty + ((sb[3 * (size_t)i + 1] + b) % pme->ny) / ty] = 1;
+        }
+    std::vector<int> tiles;
+    for (int t = 0; t < nty * ntz; ++t)
+        if (need[t]) tiles.push_back(t);
+    // every listed tile has its gate entry (sized for the smallest tiles)
+    require((size_t)nty * ntz <= M->applied.n, "refresh tiles beyond the gate array", PHB_ERR_INTERNAL);
+    M->rd_ntiles = (int)tiles.size();
+    M->rd_tiles.alloc(tiles.size());
+    M->rd_tiles.upload(tiles.data(), tiles.size());
+    sync();
+}
+
 bool refresh_direct_allowed(const phb_mc_s* M) {
     return M->pme->nx <= RD_NXMAX && refresh_direct_lds(M->pme->order) <= 64 * 1024;
 }
@@ -311,14 +354,11 @@ template <int P>
 void refresh_p(phb_mc_s* M) {
     phb_pme_s* Pm = M->pme;
     if (use_direct_refresh(M)) {
-        static bool lds_set = false;   // per spline order (one instantiation each)
-        if (!lds_set) {
-            PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_refresh_direct<P>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)refresh_direct_lds(P)));
-            lds_set = true;
-        }
-        hipLaunchKernelGGL(k_refresh_direct<P>, dim3(M->rd_ntiles), dim3(RD_THREADS), refresh_direct_lds(P), stream(),
-                           M->dev, M->phi.p, M->d.precision == PHB_FP32 ? M->phi32.p : nullptr, M->rd_tiles.p);
+        const int ty = M->dev.rd_ty, tz = M->dev.rd_tz;
+        if (ty == 8 && tz == 4) launch_refresh_direct<P, 8, 4>(M);
+        else if (ty == 8) launch_refresh_direct<P, 8, 2>(M);
+        else if (tz == 4) launch_refresh_direct<P, 4, 4>(M);
+        else launch_refresh_direct<P, 4, 2>(M);
         PHB_HIP(hipGetLastError());
         ++M->refreshes;
         return;
@@ -559,32 +599,13 @@ extern "C" int phb_mc_create(phb_atoms a, phb_params p, phb_csr neigh, phb_pme p
         M->ctl.alloc(1);
         M->ctl.upload(&c, 1);
         {   // one entry per workgroup of either refresh (direct tiles, or the inverse plane pass)
-            const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz), pme->plane_gate_blocks());
+            const size_t na = (size_t)std::max(refresh_direct_blocks(pme->ny, pme->nz, 4, 2), pme->plane_gate_blocks());
             M->applied.alloc(na);
             PHB_HIP(hipMemsetAsync(M->applied.p, 0, na * sizeof(unsigned), stream()));
         }
-        {   // Direct refresh tiles: the potential is read only at atoms' stencil points (trial
-            // atoms, W . phi, the exact stage), and positions are fixed, so a tile that no atom's
-            // stencil reaches (the vacuum of a slab geometry) is never read and is left as is.
-            // Tile 0 is always listed: its gate entry tells the chains that a refresh has run.
-            const int nty = (pme->ny + RD_TY - 1) / RD_TY, ntz = (pme->nz + RD_TZ - 1) / RD_TZ;
-            std::vector<int> sb(3 * (size_t)N);
-            if (N > 0) M->sbase.download(sb.data(), 3 * (size_t)N);
-            sync();
-            std::vector<char> need((size_t)nty * ntz, 0);
-            need[0] = 1;
-            for (int i = 0; i < N; ++i)
-                for (int c = 0; c < P; ++c) {
-                    const int tz = ((sb[3 * (size_t)i + 2] + c) % pme->nz) / RD_TZ;
-                    for (int b = 0; b < P; ++b) need[(size_t)tz * nty + ((sb[3 * (size_t)i + 1] + b) % pme->ny) / RD_TY] = 1;
-                }
-            std::vector<int> tiles;
-            for (int t = 0; t < nty * ntz; ++t)
-                if (need[t]) tiles.push_back(t);
-            M->rd_ntiles = (int)tiles.size();
-            M->rd_tiles.alloc(tiles.size());
-            M->rd_tiles.upload(tiles.data(), tiles.size());
-        }
+        M->dev.rd_ty = RD_TY;
+        M->dev.rd_tz = RD_TZ;
+        build_rd_tiles(M.get());
         // ---- device view
         McDev& D = M->dev;
         D.N = N; D.z = a->z.p; D.q = a->q.p; D.type = a->type.p;
@@ -760,6 +781,37 @@ extern "C" int phb_mc_set_mu(phb_mc M, double mu_eff) {
         require(std::isfinite(mu_eff), "phb_mc_set_mu: mu_eff must be finite");
         M->dev.mu = mu_eff;
         M->d.mu_eff = mu_eff;
+        return PHB_OK;
+    });
+}
+
+namespace phb {
+
+__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+void set_refresh_tile(phb_mc_s* M, int ty, int tz) {
+    require(refresh_tile_ok(ty, tz), "refresh tile: 8 or 4 rows along y by 4 or 2 planes along z");
+    if (ty == M->dev.rd_ty && tz == M->dev.rd_tz) return;
+    refresh(M);   // the pending changes into phi with the current tiles
+    McCtl c = read_ctl(M);
+    M->dev.rd_ty = ty;
+    M->dev.rd_tz = tz;
+    build_rd_tiles(M);
+    // phi holds every pending change: every gate entry of the new tiling says so
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((M->applied.n + 255) / 256, 1024)), dim3(256), 0,
+                       stream(), M->applied.p, M->applied.n, c.pend_seq);
+    PHB_HIP(hipGetLastError());
+    sync();
+}
+
+}  // namespace phb
+
+extern "C" int phb_mc_set_refresh_tile(phb_mc M, int ty, int tz) {
+    return guard([&] {
+        require(M, "phb_mc_set_refresh_tile: null");
+        set_refresh_tile(M, ty, tz);
         return PHB_OK;
     });
 }

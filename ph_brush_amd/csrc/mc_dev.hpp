@@ -57,6 +57,7 @@ struct McDev {
     int fnx, fny;         // nx/2+1, ny/2+1
     const double* selfw;  // per atom W_a^T G W_a (positions are fixed)
     int nx, ny, nz, P;
+    int rd_ty, rd_tz;     // direct-refresh tile: rows along y, planes along z
     int T;
     const double *eps, *sig, *shift;
     int use_shift, use_cut;

@@ -94,6 +94,10 @@ int launch_batch(const McCtl& c, long long remaining, int pending_max);
 // Full (both directions) neighbour rows, ascending, from a half list; returns the entry count.
 long long build_full_rows(int N, const phb_csr_s* half, DevBuf<int>& head, DevBuf<int>& nbr);
 McCtl read_ctl(phb_mc_s* M);
+// Direct-refresh tile list of the object's tile shape (M->dev.rd_ty x rd_tz).
+void build_rd_tiles(phb_mc_s* M);
+// Direct-refresh tile shape (folds the pending changes first; rebuilds the tile list).
+void set_refresh_tile(phb_mc_s* M, int ty, int tz);
 // Full recompute: phi = G Q from scratch, total energy, moments, E reset.
 void full_recompute(phb_mc_s* M);
 // ntrials more trials of the plain chain (refreshes, drift resets); ends early

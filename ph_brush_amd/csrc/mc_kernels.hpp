@@ -331,7 +331,8 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
 // axis, which stays in L2).  Equal to the k-space refresh up to rounding (both
 // are the same circular convolution), in one pass over the potential grid.
 //
-// Workgroup blk owns the full x rows of RD_TY y by RD_TZ z mesh points; lane =
+// Workgroup blk owns the full x rows of ty y by tz z mesh points (the MC object's
+// tile shape, D.rd_ty x D.rd_tz, at most RD_TY x RD_TZ); lane =
 // x (nx <= 64), so every row index is uniform across a wave.  RD_GROUPS groups
 // of four waves take the pending atoms p = g, g + RD_GROUPS, ...: per atom a z
 // pass over the halo rows read into registers, a y pass in place in LDS (rows
@@ -342,40 +343,45 @@ __device__ __forceinline__ void refresh_kspace_tile(const McDev& D, cplx<double>
 // device-wide fence or a shared counter would cost more than the convolution).
 // A workgroup wider than RD_THREADS (the persistent sweep's) runs a tile with its
 // first RD_THREADS threads; the others only meet the barriers, so the
-// arithmetic is the same as in the RD_THREADS launches.
+// arithmetic is the same as in the RD_THREADS launches.  Register arrays and LDS
+// are sized for the largest tile; smaller tiles leave the rest unused.  Each shape
+// is its own instantiation (the loops and register arrays are sized at compile time).
 constexpr int RD_TY = 8, RD_TZ = 4, RD_GROUPS = 2, RD_THREADS = 256 * RD_GROUPS, RD_NXMAX = 64, RD_PITCH = 64;
 
 __host__ __device__ constexpr size_t refresh_direct_lds(int P) {
     return (size_t)RD_GROUPS * RD_TZ * (RD_TY + P - 1) * RD_PITCH * sizeof(double);
 }
 
-__host__ __device__ constexpr int refresh_direct_blocks(int ny, int nz) {
-    return ((ny + RD_TY - 1) / RD_TY) * ((nz + RD_TZ - 1) / RD_TZ);
+__host__ __device__ constexpr int refresh_direct_blocks(int ny, int nz, int ty = RD_TY, int tz = RD_TZ) {
+    return ((ny + ty - 1) / ty) * ((nz + tz - 1) / tz);
 }
 
-template <int P>
-__device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __restrict__ phi, float* __restrict__ phi32,
-                                                    int blk) {
+// tile shapes a refresh can take (rows along y, planes along z)
+__host__ __device__ constexpr bool refresh_tile_ok(int ty, int tz) {
+    return (ty == 8 || ty == 4) && (tz == 4 || tz == 2);
+}
+
+template <int P, int TY, int TZ>
+__device__ __forceinline__ void refresh_direct_tile_shape(const McDev& D, double* __restrict__ phi, float* __restrict__ phi32,
+                                                          int blk, int (*s_sb)[3], double (*s_w)[3 * P],
+                                                          double* s_dq) {
     const McCtl* C = D.ctl;
     const unsigned seq = C->pend_seq;
     const int np = D.applied[blk] == seq ? 0 : C->pending_n;
     if (np == 0) return;
-    constexpr int HY = RD_TY + P - 1, HZ = RD_TZ + P - 1;
-    constexpr int RPW = (HY + 3) / 4, KPW = (RD_TZ + 3) / 4, OPW = (RD_TY * RD_TZ + 3) / 4;
-    constexpr int ABUF = RD_TZ * HY * RD_PITCH;
+    constexpr int HY = TY + P - 1, HZ = TZ + P - 1;
+    constexpr int RPW = (HY + 3) / 4, KPW = (TZ + 3) / 4, OPW = (TY * TZ + 3) / 4;
+    constexpr int ABUF = TZ * HY * RD_PITCH;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* A_all = reinterpret_cast<double*>(smem_raw);   // RD_GROUPS x [RD_TZ][HY][RD_PITCH]
-    __shared__ int s_sb[MC_KMAX][3];
-    __shared__ double s_w[MC_KMAX][3 * P];
-    __shared__ double s_dq[MC_KMAX];
+    double* A_all = reinterpret_cast<double*>(smem_raw);   // RD_GROUPS x [TZ][HY][RD_PITCH]
     const int tid = threadIdx.x, lane = tid & 63;
     const int wg = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wg >> 2, wv = wg & 3;
     const bool active = tid < RD_THREADS;   // groups grp >= RD_GROUPS only meet the barriers
     double* A = A_all + (active ? grp : 0) * ABUF;
     const int nx = D.nx, ny = D.ny, nz = D.nz, fnx = D.fnx, fny = D.fny;
-    const int nty = (ny + RD_TY - 1) / RD_TY;
-    const int tz = blk / nty, y0 = (blk - tz * nty) * RD_TY, z0 = tz * RD_TZ;
+    const int nty = (ny + TY - 1) / TY;
+    const int tz = blk / nty, y0 = (blk - tz * nty) * TY, z0 = tz * TZ;
     const bool xin = lane < nx;
     const int fx = xin ? fold(lane, nx) : 0;
     for (int e = tid; active && e < np * (3 * P + 4); e += RD_THREADS) {
@@ -387,8 +393,8 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
     double old[OPW];   // group 0 holds the tile's grid values from the start
 #pragma unroll
     for (int i = 0; i < OPW; ++i) {
-        const int r = wv * OPW + i, k = r / RD_TY, j = r - k * RD_TY;
-        const bool ok = grp == 0 && xin && r < RD_TY * RD_TZ && y0 + j < ny && z0 + k < nz;
+        const int r = wv * OPW + i, k = r / TY, j = r - k * TY;
+        const bool ok = grp == 0 && xin && r < TY * TZ && y0 + j < ny && z0 + k < nz;
         old[i] = ok ? phi[((size_t)(z0 + k) * ny + y0 + j) * nx + lane] : 0.0;
     }
     double acc[OPW];
@@ -427,7 +433,7 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
                 const int yy = wv + 4 * r;
                 if (yy < HY) {
 #pragma unroll
-                    for (int k = 0; k < RD_TZ; ++k) {
+                    for (int k = 0; k < TZ; ++k) {
                         double a = 0.0;
 #pragma unroll
                         for (int q = 0; q < P; ++q) a += w[2 * P + q] * g[r][k + P - 1 - q];
@@ -441,12 +447,12 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
 #pragma unroll
             for (int c = 0; c < KPW; ++c) {
                 const int k = wv + 4 * c;
-                if (k < RD_TZ) {
+                if (k < TZ) {
                     double v[HY];
 #pragma unroll
                     for (int yy = 0; yy < HY; ++yy) v[yy] = A[(k * HY + yy) * RD_PITCH + lane];
 #pragma unroll
-                    for (int j = 0; j < RD_TY; ++j) {
+                    for (int j = 0; j < TY; ++j) {
                         double b = 0.0;
 #pragma unroll
                         for (int u = 0; u < P; ++u) b += w[P + u] * v[j + P - 1 - u];
@@ -470,8 +476,8 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
 #pragma unroll
             for (int i = 0; i < OPW; ++i) {
                 const int r = wv * OPW + i;
-                if (r < RD_TY * RD_TZ) {
-                    const int k = r / RD_TY, j = r - k * RD_TY;
+                if (r < TY * TZ) {
+                    const int k = r / TY, j = r - k * TY;
                     const double* row = A + (k * HY + j) * RD_PITCH;
                     double s = 0.0;
 #pragma unroll
@@ -498,8 +504,8 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
     if (grp == 0) {
 #pragma unroll
         for (int i = 0; i < OPW; ++i) {
-            const int r = wv * OPW + i, k = r / RD_TY, j = r - k * RD_TY;
-            if (xin && r < RD_TY * RD_TZ && y0 + j < ny && z0 + k < nz) {
+            const int r = wv * OPW + i, k = r / TY, j = r - k * TY;
+            if (xin && r < TY * TZ && y0 + j < ny && z0 + k < nz) {
                 const size_t ix = ((size_t)(z0 + k) * ny + y0 + j) * nx + lane;
                 const double v = old[i] + acc[i];
                 phi[ix] = v;
@@ -509,6 +515,32 @@ __device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __re
     }
     __syncthreads();
     if (tid == 0) D.applied[blk] = seq;
+}
+
+// The tile of the object's shape (D.rd_ty x D.rd_tz; uniform across a launch).
+template <int P>
+__device__ __forceinline__ void refresh_direct_tile(const McDev& D, double* __restrict__ phi, float* __restrict__ phi32,
+                                                    int blk) {
+    __shared__ int s_sb[MC_KMAX][3];
+    __shared__ double s_w[MC_KMAX][3 * P];
+    __shared__ double s_dq[MC_KMAX];
+    if (D.rd_ty == 8) {
+        if (D.rd_tz == 4) refresh_direct_tile_shape<P, 8, 4>(D, phi, phi32, blk, s_sb, s_w, s_dq);
+        else refresh_direct_tile_shape<P, 8, 2>(D, phi, phi32, blk, s_sb, s_w, s_dq);
+    } else {
+        if (D.rd_tz == 4) refresh_direct_tile_shape<P, 4, 4>(D, phi, phi32, blk, s_sb, s_w, s_dq);
+        else refresh_direct_tile_shape<P, 4, 2>(D, phi, phi32, blk, s_sb, s_w, s_dq);
+    }
+}
+
+// One shape only (a launch whose shape the host knows).
+template <int P, int TY, int TZ>
+__device__ __forceinline__ void refresh_direct_tile_fixed(const McDev& D, double* __restrict__ phi,
+                                                          float* __restrict__ phi32, int blk) {
+    __shared__ int s_sb[MC_KMAX][3];
+    __shared__ double s_w[MC_KMAX][3 * P];
+    __shared__ double s_dq[MC_KMAX];
+    refresh_direct_tile_shape<P, TY, TZ>(D, phi, phi32, blk, s_sb, s_w, s_dq);
 }
 
 }  // namespace phb

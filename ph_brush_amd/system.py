@@ -455,6 +455,10 @@ class MonteCarlo(_Handle):
         L.call("phb_mc_rng_state", self.h, st)
         return np.array(st[:], dtype=np.uint32)
 
+    def set_refresh_tile(self, ty: int, tz: int) -> None:
+        """Tile of the direct potential refresh: ty rows (8 or 4) by tz planes (4 or 2)."""
+        L.call("phb_mc_set_refresh_tile", self.h, int(ty), int(tz))
+
     def set_mu(self, mu_eff: float) -> None:
         """Fixed mu_eff of the next runs (a titration moving one chain from pH to pH)."""
         L.call("phb_mc_set_mu", self.h, float(mu_eff))
@@ -674,6 +678,10 @@ class Batch(_Handle):
 
     def tune(self, spec_window: int, pending_max: int, graph_pairs: int, launch_cap: int = 0):
         L.call("phb_batch_tune", self.h, spec_window, pending_max, graph_pairs, launch_cap)
+
+    def set_refresh_tile(self, ty: int, tz: int):
+        """Direct-refresh tile of every replica: ty rows (8 or 4) by tz planes (4 or 2)."""
+        L.call("phb_batch_set_refresh_tile", self.h, ty, tz)
 
 
 class Replica:

@@ -108,5 +108,22 @@ def test_autotune_keeps_every_chain_exact(gpu):
     assert best["spec_window"] in (4, 8, 16) and best["pending_max"] in (4, 8, 16)
     assert best["graph_pairs"] in (2, 4, 8) and best["trials_per_s"] > 0
     assert best["launch_cap"] in (0, 16, 32, 64, 128, 256)
+    assert best["refresh_ty"] in (8, 4) and best["refresh_tz"] in (4, 2)
     b.run(2)
     _check_against_serial(reps, batched, 20000)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_refresh_tile_shapes_keep_every_chain_exact(gpu, persistent):
+    """Direct-refresh tiles of 4 x 2, 8 x 2, 4 x 4 and 8 x 4 points, changed
+    between sweeps (pending changes folded first), in the graph and the
+    persistent forms: every replica equals its serial run at the default tile."""
+    reps = _replicas(8000)
+    batched = _make(reps, 8000)
+    b = S.Batch([r.mc for r in batched], graph_pairs=2, persistent=persistent)
+    for ty, tz in ((4, 2), (8, 2), (4, 4), (8, 4)):
+        b.set_refresh_tile(ty, tz)
+        st = b.run(1)
+        assert st["refresh_ty"] == ty and st["refresh_tz"] == tz
+        assert st["persistent"] == int(persistent)
+    _check_against_serial(reps, batched, 8000)

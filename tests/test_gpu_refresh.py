@@ -72,3 +72,31 @@ def test_direct_refresh_fp32_form(gpu):
     assert np.abs(gd["dE"][:k] - gk["dE"][:k]).max() < 1e-4
     e_run = rd.mc.stats()["E"]
     assert abs(e_run - rd.mc.recompute()) < 1e-2
+
+
+def test_refresh_tile_shapes_serial(gpu):
+    """The direct refresh with 4 x 2, 8 x 2 and 4 x 4-point tiles, switched in the
+    middle of a run with changes pending (ragged y and z tiles: a 16 x 20 x 45
+    grid): the same trial sequence as the default 8 x 4 tiles, Delta-E and the
+    running energy to the bars above (a switch folds the pending changes in at
+    once, not at the next refresh); other shapes are refused."""
+    s, p, _ = S.config_tiny(order=5, grid=(16, 20, 45), n_perm=2, n_salt=10)
+    mc = S.McParams.at_ph(7.9)
+    n = 3000
+    ref = S.Replica(s, p, mc, seed=21, pending_max=8, recompute_interval=0, record_cap=n,
+                    refresh=L.PHB_REFRESH_DIRECT)
+    sr = ref.mc.run(n)
+    r = S.Replica(s, p, mc, seed=21, pending_max=8, recompute_interval=0, record_cap=n,
+                  refresh=L.PHB_REFRESH_DIRECT)
+    for ty, tz in ((4, 2), (8, 2), (4, 4)):
+        r.mc.set_refresh_tile(ty, tz)
+        st = r.mc.run(n // 3)
+    assert st["attempted"] == n and st["refreshes"] > 10
+    g, gr = r.mc.record(n), ref.mc.record(n)
+    for key in ("fwd", "i", "j", "acc"):
+        assert np.array_equal(g[key], gr[key]), key
+    tol = 1e-10 * np.maximum(np.abs(gr["dE"]), 1.0) + 1e-14 * abs(sr["E"])
+    assert np.all(np.abs(g["dE"] - gr["dE"]) <= tol)
+    assert abs(st["E"] - sr["E"]) <= 1e-9 * max(1.0, abs(sr["E"]))
+    with pytest.raises(L.PhbError):
+        r.mc.set_refresh_tile(6, 4)
