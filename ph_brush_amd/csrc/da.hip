@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "mc_host.hpp"
+#include "mc_kernels.hpp"
 
 namespace phb {
 
@@ -1178,16 +1179,20 @@ __global__ void k_loc_far(McDev D, DaDev A, const int* __restrict__ list, int n)
 // per atom (entries dealt in turn, summed by shuffles).  Gate as the refresh's: a
 // workgroup adds a list only if its own fapplied differs from pend_seq, then
 // records pend_seq (a launch that accepted nothing leaves pend_seq).
-constexpr int FOLD_LANES = 4;
-__global__ __launch_bounds__(256) void k_loc_fold(McDev D, DaDev A, const int* __restrict__ list, int n) {
+constexpr int FOLD_LANES = 4, FOLD_THREADS = 256;
+
+// Fold block blk, run by FOLD_THREADS threads (tid its thread); threads of a
+// workgroup that are not active (no such block) only meet the barrier.
+__device__ __forceinline__ void loc_fold_block(const McDev& D, const DaDev& A, const int* __restrict__ list, int n,
+                                               int blk, int tid, bool active) {
     const McCtl* C = D.ctl;
     const unsigned seq = C->pend_seq;
-    if (A.fapplied[blockIdx.x] == seq) return;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x, k = gt / FOLD_LANES, g = gt - k * FOLD_LANES;
-    const int f0 = *A.fold_from, pn = C->pending_n;
+    const bool run = active && A.fapplied[blk] != seq;
+    const int gt = blk * FOLD_THREADS + tid, k = gt / FOLD_LANES, g = gt - k * FOLD_LANES;
     long long s = 0;
     int a = -1;
-    if (k < n) {
+    if (run && k < n) {
+        const int f0 = *A.fold_from, pn = C->pending_n;
         a = list[k];
         const double xa = A.x[a], ya = A.y[a], za = D.z[a];
         for (int e = f0 + g; e < pn; e += FOLD_LANES) {
@@ -1200,7 +1205,30 @@ __global__ __launch_bounds__(256) void k_loc_fold(McDev D, DaDev A, const int* _
     s += __shfl_xor(s, 2, FOLD_LANES);
     if (a >= 0 && g == 0 && s != 0) A.psil[a] += s;
     __syncthreads();
-    if (threadIdx.x == 0) A.fapplied[blockIdx.x] = seq;
+    if (run && tid == 0) A.fapplied[blk] = seq;
+}
+
+__global__ __launch_bounds__(FOLD_THREADS) void k_loc_fold(McDev D, DaDev A, const int* __restrict__ list, int n) {
+    loc_fold_block(D, A, list, n, blockIdx.x, threadIdx.x, true);
+}
+
+// The fold and the direct potential refresh of the same accepted changes in one
+// launch: workgroups below ntiles refresh listed tile tiles[w] (as k_refresh_direct),
+// the others run two fold blocks each, one per half.  The fold reads charges,
+// positions and the pending list; the refresh reads the pending list and writes
+// the potential grid, so the two parts share nothing they write.
+template <int P, int TY, int TZ>
+__global__ __launch_bounds__(RD_THREADS) void k_loc_fold_refresh(McDev D, DaDev A, double* __restrict__ phi,
+                                                                 float* __restrict__ phi32,
+                                                                 const int* __restrict__ tiles, int ntiles,
+                                                                 const int* __restrict__ list, int n, int nfold) {
+    static_assert(RD_THREADS == 2 * FOLD_THREADS, "two fold blocks per refresh workgroup");
+    if ((int)blockIdx.x < ntiles) {
+        refresh_direct_tile_fixed<P, TY, TZ>(D, phi, phi32, tiles[blockIdx.x]);
+        return;
+    }
+    const int blk = 2 * ((int)blockIdx.x - ntiles) + (int)(threadIdx.x / FOLD_THREADS);
+    loc_fold_block(D, A, list, n, blk, threadIdx.x % FOLD_THREADS, blk < nfold);
 }
 
 __global__ void k_loc_fold_init(McDev D, DaDev A, int nblk) {
@@ -2177,7 +2205,7 @@ void local_rec_dyn(phb_mc_s* M, DaState* S) {
     PHB_HIP(hipGetLastError());
 }
 
-int fold_blocks(const DaState* S) { return std::max(ceil_div(S->ncand * FOLD_LANES, 256), 1); }
+int fold_blocks(const DaState* S) { return std::max(ceil_div(S->ncand * FOLD_LANES, FOLD_THREADS), 1); }
 
 // Every fold workgroup takes the current pending list as added (T is current).
 void local_fold_init(phb_mc_s* M, DaState* S) {
@@ -2189,9 +2217,47 @@ void local_fold_init(phb_mc_s* M, DaState* S) {
 // After a chain launch: the far part of its accepted changes into T of every pool atom.
 void local_fold(phb_mc_s* M, DaState* S) {
     if (S->ncand > 0)
-        hipLaunchKernelGGL(k_loc_fold, dim3(fold_blocks(S)), dim3(256), 0, stream(), M->dev, S->dev, S->cand.p,
-                           S->ncand);
+        hipLaunchKernelGGL(k_loc_fold, dim3(fold_blocks(S)), dim3(FOLD_THREADS), 0, stream(), M->dev, S->dev,
+                           S->cand.p, S->ncand);
     PHB_HIP(hipGetLastError());
+}
+
+template <int P, int TY, int TZ>
+void launch_fold_refresh(phb_mc_s* M, DaState* S) {
+    static bool lds_set = false;   // per instantiation
+    if (!lds_set) {
+        PHB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_loc_fold_refresh<P, TY, TZ>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)refresh_direct_lds(P)));
+        lds_set = true;
+    }
+    const int nf = fold_blocks(S);
+    hipLaunchKernelGGL((k_loc_fold_refresh<P, TY, TZ>), dim3(M->rd_ntiles + (nf + 1) / 2), dim3(RD_THREADS),
+                       refresh_direct_lds(P), stream(), M->dev, S->dev, M->phi.p,
+                       M->d.precision == PHB_FP32 ? M->phi32.p : nullptr, M->rd_tiles.p, M->rd_ntiles, S->cand.p,
+                       S->ncand, nf);
+}
+
+template <int P>
+void fold_refresh_p(phb_mc_s* M, DaState* S) {
+    const int ty = M->dev.rd_ty, tz = M->dev.rd_tz;
+    if (ty == 8 && tz == 4) launch_fold_refresh<P, 8, 4>(M, S);
+    else if (ty == 8) launch_fold_refresh<P, 8, 2>(M, S);
+    else if (tz == 4) launch_fold_refresh<P, 4, 4>(M, S);
+    else launch_fold_refresh<P, 4, 2>(M, S);
+    PHB_HIP(hipGetLastError());
+    ++M->refreshes;
+}
+
+// After a local-pair chain launch: the fold, then the refresh; one launch when
+// the refresh is direct and the order 4 or 5 (the others: two launches).
+void local_fold_refresh(phb_mc_s* M, DaState* S) {
+    if (S->ncand > 0 && use_direct_refresh(M) && (M->dev.P == 4 || M->dev.P == 5)) {
+        if (M->dev.P == 4) fold_refresh_p<4>(M, S);
+        else fold_refresh_p<5>(M, S);
+        return;
+    }
+    local_fold(M, S);
+    refresh(M);
 }
 
 // far_a at the current state: phi with every accepted change, W . phi into the
@@ -2909,10 +2975,12 @@ extern "C" int phb_da_run(phb_mc M, phb_surrogate s, long long ntrials, phb_da_s
                     launch_da(M, S);
                     ++M->launches;
                     ++S->launches;
-                    if (local) local_fold(M, S);
                     if (M->profiling) M->event();
-                    refresh(M);
-                    if (!local) phi_dot(M, S);
+                    if (local) local_fold_refresh(M, S);
+                    else {
+                        refresh(M);
+                        phi_dot(M, S);
+                    }
                     if (M->profiling) { M->event(); M->marks.push_back({e0, 0}); }
                 }
                 PHB_HIP(hipGetLastError());
