@@ -54,7 +54,7 @@ def titration(cfg=S.config_30k_fp64, ph_values=(6.0, 6.5, 7.0, 7.5, 8.0, 8.5, 9.
     rows = []
     for ph in ph_values:
         mc = S.McParams.at_ph(ph)
-        rep = S.Replica(s, p, mc, seed=seed, pending_max=8, recompute_interval=20000)
+        rep = S.Replica(s, p, mc, seed=seed, pending_max=2, spec_window=16, recompute_interval=20000)
         rep.mc.run(burn)
         q, ty = rep.atoms.identity()
         s_eq = S.System(s.x, s.y, s.z, q, ty, s.bonds, s.Lx, s.Ly, s.Lz, s.Lz_data)
