@@ -1465,6 +1465,7 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
     __shared__ long long s_done0;          // trials done at the round's start
     __shared__ long long s_s1, s_rounds;
     __shared__ long long s_prof[8];
+    __shared__ float s_out[3];             // output bias, mean and scale of the network
     extern __shared__ __align__(16) unsigned char s_dyn[];
     PhaseClock clk;
     const bool prof = A.prof != 0 && threadIdx.x == 0;
@@ -1562,7 +1563,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         s_defer = 0;
         for (int k = 0; k < 8; ++k) s_prof[k] = C->da_prof[k];
     }
-    const float om = net.theta[OFF_MEAN], osd = net.theta[OFF_STD], b3 = net.theta[OFF_B3];
+    // values used once per round live in LDS, not in registers across the round loop
+    if (tid == 0) {
+        s_out[0] = net.theta[OFF_B3];
+        s_out[1] = net.theta[OFF_MEAN];
+        s_out[2] = net.theta[OFF_STD];
+    }
     __syncthreads();
     clk.lap(prof, s_prof, 0);
     for (;;) {
@@ -1650,9 +1656,13 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
         //         shell-count differences
         // (the histogram rows' loads are issued first; the records' follow)
         int raw[DA_FEAT];
+        // the thread's feature indices, formed each round from an opaque copy of tid so
+        // that they are not hoisted out of the round loop and held in registers
+        int tid_r = tid;
+        asm volatile("" : "+v"(tid_r));
 #pragma unroll
         for (int q = 0; q < DA_FEAT; ++q) {
-            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
+            const int e = tid_r + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
             raw[q] = 0;
             if (e < DA_M * NN_IN && k < ntr)
                 raw[q] = (int)A.H[(size_t)s_c.i[k] * NN_IN + c] - (int)A.H[(size_t)s_c.j[k] * NN_IN + c];
@@ -1687,9 +1697,12 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             }
         }
         __syncthreads();
+        // formed again (not kept from the loop above across the records' part)
+        int tid_s = tid;
+        asm volatile("" : "+v"(tid_s));
 #pragma unroll
         for (int q = 0; q < DA_FEAT; ++q) {
-            const int e = tid + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
+            const int e = tid_s + q * DA_THREADS, k = e / NN_IN, c = e - k * NN_IN;
             if (e < DA_M * NN_IN) {
                 float v = 0.f;
                 if (k < ntr) {
@@ -1800,10 +1813,10 @@ __global__ __launch_bounds__(DA_THREADS) void k_da_chain(const McDev* __restrict
             const int k = lane;
             bool acc1 = false;
             if (k < ntr) {
-                float y = b3;
+                float y = s_out[0];
 #pragma unroll
                 for (int ct = 0; ct < 8; ++ct) y += s_y[k][ct];
-                const float f = om + osd * y;
+                const float f = s_out[1] + s_out[2] * y;
                 const int fwd = s_c.fwd[k];
                 const float sf = fwd ? f : -f;
                 double Fx;
