@@ -602,8 +602,8 @@ int phb_batch_create(const phb_mc *mcs, int n, const phb_batch_desc *d, phb_batc
 int phb_batch_destroy(phb_batch b);
 /* `sweeps` sweeps: every replica attempts as many trials as it has titratable sites per sweep. */
 int phb_batch_run(phb_batch b, int sweeps, phb_batch_stats *st);
-/* After 2 x `sweeps_per_candidate` warm-up sweeps, times speculation window x pending budget x
- * graph length on the batch (each candidate runs `sweeps_per_candidate` real sweeps), the
+/* After 2 x `sweeps_per_candidate` warm-up sweeps, times speculation window (4, 8, 16) x pending
+ * budget (2, 4, 8, 16) x graph length (2, 4, 8) on the batch (each candidate runs `sweeps_per_candidate` real sweeps), the
  * persistent form at each window x budget where every replica qualifies, per-launch trial caps
  * on the fastest graph shape, then the two forms' fastest shapes again back to back (twice the
  * sweeps), then the direct refresh's tile shape (8 or 4 rows by 4 or 2 planes) on the faster,

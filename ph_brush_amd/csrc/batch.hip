@@ -803,7 +803,7 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
         Cand bg{8, 8, 2, 0, false, ty0, tz0}, bp{8, 8, 2, 0, true, ty0, tz0};
         double rg = -1.0, rp = -1.0;
         for (int W : {4, 8, 16})
-            for (int pm : {4, 8, 16})
+            for (int pm : {2, 4, 8, 16})   // 2: a refresh after every accepted move
                 for (int pairs : {2, 4, 8}) {
                     const Cand c{W, pm, pairs, 0, false, ty0, tz0};
                     const double r = rate_of(c, spc);
@@ -811,7 +811,7 @@ extern "C" int phb_batch_autotune(phb_batch B, int sweeps_per_candidate, phb_bat
                 }
         if (pers_ok)
             for (int W : {4, 8, 16})
-                for (int pm : {4, 8, 16}) {
+                for (int pm : {2, 4, 8, 16}) {
                     const Cand c{W, pm, B->d.graph_pairs, 0, true, ty0, tz0};
                     const double r = rate_of(c, spc);
                     if (r > rp) { rp = r; bp = c; }

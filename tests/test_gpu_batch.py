@@ -105,7 +105,7 @@ def test_autotune_keeps_every_chain_exact(gpu):
     b = S.Batch([r.mc for r in batched])
     best = b.autotune(1)
     print("autotune:", best)
-    assert best["spec_window"] in (4, 8, 16) and best["pending_max"] in (4, 8, 16)
+    assert best["spec_window"] in (4, 8, 16) and best["pending_max"] in (2, 4, 8, 16)
     assert best["graph_pairs"] in (2, 4, 8) and best["trials_per_s"] > 0
     assert best["launch_cap"] in (0, 16, 32, 64, 128, 256)
     assert best["refresh_ty"] in (8, 4) and best["refresh_tz"] in (4, 2)
